@@ -1,0 +1,491 @@
+// C ABI of libfhp (declared in include/fhp.h): scenes, ray queries, render passes, gradients.
+#include "../../include/fhp.h"
+#include "fhp_internal.h"
+#include "scene_build.h"
+
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace {
+using namespace fhp;
+
+thread_local std::string g_error;
+thread_local std::string g_warning;
+
+int fail(int code, const std::string& msg) {
+    g_error = msg;
+    return code;
+}
+#define HIP_OR_FAIL(expr)                                                                         \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess) return fail(FHP_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// Device buffer owned by the library.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    hipError_t alloc(size_t n) {
+        if (n <= bytes && p) return hipSuccess;
+        release();
+        if (n == 0) return hipSuccess;
+        hipError_t e = hipMalloc(&p, n);
+        if (e == hipSuccess) bytes = n;
+        return e;
+    }
+    template <typename T> hipError_t upload(const std::vector<T>& v) {
+        hipError_t e = alloc(std::max<size_t>(v.size() * sizeof(T), 16));
+        if (e != hipSuccess || v.empty()) return e;
+        return hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+    }
+    template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+}  // namespace
+
+struct FhpScene_ {
+    HostScene h;
+    DevBuf bvh, triPos, idx, vtx, triMat, mats, texels, texDesc, lights, active, triToLight, lbvh, lbvhTri, lbvhMask;
+    DevBuf ovf;  // traversal overflow stack
+    TraceScratch ts{};
+    SceneDev dev{};
+    uint64_t deviceBytes = 0;
+};
+
+struct FhpPass_ {
+    std::string kind;
+    std::map<std::string, double> props;
+    // Wavefront buffers (forward pass).
+    DevBuf thp, prevN, rng, Lp, Ls, rayQ0, rayQ1, pathQ0, pathQ1, hitQ, shQ, pixHit, counters, stats;
+    uint32_t cap = 0, capPix = 0;
+    double get(const char* k) const {
+        auto it = props.find(k);
+        return it == props.end() ? 0.0 : it->second;
+    }
+};
+
+struct FhpGrads_ {
+    FhpScene scene = nullptr;
+    uint32_t hashSize = 1, dim = 0;
+    DevBuf tmp, agg, tex;
+    bool withTex = false;
+};
+
+namespace {
+// Property tables: names and defaults of the reference passes.
+const std::map<std::string, double>& mptDefaults() {
+    // MinimalPathTracer.h:79-83; useEmissiveNEE/emissiveSampler/countTraversal are extensions.
+    static const std::map<std::string, double> d = {{"maxBounces", 3}, {"computeDirect", 1}, {"useImportanceSampling", 1},
+                                                   {"useEmissiveNEE", 0}, {"emissiveSampler", 1}, {"countTraversal", 0}};
+    return d;
+}
+const std::map<std::string, double>& warDefaults() {
+    // WARDiffPathTracer.h:120-170 (StaticParams) and Params.slang.
+    static const std::map<std::string, double> d = {
+        {"samplesPerPixel", 1}, {"maxBounces", 0},        {"diffMode", 1},          {"sampleGenerator", 0},
+        {"fixedSeed", 0},       {"useFixedSeed", 0},      {"useBSDFSampling", 1},   {"useNEE", 1},
+        {"useMIS", 1},          {"useWAR", 0},            {"auxSampleCount", 16},   {"Log10vMFConcentration", 5},
+        {"Log10vMFConcentrationScreen", 5},               {"boundaryTermBeta", 0.01}, {"useAntitheticSampling", 1},
+        {"countTraversal", 0}};
+    return d;
+}
+
+int ensurePassBuffers(FhpPass p, uint32_t W, uint32_t H, uint32_t spp) {
+    const uint64_t n = uint64_t(W) * H * spp, npix = uint64_t(W) * H;
+    if (n > 0xffffffffull / 3) return fail(FHP_EINVAL, "frame too large");
+    if (n <= p->cap && npix <= p->capPix) return FHP_OK;
+    HIP_OR_FAIL(p->thp.alloc(n * 16));
+    HIP_OR_FAIL(p->prevN.alloc(n * 16));
+    HIP_OR_FAIL(p->rng.alloc(n * 16));
+    HIP_OR_FAIL(p->Lp.alloc(n * 16));
+    HIP_OR_FAIL(p->Ls.alloc(n * 16));
+    HIP_OR_FAIL(p->rayQ0.alloc(n * 32));
+    HIP_OR_FAIL(p->rayQ1.alloc(n * 32));
+    HIP_OR_FAIL(p->pathQ0.alloc(n * 4));
+    HIP_OR_FAIL(p->pathQ1.alloc(n * 4));
+    HIP_OR_FAIL(p->hitQ.alloc(n * 16));
+    HIP_OR_FAIL(p->shQ.alloc(n * 48));
+    HIP_OR_FAIL(p->pixHit.alloc(npix * 16));
+    HIP_OR_FAIL(p->counters.alloc(2 * kCounterStride * 4));
+    HIP_OR_FAIL(p->stats.alloc(64));
+    p->cap = uint32_t(n);
+    p->capPix = uint32_t(npix);
+    return FHP_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int fhp_get_version(void) { return 100; }
+const char* fhp_last_error(void) { return g_error.c_str(); }
+const char* fhp_last_warning(void) { return g_warning.c_str(); }
+
+// ------------------------------------------------------------------ scene
+int fhp_scene_create(const FhpSceneDesc* d, FhpScene* out) {
+    if (!d || !out) return fail(FHP_EINVAL, "null argument");
+    if (!d->h_positions || !d->h_normals || !d->h_texcoords || !d->h_indices || !d->h_tri_material || !d->h_materials ||
+        !d->h_camera)
+        return fail(FHP_EINVAL, "scene description has a null array");
+    if (d->num_triangles == 0 || d->num_vertices == 0) return fail(FHP_EBUILD, "scene has no triangles");
+    if (d->num_textures && (!d->h_tex_desc || !d->h_texels)) return fail(FHP_EINVAL, "texture arrays missing");
+    std::unique_ptr<FhpScene_> s(new FhpScene_());
+    HostScene& h = s->h;
+    h.nV = d->num_vertices; h.nT = d->num_triangles; h.nM = d->num_materials; h.nTex = d->num_textures;
+    h.pos.assign(d->h_positions, d->h_positions + 3 * size_t(h.nV));
+    h.nrm.assign(d->h_normals, d->h_normals + 3 * size_t(h.nV));
+    h.uv.assign(d->h_texcoords, d->h_texcoords + 2 * size_t(h.nV));
+    h.idx.assign(d->h_indices, d->h_indices + 3 * size_t(h.nT));
+    h.triMat.assign(d->h_tri_material, d->h_tri_material + h.nT);
+    h.mats.assign(d->h_materials, d->h_materials + size_t(h.nM) * M_STRIDE);
+    for (uint32_t i : h.idx)
+        if (i >= h.nV) return fail(FHP_EINVAL, "vertex index out of range");
+    for (uint32_t m : h.triMat)
+        if (m >= h.nM) return fail(FHP_EINVAL, "material index out of range");
+    if (h.nTex) {
+        h.texDesc.assign(d->h_tex_desc, d->h_tex_desc + 4 * size_t(h.nTex));
+        h.texels.assign(d->h_texels, d->h_texels + d->num_texels_floats);
+        for (uint32_t t = 0; t < h.nTex; t++) {
+            int64_t off = h.texDesc[4 * t], w = h.texDesc[4 * t + 1], hh = h.texDesc[4 * t + 2];
+            if (off < 0 || w <= 0 || hh <= 0 || uint64_t(off + 3 * w * hh) > d->num_texels_floats)
+                return fail(FHP_EINVAL, "texture descriptor out of range");
+        }
+    }
+    for (uint32_t m = 0; m < h.nM; m++) {
+        int tex = int(h.mats[size_t(m) * M_STRIDE + M_TEX]);
+        if (tex >= int(h.nTex)) return fail(FHP_EINVAL, "material texture index out of range");
+    }
+    std::memcpy(h.cam, d->h_camera, sizeof(h.cam));
+    try {
+        computeCamera(h);
+        buildBVH4(h, 4);
+        buildLights(h);
+        buildLightBVH(h);
+    } catch (const std::exception& e) {
+        return fail(FHP_EBUILD, e.what());
+    }
+    // Vertex records: (p.xyz, n.x) (n.yz, uv).
+    std::vector<float> vtx(8 * size_t(h.nV));
+    for (uint32_t i = 0; i < h.nV; i++) {
+        float* q = &vtx[8 * size_t(i)];
+        q[0] = h.pos[3 * i]; q[1] = h.pos[3 * i + 1]; q[2] = h.pos[3 * i + 2]; q[3] = h.nrm[3 * i];
+        q[4] = h.nrm[3 * i + 1]; q[5] = h.nrm[3 * i + 2]; q[6] = h.uv[2 * i]; q[7] = h.uv[2 * i + 1];
+    }
+    HIP_OR_FAIL(s->bvh.upload(h.bvh));
+    HIP_OR_FAIL(s->triPos.upload(h.triPos));
+    HIP_OR_FAIL(s->idx.upload(h.idx));
+    HIP_OR_FAIL(s->vtx.upload(vtx));
+    HIP_OR_FAIL(s->triMat.upload(h.triMat));
+    HIP_OR_FAIL(s->mats.upload(h.mats));
+    HIP_OR_FAIL(s->texels.upload(h.texels));
+    HIP_OR_FAIL(s->texDesc.upload(h.texDesc));
+    HIP_OR_FAIL(s->lights.upload(h.lights));
+    HIP_OR_FAIL(s->active.upload(h.activeLights));
+    HIP_OR_FAIL(s->triToLight.upload(h.triToLight));
+    HIP_OR_FAIL(s->lbvh.upload(h.lbvh));
+    HIP_OR_FAIL(s->lbvhTri.upload(h.lbvhTri));
+    HIP_OR_FAIL(s->lbvhMask.upload(h.lbvhMask));
+    // Overflow stack for the persistent traversal grid.
+    const uint32_t need = h.stackNeed + 1;
+    s->ts.gridBlocks = uint32_t(persistentBlocks());
+    s->ts.ovfDepth = need > uint32_t(kLdsDepth) ? need - uint32_t(kLdsDepth) : 0u;
+    if (s->ts.ovfDepth) HIP_OR_FAIL(s->ovf.alloc(size_t(s->ts.ovfDepth) * s->ts.gridBlocks * kBlock * 4));
+    s->ts.ovf = s->ovf.as<uint32_t>();
+
+    SceneDev& dv = s->dev;
+    dv.bvh = s->bvh.as<float4>();
+    dv.triPos = s->triPos.as<float4>();
+    dv.idx = s->idx.as<uint32_t>();
+    dv.vtx = s->vtx.as<float4>();
+    dv.triMat = s->triMat.as<uint32_t>();
+    dv.mats = s->mats.as<float>();
+    dv.texels = s->texels.as<float>();
+    dv.texDesc = s->texDesc.as<int4>();
+    dv.lights = s->lights.as<float4>();
+    dv.activeLights = s->active.as<uint32_t>();
+    dv.triToLight = s->triToLight.as<uint32_t>();
+    dv.lbvh = s->lbvh.as<float4>();
+    dv.lbvhTri = s->lbvhTri.as<uint32_t>();
+    dv.lbvhMask = s->lbvhMask.as<uint2>();
+    dv.cam.pos = mk3(h.cam[0], h.cam[1], h.cam[2]);
+    dv.cam.U = mk3(h.camU[0], h.camU[1], h.camU[2]);
+    dv.cam.V = mk3(h.camV[0], h.camV[1], h.camV[2]);
+    dv.cam.W = mk3(h.camW[0], h.camW[1], h.camW[2]);
+    dv.cam.nearZ = h.cam[12];
+    dv.cam.farZ = h.cam[13];
+    dv.nTris = h.nT;
+    dv.nLights = h.nLights;
+    dv.nActive = uint32_t(h.activeLights.size());
+    dv.nLbvh = h.nLbvh;
+    for (DevBuf* b : {&s->bvh, &s->triPos, &s->idx, &s->vtx, &s->triMat, &s->mats, &s->texels, &s->texDesc, &s->lights,
+                      &s->active, &s->triToLight, &s->lbvh, &s->lbvhTri, &s->lbvhMask, &s->ovf})
+        s->deviceBytes += b->bytes;
+    *out = s.release();
+    return FHP_OK;
+}
+
+int fhp_scene_destroy(FhpScene s) {
+    delete s;
+    return FHP_OK;
+}
+
+int fhp_scene_info(FhpScene s, FhpSceneInfo* o) {
+    if (!s || !o) return fail(FHP_EINVAL, "null argument");
+    const HostScene& h = s->h;
+    o->num_triangles = h.nT; o->num_vertices = h.nV; o->num_materials = h.nM; o->num_textures = h.nTex;
+    o->bvh_nodes = h.bvhNodes; o->bvh_leaves = h.bvhLeaves; o->bvh_depth = h.bvhDepth; o->stack_need = h.stackNeed;
+    o->num_lights = h.nLights; o->num_active_lights = uint32_t(h.activeLights.size()); o->light_bvh_nodes = h.nLbvh;
+    o->bvh_sah_cost = h.bvhSah;
+    o->device_bytes = s->deviceBytes;
+    for (int i = 0; i < 3; i++) {
+        o->camera_basis[i] = h.camU[i]; o->camera_basis[3 + i] = h.camV[i]; o->camera_basis[6 + i] = h.camW[i];
+    }
+    return FHP_OK;
+}
+
+int fhp_scene_set_materials(FhpScene s, const float* mats) {
+    if (!s || !mats) return fail(FHP_EINVAL, "null argument");
+    for (uint32_t m = 0; m < s->h.nM; m++) {
+        int tex = int(mats[size_t(m) * M_STRIDE + M_TEX]);
+        if (tex >= int(s->h.nTex)) return fail(FHP_EINVAL, "material texture index out of range");
+    }
+    s->h.mats.assign(mats, mats + size_t(s->h.nM) * M_STRIDE);
+    HIP_OR_FAIL(hipMemcpy(s->mats.p, mats, size_t(s->h.nM) * M_STRIDE * 4, hipMemcpyHostToDevice));
+    return FHP_OK;
+}
+
+int fhp_scene_set_texels(FhpScene s, const float* d_texels, uint64_t n, FhpStream stream) {
+    if (!s || !d_texels) return fail(FHP_EINVAL, "null argument");
+    if (n != s->h.texels.size()) return fail(FHP_EINVAL, "texel count mismatch");
+    HIP_OR_FAIL(hipMemcpyAsync(s->texels.p, d_texels, n * 4, hipMemcpyDeviceToDevice, hipStream_t(stream)));
+    return FHP_OK;
+}
+
+int fhp_scene_light_bvh(FhpScene s, float* nodes, uint32_t* tri, uint32_t* masks) {
+    if (!s) return fail(FHP_EINVAL, "null argument");
+    if (nodes) std::copy(s->h.lbvh.begin(), s->h.lbvh.end(), nodes);
+    if (tri) std::copy(s->h.lbvhTri.begin(), s->h.lbvhTri.end(), tri);
+    if (masks) std::copy(s->h.lbvhMask.begin(), s->h.lbvhMask.end(), masks);
+    return FHP_OK;
+}
+
+// ------------------------------------------------------------------ ray queries
+int fhp_trace_closest(FhpScene s, const float* d_rays, uint32_t n, uint32_t* d_hits, FhpStream stream) {
+    if (!s || (n && (!d_rays || !d_hits))) return fail(FHP_EINVAL, "null argument");
+    HIP_OR_FAIL(launchTraceRays(s->dev, reinterpret_cast<const float4*>(d_rays), n, reinterpret_cast<uint4*>(d_hits), nullptr,
+                                false, s->ts, hipStream_t(stream)));
+    return FHP_OK;
+}
+int fhp_trace_any(FhpScene s, const float* d_rays, uint32_t n, uint32_t* d_occ, FhpStream stream) {
+    if (!s || (n && (!d_rays || !d_occ))) return fail(FHP_EINVAL, "null argument");
+    HIP_OR_FAIL(launchTraceRays(s->dev, reinterpret_cast<const float4*>(d_rays), n, nullptr, d_occ, true, s->ts,
+                                hipStream_t(stream)));
+    return FHP_OK;
+}
+
+// ------------------------------------------------------------------ passes
+int fhp_pass_create(const char* kind, int np, const char* const* keys, const double* values, FhpPass* out) {
+    if (!kind || !out || (np > 0 && (!keys || !values))) return fail(FHP_EINVAL, "null argument");
+    std::string k(kind);
+    const std::map<std::string, double>* defs;
+    if (k == "MinimalPathTracer") defs = &mptDefaults();
+    else if (k == "WARDiffPathTracer") defs = &warDefaults();
+    else return fail(FHP_EINVAL, "unknown pass kind '" + k + "'");
+    std::unique_ptr<FhpPass_> p(new FhpPass_());
+    p->kind = k;
+    p->props = *defs;
+    g_warning.clear();
+    for (int i = 0; i < np; i++) {
+        if (!keys[i]) return fail(FHP_EINVAL, "null property key");
+        std::string key(keys[i]);
+        if (!defs->count(key)) {
+            g_warning = "Unknown property '" + key + "' in " + k + " properties.";
+            continue;
+        }
+        p->props[key] = values[i];
+        if (key == "fixedSeed") p->props["useFixedSeed"] = 1;
+    }
+    if (p->get("maxBounces") < 0 || p->get("maxBounces") > kMaxBounceSlots - 2)
+        return fail(FHP_EINVAL, "maxBounces out of range [0, 28]");
+    *out = p.release();
+    return FHP_OK;
+}
+int fhp_pass_destroy(FhpPass p) {
+    delete p;
+    return FHP_OK;
+}
+int fhp_pass_set_property(FhpPass p, const char* key, double v) {
+    if (!p || !key) return fail(FHP_EINVAL, "null argument");
+    if (!p->props.count(key)) {
+        g_warning = std::string("Unknown property '") + key + "' in " + p->kind + " properties.";
+        return FHP_OK;
+    }
+    if (std::string(key) == "maxBounces" && (v < 0 || v > kMaxBounceSlots - 2)) return fail(FHP_EINVAL, "maxBounces out of range");
+    p->props[key] = v;
+    if (std::string(key) == "fixedSeed") p->props["useFixedSeed"] = 1;
+    return FHP_OK;
+}
+int fhp_pass_get_property(FhpPass p, const char* key, double* v) {
+    if (!p || !key || !v) return fail(FHP_EINVAL, "null argument");
+    auto it = p->props.find(key);
+    if (it == p->props.end()) return fail(FHP_EINVAL, std::string("no property '") + key + "'");
+    *v = it->second;
+    return FHP_OK;
+}
+
+int fhp_pass_execute(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t spp, uint32_t frame, float* d_color,
+                     uint64_t* h_stats, FhpStream stream) {
+    if (!p || !d_color) return fail(FHP_EINVAL, "null argument");
+    if (W == 0 || H == 0 || spp == 0) return fail(FHP_EINVAL, "empty frame");
+    if (W > 65536 || H > 65536) return fail(FHP_EINVAL, "frame dimension above 65536");
+    hipStream_t st = hipStream_t(stream);
+    if (!s) {
+        // No scene: clear the output (MinimalPathTracer.cpp:121-130).
+        HIP_OR_FAIL(launchFill(d_color, uint64_t(W) * H * 4, 0.f, st));
+        return FHP_OK;
+    }
+    int rc = ensurePassBuffers(p, W, H, spp);
+    if (rc) return rc;
+    const bool count = p->get("countTraversal") != 0;
+    unsigned long long* dstats = p->stats.as<unsigned long long>();
+    if (h_stats) HIP_OR_FAIL(launchFill(reinterpret_cast<float*>(dstats), 8, 0.f, st));
+    if (p->kind == "MinimalPathTracer") {
+        MptArgs a{};
+        a.W = W; a.H = H; a.spp = spp; a.frame = frame;
+        a.maxBounces = uint32_t(p->get("maxBounces"));
+        a.computeDirect = p->get("computeDirect") != 0;
+        a.importance = p->get("useImportanceSampling") != 0;
+        a.nee = p->get("useEmissiveNEE") != 0;
+        a.lightKind = uint32_t(p->get("emissiveSampler"));
+        if (a.nee && !a.computeDirect) return fail(FHP_EINVAL, "useEmissiveNEE requires computeDirect");
+        a.nPaths = W * H * spp;
+        a.thp = p->thp.as<float4>(); a.prevN = p->prevN.as<float4>(); a.rng = p->rng.as<uint4>();
+        a.Lp = p->Lp.as<float4>(); a.Ls = p->Ls.as<float4>();
+        a.rayQ[0] = p->rayQ0.as<float4>(); a.rayQ[1] = p->rayQ1.as<float4>();
+        a.pathQ[0] = p->pathQ0.as<uint32_t>(); a.pathQ[1] = p->pathQ1.as<uint32_t>();
+        a.hitQ = p->hitQ.as<uint4>(); a.shQ = p->shQ.as<float4>(); a.pixHit = p->pixHit.as<uint4>();
+        a.counters = p->counters.as<uint32_t>();
+        HIP_OR_FAIL(launchMptFrame(s->dev, a, s->ts, reinterpret_cast<float4*>(d_color), count, h_stats ? dstats : nullptr, st));
+    } else {
+        WarArgs a{};
+        a.W = W; a.H = H; a.spp = spp;
+        a.frame = p->get("useFixedSeed") != 0 ? uint32_t(p->get("fixedSeed")) : frame * spp;
+        a.maxBounces = uint32_t(p->get("maxBounces"));
+        a.useBSDFSampling = p->get("useBSDFSampling") != 0;
+        a.useNEE = p->get("useNEE") != 0;
+        a.useMIS = p->get("useMIS") != 0;
+        a.useAntithetic = p->get("useAntitheticSampling") != 0;
+        a.out = reinterpret_cast<float4*>(d_color);
+        a.tmp = p->Lp.as<float4>();
+        a.rays = h_stats ? dstats : nullptr;
+        HIP_OR_FAIL(launchWarPrimal(s->dev, a, s->ts, st));
+    }
+    if (h_stats) {
+        unsigned long long hs[4];
+        HIP_OR_FAIL(hipMemcpyAsync(hs, dstats, 32, hipMemcpyDeviceToHost, st));
+        HIP_OR_FAIL(hipStreamSynchronize(st));
+        for (int i = 0; i < 4; i++) h_stats[i] = hs[i];
+    }
+    return FHP_OK;
+}
+
+// ------------------------------------------------------------------ gradients
+int fhp_grads_create(FhpScene s, uint32_t hashSize, int withTex, FhpGrads* out) {
+    if (!s || !out) return fail(FHP_EINVAL, "null argument");
+    if (hashSize == 0 || hashSize > 65536) return fail(FHP_EINVAL, "hash size must be in [1, 65536]");
+    std::unique_ptr<FhpGrads_> g(new FhpGrads_());
+    g->scene = s;
+    g->hashSize = hashSize;
+    g->dim = s->h.nM * 20;
+    HIP_OR_FAIL(g->tmp.alloc(size_t(g->dim) * hashSize * 4));
+    HIP_OR_FAIL(g->agg.alloc(size_t(g->dim) * 4));
+    g->withTex = withTex != 0 && !s->h.texels.empty();
+    if (g->withTex) HIP_OR_FAIL(g->tex.alloc(s->h.texels.size() * 4));
+    *out = g.release();
+    return fhp_grads_clear(*out, nullptr);
+}
+int fhp_grads_destroy(FhpGrads g) {
+    delete g;
+    return FHP_OK;
+}
+int fhp_grads_clear(FhpGrads g, FhpStream stream) {
+    if (!g) return fail(FHP_EINVAL, "null argument");
+    hipStream_t st = hipStream_t(stream);
+    HIP_OR_FAIL(launchFill(g->tmp.as<float>(), uint64_t(g->dim) * g->hashSize, 0.f, st));
+    HIP_OR_FAIL(launchFill(g->agg.as<float>(), g->dim, 0.f, st));
+    if (g->withTex) HIP_OR_FAIL(launchFill(g->tex.as<float>(), g->scene->h.texels.size(), 0.f, st));
+    return FHP_OK;
+}
+int fhp_grads_aggregate(FhpGrads g, FhpStream stream) {
+    if (!g) return fail(FHP_EINVAL, "null argument");
+    HIP_OR_FAIL(launchAggregate(g->tmp.as<float>(), g->hashSize, g->dim, g->agg.as<float>(), hipStream_t(stream)));
+    return FHP_OK;
+}
+int fhp_grads_get(FhpGrads g, float** dm, uint64_t* nm, float** dt, uint64_t* nt) {
+    if (!g) return fail(FHP_EINVAL, "null argument");
+    if (dm) *dm = g->agg.as<float>();
+    if (nm) *nm = g->dim;
+    if (dt) *dt = g->withTex ? g->tex.as<float>() : nullptr;
+    if (nt) *nt = g->withTex ? g->scene->h.texels.size() : 0;
+    return FHP_OK;
+}
+
+int fhp_grads_copy(FhpGrads g, int which, float* dst, uint64_t n, FhpStream stream) {
+    if (!g || !dst) return fail(FHP_EINVAL, "null argument");
+    const void* src = which == 0 ? g->agg.p : g->tex.p;
+    const uint64_t have = which == 0 ? g->dim : (g->withTex ? g->scene->h.texels.size() : 0);
+    if (which != 0 && which != 1) return fail(FHP_EINVAL, "which must be 0 (material) or 1 (texture)");
+    if (n != have || have == 0) return fail(FHP_EINVAL, "gradient size mismatch");
+    HIP_OR_FAIL(hipMemcpyAsync(dst, src, n * 4, hipMemcpyDeviceToDevice, hipStream_t(stream)));
+    return FHP_OK;
+}
+
+int fhp_pass_backward(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t spp, uint32_t frame, const float* d_dLdI,
+                      FhpGrads g, uint64_t* h_stats, FhpStream stream) {
+    if (!p || !s || !d_dLdI || !g) return fail(FHP_EINVAL, "null argument");
+    if (p->kind != "WARDiffPathTracer") return fail(FHP_ESTATE, "backward needs a WARDiffPathTracer pass");
+    if (g->scene != s) return fail(FHP_EINVAL, "gradients belong to another scene");
+    if (W == 0 || H == 0 || spp == 0) return fail(FHP_EINVAL, "empty frame");
+    if (p->get("useWAR") != 0) return fail(FHP_EINVAL, "useWAR=1 (vertex gradients) is not available in this build");
+    hipStream_t st = hipStream_t(stream);
+    int rc = ensurePassBuffers(p, 1, 1, 1);
+    if (rc) return rc;
+    unsigned long long* dstats = p->stats.as<unsigned long long>();
+    if (h_stats) HIP_OR_FAIL(launchFill(reinterpret_cast<float*>(dstats), 8, 0.f, st));
+    WarArgs a{};
+    a.W = W; a.H = H; a.spp = spp;
+    a.frame = p->get("useFixedSeed") != 0 ? uint32_t(p->get("fixedSeed")) : frame * spp;
+    a.maxBounces = uint32_t(p->get("maxBounces"));
+    a.useBSDFSampling = p->get("useBSDFSampling") != 0;
+    a.useNEE = p->get("useNEE") != 0;
+    a.useMIS = p->get("useMIS") != 0;
+    a.useAntithetic = p->get("useAntitheticSampling") != 0;
+    a.dLdI = d_dLdI;
+    a.gradMat = g->tmp.as<float>();
+    a.hashSize = g->hashSize;
+    a.matGradDim = g->dim;
+    a.gradTex = g->withTex ? g->tex.as<float>() : nullptr;
+    a.rays = h_stats ? dstats : nullptr;
+    HIP_OR_FAIL(launchWarAdjoint(s->dev, a, s->ts, st));
+    if (h_stats) {
+        unsigned long long hs[4];
+        HIP_OR_FAIL(hipMemcpyAsync(hs, dstats, 32, hipMemcpyDeviceToHost, st));
+        HIP_OR_FAIL(hipStreamSynchronize(st));
+        for (int i = 0; i < 4; i++) h_stats[i] = hs[i];
+    }
+    return FHP_OK;
+}
+
+}  // extern "C"

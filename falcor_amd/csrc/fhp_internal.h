@@ -1,0 +1,66 @@
+// Internal interfaces between the C-ABI host code and the HIP kernels of libfhp.
+#pragma once
+#include "fhp_scene.h"
+
+namespace fhp {
+
+constexpr int kBlock = 128;      // threads per block for traversal kernels (2 waves)
+constexpr int kLdsDepth = 24;    // LDS stack entries per lane (24 * 128 * 4 B = 12 KiB per block)
+constexpr int kMaxBounceSlots = 30;
+constexpr int kCounterStride = 32;  // counters: [0, 32) ray queues by depth, [32, 64) shadow queues
+
+// Per-path state and queues of the wavefront forward pass. Slots are indexed by path id
+// (pixel * spp + sample); queues are compacted each bounce.
+struct MptArgs {
+    uint32_t W, H, spp, frame, maxBounces;
+    uint32_t computeDirect, importance, nee, lightKind;
+    uint32_t nPaths;
+    float4* thp;     // thp.xyz, pdf of the last scatter
+    float4* prevN;   // shading normal of the last scatter, flags (bit 0: delta lobe)
+    uint4* rng;      // xoshiro128** state
+    float4* Lp;      // radiance added at the primary hit (outColor of tracePath)
+    float4* Ls;      // radiance gathered along the scatter chain (rayData.radiance)
+    float4* rayQ[2]; // 2 float4 per ray: (o, tmin) (d, tmax)
+    uint32_t* pathQ[2];
+    uint4* hitQ;     // (tri, t, b1, b2)
+    float4* shQ;     // 3 float4 per shadow ray: (o, pathId) (d, tmax) (contrib, flags)
+    uint4* pixHit;   // primary hits per pixel
+    uint32_t* counters;
+};
+
+struct TraceScratch {
+    uint32_t* ovf;     // overflow stack (ovfDepth words per thread of the persistent grid)
+    uint32_t ovfDepth;
+    uint32_t gridBlocks;
+};
+
+struct KernelStats {
+    unsigned long long* d;  // [nodes, tris] when counting is enabled
+};
+
+// Launchers (fhp_kernels.hip). All are asynchronous on `stream`.
+hipError_t launchTraceRays(const SceneDev& s, const float4* rays, uint32_t n, uint4* hits, uint32_t* occluded, bool any,
+                           const TraceScratch& ts, hipStream_t stream);
+hipError_t launchMptFrame(const SceneDev& s, const MptArgs& a, const TraceScratch& ts, float4* out, bool count,
+                          unsigned long long* d_counts, hipStream_t stream);
+
+// WARDiffPathTracer primal and adjoint (megakernel, one path per thread).
+struct WarArgs {
+    uint32_t W, H, spp, frame, maxBounces;
+    uint32_t useBSDFSampling, useNEE, useMIS, useAntithetic;
+    float4* out;            // primal image (RGBA)
+    float4* tmp;            // per-sample radiance (W * H * spp)
+    const float* dLdI;      // adjoint input (3 floats per pixel)
+    float* gradMat;         // material gradients [hashSize][nMat * 20]
+    uint32_t hashSize, matGradDim;
+    float* gradTex;         // per-texel baseColor gradients (texel layout), or null
+    unsigned long long* rays;  // [closest, visibility] counters, or null
+};
+hipError_t launchWarPrimal(const SceneDev& s, const WarArgs& a, const TraceScratch& ts, hipStream_t stream);
+hipError_t launchWarAdjoint(const SceneDev& s, const WarArgs& a, const TraceScratch& ts, hipStream_t stream);
+hipError_t launchAggregate(const float* tmp, uint32_t hashSize, uint32_t dim, float* out, hipStream_t stream);
+hipError_t launchFill(float* p, uint64_t n, float v, hipStream_t stream);
+
+int persistentBlocks();
+
+}  // namespace fhp

@@ -1,0 +1,377 @@
+// Wavefront kernels of the MinimalPathTracer forward pass and the raw ray-query kernels.
+//
+// Frame structure (one launch sequence per frame, fixed grid shapes so it can be captured):
+//   primary rays (per pixel) -> closest hit -> shadePrimary (per path: emission, NEE, scatter)
+//   -> shadow rays -> for depth 0..maxBounces: closest hit -> shade -> shadow rays -> resolve.
+// Queues are compacted with a wave ballot + one atomic per wave; per-path state lives in
+// path-indexed slots so queue order never changes results.
+#include "fhp_internal.h"
+#include "fhp_trace.h"
+
+namespace fhp {
+
+// ------------------------------------------------------------------ helpers
+__device__ __forceinline__ uint32_t waveAppend(bool pred, uint32_t* counter) {
+    const uint64_t mask = __ballot(pred);
+    uint32_t base = 0;
+    if (mask) {
+        const int leader = __ffsll((unsigned long long)mask) - 1;
+        if (int(__lane_id()) == leader) base = atomicAdd(counter, uint32_t(__popcll(mask)));
+        base = __shfl(base, leader);
+    }
+    return base + uint32_t(__popcll(mask & __lanemask_lt()));
+}
+
+__device__ __forceinline__ TraceStack makeStack(uint32_t* lds, const TraceScratch& ts) {
+    const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
+    return TraceStack{lds + threadIdx.x, uint32_t(kBlock), uint32_t(kLdsDepth), ts.ovf + gtid, ts.gridBlocks * kBlock,
+                      ts.ovfDepth};
+}
+
+int persistentBlocks() {
+    static int blocks = 0;
+    if (!blocks) {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            cus = 256;
+        // 8 blocks of 2 waves per CU: 16 waves per CU (4 per SIMD) keeps the LDS stack at 96 KiB/CU.
+        blocks = cus * 8;
+    }
+    return blocks;
+}
+
+// ------------------------------------------------------------------ raw ray queries
+template <bool ANY>
+__global__ __launch_bounds__(kBlock) void kTraceRays(SceneDev s, const float4* __restrict__ rays, uint32_t n,
+                                                     uint4* __restrict__ hits, uint32_t* __restrict__ occl, TraceScratch ts) {
+    __shared__ uint32_t lds[kLdsDepth * kBlock];
+    TraceStack st = makeStack(lds, ts);
+    const uint32_t G = gridDim.x * kBlock;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += G) {
+        const float4 a = rays[2 * i], b = rays[2 * i + 1];
+        HitRec h;
+        bool hit = traceBVH4<ANY, false>(s, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), a.w, b.w, st, h);
+        if (ANY) occl[i] = hit ? 1u : 0u;
+        else hits[i] = make_uint4(h.tri, f2u(h.t), f2u(h.b1), f2u(h.b2));
+    }
+}
+
+hipError_t launchTraceRays(const SceneDev& s, const float4* rays, uint32_t n, uint4* hits, uint32_t* occluded, bool any,
+                           const TraceScratch& ts, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    uint32_t blocks = std::min<uint32_t>(ts.gridBlocks, (n + kBlock - 1) / kBlock);
+    if (any) hipLaunchKernelGGL(kTraceRays<true>, dim3(blocks), dim3(kBlock), 0, stream, s, rays, n, hits, occluded, ts);
+    else hipLaunchKernelGGL(kTraceRays<false>, dim3(blocks), dim3(kBlock), 0, stream, s, rays, n, hits, occluded, ts);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ MinimalPathTracer wavefront
+__global__ void kZeroU32(uint32_t* p, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+// Pinhole primary ray through the pixel centre (Camera.slang:46-77, VBufferRT.slang:56-66).
+__device__ __forceinline__ void primaryRay(const CameraDev& c, uint32_t px, uint32_t py, uint32_t W, uint32_t H, f3& o, f3& d,
+                                           float& tmin, float& tmax) {
+    float p_x = (float(px) + 0.5f) / float(W), p_y = (float(py) + 0.5f) / float(H);
+    float ndc_x = 2.f * p_x + -1.f, ndc_y = -2.f * p_y + 1.f;
+    f3 dir = c.U * ndc_x + c.V * ndc_y + c.W;
+    d = normalize(dir);
+    o = c.pos;
+    float invCos = 1.f / dot(normalize(c.W), d);
+    tmin = c.nearZ * invCos;
+    tmax = c.farZ * invCos;
+}
+
+__global__ void kPrimaryRays(SceneDev s, MptArgs a) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.W * a.H) return;
+    f3 o, d;
+    float tmin, tmax;
+    primaryRay(s.cam, p % a.W, p / a.W, a.W, a.H, o, d, tmin, tmax);
+    a.rayQ[0][2 * p] = make_float4(o.x, o.y, o.z, tmin);
+    a.rayQ[0][2 * p + 1] = make_float4(d.x, d.y, d.z, tmax);
+}
+
+template <bool ANY, bool COUNT>
+__global__ __launch_bounds__(kBlock) void kTraceQueue(SceneDev s, const float4* __restrict__ rays, const uint32_t* countPtr,
+                                                      uint32_t countConst, uint4* __restrict__ hits, TraceScratch ts,
+                                                      unsigned long long* stats) {
+    __shared__ uint32_t lds[kLdsDepth * kBlock];
+    TraceStack st = makeStack(lds, ts);
+    const uint32_t n = countPtr ? *countPtr : countConst;
+    const uint32_t G = gridDim.x * kBlock;
+    TraceCounters cnt{0, 0};
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += G) {
+        const float4 ra = rays[2 * i], rb = rays[2 * i + 1];
+        HitRec h;
+        traceBVH4<ANY, COUNT>(s, mk3(ra.x, ra.y, ra.z), mk3(rb.x, rb.y, rb.z), ra.w, rb.w, st, h, &cnt);
+        hits[i] = make_uint4(h.tri, f2u(h.t), f2u(h.b1), f2u(h.b2));
+    }
+    if (COUNT) {
+        atomicAdd(&stats[0], (unsigned long long)cnt.nodes);
+        atomicAdd(&stats[1], (unsigned long long)cnt.tris);
+    }
+}
+
+// Emissive next-event estimation at a shading point (extension of MinimalPathTracer; semantics of
+// PTUtils.slang:177-249 without reparameterization). Produces an unoccluded contribution and the
+// shadow ray that gates it.
+__device__ bool neeSample(const SceneDev& s, uint32_t kind, const Shading& sd, const Material& m, Rng& rng, f3& org, f3& dir,
+                          float& dist, f3& contrib) {
+    LightSample ls;
+    if (!sampleLight(s, int(kind), sd.posW, sd.N, rng, ls)) return false;
+    if (!(ls.pdf > 0.f)) return false;
+    f3 Li = ls.Le / ls.pdf;
+    if (!anyPos(Li)) return false;
+    org = rayOriginOffset(sd.posW, dot(sd.faceN, ls.dir) >= 0.f ? sd.faceN : -sd.faceN);
+    f3 lightPos = rayOriginOffset(ls.posW, ls.normalW);
+    f3 toL = lightPos - org;
+    dist = length(toL);
+    dir = normalize(toL);
+    f3 f = matEval(sd, m, dir);
+    if (!anyPos(f)) return false;
+    float mis = misPower(ls.pdf, matPdf(sd, m, dir));
+    contrib = f * Li * mis;
+    return true;
+}
+
+__device__ __forceinline__ void pushShadow(const MptArgs& a, uint32_t slot, f3 o, f3 d, float tmax, f3 c, uint32_t pathId,
+                                           uint32_t flags) {
+    float4* q = a.shQ + 3 * size_t(slot);
+    q[0] = make_float4(o.x, o.y, o.z, u2f(pathId));
+    q[1] = make_float4(d.x, d.y, d.z, tmax);
+    q[2] = make_float4(c.x, c.y, c.z, u2f(flags));
+}
+__device__ __forceinline__ void pushRay(const MptArgs& a, int q, uint32_t slot, f3 o, f3 d, uint32_t pathId) {
+    a.rayQ[q][2 * size_t(slot)] = make_float4(o.x, o.y, o.z, 0.f);
+    a.rayQ[q][2 * size_t(slot) + 1] = make_float4(d.x, d.y, d.z, kFltMax);
+    a.pathQ[q][slot] = pathId;
+}
+
+// Primary-hit shading, one thread per path (MinimalPathTracer.rt.slang:392-449).
+__global__ void kShadePrimary(SceneDev s, MptArgs a) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.nPaths) return;
+    const uint32_t pix = p / a.spp, k = p % a.spp;
+    const uint32_t px = pix % a.W, py = pix / a.W;
+    const uint4 h = a.pixHit[pix];
+    bool cont = false, shadow = false;
+    f3 newO, newD, shO, shD, shC;
+    float shT = 0.f;
+    if (h.x != kInvalid) {
+        f3 o, d;
+        float tmin, tmax;
+        primaryRay(s.cam, px, py, a.W, a.H, o, d, tmin, tmax);
+        Shading sd = shadingFromHit(s, h.x, u2f(h.z), u2f(h.w), d);
+        Material m = loadMaterial(s, s.mats, sd, sd, nullptr);
+        Rng rng = rngSeed(px, py, a.frame * a.spp + k);
+        f3 org = rayOriginFromShading(sd);
+        f3 Lp = mk3(0.f);
+        if (a.computeDirect) {
+            Lp = Lp + m.emission;
+            if (a.nee) shadow = neeSample(s, a.lightKind, sd, m, rng, shO, shD, shT, shC);
+        }
+        f3 thp = mk3(1.f);
+        Scatter sc;
+        float pdf = 0.f;
+        uint32_t flags = 0;
+        if (matSample(sd, m, rng, sc, a.importance != 0)) {
+            thp = thp * sc.weight;
+            cont = anyPos(thp);
+            newO = org;
+            newD = sc.wo;
+            pdf = sc.pdf;
+            flags = sc.delta ? 1u : 0u;
+        }
+        a.thp[p] = make_float4(thp.x, thp.y, thp.z, pdf);
+        a.prevN[p] = make_float4(sd.N.x, sd.N.y, sd.N.z, u2f(flags));
+        a.rng[p] = make_uint4(rng.s0, rng.s1, rng.s2, rng.s3);
+        a.Lp[p] = make_float4(Lp.x, Lp.y, Lp.z, 0.f);
+    } else {
+        a.Lp[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    a.Ls[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+    uint32_t slot = waveAppend(cont, &a.counters[0]);
+    if (cont) pushRay(a, 0, slot, newO, newD, p);
+    uint32_t sslot = waveAppend(shadow, &a.counters[kCounterStride + 0]);
+    if (shadow) pushShadow(a, sslot, shO, shD, shT, shC, p, 1u);
+}
+
+// handleHit for scatter depth `depth` (MinimalPathTracer.rt.slang:324-376), one thread per queued ray.
+__global__ void kShade(SceneDev s, MptArgs a, uint32_t depth) {
+    const int qi = depth & 1, qo = qi ^ 1;
+    const uint32_t n = a.counters[depth];
+    const uint32_t G = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += G) {
+        const uint32_t i = base + threadIdx.x;
+        bool cont = false, shadow = false;
+        f3 newO, newD, shO, shD, shC;
+        float shT = 0.f;
+        uint32_t pathId = 0;
+        if (i < n) {
+            pathId = a.pathQ[qi][i];
+            const uint4 h = a.hitQ[i];
+            if (h.x != kInvalid) {
+                const float4 ro = a.rayQ[qi][2 * size_t(i)], rd = a.rayQ[qi][2 * size_t(i) + 1];
+                const f3 o = mk3(ro.x, ro.y, ro.z), d = mk3(rd.x, rd.y, rd.z);
+                float4 t4 = a.thp[pathId], n4 = a.prevN[pathId];
+                f3 thp = mk3(t4.x, t4.y, t4.z);
+                const bool prevDelta = (f2u(n4.w) & 1u) != 0u;
+                uint4 r4 = a.rng[pathId];
+                Rng rng{r4.x, r4.y, r4.z, r4.w};
+                Shading sd = shadingFromHit(s, h.x, u2f(h.z), u2f(h.w), d);
+                Material m = loadMaterial(s, s.mats, sd, sd, nullptr);
+                float4 ls4 = a.Ls[pathId];
+                f3 Ls = mk3(ls4.x, ls4.y, ls4.z);
+                if (a.computeDirect || depth > 0) {
+                    f3 e = m.emission;
+                    if (a.nee && anyPos(e) && !prevDelta) {
+                        float lp = lightPdf(s, int(a.lightKind), o, mk3(n4.x, n4.y, n4.z), s.triToLight[h.x], sd.posW,
+                                            sd.orientedFaceN());
+                        e = e * misPower(t4.w, lp);
+                    }
+                    Ls = Ls + thp * e;
+                    a.Ls[pathId] = make_float4(Ls.x, Ls.y, Ls.z, 0.f);
+                }
+                if (depth < a.maxBounces) {
+                    f3 org = rayOriginFromShading(sd);
+                    if (a.nee) {
+                        shadow = neeSample(s, a.lightKind, sd, m, rng, shO, shD, shT, shC);
+                        if (shadow) shC = thp * shC;
+                    }
+                    Scatter sc;
+                    if (matSample(sd, m, rng, sc, a.importance != 0)) {
+                        thp = thp * sc.weight;
+                        cont = anyPos(thp);
+                        newO = org;
+                        newD = sc.wo;
+                        a.thp[pathId] = make_float4(thp.x, thp.y, thp.z, sc.pdf);
+                        a.prevN[pathId] = make_float4(sd.N.x, sd.N.y, sd.N.z, u2f(sc.delta ? 1u : 0u));
+                    }
+                    a.rng[pathId] = make_uint4(rng.s0, rng.s1, rng.s2, rng.s3);
+                }
+            }
+        }
+        uint32_t slot = waveAppend(cont, &a.counters[depth + 1]);
+        if (cont) pushRay(a, qo, slot, newO, newD, pathId);
+        uint32_t sslot = waveAppend(shadow, &a.counters[kCounterStride + depth + 1]);
+        if (shadow) pushShadow(a, sslot, shO, shD, shT, shC, pathId, 0u);
+    }
+}
+
+// Shadow rays: any hit in [0, dist]; an unoccluded ray adds its contribution to its path.
+template <bool COUNT>
+__global__ __launch_bounds__(kBlock) void kShadow(SceneDev s, MptArgs a, uint32_t slotIdx, TraceScratch ts,
+                                                  unsigned long long* stats) {
+    __shared__ uint32_t lds[kLdsDepth * kBlock];
+    TraceStack st = makeStack(lds, ts);
+    const uint32_t n = a.counters[kCounterStride + slotIdx];
+    const uint32_t G = gridDim.x * kBlock;
+    TraceCounters cnt{0, 0};
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += G) {
+        const float4* q = a.shQ + 3 * size_t(i);
+        const float4 q0 = q[0], q1 = q[1];
+        HitRec h;
+        bool occ = traceBVH4<true, COUNT>(s, mk3(q0.x, q0.y, q0.z), mk3(q1.x, q1.y, q1.z), 0.f, q1.w, st, h, &cnt);
+        if (!occ) {
+            const float4 q2 = q[2];
+            const uint32_t pathId = f2u(q0.w);
+            float4* L = (f2u(q2.w) & 1u) ? &a.Lp[pathId] : &a.Ls[pathId];
+            float4 v = *L;
+            *L = make_float4(v.x + q2.x, v.y + q2.y, v.z + q2.z, 0.f);
+        }
+    }
+    if (COUNT) {
+        atomicAdd(&stats[0], (unsigned long long)cnt.nodes);
+        atomicAdd(&stats[1], (unsigned long long)cnt.tris);
+    }
+}
+
+// outColor + radiance per sample, averaged over samples like AccumulatePass Single mode.
+__global__ void kResolve(MptArgs a, float4* out) {
+    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= a.W * a.H) return;
+    f3 sum = mk3(0.f);
+    for (uint32_t k = 0; k < a.spp; k++) {
+        const uint32_t p = pix * a.spp + k;
+        const float4 lp = a.Lp[p], ls = a.Ls[p];
+        sum = sum + (mk3(lp.x, lp.y, lp.z) + mk3(ls.x, ls.y, ls.z));
+    }
+    const float w = 1.0f / float(a.spp);
+    out[pix] = make_float4(sum.x * w, sum.y * w, sum.z * w, 1.f);
+}
+
+__global__ void kRayCounts(const MptArgs a, uint32_t nb, unsigned long long* out) {
+    // out[0] += closest-hit rays (primary + all scatter queues), out[1] += shadow rays.
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        unsigned long long c = (unsigned long long)a.W * a.H, v = 0;
+        for (uint32_t b = 0; b < nb; b++) c += a.counters[b];
+        for (uint32_t b = 0; b <= nb; b++) v += a.counters[kCounterStride + b];
+        out[0] += c;
+        out[1] += v;
+    }
+}
+
+hipError_t launchMptFrame(const SceneDev& s, const MptArgs& a, const TraceScratch& ts, float4* out, bool count,
+                          unsigned long long* d_counts, hipStream_t st) {
+    const uint32_t npix = a.W * a.H;
+    const int T = 256;
+    hipLaunchKernelGGL(kZeroU32, dim3(1), dim3(2 * kCounterStride), 0, st, a.counters, 2u * kCounterStride);
+    hipLaunchKernelGGL(kPrimaryRays, dim3((npix + T - 1) / T), dim3(T), 0, st, s, a);
+    const uint32_t tb = std::min<uint32_t>(ts.gridBlocks, (npix + kBlock - 1) / kBlock);
+    unsigned long long* tstat = count ? d_counts + 2 : nullptr;
+    if (count)
+        hipLaunchKernelGGL((kTraceQueue<false, true>), dim3(tb), dim3(kBlock), 0, st, s, a.rayQ[0], nullptr, npix, a.pixHit, ts, tstat);
+    else
+        hipLaunchKernelGGL((kTraceQueue<false, false>), dim3(tb), dim3(kBlock), 0, st, s, a.rayQ[0], nullptr, npix, a.pixHit, ts, tstat);
+    hipLaunchKernelGGL(kShadePrimary, dim3((a.nPaths + T - 1) / T), dim3(T), 0, st, s, a);
+    const uint32_t pb = ts.gridBlocks;  // persistent grid for queue kernels
+    if (count) hipLaunchKernelGGL(kShadow<true>, dim3(pb), dim3(kBlock), 0, st, s, a, 0u, ts, tstat);
+    else hipLaunchKernelGGL(kShadow<false>, dim3(pb), dim3(kBlock), 0, st, s, a, 0u, ts, tstat);
+    const uint32_t shadeBlocks = std::min<uint32_t>((a.nPaths + T - 1) / T, uint32_t(persistentBlocks()) * 4);
+    for (uint32_t b = 0; b <= a.maxBounces; b++) {
+        const float4* rq = a.rayQ[b & 1];
+        if (count)
+            hipLaunchKernelGGL((kTraceQueue<false, true>), dim3(pb), dim3(kBlock), 0, st, s, rq, a.counters + b, 0u, a.hitQ, ts, tstat);
+        else
+            hipLaunchKernelGGL((kTraceQueue<false, false>), dim3(pb), dim3(kBlock), 0, st, s, rq, a.counters + b, 0u, a.hitQ, ts, tstat);
+        hipLaunchKernelGGL(kShade, dim3(shadeBlocks), dim3(T), 0, st, s, a, b);
+        if (a.nee && b < a.maxBounces) {
+            if (count) hipLaunchKernelGGL(kShadow<true>, dim3(pb), dim3(kBlock), 0, st, s, a, b + 1, ts, tstat);
+            else hipLaunchKernelGGL(kShadow<false>, dim3(pb), dim3(kBlock), 0, st, s, a, b + 1, ts, tstat);
+        }
+    }
+    hipLaunchKernelGGL(kResolve, dim3((npix + T - 1) / T), dim3(T), 0, st, a, out);
+    if (d_counts) hipLaunchKernelGGL(kRayCounts, dim3(1), dim3(64), 0, st, a, a.maxBounces + 1, d_counts);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ small utilities
+__global__ void kFill(float* p, uint64_t n, float v) {
+    uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    uint64_t G = uint64_t(gridDim.x) * blockDim.x;
+    for (; i < n; i += G) p[i] = v;
+}
+hipError_t launchFill(float* p, uint64_t n, float v, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    uint64_t blocks = std::min<uint64_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(kFill, dim3(uint32_t(blocks)), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}
+// Column sum over hash rows (AggregateGradients.cs.slang:50-58).
+__global__ void kAggregate(const float* tmp, uint32_t hashSize, uint32_t dim, float* out) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= dim) return;
+    float s = 0.f;
+    for (uint32_t h = 0; h < hashSize; h++) s += tmp[size_t(h) * dim + i];
+    out[i] = s;
+}
+hipError_t launchAggregate(const float* tmp, uint32_t hashSize, uint32_t dim, float* out, hipStream_t st) {
+    if (dim == 0) return hipSuccess;
+    hipLaunchKernelGGL(kAggregate, dim3((dim + 255) / 256), dim3(256), 0, st, tmp, hashSize, dim, out);
+    return hipGetLastError();
+}
+
+}  // namespace fhp

@@ -1,0 +1,499 @@
+// Device-resident scene layout and shading routines of the fhp core.
+//
+// HBM layout (all arrays 16-byte aligned, built once per scene by fhp_scene_create):
+//   bvh      BVH4 nodes, 128 B each = 8 x float4: lo.x[4] hi.x[4] lo.y[4] hi.y[4] lo.z[4] hi.z[4]
+//            child[4] (uint bits) and one spare float4. One node is one 128-byte cache line.
+//   triPos   48 B per triangle in BVH leaf order: (v0, origTri) (v1, 0) (v2, 0).
+//   idx      3 x u32 per triangle (original order); vtx 32 B per vertex: (p.xyz, n.x) (n.yz, uv).
+//   triMat   u32 material id per triangle; mats 64 B per material (16 floats, see MatField).
+//   texels   RGB fp32 texels of all textures; texDesc int4 (offset, width, height, 0).
+//   lights   80 B per emissive triangle; lbvh 64 B per light-BVH node.
+#pragma once
+#include "fhp_math.h"
+
+namespace fhp {
+
+// Material record (16 floats), identical to the scene description the Python layer builds.
+enum MatField { M_TYPE = 0, M_BASE = 1, M_ROUGH = 4, M_METAL = 5, M_IOR = 6, M_EMIT = 7, M_EMIT_FACTOR = 10,
+                M_DOUBLE_SIDED = 11, M_TEX = 12, M_STRIDE = 16 };
+enum MatType { MAT_STANDARD = 0, MAT_PBRT_DIFFUSE = 1 };
+
+// BVH4 child reference: bit 31 = leaf, bits 28..30 = count - 1, bits 0..27 = first triangle.
+constexpr uint32_t kLeafBit = 0x80000000u;
+FHP_HD bool childIsLeaf(uint32_t c) { return (c & kLeafBit) != 0u; }
+FHP_HD uint32_t leafFirst(uint32_t c) { return c & 0x0fffffffu; }
+FHP_HD uint32_t leafCount(uint32_t c) { return ((c >> 28) & 7u) + 1u; }
+FHP_HD uint32_t makeLeaf(uint32_t first, uint32_t count) { return kLeafBit | ((count - 1u) << 28) | first; }
+
+struct CameraDev {
+    f3 pos, U, V, W;
+    float nearZ, farZ;
+};
+
+struct SceneDev {
+    const float4* bvh;
+    const float4* triPos;
+    const uint32_t* idx;
+    const float4* vtx;
+    const uint32_t* triMat;
+    const float* mats;
+    const float* texels;
+    const int4* texDesc;
+    const float4* lights;        // 5 x float4 per light: (p0, area) (p1, flux) (p2, tri) (normal, 0) (Le, 0)
+    const uint32_t* activeLights;
+    const uint32_t* triToLight;
+    const float4* lbvh;          // 4 x float4 per node: (origin, flux) (extent, cosCone) (coneDir, 0) (info)
+    const uint32_t* lbvhTri;
+    const uint2* lbvhMask;
+    CameraDev cam;
+    uint32_t nTris, nLights, nActive, nLbvh;
+};
+
+// ------------------------------------------------------------------ shading data
+// Scene.slang:440-476 and MaterialFactory.slang:78-104 with world-space geometry.
+struct Shading {
+    f3 posW, V, faceN, N, T, B;  // (T, B, N) is the shading frame
+    float uvx, uvy;
+    uint32_t mat;
+    bool frontFacing;
+    FHP_HD f3 toLocal(f3 v) const { return mk3(dot(v, T), dot(v, B), dot(v, N)); }
+    FHP_HD f3 fromLocal(f3 v) const { return T * v.x + B * v.y + N * v.z; }
+    FHP_HD f3 orientedFaceN() const { return frontFacing ? faceN : -faceN; }
+};
+FHP_HD void setFrame(Shading& sd, f3 n) {
+    // ShadingFrame.slang:76-82/:127-156 without a valid tangent: T = perp_stark(N), B = N x T.
+    sd.N = normalize(n);
+    sd.T = perpStark(sd.N);
+    sd.B = cross(sd.N, sd.T);
+}
+FHP_HD void loadVertex(const SceneDev& s, uint32_t i, f3& p, f3& n, float& u, float& v) {
+    float4 a = s.vtx[2 * i], b = s.vtx[2 * i + 1];
+    p = mk3(a.x, a.y, a.z);
+    n = mk3(a.w, b.x, b.y);
+    u = b.z; v = b.w;
+}
+// Shading data from a triangle hit with DXR-style barycentrics (b1, b2).
+FHP_HD Shading shadingFromHit(const SceneDev& s, uint32_t tri, float b1, float b2, f3 rayDir) {
+    Shading sd;
+    uint32_t i0 = s.idx[3 * tri], i1 = s.idx[3 * tri + 1], i2 = s.idx[3 * tri + 2];
+    f3 p0, p1, p2, n0, n1, n2;
+    float u0, v0, u1, v1, u2, v2;
+    loadVertex(s, i0, p0, n0, u0, v0);
+    loadVertex(s, i1, p1, n1, u1, v1);
+    loadVertex(s, i2, p2, n2, u2, v2);
+    float b0 = 1.f - b1 - b2;
+    f3 p = p0 * b0; p = p + p1 * b1; p = p + p2 * b2;
+    f3 n = n0 * b0; n = n + n1 * b1; n = n + n2 * b2;
+    float ux = u0 * b0; ux = ux + u1 * b1; ux = ux + u2 * b2;
+    float uy = v0 * b0; uy = uy + v1 * b1; uy = uy + v2 * b2;
+    sd.faceN = normalize(cross(p1 - p0, p2 - p0));
+    sd.posW = p;
+    sd.uvx = ux; sd.uvy = uy;
+    sd.V = -rayDir;
+    setFrame(sd, normalize(n));
+    sd.frontFacing = dot(sd.V, sd.faceN) >= 0.f;
+    sd.mat = s.triMat[tri];
+    return sd;
+}
+FHP_HD f3 rayOriginFromShading(const Shading& sd, bool viewside = true) {
+    // ShadingData.slang:77-80
+    return rayOriginOffset(sd.posW, (sd.frontFacing == viewside) ? sd.faceN : -sd.faceN);
+}
+
+// Bilinear fetch at mip 0 with wrap addressing (MaterialSystem.slang:308-320, default sampler
+// MaterialSystem.cpp:74-77). Texel ids and weights are returned for the adjoint.
+struct TexTap {
+    uint32_t base;      // float offset of the texture
+    uint32_t t[4];      // texel indices (row-major)
+    float w[4];
+};
+FHP_HD float wrap01(float x) { return x - floorf(x); }
+FHP_HD int imod(int a, int m) { int r = a % m; return r < 0 ? r + m : r; }
+FHP_HD f3 texFetch(const SceneDev& s, int tex, float u, float v, TexTap* tap) {
+    int4 td = s.texDesc[tex];
+    int W = td.y, H = td.z;
+    float x = wrap01(u) * float(W) - 0.5f, y = wrap01(v) * float(H) - 0.5f;
+    float fx0 = floorf(x), fy0 = floorf(y);
+    float fx = x - fx0, fy = y - fy0;
+    int x0 = int(fx0), y0 = int(fy0);
+    int xa = imod(x0, W), xb = imod(x0 + 1, W), ya = imod(y0, H), yb = imod(y0 + 1, H);
+    const float* base = s.texels + td.x;
+    const float* q00 = base + 3 * (size_t(ya) * W + xa);
+    const float* q10 = base + 3 * (size_t(ya) * W + xb);
+    const float* q01 = base + 3 * (size_t(yb) * W + xa);
+    const float* q11 = base + 3 * (size_t(yb) * W + xb);
+    float wx0 = 1.f - fx, wy0 = 1.f - fy;
+    f3 top = mk3(q00[0], q00[1], q00[2]) * wx0 + mk3(q10[0], q10[1], q10[2]) * fx;
+    f3 bot = mk3(q01[0], q01[1], q01[2]) * wx0 + mk3(q11[0], q11[1], q11[2]) * fx;
+    if (tap) {
+        tap->base = uint32_t(td.x);
+        tap->t[0] = uint32_t(ya * W + xa); tap->t[1] = uint32_t(ya * W + xb);
+        tap->t[2] = uint32_t(yb * W + xa); tap->t[3] = uint32_t(yb * W + xb);
+        tap->w[0] = wx0 * wy0; tap->w[1] = fx * wy0; tap->w[2] = wx0 * fy; tap->w[3] = fx * fy;
+    }
+    return top * wy0 + bot * fy;
+}
+
+// ------------------------------------------------------------------ material instance
+struct Material {
+    int type;
+    f3 diffuse, specular, emission, base;
+    float roughness, metallic, F0;
+    bool doubleSided;
+    int tex;
+};
+// StandardMaterial.slang:50-126 / PBRTDiffuseMaterial.slang (reflection lobes only).
+FHP_HD Material loadMaterial(const SceneDev& s, const float* mats, const Shading& sd, Shading& frameOut, TexTap* tap) {
+    Material m;
+    const float* r = mats + size_t(sd.mat) * M_STRIDE;
+    m.type = int(r[M_TYPE]);
+    m.tex = int(r[M_TEX]);
+    f3 base = mk3(r[M_BASE], r[M_BASE + 1], r[M_BASE + 2]);
+    if (m.tex >= 0) base = texFetch(s, m.tex, sd.uvx, sd.uvy, tap);
+    m.base = base;
+    m.doubleSided = r[M_DOUBLE_SIDED] != 0.f;
+    // ShadingUtils.slang:107-117 (flipShadingNormal)
+    if (!sd.frontFacing && m.doubleSided) frameOut.N = -frameOut.N;
+    if (m.type == MAT_PBRT_DIFFUSE) {
+        m.diffuse = base; m.specular = mk3(0.f); m.roughness = 0.f; m.metallic = 0.f; m.F0 = 0.f;
+        m.emission = mk3(0.f);
+        return m;
+    }
+    float ior = r[M_IOR];
+    float f = (ior - 1.f) / (ior + 1.f);
+    m.F0 = f * f;
+    m.metallic = r[M_METAL];
+    m.diffuse = base + (mk3(0.f) - base) * m.metallic;                    // lerp(base, 0, metallic)
+    m.specular = mk3(m.F0) + (base - mk3(m.F0)) * m.metallic;             // lerp(F0, base, metallic)
+    m.roughness = r[M_ROUGH];
+    m.emission = mk3(0.f);
+    if (sd.frontFacing) m.emission = mk3(r[M_EMIT], r[M_EMIT + 1], r[M_EMIT + 2]) * r[M_EMIT_FACTOR];
+    return m;
+}
+
+// StandardBSDF (StandardBSDF.slang:125-199, reflection lobes: Frostbite diffuse + GGX).
+struct Bsdf {
+    f3 diff, spec;
+    float rough, alpha, pDiff, pSpec;
+    FHP_HD void init(f3 wi, const Material& m) {
+        diff = m.diffuse; spec = m.specular; rough = m.roughness;
+        alpha = rough * rough;
+        if (alpha < kMinGGXAlpha) alpha = 0.f;
+        float metallicBRDF = m.metallic * (1.f - 0.f);
+        float dielectricBSDF = (1.f - m.metallic) * (1.f - 0.f);
+        float dw = luminance(m.diffuse);
+        float sw = luminance(schlick3(m.specular, 1.f, wi.z));
+        pDiff = dw * dielectricBSDF * (1.f - 0.f);
+        pSpec = sw * (metallicBRDF + dielectricBSDF);
+        float norm = pDiff + 0.f + pSpec + 0.f;
+        if (norm > 0.f) {
+            norm = 1.f / norm;
+            pDiff *= norm;
+            pSpec *= norm;
+        }
+    }
+    // FrostbiteDiffuseBRDF.slang:84-96 (returns f * pi without the cosine)
+    FHP_HD f3 frostbite(f3 wi, f3 wo) const {
+        f3 h = normalize(wi + wo);
+        float woDotH = dot(wo, h);
+        float energyBias = 0.f + (0.5f - 0.f) * rough;
+        float energyFactor = 1.f + (1.f / 1.51f - 1.f) * rough;
+        float fd90 = energyBias + 2.f * woDotH * woDotH * rough;
+        float wiS = schlick1(1.f, fd90, wi.z);
+        float woS = schlick1(1.f, fd90, wo.z);
+        return diff * wiS * woS * energyFactor;
+    }
+    FHP_HD f3 evalDiffuse(f3 wi, f3 wo) const {
+        if (fminf_(wi.z, wo.z) < kMinCosTheta) return mk3(0.f);
+        return frostbite(wi, wo) * kInvPi * wo.z;
+    }
+    // SpecularMicrofacet.slang:58-83
+    FHP_HD f3 evalSpecular(f3 wi, f3 wo) const {
+        if (fminf_(wi.z, wo.z) < kMinCosTheta) return mk3(0.f);
+        if (alpha == 0.f) return mk3(0.f);
+        f3 h = normalize(wi + wo);
+        float wiDotH = dot(wi, h);
+        float D = ggxD(alpha, h.z);
+        float G = ggxG(alpha, wi.z, wo.z);
+        f3 F = schlick3(spec, 1.f, wiDotH);
+        return F * D * G * 0.25f / wi.z;
+    }
+    FHP_HD float pdfDiffuse(f3 wi, f3 wo) const {
+        if (fminf_(wi.z, wo.z) < kMinCosTheta) return 0.f;
+        return kInvPi * wo.z;
+    }
+    FHP_HD float pdfSpecular(f3 wi, f3 wo) const {
+        if (fminf_(wi.z, wo.z) < kMinCosTheta) return 0.f;
+        if (alpha == 0.f) return 0.f;
+        f3 h = normalize(wi + wo);
+        float wiDotH = dot(wi, h);
+        return ggxPdfVndf(alpha, wi, h) / (4.f * wiDotH);
+    }
+    FHP_HD f3 eval(f3 wi, f3 wo) const {
+        f3 r = mk3(0.f);
+        if (pDiff > 0.f) r = r + evalDiffuse(wi, wo) * ((1.f - 0.f) * (1.f - 0.f));
+        if (pSpec > 0.f) r = r + evalSpecular(wi, wo) * (1.f - 0.f);
+        return r;
+    }
+    FHP_HD float evalPdf(f3 wi, f3 wo) const {
+        float p = 0.f;
+        if (pDiff > 0.f) p += pDiff * pdfDiffuse(wi, wo);
+        if (pSpec > 0.f) p += pSpec * pdfSpecular(wi, wo);
+        return p;
+    }
+    // SpecularMicrofacet.slang:85-157
+    FHP_HD bool sampleSpec(f3 wi, f3& wo, float& pdf, f3& w, bool& delta, Rng& rng) const {
+        wo = mk3(0.f); w = mk3(0.f); pdf = 0.f;
+        if (wi.z < kMinCosTheta) return false;
+        if (alpha == 0.f) {
+            wo = mk3(-wi.x, -wi.y, wi.z);
+            pdf = 0.f;
+            w = schlick3(spec, 1.f, wi.z);
+            delta = true;
+            return true;
+        }
+        float ux = rng.u1(), uy = rng.u1();
+        f3 h = ggxSampleVndf(alpha, wi, ux, uy, pdf);
+        float wiDotH = dot(wi, h);
+        wo = 2.f * wiDotH * h - wi;
+        if (wo.z < kMinCosTheta) return false;
+        float G = ggxG(alpha, wi.z, wo.z);
+        float GOverG1wo = G * (1.f + ggxLambda(alpha * alpha, wi.z));
+        f3 F = schlick3(spec, 1.f, wiDotH);
+        pdf /= (4.f * wiDotH);
+        w = F * GOverG1wo;
+        return true;
+    }
+    // StandardBSDF.slang:242-313 (one-sample lobe selection)
+    FHP_HD bool sample(f3 wi, f3& wo, float& pdf, f3& w, bool& delta, Rng& rng) const {
+        wo = mk3(0.f); w = mk3(0.f); pdf = 0.f; delta = false;
+        bool valid = false;
+        float uSel = rng.u1();
+        if (uSel < pDiff) {
+            float ux = rng.u1(), uy = rng.u1();
+            wo = cosineHemisphere(ux, uy, pdf);
+            if (fminf_(wi.z, wo.z) < kMinCosTheta) { w = mk3(0.f); valid = false; }
+            else { w = frostbite(wi, wo); valid = true; }
+            w = w / pDiff;
+            w = w * ((1.f - 0.f) * (1.f - 0.f));
+            pdf *= pDiff;
+            if (pSpec > 0.f) pdf += pSpec * pdfSpecular(wi, wo);
+        } else if (uSel < pDiff + 0.f) {
+        } else if (uSel < pDiff + 0.f + pSpec) {
+            valid = sampleSpec(wi, wo, pdf, w, delta, rng);
+            w = w / pSpec;
+            w = w * (1.f - 0.f);
+            pdf *= pSpec;
+            if (pDiff > 0.f) pdf += pDiff * pdfDiffuse(wi, wo);
+        }
+        return valid;
+    }
+};
+
+// ShadingUtils.slang:214-240 (reflection or transmission, fullSphere = false) and :155-181.
+FHP_HD bool validRT(const Shading& sd, f3 wiL, f3 woL, f3 wo) {
+    bool isT = wiL.z * woL.z < 0.f;
+    if (fminf_(wiL.z, fabsf(woL.z)) < kMinCosTheta) return false;
+    bool wiTop = sd.frontFacing, woTop = dot(wo, sd.faceN) >= 0.f;
+    return isT ? (wiTop != woTop) : (wiTop == woTop);
+}
+FHP_HD bool validR(const Shading& sd, f3 wiL, f3 woL, f3 wo) {
+    if (fminf_(wiL.z, woL.z) < kMinCosTheta) return false;
+    return sd.frontFacing == (dot(wo, sd.faceN) >= 0.f);
+}
+// StandardMaterialInstance.slang:63-74 / PBRTDiffuseMaterialInstance.slang:54-64
+FHP_HD f3 matEval(const Shading& sd, const Material& m, f3 wo) {
+    f3 wiL = sd.toLocal(sd.V), woL = sd.toLocal(wo);
+    if (m.type == MAT_PBRT_DIFFUSE) {
+        if (!validR(sd, wiL, woL, wo)) return mk3(0.f);
+        return m.diffuse * fmaxf_(woL.z, 0.f) * kInvPi;
+    }
+    if (!validRT(sd, wiL, woL, wo)) return mk3(0.f);
+    Bsdf b;
+    b.init(wiL, m);
+    return b.eval(wiL, woL);
+}
+FHP_HD float matPdf(const Shading& sd, const Material& m, f3 wo) {
+    f3 wiL = sd.toLocal(sd.V), woL = sd.toLocal(wo);
+    if (m.type == MAT_PBRT_DIFFUSE) return fmaxf_(woL.z, 0.f) * kInvPi;
+    if (!validRT(sd, wiL, woL, wo)) return 0.f;
+    Bsdf b;
+    b.init(wiL, m);
+    return b.evalPdf(wiL, woL);
+}
+struct Scatter { f3 wo, weight; float pdf; bool delta; };
+// StandardMaterialInstance.slang:89-110 (+ sampleReference :202-231) / PBRTDiffuseMaterialInstance.slang:66-78
+FHP_HD bool matSample(const Shading& sd, const Material& m, Rng& rng, Scatter& s, bool importance) {
+    f3 wiL = sd.toLocal(sd.V), woL;
+    s.delta = false;
+    if (m.type == MAT_PBRT_DIFFUSE) {
+        float ux = rng.u1(), uy = rng.u1();
+        woL = cosineHemisphere(ux, uy, s.pdf);
+        s.weight = m.diffuse;
+        s.wo = sd.fromLocal(woL);
+        return validR(sd, wiL, woL, s.wo) && s.pdf != 0.f;
+    }
+    Bsdf b;
+    if (!importance) {
+        float ux = rng.u1(), uy = rng.u1();
+        woL = cosineHemisphere(ux, uy, s.pdf);
+        if (fminf_(wiL.z, woL.z) < kMinCosTheta || s.pdf == 0.f) return false;
+        b.init(wiL, m);
+        s.weight = b.eval(wiL, woL) / s.pdf;
+    } else {
+        b.init(wiL, m);
+        if (!b.sample(wiL, woL, s.pdf, s.weight, s.delta, rng)) return false;
+    }
+    s.wo = sd.fromLocal(woL);
+    return validRT(sd, wiL, woL, s.wo);
+}
+
+// ------------------------------------------------------------------ emissive lights
+struct LightSample { f3 posW, normalW, Le, dir; float pdf, distance; uint32_t light; };
+// EmissiveLightSamplerHelpers.slang:54-101
+FHP_HD bool sampleTri(const SceneDev& s, f3 posW, uint32_t li, float ux, float uy, LightSample& ls) {
+    const float4* L = s.lights + 5 * size_t(li);
+    float4 a = L[0], b = L[1], c = L[2], n = L[3], e = L[4];
+    ls.light = li;
+    f3 bc = triangleSample(ux, uy);
+    ls.posW = mk3(a.x, a.y, a.z) * bc.x + mk3(b.x, b.y, b.z) * bc.y + mk3(c.x, c.y, c.z) * bc.z;
+    f3 toL = ls.posW - posW;
+    const float d2 = fmaxf_(kFltMin, dot(toL, toL));
+    ls.distance = sqrtf(d2);
+    ls.dir = toL / ls.distance;
+    ls.normalW = mk3(n.x, n.y, n.z);
+    float cosT = dot(ls.normalW, -ls.dir);
+    if (cosT <= 0.f) return false;
+    ls.Le = mk3(e.x, e.y, e.z);
+    float denom = fmaxf_(kFltMin, cosT * a.w);
+    ls.pdf = d2 / denom;
+    return true;
+}
+// EmissiveLightSamplerHelpers.slang:106-124
+FHP_HD float triPdf(const SceneDev& s, f3 posW, uint32_t li, f3 hitPos, f3 hitN) {
+    f3 toL = hitPos - posW;
+    const float d2 = dot(toL, toL);
+    if (d2 <= kFltMin) return 0.f;
+    f3 L = toL / sqrtf(d2);
+    float cosT = dot(hitN, -L);
+    if (cosT <= 0.f) return 0.f;
+    float denom = fmaxf_(kFltMin, cosT * s.lights[5 * size_t(li)].w);
+    return d2 / denom;
+}
+
+// Light BVH node accessors: info = (right child | leaf bit, triOffset, triCount, 0).
+FHP_HD float cosSubClamped(float sA, float cA, float sB, float cB) { return cA > cB ? 1.f : cA * cB + sA * sB; }
+FHP_HD float sinSubClamped(float sA, float cA, float sB, float cB) { return cA > cB ? 0.f : sA * cB - cA * sB; }
+// LightBVHSampler.slang:219-273 (sphere solid-angle bound, bounding and lighting cones)
+FHP_HD float lbvhImportance(const SceneDev& s, f3 posW, f3 n, uint32_t node) {
+    const float4* N = s.lbvh + 4 * size_t(node);
+    float4 q0 = N[0], q1 = N[1], q2 = N[2];
+    f3 origin = mk3(q0.x, q0.y, q0.z), extent = mk3(q1.x, q1.y, q1.z), coneDir = mk3(q2.x, q2.y, q2.z);
+    float flux = q0.w, cosCone = q1.w;
+    float distance = length(origin - posW);
+    // boundCosineTerm (LightBVHSampler.slang:181-210) with GeometryHelpers.slang:222-240.
+    float sinBC = 0.f, cosBC;
+    float sqrR = dot(extent, extent);
+    f3 cv = origin - posW;
+    float cd2 = dot(cv, cv);
+    if (cd2 < sqrR) { sinBC = 0.f; cosBC = -1.f; }
+    else { float s2 = sqrR / cd2; cosBC = sqrtf(1.f - s2); sinBC = sqrtf(s2); }
+    f3 L = normalize(origin - posW);
+    float cosL = fminf_(fmaxf_(dot(n, L), -1.f), 1.f);
+    float sinL = sqrtf(1.f - cosL * cosL);
+    float NdotL = saturate(cosSubClamped(sinL, cosL, sinBC, cosBC));
+    float orient = 1.f;
+    f3 dirTo = (origin - posW) / distance;
+    if (cosCone != -1.f && cosCone > 0.f) {
+        float sinCone = sqrtf(fmaxf_(0.f, 1.f - cosCone * cosCone));
+        float cosT = dot(coneDir, -dirTo);
+        float sinT = sqrtf(fmaxf_(0.f, 1.f - cosT * cosT));
+        float sinBC2 = sqrtf(fmaxf_(0.f, 1 - cosBC * cosBC));
+        float c0 = cosSubClamped(sinT, cosT, sinCone, cosCone);
+        float s0 = sinSubClamped(sinT, cosT, sinCone, cosCone);
+        float cp = cosSubClamped(s0, c0, sinBC2, cosBC);
+        orient = fmaxf_(0.f, cp);
+    }
+    float halfR = fmaxf_(extent.x, fmaxf_(extent.y, extent.z));
+    distance = fmaxf_(halfR, distance);
+    return (flux * NdotL) * orient / (distance * distance);
+}
+FHP_HD uint4 lbvhInfo(const SceneDev& s, uint32_t node) {
+    float4 q = s.lbvh[4 * size_t(node) + 3];
+    return make_uint4(f2u(q.x), f2u(q.y), f2u(q.z), f2u(q.w));
+}
+// LightBVHSampler.slang:284-323, :375-419 (uniform pick in leaf), :430-446
+FHP_HD bool lbvhSample(const SceneDev& s, f3 posW, f3 n, float u, float& pdf, uint32_t& light) {
+    if (s.nLbvh == 0) return false;
+    pdf = 1.f;
+    uint32_t node = 0;
+    uint4 info = lbvhInfo(s, node);
+    while (!(info.x & kLeafBit)) {
+        uint32_t l = node + 1, r = info.x;
+        float li = lbvhImportance(s, posW, n, l), ri = lbvhImportance(s, posW, n, r);
+        float tot = li + ri;
+        if (tot == 0.f) return false;
+        float pL = li / tot, pR = 1.0f - pL;
+        if (u < pL) { u = u / pL; pdf *= pL; node = l; }
+        else { u = (u - pL) / pR; pdf *= pR; node = r; }
+        info = lbvhInfo(s, node);
+    }
+    uint32_t cnt = info.z;
+    uint32_t idx = min(uint32_t(u * float(cnt)), cnt - 1u);
+    light = s.lbvhTri[info.y + idx];
+    pdf *= 1.0f / float(cnt);
+    return true;
+}
+// LightBVHSampler.slang:108-124, :455-492, :502-533
+FHP_HD float lbvhSelPdf(const SceneDev& s, f3 posW, f3 n, uint32_t light) {
+    if (s.nLbvh == 0) return 0.f;
+    uint2 m = s.lbvhMask[light];
+    uint64_t mask = (uint64_t(m.y) << 32) | m.x;
+    float p = 1.f;
+    uint32_t node = 0;
+    uint4 info = lbvhInfo(s, node);
+    while (!(info.x & kLeafBit)) {
+        uint32_t l = node + 1, r = info.x;
+        float li = lbvhImportance(s, posW, n, l), ri = lbvhImportance(s, posW, n, r);
+        float tot = li + ri;
+        if (tot == 0.f) return 0.f;
+        float pL = li / tot, pR = 1.0f - pL;
+        if ((mask & 1ull) == 0) { p *= pL; node = l; } else { p *= pR; node = r; }
+        mask >>= 1;
+        info = lbvhInfo(s, node);
+    }
+    if (p == 0.f) return 0.f;
+    return p * (1.0f / float(info.z));
+}
+enum { LS_UNIFORM = 0, LS_LIGHTBVH = 1 };
+// EmissiveUniformSampler.slang:55-74 / LightBVHSampler.slang:75-94
+FHP_HD bool sampleLight(const SceneDev& s, int kind, f3 posW, f3 n, Rng& rng, LightSample& ls) {
+    if (s.nActive == 0) return false;
+    float uL = rng.u1();
+    uint32_t li;
+    float sel;
+    if (kind == LS_UNIFORM) {
+        uint32_t idx = min(uint32_t(uL * float(s.nActive)), s.nActive - 1u);
+        li = s.activeLights[idx];
+        sel = 1.f / float(s.nActive);
+    } else {
+        if (!lbvhSample(s, posW, n, uL, sel, li)) return false;
+    }
+    float ux = rng.u1(), uy = rng.u1();
+    if (!sampleTri(s, posW, li, ux, uy, ls)) return false;
+    ls.pdf *= sel;
+    return true;
+}
+FHP_HD float lightPdf(const SceneDev& s, int kind, f3 posW, f3 n, uint32_t li, f3 hitPos, f3 hitN) {
+    if (li == kInvalid) return 0.f;
+    float sel = kind == LS_UNIFORM ? (s.nActive ? 1.f / float(s.nActive) : 0.f) : lbvhSelPdf(s, posW, n, li);
+    return sel * triPdf(s, posW, li, hitPos, hitN);
+}
+// PTUtils.slang:87-93
+FHP_HD float misPower(float p0, float p1) {
+    float q0 = (1.f * p0) * (1.f * p0);
+    float q1 = (1.f * p1) * (1.f * p1);
+    return q0 / (q0 + q1);
+}
+
+}  // namespace fhp

@@ -1,0 +1,140 @@
+// BVH4 traversal for gfx950: closest-hit and any-hit queries.
+//
+// The stack lives in LDS, one 32-bit slot per entry laid out [entry][lane] so the 64 lanes of a
+// wave touch 64 consecutive dwords (no bank conflicts). Entries beyond the LDS depth spill to a
+// per-thread overflow array in HBM, so traversal is exact for any tree depth.
+//
+// Closest hit follows the reference's TraceRay contract (MinimalPathTracer.rt.slang:233-243,
+// RaytracingInline.slang:86-250) with a deterministic tie rule: among hits with tmin <= t <= tmax
+// the smallest t wins, equal t resolves to the lowest triangle id. Triangles use the watertight
+// test, so the answer is the same whatever the tree shape.
+#pragma once
+#include "fhp_scene.h"
+
+namespace fhp {
+
+struct HitRec {
+    uint32_t tri;
+    float t, b1, b2;
+};
+
+struct TraceStack {
+    uint32_t* lds;       // this lane's slot 0; entry i at lds[i * ldsStride]
+    uint32_t ldsStride;  // threads per block
+    uint32_t ldsDepth;
+    uint32_t* ovf;       // this thread's overflow slot 0; entry i at ovf[i * ovfStride]
+    uint32_t ovfStride;
+    uint32_t ovfDepth;
+    __device__ __forceinline__ void put(uint32_t i, uint32_t v) {
+        if (i < ldsDepth) lds[i * ldsStride] = v;
+        else ovf[(i - ldsDepth) * ovfStride] = v;
+    }
+    __device__ __forceinline__ uint32_t get(uint32_t i) const {
+        return i < ldsDepth ? lds[i * ldsStride] : ovf[(i - ldsDepth) * ovfStride];
+    }
+    __device__ __forceinline__ uint32_t capacity() const { return ldsDepth + ovfDepth; }
+};
+
+struct TraceCounters {
+    uint64_t nodes, tris;
+};
+
+// Slab test of the 4 children of one node. Returns a 4-bit hit mask and entry distances.
+// Conservative: the exit distance is scaled by (1 + 4 ulp) and boxes are padded at build time.
+__device__ __forceinline__ uint32_t testNode4(const float4* __restrict__ node, f3 o, f3 inv, float tmin, float tmax,
+                                              float tn[4], uint4& child) {
+    const float4 lx = node[0], hx = node[1], ly = node[2], hy = node[3], lz = node[4], hz = node[5];
+    const float4 cb = node[6];
+    child = make_uint4(f2u(cb.x), f2u(cb.y), f2u(cb.z), f2u(cb.w));
+    const float lox[4] = {lx.x, lx.y, lx.z, lx.w}, hix[4] = {hx.x, hx.y, hx.z, hx.w};
+    const float loy[4] = {ly.x, ly.y, ly.z, ly.w}, hiy[4] = {hy.x, hy.y, hy.z, hy.w};
+    const float loz[4] = {lz.x, lz.y, lz.z, lz.w}, hiz[4] = {hz.x, hz.y, hz.z, hz.w};
+    const uint32_t ch[4] = {child.x, child.y, child.z, child.w};
+    uint32_t mask = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        float ax = (lox[i] - o.x) * inv.x, bx = (hix[i] - o.x) * inv.x;
+        float ay = (loy[i] - o.y) * inv.y, by = (hiy[i] - o.y) * inv.y;
+        float az = (loz[i] - o.z) * inv.z, bz = (hiz[i] - o.z) * inv.z;
+        float n = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), tmin));
+        float f = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), tmax));
+        f = f * 1.00000048f;
+        tn[i] = n;
+        if (n <= f && ch[i] != kInvalid) mask |= 1u << i;
+    }
+    return mask;
+}
+
+FHP_HD f3 safeInv(f3 d) {
+    // Direction components below 1e-20 in magnitude are nudged so 1/d stays finite; boxes are
+    // padded, so the slab test stays conservative.
+    float x = fabsf(d.x) < 1e-20f ? copysignf(1e-20f, d.x) : d.x;
+    float y = fabsf(d.y) < 1e-20f ? copysignf(1e-20f, d.y) : d.y;
+    float z = fabsf(d.z) < 1e-20f ? copysignf(1e-20f, d.z) : d.z;
+    return mk3(1.f / x, 1.f / y, 1.f / z);
+}
+
+// Closest hit (ANY = false) or first hit (ANY = true) in [tmin, tmax]. Returns true on a hit.
+template <bool ANY, bool COUNT>
+__device__ bool traceBVH4(const SceneDev& s, f3 o, f3 d, float tmin, float tmax, TraceStack& st, HitRec& hit,
+                          TraceCounters* cnt = nullptr) {
+    hit.tri = kInvalid;
+    hit.t = tmax;
+    hit.b1 = hit.b2 = 0.f;
+    const f3 inv = safeInv(d);
+    uint32_t sp = 0;
+    uint32_t cur = 0;  // root node (always internal)
+    for (;;) {
+        if (!childIsLeaf(cur)) {
+            const float4* node = s.bvh + 8 * size_t(cur);
+            float tn[4];
+            uint4 child;
+            if (COUNT) cnt->nodes++;
+            uint32_t mask = testNode4(node, o, inv, tmin, hit.t, tn, child);
+            const uint32_t ch[4] = {child.x, child.y, child.z, child.w};
+            if (mask) {
+                // Order hit children by entry distance: nearest becomes current, the rest are pushed
+                // farthest first.
+                uint32_t ids[4];
+                float ts[4];
+                int k = 0;
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+                    if (mask & (1u << i)) { ids[k] = ch[i]; ts[k] = tn[i]; k++; }
+                for (int a = 1; a < k; a++) {
+                    uint32_t ia = ids[a];
+                    float ta = ts[a];
+                    int b = a - 1;
+                    while (b >= 0 && ts[b] < ta) { ts[b + 1] = ts[b]; ids[b + 1] = ids[b]; b--; }
+                    ts[b + 1] = ta; ids[b + 1] = ia;
+                }
+                // ids[] now sorted farthest first; the last one is the nearest.
+                // The scene builder sizes the stack to the tree's exact worst case
+                // (sum of (children - 1) along any root-to-leaf path), so sp stays in range.
+                for (int a = 0; a < k - 1; a++) st.put(sp++, ids[a]);
+                cur = ids[k - 1];
+                continue;
+            }
+        } else {
+            const uint32_t first = leafFirst(cur), count = leafCount(cur);
+            for (uint32_t j = 0; j < count; j++) {
+                const float4* tp = s.triPos + 3 * size_t(first + j);
+                const float4 a = tp[0], b = tp[1], c = tp[2];
+                const uint32_t id = f2u(a.w);
+                float t, b1, b2;
+                if (COUNT) cnt->tris++;
+                if (watertight(o, d, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), mk3(c.x, c.y, c.z), t, b1, b2)) {
+                    if (t >= tmin && t <= hit.t && (t < hit.t || hit.tri == kInvalid || id < hit.tri)) {
+                        hit.tri = id; hit.t = t; hit.b1 = b1; hit.b2 = b2;
+                        if (ANY) return true;
+                    }
+                }
+            }
+        }
+        if (sp == 0) break;
+        cur = st.get(--sp);
+    }
+    return hit.tri != kInvalid;
+}
+
+}  // namespace fhp

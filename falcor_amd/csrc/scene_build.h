@@ -1,0 +1,44 @@
+// Host-side scene build: BVH4 over triangles, light collection and light BVH.
+// These replace the driver-built BLAS/TLAS (Scene.cpp:3376, :3936), LightCollection::build
+// (LightCollection.cpp:220) and LightBVHBuilder::build (LightBVHBuilder.cpp:228).
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace fhp {
+
+struct HostScene {
+    // Input (world space).
+    uint32_t nV = 0, nT = 0, nM = 0, nTex = 0;
+    std::vector<float> pos, nrm, uv;  // 3, 3, 2 floats per vertex
+    std::vector<uint32_t> idx, triMat;
+    std::vector<float> mats;
+    std::vector<int32_t> texDesc;
+    std::vector<float> texels;
+    float cam[14] = {};  // pos, target, up, fovY, aspect, focalDistance, nearZ, farZ
+    // Camera basis (Camera.cpp:187-193).
+    float camU[3], camV[3], camW[3];
+
+    // BVH4 (8 float4 per node) and triangles in leaf order (12 floats per triangle).
+    std::vector<float> bvh;
+    std::vector<float> triPos;
+    uint32_t bvhNodes = 0, bvhDepth = 0, stackNeed = 0, bvhLeaves = 0;
+    double bvhSah = 0.0;
+
+    // Lights: 20 floats per light, see SceneDev::lights.
+    std::vector<float> lights;
+    std::vector<uint32_t> activeLights, triToLight;
+    // Light BVH: 16 floats per node, leaf triangle lists and per-light traversal bitmasks.
+    std::vector<float> lbvh;
+    std::vector<uint32_t> lbvhTri;
+    std::vector<uint32_t> lbvhMask;  // 2 words per light
+    uint32_t nLights = 0, nLbvh = 0;
+};
+
+void computeCamera(HostScene& s);
+void buildBVH4(HostScene& s, uint32_t maxLeaf = 4);
+void buildLights(HostScene& s);
+void buildLightBVH(HostScene& s);
+
+}  // namespace fhp

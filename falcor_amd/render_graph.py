@@ -1,0 +1,401 @@
+"""Falcor-style render-pass / render-graph surface over libfhp.
+
+Mirrors the parts of Falcor's Python API that MinimalPathTracer and WARDiffPathTracer users
+touch: Testbed (Source/Falcor/Core/Testbed.cpp:703-768), RenderGraph.create_pass / add_edge /
+mark_output / get_output (Source/Falcor/RenderGraph/RenderGraph.cpp:725-810), the pass property
+dictionaries (MinimalPathTracer.cpp:61-89, WARDiffPathTracer.cpp:53-144) and the WAR pass's
+scene_gradients / run_backward / dL_dI properties (WARDiffPathTracer.cpp:697-715).
+All images are torch tensors on the current HIP device; the work runs in libfhp's kernels.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import warnings
+
+import numpy as np
+
+from . import _lib
+from .scene import MAT_STRIDE, SceneData
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _device():
+    torch = _torch()
+    if not torch.cuda.is_available():
+        raise _lib.FhpError("no HIP device available: fhp passes run on the GPU only")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+# --------------------------------------------------------------------------- scene
+class Scene:
+    """Device-resident scene (BVH4, light collection, light BVH, materials, textures)."""
+
+    def __init__(self, data: SceneData):
+        L = _lib.load()
+        self.data = data
+        self._arrays = dict(
+            pos=np.ascontiguousarray(data.positions, np.float32), nrm=np.ascontiguousarray(data.normals, np.float32),
+            uv=np.ascontiguousarray(data.texcoords, np.float32), idx=np.ascontiguousarray(data.indices, np.uint32),
+            tm=np.ascontiguousarray(data.tri_material, np.uint32), mats=np.ascontiguousarray(data.materials, np.float32),
+            td=np.ascontiguousarray(data.tex_desc, np.int32), tx=np.ascontiguousarray(data.texels, np.float32),
+            cam=np.ascontiguousarray(data.camera_array(), np.float32))
+        a = self._arrays
+        d = _lib.SceneDesc()
+        d.num_vertices = a["pos"].shape[0]
+        d.h_positions, d.h_normals, d.h_texcoords = a["pos"].ctypes.data, a["nrm"].ctypes.data, a["uv"].ctypes.data
+        d.num_triangles = a["idx"].shape[0]
+        d.h_indices, d.h_tri_material = a["idx"].ctypes.data, a["tm"].ctypes.data
+        d.num_materials = a["mats"].shape[0]
+        d.h_materials = a["mats"].ctypes.data
+        d.num_textures = a["td"].shape[0]
+        d.h_tex_desc = a["td"].ctypes.data if a["td"].size else None
+        d.num_texels_floats = a["tx"].size
+        d.h_texels = a["tx"].ctypes.data if a["tx"].size else None
+        d.h_camera = a["cam"].ctypes.data
+        h = C.c_void_p()
+        _lib.check(L.fhp_scene_create(C.byref(d), C.byref(h)))
+        self.handle = h
+        self.materials = a["mats"].copy()
+
+    def close(self):
+        if getattr(self, "handle", None):
+            _lib.load().fhp_scene_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self) -> dict:
+        i = _lib.SceneInfo()
+        _lib.check(_lib.load().fhp_scene_info(self.handle, C.byref(i)))
+        out = {k: getattr(i, k) for k, _ in _lib.SceneInfo._fields_ if k != "camera_basis"}
+        out["camera_basis"] = np.array(i.camera_basis[:], np.float32)
+        return out
+
+    # Scene::get/set_material_params (Scene.cpp:4620-4643)
+    def get_material_params(self) -> np.ndarray:
+        return self.materials.copy()
+
+    def set_material_params(self, mats: np.ndarray):
+        m = np.ascontiguousarray(mats, np.float32)
+        if m.shape != self.materials.shape:
+            raise ValueError(f"expected materials of shape {self.materials.shape}")
+        _lib.check(_lib.load().fhp_scene_set_materials(self.handle, m.ctypes.data))
+        self.materials = m.copy()
+
+    def set_texels(self, texels):
+        """Copies a device tensor of texels (same layout as SceneData.texels) into the scene."""
+        torch = _torch()
+        t = texels.detach().to(device=_device(), dtype=torch.float32).contiguous()
+        _lib.check(_lib.load().fhp_scene_set_texels(self.handle, t.data_ptr(), t.numel(), _lib.stream_ptr()))
+
+    def light_bvh(self):
+        i = self.info()
+        nodes = np.zeros((i["light_bvh_nodes"], 16), np.float32)
+        tri = np.zeros(max(i["num_active_lights"], 1), np.uint32)
+        masks = np.zeros((max(i["num_lights"], 1), 2), np.uint32)
+        _lib.check(_lib.load().fhp_scene_light_bvh(self.handle, nodes.ctypes.data, tri.ctypes.data, masks.ctypes.data))
+        return nodes, tri[: i["num_active_lights"]], masks[: i["num_lights"]]
+
+    # Ray queries (TraceRay / RayQuery replacements).
+    def trace_closest(self, rays):
+        """rays: float32 tensor [N, 8] on the device -> int32-bit tensor [N, 4] (tri, t, b1, b2 bits)."""
+        torch = _torch()
+        r = rays.contiguous()
+        hits = torch.empty((r.shape[0], 4), dtype=torch.int32, device=r.device)
+        _lib.check(_lib.load().fhp_trace_closest(self.handle, r.data_ptr(), r.shape[0], hits.data_ptr(), _lib.stream_ptr()))
+        return hits
+
+    def trace_any(self, rays):
+        torch = _torch()
+        r = rays.contiguous()
+        occ = torch.empty((r.shape[0],), dtype=torch.int32, device=r.device)
+        _lib.check(_lib.load().fhp_trace_any(self.handle, r.data_ptr(), r.shape[0], occ.data_ptr(), _lib.stream_ptr()))
+        return occ
+
+
+# --------------------------------------------------------------------------- passes
+class RenderPass:
+    kind = ""
+    outputs: tuple = ()
+
+    def __init__(self, props: dict | None = None, name: str | None = None):
+        self.name = name or self.kind
+        self.properties = dict(props or {})
+        self.scene: Scene | None = None
+        self.frame_count = 0
+
+    def set_scene(self, scene: Scene | None):
+        self.scene = scene
+        self.frame_count = 0
+
+
+class _NativePass(RenderPass):
+    def __init__(self, props=None, name=None):
+        super().__init__(props, name)
+        keys, vals = [], []
+        for k, v in self.properties.items():
+            v = self._encode(k, v)
+            keys.append(k.encode())
+            vals.append(float(v))
+        karr = (C.c_char_p * max(len(keys), 1))(*keys)
+        varr = (C.c_double * max(len(vals), 1))(*vals)
+        h = C.c_void_p()
+        _lib.check(_lib.load().fhp_pass_create(self.kind.encode(), len(keys), karr, varr, C.byref(h)))
+        w = _lib.last_warning()
+        if w:
+            warnings.warn(w)
+        self.handle = h
+
+    @staticmethod
+    def _encode(key, value):
+        if key == "diffMode" and isinstance(value, str):
+            return {"Primal": 0, "BackwardDiff": 1, "ForwardDiffDebug": 2, "BackwardDiffDebug": 3}[value]
+        if key == "emissiveSampler" and isinstance(value, str):
+            return {"Uniform": 0, "LightBVH": 1}[value]
+        return float(value)
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            try:
+                _lib.load().fhp_pass_destroy(self.handle)
+            except Exception:
+                pass
+            self.handle = None
+
+    def set_property(self, key: str, value):
+        _lib.check(_lib.load().fhp_pass_set_property(self.handle, key.encode(), float(self._encode(key, value))))
+        w = _lib.last_warning()
+        if w:
+            warnings.warn(w)
+        self.properties[key] = value
+
+    def get_property(self, key: str) -> float:
+        v = C.c_double()
+        _lib.check(_lib.load().fhp_pass_get_property(self.handle, key.encode(), C.byref(v)))
+        return v.value
+
+    def render(self, width: int, height: int, spp: int = 1, frame: int | None = None, out=None, stats: bool = False):
+        """Renders one frame of `spp` samples per pixel into a [H, W, 4] float32 tensor."""
+        torch = _torch()
+        dev = _device()
+        if out is None:
+            out = torch.empty((height, width, 4), dtype=torch.float32, device=dev)
+        f = self.frame_count if frame is None else frame
+        st = np.zeros(4, np.uint64) if stats else None
+        _lib.check(_lib.load().fhp_pass_execute(self.handle, self.scene.handle if self.scene else None, width, height, spp, f,
+                                                out.data_ptr(), st.ctypes.data if stats else None, _lib.stream_ptr()))
+        if frame is None:
+            self.frame_count += 1
+        return (out, st) if stats else out
+
+    def execute(self, render_data: dict):
+        out = render_data["color"]
+        h, w = out.shape[0], out.shape[1]
+        self.render(w, h, 1, out=out)
+
+
+class MinimalPathTracer(_NativePass):
+    """MinimalPathTracer (MinimalPathTracer.cpp). Properties: maxBounces, computeDirect,
+    useImportanceSampling; extensions useEmissiveNEE, emissiveSampler ('Uniform'|'LightBVH')."""
+
+    kind = "MinimalPathTracer"
+    outputs = ("color",)
+
+
+class WARDiffPathTracer(_NativePass):
+    """WARDiffPathTracer (WARDiffPathTracer.cpp). Outputs color and dColor."""
+
+    kind = "WARDiffPathTracer"
+    outputs = ("color", "dColor")
+
+    def __init__(self, props=None, name=None):
+        super().__init__(props, name)
+        self.scene_gradients: SceneGradients | None = None
+        self.run_backward = 0
+        self.dL_dI = None
+        self.mesh_to_optimize = 0
+
+    def set_mesh_to_optimize(self, mesh_id: int):
+        self.mesh_to_optimize = int(mesh_id)
+
+    def backward(self, width: int, height: int, spp: int = 1, frame: int | None = None, stats: bool = False):
+        """Accumulates d(sum dL_dI . color)/d(params) into scene_gradients (raw sum over samples)."""
+        if self.scene_gradients is None or self.dL_dI is None:
+            raise _lib.FhpError("backward needs scene_gradients and dL_dI")
+        torch = _torch()
+        d = self.dL_dI.detach().to(device=_device(), dtype=torch.float32).contiguous()
+        if d.numel() != width * height * 3:
+            raise ValueError("dL_dI must hold width * height * 3 floats")
+        f = self.frame_count if frame is None else frame
+        st = np.zeros(4, np.uint64) if stats else None
+        _lib.check(_lib.load().fhp_pass_backward(self.handle, self.scene.handle, width, height, spp, f, d.data_ptr(),
+                                                 self.scene_gradients.handle, st.ctypes.data if stats else None,
+                                                 _lib.stream_ptr()))
+        if frame is None:
+            self.frame_count += 1
+        return st
+
+
+class AccumulatePass(RenderPass):
+    """Running average over frames, Single precision mode (Accumulate.cs.slang:76-114)."""
+
+    kind = "AccumulatePass"
+    outputs = ("output",)
+
+    def __init__(self, props=None, name=None):
+        super().__init__(props, name)
+        self.reset()
+
+    def reset(self):
+        self._sum = None
+        self._n = 0
+
+    def accumulate(self, cur):
+        if not self.properties.get("enabled", True):
+            return cur
+        self._sum = cur.clone() if self._sum is None else self._sum + cur
+        self._n += 1
+        return self._sum * (1.0 / self._n)
+
+
+class SceneGradients:
+    """SceneGradients (Source/Falcor/DiffRendering/SceneGradients.cpp:42-206) for material and
+    per-texel baseColor gradients."""
+
+    MATERIAL = "Material"
+    TEXTURE = "Texture"
+
+    def __init__(self, scene: Scene, hash_size: int = 256, texture_grads: bool = True):
+        h = C.c_void_p()
+        _lib.check(_lib.load().fhp_grads_create(scene.handle, hash_size, int(texture_grads), C.byref(h)))
+        self.handle = h
+        self.scene = scene
+        self.hash_size = hash_size
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            try:
+                _lib.load().fhp_grads_destroy(self.handle)
+            except Exception:
+                pass
+            self.handle = None
+
+    def clear_all_grads(self):
+        _lib.check(_lib.load().fhp_grads_clear(self.handle, _lib.stream_ptr()))
+
+    def aggregate_all_grads(self):
+        _lib.check(_lib.load().fhp_grads_aggregate(self.handle, _lib.stream_ptr()))
+
+    def get_grad_types(self):
+        return [self.MATERIAL, self.TEXTURE]
+
+    def get_grads_buffer(self, grad_type: str):
+        """Device tensor view of the aggregated gradients (copied to a new tensor)."""
+        torch = _torch()
+        dm, nm, dt, nt = C.c_void_p(), C.c_uint64(), C.c_void_p(), C.c_uint64()
+        _lib.check(_lib.load().fhp_grads_get(self.handle, C.byref(dm), C.byref(nm), C.byref(dt), C.byref(nt)))
+        n = nm.value if grad_type == self.MATERIAL else nt.value
+        out = torch.empty(n, dtype=torch.float32, device=_device())
+        if n:
+            which = 0 if grad_type == self.MATERIAL else 1
+            _lib.check(_lib.load().fhp_grads_copy(self.handle, which, out.data_ptr(), n, _lib.stream_ptr()))
+        return out
+
+
+# --------------------------------------------------------------------------- graph / testbed
+_PASS_TYPES = {"MinimalPathTracer": MinimalPathTracer, "WARDiffPathTracer": WARDiffPathTracer,
+               "AccumulatePass": AccumulatePass}
+
+
+class RenderGraph:
+    def __init__(self, name: str = "RenderGraph"):
+        self.name = name
+        self.passes: dict[str, RenderPass] = {}
+        self.edges: list[tuple[str, str]] = []
+        self.marked: list[str] = []
+        self._outputs: dict[str, object] = {}
+
+    def create_pass(self, name: str, pass_type: str, props: dict | None = None) -> RenderPass:
+        if pass_type not in _PASS_TYPES:
+            raise ValueError(f"unknown pass type '{pass_type}'")
+        p = _PASS_TYPES[pass_type](props, name)
+        self.passes[name] = p
+        return p
+
+    def add_edge(self, src: str, dst: str):
+        self.edges.append((src, dst))
+
+    def mark_output(self, name: str):
+        self.marked.append(name)
+
+    def get_output(self, name: str):
+        return self._outputs[name]
+
+    def _order(self):
+        deps = {n: set() for n in self.passes}
+        for s, d in self.edges:
+            deps[d.split(".")[0]].add(s.split(".")[0])
+        done, order = set(), []
+        while len(order) < len(self.passes):
+            ready = [n for n in self.passes if n not in done and deps[n] <= done]
+            if not ready:
+                raise RuntimeError("render graph has a cycle")
+            for n in ready:
+                done.add(n)
+                order.append(n)
+        return order
+
+    def execute(self, scene: Scene | None, width: int, height: int):
+        torch = _torch()
+        vals: dict[str, object] = {}
+        for n in self._order():
+            p = self.passes[n]
+            if isinstance(p, _NativePass):
+                if p.scene is not scene:
+                    p.set_scene(scene)
+                if isinstance(p, WARDiffPathTracer) and p.run_backward:
+                    p.backward(width, height, 1)
+                    vals[f"{n}.dColor"] = torch.zeros((height, width, 4), device=_device())
+                    vals[f"{n}.color"] = torch.zeros((height, width, 4), device=_device())
+                else:
+                    vals[f"{n}.color"] = p.render(width, height, 1)
+                    if isinstance(p, WARDiffPathTracer):
+                        vals[f"{n}.dColor"] = torch.zeros((height, width, 4), device=_device())
+            elif isinstance(p, AccumulatePass):
+                srcs = [s for s, d in self.edges if d == f"{n}.input"]
+                if srcs:
+                    vals[f"{n}.output"] = p.accumulate(vals[srcs[0]])
+        self._outputs = vals
+        return vals
+
+
+class Testbed:
+    """Headless frame driver (Testbed.cpp:72-112)."""
+
+    def __init__(self, width: int = 512, height: int = 512, create_window: bool = False, gpu: int = 0):
+        self.width, self.height = width, height
+        self.scene: Scene | None = None
+        self.render_graph: RenderGraph | None = None
+        self.frame_count = 0
+
+    def load_scene(self, data: SceneData) -> Scene:
+        self.scene = Scene(data)
+        return self.scene
+
+    def create_render_graph(self, name: str = "RenderGraph") -> RenderGraph:
+        return RenderGraph(name)
+
+    def frame(self):
+        if self.render_graph is None:
+            raise RuntimeError("no render graph")
+        self.render_graph.execute(self.scene, self.width, self.height)
+        self.frame_count += 1

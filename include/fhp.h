@@ -1,0 +1,151 @@
+/* fhp: MI355X-native path-tracing core behind Falcor's render-pass interface.
+ *
+ * C ABI of libfhp.so. Plain pointers and sizes only. "d_" pointers are HIP device pointers
+ * (for example torch-ROCm tensor data); "h_" pointers are host memory. Every call returns 0 on
+ * success or a negative FHP_E* code; fhp_last_error() gives the message of the last failure
+ * on the calling thread. Handles are not thread-safe; work is issued on the stream passed in
+ * (0 = the legacy default stream) and no call synchronises the host unless stated.
+ *
+ * Each entry point names the reference interface it replaces (paths relative to the Falcor
+ * source tree).
+ */
+#ifndef FHP_H
+#define FHP_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FHP_OK 0
+#define FHP_EINVAL -1   /* bad argument (null pointer, size mismatch, unknown pass kind) */
+#define FHP_EHIP -2     /* a HIP runtime call failed */
+#define FHP_ESTATE -3   /* handle in the wrong state (no scene, buffers missing) */
+#define FHP_EBUILD -4   /* scene build failed (empty scene, limits exceeded) */
+
+typedef struct FhpScene_* FhpScene;
+typedef struct FhpPass_* FhpPass;
+typedef struct FhpGrads_* FhpGrads;
+typedef void* FhpStream; /* hipStream_t */
+
+/* Library version (major * 10000 + minor * 100 + patch). */
+int fhp_get_version(void);
+/* Message of the last failed call on this thread ("" if none). */
+const char* fhp_last_error(void);
+/* Message of the last warning (for example an unknown property), "" if none. */
+const char* fhp_last_warning(void);
+
+/* ---------------------------------------------------------------- scene
+ * Scene description: world-space triangle geometry (instances flattened by the caller).
+ * Materials are 16 floats each: [type (0 StandardMaterial, 1 PBRTDiffuseMaterial),
+ * baseColor rgb, roughness, metallic, ior, emissive rgb, emissiveFactor, doubleSided,
+ * baseColor texture index (-1 none), 0, 0, 0]. Textures are RGB fp32, texDesc = 4 ints
+ * (float offset into texels, width, height, 0). Camera = 14 floats [position, target, up,
+ * fovY (radians), aspect, focalDistance, nearZ, farZ].
+ * Replaces Scene::create + SceneBuilder (Source/Falcor/Scene/Scene.cpp, SceneBuilder.cpp),
+ * the BLAS/TLAS builds (Scene.cpp:3376, :3936), LightCollection::build
+ * (Source/Falcor/Scene/Lights/LightCollection.cpp:220) and LightBVHBuilder::build
+ * (Source/Falcor/Rendering/Lights/LightBVHBuilder.cpp:228). */
+typedef struct FhpSceneDesc {
+    uint32_t num_vertices;
+    const float* h_positions; /* 3 * num_vertices */
+    const float* h_normals;   /* 3 * num_vertices */
+    const float* h_texcoords; /* 2 * num_vertices */
+    uint32_t num_triangles;
+    const uint32_t* h_indices;     /* 3 * num_triangles */
+    const uint32_t* h_tri_material; /* num_triangles */
+    uint32_t num_materials;
+    const float* h_materials; /* 16 * num_materials */
+    uint32_t num_textures;
+    const int32_t* h_tex_desc; /* 4 * num_textures */
+    uint64_t num_texels_floats;
+    const float* h_texels;
+    const float* h_camera; /* 14 floats */
+} FhpSceneDesc;
+
+typedef struct FhpSceneInfo {
+    uint32_t num_triangles, num_vertices, num_materials, num_textures;
+    uint32_t bvh_nodes, bvh_leaves, bvh_depth, stack_need;
+    uint32_t num_lights, num_active_lights, light_bvh_nodes;
+    double bvh_sah_cost;
+    uint64_t device_bytes;
+    float camera_basis[9]; /* U, V, W as computed by Camera.cpp:187-193 */
+} FhpSceneInfo;
+
+int fhp_scene_create(const FhpSceneDesc* desc, FhpScene* out);
+int fhp_scene_destroy(FhpScene scene);
+int fhp_scene_info(FhpScene scene, FhpSceneInfo* out);
+/* Replace all material records (host memory, 16 * num_materials floats).
+ * Replaces Scene::setMaterialParams (Source/Falcor/Scene/Scene.cpp:4620-4643). */
+int fhp_scene_set_materials(FhpScene scene, const float* h_materials);
+/* Copy texels from device memory into the scene (same layout as the description). */
+int fhp_scene_set_texels(FhpScene scene, const float* d_texels, uint64_t num_floats, FhpStream stream);
+/* Light BVH export (for checks): nodes 16 floats each, leaf light lists, per-light bitmasks
+ * (2 words each). Sizes from fhp_scene_info. */
+int fhp_scene_light_bvh(FhpScene scene, float* h_nodes, uint32_t* h_tri, uint32_t* h_masks);
+
+/* ---------------------------------------------------------------- ray queries
+ * Rays: 8 floats each (origin xyz, tmin, direction xyz, tmax). Hits: 4 words each
+ * (triangle id or 0xffffffff, t, b1, b2) with b1, b2 the barycentric weights of vertices
+ * 1 and 2 (HitInfo.slang:76-79). Closest hit replaces TraceRay with ray flags 0
+ * (Source/RenderPasses/MinimalPathTracer/MinimalPathTracer.rt.slang:233-243) and
+ * SceneRayQuery::traceRay (Source/Falcor/Scene/RaytracingInline.slang:86-250); any hit
+ * replaces the RAY_FLAG_ACCEPT_FIRST_HIT_AND_END_SEARCH shadow query (:205-226) and
+ * traceSceneVisibilityRay (RaytracingInline.slang:253-347): occluded[i] = 1 if any triangle
+ * is hit in [tmin, tmax]. */
+int fhp_trace_closest(FhpScene scene, const float* d_rays, uint32_t num_rays, uint32_t* d_hits, FhpStream stream);
+int fhp_trace_any(FhpScene scene, const float* d_rays, uint32_t num_rays, uint32_t* d_occluded, FhpStream stream);
+
+/* ---------------------------------------------------------------- render passes
+ * kind: "MinimalPathTracer" or "WARDiffPathTracer". Properties are (key, value) pairs with
+ * numeric values; enumerations are passed as their integer value. Unknown keys are ignored
+ * with a warning (fhp_last_warning), as the reference's parseProperties does.
+ * Replaces registerPlugin/RenderPass::create (MinimalPathTracer.cpp:32-35, :66-89;
+ * WARDiffPathTracer.cpp:31-35, :72-144). */
+int fhp_pass_create(const char* kind, int num_props, const char* const* keys, const double* values, FhpPass* out);
+int fhp_pass_destroy(FhpPass pass);
+int fhp_pass_set_property(FhpPass pass, const char* key, double value);
+int fhp_pass_get_property(FhpPass pass, const char* key, double* value);
+
+/* One frame of the pass into d_color (height * width * 4 floats, RGBA, row-major).
+ * spp samples per pixel are rendered in one launch sequence and averaged as
+ * AccumulatePass Single mode does (Accumulate.cs.slang:76-114); sample k of frame f uses
+ * sample number f * spp + k (MinimalPathTracer) or frame seed f * spp + k (WARDiffPathTracer).
+ * Replaces RenderPass::execute (MinimalPathTracer.cpp:110-199, WARDiffPathTracer.cpp:221-...).
+ * stats (optional, 4 words): closest-hit rays, visibility rays, BVH nodes visited, triangles
+ * tested (the last two only when the pass property "countTraversal" is 1). */
+int fhp_pass_execute(FhpPass pass, FhpScene scene, uint32_t width, uint32_t height, uint32_t spp, uint32_t frame,
+                     float* d_color, uint64_t* h_stats, FhpStream stream);
+
+/* ---------------------------------------------------------------- gradients
+ * Replaces SceneGradients (Source/Falcor/DiffRendering/SceneGradients.cpp:42-206).
+ * Material gradients: 20 floats per material (StandardMaterialParamLayout.slang:32-41),
+ * accumulated in hash_size replicas selected by jenkinsHash(pixelID) % hash_size
+ * (GradientIOWrapper.slang:39-42) and summed by fhp_grads_aggregate. Texture gradients
+ * (per-texel baseColor, an extension) share the texel layout of the scene. */
+int fhp_grads_create(FhpScene scene, uint32_t material_hash_size, int with_texture_grads, FhpGrads* out);
+int fhp_grads_destroy(FhpGrads grads);
+int fhp_grads_clear(FhpGrads grads, FhpStream stream);
+int fhp_grads_aggregate(FhpGrads grads, FhpStream stream);
+/* Device pointers and sizes (in floats) of the aggregated material gradients and of the
+ * texture gradients (NULL / 0 when disabled). */
+int fhp_grads_get(FhpGrads grads, float** d_material, uint64_t* material_floats, float** d_texture,
+                  uint64_t* texture_floats);
+/* Copies the aggregated material gradients (which = 0) or the texture gradients (which = 1)
+ * into d_dst (num_floats must equal the size reported by fhp_grads_get).
+ * Replaces SceneGradients::getGradsBuffer + Buffer::copy_to_torch (common.py:103-110). */
+int fhp_grads_copy(FhpGrads grads, int which, float* d_dst, uint64_t num_floats, FhpStream stream);
+
+/* Adjoint of WARDiffPathTracer's image: accumulates d(sum_pixels dLdI . I)/d(params) into
+ * grads by path replay (recomputes each path from its seed; no path storage). d_dLdI is
+ * height * width * 3 floats. Frame/sample numbering as in fhp_pass_execute.
+ * Replaces rayGen's bwd_diff(tracePaths) branch (WARDiffPathTracer.rt.slang:161-180) and
+ * the host loop of render_grad (scripts/inv-rendering/common.py:90-114). */
+int fhp_pass_backward(FhpPass pass, FhpScene scene, uint32_t width, uint32_t height, uint32_t spp, uint32_t frame,
+                      const float* d_dLdI, FhpGrads grads, uint64_t* h_stats, FhpStream stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FHP_H */

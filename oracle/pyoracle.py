@@ -1,0 +1,257 @@
+"""ctypes wrapper of the oracle library (oracle/_build/liboracle.so). Test infrastructure only."""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "_build" / "liboracle.so"
+_lib = None
+
+
+def build_oracle(force: bool = False) -> Path:
+    """Compiles the oracle with its Makefile (g++, -ffp-contract=off)."""
+    cmd = ["make", "-C", str(HERE)]
+    if force:
+        cmd.insert(1, "-B")
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"oracle build failed:\n{res.stdout}\n{res.stderr}")
+    return LIB
+
+
+def oracle_lib():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build_oracle()
+        _lib = C.CDLL(str(LIB))
+        L = _lib
+        vp, u32, u64, i32, f32 = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int, C.c_float
+        sig = {
+            "fo_splitmix64": (None, [u64, i32, vp]),
+            "fo_xoshiro128ss": (None, [vp, i32, vp]),
+            "fo_sample_gen": (None, [u32, u32, u32, i32, vp]),
+            "fo_interleave": (None, [i32, vp, vp, vp]),
+            "fo_jenkins": (None, [i32, vp, vp]),
+            "fo_offset_ray": (None, [i32, vp, vp, vp]),
+            "fo_intersect": (None, [i32, vp, vp, vp]),
+            "fo_intersect_f64": (None, [i32, vp, vp, vp]),
+            "fo_sample_cosine": (None, [i32, vp, vp]),
+            "fo_sample_vmf": (None, [i32, vp, f32, vp]),
+            "fo_bsdf_eval": (None, [i32, vp, vp, vp, vp]),
+            "fo_bsdf_sample": (None, [i32, vp, vp, vp, vp]),
+            "fo_scene_create": (vp, [u32, vp, vp, vp, u32, vp, vp, u32, vp, u32, vp, u64, vp, vp]),
+            "fo_scene_destroy": (None, [vp]),
+            "fo_scene_set_materials": (None, [vp, vp]),
+            "fo_scene_set_texels": (None, [vp, vp]),
+            "fo_scene_camera": (None, [vp, vp]),
+            "fo_scene_set_sampling_params": (None, [vp, vp, vp]),
+            "fo_scene_light_count": (u32, [vp]),
+            "fo_scene_active_light_count": (u32, [vp]),
+            "fo_scene_lights": (None, [vp, vp]),
+            "fo_lbvh_node_count": (u32, [vp]),
+            "fo_lbvh_nodes": (None, [vp, vp, vp, vp]),
+            "fo_light_sample": (None, [vp, i32, i32, vp, vp, vp, vp]),
+            "fo_light_selection_pdfs": (None, [vp, i32, vp, vp, vp]),
+            "fo_trace": (None, [vp, i32, vp, vp, i32, i32, i32]),
+            "fo_render_mpt": (None, [vp, u32, u32, u32, u32, u32, u32, i32, i32, vp, vp]),
+            "fo_record_mpt": (u64, [vp, u32, u32, u32, u32, u32, u32, u32, i32, u64, vp]),
+            "fo_render_war": (None, [vp, u32, u32, u32, u32, u32, u32, i32, vp, vp]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+    return _lib
+
+
+def _p(a: np.ndarray) -> int:
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data
+
+
+class OracleScene:
+    """Oracle copy of a falcor_amd.scene.SceneData."""
+
+    def __init__(self, data):
+        L = oracle_lib()
+        self._keep = [np.ascontiguousarray(data.positions, np.float32), np.ascontiguousarray(data.normals, np.float32),
+                      np.ascontiguousarray(data.texcoords, np.float32), np.ascontiguousarray(data.indices, np.uint32),
+                      np.ascontiguousarray(data.tri_material, np.uint32), np.ascontiguousarray(data.materials, np.float32),
+                      np.ascontiguousarray(data.tex_desc, np.int32), np.ascontiguousarray(data.texels, np.float32),
+                      np.ascontiguousarray(data.camera_array(), np.float32)]
+        p, n, uv, idx, tm, mats, td, tx, cam = self._keep
+        self.num_triangles = idx.shape[0]
+        self.num_materials = mats.shape[0]
+        self.h = L.fo_scene_create(p.shape[0], _p(p), _p(n), _p(uv), idx.shape[0], _p(idx), _p(tm), mats.shape[0],
+                                   _p(mats), td.shape[0], _p(td) if td.size else None, tx.size, _p(tx) if tx.size else None,
+                                   _p(cam))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            oracle_lib().fo_scene_destroy(self.h)
+            self.h = None
+
+    def set_materials(self, mats: np.ndarray):
+        m = np.ascontiguousarray(mats, np.float32)
+        assert m.shape == (self.num_materials, 16)
+        self._mats = m
+        oracle_lib().fo_scene_set_materials(self.h, _p(m))
+
+    def set_texels(self, texels: np.ndarray):
+        t = np.ascontiguousarray(texels, np.float32)
+        self._tex = t
+        oracle_lib().fo_scene_set_texels(self.h, _p(t))
+
+    def freeze_sampling(self, mats=None, texels=None):
+        """Sampling and pdfs use (mats, texels); evaluation uses the current parameters. None disables."""
+        if mats is None:
+            oracle_lib().fo_scene_set_sampling_params(self.h, None, None)
+            return
+        self._smats = np.ascontiguousarray(mats, np.float32)
+        self._stex = None if texels is None else np.ascontiguousarray(texels, np.float32)
+        oracle_lib().fo_scene_set_sampling_params(self.h, _p(self._smats), None if self._stex is None else _p(self._stex))
+
+    def camera_basis(self) -> np.ndarray:
+        out = np.zeros(9, np.float32)
+        oracle_lib().fo_scene_camera(self.h, _p(out))
+        return out
+
+    def lights(self) -> np.ndarray:
+        n = oracle_lib().fo_scene_light_count(self.h)
+        out = np.zeros((n, 18), np.float32)
+        if n:
+            oracle_lib().fo_scene_lights(self.h, _p(out))
+        return out
+
+    def light_bvh(self):
+        L = oracle_lib()
+        nn = L.fo_lbvh_node_count(self.h)
+        nl = L.fo_scene_light_count(self.h)
+        nodes = np.zeros((nn, 15), np.float32)
+        tri = np.zeros(max(nl, 1), np.uint32)
+        masks = np.zeros(max(nl, 1), np.uint64)
+        L.fo_lbvh_nodes(self.h, _p(nodes), _p(tri), _p(masks))
+        return nodes, tri[: L.fo_scene_active_light_count(self.h)], masks[:nl]
+
+    def trace(self, rays: np.ndarray, any_hit=False, brute=False, threads=0) -> np.ndarray:
+        """rays float32 [N, 8] (o, tmin, d, tmax) -> uint32 [N, 4] (tri, t bits, b1 bits, b2 bits)."""
+        r = np.ascontiguousarray(rays, np.float32)
+        out = np.zeros((r.shape[0], 4), np.uint32)
+        oracle_lib().fo_trace(self.h, r.shape[0], _p(r), _p(out), int(brute), int(any_hit), threads)
+        return out
+
+    def render_mpt(self, W, H, spp=1, frame=0, max_bounces=3, compute_direct=True, importance=True, nee=False,
+                   light_sampler=1, brute=False, threads=0):
+        out = np.zeros((H, W, 3), np.float32)
+        stats = np.zeros(4, np.uint64)
+        flags = int(compute_direct) | (int(importance) << 1) | (int(nee) << 2) | (int(brute) << 3)
+        oracle_lib().fo_render_mpt(self.h, W, H, spp, frame, max_bounces, flags, light_sampler, threads, _p(out), _p(stats))
+        return out, stats
+
+    def record_mpt(self, W, H, p0, p1, sample_number=0, max_bounces=3, compute_direct=True, importance=True, nee=False,
+                   light_sampler=1, brute=False, cap=1 << 22):
+        rec = np.zeros((cap, 13), np.float32)
+        flags = int(compute_direct) | (int(importance) << 1) | (int(nee) << 2) | (int(brute) << 3)
+        n = oracle_lib().fo_record_mpt(self.h, W, H, p0, p1, sample_number, max_bounces, flags, light_sampler, cap, _p(rec))
+        return rec[: min(n, cap)]
+
+    def render_war(self, W, H, spp=1, frame=0, max_bounces=0, bsdf_sampling=True, nee=True, mis=True, antithetic=True,
+                   brute=False, threads=0):
+        out = np.zeros((H, W, 3), np.float32)
+        stats = np.zeros(4, np.uint64)
+        flags = int(bsdf_sampling) | (int(nee) << 1) | (int(mis) << 2) | (int(antithetic) << 3) | (int(brute) << 4)
+        oracle_lib().fo_render_war(self.h, W, H, spp, frame, max_bounces, flags, threads, _p(out), _p(stats))
+        return out, stats
+
+
+class Oracle:
+    """Stateless oracle functions (PRNG, hashes, geometry helpers, BSDFs)."""
+
+    @staticmethod
+    def splitmix64(state: int, n: int) -> np.ndarray:
+        out = np.zeros(n, np.uint64)
+        oracle_lib().fo_splitmix64(state, n, _p(out))
+        return out
+
+    @staticmethod
+    def xoshiro128ss(seed, n: int) -> np.ndarray:
+        s = np.ascontiguousarray(seed, np.uint32)
+        out = np.zeros(n, np.uint32)
+        oracle_lib().fo_xoshiro128ss(_p(s), n, _p(out))
+        return out
+
+    @staticmethod
+    def sample_gen(px, py, sample_number, n) -> np.ndarray:
+        out = np.zeros(n, np.float32)
+        oracle_lib().fo_sample_gen(px, py, sample_number, n, _p(out))
+        return out
+
+    @staticmethod
+    def interleave(x, y) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.uint32)
+        y = np.ascontiguousarray(y, np.uint32)
+        out = np.zeros(x.size, np.uint32)
+        oracle_lib().fo_interleave(x.size, _p(x), _p(y), _p(out))
+        return out
+
+    @staticmethod
+    def jenkins(x) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.uint32)
+        out = np.zeros(x.size, np.uint32)
+        oracle_lib().fo_jenkins(x.size, _p(x), _p(out))
+        return out
+
+    @staticmethod
+    def offset_ray(pos, nrm) -> np.ndarray:
+        p = np.ascontiguousarray(pos, np.float32)
+        n = np.ascontiguousarray(nrm, np.float32)
+        out = np.zeros_like(p)
+        oracle_lib().fo_offset_ray(p.shape[0], _p(p), _p(n), _p(out))
+        return out
+
+    @staticmethod
+    def intersect(rays, tris, f64=False) -> np.ndarray:
+        dt = np.float64 if f64 else np.float32
+        r = np.ascontiguousarray(rays, dt)
+        t = np.ascontiguousarray(tris, dt)
+        out = np.zeros((r.shape[0], 4), dt)
+        fn = oracle_lib().fo_intersect_f64 if f64 else oracle_lib().fo_intersect
+        fn(r.shape[0], _p(r), _p(t), _p(out))
+        return out
+
+    @staticmethod
+    def sample_cosine(u) -> np.ndarray:
+        u = np.ascontiguousarray(u, np.float32)
+        out = np.zeros((u.shape[0], 4), np.float32)
+        oracle_lib().fo_sample_cosine(u.shape[0], _p(u), _p(out))
+        return out
+
+    @staticmethod
+    def sample_vmf(u, kappa) -> np.ndarray:
+        u = np.ascontiguousarray(u, np.float32)
+        out = np.zeros((u.shape[0], 3), np.float32)
+        oracle_lib().fo_sample_vmf(u.shape[0], _p(u), kappa, _p(out))
+        return out
+
+    @staticmethod
+    def bsdf_eval(mat, wi, wo) -> np.ndarray:
+        m = np.ascontiguousarray(mat, np.float32)
+        wi = np.ascontiguousarray(wi, np.float32)
+        wo = np.ascontiguousarray(wo, np.float32)
+        out = np.zeros((wi.shape[0], 4), np.float32)
+        oracle_lib().fo_bsdf_eval(wi.shape[0], _p(m), _p(wi), _p(wo), _p(out))
+        return out
+
+    @staticmethod
+    def bsdf_sample(mat, wi, seeds) -> np.ndarray:
+        m = np.ascontiguousarray(mat, np.float32)
+        wi = np.ascontiguousarray(wi, np.float32)
+        s = np.ascontiguousarray(seeds, np.uint32)
+        out = np.zeros((wi.shape[0], 8), np.float32)
+        oracle_lib().fo_bsdf_sample(wi.shape[0], _p(m), _p(wi), _p(s), _p(out))
+        return out

@@ -1,0 +1,68 @@
+"""C-ABI checks that need no GPU: the library loads, exports every entry point declared in
+include/fhp.h, and reports argument errors with codes and messages (no device calls)."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def _declared():
+    text = (ROOT / "include" / "fhp.h").read_text()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(fhp_\w+)\(", text, re.M)))
+
+
+def test_header_declares_expected_entry_points():
+    names = _declared()
+    for n in ("fhp_scene_create", "fhp_trace_closest", "fhp_trace_any", "fhp_pass_execute", "fhp_pass_backward",
+              "fhp_grads_create", "fhp_grads_aggregate"):
+        assert n in names
+
+
+def test_library_exports_every_declared_symbol():
+    from falcor_amd import _lib
+    lib = _lib.load()
+    for n in _declared():
+        assert hasattr(lib, n), n
+    assert set(_declared()) == set(_lib.EXPORTED)
+
+
+def test_argument_errors_without_device():
+    from falcor_amd import _lib
+    lib = _lib.load()
+    h = C.c_void_p()
+    assert lib.fhp_scene_create(None, C.byref(h)) == -1
+    assert "null" in lib.fhp_last_error().decode()
+    keys = (C.c_char_p * 1)(b"maxBounces")
+    vals = (C.c_double * 1)(3.0)
+    assert lib.fhp_pass_create(b"NoSuchPass", 1, keys, vals, C.byref(h)) == -1
+    assert "unknown pass kind" in lib.fhp_last_error().decode()
+
+
+def test_pass_properties_defaults_and_unknown_keys():
+    """Unknown properties warn (MinimalPathTracer.cpp:86); defaults follow MinimalPathTracer.h:79-83."""
+    from falcor_amd import _lib
+    lib = _lib.load()
+    keys = (C.c_char_p * 2)(b"maxBounces", b"bogus")
+    vals = (C.c_double * 2)(5.0, 1.0)
+    h = C.c_void_p()
+    assert lib.fhp_pass_create(b"MinimalPathTracer", 2, keys, vals, C.byref(h)) == 0
+    assert "bogus" in lib.fhp_last_warning().decode()
+    v = C.c_double()
+    assert lib.fhp_pass_get_property(h, b"maxBounces", C.byref(v)) == 0 and v.value == 5.0
+    assert lib.fhp_pass_get_property(h, b"computeDirect", C.byref(v)) == 0 and v.value == 1.0
+    assert lib.fhp_pass_set_property(h, b"maxBounces", 99.0) == -1
+    lib.fhp_pass_destroy(h)
+    keys = (C.c_char_p * 1)(b"maxBounces")
+    vals = (C.c_double * 1)(3.0)
+    assert lib.fhp_pass_create(b"WARDiffPathTracer", 1, keys, vals, C.byref(h)) == 0
+    assert lib.fhp_pass_get_property(h, b"Log10vMFConcentration", C.byref(v)) == 0 and v.value == 5.0
+    lib.fhp_pass_destroy(h)
+
+
+def test_product_never_imports_the_oracle():
+    for p in (ROOT / "falcor_amd").rglob("*.py"):
+        src = p.read_text()
+        assert not re.search(r"^\s*(import|from)\s+oracle\b", src, re.M), p
