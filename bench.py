@@ -1,0 +1,257 @@
+#!/usr/bin/env python
+"""Throughput benchmark of the fhp path-tracing core on one MI355X.
+
+A step is one forward frame of the sphere-field workload: 1024 icospheres (1,310,720 triangles),
+4096 emissive triangles sampled through the light BVH, 1920x1080, 4 spp, max 6 bounces,
+MinimalPathTracer with emissive next-event estimation (FP32). `value` is closest-hit plus
+visibility rays per second over the timed steps (Mrays/s), with all inputs resident in HBM.
+
+Also reported on the same JSON line:
+  * roofline of the dominant kernel (closest-hit BVH4 traversal): algorithmic bytes per ray
+    (32 B ray + 16 B hit + 128 B per visited node + 48 B per tested triangle, node/triangle
+    counts measured on the same frame) over the kernel time from HIP events on its stream;
+  * diff: ms per forward+adjoint iteration of the differentiable-materials workload (64
+    textured spheres, 512x512 RGB fp32 albedo textures, 512x512 image, 16 spp forward +
+    16 spp adjoint), batched vs the plain per-frame loop, and the gradient relative error
+    against central finite differences (eps 1e-3) on 256 texels;
+  * cpu_baseline: the oracle (C++ restatement, all host cores) on the same forward workload.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--skip-cpu] [--skip-diff]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "Mrays/s fwd; ms per fwd+adjoint iter at 512² 16spp; grad rel-err vs FD"
+HBM_PEAK_GBS = 8000.0
+NODE_BYTES, TRI_BYTES, RAY_BYTES, HIT_BYTES, OCC_BYTES = 128, 48, 32, 16, 4
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def forward_bench(steps: int, warmup: int):
+    import torch
+    from falcor_amd import scene as S
+    from falcor_amd.render_graph import MinimalPathTracer, Scene
+
+    W, H, SPP, B = 1920, 1080, 4, 6
+    t0 = time.time()
+    data = S.sphere_field(1)
+    t1 = time.time()
+    sc = Scene(data)
+    torch.cuda.synchronize()
+    t2 = time.time()
+    info = sc.info()
+    log(f"scene: {info['num_triangles']} tris, {info['num_active_lights']} lights, bvh {info['bvh_nodes']} nodes "
+        f"depth {info['bvh_depth']} stack {info['stack_need']}; build {t2 - t1:.2f}s (geometry {t1 - t0:.2f}s)")
+    props = {"maxBounces": B, "useEmissiveNEE": True, "emissiveSampler": "LightBVH"}
+    p = MinimalPathTracer(dict(props, timeKernels=1))
+    p.set_scene(sc)
+    out = torch.empty((H, W, 4), device="cuda")
+    for _ in range(warmup):
+        p.render(W, H, SPP, out=out)
+    torch.cuda.synchronize()
+    p.kernel_times(reset=True)
+    rays_c = rays_v = 0
+    torch.cuda.synchronize()
+    ts = time.perf_counter()
+    for _ in range(steps):
+        _, st = p.render(W, H, SPP, out=out, stats=True)
+        rays_c += int(st[0])
+        rays_v += int(st[1])
+    torch.cuda.synchronize()
+    te = time.perf_counter()
+    kt = p.kernel_times(reset=True)
+    sec = te - ts
+    mrays = (rays_c + rays_v) / sec / 1e6
+    # Traversal counts of one frame (separate, untimed run).
+    pc = MinimalPathTracer(dict(props, countTraversal=1))
+    pc.set_scene(sc)
+    _, cs = pc.render(W, H, SPP, frame=0, out=out, stats=True)
+    n_node = cs[2] / max(cs[0], 1)
+    n_tri = cs[3] / max(cs[0], 1)
+    bytes_per_ray = RAY_BYTES + HIT_BYTES + NODE_BYTES * n_node + TRI_BYTES * n_tri
+    trace_ms, trace_launches = kt["trace"]
+    trace_bytes = bytes_per_ray * rays_c
+    achieved = trace_bytes / (trace_ms * 1e-3) / 1e9 if trace_ms > 0 else 0.0
+    img = out[..., :3].float()
+    res = dict(
+        mrays=mrays, ms_per_step=sec / steps * 1e3, rays_closest=rays_c / steps, rays_visibility=rays_v / steps,
+        kernel_ms=kt, nodes_per_ray=float(n_node), tris_per_ray=float(n_tri), bytes_per_ray=float(bytes_per_ray),
+        trace_launches_per_step=trace_launches / steps, trace_avg_launch_ms=trace_ms / max(trace_launches, 1),
+        achieved_gbs=achieved, image_mean=float(img.mean()), build_s=t2 - t1, info=info, W=W, H=H, spp=SPP, B=B,
+        data=data)
+    log(f"forward: {mrays:.1f} Mrays/s, {res['ms_per_step']:.2f} ms/frame, {rays_c / steps / 1e6:.2f}M closest + "
+        f"{rays_v / steps / 1e6:.2f}M visibility rays/frame; nodes/ray {n_node:.1f} tris/ray {n_tri:.1f}; kernels {kt}")
+    return res
+
+
+def cpu_baseline(data, W, H, spp, B):
+    import oracle
+    osc = oracle.OracleScene(data)
+    threads = os.cpu_count() or 1
+    t = time.perf_counter()
+    _, st = osc.render_mpt(W, H, spp=spp, frame=0, max_bounces=B, nee=True, light_sampler=1, threads=threads)
+    dt = time.perf_counter() - t
+    return dict(value=(int(st[0]) + int(st[1])) / dt / 1e6, unit="Mrays/s", cores=threads, kind="port",
+                sample=f"one {W}x{H} frame of the same workload ({spp} spp, {B} bounces, LightBVH NEE), "
+                       f"oracle C++ restatement on a std::thread pool, {dt:.1f} s")
+
+
+def diff_bench(iters: int = 3, fd_texels: int = 256):
+    import torch
+    from falcor_amd import scene as S
+    from falcor_amd.render_graph import Scene, SceneGradients, WARDiffPathTracer
+
+    W = H = 512
+    SPP, B = 16, 1
+    tgt = S.textured_spheres(3, target=True)
+    init = S.textured_spheres(3, target=False)
+    sc_t = Scene(tgt)
+    sc = Scene(init)
+    props = {"maxBounces": B, "useWAR": False}
+    pt = WARDiffPathTracer(props)
+    pt.set_scene(sc_t)
+    target = pt.render(W, H, SPP, frame=1000)[..., :3].clone()
+    p = WARDiffPathTracer(dict(props, timeKernels=1))
+    p.set_scene(sc)
+    g = SceneGradients(sc, hash_size=256, texture_grads=True)
+    p.scene_gradients = g
+    n_pix = W * H * 3
+
+    def batched(frame):
+        img = p.render(W, H, SPP, frame=2 * frame)[..., :3]
+        p.dL_dI = (2.0 / n_pix) * (img - target)
+        g.clear_all_grads()
+        p.backward(W, H, SPP, frame=2 * frame + 1)
+        g.aggregate_all_grads()
+        return img
+
+    def plain(frame):
+        # The reference's per-frame loop (scripts/inv-rendering/common.py:80-114): one launch per
+        # sample, host synchronisation after each frame, forward then adjoint.
+        acc = None
+        for k in range(SPP):
+            img = p.render(W, H, 1, frame=frame * 2 * SPP + k)[..., :3]
+            torch.cuda.synchronize()
+            acc = img.clone() if acc is None else acc + img
+        img = acc * (1.0 / SPP)
+        p.dL_dI = (2.0 / n_pix) * (img - target)
+        g.clear_all_grads()
+        for k in range(SPP):
+            p.backward(W, H, 1, frame=frame * 2 * SPP + SPP + k)
+            torch.cuda.synchronize()
+        g.aggregate_all_grads()
+        return img
+
+    for fn in (batched, plain):
+        fn(0)
+    torch.cuda.synchronize()
+    res = {}
+    for name, fn in (("batched", batched), ("plain", plain)):
+        t = time.perf_counter()
+        for i in range(iters):
+            fn(i + 1)
+        torch.cuda.synchronize()
+        res[name] = (time.perf_counter() - t) / iters * 1e3
+    kt = p.kernel_times(reset=True)
+    log(f"diff: batched {res['batched']:.1f} ms/iter, plain {res['plain']:.1f} ms/iter; kernels {kt}")
+
+    # Gradient check: central differences of L = sum(dLdI * I) on random texels with nonzero
+    # gradient, sampling frozen at the current parameters (the function the adjoint differentiates).
+    img = p.render(W, H, SPP, frame=7)[..., :3]
+    dLdI = ((2.0 / n_pix) * (img - target)).contiguous()
+    p.dL_dI = dLdI
+    g.clear_all_grads()
+    p.backward(W, H, SPP, frame=7)
+    g.aggregate_all_grads()
+    gtex = g.get_grads_buffer(g.TEXTURE) / SPP
+    cand = torch.nonzero(gtex.abs() > 1e-3 * gtex.abs().max()).flatten().cpu().numpy()
+    rng = np.random.default_rng(0)
+    pick = rng.choice(cand, size=min(fd_texels, cand.size), replace=False)
+    texels = torch.from_numpy(init.texels).cuda()
+    sc.freeze_sampling(True)
+    eps = 1e-3
+    fd = []
+    t = time.perf_counter()
+    d64 = dLdI.double()
+    for ti in pick:
+        vals = []
+        for s in (eps, -eps):
+            tx = texels.clone()
+            tx[int(ti)] += s
+            sc.set_texels(tx)
+            im = p.render(W, H, SPP, frame=7)[..., :3]
+            vals.append(float((im.double() * d64).sum()))
+        fd.append((vals[0] - vals[1]) / (2 * eps))
+    sc.set_texels(texels)
+    sc.freeze_sampling(False)
+    torch.cuda.synchronize()
+    ad = gtex[torch.from_numpy(pick).cuda()].double().cpu().numpy()
+    fd = np.array(fd)
+    rel = float(np.linalg.norm(ad - fd) / max(np.linalg.norm(fd), 1e-30))
+    log(f"diff: grad rel-err vs FD on {pick.size} texels = {rel:.2e} ({time.perf_counter() - t:.1f}s)")
+    return dict(ms_per_iter_batched=res["batched"], ms_per_iter_plain=res["plain"],
+                speedup_batched_vs_plain=res["plain"] / res["batched"], grad_rel_err_fd=rel, fd_texels=int(pick.size),
+                kernel_ms=kt)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-diff", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build measures one GPU only (the path shards by views/tiles with no exchange "
+              "except image gather and one gradient all-reduce; see DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    if not torch.cuda.is_available():
+        print("bench.py needs a HIP device", file=sys.stderr)
+        sys.exit(1)
+    from falcor_amd import _lib
+    _lib.load()
+
+    fwd = forward_bench(args.steps, args.warmup)
+    line = {
+        "metric": METRIC, "value": round(fwd["mrays"], 3), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(fwd["ms_per_step"], 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded sphere field)",
+        "config": {"workload": "forward path tracing, 1024 icospheres (1,310,720 tris), 4096 emissive tris via "
+                               "LightBVH NEE, 1920x1080, 4 spp, max 6 bounces",
+                   "rays_closest_per_step": int(fwd["rays_closest"]),
+                   "rays_visibility_per_step": int(fwd["rays_visibility"]),
+                   "bvh_build_s": round(fwd["build_s"], 2)},
+        "roofline": {"bound": "hbm", "achieved": round(fwd["achieved_gbs"], 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(fwd["achieved_gbs"] / HBM_PEAK_GBS, 5), "traffic": None,
+                     "kernel": "kTraceQueue (closest-hit BVH4 traversal)",
+                     "bytes_per_ray": round(fwd["bytes_per_ray"], 1), "nodes_per_ray": round(fwd["nodes_per_ray"], 2),
+                     "tris_per_ray": round(fwd["tris_per_ray"], 2),
+                     "avg_launch_ms": round(fwd["trace_avg_launch_ms"], 4),
+                     "launches_per_step": fwd["trace_launches_per_step"]},
+    }
+    if not args.skip_diff:
+        line["diff"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in diff_bench().items()}
+    if not args.skip_cpu:
+        line["cpu_baseline"] = cpu_baseline(fwd["data"], fwd["W"], fwd["H"], fwd["spp"], fwd["B"])
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

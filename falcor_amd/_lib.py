@@ -60,6 +60,7 @@ _SIGS = {
     "fhp_scene_info": (C.c_int, [C.c_void_p, C.POINTER(SceneInfo)]),
     "fhp_scene_set_materials": (C.c_int, [C.c_void_p, C.c_void_p]),
     "fhp_scene_set_texels": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
+    "fhp_scene_freeze_sampling": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     "fhp_scene_light_bvh": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "fhp_trace_closest": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
     "fhp_trace_any": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
@@ -70,6 +71,7 @@ _SIGS = {
     "fhp_pass_get_property": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_double)]),
     "fhp_pass_execute": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p,
                                    C.c_void_p, C.c_void_p]),
+    "fhp_pass_kernel_times": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
     "fhp_grads_create": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int, C.POINTER(C.c_void_p)]),
     "fhp_grads_destroy": (C.c_int, [C.c_void_p]),
     "fhp_grads_clear": (C.c_int, [C.c_void_p, C.c_void_p]),

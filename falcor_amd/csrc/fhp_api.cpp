@@ -60,6 +60,7 @@ struct FhpScene_ {
     HostScene h;
     DevBuf bvh, triPos, idx, vtx, triMat, mats, texels, texDesc, lights, active, triToLight, lbvh, lbvhTri, lbvhMask;
     DevBuf ovf;  // traversal overflow stack
+    DevBuf matsS, texelsS;  // frozen sampling parameters (finite-difference mode)
     TraceScratch ts{};
     SceneDev dev{};
     uint64_t deviceBytes = 0;
@@ -71,6 +72,11 @@ struct FhpPass_ {
     // Wavefront buffers (forward pass).
     DevBuf thp, prevN, rng, Lp, Ls, rayQ0, rayQ1, pathQ0, pathQ1, hitQ, shQ, pixHit, counters, stats;
     uint32_t cap = 0, capPix = 0;
+    KernelTimer timer;
+    ~FhpPass_() {
+        if (timer.created)
+            for (auto& e : timer.ev) (void)hipEventDestroy(e);
+    }
     double get(const char* k) const {
         auto it = props.find(k);
         return it == props.end() ? 0.0 : it->second;
@@ -89,7 +95,8 @@ namespace {
 const std::map<std::string, double>& mptDefaults() {
     // MinimalPathTracer.h:79-83; useEmissiveNEE/emissiveSampler/countTraversal are extensions.
     static const std::map<std::string, double> d = {{"maxBounces", 3}, {"computeDirect", 1}, {"useImportanceSampling", 1},
-                                                   {"useEmissiveNEE", 0}, {"emissiveSampler", 1}, {"countTraversal", 0}};
+                                                   {"useEmissiveNEE", 0}, {"emissiveSampler", 1}, {"countTraversal", 0},
+                                                   {"timeKernels", 0}};
     return d;
 }
 const std::map<std::string, double>& warDefaults() {
@@ -99,7 +106,7 @@ const std::map<std::string, double>& warDefaults() {
         {"fixedSeed", 0},       {"useFixedSeed", 0},      {"useBSDFSampling", 1},   {"useNEE", 1},
         {"useMIS", 1},          {"useWAR", 0},            {"auxSampleCount", 16},   {"Log10vMFConcentration", 5},
         {"Log10vMFConcentrationScreen", 5},               {"boundaryTermBeta", 0.01}, {"useAntitheticSampling", 1},
-        {"countTraversal", 0}};
+        {"countTraversal", 0},  {"timeKernels", 0}};
     return d;
 }
 
@@ -127,9 +134,41 @@ int ensurePassBuffers(FhpPass p, uint32_t W, uint32_t H, uint32_t spp) {
 }
 }  // namespace
 
+int prepareTimer(FhpPass p) {
+    p->timer.on = p->get("timeKernels") != 0;
+    if (p->timer.on && !p->timer.created) {
+        for (auto& e : p->timer.ev) HIP_OR_FAIL(hipEventCreate(&e));
+        p->timer.created = true;
+    }
+    p->timer.n = 0;
+    return FHP_OK;
+}
+int collectTimer(FhpPass p, hipStream_t st) {
+    if (!p->timer.on || p->timer.n == 0) return FHP_OK;
+    HIP_OR_FAIL(hipStreamSynchronize(st));
+    for (int i = 0; i < p->timer.n; i++) {
+        float ms = 0.f;
+        HIP_OR_FAIL(hipEventElapsedTime(&ms, p->timer.ev[2 * i], p->timer.ev[2 * i + 1]));
+        p->timer.ms[p->timer.kind[i]] += ms;
+        p->timer.launches[p->timer.kind[i]]++;
+    }
+    p->timer.n = 0;
+    return FHP_OK;
+}
+
 extern "C" {
 
 int fhp_get_version(void) { return 100; }
+
+int fhp_pass_kernel_times(FhpPass p, double* ms, uint64_t* launches, int reset) {
+    if (!p) return fail(FHP_EINVAL, "null argument");
+    for (int i = 0; i < KK_COUNT; i++) {
+        if (ms) ms[i] = p->timer.ms[i];
+        if (launches) launches[i] = p->timer.launches[i];
+        if (reset) { p->timer.ms[i] = 0.0; p->timer.launches[i] = 0; }
+    }
+    return FHP_OK;
+}
 const char* fhp_last_error(void) { return g_error.c_str(); }
 const char* fhp_last_warning(void) { return g_warning.c_str(); }
 
@@ -212,6 +251,8 @@ int fhp_scene_create(const FhpSceneDesc* d, FhpScene* out) {
     dv.triMat = s->triMat.as<uint32_t>();
     dv.mats = s->mats.as<float>();
     dv.texels = s->texels.as<float>();
+    dv.matsS = dv.mats;
+    dv.texelsS = dv.texels;
     dv.texDesc = s->texDesc.as<int4>();
     dv.lights = s->lights.as<float4>();
     dv.activeLights = s->active.as<uint32_t>();
@@ -270,6 +311,23 @@ int fhp_scene_set_texels(FhpScene s, const float* d_texels, uint64_t n, FhpStrea
     if (!s || !d_texels) return fail(FHP_EINVAL, "null argument");
     if (n != s->h.texels.size()) return fail(FHP_EINVAL, "texel count mismatch");
     HIP_OR_FAIL(hipMemcpyAsync(s->texels.p, d_texels, n * 4, hipMemcpyDeviceToDevice, hipStream_t(stream)));
+    return FHP_OK;
+}
+
+int fhp_scene_freeze_sampling(FhpScene s, int enable, FhpStream stream) {
+    if (!s) return fail(FHP_EINVAL, "null argument");
+    hipStream_t st = hipStream_t(stream);
+    if (!enable) {
+        s->dev.matsS = s->dev.mats;
+        s->dev.texelsS = s->dev.texels;
+        return FHP_OK;
+    }
+    HIP_OR_FAIL(s->matsS.alloc(s->mats.bytes));
+    HIP_OR_FAIL(hipMemcpyAsync(s->matsS.p, s->mats.p, s->mats.bytes, hipMemcpyDeviceToDevice, st));
+    HIP_OR_FAIL(s->texelsS.alloc(s->texels.bytes));
+    HIP_OR_FAIL(hipMemcpyAsync(s->texelsS.p, s->texels.p, s->texels.bytes, hipMemcpyDeviceToDevice, st));
+    s->dev.matsS = s->matsS.as<float>();
+    s->dev.texelsS = s->texelsS.as<float>();
     return FHP_OK;
 }
 
@@ -360,7 +418,7 @@ int fhp_pass_execute(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t spp
     if (rc) return rc;
     const bool count = p->get("countTraversal") != 0;
     unsigned long long* dstats = p->stats.as<unsigned long long>();
-    if (h_stats) HIP_OR_FAIL(launchFill(reinterpret_cast<float*>(dstats), 8, 0.f, st));
+    if (h_stats) HIP_OR_FAIL(launchFill(reinterpret_cast<float*>(dstats), 12, 0.f, st));
     if (p->kind == "MinimalPathTracer") {
         MptArgs a{};
         a.W = W; a.H = H; a.spp = spp; a.frame = frame;
@@ -377,7 +435,10 @@ int fhp_pass_execute(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t spp
         a.pathQ[0] = p->pathQ0.as<uint32_t>(); a.pathQ[1] = p->pathQ1.as<uint32_t>();
         a.hitQ = p->hitQ.as<uint4>(); a.shQ = p->shQ.as<float4>(); a.pixHit = p->pixHit.as<uint4>();
         a.counters = p->counters.as<uint32_t>();
-        HIP_OR_FAIL(launchMptFrame(s->dev, a, s->ts, reinterpret_cast<float4*>(d_color), count, h_stats ? dstats : nullptr, st));
+        if ((rc = prepareTimer(p))) return rc;
+        HIP_OR_FAIL(launchMptFrame(s->dev, a, s->ts, reinterpret_cast<float4*>(d_color), count, h_stats ? dstats : nullptr, st,
+                                   &p->timer));
+        if ((rc = collectTimer(p, st))) return rc;
     } else {
         WarArgs a{};
         a.W = W; a.H = H; a.spp = spp;
@@ -390,13 +451,15 @@ int fhp_pass_execute(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t spp
         a.out = reinterpret_cast<float4*>(d_color);
         a.tmp = p->Lp.as<float4>();
         a.rays = h_stats ? dstats : nullptr;
-        HIP_OR_FAIL(launchWarPrimal(s->dev, a, s->ts, st));
+        if ((rc = prepareTimer(p))) return rc;
+        HIP_OR_FAIL(launchWarPrimal(s->dev, a, s->ts, st, &p->timer));
+        if ((rc = collectTimer(p, st))) return rc;
     }
     if (h_stats) {
-        unsigned long long hs[4];
-        HIP_OR_FAIL(hipMemcpyAsync(hs, dstats, 32, hipMemcpyDeviceToHost, st));
+        unsigned long long hs[6];
+        HIP_OR_FAIL(hipMemcpyAsync(hs, dstats, 48, hipMemcpyDeviceToHost, st));
         HIP_OR_FAIL(hipStreamSynchronize(st));
-        for (int i = 0; i < 4; i++) h_stats[i] = hs[i];
+        for (int i = 0; i < 6; i++) h_stats[i] = hs[i];
     }
     return FHP_OK;
 }
@@ -463,7 +526,7 @@ int fhp_pass_backward(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t sp
     int rc = ensurePassBuffers(p, 1, 1, 1);
     if (rc) return rc;
     unsigned long long* dstats = p->stats.as<unsigned long long>();
-    if (h_stats) HIP_OR_FAIL(launchFill(reinterpret_cast<float*>(dstats), 8, 0.f, st));
+    if (h_stats) HIP_OR_FAIL(launchFill(reinterpret_cast<float*>(dstats), 12, 0.f, st));
     WarArgs a{};
     a.W = W; a.H = H; a.spp = spp;
     a.frame = p->get("useFixedSeed") != 0 ? uint32_t(p->get("fixedSeed")) : frame * spp;
@@ -478,12 +541,14 @@ int fhp_pass_backward(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t sp
     a.matGradDim = g->dim;
     a.gradTex = g->withTex ? g->tex.as<float>() : nullptr;
     a.rays = h_stats ? dstats : nullptr;
-    HIP_OR_FAIL(launchWarAdjoint(s->dev, a, s->ts, st));
+    if ((rc = prepareTimer(p))) return rc;
+    HIP_OR_FAIL(launchWarAdjoint(s->dev, a, s->ts, st, &p->timer));
+    if ((rc = collectTimer(p, st))) return rc;
     if (h_stats) {
-        unsigned long long hs[4];
-        HIP_OR_FAIL(hipMemcpyAsync(hs, dstats, 32, hipMemcpyDeviceToHost, st));
+        unsigned long long hs[6];
+        HIP_OR_FAIL(hipMemcpyAsync(hs, dstats, 48, hipMemcpyDeviceToHost, st));
         HIP_OR_FAIL(hipStreamSynchronize(st));
-        for (int i = 0; i < 4; i++) h_stats[i] = hs[i];
+        for (int i = 0; i < 6; i++) h_stats[i] = hs[i];
     }
     return FHP_OK;
 }

@@ -38,11 +38,34 @@ struct KernelStats {
     unsigned long long* d;  // [nodes, tris] when counting is enabled
 };
 
+// Optional per-launch timing with HIP events on the launch stream, grouped by kernel kind.
+enum KernelKind { KK_TRACE = 0, KK_SHADOW = 1, KK_SHADE = 2, KK_WAR_PRIMAL = 3, KK_WAR_ADJOINT = 4, KK_OTHER = 5,
+                  KK_COUNT = 6 };
+struct KernelTimer {
+    static constexpr int kMax = 96;
+    hipEvent_t ev[2 * kMax];
+    int kind[kMax];
+    int n = 0;
+    bool on = false, created = false;
+    double ms[KK_COUNT] = {};
+    unsigned long long launches[KK_COUNT] = {};
+    void begin(int k, hipStream_t st) {
+        if (!on || n >= kMax) return;
+        (void)hipEventRecord(ev[2 * n], st);
+        kind[n] = k;
+    }
+    void end(hipStream_t st) {
+        if (!on || n >= kMax) return;
+        (void)hipEventRecord(ev[2 * n + 1], st);
+        n++;
+    }
+};
+
 // Launchers (fhp_kernels.hip). All are asynchronous on `stream`.
 hipError_t launchTraceRays(const SceneDev& s, const float4* rays, uint32_t n, uint4* hits, uint32_t* occluded, bool any,
                            const TraceScratch& ts, hipStream_t stream);
 hipError_t launchMptFrame(const SceneDev& s, const MptArgs& a, const TraceScratch& ts, float4* out, bool count,
-                          unsigned long long* d_counts, hipStream_t stream);
+                          unsigned long long* d_counts, hipStream_t stream, KernelTimer* timer);
 
 // WARDiffPathTracer primal and adjoint (megakernel, one path per thread).
 struct WarArgs {
@@ -56,8 +79,10 @@ struct WarArgs {
     float* gradTex;         // per-texel baseColor gradients (texel layout), or null
     unsigned long long* rays;  // [closest, visibility] counters, or null
 };
-hipError_t launchWarPrimal(const SceneDev& s, const WarArgs& a, const TraceScratch& ts, hipStream_t stream);
-hipError_t launchWarAdjoint(const SceneDev& s, const WarArgs& a, const TraceScratch& ts, hipStream_t stream);
+hipError_t launchWarPrimal(const SceneDev& s, const WarArgs& a, const TraceScratch& ts, hipStream_t stream,
+                           KernelTimer* timer);
+hipError_t launchWarAdjoint(const SceneDev& s, const WarArgs& a, const TraceScratch& ts, hipStream_t stream,
+                            KernelTimer* timer);
 hipError_t launchAggregate(const float* tmp, uint32_t hashSize, uint32_t dim, float* out, hipStream_t stream);
 hipError_t launchFill(float* p, uint64_t n, float v, hipStream_t stream);
 

@@ -52,7 +52,7 @@ __global__ __launch_bounds__(kBlock) void kTraceRays(SceneDev s, const float4* _
         HitRec h;
         bool hit = traceBVH4<ANY, false>(s, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), a.w, b.w, st, h);
         if (ANY) occl[i] = hit ? 1u : 0u;
-        else hits[i] = make_uint4(h.tri, f2u(h.t), f2u(h.b1), f2u(h.b2));
+        else hits[i] = hit ? make_uint4(h.tri, f2u(h.t), f2u(h.b1), f2u(h.b2)) : make_uint4(kInvalid, 0u, 0u, 0u);
     }
 }
 
@@ -106,8 +106,8 @@ __global__ __launch_bounds__(kBlock) void kTraceQueue(SceneDev s, const float4* 
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += G) {
         const float4 ra = rays[2 * i], rb = rays[2 * i + 1];
         HitRec h;
-        traceBVH4<ANY, COUNT>(s, mk3(ra.x, ra.y, ra.z), mk3(rb.x, rb.y, rb.z), ra.w, rb.w, st, h, &cnt);
-        hits[i] = make_uint4(h.tri, f2u(h.t), f2u(h.b1), f2u(h.b2));
+        const bool hit = traceBVH4<ANY, COUNT>(s, mk3(ra.x, ra.y, ra.z), mk3(rb.x, rb.y, rb.z), ra.w, rb.w, st, h, &cnt);
+        hits[i] = hit ? make_uint4(h.tri, f2u(h.t), f2u(h.b1), f2u(h.b2)) : make_uint4(kInvalid, 0u, 0u, 0u);
     }
     if (COUNT) {
         atomicAdd(&stats[0], (unsigned long long)cnt.nodes);
@@ -165,7 +165,7 @@ __global__ void kShadePrimary(SceneDev s, MptArgs a) {
         float tmin, tmax;
         primaryRay(s.cam, px, py, a.W, a.H, o, d, tmin, tmax);
         Shading sd = shadingFromHit(s, h.x, u2f(h.z), u2f(h.w), d);
-        Material m = loadMaterial(s, s.mats, sd, sd, nullptr);
+        Material m = loadMaterial(s, s.mats, s.texels, sd, sd, nullptr);
         Rng rng = rngSeed(px, py, a.frame * a.spp + k);
         f3 org = rayOriginFromShading(sd);
         f3 Lp = mk3(0.f);
@@ -222,7 +222,7 @@ __global__ void kShade(SceneDev s, MptArgs a, uint32_t depth) {
                 uint4 r4 = a.rng[pathId];
                 Rng rng{r4.x, r4.y, r4.z, r4.w};
                 Shading sd = shadingFromHit(s, h.x, u2f(h.z), u2f(h.w), d);
-                Material m = loadMaterial(s, s.mats, sd, sd, nullptr);
+                Material m = loadMaterial(s, s.mats, s.texels, sd, sd, nullptr);
                 float4 ls4 = a.Ls[pathId];
                 f3 Ls = mk3(ls4.x, ls4.y, ls4.z);
                 if (a.computeDirect || depth > 0) {
@@ -315,32 +315,49 @@ __global__ void kRayCounts(const MptArgs a, uint32_t nb, unsigned long long* out
 }
 
 hipError_t launchMptFrame(const SceneDev& s, const MptArgs& a, const TraceScratch& ts, float4* out, bool count,
-                          unsigned long long* d_counts, hipStream_t st) {
+                          unsigned long long* d_counts, hipStream_t st, KernelTimer* tm) {
+    KernelTimer dummy;
+    if (!tm) tm = &dummy;
     const uint32_t npix = a.W * a.H;
     const int T = 256;
     hipLaunchKernelGGL(kZeroU32, dim3(1), dim3(2 * kCounterStride), 0, st, a.counters, 2u * kCounterStride);
     hipLaunchKernelGGL(kPrimaryRays, dim3((npix + T - 1) / T), dim3(T), 0, st, s, a);
     const uint32_t tb = std::min<uint32_t>(ts.gridBlocks, (npix + kBlock - 1) / kBlock);
-    unsigned long long* tstat = count ? d_counts + 2 : nullptr;
+    unsigned long long* tstat = count ? d_counts + 2 : nullptr;   // closest-hit nodes, triangles
+    unsigned long long* sstat = count ? d_counts + 4 : nullptr;   // shadow nodes, triangles
+    tm->begin(KK_TRACE, st);
     if (count)
         hipLaunchKernelGGL((kTraceQueue<false, true>), dim3(tb), dim3(kBlock), 0, st, s, a.rayQ[0], nullptr, npix, a.pixHit, ts, tstat);
     else
         hipLaunchKernelGGL((kTraceQueue<false, false>), dim3(tb), dim3(kBlock), 0, st, s, a.rayQ[0], nullptr, npix, a.pixHit, ts, tstat);
+    tm->end(st);
+    tm->begin(KK_SHADE, st);
     hipLaunchKernelGGL(kShadePrimary, dim3((a.nPaths + T - 1) / T), dim3(T), 0, st, s, a);
+    tm->end(st);
     const uint32_t pb = ts.gridBlocks;  // persistent grid for queue kernels
-    if (count) hipLaunchKernelGGL(kShadow<true>, dim3(pb), dim3(kBlock), 0, st, s, a, 0u, ts, tstat);
-    else hipLaunchKernelGGL(kShadow<false>, dim3(pb), dim3(kBlock), 0, st, s, a, 0u, ts, tstat);
+    if (a.nee) {
+        tm->begin(KK_SHADOW, st);
+        if (count) hipLaunchKernelGGL(kShadow<true>, dim3(pb), dim3(kBlock), 0, st, s, a, 0u, ts, sstat);
+        else hipLaunchKernelGGL(kShadow<false>, dim3(pb), dim3(kBlock), 0, st, s, a, 0u, ts, sstat);
+        tm->end(st);
+    }
     const uint32_t shadeBlocks = std::min<uint32_t>((a.nPaths + T - 1) / T, uint32_t(persistentBlocks()) * 4);
     for (uint32_t b = 0; b <= a.maxBounces; b++) {
         const float4* rq = a.rayQ[b & 1];
+        tm->begin(KK_TRACE, st);
         if (count)
             hipLaunchKernelGGL((kTraceQueue<false, true>), dim3(pb), dim3(kBlock), 0, st, s, rq, a.counters + b, 0u, a.hitQ, ts, tstat);
         else
             hipLaunchKernelGGL((kTraceQueue<false, false>), dim3(pb), dim3(kBlock), 0, st, s, rq, a.counters + b, 0u, a.hitQ, ts, tstat);
+        tm->end(st);
+        tm->begin(KK_SHADE, st);
         hipLaunchKernelGGL(kShade, dim3(shadeBlocks), dim3(T), 0, st, s, a, b);
+        tm->end(st);
         if (a.nee && b < a.maxBounces) {
-            if (count) hipLaunchKernelGGL(kShadow<true>, dim3(pb), dim3(kBlock), 0, st, s, a, b + 1, ts, tstat);
-            else hipLaunchKernelGGL(kShadow<false>, dim3(pb), dim3(kBlock), 0, st, s, a, b + 1, ts, tstat);
+            tm->begin(KK_SHADOW, st);
+            if (count) hipLaunchKernelGGL(kShadow<true>, dim3(pb), dim3(kBlock), 0, st, s, a, b + 1, ts, sstat);
+            else hipLaunchKernelGGL(kShadow<false>, dim3(pb), dim3(kBlock), 0, st, s, a, b + 1, ts, sstat);
+            tm->end(st);
         }
     }
     hipLaunchKernelGGL(kResolve, dim3((npix + T - 1) / T), dim3(T), 0, st, a, out);

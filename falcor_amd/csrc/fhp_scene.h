@@ -38,6 +38,8 @@ struct SceneDev {
     const uint32_t* triMat;
     const float* mats;
     const float* texels;
+    const float* matsS;    // sampling parameters: == mats / texels unless frozen for finite differences
+    const float* texelsS;
     const int4* texDesc;
     const float4* lights;        // 5 x float4 per light: (p0, area) (p1, flux) (p2, tri) (normal, 0) (Le, 0)
     const uint32_t* activeLights;
@@ -109,7 +111,7 @@ struct TexTap {
 };
 FHP_HD float wrap01(float x) { return x - floorf(x); }
 FHP_HD int imod(int a, int m) { int r = a % m; return r < 0 ? r + m : r; }
-FHP_HD f3 texFetch(const SceneDev& s, int tex, float u, float v, TexTap* tap) {
+FHP_HD f3 texFetch(const SceneDev& s, const float* texels, int tex, float u, float v, TexTap* tap) {
     int4 td = s.texDesc[tex];
     int W = td.y, H = td.z;
     float x = wrap01(u) * float(W) - 0.5f, y = wrap01(v) * float(H) - 0.5f;
@@ -117,7 +119,7 @@ FHP_HD f3 texFetch(const SceneDev& s, int tex, float u, float v, TexTap* tap) {
     float fx = x - fx0, fy = y - fy0;
     int x0 = int(fx0), y0 = int(fy0);
     int xa = imod(x0, W), xb = imod(x0 + 1, W), ya = imod(y0, H), yb = imod(y0 + 1, H);
-    const float* base = s.texels + td.x;
+    const float* base = texels + td.x;
     const float* q00 = base + 3 * (size_t(ya) * W + xa);
     const float* q10 = base + 3 * (size_t(ya) * W + xb);
     const float* q01 = base + 3 * (size_t(yb) * W + xa);
@@ -143,13 +145,14 @@ struct Material {
     int tex;
 };
 // StandardMaterial.slang:50-126 / PBRTDiffuseMaterial.slang (reflection lobes only).
-FHP_HD Material loadMaterial(const SceneDev& s, const float* mats, const Shading& sd, Shading& frameOut, TexTap* tap) {
+FHP_HD Material loadMaterial(const SceneDev& s, const float* mats, const float* texels, const Shading& sd, Shading& frameOut,
+                             TexTap* tap) {
     Material m;
     const float* r = mats + size_t(sd.mat) * M_STRIDE;
     m.type = int(r[M_TYPE]);
     m.tex = int(r[M_TEX]);
     f3 base = mk3(r[M_BASE], r[M_BASE + 1], r[M_BASE + 2]);
-    if (m.tex >= 0) base = texFetch(s, m.tex, sd.uvx, sd.uvy, tap);
+    if (m.tex >= 0) base = texFetch(s, texels, m.tex, sd.uvx, sd.uvy, tap);
     m.base = base;
     m.doubleSided = r[M_DOUBLE_SIDED] != 0.f;
     // ShadingUtils.slang:107-117 (flipShadingNormal)
@@ -311,6 +314,22 @@ FHP_HD f3 matEval(const Shading& sd, const Material& m, f3 wo) {
     if (!validRT(sd, wiL, woL, wo)) return mk3(0.f);
     Bsdf b;
     b.init(wiL, m);
+    return b.eval(wiL, woL);
+}
+// Evaluation with lobe-selection probabilities taken from another parameter set (the sampling
+// parameters of the finite-difference mode); identical to matEval when both sets are equal.
+FHP_HD f3 matEvalGated(const Shading& sd, const Material& m, const Material& mg, f3 wo) {
+    f3 wiL = sd.toLocal(sd.V), woL = sd.toLocal(wo);
+    if (m.type == MAT_PBRT_DIFFUSE) {
+        if (!validR(sd, wiL, woL, wo)) return mk3(0.f);
+        return m.diffuse * fmaxf_(woL.z, 0.f) * kInvPi;
+    }
+    if (!validRT(sd, wiL, woL, wo)) return mk3(0.f);
+    Bsdf b, g;
+    b.init(wiL, m);
+    g.init(wiL, mg);
+    b.pDiff = g.pDiff;
+    b.pSpec = g.pSpec;
     return b.eval(wiL, woL);
 }
 FHP_HD float matPdf(const Shading& sd, const Material& m, f3 wo) {

@@ -188,7 +188,14 @@ __device__ f3 warPath(const SceneDev& s, const WarArgs& a, uint32_t px, uint32_t
         sd.V = -rayD;
         setFrame(sd, normalW);
         TexTap tap;
-        Material m = loadMaterial(s, s.mats, sd, sd, &tap);
+        const Shading sd0 = sd;
+        Material m = loadMaterial(s, s.mats, s.texels, sd0, sd, &tap);
+        // Sampling parameters differ from the evaluated ones only in the finite-difference mode.
+        Material mS = m;
+        if (s.matsS != s.mats || s.texelsS != s.texels) {
+            Shading unused = sd0;
+            mS = loadMaterial(s, s.matsS, s.texelsS, sd0, unused, nullptr);
+        }
         MatGrad g;
         if (ADJ) zero(g);
         const bool isPrimary = len == 0;
@@ -226,12 +233,12 @@ __device__ f3 warPath(const SceneDev& s, const WarArgs& a, uint32_t px, uint32_t
                         float dist = length(wo);
                         wo = wo / dist;
                         float a2s = (dist * dist) / dot(ls.normalW, -wo);
-                        f3 f = matEval(sd, m, wo);
+                        f3 f = matEvalGated(sd, m, mS, wo);
                         f3 w = Li * f * a2s / a2s;
                         float misN = 1.f;
                         if (a.useMIS) {
                             float sp;
-                            if (a.useBSDFSampling || m.type == MAT_PBRT_DIFFUSE) sp = matPdf(sd, m, ldir);
+                            if (a.useBSDFSampling || m.type == MAT_PBRT_DIFFUSE) sp = matPdf(sd, mS, ldir);
                             else {
                                 f3 wiL = sd.toLocal(sd.V), woL = sd.toLocal(ldir);
                                 sp = (validRT(sd, wiL, woL, ldir) && fminf_(wiL.z, woL.z) >= kMinCosTheta) ? woL.z * kInvPi : 0.f;
@@ -250,10 +257,10 @@ __device__ f3 warPath(const SceneDev& s, const WarArgs& a, uint32_t px, uint32_t
         if (!stop) {
             // generateScatterRay (PTUtils.slang:136-172)
             Scatter sc;
-            if (matSample(sd, m, rng, sc, a.useBSDFSampling != 0) && sc.pdf > 0.f) {
+            if (matSample(sd, mS, rng, sc, a.useBSDFSampling != 0) && sc.pdf > 0.f) {
                 rayD = sc.wo;
                 rayO = rayOriginOffset(posW, sd.frontFacing ? sd.faceN : -sd.faceN);
-                fS = matEval(sd, m, rayD);
+                fS = matEvalGated(sd, m, mS, rayD);
                 woS = rayD;
                 thp = thp * (fS / sc.pdf);
                 prevN = sd.orientedFaceN();
@@ -325,21 +332,25 @@ __global__ void kWarResolve(const float4* tmp, uint32_t npix, uint32_t spp, floa
 
 }  // namespace
 
-hipError_t launchWarPrimal(const SceneDev& s, const WarArgs& a, const TraceScratch& ts, hipStream_t st) {
+hipError_t launchWarPrimal(const SceneDev& s, const WarArgs& a, const TraceScratch& ts, hipStream_t st, KernelTimer* tm) {
     const uint32_t n = a.W * a.H * a.spp;
     const uint32_t blocks = std::min<uint32_t>(ts.gridBlocks, (n + kBlock - 1) / kBlock);
     WarArgs b = a;
     b.out = a.tmp;
+    if (tm) tm->begin(KK_WAR_PRIMAL, st);
     hipLaunchKernelGGL(kWar<false>, dim3(blocks), dim3(kBlock), 0, st, s, b, ts);
+    if (tm) tm->end(st);
     const uint32_t npix = a.W * a.H;
     hipLaunchKernelGGL(kWarResolve, dim3((npix + 255) / 256), dim3(256), 0, st, a.tmp, npix, a.spp, a.out);
     return hipGetLastError();
 }
 
-hipError_t launchWarAdjoint(const SceneDev& s, const WarArgs& a, const TraceScratch& ts, hipStream_t st) {
+hipError_t launchWarAdjoint(const SceneDev& s, const WarArgs& a, const TraceScratch& ts, hipStream_t st, KernelTimer* tm) {
     const uint32_t n = a.W * a.H * a.spp;
     const uint32_t blocks = std::min<uint32_t>(ts.gridBlocks, (n + kBlock - 1) / kBlock);
+    if (tm) tm->begin(KK_WAR_ADJOINT, st);
     hipLaunchKernelGGL(kWar<true>, dim3(blocks), dim3(kBlock), 0, st, s, a, ts);
+    if (tm) tm->end(st);
     return hipGetLastError();
 }
 

@@ -97,6 +97,10 @@ class Scene:
         t = texels.detach().to(device=_device(), dtype=torch.float32).contiguous()
         _lib.check(_lib.load().fhp_scene_set_texels(self.handle, t.data_ptr(), t.numel(), _lib.stream_ptr()))
 
+    def freeze_sampling(self, enable: bool = True):
+        """Finite-difference mode: keep sampling/pdfs at the current parameters (see fhp.h)."""
+        _lib.check(_lib.load().fhp_scene_freeze_sampling(self.handle, int(enable), _lib.stream_ptr()))
+
     def light_bvh(self):
         i = self.info()
         nodes = np.zeros((i["light_bvh_nodes"], 16), np.float32)
@@ -183,6 +187,15 @@ class _NativePass(RenderPass):
         _lib.check(_lib.load().fhp_pass_get_property(self.handle, key.encode(), C.byref(v)))
         return v.value
 
+    KERNEL_KINDS = ("trace", "shadow", "shade", "war_primal", "war_adjoint", "other")
+
+    def kernel_times(self, reset: bool = True) -> dict:
+        """Per-kind kernel time (ms) and launch counts recorded with HIP events (property timeKernels=1)."""
+        ms = np.zeros(6, np.float64)
+        n = np.zeros(6, np.uint64)
+        _lib.check(_lib.load().fhp_pass_kernel_times(self.handle, ms.ctypes.data, n.ctypes.data, int(reset)))
+        return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(self.KERNEL_KINDS)}
+
     def render(self, width: int, height: int, spp: int = 1, frame: int | None = None, out=None, stats: bool = False):
         """Renders one frame of `spp` samples per pixel into a [H, W, 4] float32 tensor."""
         torch = _torch()
@@ -190,7 +203,7 @@ class _NativePass(RenderPass):
         if out is None:
             out = torch.empty((height, width, 4), dtype=torch.float32, device=dev)
         f = self.frame_count if frame is None else frame
-        st = np.zeros(4, np.uint64) if stats else None
+        st = np.zeros(6, np.uint64) if stats else None
         _lib.check(_lib.load().fhp_pass_execute(self.handle, self.scene.handle if self.scene else None, width, height, spp, f,
                                                 out.data_ptr(), st.ctypes.data if stats else None, _lib.stream_ptr()))
         if frame is None:
@@ -236,7 +249,7 @@ class WARDiffPathTracer(_NativePass):
         if d.numel() != width * height * 3:
             raise ValueError("dL_dI must hold width * height * 3 floats")
         f = self.frame_count if frame is None else frame
-        st = np.zeros(4, np.uint64) if stats else None
+        st = np.zeros(6, np.uint64) if stats else None
         _lib.check(_lib.load().fhp_pass_backward(self.handle, self.scene.handle, width, height, spp, f, d.data_ptr(),
                                                  self.scene_gradients.handle, st.ctypes.data if stats else None,
                                                  _lib.stream_ptr()))

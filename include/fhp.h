@@ -81,14 +81,20 @@ int fhp_scene_info(FhpScene scene, FhpSceneInfo* out);
 int fhp_scene_set_materials(FhpScene scene, const float* h_materials);
 /* Copy texels from device memory into the scene (same layout as the description). */
 int fhp_scene_set_texels(FhpScene scene, const float* d_texels, uint64_t num_floats, FhpStream stream);
+/* Finite-difference support: enable = 1 snapshots the current materials and texels as the
+ * sampling parameters of WARDiffPathTracer (lobe selection, direction sampling, pdfs and MIS),
+ * so later parameter changes only move BSDF values and emission, the quantities the adjoint
+ * differentiates (sampling is detached in StandardBSDF.slang:155-178 and PTUtils.slang:136-249).
+ * enable = 0 returns to normal rendering. No counterpart in the reference. */
+int fhp_scene_freeze_sampling(FhpScene scene, int enable, FhpStream stream);
 /* Light BVH export (for checks): nodes 16 floats each, leaf light lists, per-light bitmasks
  * (2 words each). Sizes from fhp_scene_info. */
 int fhp_scene_light_bvh(FhpScene scene, float* h_nodes, uint32_t* h_tri, uint32_t* h_masks);
 
 /* ---------------------------------------------------------------- ray queries
  * Rays: 8 floats each (origin xyz, tmin, direction xyz, tmax). Hits: 4 words each
- * (triangle id or 0xffffffff, t, b1, b2) with b1, b2 the barycentric weights of vertices
- * 1 and 2 (HitInfo.slang:76-79). Closest hit replaces TraceRay with ray flags 0
+ * (triangle id, t, b1, b2) with b1, b2 the barycentric weights of vertices 1 and 2
+ * (HitInfo.slang:76-79); a miss is (0xffffffff, 0, 0, 0). Closest hit replaces TraceRay with ray flags 0
  * (Source/RenderPasses/MinimalPathTracer/MinimalPathTracer.rt.slang:233-243) and
  * SceneRayQuery::traceRay (Source/Falcor/Scene/RaytracingInline.slang:86-250); any hit
  * replaces the RAY_FLAG_ACCEPT_FIRST_HIT_AND_END_SEARCH shadow query (:205-226) and
@@ -113,10 +119,18 @@ int fhp_pass_get_property(FhpPass pass, const char* key, double* value);
  * AccumulatePass Single mode does (Accumulate.cs.slang:76-114); sample k of frame f uses
  * sample number f * spp + k (MinimalPathTracer) or frame seed f * spp + k (WARDiffPathTracer).
  * Replaces RenderPass::execute (MinimalPathTracer.cpp:110-199, WARDiffPathTracer.cpp:221-...).
- * stats (optional, 4 words): closest-hit rays, visibility rays, BVH nodes visited, triangles
- * tested (the last two only when the pass property "countTraversal" is 1). */
+ * stats (optional, 6 words): closest-hit rays, visibility rays, then BVH nodes visited and
+ * triangles tested by closest-hit and by visibility queries (the last four only when the pass
+ * property "countTraversal" is 1; MinimalPathTracer only). */
 int fhp_pass_execute(FhpPass pass, FhpScene scene, uint32_t width, uint32_t height, uint32_t spp, uint32_t frame,
                      float* d_color, uint64_t* h_stats, FhpStream stream);
+
+/* Accumulated kernel times (ms) and launch counts by kernel kind, recorded with HIP events on
+ * the launch stream when the pass property "timeKernels" is 1: [0] closest-hit traversal,
+ * [1] shadow traversal, [2] shading, [3] WAR primal, [4] WAR adjoint, [5] other.
+ * reset != 0 clears the accumulators. Replaces the GPU timer events of FALCOR_PROFILE
+ * (Source/Falcor/Utils/Timing/Profiler.h:305). */
+int fhp_pass_kernel_times(FhpPass pass, double* ms, uint64_t* launches, int reset);
 
 /* ---------------------------------------------------------------- gradients
  * Replaces SceneGradients (Source/Falcor/DiffRendering/SceneGradients.cpp:42-206).
