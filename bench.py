@@ -16,7 +16,8 @@ Also reported on the same JSON line:
     against central finite differences (eps 1e-3) on 256 texels;
   * cpu_baseline: the oracle (C++ restatement, all host cores) on the same forward workload.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--skip-cpu] [--skip-diff]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--skip-cpu] [--skip-diff] [--no-tune]
+       python bench.py --copy-calibration   (1 GiB copies: scale for rocprofv3 FETCH_SIZE/WRITE_SIZE)
 """
 from __future__ import annotations
 
@@ -42,8 +43,9 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def forward_bench(steps: int, warmup: int):
+def forward_bench(steps: int, warmup: int, tune: bool):
     import torch
+    from falcor_amd import autotune
     from falcor_amd import scene as S
     from falcor_amd.render_graph import MinimalPathTracer, Scene
 
@@ -61,6 +63,16 @@ def forward_bench(steps: int, warmup: int):
     p = MinimalPathTracer(dict(props, timeKernels=1))
     p.set_scene(sc)
     out = torch.empty((H, W, 4), device="cuda")
+    launch = {k: int(p.get_property(k)) for k in autotune.LAUNCH_KEYS}
+    if tune:
+        # Untimed: pick the traversal launch configuration on this frame shape.
+        p.set_property("timeKernels", 0)
+        tr = autotune.tune(p, lambda i: p.render(W, H, SPP, frame=1000 + i, out=out), reps=2, warmup=1)
+        p.set_property("timeKernels", 1)
+        launch = dict(tr["best"])
+        table = sorted(tr["table"], key=lambda r: r[1])
+        log("autotune (fastest first): " + "; ".join(
+            f"{c['traceBlockSize']}/{c['traceStackDepth']}/{c['traceBlocksPerCU']}: {ms:.3f} ms" for c, ms in table))
     for _ in range(warmup):
         p.render(W, H, SPP, out=out)
     torch.cuda.synchronize()
@@ -78,11 +90,13 @@ def forward_bench(steps: int, warmup: int):
     sec = te - ts
     mrays = (rays_c + rays_v) / sec / 1e6
     # Traversal counts of one frame (separate, untimed run).
-    pc = MinimalPathTracer(dict(props, countTraversal=1))
+    pc = MinimalPathTracer(dict(props, countTraversal=1, **launch))
     pc.set_scene(sc)
     _, cs = pc.render(W, H, SPP, frame=0, out=out, stats=True)
     n_node = cs[2] / max(cs[0], 1)
     n_tri = cs[3] / max(cs[0], 1)
+    s_node = cs[4] / max(cs[1], 1)
+    s_tri = cs[5] / max(cs[1], 1)
     bytes_per_ray = RAY_BYTES + HIT_BYTES + NODE_BYTES * n_node + TRI_BYTES * n_tri
     trace_ms, trace_launches = kt["trace"]
     trace_bytes = bytes_per_ray * rays_c
@@ -93,9 +107,10 @@ def forward_bench(steps: int, warmup: int):
         kernel_ms=kt, nodes_per_ray=float(n_node), tris_per_ray=float(n_tri), bytes_per_ray=float(bytes_per_ray),
         trace_launches_per_step=trace_launches / steps, trace_avg_launch_ms=trace_ms / max(trace_launches, 1),
         achieved_gbs=achieved, image_mean=float(img.mean()), build_s=t2 - t1, info=info, W=W, H=H, spp=SPP, B=B,
-        data=data)
+        data=data, launch=launch, shadow_nodes_per_ray=float(s_node), shadow_tris_per_ray=float(s_tri))
     log(f"forward: {mrays:.1f} Mrays/s, {res['ms_per_step']:.2f} ms/frame, {rays_c / steps / 1e6:.2f}M closest + "
-        f"{rays_v / steps / 1e6:.2f}M visibility rays/frame; nodes/ray {n_node:.1f} tris/ray {n_tri:.1f}; kernels {kt}")
+        f"{rays_v / steps / 1e6:.2f}M visibility rays/frame; nodes/ray {n_node:.1f} tris/ray {n_tri:.1f} "
+        f"(visibility {s_node:.1f} / {s_tri:.1f}); launch {launch}; kernels {kt}")
     return res
 
 
@@ -216,6 +231,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-diff", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="use the default traversal launch configuration")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device-to-device copy 3 times (scale for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only (the path shards by views/tiles with no exchange "
@@ -225,10 +243,18 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a HIP device", file=sys.stderr)
         sys.exit(1)
+    if args.copy_calibration:
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy calibration: 3 x 1 GiB device-to-device copies")
+        return
     from falcor_amd import _lib
     _lib.load()
 
-    fwd = forward_bench(args.steps, args.warmup)
+    fwd = forward_bench(args.steps, args.warmup, not args.no_tune)
     line = {
         "metric": METRIC, "value": round(fwd["mrays"], 3), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(fwd["ms_per_step"], 3), "higher_is_better": True,
@@ -237,6 +263,9 @@ def main():
                                "LightBVH NEE, 1920x1080, 4 spp, max 6 bounces",
                    "rays_closest_per_step": int(fwd["rays_closest"]),
                    "rays_visibility_per_step": int(fwd["rays_visibility"]),
+                   "traversal_launch": fwd["launch"],
+                   "visibility_nodes_per_ray": round(fwd["shadow_nodes_per_ray"], 2),
+                   "visibility_tris_per_ray": round(fwd["shadow_tris_per_ray"], 2),
                    "bvh_build_s": round(fwd["build_s"], 2)},
         "roofline": {"bound": "hbm", "achieved": round(fwd["achieved_gbs"], 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(fwd["achieved_gbs"] / HBM_PEAK_GBS, 5), "traffic": None,

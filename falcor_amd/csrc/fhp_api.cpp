@@ -61,7 +61,8 @@ struct FhpScene_ {
     DevBuf bvh, triPos, idx, vtx, triMat, mats, texels, texDesc, lights, active, triToLight, lbvh, lbvhTri, lbvhMask;
     DevBuf ovf;  // traversal overflow stack
     DevBuf matsS, texelsS;  // frozen sampling parameters (finite-difference mode)
-    TraceScratch ts{};
+    TraceScratch ts{};      // default traversal configuration (ray queries)
+    uint32_t stackNeed = 0; // traversal stack entries a ray can need (from the BVH build)
     SceneDev dev{};
     uint64_t deviceBytes = 0;
 };
@@ -73,9 +74,21 @@ struct FhpPass_ {
     DevBuf thp, prevN, rng, Lp, Ls, rayQ0, rayQ1, pathQ0, pathQ1, hitQ, shQ, pixHit, counters, stats;
     uint32_t cap = 0, capPix = 0;
     KernelTimer timer;
+    // Captured bounce loop of the forward pass (property useGraph), keyed by everything the
+    // captured kernels read by value.
+    hipGraphExec_t graph = nullptr;
+    hipStream_t capStream = nullptr;
+    std::vector<uint8_t> graphKey;
+    void dropGraph() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        graph = nullptr;
+        graphKey.clear();
+    }
     ~FhpPass_() {
         if (timer.created)
             for (auto& e : timer.ev) (void)hipEventDestroy(e);
+        dropGraph();
+        if (capStream) (void)hipStreamDestroy(capStream);
     }
     double get(const char* k) const {
         auto it = props.find(k);
@@ -96,7 +109,8 @@ const std::map<std::string, double>& mptDefaults() {
     // MinimalPathTracer.h:79-83; useEmissiveNEE/emissiveSampler/countTraversal are extensions.
     static const std::map<std::string, double> d = {{"maxBounces", 3}, {"computeDirect", 1}, {"useImportanceSampling", 1},
                                                    {"useEmissiveNEE", 0}, {"emissiveSampler", 1}, {"countTraversal", 0},
-                                                   {"timeKernels", 0}};
+                                                   {"timeKernels", 0}, {"useGraph", 1}, {"traceBlockSize", kDefaultBlock},
+                                                   {"traceStackDepth", kDefaultLds}, {"traceBlocksPerCU", kDefaultBlocksPerCU}};
     return d;
 }
 const std::map<std::string, double>& warDefaults() {
@@ -106,14 +120,39 @@ const std::map<std::string, double>& warDefaults() {
         {"fixedSeed", 0},       {"useFixedSeed", 0},      {"useBSDFSampling", 1},   {"useNEE", 1},
         {"useMIS", 1},          {"useWAR", 0},            {"auxSampleCount", 16},   {"Log10vMFConcentration", 5},
         {"Log10vMFConcentrationScreen", 5},               {"boundaryTermBeta", 0.01}, {"useAntitheticSampling", 1},
-        {"countTraversal", 0},  {"timeKernels", 0}};
+        {"countTraversal", 0},  {"timeKernels", 0},       {"traceBlockSize", kDefaultBlock},
+        {"traceStackDepth", kDefaultLds},                 {"traceBlocksPerCU", kDefaultBlocksPerCU}};
     return d;
+}
+
+constexpr uint32_t kMinLds = 16;
+constexpr uint32_t kMaxThreadsPerCU = 2048;  // 32 resident waves per CU
+
+TraceScratch traceConfig(const FhpScene_* s, uint32_t block, uint32_t lds, uint32_t blocksPerCU) {
+    TraceScratch t{};
+    t.block = block;
+    t.lds = lds;
+    t.gridBlocks = uint32_t(numCUs()) * blocksPerCU;
+    t.ovfDepth = s->stackNeed > lds ? s->stackNeed - lds : 0u;
+    t.ovf = s->ovf.as<uint32_t>();
+    return t;
+}
+// Validates the traversal launch properties (traceBlockSize, traceStackDepth, traceBlocksPerCU).
+bool validLaunch(double block, double lds, double bpc) {
+    if (!validTraceConfig(uint32_t(block), uint32_t(lds)) || double(uint32_t(block)) != block || double(uint32_t(lds)) != lds)
+        return false;
+    return bpc >= 1 && double(uint32_t(bpc)) == bpc && uint32_t(bpc) * uint32_t(block) <= kMaxThreadsPerCU;
+}
+TraceScratch passTrace(const FhpScene_* s, const FhpPass_* p) {
+    return traceConfig(s, uint32_t(p->get("traceBlockSize")), uint32_t(p->get("traceStackDepth")),
+                       uint32_t(p->get("traceBlocksPerCU")));
 }
 
 int ensurePassBuffers(FhpPass p, uint32_t W, uint32_t H, uint32_t spp) {
     const uint64_t n = uint64_t(W) * H * spp, npix = uint64_t(W) * H;
     if (n > 0xffffffffull / 3) return fail(FHP_EINVAL, "frame too large");
     if (n <= p->cap && npix <= p->capPix) return FHP_OK;
+    p->dropGraph();
     HIP_OR_FAIL(p->thp.alloc(n * 16));
     HIP_OR_FAIL(p->prevN.alloc(n * 16));
     HIP_OR_FAIL(p->rng.alloc(n * 16));
@@ -126,7 +165,7 @@ int ensurePassBuffers(FhpPass p, uint32_t W, uint32_t H, uint32_t spp) {
     HIP_OR_FAIL(p->hitQ.alloc(n * 16));
     HIP_OR_FAIL(p->shQ.alloc(n * 48));
     HIP_OR_FAIL(p->pixHit.alloc(npix * 16));
-    HIP_OR_FAIL(p->counters.alloc(2 * kCounterStride * 4));
+    HIP_OR_FAIL(p->counters.alloc(2 * kCounterStride * 4 + 16));
     HIP_OR_FAIL(p->stats.alloc(64));
     p->cap = uint32_t(n);
     p->capPix = uint32_t(npix);
@@ -237,11 +276,11 @@ int fhp_scene_create(const FhpSceneDesc* d, FhpScene* out) {
     HIP_OR_FAIL(s->lbvhTri.upload(h.lbvhTri));
     HIP_OR_FAIL(s->lbvhMask.upload(h.lbvhMask));
     // Overflow stack for the persistent traversal grid.
-    const uint32_t need = h.stackNeed + 1;
-    s->ts.gridBlocks = uint32_t(persistentBlocks());
-    s->ts.ovfDepth = need > uint32_t(kLdsDepth) ? need - uint32_t(kLdsDepth) : 0u;
-    if (s->ts.ovfDepth) HIP_OR_FAIL(s->ovf.alloc(size_t(s->ts.ovfDepth) * s->ts.gridBlocks * kBlock * 4));
-    s->ts.ovf = s->ovf.as<uint32_t>();
+    // Sized for the smallest LDS stack and the largest grid any configuration launches.
+    s->stackNeed = h.stackNeed + 1;
+    const uint32_t worst = s->stackNeed > kMinLds ? s->stackNeed - kMinLds : 0u;
+    if (worst) HIP_OR_FAIL(s->ovf.alloc(size_t(worst) * uint32_t(numCUs()) * kMaxThreadsPerCU * 4));
+    s->ts = traceConfig(s.get(), kDefaultBlock, kDefaultLds, kDefaultBlocksPerCU);
 
     SceneDev& dv = s->dev;
     dv.bvh = s->bvh.as<float4>();
@@ -377,6 +416,9 @@ int fhp_pass_create(const char* kind, int np, const char* const* keys, const dou
     }
     if (p->get("maxBounces") < 0 || p->get("maxBounces") > kMaxBounceSlots - 2)
         return fail(FHP_EINVAL, "maxBounces out of range [0, 28]");
+    if (!validLaunch(p->get("traceBlockSize"), p->get("traceStackDepth"), p->get("traceBlocksPerCU")))
+        return fail(FHP_EINVAL, "traversal launch: traceBlockSize in {64,128,256}, traceStackDepth in {16,24,32}, "
+                                "traceBlocksPerCU >= 1 with traceBlocksPerCU*traceBlockSize <= 2048");
     *out = p.release();
     return FHP_OK;
 }
@@ -391,6 +433,15 @@ int fhp_pass_set_property(FhpPass p, const char* key, double v) {
         return FHP_OK;
     }
     if (std::string(key) == "maxBounces" && (v < 0 || v > kMaxBounceSlots - 2)) return fail(FHP_EINVAL, "maxBounces out of range");
+    const std::string k(key);
+    if (k == "traceBlockSize" || k == "traceStackDepth" || k == "traceBlocksPerCU") {
+        const double b = k == "traceBlockSize" ? v : p->get("traceBlockSize");
+        const double l = k == "traceStackDepth" ? v : p->get("traceStackDepth");
+        const double c = k == "traceBlocksPerCU" ? v : p->get("traceBlocksPerCU");
+        if (!validLaunch(b, l, c))
+            return fail(FHP_EINVAL, "traversal launch: traceBlockSize in {64,128,256}, traceStackDepth in {16,24,32}, "
+                                    "traceBlocksPerCU >= 1 with traceBlocksPerCU*traceBlockSize <= 2048");
+    }
     p->props[key] = v;
     if (std::string(key) == "fixedSeed") p->props["useFixedSeed"] = 1;
     return FHP_OK;
@@ -419,6 +470,7 @@ int fhp_pass_execute(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t spp
     const bool count = p->get("countTraversal") != 0;
     unsigned long long* dstats = p->stats.as<unsigned long long>();
     if (h_stats) HIP_OR_FAIL(launchFill(reinterpret_cast<float*>(dstats), 12, 0.f, st));
+    const TraceScratch ts = passTrace(s, p);
     if (p->kind == "MinimalPathTracer") {
         MptArgs a{};
         a.W = W; a.H = H; a.spp = spp; a.frame = frame;
@@ -436,8 +488,38 @@ int fhp_pass_execute(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t spp
         a.hitQ = p->hitQ.as<uint4>(); a.shQ = p->shQ.as<float4>(); a.pixHit = p->pixHit.as<uint4>();
         a.counters = p->counters.as<uint32_t>();
         if ((rc = prepareTimer(p))) return rc;
-        HIP_OR_FAIL(launchMptFrame(s->dev, a, s->ts, reinterpret_cast<float4*>(d_color), count, h_stats ? dstats : nullptr, st,
-                                   &p->timer));
+        const bool graph = p->get("useGraph") != 0 && !count && !p->timer.on;
+        if (!graph) {
+            HIP_OR_FAIL(launchMptFrame(s->dev, a, ts, reinterpret_cast<float4*>(d_color), count,
+                                       h_stats ? dstats : nullptr, st, &p->timer));
+        } else {
+            // Key: scene and argument bytes with the frame number zeroed (the kernels read it from
+            // device memory, written by the prologue).
+            MptArgs ka;
+            std::memcpy(&ka, &a, sizeof(MptArgs));
+            ka.frame = 0;
+            std::vector<uint8_t> key(sizeof(SceneDev) + sizeof(MptArgs) + sizeof(TraceScratch));
+            std::memcpy(key.data(), &s->dev, sizeof(SceneDev));
+            std::memcpy(key.data() + sizeof(SceneDev), &ka, sizeof(MptArgs));
+            std::memcpy(key.data() + sizeof(SceneDev) + sizeof(MptArgs), &ts, sizeof(TraceScratch));
+            if (key != p->graphKey) {
+                p->dropGraph();
+                if (!p->capStream) HIP_OR_FAIL(hipStreamCreateWithFlags(&p->capStream, hipStreamNonBlocking));
+                hipGraph_t g = nullptr;
+                HIP_OR_FAIL(hipStreamBeginCapture(p->capStream, hipStreamCaptureModeThreadLocal));
+                hipError_t ce = launchMptBounces(s->dev, a, ts, false, nullptr, p->capStream, nullptr);
+                hipError_t ee = hipStreamEndCapture(p->capStream, &g);
+                HIP_OR_FAIL(ce);
+                HIP_OR_FAIL(ee);
+                hipError_t ie = hipGraphInstantiate(&p->graph, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                HIP_OR_FAIL(ie);
+                p->graphKey = key;
+            }
+            HIP_OR_FAIL(launchMptPrologue(a, st));
+            HIP_OR_FAIL(hipGraphLaunch(p->graph, st));
+            HIP_OR_FAIL(launchMptEpilogue(a, reinterpret_cast<float4*>(d_color), h_stats ? dstats : nullptr, st));
+        }
         if ((rc = collectTimer(p, st))) return rc;
     } else {
         WarArgs a{};
@@ -452,7 +534,7 @@ int fhp_pass_execute(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t spp
         a.tmp = p->Lp.as<float4>();
         a.rays = h_stats ? dstats : nullptr;
         if ((rc = prepareTimer(p))) return rc;
-        HIP_OR_FAIL(launchWarPrimal(s->dev, a, s->ts, st, &p->timer));
+        HIP_OR_FAIL(launchWarPrimal(s->dev, a, passTrace(s, p), st, &p->timer));
         if ((rc = collectTimer(p, st))) return rc;
     }
     if (h_stats) {
@@ -542,7 +624,7 @@ int fhp_pass_backward(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t sp
     a.gradTex = g->withTex ? g->tex.as<float>() : nullptr;
     a.rays = h_stats ? dstats : nullptr;
     if ((rc = prepareTimer(p))) return rc;
-    HIP_OR_FAIL(launchWarAdjoint(s->dev, a, s->ts, st, &p->timer));
+    HIP_OR_FAIL(launchWarAdjoint(s->dev, a, passTrace(s, p), st, &p->timer));
     if ((rc = collectTimer(p, st))) return rc;
     if (h_stats) {
         unsigned long long hs[6];

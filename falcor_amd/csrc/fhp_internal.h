@@ -1,11 +1,18 @@
 // Internal interfaces between the C-ABI host code and the HIP kernels of libfhp.
 #pragma once
 #include "fhp_scene.h"
+#include <type_traits>
 
 namespace fhp {
 
-constexpr int kBlock = 128;      // threads per block for traversal kernels (2 waves)
-constexpr int kLdsDepth = 24;    // LDS stack entries per lane (24 * 128 * 4 B = 12 KiB per block)
+// Traversal launch configurations. Block size (threads), LDS stack depth (entries per lane) and
+// grid size (blocks per CU) are chosen per pass (autotuned); these are the compiled variants.
+constexpr int kDefaultBlock = 128;
+constexpr int kDefaultLds = 24;
+constexpr int kDefaultBlocksPerCU = 8;
+inline bool validTraceConfig(uint32_t block, uint32_t lds) {
+    return (block == 64 || block == 128 || block == 256) && (lds == 16 || lds == 24 || lds == 32);
+}
 constexpr int kMaxBounceSlots = 30;
 constexpr int kCounterStride = 32;  // counters: [0, 32) ray queues by depth, [32, 64) shadow queues
 
@@ -25,14 +32,34 @@ struct MptArgs {
     uint4* hitQ;     // (tri, t, b1, b2)
     float4* shQ;     // 3 float4 per shadow ray: (o, pathId) (d, tmax) (contrib, flags)
     uint4* pixHit;   // primary hits per pixel
-    uint32_t* counters;
+    uint32_t* counters;   // [0, 64) queue counters, [64] frame number (read by the kernels so a
+                          // captured graph can be replayed for any frame)
 };
 
 struct TraceScratch {
     uint32_t* ovf;     // overflow stack (ovfDepth words per thread of the persistent grid)
     uint32_t ovfDepth;
-    uint32_t gridBlocks;
+    uint32_t gridBlocks;  // persistent grid size in blocks
+    uint32_t block;       // threads per block (64, 128, 256)
+    uint32_t lds;         // LDS stack entries per lane (16, 24, 32)
 };
+
+// Calls fn(integral_constant<block>, integral_constant<lds>) for the configuration in ts.
+template <int V> using IC = std::integral_constant<int, V>;
+template <typename Fn> hipError_t withTraceConfig(const TraceScratch& ts, Fn&& fn) {
+    switch (ts.block * 100 + ts.lds) {
+        case 6416: return fn(IC<64>{}, IC<16>{});
+        case 6424: return fn(IC<64>{}, IC<24>{});
+        case 6432: return fn(IC<64>{}, IC<32>{});
+        case 12816: return fn(IC<128>{}, IC<16>{});
+        case 12824: return fn(IC<128>{}, IC<24>{});
+        case 12832: return fn(IC<128>{}, IC<32>{});
+        case 25616: return fn(IC<256>{}, IC<16>{});
+        case 25624: return fn(IC<256>{}, IC<24>{});
+        case 25632: return fn(IC<256>{}, IC<32>{});
+        default: return hipErrorInvalidValue;
+    }
+}
 
 struct KernelStats {
     unsigned long long* d;  // [nodes, tris] when counting is enabled
@@ -64,8 +91,15 @@ struct KernelTimer {
 // Launchers (fhp_kernels.hip). All are asynchronous on `stream`.
 hipError_t launchTraceRays(const SceneDev& s, const float4* rays, uint32_t n, uint4* hits, uint32_t* occluded, bool any,
                            const TraceScratch& ts, hipStream_t stream);
+// Full frame: frame-number store, bounce loop, resolve, optional ray counts.
 hipError_t launchMptFrame(const SceneDev& s, const MptArgs& a, const TraceScratch& ts, float4* out, bool count,
                           unsigned long long* d_counts, hipStream_t stream, KernelTimer* timer);
+// Pieces used with graph capture: the bounce loop (capturable, no host-varying arguments) and
+// the per-call prologue/epilogue launched outside the graph.
+hipError_t launchMptBounces(const SceneDev& s, const MptArgs& a, const TraceScratch& ts, bool count,
+                            unsigned long long* d_counts, hipStream_t stream, KernelTimer* timer);
+hipError_t launchMptPrologue(const MptArgs& a, hipStream_t stream);
+hipError_t launchMptEpilogue(const MptArgs& a, float4* out, unsigned long long* d_counts, hipStream_t stream);
 
 // WARDiffPathTracer primal and adjoint (megakernel, one path per thread).
 struct WarArgs {
@@ -86,6 +120,6 @@ hipError_t launchWarAdjoint(const SceneDev& s, const WarArgs& a, const TraceScra
 hipError_t launchAggregate(const float* tmp, uint32_t hashSize, uint32_t dim, float* out, hipStream_t stream);
 hipError_t launchFill(float* p, uint64_t n, float v, hipStream_t stream);
 
-int persistentBlocks();
+int numCUs();
 
 }  // namespace fhp

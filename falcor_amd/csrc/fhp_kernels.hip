@@ -22,32 +22,29 @@ __device__ __forceinline__ uint32_t waveAppend(bool pred, uint32_t* counter) {
     return base + uint32_t(__popcll(mask & __lanemask_lt()));
 }
 
-__device__ __forceinline__ TraceStack makeStack(uint32_t* lds, const TraceScratch& ts) {
-    const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
-    return TraceStack{lds + threadIdx.x, uint32_t(kBlock), uint32_t(kLdsDepth), ts.ovf + gtid, ts.gridBlocks * kBlock,
-                      ts.ovfDepth};
+template <int B, int D> __device__ __forceinline__ TraceStack makeStack(uint32_t* lds, const TraceScratch& ts) {
+    const uint32_t gtid = blockIdx.x * B + threadIdx.x;
+    return TraceStack{lds + threadIdx.x, uint32_t(B), uint32_t(D), ts.ovf + gtid, ts.gridBlocks * B, ts.ovfDepth};
 }
 
-int persistentBlocks() {
-    static int blocks = 0;
-    if (!blocks) {
-        int dev = 0, cus = 0;
+int numCUs() {
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
         if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
             cus = 256;
-        // 8 blocks of 2 waves per CU: 16 waves per CU (4 per SIMD) keeps the LDS stack at 96 KiB/CU.
-        blocks = cus * 8;
     }
-    return blocks;
+    return cus;
 }
 
 // ------------------------------------------------------------------ raw ray queries
-template <bool ANY>
-__global__ __launch_bounds__(kBlock) void kTraceRays(SceneDev s, const float4* __restrict__ rays, uint32_t n,
-                                                     uint4* __restrict__ hits, uint32_t* __restrict__ occl, TraceScratch ts) {
-    __shared__ uint32_t lds[kLdsDepth * kBlock];
-    TraceStack st = makeStack(lds, ts);
-    const uint32_t G = gridDim.x * kBlock;
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += G) {
+template <int B, int D, bool ANY>
+__global__ __launch_bounds__(B) void kTraceRays(SceneDev s, const float4* __restrict__ rays, uint32_t n,
+                                                uint4* __restrict__ hits, uint32_t* __restrict__ occl, TraceScratch ts) {
+    __shared__ uint32_t lds[D * B];
+    TraceStack st = makeStack<B, D>(lds, ts);
+    const uint32_t G = gridDim.x * B;
+    for (uint32_t i = blockIdx.x * B + threadIdx.x; i < n; i += G) {
         const float4 a = rays[2 * i], b = rays[2 * i + 1];
         HitRec h;
         bool hit = traceBVH4<ANY, false>(s, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), a.w, b.w, st, h);
@@ -59,10 +56,13 @@ __global__ __launch_bounds__(kBlock) void kTraceRays(SceneDev s, const float4* _
 hipError_t launchTraceRays(const SceneDev& s, const float4* rays, uint32_t n, uint4* hits, uint32_t* occluded, bool any,
                            const TraceScratch& ts, hipStream_t stream) {
     if (n == 0) return hipSuccess;
-    uint32_t blocks = std::min<uint32_t>(ts.gridBlocks, (n + kBlock - 1) / kBlock);
-    if (any) hipLaunchKernelGGL(kTraceRays<true>, dim3(blocks), dim3(kBlock), 0, stream, s, rays, n, hits, occluded, ts);
-    else hipLaunchKernelGGL(kTraceRays<false>, dim3(blocks), dim3(kBlock), 0, stream, s, rays, n, hits, occluded, ts);
-    return hipGetLastError();
+    return withTraceConfig(ts, [&](auto Bc, auto Dc) {
+        constexpr int B = decltype(Bc)::value, D = decltype(Dc)::value;
+        uint32_t blocks = std::min<uint32_t>(ts.gridBlocks, (n + B - 1) / B);
+        if (any) hipLaunchKernelGGL((kTraceRays<B, D, true>), dim3(blocks), dim3(B), 0, stream, s, rays, n, hits, occluded, ts);
+        else hipLaunchKernelGGL((kTraceRays<B, D, false>), dim3(blocks), dim3(B), 0, stream, s, rays, n, hits, occluded, ts);
+        return hipGetLastError();
+    });
 }
 
 // ------------------------------------------------------------------ MinimalPathTracer wavefront
@@ -94,16 +94,16 @@ __global__ void kPrimaryRays(SceneDev s, MptArgs a) {
     a.rayQ[0][2 * p + 1] = make_float4(d.x, d.y, d.z, tmax);
 }
 
-template <bool ANY, bool COUNT>
-__global__ __launch_bounds__(kBlock) void kTraceQueue(SceneDev s, const float4* __restrict__ rays, const uint32_t* countPtr,
-                                                      uint32_t countConst, uint4* __restrict__ hits, TraceScratch ts,
-                                                      unsigned long long* stats) {
-    __shared__ uint32_t lds[kLdsDepth * kBlock];
-    TraceStack st = makeStack(lds, ts);
+template <int B, int D, bool ANY, bool COUNT>
+__global__ __launch_bounds__(B) void kTraceQueue(SceneDev s, const float4* __restrict__ rays, const uint32_t* countPtr,
+                                                 uint32_t countConst, uint4* __restrict__ hits, TraceScratch ts,
+                                                 unsigned long long* stats) {
+    __shared__ uint32_t lds[D * B];
+    TraceStack st = makeStack<B, D>(lds, ts);
     const uint32_t n = countPtr ? *countPtr : countConst;
-    const uint32_t G = gridDim.x * kBlock;
+    const uint32_t G = gridDim.x * B;
     TraceCounters cnt{0, 0};
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += G) {
+    for (uint32_t i = blockIdx.x * B + threadIdx.x; i < n; i += G) {
         const float4 ra = rays[2 * i], rb = rays[2 * i + 1];
         HitRec h;
         const bool hit = traceBVH4<ANY, COUNT>(s, mk3(ra.x, ra.y, ra.z), mk3(rb.x, rb.y, rb.z), ra.w, rb.w, st, h, &cnt);
@@ -166,7 +166,7 @@ __global__ void kShadePrimary(SceneDev s, MptArgs a) {
         primaryRay(s.cam, px, py, a.W, a.H, o, d, tmin, tmax);
         Shading sd = shadingFromHit(s, h.x, u2f(h.z), u2f(h.w), d);
         Material m = loadMaterial(s, s.mats, s.texels, sd, sd, nullptr);
-        Rng rng = rngSeed(px, py, a.frame * a.spp + k);
+        Rng rng = rngSeed(px, py, a.counters[2 * kCounterStride] * a.spp + k);
         f3 org = rayOriginFromShading(sd);
         f3 Lp = mk3(0.f);
         if (a.computeDirect) {
@@ -262,15 +262,15 @@ __global__ void kShade(SceneDev s, MptArgs a, uint32_t depth) {
 }
 
 // Shadow rays: any hit in [0, dist]; an unoccluded ray adds its contribution to its path.
-template <bool COUNT>
-__global__ __launch_bounds__(kBlock) void kShadow(SceneDev s, MptArgs a, uint32_t slotIdx, TraceScratch ts,
-                                                  unsigned long long* stats) {
-    __shared__ uint32_t lds[kLdsDepth * kBlock];
-    TraceStack st = makeStack(lds, ts);
+template <int B, int D, bool COUNT>
+__global__ __launch_bounds__(B) void kShadow(SceneDev s, MptArgs a, uint32_t slotIdx, TraceScratch ts,
+                                             unsigned long long* stats) {
+    __shared__ uint32_t lds[D * B];
+    TraceStack st = makeStack<B, D>(lds, ts);
     const uint32_t n = a.counters[kCounterStride + slotIdx];
-    const uint32_t G = gridDim.x * kBlock;
+    const uint32_t G = gridDim.x * B;
     TraceCounters cnt{0, 0};
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += G) {
+    for (uint32_t i = blockIdx.x * B + threadIdx.x; i < n; i += G) {
         const float4* q = a.shQ + 3 * size_t(i);
         const float4 q0 = q[0], q1 = q[1];
         HitRec h;
@@ -314,54 +314,79 @@ __global__ void kRayCounts(const MptArgs a, uint32_t nb, unsigned long long* out
     }
 }
 
+__global__ void kSetU32(uint32_t* p, uint32_t v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
+hipError_t launchMptPrologue(const MptArgs& a, hipStream_t st) {
+    hipLaunchKernelGGL(kSetU32, dim3(1), dim3(1), 0, st, a.counters + 2 * kCounterStride, a.frame);
+    return hipGetLastError();
+}
+
+hipError_t launchMptEpilogue(const MptArgs& a, float4* out, unsigned long long* d_counts, hipStream_t st) {
+    const uint32_t npix = a.W * a.H;
+    hipLaunchKernelGGL(kResolve, dim3((npix + 255) / 256), dim3(256), 0, st, a, out);
+    if (d_counts) hipLaunchKernelGGL(kRayCounts, dim3(1), dim3(64), 0, st, a, a.maxBounces + 1, d_counts);
+    return hipGetLastError();
+}
+
 hipError_t launchMptFrame(const SceneDev& s, const MptArgs& a, const TraceScratch& ts, float4* out, bool count,
                           unsigned long long* d_counts, hipStream_t st, KernelTimer* tm) {
+    hipError_t e = launchMptPrologue(a, st);
+    if (e != hipSuccess) return e;
+    e = launchMptBounces(s, a, ts, count, d_counts, st, tm);
+    if (e != hipSuccess) return e;
+    return launchMptEpilogue(a, out, d_counts, st);
+}
+
+hipError_t launchMptBounces(const SceneDev& s, const MptArgs& a, const TraceScratch& ts, bool count,
+                            unsigned long long* d_counts, hipStream_t st, KernelTimer* tm) {
     KernelTimer dummy;
     if (!tm) tm = &dummy;
     const uint32_t npix = a.W * a.H;
     const int T = 256;
     hipLaunchKernelGGL(kZeroU32, dim3(1), dim3(2 * kCounterStride), 0, st, a.counters, 2u * kCounterStride);
     hipLaunchKernelGGL(kPrimaryRays, dim3((npix + T - 1) / T), dim3(T), 0, st, s, a);
-    const uint32_t tb = std::min<uint32_t>(ts.gridBlocks, (npix + kBlock - 1) / kBlock);
     unsigned long long* tstat = count ? d_counts + 2 : nullptr;   // closest-hit nodes, triangles
     unsigned long long* sstat = count ? d_counts + 4 : nullptr;   // shadow nodes, triangles
-    tm->begin(KK_TRACE, st);
-    if (count)
-        hipLaunchKernelGGL((kTraceQueue<false, true>), dim3(tb), dim3(kBlock), 0, st, s, a.rayQ[0], nullptr, npix, a.pixHit, ts, tstat);
-    else
-        hipLaunchKernelGGL((kTraceQueue<false, false>), dim3(tb), dim3(kBlock), 0, st, s, a.rayQ[0], nullptr, npix, a.pixHit, ts, tstat);
-    tm->end(st);
+    const TraceScratch& tsc = ts;
+    auto traceQ = [&](const float4* rays, const uint32_t* cptr, uint32_t cconst, uint4* hits, uint32_t maxBlocks) {
+        tm->begin(KK_TRACE, st);
+        hipError_t e = withTraceConfig(tsc, [&](auto Bc, auto Dc) {
+            constexpr int B = decltype(Bc)::value, D = decltype(Dc)::value;
+            const uint32_t blocks = std::min<uint32_t>(tsc.gridBlocks, maxBlocks);
+            if (count) hipLaunchKernelGGL((kTraceQueue<B, D, false, true>), dim3(blocks), dim3(B), 0, st, s, rays, cptr, cconst, hits, tsc, tstat);
+            else hipLaunchKernelGGL((kTraceQueue<B, D, false, false>), dim3(blocks), dim3(B), 0, st, s, rays, cptr, cconst, hits, tsc, tstat);
+            return hipGetLastError();
+        });
+        tm->end(st);
+        return e;
+    };
+    auto shadow = [&](uint32_t slot) {
+        tm->begin(KK_SHADOW, st);
+        hipError_t e = withTraceConfig(tsc, [&](auto Bc, auto Dc) {
+            constexpr int B = decltype(Bc)::value, D = decltype(Dc)::value;
+            if (count) hipLaunchKernelGGL((kShadow<B, D, true>), dim3(tsc.gridBlocks), dim3(B), 0, st, s, a, slot, tsc, sstat);
+            else hipLaunchKernelGGL((kShadow<B, D, false>), dim3(tsc.gridBlocks), dim3(B), 0, st, s, a, slot, tsc, sstat);
+            return hipGetLastError();
+        });
+        tm->end(st);
+        return e;
+    };
+    hipError_t e = traceQ(a.rayQ[0], nullptr, npix, a.pixHit, (npix + tsc.block - 1) / tsc.block);
+    if (e != hipSuccess) return e;
     tm->begin(KK_SHADE, st);
     hipLaunchKernelGGL(kShadePrimary, dim3((a.nPaths + T - 1) / T), dim3(T), 0, st, s, a);
     tm->end(st);
-    const uint32_t pb = ts.gridBlocks;  // persistent grid for queue kernels
-    if (a.nee) {
-        tm->begin(KK_SHADOW, st);
-        if (count) hipLaunchKernelGGL(kShadow<true>, dim3(pb), dim3(kBlock), 0, st, s, a, 0u, ts, sstat);
-        else hipLaunchKernelGGL(kShadow<false>, dim3(pb), dim3(kBlock), 0, st, s, a, 0u, ts, sstat);
-        tm->end(st);
-    }
-    const uint32_t shadeBlocks = std::min<uint32_t>((a.nPaths + T - 1) / T, uint32_t(persistentBlocks()) * 4);
+    if (a.nee && (e = shadow(0u)) != hipSuccess) return e;
+    const uint32_t shadeBlocks = std::min<uint32_t>((a.nPaths + T - 1) / T, uint32_t(numCUs()) * 32);
     for (uint32_t b = 0; b <= a.maxBounces; b++) {
-        const float4* rq = a.rayQ[b & 1];
-        tm->begin(KK_TRACE, st);
-        if (count)
-            hipLaunchKernelGGL((kTraceQueue<false, true>), dim3(pb), dim3(kBlock), 0, st, s, rq, a.counters + b, 0u, a.hitQ, ts, tstat);
-        else
-            hipLaunchKernelGGL((kTraceQueue<false, false>), dim3(pb), dim3(kBlock), 0, st, s, rq, a.counters + b, 0u, a.hitQ, ts, tstat);
-        tm->end(st);
+        if ((e = traceQ(a.rayQ[b & 1], a.counters + b, 0u, a.hitQ, 0xffffffffu)) != hipSuccess) return e;
         tm->begin(KK_SHADE, st);
         hipLaunchKernelGGL(kShade, dim3(shadeBlocks), dim3(T), 0, st, s, a, b);
         tm->end(st);
-        if (a.nee && b < a.maxBounces) {
-            tm->begin(KK_SHADOW, st);
-            if (count) hipLaunchKernelGGL(kShadow<true>, dim3(pb), dim3(kBlock), 0, st, s, a, b + 1, ts, sstat);
-            else hipLaunchKernelGGL(kShadow<false>, dim3(pb), dim3(kBlock), 0, st, s, a, b + 1, ts, sstat);
-            tm->end(st);
-        }
+        if (a.nee && b < a.maxBounces && (e = shadow(b + 1)) != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(kResolve, dim3((npix + T - 1) / T), dim3(T), 0, st, a, out);
-    if (d_counts) hipLaunchKernelGGL(kRayCounts, dim3(1), dim3(64), 0, st, a, a.maxBounces + 1, d_counts);
     return hipGetLastError();
 }
 

@@ -285,15 +285,15 @@ __device__ f3 warPath(const SceneDev& s, const WarArgs& a, uint32_t px, uint32_t
     return radiance;
 }
 
-template <bool ADJ>
-__global__ __launch_bounds__(kBlock) void kWar(SceneDev s, WarArgs a, TraceScratch ts) {
-    __shared__ uint32_t lds[kLdsDepth * kBlock];
-    TraceStack st{lds + threadIdx.x, uint32_t(kBlock), uint32_t(kLdsDepth), ts.ovf + blockIdx.x * kBlock + threadIdx.x,
-                  ts.gridBlocks * kBlock, ts.ovfDepth};
+template <int B, int D, bool ADJ>
+__global__ __launch_bounds__(B) void kWar(SceneDev s, WarArgs a, TraceScratch ts) {
+    __shared__ uint32_t lds[D * B];
+    TraceStack st{lds + threadIdx.x, uint32_t(B), uint32_t(D), ts.ovf + blockIdx.x * B + threadIdx.x, ts.gridBlocks * B,
+                  ts.ovfDepth};
     const uint32_t n = a.W * a.H * a.spp;
-    const uint32_t G = gridDim.x * kBlock;
+    const uint32_t G = gridDim.x * B;
     unsigned long long nc = 0, nv = 0;
-    for (uint32_t id = blockIdx.x * kBlock + threadIdx.x; id < n; id += G) {
+    for (uint32_t id = blockIdx.x * B + threadIdx.x; id < n; id += G) {
         const uint32_t pix = id / a.spp, k = id % a.spp;
         const uint32_t px = pix % a.W, py = pix / a.W;
         const uint32_t seed = a.frame + k;
@@ -334,12 +334,17 @@ __global__ void kWarResolve(const float4* tmp, uint32_t npix, uint32_t spp, floa
 
 hipError_t launchWarPrimal(const SceneDev& s, const WarArgs& a, const TraceScratch& ts, hipStream_t st, KernelTimer* tm) {
     const uint32_t n = a.W * a.H * a.spp;
-    const uint32_t blocks = std::min<uint32_t>(ts.gridBlocks, (n + kBlock - 1) / kBlock);
     WarArgs b = a;
     b.out = a.tmp;
     if (tm) tm->begin(KK_WAR_PRIMAL, st);
-    hipLaunchKernelGGL(kWar<false>, dim3(blocks), dim3(kBlock), 0, st, s, b, ts);
+    hipError_t e = withTraceConfig(ts, [&](auto Bc, auto Dc) {
+        constexpr int B = decltype(Bc)::value, D = decltype(Dc)::value;
+        const uint32_t blocks = std::min<uint32_t>(ts.gridBlocks, (n + B - 1) / B);
+        hipLaunchKernelGGL((kWar<B, D, false>), dim3(blocks), dim3(B), 0, st, s, b, ts);
+        return hipGetLastError();
+    });
     if (tm) tm->end(st);
+    if (e != hipSuccess) return e;
     const uint32_t npix = a.W * a.H;
     hipLaunchKernelGGL(kWarResolve, dim3((npix + 255) / 256), dim3(256), 0, st, a.tmp, npix, a.spp, a.out);
     return hipGetLastError();
@@ -347,10 +352,15 @@ hipError_t launchWarPrimal(const SceneDev& s, const WarArgs& a, const TraceScrat
 
 hipError_t launchWarAdjoint(const SceneDev& s, const WarArgs& a, const TraceScratch& ts, hipStream_t st, KernelTimer* tm) {
     const uint32_t n = a.W * a.H * a.spp;
-    const uint32_t blocks = std::min<uint32_t>(ts.gridBlocks, (n + kBlock - 1) / kBlock);
     if (tm) tm->begin(KK_WAR_ADJOINT, st);
-    hipLaunchKernelGGL(kWar<true>, dim3(blocks), dim3(kBlock), 0, st, s, a, ts);
+    hipError_t e = withTraceConfig(ts, [&](auto Bc, auto Dc) {
+        constexpr int B = decltype(Bc)::value, D = decltype(Dc)::value;
+        const uint32_t blocks = std::min<uint32_t>(ts.gridBlocks, (n + B - 1) / B);
+        hipLaunchKernelGGL((kWar<B, D, true>), dim3(blocks), dim3(B), 0, st, s, a, ts);
+        return hipGetLastError();
+    });
     if (tm) tm->end(st);
+    if (e != hipSuccess) return e;
     return hipGetLastError();
 }
 

@@ -62,6 +62,34 @@ def test_pass_properties_defaults_and_unknown_keys():
     lib.fhp_pass_destroy(h)
 
 
+def test_traversal_launch_properties_are_validated():
+    """traceBlockSize / traceStackDepth / traceBlocksPerCU accept only compiled variants that fit a CU."""
+    from falcor_amd import _lib
+    from falcor_amd import autotune
+    lib = _lib.load()
+    h = C.c_void_p()
+    assert lib.fhp_pass_create(b"MinimalPathTracer", 0, None, None, C.byref(h)) == 0
+    v = C.c_double()
+    for k, d in ((b"traceBlockSize", 128.0), (b"traceStackDepth", 24.0), (b"traceBlocksPerCU", 8.0)):
+        assert lib.fhp_pass_get_property(h, k, C.byref(v)) == 0 and v.value == d
+    assert lib.fhp_pass_set_property(h, b"traceBlockSize", 96.0) == -1
+    assert lib.fhp_pass_set_property(h, b"traceStackDepth", 20.0) == -1
+    assert lib.fhp_pass_set_property(h, b"traceBlocksPerCU", 0.0) == -1
+    assert lib.fhp_pass_set_property(h, b"traceBlocksPerCU", 17.0) == -1   # 17 * 128 > 2048 threads per CU
+    assert lib.fhp_pass_set_property(h, b"traceBlocksPerCU", 16.0) == 0
+    assert lib.fhp_pass_set_property(h, b"traceBlockSize", 256.0) == -1    # 16 * 256 > 2048
+    for cfg in autotune.candidates():
+        assert lib.fhp_pass_set_property(h, b"traceBlocksPerCU", 1.0) == 0
+        assert lib.fhp_pass_set_property(h, b"traceBlockSize", float(cfg["traceBlockSize"])) == 0
+        assert lib.fhp_pass_set_property(h, b"traceStackDepth", float(cfg["traceStackDepth"])) == 0
+        assert lib.fhp_pass_set_property(h, b"traceBlocksPerCU", float(cfg["traceBlocksPerCU"])) == 0
+    lib.fhp_pass_destroy(h)
+    keys = (C.c_char_p * 2)(b"traceBlockSize", b"traceBlocksPerCU")
+    vals = (C.c_double * 2)(64.0, 40.0)
+    assert lib.fhp_pass_create(b"WARDiffPathTracer", 2, keys, vals, C.byref(h)) == -1
+    assert "traceBlocksPerCU" in lib.fhp_last_error().decode()
+
+
 def test_product_never_imports_the_oracle():
     for p in (ROOT / "falcor_amd").rglob("*.py"):
         src = p.read_text()
