@@ -7,16 +7,18 @@ MinimalPathTracer with emissive next-event estimation (FP32). `value` is closest
 visibility rays per second over the timed steps (Mrays/s), with all inputs resident in HBM.
 
 Also reported on the same JSON line:
-  * roofline of the dominant kernel (closest-hit BVH4 traversal): algorithmic bytes per ray
-    (32 B ray + 16 B hit + 128 B per visited node + 48 B per tested triangle, node/triangle
-    counts measured on the same frame) over the kernel time from HIP events on its stream;
+  * roofline of the dominant kernel (kTraceBounce, the BVH4 traversal of every bounce's
+    closest-hit and visibility rays): algorithmic bytes per closest-hit ray (32 B ray + 16 B hit
+    + 128 B per visited node + 48 B per tested triangle) and per visibility ray (48 B record +
+    nodes + triangles), node/triangle counts measured on the same frame, over the kernel time
+    from HIP events on its stream;
   * diff: ms per forward+adjoint iteration of the differentiable-materials workload (64
     textured spheres, 512x512 RGB fp32 albedo textures, 512x512 image, 16 spp forward +
     16 spp adjoint), batched vs the plain per-frame loop, and the gradient relative error
     against central finite differences (eps 1e-3) on 256 texels;
   * cpu_baseline: the oracle (C++ restatement, all host cores) on the same forward workload.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--skip-cpu] [--skip-diff] [--no-tune]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--skip-cpu] [--skip-diff] [--tune [--tune-out F]]
        python bench.py --copy-calibration   (1 GiB copies: scale for rocprofv3 FETCH_SIZE/WRITE_SIZE)
 """
 from __future__ import annotations
@@ -36,14 +38,15 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "Mrays/s fwd; ms per fwd+adjoint iter at 512² 16spp; grad rel-err vs FD"
 HBM_PEAK_GBS = 8000.0
-NODE_BYTES, TRI_BYTES, RAY_BYTES, HIT_BYTES, OCC_BYTES = 128, 48, 32, 16, 4
+NODE_BYTES, TRI_BYTES, RAY_BYTES, HIT_BYTES, VIS_RECORD_BYTES = 128, 48, 32, 16, 48
+TUNE_TAG = "sphere_field_1920x1080_4spp_b6_lightbvh_nee"
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def forward_bench(steps: int, warmup: int, tune: bool):
+def forward_bench(steps: int, warmup: int, tune: bool, tune_out: str | None):
     import torch
     from falcor_amd import autotune
     from falcor_amd import scene as S
@@ -63,6 +66,9 @@ def forward_bench(steps: int, warmup: int, tune: bool):
     p = MinimalPathTracer(dict(props, timeKernels=1))
     p.set_scene(sc)
     out = torch.empty((H, W, 4), device="cuda")
+    launch = autotune.load("MinimalPathTracer", TUNE_TAG)
+    if launch is not None and not tune:
+        autotune.apply(p, launch)
     launch = {k: int(p.get_property(k)) for k in autotune.LAUNCH_KEYS}
     if tune:
         # Untimed: pick the traversal launch configuration on this frame shape.
@@ -73,6 +79,8 @@ def forward_bench(steps: int, warmup: int, tune: bool):
         table = sorted(tr["table"], key=lambda r: r[1])
         log("autotune (fastest first): " + "; ".join(
             f"{c['traceBlockSize']}/{c['traceStackDepth']}/{c['traceBlocksPerCU']}: {ms:.3f} ms" for c, ms in table))
+        if tune_out:
+            autotune.save("MinimalPathTracer", TUNE_TAG, launch, path=tune_out)
     for _ in range(warmup):
         p.render(W, H, SPP, out=out)
     torch.cuda.synchronize()
@@ -97,14 +105,19 @@ def forward_bench(steps: int, warmup: int, tune: bool):
     n_tri = cs[3] / max(cs[0], 1)
     s_node = cs[4] / max(cs[1], 1)
     s_tri = cs[5] / max(cs[1], 1)
+    # Algorithmic bytes of the traversal kernel (kTraceBounce: primary, closest-hit and visibility
+    # rays of every bounce): each closest-hit ray reads its 32 B ray and writes a 16 B hit, each
+    # visibility ray reads its 48 B record, and every visited node / tested triangle is read once.
     bytes_per_ray = RAY_BYTES + HIT_BYTES + NODE_BYTES * n_node + TRI_BYTES * n_tri
+    bytes_per_vis = VIS_RECORD_BYTES + NODE_BYTES * s_node + TRI_BYTES * s_tri
     trace_ms, trace_launches = kt["trace"]
-    trace_bytes = bytes_per_ray * rays_c
+    trace_bytes = bytes_per_ray * rays_c + bytes_per_vis * rays_v
     achieved = trace_bytes / (trace_ms * 1e-3) / 1e9 if trace_ms > 0 else 0.0
     img = out[..., :3].float()
     res = dict(
         mrays=mrays, ms_per_step=sec / steps * 1e3, rays_closest=rays_c / steps, rays_visibility=rays_v / steps,
         kernel_ms=kt, nodes_per_ray=float(n_node), tris_per_ray=float(n_tri), bytes_per_ray=float(bytes_per_ray),
+        bytes_per_vis=float(bytes_per_vis), bytes_per_launch=trace_bytes / max(trace_launches, 1),
         trace_launches_per_step=trace_launches / steps, trace_avg_launch_ms=trace_ms / max(trace_launches, 1),
         achieved_gbs=achieved, image_mean=float(img.mean()), build_s=t2 - t1, info=info, W=W, H=H, spp=SPP, B=B,
         data=data, launch=launch, shadow_nodes_per_ray=float(s_node), shadow_tris_per_ray=float(s_tri))
@@ -231,7 +244,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-diff", action="store_true")
-    ap.add_argument("--no-tune", action="store_true", help="use the default traversal launch configuration")
+    ap.add_argument("--tune", action="store_true",
+                    help="autotune the traversal launch on this workload (untimed) instead of using the tuned table")
+    ap.add_argument("--tune-out", default=None, help="with --tune: JSON file to record the chosen launch in")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy 3 times (scale for HBM byte counters)")
     args = ap.parse_args()
@@ -254,7 +269,7 @@ def main():
     from falcor_amd import _lib
     _lib.load()
 
-    fwd = forward_bench(args.steps, args.warmup, not args.no_tune)
+    fwd = forward_bench(args.steps, args.warmup, args.tune, args.tune_out)
     line = {
         "metric": METRIC, "value": round(fwd["mrays"], 3), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(fwd["ms_per_step"], 3), "higher_is_better": True,
@@ -269,8 +284,10 @@ def main():
                    "bvh_build_s": round(fwd["build_s"], 2)},
         "roofline": {"bound": "hbm", "achieved": round(fwd["achieved_gbs"], 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(fwd["achieved_gbs"] / HBM_PEAK_GBS, 5), "traffic": None,
-                     "kernel": "kTraceQueue (closest-hit BVH4 traversal)",
+                     "kernel": "kTraceBounce (BVH4 traversal of the primary, closest-hit and visibility rays)",
+                     "bytes_per_launch": round(fwd["bytes_per_launch"]),
                      "bytes_per_ray": round(fwd["bytes_per_ray"], 1), "nodes_per_ray": round(fwd["nodes_per_ray"], 2),
+                     "bytes_per_visibility_ray": round(fwd["bytes_per_vis"], 1),
                      "tris_per_ray": round(fwd["tris_per_ray"], 2),
                      "avg_launch_ms": round(fwd["trace_avg_launch_ms"], 4),
                      "launches_per_step": fwd["trace_launches_per_step"]},

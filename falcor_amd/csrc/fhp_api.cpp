@@ -110,7 +110,8 @@ const std::map<std::string, double>& mptDefaults() {
     static const std::map<std::string, double> d = {{"maxBounces", 3}, {"computeDirect", 1}, {"useImportanceSampling", 1},
                                                    {"useEmissiveNEE", 0}, {"emissiveSampler", 1}, {"countTraversal", 0},
                                                    {"timeKernels", 0}, {"useGraph", 1}, {"traceBlockSize", kDefaultBlock},
-                                                   {"traceStackDepth", kDefaultLds}, {"traceBlocksPerCU", kDefaultBlocksPerCU}};
+                                                   {"traceStackDepth", kDefaultLds}, {"traceBlocksPerCU", kDefaultBlocksPerCU},
+                                                   {"splitVisibilityLaunch", 0}};
     return d;
 }
 const std::map<std::string, double>& warDefaults() {
@@ -479,6 +480,7 @@ int fhp_pass_execute(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t spp
         a.importance = p->get("useImportanceSampling") != 0;
         a.nee = p->get("useEmissiveNEE") != 0;
         a.lightKind = uint32_t(p->get("emissiveSampler"));
+        a.separateShadow = p->get("splitVisibilityLaunch") != 0;
         if (a.nee && !a.computeDirect) return fail(FHP_EINVAL, "useEmissiveNEE requires computeDirect");
         a.nPaths = W * H * spp;
         a.thp = p->thp.as<float4>(); a.prevN = p->prevN.as<float4>(); a.rng = p->rng.as<uint4>();

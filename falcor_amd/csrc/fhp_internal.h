@@ -22,6 +22,7 @@ struct MptArgs {
     uint32_t W, H, spp, frame, maxBounces;
     uint32_t computeDirect, importance, nee, lightKind;
     uint32_t nPaths;
+    uint32_t separateShadow;  // 1: visibility rays in launches of their own (else shared with closest-hit)
     float4* thp;     // thp.xyz, pdf of the last scatter
     float4* prevN;   // shading normal of the last scatter, flags (bit 0: delta lobe)
     uint4* rng;      // xoshiro128** state

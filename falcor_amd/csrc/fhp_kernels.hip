@@ -289,6 +289,55 @@ __global__ __launch_bounds__(B) void kShadow(SceneDev s, MptArgs a, uint32_t slo
     }
 }
 
+// One traversal launch per bounce: the visibility rays queued for a bounce (count at nSPtr, none
+// when null) and its closest-hit rays (count at nCPtr, or nCConst when null) share a persistent
+// grid. Visibility rays come first in the index space; a wave mixes the two kinds only at the
+// boundary. Closest hits go to hits[j], unoccluded visibility rays add their contribution to
+// their path.
+template <int B, int D, bool COUNT>
+__global__ __launch_bounds__(B) void kTraceBounce(SceneDev s, MptArgs a, const float4* __restrict__ rays,
+                                                  const uint32_t* nCPtr, uint32_t nCConst, uint4* __restrict__ hits,
+                                                  const uint32_t* nSPtr, TraceScratch ts, unsigned long long* stats) {
+    __shared__ uint32_t lds[D * B];
+    TraceStack st = makeStack<B, D>(lds, ts);
+    const uint32_t nS = nSPtr ? *nSPtr : 0u;
+    const uint32_t n = nS + (nCPtr ? *nCPtr : nCConst);
+    const uint32_t G = gridDim.x * B;
+    TraceCounters cc{0, 0}, cs{0, 0};
+    for (uint32_t i = blockIdx.x * B + threadIdx.x; i < n; i += G) {
+        const bool vis = i < nS;
+        const uint32_t j = vis ? i : i - nS;
+        const float4* r = vis ? a.shQ + 3 * size_t(j) : rays + 2 * size_t(j);
+        const float4 r0 = r[0], r1 = r[1];
+        HitRec h;
+        TraceCounters c{0, 0};
+        const bool hit = traceBVH4<false, COUNT>(s, mk3(r0.x, r0.y, r0.z), mk3(r1.x, r1.y, r1.z), vis ? 0.f : r0.w, r1.w,
+                                                 st, h, &c, vis);
+        if (COUNT) {
+            TraceCounters& t = vis ? cs : cc;
+            t.nodes += c.nodes;
+            t.tris += c.tris;
+        }
+        if (vis) {
+            if (!hit) {
+                const float4 r2 = r[2];
+                const uint32_t pathId = f2u(r0.w);
+                float4* L = (f2u(r2.w) & 1u) ? &a.Lp[pathId] : &a.Ls[pathId];
+                const float4 v = *L;
+                *L = make_float4(v.x + r2.x, v.y + r2.y, v.z + r2.z, 0.f);
+            }
+        } else {
+            hits[j] = hit ? make_uint4(h.tri, f2u(h.t), f2u(h.b1), f2u(h.b2)) : make_uint4(kInvalid, 0u, 0u, 0u);
+        }
+    }
+    if (COUNT) {
+        atomicAdd(&stats[0], (unsigned long long)cc.nodes);
+        atomicAdd(&stats[1], (unsigned long long)cc.tris);
+        atomicAdd(&stats[2], (unsigned long long)cs.nodes);
+        atomicAdd(&stats[3], (unsigned long long)cs.tris);
+    }
+}
+
 // outColor + radiance per sample, averaged over samples like AccumulatePass Single mode.
 __global__ void kResolve(MptArgs a, float4* out) {
     const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
@@ -373,19 +422,41 @@ hipError_t launchMptBounces(const SceneDev& s, const MptArgs& a, const TraceScra
         tm->end(st);
         return e;
     };
-    hipError_t e = traceQ(a.rayQ[0], nullptr, npix, a.pixHit, (npix + tsc.block - 1) / tsc.block);
+    auto bounce = [&](const float4* rays, const uint32_t* nC, uint32_t nCConst, uint4* hits, const uint32_t* nS,
+                      uint32_t maxBlocks) {
+        tm->begin(KK_TRACE, st);
+        hipError_t e = withTraceConfig(tsc, [&](auto Bc, auto Dc) {
+            constexpr int B = decltype(Bc)::value, D = decltype(Dc)::value;
+            const uint32_t blocks = std::min<uint32_t>(tsc.gridBlocks, maxBlocks);
+            if (count)
+                hipLaunchKernelGGL((kTraceBounce<B, D, true>), dim3(blocks), dim3(B), 0, st, s, a, rays, nC, nCConst, hits, nS,
+                                   tsc, tstat);
+            else
+                hipLaunchKernelGGL((kTraceBounce<B, D, false>), dim3(blocks), dim3(B), 0, st, s, a, rays, nC, nCConst, hits, nS,
+                                   tsc, tstat);
+            return hipGetLastError();
+        });
+        tm->end(st);
+        return e;
+    };
+    // Primary rays: one per pixel (the reference's V-buffer), shared by the pixel's samples.
+    hipError_t e = a.separateShadow ? traceQ(a.rayQ[0], nullptr, npix, a.pixHit, (npix + tsc.block - 1) / tsc.block)
+                                    : bounce(a.rayQ[0], nullptr, npix, a.pixHit, nullptr, (npix + tsc.block - 1) / tsc.block);
     if (e != hipSuccess) return e;
     tm->begin(KK_SHADE, st);
     hipLaunchKernelGGL(kShadePrimary, dim3((a.nPaths + T - 1) / T), dim3(T), 0, st, s, a);
     tm->end(st);
-    if (a.nee && (e = shadow(0u)) != hipSuccess) return e;
     const uint32_t shadeBlocks = std::min<uint32_t>((a.nPaths + T - 1) / T, uint32_t(numCUs()) * 32);
     for (uint32_t b = 0; b <= a.maxBounces; b++) {
-        if ((e = traceQ(a.rayQ[b & 1], a.counters + b, 0u, a.hitQ, 0xffffffffu)) != hipSuccess) return e;
+        // Closest-hit rays of bounce b with the visibility rays produced alongside them.
+        if ((e = (a.separateShadow ? traceQ(a.rayQ[b & 1], a.counters + b, 0u, a.hitQ, 0xffffffffu)
+                                   : bounce(a.rayQ[b & 1], a.counters + b, 0u, a.hitQ,
+                                            a.nee ? a.counters + kCounterStride + b : nullptr, 0xffffffffu))) != hipSuccess)
+            return e;
+        if (a.separateShadow && a.nee && (e = shadow(b)) != hipSuccess) return e;
         tm->begin(KK_SHADE, st);
         hipLaunchKernelGGL(kShade, dim3(shadeBlocks), dim3(T), 0, st, s, a, b);
         tm->end(st);
-        if (a.nee && b < a.maxBounces && (e = shadow(b + 1)) != hipSuccess) return e;
     }
     return hipGetLastError();
 }

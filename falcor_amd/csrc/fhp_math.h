@@ -41,6 +41,7 @@ constexpr float kTwoPi = 6.28318530717958647693f;
 constexpr float kMinCosTheta = 1e-6f;    // IBSDF.slang:35
 constexpr float kMinGGXAlpha = 0.0064f;  // StandardBSDF.slang:53
 constexpr float kFltMax = 3.402823466e+38f;
+constexpr float kInf = __builtin_huge_valf();
 constexpr float kFltMin = 1.175494351e-38f;
 constexpr uint32_t kInvalid = 0xffffffffu;
 

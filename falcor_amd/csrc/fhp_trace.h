@@ -65,6 +65,17 @@ __device__ __forceinline__ uint32_t testNode4(const float4* __restrict__ node, f
     return mask;
 }
 
+// Compare-exchange on (distance, child) pairs: afterwards ta <= tb.
+__device__ __forceinline__ void sortPair(float& ta, uint32_t& ca, float& tb, uint32_t& cb) {
+    const bool sw = tb < ta;
+    const float t = sw ? tb : ta;
+    const uint32_t c = sw ? cb : ca;
+    tb = sw ? ta : tb;
+    cb = sw ? ca : cb;
+    ta = t;
+    ca = c;
+}
+
 FHP_HD f3 safeInv(f3 d) {
     // Direction components below 1e-20 in magnitude are nudged so 1/d stays finite; boxes are
     // padded, so the slab test stays conservative.
@@ -75,9 +86,10 @@ FHP_HD f3 safeInv(f3 d) {
 }
 
 // Closest hit (ANY = false) or first hit (ANY = true) in [tmin, tmax]. Returns true on a hit.
+// `anyRt` selects first-hit per lane at run time (kernels that trace both kinds of ray).
 template <bool ANY, bool COUNT>
-__device__ bool traceBVH4(const SceneDev& s, f3 o, f3 d, float tmin, float tmax, TraceStack& st, HitRec& hit,
-                          TraceCounters* cnt = nullptr) {
+__device__ __forceinline__ bool traceBVH4(const SceneDev& s, f3 o, f3 d, float tmin, float tmax, TraceStack& st,
+                                          HitRec& hit, TraceCounters* cnt = nullptr, bool anyRt = false) {
     hit.tri = kInvalid;
     hit.t = tmax;
     hit.b1 = hit.b2 = 0.f;
@@ -93,26 +105,24 @@ __device__ bool traceBVH4(const SceneDev& s, f3 o, f3 d, float tmin, float tmax,
             uint32_t mask = testNode4(node, o, inv, tmin, hit.t, tn, child);
             const uint32_t ch[4] = {child.x, child.y, child.z, child.w};
             if (mask) {
-                // Order hit children by entry distance: nearest becomes current, the rest are pushed
-                // farthest first.
-                uint32_t ids[4];
-                float ts[4];
-                int k = 0;
-#pragma unroll
-                for (int i = 0; i < 4; i++)
-                    if (mask & (1u << i)) { ids[k] = ch[i]; ts[k] = tn[i]; k++; }
-                for (int a = 1; a < k; a++) {
-                    uint32_t ia = ids[a];
-                    float ta = ts[a];
-                    int b = a - 1;
-                    while (b >= 0 && ts[b] < ta) { ts[b + 1] = ts[b]; ids[b + 1] = ids[b]; b--; }
-                    ts[b + 1] = ta; ids[b + 1] = ia;
-                }
-                // ids[] now sorted farthest first; the last one is the nearest.
+                // Order hit children by entry distance with a 4-input sorting network (missed
+                // children sort last at +inf): the nearest becomes current, the others are pushed
+                // farthest first. All indices are static, so everything stays in registers.
+                float t0 = (mask & 1u) ? tn[0] : kInf, t1 = (mask & 2u) ? tn[1] : kInf;
+                float t2 = (mask & 4u) ? tn[2] : kInf, t3 = (mask & 8u) ? tn[3] : kInf;
+                uint32_t c0 = ch[0], c1 = ch[1], c2 = ch[2], c3 = ch[3];
+                sortPair(t0, c0, t1, c1);
+                sortPair(t2, c2, t3, c3);
+                sortPair(t0, c0, t2, c2);
+                sortPair(t1, c1, t3, c3);
+                sortPair(t1, c1, t2, c2);
                 // The scene builder sizes the stack to the tree's exact worst case
                 // (sum of (children - 1) along any root-to-leaf path), so sp stays in range.
-                for (int a = 0; a < k - 1; a++) st.put(sp++, ids[a]);
-                cur = ids[k - 1];
+                const uint32_t k = __popc(mask);
+                if (k > 3u) st.put(sp++, c3);
+                if (k > 2u) st.put(sp++, c2);
+                if (k > 1u) st.put(sp++, c1);
+                cur = c0;
                 continue;
             }
         } else {
@@ -126,7 +136,7 @@ __device__ bool traceBVH4(const SceneDev& s, f3 o, f3 d, float tmin, float tmax,
                 if (watertight(o, d, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), mk3(c.x, c.y, c.z), t, b1, b2)) {
                     if (t >= tmin && t <= hit.t && (t < hit.t || hit.tri == kInvalid || id < hit.tri)) {
                         hit.tri = id; hit.t = t; hit.b1 = b1; hit.b2 = b2;
-                        if (ANY) return true;
+                        if (ANY || anyRt) return true;
                     }
                 }
             }

@@ -143,6 +143,9 @@ def test_launch_configurations_give_identical_results(gpu):
         autotune.apply(pw, cfg)
         assert torch.equal(pm.render(64, 48, spp=2, frame=3), ref_m), cfg
         assert torch.equal(pw.render(64, 48, spp=2, frame=3), ref_w), cfg
+    # Visibility rays in launches of their own instead of sharing the closest-hit launch.
+    pm.set_property("splitVisibilityLaunch", 1)
+    assert torch.equal(pm.render(64, 48, spp=2, frame=3), ref_m)
 
 
 def test_no_scene_clears_output(gpu):
