@@ -166,7 +166,7 @@ int ensurePassBuffers(FhpPass p, uint32_t W, uint32_t H, uint32_t spp) {
     HIP_OR_FAIL(p->hitQ.alloc(n * 16));
     HIP_OR_FAIL(p->shQ.alloc(n * 48));
     HIP_OR_FAIL(p->pixHit.alloc(npix * 16));
-    HIP_OR_FAIL(p->counters.alloc(kCounterWords * 4));
+    HIP_OR_FAIL(p->counters.alloc(2 * kCounterStride * 4 + 16));
     HIP_OR_FAIL(p->stats.alloc(64));
     p->cap = uint32_t(n);
     p->capPix = uint32_t(npix);

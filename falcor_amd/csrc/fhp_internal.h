@@ -14,13 +14,7 @@ inline bool validTraceConfig(uint32_t block, uint32_t lds) {
     return (block == 64 || block == 128 || block == 256) && (lds == 16 || lds == 24 || lds == 32);
 }
 constexpr int kMaxBounceSlots = 30;
-constexpr int kCounterStride = 32;
-// Counter buffer layout (32-bit words): [0, 32) closest-hit queue lengths by depth, [32, 64)
-// visibility queue lengths by slot, [64] frame number, [kFetchBase, +kXcds * kMaxBounceSlots)
-// work-fetch cursors of the traversal launches (kXcds per launch).
-constexpr int kXcds = 8;               // XCDs of an MI355X; blocks are dispatched to them round-robin
-constexpr int kFetchBase = 128;
-constexpr int kCounterWords = kFetchBase + kXcds * kMaxBounceSlots;
+constexpr int kCounterStride = 32;  // counters: [0, 32) ray queues by depth, [32, 64) shadow queues
 
 // Per-path state and queues of the wavefront forward pass. Slots are indexed by path id
 // (pixel * spp + sample); queues are compacted each bounce.

@@ -294,34 +294,17 @@ __global__ __launch_bounds__(B) void kShadow(SceneDev s, MptArgs a, uint32_t slo
 // grid. Visibility rays come first in the index space; a wave mixes the two kinds only at the
 // boundary. Closest hits go to hits[j], unoccluded visibility rays add their contribution to
 // their path.
-//
-// Work distribution: the index space is cut into kXcds contiguous segments, one per XCD (blocks go
-// to XCDs round-robin, so block b runs on XCD b % kXcds). Queue order follows pixel order, so each
-// XCD's L2 serves the rays of one band of the image. Waves fetch 64 rays at a time from their own
-// segment's cursor, then help the other segments in turn until every segment is drained.
 template <int B, int D, bool COUNT>
 __global__ __launch_bounds__(B) void kTraceBounce(SceneDev s, MptArgs a, const float4* __restrict__ rays,
                                                   const uint32_t* nCPtr, uint32_t nCConst, uint4* __restrict__ hits,
-                                                  const uint32_t* nSPtr, uint32_t* fetch, TraceScratch ts,
-                                                  unsigned long long* stats) {
+                                                  const uint32_t* nSPtr, TraceScratch ts, unsigned long long* stats) {
     __shared__ uint32_t lds[D * B];
     TraceStack st = makeStack<B, D>(lds, ts);
     const uint32_t nS = nSPtr ? *nSPtr : 0u;
     const uint32_t n = nS + (nCPtr ? *nCPtr : nCConst);
-    const uint32_t seg = ((n + kXcds - 1) / kXcds + 63u) & ~63u;
-    const uint32_t home = blockIdx.x % kXcds;
-    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t G = gridDim.x * B;
     TraceCounters cc{0, 0}, cs{0, 0};
-    for (uint32_t t = 0; t < uint32_t(kXcds); t++) {
-      const uint32_t x = (home + t) % kXcds;
-      const uint32_t segBeg = x * seg, segEnd = min(n, segBeg + seg);
-      while (segBeg < segEnd) {
-        uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(&fetch[x], 64u);
-        base = __builtin_amdgcn_readfirstlane(base);
-        if (base >= segEnd - segBeg) break;
-        const uint32_t i = segBeg + base + lane;
-        if (i >= segEnd) continue;
+    for (uint32_t i = blockIdx.x * B + threadIdx.x; i < n; i += G) {
         const bool vis = i < nS;
         const uint32_t j = vis ? i : i - nS;
         const float4* r = vis ? a.shQ + 3 * size_t(j) : rays + 2 * size_t(j);
@@ -346,7 +329,6 @@ __global__ __launch_bounds__(B) void kTraceBounce(SceneDev s, MptArgs a, const f
         } else {
             hits[j] = hit ? make_uint4(h.tri, f2u(h.t), f2u(h.b1), f2u(h.b2)) : make_uint4(kInvalid, 0u, 0u, 0u);
         }
-      }
     }
     if (COUNT) {
         atomicAdd(&stats[0], (unsigned long long)cc.nodes);
@@ -413,8 +395,6 @@ hipError_t launchMptBounces(const SceneDev& s, const MptArgs& a, const TraceScra
     const uint32_t npix = a.W * a.H;
     const int T = 256;
     hipLaunchKernelGGL(kZeroU32, dim3(1), dim3(2 * kCounterStride), 0, st, a.counters, 2u * kCounterStride);
-    hipLaunchKernelGGL(kZeroU32, dim3(1), dim3(kXcds * kMaxBounceSlots), 0, st, a.counters + kFetchBase,
-                       uint32_t(kXcds * kMaxBounceSlots));
     hipLaunchKernelGGL(kPrimaryRays, dim3((npix + T - 1) / T), dim3(T), 0, st, s, a);
     unsigned long long* tstat = count ? d_counts + 2 : nullptr;   // closest-hit nodes, triangles
     unsigned long long* sstat = count ? d_counts + 4 : nullptr;   // shadow nodes, triangles
@@ -443,18 +423,17 @@ hipError_t launchMptBounces(const SceneDev& s, const MptArgs& a, const TraceScra
         return e;
     };
     auto bounce = [&](const float4* rays, const uint32_t* nC, uint32_t nCConst, uint4* hits, const uint32_t* nS,
-                      uint32_t maxBlocks, uint32_t launchSlot) {
-        uint32_t* fetch = a.counters + kFetchBase + kXcds * launchSlot;
+                      uint32_t maxBlocks) {
         tm->begin(KK_TRACE, st);
         hipError_t e = withTraceConfig(tsc, [&](auto Bc, auto Dc) {
             constexpr int B = decltype(Bc)::value, D = decltype(Dc)::value;
             const uint32_t blocks = std::min<uint32_t>(tsc.gridBlocks, maxBlocks);
             if (count)
                 hipLaunchKernelGGL((kTraceBounce<B, D, true>), dim3(blocks), dim3(B), 0, st, s, a, rays, nC, nCConst, hits, nS,
-                                   fetch, tsc, tstat);
+                                   tsc, tstat);
             else
                 hipLaunchKernelGGL((kTraceBounce<B, D, false>), dim3(blocks), dim3(B), 0, st, s, a, rays, nC, nCConst, hits, nS,
-                                   fetch, tsc, tstat);
+                                   tsc, tstat);
             return hipGetLastError();
         });
         tm->end(st);
@@ -462,7 +441,7 @@ hipError_t launchMptBounces(const SceneDev& s, const MptArgs& a, const TraceScra
     };
     // Primary rays: one per pixel (the reference's V-buffer), shared by the pixel's samples.
     hipError_t e = a.separateShadow ? traceQ(a.rayQ[0], nullptr, npix, a.pixHit, (npix + tsc.block - 1) / tsc.block)
-                                    : bounce(a.rayQ[0], nullptr, npix, a.pixHit, nullptr, 0xffffffffu, 0u);
+                                    : bounce(a.rayQ[0], nullptr, npix, a.pixHit, nullptr, (npix + tsc.block - 1) / tsc.block);
     if (e != hipSuccess) return e;
     tm->begin(KK_SHADE, st);
     hipLaunchKernelGGL(kShadePrimary, dim3((a.nPaths + T - 1) / T), dim3(T), 0, st, s, a);
@@ -472,8 +451,7 @@ hipError_t launchMptBounces(const SceneDev& s, const MptArgs& a, const TraceScra
         // Closest-hit rays of bounce b with the visibility rays produced alongside them.
         if ((e = (a.separateShadow ? traceQ(a.rayQ[b & 1], a.counters + b, 0u, a.hitQ, 0xffffffffu)
                                    : bounce(a.rayQ[b & 1], a.counters + b, 0u, a.hitQ,
-                                            a.nee ? a.counters + kCounterStride + b : nullptr, 0xffffffffu, b + 1))) !=
-             hipSuccess)
+                                            a.nee ? a.counters + kCounterStride + b : nullptr, 0xffffffffu))) != hipSuccess)
             return e;
         if (a.separateShadow && a.nee && (e = shadow(b)) != hipSuccess) return e;
         tm->begin(KK_SHADE, st);
