@@ -42,6 +42,21 @@ NODE_BYTES, TRI_BYTES, RAY_BYTES, HIT_BYTES, VIS_RECORD_BYTES = 128, 48, 32, 16,
 TUNE_TAG = "sphere_field_1920x1080_4spp_b6_lightbvh_nee"
 
 
+TRAFFIC_PROFILE = ROOT / "profiles" / "kTraceBounce_sphere_field_1920x1080_4spp.md"
+
+
+def profiled_traffic():
+    """HBM bytes per launch of the traversal kernel from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (counters need their own profiler runs, so they are not measured here)."""
+    try:
+        for line in TRAFFIC_PROFILE.read_text().splitlines():
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -283,7 +298,8 @@ def main():
                    "visibility_tris_per_ray": round(fwd["shadow_tris_per_ray"], 2),
                    "bvh_build_s": round(fwd["build_s"], 2)},
         "roofline": {"bound": "hbm", "achieved": round(fwd["achieved_gbs"], 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(fwd["achieved_gbs"] / HBM_PEAK_GBS, 5), "traffic": None,
+                     "frac": round(fwd["achieved_gbs"] / HBM_PEAK_GBS, 5), "traffic": profiled_traffic(),
+                     "traffic_source": str(TRAFFIC_PROFILE.relative_to(ROOT)),
                      "kernel": "kTraceBounce (BVH4 traversal of the primary, closest-hit and visibility rays)",
                      "bytes_per_launch": round(fwd["bytes_per_launch"]),
                      "bytes_per_ray": round(fwd["bytes_per_ray"], 1), "nodes_per_ray": round(fwd["nodes_per_ray"], 2),
