@@ -15,21 +15,26 @@ import json
 import time
 from pathlib import Path
 
-LAUNCH_KEYS = ("traceBlockSize", "traceStackDepth", "traceBlocksPerCU")
+LAUNCH_KEYS = ("traceBlockSize", "traceStackDepth", "traceBlocksPerCU", "traceRefill")
 BLOCK_SIZES = (64, 128, 256)
 STACK_DEPTHS = (16, 24, 32)
 THREADS_PER_CU = (512, 1024, 1536, 2048)
 DEFAULT_FILE = Path(__file__).resolve().parent / "tuned_launch.json"
 
 
-def candidates(block_sizes=BLOCK_SIZES, stack_depths=STACK_DEPTHS, threads_per_cu=THREADS_PER_CU):
-    """All (block, lds, blocksPerCU) triples the library accepts from the given axes."""
+def candidates(block_sizes=BLOCK_SIZES, stack_depths=STACK_DEPTHS, threads_per_cu=THREADS_PER_CU, refill=None):
+    """All (block, lds, blocksPerCU[, refill]) combinations the library accepts from the given
+    axes. `refill` (MinimalPathTracer only) is a tuple of lane schedules to try, e.g. (0, 1)."""
     out = []
     for b in block_sizes:
         for d in stack_depths:
             for t in threads_per_cu:
                 if t % b == 0 and t // b >= 1:
-                    out.append({"traceBlockSize": b, "traceStackDepth": d, "traceBlocksPerCU": t // b})
+                    for r in (refill or (None,)):
+                        c = {"traceBlockSize": b, "traceStackDepth": d, "traceBlocksPerCU": t // b}
+                        if r is not None:
+                            c["traceRefill"] = int(r)
+                        out.append(c)
     return out
 
 
@@ -53,7 +58,7 @@ def tune(pass_, run, configs=None, reps: int = 3, warmup: int = 1, log=None) -> 
     Returns {"best": {key: value}, "ms": best ms per frame, "table": [(config, ms), ...]}.
     """
     configs = candidates() if configs is None else configs
-    saved = {k: pass_.get_property(k) for k in LAUNCH_KEYS}
+    saved = {k: pass_.get_property(k) for k in LAUNCH_KEYS if k in configs[0]} if configs else {}
     table = []
     for cfg in configs:
         apply(pass_, cfg)
@@ -81,7 +86,7 @@ def _arch() -> str:
 def save(kind: str, tag: str, best: dict, path: Path | str = DEFAULT_FILE, arch: str | None = None):
     path = Path(path)
     db = json.loads(path.read_text()) if path.exists() else {}
-    db.setdefault(arch or _arch(), {}).setdefault(kind, {})[tag] = {k: int(best[k]) for k in LAUNCH_KEYS} | {
+    db.setdefault(arch or _arch(), {}).setdefault(kind, {})[tag] = {k: int(best[k]) for k in LAUNCH_KEYS if k in best} | {
         "tuned": time.strftime("%Y-%m-%d")}
     path.write_text(json.dumps(db, indent=1, sort_keys=True) + "\n")
 
@@ -92,7 +97,7 @@ def load(kind: str, tag: str, path: Path | str = DEFAULT_FILE, arch: str | None 
         return None
     db = json.loads(path.read_text())
     ent = db.get(arch or _arch(), {}).get(kind, {}).get(tag)
-    return None if ent is None else {k: int(ent[k]) for k in LAUNCH_KEYS}
+    return None if ent is None else {k: int(ent[k]) for k in LAUNCH_KEYS if k in ent}
 
 
 def apply(pass_, cfg: dict):
@@ -102,3 +107,5 @@ def apply(pass_, cfg: dict):
     pass_.set_property("traceBlockSize", cfg["traceBlockSize"])
     pass_.set_property("traceStackDepth", cfg["traceStackDepth"])
     pass_.set_property("traceBlocksPerCU", cfg["traceBlocksPerCU"])
+    if "traceRefill" in cfg:
+        pass_.set_property("traceRefill", cfg["traceRefill"])

@@ -111,7 +111,7 @@ const std::map<std::string, double>& mptDefaults() {
                                                    {"useEmissiveNEE", 0}, {"emissiveSampler", 1}, {"countTraversal", 0},
                                                    {"timeKernels", 0}, {"useGraph", 1}, {"traceBlockSize", kDefaultBlock},
                                                    {"traceStackDepth", kDefaultLds}, {"traceBlocksPerCU", kDefaultBlocksPerCU},
-                                                   {"splitVisibilityLaunch", 0}};
+                                                   {"splitVisibilityLaunch", 0}, {"traceRefill", 1}};
     return d;
 }
 const std::map<std::string, double>& warDefaults() {
@@ -145,8 +145,10 @@ bool validLaunch(double block, double lds, double bpc) {
     return bpc >= 1 && double(uint32_t(bpc)) == bpc && uint32_t(bpc) * uint32_t(block) <= kMaxThreadsPerCU;
 }
 TraceScratch passTrace(const FhpScene_* s, const FhpPass_* p) {
-    return traceConfig(s, uint32_t(p->get("traceBlockSize")), uint32_t(p->get("traceStackDepth")),
-                       uint32_t(p->get("traceBlocksPerCU")));
+    TraceScratch t = traceConfig(s, uint32_t(p->get("traceBlockSize")), uint32_t(p->get("traceStackDepth")),
+                                 uint32_t(p->get("traceBlocksPerCU")));
+    t.refill = p->get("traceRefill") != 0;
+    return t;
 }
 
 int ensurePassBuffers(FhpPass p, uint32_t W, uint32_t H, uint32_t spp) {

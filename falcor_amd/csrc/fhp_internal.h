@@ -43,6 +43,7 @@ struct TraceScratch {
     uint32_t gridBlocks;  // persistent grid size in blocks
     uint32_t block;       // threads per block (64, 128, 256)
     uint32_t lds;         // LDS stack entries per lane (16, 24, 32)
+    uint32_t refill;      // 1: a lane starts its next ray as soon as its current one finishes
 };
 
 // Calls fn(integral_constant<block>, integral_constant<lds>) for the configuration in ts.

@@ -294,7 +294,12 @@ __global__ __launch_bounds__(B) void kShadow(SceneDev s, MptArgs a, uint32_t slo
 // grid. Visibility rays come first in the index space; a wave mixes the two kinds only at the
 // boundary. Closest hits go to hits[j], unoccluded visibility rays add their contribution to
 // their path.
-template <int B, int D, bool COUNT>
+//
+// REFILL selects the lane schedule: false traces each lane's rays one after the other to
+// completion (a wave waits for its slowest ray before any lane starts the next), true starts a
+// lane's next ray as soon as its current one is finished, so lanes stay busy while the wave's
+// other rays are still in flight. Both give identical results.
+template <int B, int D, bool COUNT, bool REFILL>
 __global__ __launch_bounds__(B) void kTraceBounce(SceneDev s, MptArgs a, const float4* __restrict__ rays,
                                                   const uint32_t* nCPtr, uint32_t nCConst, uint4* __restrict__ hits,
                                                   const uint32_t* nSPtr, TraceScratch ts, unsigned long long* stats) {
@@ -304,30 +309,62 @@ __global__ __launch_bounds__(B) void kTraceBounce(SceneDev s, MptArgs a, const f
     const uint32_t n = nS + (nCPtr ? *nCPtr : nCConst);
     const uint32_t G = gridDim.x * B;
     TraceCounters cc{0, 0}, cs{0, 0};
-    for (uint32_t i = blockIdx.x * B + threadIdx.x; i < n; i += G) {
+    TravState r;
+    // Ray i: visibility record i (i < nS) or closest-hit ray i - nS.
+    auto startRay = [&](uint32_t i) {
         const bool vis = i < nS;
-        const uint32_t j = vis ? i : i - nS;
-        const float4* r = vis ? a.shQ + 3 * size_t(j) : rays + 2 * size_t(j);
-        const float4 r0 = r[0], r1 = r[1];
-        HitRec h;
-        TraceCounters c{0, 0};
-        const bool hit = traceBVH4<false, COUNT>(s, mk3(r0.x, r0.y, r0.z), mk3(r1.x, r1.y, r1.z), vis ? 0.f : r0.w, r1.w,
-                                                 st, h, &c, vis);
-        if (COUNT) {
-            TraceCounters& t = vis ? cs : cc;
-            t.nodes += c.nodes;
-            t.tris += c.tris;
-        }
-        if (vis) {
+        const float4* q = vis ? a.shQ + 3 * size_t(i) : rays + 2 * size_t(i - nS);
+        const float4 r0 = q[0], r1 = q[1];
+        r.start(mk3(r0.x, r0.y, r0.z), mk3(r1.x, r1.y, r1.z), vis ? 0.f : r0.w, r1.w, vis);
+    };
+    auto finishRay = [&](uint32_t i) {
+        const bool hit = r.hit.tri != kInvalid;
+        if (i < nS) {
             if (!hit) {
-                const float4 r2 = r[2];
-                const uint32_t pathId = f2u(r0.w);
+                const float4* q = a.shQ + 3 * size_t(i);
+                const float4 r2 = q[2];
+                const uint32_t pathId = f2u(q[0].w);
                 float4* L = (f2u(r2.w) & 1u) ? &a.Lp[pathId] : &a.Ls[pathId];
                 const float4 v = *L;
                 *L = make_float4(v.x + r2.x, v.y + r2.y, v.z + r2.z, 0.f);
             }
         } else {
-            hits[j] = hit ? make_uint4(h.tri, f2u(h.t), f2u(h.b1), f2u(h.b2)) : make_uint4(kInvalid, 0u, 0u, 0u);
+            const HitRec& h = r.hit;
+            hits[i - nS] = hit ? make_uint4(h.tri, f2u(h.t), f2u(h.b1), f2u(h.b2)) : make_uint4(kInvalid, 0u, 0u, 0u);
+        }
+    };
+    uint32_t i = blockIdx.x * B + threadIdx.x;
+    if (REFILL) {
+        bool have = i < n;
+        if (have) startRay(i);
+        while (have) {
+            TraceCounters c{0, 0};
+            bool live = r.nodes<COUNT>(s, st, &c);
+            if (live) live = r.leaf<COUNT>(s, st, &c);
+            if (COUNT) {
+                TraceCounters& t = i < nS ? cs : cc;
+                t.nodes += c.nodes;
+                t.tris += c.tris;
+            }
+            if (!live) {
+                finishRay(i);
+                i += G;
+                have = i < n;
+                if (have) startRay(i);
+            }
+        }
+    } else {
+        for (; i < n; i += G) {
+            startRay(i);
+            TraceCounters c{0, 0};
+            while (r.nodes<COUNT>(s, st, &c) && r.leaf<COUNT>(s, st, &c)) {
+            }
+            if (COUNT) {
+                TraceCounters& t = i < nS ? cs : cc;
+                t.nodes += c.nodes;
+                t.tris += c.tris;
+            }
+            finishRay(i);
         }
     }
     if (COUNT) {
@@ -428,12 +465,18 @@ hipError_t launchMptBounces(const SceneDev& s, const MptArgs& a, const TraceScra
         hipError_t e = withTraceConfig(tsc, [&](auto Bc, auto Dc) {
             constexpr int B = decltype(Bc)::value, D = decltype(Dc)::value;
             const uint32_t blocks = std::min<uint32_t>(tsc.gridBlocks, maxBlocks);
-            if (count)
-                hipLaunchKernelGGL((kTraceBounce<B, D, true>), dim3(blocks), dim3(B), 0, st, s, a, rays, nC, nCConst, hits, nS,
-                                   tsc, tstat);
+            if (count && tsc.refill)
+                hipLaunchKernelGGL((kTraceBounce<B, D, true, true>), dim3(blocks), dim3(B), 0, st, s, a, rays, nC, nCConst, hits,
+                                   nS, tsc, tstat);
+            else if (count)
+                hipLaunchKernelGGL((kTraceBounce<B, D, true, false>), dim3(blocks), dim3(B), 0, st, s, a, rays, nC, nCConst,
+                                   hits, nS, tsc, tstat);
+            else if (tsc.refill)
+                hipLaunchKernelGGL((kTraceBounce<B, D, false, true>), dim3(blocks), dim3(B), 0, st, s, a, rays, nC, nCConst,
+                                   hits, nS, tsc, tstat);
             else
-                hipLaunchKernelGGL((kTraceBounce<B, D, false>), dim3(blocks), dim3(B), 0, st, s, a, rays, nC, nCConst, hits, nS,
-                                   tsc, tstat);
+                hipLaunchKernelGGL((kTraceBounce<B, D, false, false>), dim3(blocks), dim3(B), 0, st, s, a, rays, nC, nCConst,
+                                   hits, nS, tsc, tstat);
             return hipGetLastError();
         });
         tm->end(st);

@@ -85,32 +85,44 @@ FHP_HD f3 safeInv(f3 d) {
     return mk3(1.f / x, 1.f / y, 1.f / z);
 }
 
-// Closest hit (ANY = false) or first hit (ANY = true) in [tmin, tmax]. Returns true on a hit.
-// `anyRt` selects first-hit per lane at run time (kernels that trace both kinds of ray).
-template <bool ANY, bool COUNT>
-__device__ __forceinline__ bool traceBVH4(const SceneDev& s, f3 o, f3 d, float tmin, float tmax, TraceStack& st,
-                                          HitRec& hit, TraceCounters* cnt = nullptr, bool anyRt = false) {
-    hit.tri = kInvalid;
-    hit.t = tmax;
-    hit.b1 = hit.b2 = 0.f;
-    const f3 inv = safeInv(d);
-    uint32_t sp = 0;
-    uint32_t cur = 0;  // root node (always internal)
-    for (;;) {
-        if (!childIsLeaf(cur)) {
+// Resumable traversal of one ray: `nodes` walks internal nodes nearest-first until the current
+// node is a leaf (the while-while scheme: a wave's lanes reach their next leaf together), `leaf`
+// tests that leaf's triangles and moves on. Both return false once the ray is finished (stack
+// empty, or the first hit of an any-hit ray). A kernel can therefore start a new ray on a lane as
+// soon as the lane's previous ray is finished, instead of waiting for the slowest lane.
+struct TravState {
+    f3 o, d, inv;
+    float tmin;
+    HitRec hit;
+    uint32_t cur, sp;
+    bool any;
+    __device__ __forceinline__ void start(f3 org, f3 dir, float t0, float t1, bool anyHit) {
+        o = org;
+        d = dir;
+        inv = safeInv(dir);
+        tmin = t0;
+        hit.tri = kInvalid;
+        hit.t = t1;
+        hit.b1 = hit.b2 = 0.f;
+        cur = 0;  // root node (always internal)
+        sp = 0;
+        any = anyHit;
+    }
+    template <bool COUNT>
+    __device__ __forceinline__ bool nodes(const SceneDev& s, TraceStack& st, TraceCounters* cnt) {
+        while (!childIsLeaf(cur)) {
             const float4* node = s.bvh + 8 * size_t(cur);
             float tn[4];
             uint4 child;
             if (COUNT) cnt->nodes++;
-            uint32_t mask = testNode4(node, o, inv, tmin, hit.t, tn, child);
-            const uint32_t ch[4] = {child.x, child.y, child.z, child.w};
+            const uint32_t mask = testNode4(node, o, inv, tmin, hit.t, tn, child);
             if (mask) {
                 // Order hit children by entry distance with a 4-input sorting network (missed
                 // children sort last at +inf): the nearest becomes current, the others are pushed
                 // farthest first. All indices are static, so everything stays in registers.
                 float t0 = (mask & 1u) ? tn[0] : kInf, t1 = (mask & 2u) ? tn[1] : kInf;
                 float t2 = (mask & 4u) ? tn[2] : kInf, t3 = (mask & 8u) ? tn[3] : kInf;
-                uint32_t c0 = ch[0], c1 = ch[1], c2 = ch[2], c3 = ch[3];
+                uint32_t c0 = child.x, c1 = child.y, c2 = child.z, c3 = child.w;
                 sortPair(t0, c0, t1, c1);
                 sortPair(t2, c2, t3, c3);
                 sortPair(t0, c0, t2, c2);
@@ -123,27 +135,45 @@ __device__ __forceinline__ bool traceBVH4(const SceneDev& s, f3 o, f3 d, float t
                 if (k > 2u) st.put(sp++, c2);
                 if (k > 1u) st.put(sp++, c1);
                 cur = c0;
-                continue;
+            } else {
+                if (sp == 0) return false;
+                cur = st.get(--sp);
             }
-        } else {
-            const uint32_t first = leafFirst(cur), count = leafCount(cur);
-            for (uint32_t j = 0; j < count; j++) {
-                const float4* tp = s.triPos + 3 * size_t(first + j);
-                const float4 a = tp[0], b = tp[1], c = tp[2];
-                const uint32_t id = f2u(a.w);
-                float t, b1, b2;
-                if (COUNT) cnt->tris++;
-                if (watertight(o, d, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), mk3(c.x, c.y, c.z), t, b1, b2)) {
-                    if (t >= tmin && t <= hit.t && (t < hit.t || hit.tri == kInvalid || id < hit.tri)) {
-                        hit.tri = id; hit.t = t; hit.b1 = b1; hit.b2 = b2;
-                        if (ANY || anyRt) return true;
-                    }
+        }
+        return true;
+    }
+    template <bool COUNT>
+    __device__ __forceinline__ bool leaf(const SceneDev& s, TraceStack& st, TraceCounters* cnt) {
+        const uint32_t first = leafFirst(cur), count = leafCount(cur);
+        for (uint32_t j = 0; j < count; j++) {
+            const float4* tp = s.triPos + 3 * size_t(first + j);
+            const float4 a = tp[0], b = tp[1], c = tp[2];
+            const uint32_t id = f2u(a.w);
+            float t, b1, b2;
+            if (COUNT) cnt->tris++;
+            if (watertight(o, d, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), mk3(c.x, c.y, c.z), t, b1, b2)) {
+                if (t >= tmin && t <= hit.t && (t < hit.t || hit.tri == kInvalid || id < hit.tri)) {
+                    hit.tri = id; hit.t = t; hit.b1 = b1; hit.b2 = b2;
+                    if (any) return false;
                 }
             }
         }
-        if (sp == 0) break;
+        if (sp == 0) return false;
         cur = st.get(--sp);
+        return true;
     }
+};
+
+// Closest hit (ANY = false) or first hit (ANY = true) in [tmin, tmax]. Returns true on a hit.
+// `anyRt` selects first-hit per lane at run time (kernels that trace both kinds of ray).
+template <bool ANY, bool COUNT>
+__device__ __forceinline__ bool traceBVH4(const SceneDev& s, f3 o, f3 d, float tmin, float tmax, TraceStack& st,
+                                          HitRec& hit, TraceCounters* cnt = nullptr, bool anyRt = false) {
+    TravState r;
+    r.start(o, d, tmin, tmax, ANY || anyRt);
+    while (r.nodes<COUNT>(s, st, cnt) && r.leaf<COUNT>(s, st, cnt)) {
+    }
+    hit = r.hit;
     return hit.tri != kInvalid;
 }
 
