@@ -339,8 +339,7 @@ __global__ __launch_bounds__(B) void kTraceBounce(SceneDev s, MptArgs a, const f
         if (have) startRay(i);
         while (have) {
             TraceCounters c{0, 0};
-            bool live = r.nodes<COUNT>(s, st, &c);
-            if (live) live = r.leaf<COUNT>(s, st, &c);
+            const bool live = r.step<COUNT>(s, st, &c);
             if (COUNT) {
                 TraceCounters& t = i < nS ? cs : cc;
                 t.nodes += c.nodes;
@@ -357,7 +356,7 @@ __global__ __launch_bounds__(B) void kTraceBounce(SceneDev s, MptArgs a, const f
         for (; i < n; i += G) {
             startRay(i);
             TraceCounters c{0, 0};
-            while (r.nodes<COUNT>(s, st, &c) && r.leaf<COUNT>(s, st, &c)) {
+            while (r.step<COUNT>(s, st, &c)) {
             }
             if (COUNT) {
                 TraceCounters& t = i < nS ? cs : cc;

@@ -85,11 +85,11 @@ FHP_HD f3 safeInv(f3 d) {
     return mk3(1.f / x, 1.f / y, 1.f / z);
 }
 
-// Resumable traversal of one ray: `nodes` walks internal nodes nearest-first until the current
-// node is a leaf (the while-while scheme: a wave's lanes reach their next leaf together), `leaf`
-// tests that leaf's triangles and moves on. Both return false once the ray is finished (stack
-// empty, or the first hit of an any-hit ray). A kernel can therefore start a new ray on a lane as
-// soon as the lane's previous ray is finished, instead of waiting for the slowest lane.
+// Resumable traversal of one ray. Each `step` visits one node: an internal node's children are
+// tested and the nearest hit child becomes current (the others are pushed), or a leaf's triangles
+// are tested and the next node is popped. `step` returns false once the ray is finished (stack
+// empty, or the first hit of an any-hit ray), so a kernel can start a new ray on a lane as soon as
+// the lane's previous ray is finished instead of waiting for the slowest lane of the wave.
 struct TravState {
     f3 o, d, inv;
     float tmin;
@@ -109,8 +109,8 @@ struct TravState {
         any = anyHit;
     }
     template <bool COUNT>
-    __device__ __forceinline__ bool nodes(const SceneDev& s, TraceStack& st, TraceCounters* cnt) {
-        while (!childIsLeaf(cur)) {
+    __device__ __forceinline__ bool step(const SceneDev& s, TraceStack& st, TraceCounters* cnt) {
+        if (!childIsLeaf(cur)) {
             const float4* node = s.bvh + 8 * size_t(cur);
             float tn[4];
             uint4 child;
@@ -135,26 +135,21 @@ struct TravState {
                 if (k > 2u) st.put(sp++, c2);
                 if (k > 1u) st.put(sp++, c1);
                 cur = c0;
-            } else {
-                if (sp == 0) return false;
-                cur = st.get(--sp);
+                return true;
             }
-        }
-        return true;
-    }
-    template <bool COUNT>
-    __device__ __forceinline__ bool leaf(const SceneDev& s, TraceStack& st, TraceCounters* cnt) {
-        const uint32_t first = leafFirst(cur), count = leafCount(cur);
-        for (uint32_t j = 0; j < count; j++) {
-            const float4* tp = s.triPos + 3 * size_t(first + j);
-            const float4 a = tp[0], b = tp[1], c = tp[2];
-            const uint32_t id = f2u(a.w);
-            float t, b1, b2;
-            if (COUNT) cnt->tris++;
-            if (watertight(o, d, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), mk3(c.x, c.y, c.z), t, b1, b2)) {
-                if (t >= tmin && t <= hit.t && (t < hit.t || hit.tri == kInvalid || id < hit.tri)) {
-                    hit.tri = id; hit.t = t; hit.b1 = b1; hit.b2 = b2;
-                    if (any) return false;
+        } else {
+            const uint32_t first = leafFirst(cur), count = leafCount(cur);
+            for (uint32_t j = 0; j < count; j++) {
+                const float4* tp = s.triPos + 3 * size_t(first + j);
+                const float4 a = tp[0], b = tp[1], c = tp[2];
+                const uint32_t id = f2u(a.w);
+                float t, b1, b2;
+                if (COUNT) cnt->tris++;
+                if (watertight(o, d, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), mk3(c.x, c.y, c.z), t, b1, b2)) {
+                    if (t >= tmin && t <= hit.t && (t < hit.t || hit.tri == kInvalid || id < hit.tri)) {
+                        hit.tri = id; hit.t = t; hit.b1 = b1; hit.b2 = b2;
+                        if (any) return false;
+                    }
                 }
             }
         }
@@ -171,7 +166,7 @@ __device__ __forceinline__ bool traceBVH4(const SceneDev& s, f3 o, f3 d, float t
                                           HitRec& hit, TraceCounters* cnt = nullptr, bool anyRt = false) {
     TravState r;
     r.start(o, d, tmin, tmax, ANY || anyRt);
-    while (r.nodes<COUNT>(s, st, cnt) && r.leaf<COUNT>(s, st, cnt)) {
+    while (r.step<COUNT>(s, st, cnt)) {
     }
     hit = r.hit;
     return hit.tri != kInvalid;
