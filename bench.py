@@ -84,17 +84,17 @@ def forward_bench(steps: int, warmup: int, tune: bool, tune_out: str | None):
     launch = autotune.load("MinimalPathTracer", TUNE_TAG)
     if launch is not None and not tune:
         autotune.apply(p, launch)
-    launch = {k: int(p.get_property(k)) for k in autotune.LAUNCH_KEYS}  # MinimalPathTracer has all four
+    launch = {k: int(p.get_property(k)) for k in autotune.LAUNCH_KEYS}
     if tune:
         # Untimed: pick the traversal launch configuration on this frame shape.
         p.set_property("timeKernels", 0)
         tr = autotune.tune(p, lambda i: p.render(W, H, SPP, frame=1000 + i, out=out),
-                           configs=autotune.candidates(refill=(0, 1)), reps=2, warmup=1)
+                           reps=2, warmup=1)
         p.set_property("timeKernels", 1)
         launch = dict(tr["best"])
         table = sorted(tr["table"], key=lambda r: r[1])
         log("autotune (fastest first): " + "; ".join(
-            f"{c['traceBlockSize']}/{c['traceStackDepth']}/{c['traceBlocksPerCU']}/r{c['traceRefill']}: {ms:.3f} ms"
+            f"{c['traceBlockSize']}/{c['traceStackDepth']}/{c['traceBlocksPerCU']}: {ms:.3f} ms"
             for c, ms in table))
         if tune_out:
             autotune.save("MinimalPathTracer", TUNE_TAG, launch, path=tune_out)

@@ -15,26 +15,21 @@ import json
 import time
 from pathlib import Path
 
-LAUNCH_KEYS = ("traceBlockSize", "traceStackDepth", "traceBlocksPerCU", "traceRefill")
+LAUNCH_KEYS = ("traceBlockSize", "traceStackDepth", "traceBlocksPerCU")
 BLOCK_SIZES = (64, 128, 256)
 STACK_DEPTHS = (16, 24, 32)
 THREADS_PER_CU = (512, 1024, 1536, 2048)
 DEFAULT_FILE = Path(__file__).resolve().parent / "tuned_launch.json"
 
 
-def candidates(block_sizes=BLOCK_SIZES, stack_depths=STACK_DEPTHS, threads_per_cu=THREADS_PER_CU, refill=None):
-    """All (block, lds, blocksPerCU[, refill]) combinations the library accepts from the given
-    axes. `refill` (MinimalPathTracer only) is a tuple of lane schedules to try, e.g. (0, 1)."""
+def candidates(block_sizes=BLOCK_SIZES, stack_depths=STACK_DEPTHS, threads_per_cu=THREADS_PER_CU):
+    """All (block, lds, blocksPerCU) triples the library accepts from the given axes."""
     out = []
     for b in block_sizes:
         for d in stack_depths:
             for t in threads_per_cu:
                 if t % b == 0 and t // b >= 1:
-                    for r in (refill or (None,)):
-                        c = {"traceBlockSize": b, "traceStackDepth": d, "traceBlocksPerCU": t // b}
-                        if r is not None:
-                            c["traceRefill"] = int(r)
-                        out.append(c)
+                    out.append({"traceBlockSize": b, "traceStackDepth": d, "traceBlocksPerCU": t // b})
     return out
 
 
@@ -107,5 +102,3 @@ def apply(pass_, cfg: dict):
     pass_.set_property("traceBlockSize", cfg["traceBlockSize"])
     pass_.set_property("traceStackDepth", cfg["traceStackDepth"])
     pass_.set_property("traceBlocksPerCU", cfg["traceBlocksPerCU"])
-    if "traceRefill" in cfg:
-        pass_.set_property("traceRefill", cfg["traceRefill"])

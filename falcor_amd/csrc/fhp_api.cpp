@@ -71,7 +71,7 @@ struct FhpPass_ {
     std::string kind;
     std::map<std::string, double> props;
     // Wavefront buffers (forward pass).
-    DevBuf thp, prevN, rng, Lp, Ls, rayQ0, rayQ1, pathQ0, pathQ1, hitQ, shQ, pixHit, counters, stats;
+    DevBuf thp, prevN, rng, Lp, Ls, rayQ0, rayQ1, pathQ0, pathQ1, hitQ, hitIdx, shQ, pixHit, counters, stats;
     uint32_t cap = 0, capPix = 0;
     KernelTimer timer;
     // Captured bounce loop of the forward pass (property useGraph), keyed by everything the
@@ -111,7 +111,7 @@ const std::map<std::string, double>& mptDefaults() {
                                                    {"useEmissiveNEE", 0}, {"emissiveSampler", 1}, {"countTraversal", 0},
                                                    {"timeKernels", 0}, {"useGraph", 1}, {"traceBlockSize", kDefaultBlock},
                                                    {"traceStackDepth", kDefaultLds}, {"traceBlocksPerCU", kDefaultBlocksPerCU},
-                                                   {"splitVisibilityLaunch", 0}, {"traceRefill", 1}};
+                                                   {"splitVisibilityLaunch", 0}};
     return d;
 }
 const std::map<std::string, double>& warDefaults() {
@@ -147,7 +147,6 @@ bool validLaunch(double block, double lds, double bpc) {
 TraceScratch passTrace(const FhpScene_* s, const FhpPass_* p) {
     TraceScratch t = traceConfig(s, uint32_t(p->get("traceBlockSize")), uint32_t(p->get("traceStackDepth")),
                                  uint32_t(p->get("traceBlocksPerCU")));
-    t.refill = p->get("traceRefill") != 0;
     return t;
 }
 
@@ -168,7 +167,8 @@ int ensurePassBuffers(FhpPass p, uint32_t W, uint32_t H, uint32_t spp) {
     HIP_OR_FAIL(p->hitQ.alloc(n * 16));
     HIP_OR_FAIL(p->shQ.alloc(n * 48));
     HIP_OR_FAIL(p->pixHit.alloc(npix * 16));
-    HIP_OR_FAIL(p->counters.alloc(2 * kCounterStride * 4 + 16));
+    HIP_OR_FAIL(p->counters.alloc(kCounterWords * 4));
+    HIP_OR_FAIL(p->hitIdx.alloc(n * 4));
     HIP_OR_FAIL(p->stats.alloc(64));
     p->cap = uint32_t(n);
     p->capPix = uint32_t(npix);
@@ -489,7 +489,7 @@ int fhp_pass_execute(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t spp
         a.Lp = p->Lp.as<float4>(); a.Ls = p->Ls.as<float4>();
         a.rayQ[0] = p->rayQ0.as<float4>(); a.rayQ[1] = p->rayQ1.as<float4>();
         a.pathQ[0] = p->pathQ0.as<uint32_t>(); a.pathQ[1] = p->pathQ1.as<uint32_t>();
-        a.hitQ = p->hitQ.as<uint4>(); a.shQ = p->shQ.as<float4>(); a.pixHit = p->pixHit.as<uint4>();
+        a.hitQ = p->hitQ.as<uint4>(); a.hitIdx = p->hitIdx.as<uint32_t>(); a.shQ = p->shQ.as<float4>(); a.pixHit = p->pixHit.as<uint4>();
         a.counters = p->counters.as<uint32_t>();
         if ((rc = prepareTimer(p))) return rc;
         const bool graph = p->get("useGraph") != 0 && !count && !p->timer.on;

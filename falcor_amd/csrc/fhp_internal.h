@@ -14,7 +14,11 @@ inline bool validTraceConfig(uint32_t block, uint32_t lds) {
     return (block == 64 || block == 128 || block == 256) && (lds == 16 || lds == 24 || lds == 32);
 }
 constexpr int kMaxBounceSlots = 30;
-constexpr int kCounterStride = 32;  // counters: [0, 32) ray queues by depth, [32, 64) shadow queues
+constexpr int kCounterStride = 32;
+// Counter words: [0, 32) closest-hit queue lengths by depth, [32, 64) visibility queue lengths by
+// slot, [64] frame number, [kHitCount, +32) compacted hit counts by depth.
+constexpr int kHitCount = 3 * kCounterStride;
+constexpr int kCounterWords = 4 * kCounterStride;  // counters: [0, 32) ray queues by depth, [32, 64) shadow queues
 
 // Per-path state and queues of the wavefront forward pass. Slots are indexed by path id
 // (pixel * spp + sample); queues are compacted each bounce.
@@ -23,6 +27,7 @@ struct MptArgs {
     uint32_t computeDirect, importance, nee, lightKind;
     uint32_t nPaths;
     uint32_t separateShadow;  // 1: visibility rays in launches of their own (else shared with closest-hit)
+                              // 0 also compacts each bounce's hits: hitQ[k] / hitIdx[k], count in counters[kHitCount + depth]
     float4* thp;     // thp.xyz, pdf of the last scatter
     float4* prevN;   // shading normal of the last scatter, flags (bit 0: delta lobe)
     uint4* rng;      // xoshiro128** state
@@ -31,6 +36,7 @@ struct MptArgs {
     float4* rayQ[2]; // 2 float4 per ray: (o, tmin) (d, tmax)
     uint32_t* pathQ[2];
     uint4* hitQ;     // (tri, t, b1, b2)
+    uint32_t* hitIdx;  // ray-queue index of each compacted hit
     float4* shQ;     // 3 float4 per shadow ray: (o, pathId) (d, tmax) (contrib, flags)
     uint4* pixHit;   // primary hits per pixel
     uint32_t* counters;   // [0, 64) queue counters, [64] frame number (read by the kernels so a
@@ -43,7 +49,6 @@ struct TraceScratch {
     uint32_t gridBlocks;  // persistent grid size in blocks
     uint32_t block;       // threads per block (64, 128, 256)
     uint32_t lds;         // LDS stack entries per lane (16, 24, 32)
-    uint32_t refill;      // 1: a lane starts its next ray as soon as its current one finishes
 };
 
 // Calls fn(integral_constant<block>, integral_constant<lds>) for the configuration in ts.

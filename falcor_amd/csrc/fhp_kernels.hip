@@ -2,7 +2,8 @@
 //
 // Frame structure (one launch sequence per frame, fixed grid shapes so it can be captured):
 //   primary rays (per pixel) -> closest hit -> shadePrimary (per path: emission, NEE, scatter)
-//   -> shadow rays -> for depth 0..maxBounces: closest hit -> shade -> shadow rays -> resolve.
+//   -> for depth 0..maxBounces: [visibility rays of the previous shading + closest hit, hits
+//   compacted] -> shade -> resolve.
 // Queues are compacted with a wave ballot + one atomic per wave; per-path state lives in
 // path-indexed slots so queue order never changes results.
 #include "fhp_internal.h"
@@ -200,19 +201,23 @@ __global__ void kShadePrimary(SceneDev s, MptArgs a) {
 }
 
 // handleHit for scatter depth `depth` (MinimalPathTracer.rt.slang:324-376), one thread per queued ray.
+// Runs on the compacted hits of the bounce (entry k: hit hitQ[k] of ray hitIdx[k]) unless
+// visibility rays have launches of their own, then on every queued ray (hitQ[i] of ray i).
 __global__ void kShade(SceneDev s, MptArgs a, uint32_t depth) {
     const int qi = depth & 1, qo = qi ^ 1;
-    const uint32_t n = a.counters[depth];
+    const bool compact = !a.separateShadow;
+    const uint32_t n = compact ? a.counters[kHitCount + depth] : a.counters[depth];
     const uint32_t G = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += G) {
-        const uint32_t i = base + threadIdx.x;
+        const uint32_t k = base + threadIdx.x;
         bool cont = false, shadow = false;
         f3 newO, newD, shO, shD, shC;
         float shT = 0.f;
         uint32_t pathId = 0;
-        if (i < n) {
+        if (k < n) {
+            const uint32_t i = compact ? a.hitIdx[k] : k;
             pathId = a.pathQ[qi][i];
-            const uint4 h = a.hitQ[i];
+            const uint4 h = a.hitQ[k];
             if (h.x != kInvalid) {
                 const float4 ro = a.rayQ[qi][2 * size_t(i)], rd = a.rayQ[qi][2 * size_t(i) + 1];
                 const f3 o = mk3(ro.x, ro.y, ro.z), d = mk3(rd.x, rd.y, rd.z);
@@ -295,75 +300,55 @@ __global__ __launch_bounds__(B) void kShadow(SceneDev s, MptArgs a, uint32_t slo
 // boundary. Closest hits go to hits[j], unoccluded visibility rays add their contribution to
 // their path.
 //
-// REFILL selects the lane schedule: false traces each lane's rays one after the other to
-// completion (a wave waits for its slowest ray before any lane starts the next), true starts a
-// lane's next ray as soon as its current one is finished, so lanes stay busy while the wave's
-// other rays are still in flight. Both give identical results.
-template <int B, int D, bool COUNT, bool REFILL>
+// Closest hits go to hits[j] for every ray j when hitCount is null (primary rays, one per
+// pixel); otherwise only rays that hit something are appended (hits[k], hitIdx[k] = j, count at
+// *hitCount), so the shading launch runs on dense lanes. A miss needs no shading: the scenes have
+// no environment light (MinimalPathTracer.rt.slang:378-390 adds nothing for a black background).
+template <int B, int D, bool COUNT>
 __global__ __launch_bounds__(B) void kTraceBounce(SceneDev s, MptArgs a, const float4* __restrict__ rays,
                                                   const uint32_t* nCPtr, uint32_t nCConst, uint4* __restrict__ hits,
-                                                  const uint32_t* nSPtr, TraceScratch ts, unsigned long long* stats) {
+                                                  uint32_t* __restrict__ hitIdx, uint32_t* hitCount, const uint32_t* nSPtr,
+                                                  TraceScratch ts, unsigned long long* stats) {
     __shared__ uint32_t lds[D * B];
     TraceStack st = makeStack<B, D>(lds, ts);
     const uint32_t nS = nSPtr ? *nSPtr : 0u;
     const uint32_t n = nS + (nCPtr ? *nCPtr : nCConst);
     const uint32_t G = gridDim.x * B;
     TraceCounters cc{0, 0}, cs{0, 0};
-    TravState r;
-    // Ray i: visibility record i (i < nS) or closest-hit ray i - nS.
-    auto startRay = [&](uint32_t i) {
+    for (uint32_t i = blockIdx.x * B + threadIdx.x; i < n; i += G) {
+        // Ray i: visibility record i (i < nS) or closest-hit ray i - nS.
         const bool vis = i < nS;
-        const float4* q = vis ? a.shQ + 3 * size_t(i) : rays + 2 * size_t(i - nS);
+        const uint32_t j = vis ? i : i - nS;
+        const float4* q = vis ? a.shQ + 3 * size_t(j) : rays + 2 * size_t(j);
         const float4 r0 = q[0], r1 = q[1];
+        TravState r;
         r.start(mk3(r0.x, r0.y, r0.z), mk3(r1.x, r1.y, r1.z), vis ? 0.f : r0.w, r1.w, vis);
-    };
-    auto finishRay = [&](uint32_t i) {
+        TraceCounters c{0, 0};
+        while (r.step<COUNT>(s, st, &c)) {
+        }
+        if (COUNT) {
+            TraceCounters& t = vis ? cs : cc;
+            t.nodes += c.nodes;
+            t.tris += c.tris;
+        }
         const bool hit = r.hit.tri != kInvalid;
-        if (i < nS) {
-            if (!hit) {
-                const float4* q = a.shQ + 3 * size_t(i);
-                const float4 r2 = q[2];
-                const uint32_t pathId = f2u(q[0].w);
-                float4* L = (f2u(r2.w) & 1u) ? &a.Lp[pathId] : &a.Ls[pathId];
-                const float4 v = *L;
-                *L = make_float4(v.x + r2.x, v.y + r2.y, v.z + r2.z, 0.f);
-            }
+        if (vis && !hit) {
+            const float4 r2 = q[2];
+            const uint32_t pathId = f2u(r0.w);
+            float4* L = (f2u(r2.w) & 1u) ? &a.Lp[pathId] : &a.Ls[pathId];
+            const float4 v = *L;
+            *L = make_float4(v.x + r2.x, v.y + r2.y, v.z + r2.z, 0.f);
+        }
+        const HitRec& h = r.hit;
+        if (!hitCount) {
+            if (!vis) hits[j] = hit ? make_uint4(h.tri, f2u(h.t), f2u(h.b1), f2u(h.b2)) : make_uint4(kInvalid, 0u, 0u, 0u);
         } else {
-            const HitRec& h = r.hit;
-            hits[i - nS] = hit ? make_uint4(h.tri, f2u(h.t), f2u(h.b1), f2u(h.b2)) : make_uint4(kInvalid, 0u, 0u, 0u);
-        }
-    };
-    uint32_t i = blockIdx.x * B + threadIdx.x;
-    if (REFILL) {
-        bool have = i < n;
-        if (have) startRay(i);
-        while (have) {
-            TraceCounters c{0, 0};
-            const bool live = r.step<COUNT>(s, st, &c);
-            if (COUNT) {
-                TraceCounters& t = i < nS ? cs : cc;
-                t.nodes += c.nodes;
-                t.tris += c.tris;
+            const bool keep = !vis && hit;
+            const uint32_t k = waveAppend(keep, hitCount);
+            if (keep) {
+                hits[k] = make_uint4(h.tri, f2u(h.t), f2u(h.b1), f2u(h.b2));
+                hitIdx[k] = j;
             }
-            if (!live) {
-                finishRay(i);
-                i += G;
-                have = i < n;
-                if (have) startRay(i);
-            }
-        }
-    } else {
-        for (; i < n; i += G) {
-            startRay(i);
-            TraceCounters c{0, 0};
-            while (r.step<COUNT>(s, st, &c)) {
-            }
-            if (COUNT) {
-                TraceCounters& t = i < nS ? cs : cc;
-                t.nodes += c.nodes;
-                t.tris += c.tris;
-            }
-            finishRay(i);
         }
     }
     if (COUNT) {
@@ -431,6 +416,7 @@ hipError_t launchMptBounces(const SceneDev& s, const MptArgs& a, const TraceScra
     const uint32_t npix = a.W * a.H;
     const int T = 256;
     hipLaunchKernelGGL(kZeroU32, dim3(1), dim3(2 * kCounterStride), 0, st, a.counters, 2u * kCounterStride);
+    hipLaunchKernelGGL(kZeroU32, dim3(1), dim3(kCounterStride), 0, st, a.counters + kHitCount, uint32_t(kCounterStride));
     hipLaunchKernelGGL(kPrimaryRays, dim3((npix + T - 1) / T), dim3(T), 0, st, s, a);
     unsigned long long* tstat = count ? d_counts + 2 : nullptr;   // closest-hit nodes, triangles
     unsigned long long* sstat = count ? d_counts + 4 : nullptr;   // shadow nodes, triangles
@@ -458,24 +444,18 @@ hipError_t launchMptBounces(const SceneDev& s, const MptArgs& a, const TraceScra
         tm->end(st);
         return e;
     };
-    auto bounce = [&](const float4* rays, const uint32_t* nC, uint32_t nCConst, uint4* hits, const uint32_t* nS,
-                      uint32_t maxBlocks) {
+    auto bounce = [&](const float4* rays, const uint32_t* nC, uint32_t nCConst, uint4* hits, uint32_t* hitIdx,
+                      uint32_t* hitCount, const uint32_t* nS, uint32_t maxBlocks) {
         tm->begin(KK_TRACE, st);
         hipError_t e = withTraceConfig(tsc, [&](auto Bc, auto Dc) {
             constexpr int B = decltype(Bc)::value, D = decltype(Dc)::value;
             const uint32_t blocks = std::min<uint32_t>(tsc.gridBlocks, maxBlocks);
-            if (count && tsc.refill)
-                hipLaunchKernelGGL((kTraceBounce<B, D, true, true>), dim3(blocks), dim3(B), 0, st, s, a, rays, nC, nCConst, hits,
-                                   nS, tsc, tstat);
-            else if (count)
-                hipLaunchKernelGGL((kTraceBounce<B, D, true, false>), dim3(blocks), dim3(B), 0, st, s, a, rays, nC, nCConst,
-                                   hits, nS, tsc, tstat);
-            else if (tsc.refill)
-                hipLaunchKernelGGL((kTraceBounce<B, D, false, true>), dim3(blocks), dim3(B), 0, st, s, a, rays, nC, nCConst,
-                                   hits, nS, tsc, tstat);
+            if (count)
+                hipLaunchKernelGGL((kTraceBounce<B, D, true>), dim3(blocks), dim3(B), 0, st, s, a, rays, nC, nCConst, hits,
+                                   hitIdx, hitCount, nS, tsc, tstat);
             else
-                hipLaunchKernelGGL((kTraceBounce<B, D, false, false>), dim3(blocks), dim3(B), 0, st, s, a, rays, nC, nCConst,
-                                   hits, nS, tsc, tstat);
+                hipLaunchKernelGGL((kTraceBounce<B, D, false>), dim3(blocks), dim3(B), 0, st, s, a, rays, nC, nCConst, hits,
+                                   hitIdx, hitCount, nS, tsc, tstat);
             return hipGetLastError();
         });
         tm->end(st);
@@ -483,7 +463,8 @@ hipError_t launchMptBounces(const SceneDev& s, const MptArgs& a, const TraceScra
     };
     // Primary rays: one per pixel (the reference's V-buffer), shared by the pixel's samples.
     hipError_t e = a.separateShadow ? traceQ(a.rayQ[0], nullptr, npix, a.pixHit, (npix + tsc.block - 1) / tsc.block)
-                                    : bounce(a.rayQ[0], nullptr, npix, a.pixHit, nullptr, (npix + tsc.block - 1) / tsc.block);
+                                    : bounce(a.rayQ[0], nullptr, npix, a.pixHit, nullptr, nullptr, nullptr,
+                                             (npix + tsc.block - 1) / tsc.block);
     if (e != hipSuccess) return e;
     tm->begin(KK_SHADE, st);
     hipLaunchKernelGGL(kShadePrimary, dim3((a.nPaths + T - 1) / T), dim3(T), 0, st, s, a);
@@ -492,7 +473,7 @@ hipError_t launchMptBounces(const SceneDev& s, const MptArgs& a, const TraceScra
     for (uint32_t b = 0; b <= a.maxBounces; b++) {
         // Closest-hit rays of bounce b with the visibility rays produced alongside them.
         if ((e = (a.separateShadow ? traceQ(a.rayQ[b & 1], a.counters + b, 0u, a.hitQ, 0xffffffffu)
-                                   : bounce(a.rayQ[b & 1], a.counters + b, 0u, a.hitQ,
+                                   : bounce(a.rayQ[b & 1], a.counters + b, 0u, a.hitQ, a.hitIdx, a.counters + kHitCount + b,
                                             a.nee ? a.counters + kCounterStride + b : nullptr, 0xffffffffu))) != hipSuccess)
             return e;
         if (a.separateShadow && a.nee && (e = shadow(b)) != hipSuccess) return e;

@@ -78,12 +78,11 @@ def test_traversal_launch_properties_are_validated():
     assert lib.fhp_pass_set_property(h, b"traceBlocksPerCU", 17.0) == -1   # 17 * 128 > 2048 threads per CU
     assert lib.fhp_pass_set_property(h, b"traceBlocksPerCU", 16.0) == 0
     assert lib.fhp_pass_set_property(h, b"traceBlockSize", 256.0) == -1    # 16 * 256 > 2048
-    for cfg in autotune.candidates(refill=(0, 1)):
+    for cfg in autotune.candidates():
         assert lib.fhp_pass_set_property(h, b"traceBlocksPerCU", 1.0) == 0
         assert lib.fhp_pass_set_property(h, b"traceBlockSize", float(cfg["traceBlockSize"])) == 0
         assert lib.fhp_pass_set_property(h, b"traceStackDepth", float(cfg["traceStackDepth"])) == 0
         assert lib.fhp_pass_set_property(h, b"traceBlocksPerCU", float(cfg["traceBlocksPerCU"])) == 0
-        assert lib.fhp_pass_set_property(h, b"traceRefill", float(cfg["traceRefill"])) == 0
     lib.fhp_pass_destroy(h)
     keys = (C.c_char_p * 2)(b"traceBlockSize", b"traceBlocksPerCU")
     vals = (C.c_double * 2)(64.0, 40.0)

@@ -136,11 +136,11 @@ def test_launch_configurations_give_identical_results(gpu):
     pw.set_scene(sc)
     ref_m = pm.render(64, 48, spp=2, frame=3).clone()
     ref_w = pw.render(64, 48, spp=2, frame=3).clone()
-    configs = autotune.candidates(threads_per_cu=(256, 2048), refill=(0, 1))
+    configs = autotune.candidates(threads_per_cu=(256, 2048))
     for cfg in configs:
         print(f"launch {cfg}", flush=True)
         autotune.apply(pm, cfg)
-        autotune.apply(pw, {k: v for k, v in cfg.items() if k != "traceRefill"})
+        autotune.apply(pw, cfg)
         assert torch.equal(pm.render(64, 48, spp=2, frame=3), ref_m), cfg
         assert torch.equal(pw.render(64, 48, spp=2, frame=3), ref_w), cfg
     # Visibility rays in launches of their own instead of sharing the closest-hit launch.
