@@ -25,7 +25,8 @@ __device__ __forceinline__ uint32_t waveAppend(bool pred, uint32_t* counter) {
 
 template <int B, int D> __device__ __forceinline__ TraceStack makeStack(uint32_t* lds, const TraceScratch& ts) {
     const uint32_t gtid = blockIdx.x * B + threadIdx.x;
-    return TraceStack{lds + threadIdx.x, uint32_t(B), uint32_t(D), ts.ovf + gtid, ts.gridBlocks * B, ts.ovfDepth};
+    return TraceStack{(LdsU32*)(lds + threadIdx.x), uint32_t(B), uint32_t(D), (GlobalU32*)(ts.ovf + gtid), ts.gridBlocks * B,
+                      ts.ovfDepth};
 }
 
 int numCUs() {

@@ -18,11 +18,17 @@ struct HitRec {
     float t, b1, b2;
 };
 
+// Address-space-qualified pointers keep stack traffic on ds_read/ds_write (LDS) and
+// global_load/global_store (overflow); through generic pointers the compiler merges the two
+// paths into flat accesses, which are slower for LDS.
+typedef __attribute__((address_space(3))) uint32_t LdsU32;
+typedef __attribute__((address_space(1))) uint32_t GlobalU32;
+
 struct TraceStack {
-    uint32_t* lds;       // this lane's slot 0; entry i at lds[i * ldsStride]
+    LdsU32* lds;         // this lane's slot 0; entry i at lds[i * ldsStride]
     uint32_t ldsStride;  // threads per block
     uint32_t ldsDepth;
-    uint32_t* ovf;       // this thread's overflow slot 0; entry i at ovf[i * ovfStride]
+    GlobalU32* ovf;      // this thread's overflow slot 0; entry i at ovf[i * ovfStride]
     uint32_t ovfStride;
     uint32_t ovfDepth;
     __device__ __forceinline__ void put(uint32_t i, uint32_t v) {

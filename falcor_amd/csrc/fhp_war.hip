@@ -288,8 +288,8 @@ __device__ f3 warPath(const SceneDev& s, const WarArgs& a, uint32_t px, uint32_t
 template <int B, int D, bool ADJ>
 __global__ __launch_bounds__(B) void kWar(SceneDev s, WarArgs a, TraceScratch ts) {
     __shared__ uint32_t lds[D * B];
-    TraceStack st{lds + threadIdx.x, uint32_t(B), uint32_t(D), ts.ovf + blockIdx.x * B + threadIdx.x, ts.gridBlocks * B,
-                  ts.ovfDepth};
+    TraceStack st{(LdsU32*)(lds + threadIdx.x), uint32_t(B), uint32_t(D), (GlobalU32*)(ts.ovf + blockIdx.x * B + threadIdx.x),
+                  ts.gridBlocks * B, ts.ovfDepth};
     const uint32_t n = a.W * a.H * a.spp;
     const uint32_t G = gridDim.x * B;
     unsigned long long nc = 0, nv = 0;
