@@ -37,6 +37,8 @@ class SceneDesc(C.Structure):
         ("num_texels_floats", C.c_uint64),
         ("h_texels", C.c_void_p),
         ("h_camera", C.c_void_p),
+        ("num_meshes", C.c_uint32),
+        ("h_mesh_vertex_offsets", C.c_void_p),
     ]
 
 
@@ -45,7 +47,8 @@ class SceneInfo(C.Structure):
         ("num_triangles", C.c_uint32), ("num_vertices", C.c_uint32), ("num_materials", C.c_uint32),
         ("num_textures", C.c_uint32), ("bvh_nodes", C.c_uint32), ("bvh_leaves", C.c_uint32),
         ("bvh_depth", C.c_uint32), ("stack_need", C.c_uint32), ("num_lights", C.c_uint32),
-        ("num_active_lights", C.c_uint32), ("light_bvh_nodes", C.c_uint32), ("bvh_sah_cost", C.c_double),
+        ("num_active_lights", C.c_uint32), ("light_bvh_nodes", C.c_uint32), ("num_meshes", C.c_uint32),
+        ("bvh_sah_cost", C.c_double),
         ("device_bytes", C.c_uint64), ("camera_basis", C.c_float * 9),
     ]
 
@@ -58,6 +61,7 @@ _SIGS = {
     "fhp_scene_create": (C.c_int, [C.POINTER(SceneDesc), C.POINTER(C.c_void_p)]),
     "fhp_scene_destroy": (C.c_int, [C.c_void_p]),
     "fhp_scene_info": (C.c_int, [C.c_void_p, C.POINTER(SceneInfo)]),
+    "fhp_scene_update_vertices": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
     "fhp_scene_set_materials": (C.c_int, [C.c_void_p, C.c_void_p]),
     "fhp_scene_set_texels": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
     "fhp_scene_freeze_sampling": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
@@ -78,6 +82,7 @@ _SIGS = {
     "fhp_grads_aggregate": (C.c_int, [C.c_void_p, C.c_void_p]),
     "fhp_grads_get": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_uint64), C.POINTER(C.c_void_p),
                                 C.POINTER(C.c_uint64)]),
+    "fhp_grads_set_mesh": (C.c_int, [C.c_void_p, C.c_uint32]),
     "fhp_grads_copy": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_uint64, C.c_void_p]),
     "fhp_pass_backward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p,
                                     C.c_void_p, C.c_void_p, C.c_void_p]),

@@ -101,6 +101,10 @@ struct FhpGrads_ {
     uint32_t hashSize = 1, dim = 0;
     DevBuf tmp, agg, tex;
     bool withTex = false;
+    // MeshPosition / MeshNormal gradients of one mesh: hashSize rows of 3 * meshVerts floats each.
+    bool withMesh = false;
+    uint32_t mesh = 0, meshV0 = 0, meshVerts = 0;
+    DevBuf posTmp, nrmTmp, posAgg, nrmAgg;
 };
 
 namespace {
@@ -198,6 +202,83 @@ int collectTimer(FhpPass p, hipStream_t st) {
     return FHP_OK;
 }
 
+// Builds the BVH, light collection and light BVH of s->h and uploads everything the kernels
+// read. Called at creation and after vertex updates (buffers are reused when large enough; a
+// frozen sampling copy of the materials stays in place).
+int buildAndUpload(FhpScene_* s) {
+    HostScene& h = s->h;
+    try {
+        buildBVH4(h, 4);
+        buildLights(h);
+        buildLightBVH(h);
+    } catch (const std::exception& e) {
+        return fail(FHP_EBUILD, e.what());
+    }
+    // Vertex records: (p.xyz, n.x) (n.yz, uv).
+    std::vector<float> vtx(8 * size_t(h.nV));
+    for (uint32_t i = 0; i < h.nV; i++) {
+        float* q = &vtx[8 * size_t(i)];
+        q[0] = h.pos[3 * i]; q[1] = h.pos[3 * i + 1]; q[2] = h.pos[3 * i + 2]; q[3] = h.nrm[3 * i];
+        q[4] = h.nrm[3 * i + 1]; q[5] = h.nrm[3 * i + 2]; q[6] = h.uv[2 * i]; q[7] = h.uv[2 * i + 1];
+    }
+    HIP_OR_FAIL(s->bvh.upload(h.bvh));
+    HIP_OR_FAIL(s->triPos.upload(h.triPos));
+    HIP_OR_FAIL(s->idx.upload(h.idx));
+    HIP_OR_FAIL(s->vtx.upload(vtx));
+    HIP_OR_FAIL(s->triMat.upload(h.triMat));
+    HIP_OR_FAIL(s->mats.upload(h.mats));
+    HIP_OR_FAIL(s->texels.upload(h.texels));
+    HIP_OR_FAIL(s->texDesc.upload(h.texDesc));
+    HIP_OR_FAIL(s->lights.upload(h.lights));
+    HIP_OR_FAIL(s->active.upload(h.activeLights));
+    HIP_OR_FAIL(s->triToLight.upload(h.triToLight));
+    HIP_OR_FAIL(s->lbvh.upload(h.lbvh));
+    HIP_OR_FAIL(s->lbvhTri.upload(h.lbvhTri));
+    HIP_OR_FAIL(s->lbvhMask.upload(h.lbvhMask));
+    // Overflow stack for the persistent traversal grid, sized for the smallest LDS stack and the
+    // largest grid any configuration launches.
+    s->stackNeed = h.stackNeed + 1;
+    const uint32_t worst = s->stackNeed > kMinLds ? s->stackNeed - kMinLds : 0u;
+    if (worst) HIP_OR_FAIL(s->ovf.alloc(size_t(worst) * uint32_t(numCUs()) * kMaxThreadsPerCU * 4));
+    s->ts = traceConfig(s, kDefaultBlock, kDefaultLds, kDefaultBlocksPerCU);
+
+    SceneDev& dv = s->dev;
+    const bool frozen = s->matsS.p != nullptr && dv.matsS == s->matsS.as<float>();
+    dv.bvh = s->bvh.as<float4>();
+    dv.triPos = s->triPos.as<float4>();
+    dv.idx = s->idx.as<uint32_t>();
+    dv.vtx = s->vtx.as<float4>();
+    dv.triMat = s->triMat.as<uint32_t>();
+    dv.mats = s->mats.as<float>();
+    dv.texels = s->texels.as<float>();
+    if (!frozen) {
+        dv.matsS = dv.mats;
+        dv.texelsS = dv.texels;
+    }
+    dv.texDesc = s->texDesc.as<int4>();
+    dv.lights = s->lights.as<float4>();
+    dv.activeLights = s->active.as<uint32_t>();
+    dv.triToLight = s->triToLight.as<uint32_t>();
+    dv.lbvh = s->lbvh.as<float4>();
+    dv.lbvhTri = s->lbvhTri.as<uint32_t>();
+    dv.lbvhMask = s->lbvhMask.as<uint2>();
+    dv.cam.pos = mk3(h.cam[0], h.cam[1], h.cam[2]);
+    dv.cam.U = mk3(h.camU[0], h.camU[1], h.camU[2]);
+    dv.cam.V = mk3(h.camV[0], h.camV[1], h.camV[2]);
+    dv.cam.W = mk3(h.camW[0], h.camW[1], h.camW[2]);
+    dv.cam.nearZ = h.cam[12];
+    dv.cam.farZ = h.cam[13];
+    dv.nTris = h.nT;
+    dv.nLights = h.nLights;
+    dv.nActive = uint32_t(h.activeLights.size());
+    dv.nLbvh = h.nLbvh;
+    s->deviceBytes = 0;
+    for (DevBuf* b : {&s->bvh, &s->triPos, &s->idx, &s->vtx, &s->triMat, &s->mats, &s->texels, &s->texDesc, &s->lights,
+                      &s->active, &s->triToLight, &s->lbvh, &s->lbvhTri, &s->lbvhMask, &s->ovf, &s->matsS, &s->texelsS})
+        s->deviceBytes += b->bytes;
+    return FHP_OK;
+}
+
 extern "C" {
 
 int fhp_get_version(void) { return 100; }
@@ -249,74 +330,36 @@ int fhp_scene_create(const FhpSceneDesc* d, FhpScene* out) {
         if (tex >= int(h.nTex)) return fail(FHP_EINVAL, "material texture index out of range");
     }
     std::memcpy(h.cam, d->h_camera, sizeof(h.cam));
+    if (d->num_meshes) {
+        if (!d->h_mesh_vertex_offsets) return fail(FHP_EINVAL, "mesh vertex offsets missing");
+        h.meshOff.assign(d->h_mesh_vertex_offsets, d->h_mesh_vertex_offsets + d->num_meshes + 1);
+        if (h.meshOff.front() != 0 || h.meshOff.back() != h.nV) return fail(FHP_EINVAL, "mesh vertex offsets must span [0, num_vertices]");
+        for (uint32_t m = 0; m < d->num_meshes; m++)
+            if (h.meshOff[m] > h.meshOff[m + 1]) return fail(FHP_EINVAL, "mesh vertex offsets must not decrease");
+    } else {
+        h.meshOff = {0u, h.nV};
+    }
     try {
         computeCamera(h);
-        buildBVH4(h, 4);
-        buildLights(h);
-        buildLightBVH(h);
     } catch (const std::exception& e) {
         return fail(FHP_EBUILD, e.what());
     }
-    // Vertex records: (p.xyz, n.x) (n.yz, uv).
-    std::vector<float> vtx(8 * size_t(h.nV));
-    for (uint32_t i = 0; i < h.nV; i++) {
-        float* q = &vtx[8 * size_t(i)];
-        q[0] = h.pos[3 * i]; q[1] = h.pos[3 * i + 1]; q[2] = h.pos[3 * i + 2]; q[3] = h.nrm[3 * i];
-        q[4] = h.nrm[3 * i + 1]; q[5] = h.nrm[3 * i + 2]; q[6] = h.uv[2 * i]; q[7] = h.uv[2 * i + 1];
-    }
-    HIP_OR_FAIL(s->bvh.upload(h.bvh));
-    HIP_OR_FAIL(s->triPos.upload(h.triPos));
-    HIP_OR_FAIL(s->idx.upload(h.idx));
-    HIP_OR_FAIL(s->vtx.upload(vtx));
-    HIP_OR_FAIL(s->triMat.upload(h.triMat));
-    HIP_OR_FAIL(s->mats.upload(h.mats));
-    HIP_OR_FAIL(s->texels.upload(h.texels));
-    HIP_OR_FAIL(s->texDesc.upload(h.texDesc));
-    HIP_OR_FAIL(s->lights.upload(h.lights));
-    HIP_OR_FAIL(s->active.upload(h.activeLights));
-    HIP_OR_FAIL(s->triToLight.upload(h.triToLight));
-    HIP_OR_FAIL(s->lbvh.upload(h.lbvh));
-    HIP_OR_FAIL(s->lbvhTri.upload(h.lbvhTri));
-    HIP_OR_FAIL(s->lbvhMask.upload(h.lbvhMask));
-    // Overflow stack for the persistent traversal grid.
-    // Sized for the smallest LDS stack and the largest grid any configuration launches.
-    s->stackNeed = h.stackNeed + 1;
-    const uint32_t worst = s->stackNeed > kMinLds ? s->stackNeed - kMinLds : 0u;
-    if (worst) HIP_OR_FAIL(s->ovf.alloc(size_t(worst) * uint32_t(numCUs()) * kMaxThreadsPerCU * 4));
-    s->ts = traceConfig(s.get(), kDefaultBlock, kDefaultLds, kDefaultBlocksPerCU);
-
-    SceneDev& dv = s->dev;
-    dv.bvh = s->bvh.as<float4>();
-    dv.triPos = s->triPos.as<float4>();
-    dv.idx = s->idx.as<uint32_t>();
-    dv.vtx = s->vtx.as<float4>();
-    dv.triMat = s->triMat.as<uint32_t>();
-    dv.mats = s->mats.as<float>();
-    dv.texels = s->texels.as<float>();
-    dv.matsS = dv.mats;
-    dv.texelsS = dv.texels;
-    dv.texDesc = s->texDesc.as<int4>();
-    dv.lights = s->lights.as<float4>();
-    dv.activeLights = s->active.as<uint32_t>();
-    dv.triToLight = s->triToLight.as<uint32_t>();
-    dv.lbvh = s->lbvh.as<float4>();
-    dv.lbvhTri = s->lbvhTri.as<uint32_t>();
-    dv.lbvhMask = s->lbvhMask.as<uint2>();
-    dv.cam.pos = mk3(h.cam[0], h.cam[1], h.cam[2]);
-    dv.cam.U = mk3(h.camU[0], h.camU[1], h.camU[2]);
-    dv.cam.V = mk3(h.camV[0], h.camV[1], h.camV[2]);
-    dv.cam.W = mk3(h.camW[0], h.camW[1], h.camW[2]);
-    dv.cam.nearZ = h.cam[12];
-    dv.cam.farZ = h.cam[13];
-    dv.nTris = h.nT;
-    dv.nLights = h.nLights;
-    dv.nActive = uint32_t(h.activeLights.size());
-    dv.nLbvh = h.nLbvh;
-    for (DevBuf* b : {&s->bvh, &s->triPos, &s->idx, &s->vtx, &s->triMat, &s->mats, &s->texels, &s->texDesc, &s->lights,
-                      &s->active, &s->triToLight, &s->lbvh, &s->lbvhTri, &s->lbvhMask, &s->ovf})
-        s->deviceBytes += b->bytes;
+    int rc = buildAndUpload(s.get());
+    if (rc) return rc;
     *out = s.release();
     return FHP_OK;
+}
+
+int fhp_scene_update_vertices(FhpScene s, uint32_t mesh, const float* pos, const float* nrm) {
+    if (!s || !pos) return fail(FHP_EINVAL, "null argument");
+    HostScene& h = s->h;
+    if (mesh + 1 >= h.meshOff.size()) return fail(FHP_EINVAL, "mesh id out of range");
+    const uint32_t v0 = h.meshOff[mesh], v1 = h.meshOff[mesh + 1];
+    for (size_t i = 0; i < 3 * size_t(v1 - v0); i++)
+        if (!std::isfinite(pos[i]) || (nrm && !std::isfinite(nrm[i]))) return fail(FHP_EINVAL, "non-finite vertex data");
+    std::copy(pos, pos + 3 * size_t(v1 - v0), h.pos.begin() + 3 * size_t(v0));
+    if (nrm) std::copy(nrm, nrm + 3 * size_t(v1 - v0), h.nrm.begin() + 3 * size_t(v0));
+    return buildAndUpload(s);
 }
 
 int fhp_scene_destroy(FhpScene s) {
@@ -330,6 +373,7 @@ int fhp_scene_info(FhpScene s, FhpSceneInfo* o) {
     o->num_triangles = h.nT; o->num_vertices = h.nV; o->num_materials = h.nM; o->num_textures = h.nTex;
     o->bvh_nodes = h.bvhNodes; o->bvh_leaves = h.bvhLeaves; o->bvh_depth = h.bvhDepth; o->stack_need = h.stackNeed;
     o->num_lights = h.nLights; o->num_active_lights = uint32_t(h.activeLights.size()); o->light_bvh_nodes = h.nLbvh;
+    o->num_meshes = uint32_t(h.meshOff.size() - 1);
     o->bvh_sah_cost = h.bvhSah;
     o->device_bytes = s->deviceBytes;
     for (int i = 0; i < 3; i++) {
@@ -575,12 +619,40 @@ int fhp_grads_clear(FhpGrads g, FhpStream stream) {
     HIP_OR_FAIL(launchFill(g->tmp.as<float>(), uint64_t(g->dim) * g->hashSize, 0.f, st));
     HIP_OR_FAIL(launchFill(g->agg.as<float>(), g->dim, 0.f, st));
     if (g->withTex) HIP_OR_FAIL(launchFill(g->tex.as<float>(), g->scene->h.texels.size(), 0.f, st));
+    if (g->withMesh) {
+        const uint64_t n = 3ull * g->meshVerts;
+        HIP_OR_FAIL(launchFill(g->posTmp.as<float>(), n * g->hashSize, 0.f, st));
+        HIP_OR_FAIL(launchFill(g->nrmTmp.as<float>(), n * g->hashSize, 0.f, st));
+        HIP_OR_FAIL(launchFill(g->posAgg.as<float>(), n, 0.f, st));
+        HIP_OR_FAIL(launchFill(g->nrmAgg.as<float>(), n, 0.f, st));
+    }
     return FHP_OK;
 }
 int fhp_grads_aggregate(FhpGrads g, FhpStream stream) {
     if (!g) return fail(FHP_EINVAL, "null argument");
     HIP_OR_FAIL(launchAggregate(g->tmp.as<float>(), g->hashSize, g->dim, g->agg.as<float>(), hipStream_t(stream)));
+    if (g->withMesh) {
+        HIP_OR_FAIL(launchAggregate(g->posTmp.as<float>(), g->hashSize, 3 * g->meshVerts, g->posAgg.as<float>(),
+                                    hipStream_t(stream)));
+        HIP_OR_FAIL(launchAggregate(g->nrmTmp.as<float>(), g->hashSize, 3 * g->meshVerts, g->nrmAgg.as<float>(),
+                                    hipStream_t(stream)));
+    }
     return FHP_OK;
+}
+int fhp_grads_set_mesh(FhpGrads g, uint32_t mesh) {
+    if (!g) return fail(FHP_EINVAL, "null argument");
+    const HostScene& h = g->scene->h;
+    if (mesh + 1 >= h.meshOff.size()) return fail(FHP_EINVAL, "mesh id out of range");
+    g->mesh = mesh;
+    g->meshV0 = h.meshOff[mesh];
+    g->meshVerts = h.meshOff[mesh + 1] - h.meshOff[mesh];
+    const size_t n = 3ull * g->meshVerts * 4;
+    HIP_OR_FAIL(g->posTmp.alloc(n * g->hashSize));
+    HIP_OR_FAIL(g->nrmTmp.alloc(n * g->hashSize));
+    HIP_OR_FAIL(g->posAgg.alloc(n));
+    HIP_OR_FAIL(g->nrmAgg.alloc(n));
+    g->withMesh = true;
+    return fhp_grads_clear(g, nullptr);
 }
 int fhp_grads_get(FhpGrads g, float** dm, uint64_t* nm, float** dt, uint64_t* nt) {
     if (!g) return fail(FHP_EINVAL, "null argument");
@@ -593,9 +665,11 @@ int fhp_grads_get(FhpGrads g, float** dm, uint64_t* nm, float** dt, uint64_t* nt
 
 int fhp_grads_copy(FhpGrads g, int which, float* dst, uint64_t n, FhpStream stream) {
     if (!g || !dst) return fail(FHP_EINVAL, "null argument");
-    const void* src = which == 0 ? g->agg.p : g->tex.p;
-    const uint64_t have = which == 0 ? g->dim : (g->withTex ? g->scene->h.texels.size() : 0);
-    if (which != 0 && which != 1) return fail(FHP_EINVAL, "which must be 0 (material) or 1 (texture)");
+    if (which < 0 || which > 3) return fail(FHP_EINVAL, "which must be 0 (material), 1 (texture), 2 (mesh position) or 3 (mesh normal)");
+    const void* src = which == 0 ? g->agg.p : which == 1 ? g->tex.p : which == 2 ? g->posAgg.p : g->nrmAgg.p;
+    const uint64_t have = which == 0   ? g->dim
+                          : which == 1 ? (g->withTex ? g->scene->h.texels.size() : 0)
+                                       : (g->withMesh ? 3ull * g->meshVerts : 0);
     if (n != have || have == 0) return fail(FHP_EINVAL, "gradient size mismatch");
     HIP_OR_FAIL(hipMemcpyAsync(dst, src, n * 4, hipMemcpyDeviceToDevice, hipStream_t(stream)));
     return FHP_OK;
@@ -607,7 +681,16 @@ int fhp_pass_backward(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t sp
     if (p->kind != "WARDiffPathTracer") return fail(FHP_ESTATE, "backward needs a WARDiffPathTracer pass");
     if (g->scene != s) return fail(FHP_EINVAL, "gradients belong to another scene");
     if (W == 0 || H == 0 || spp == 0) return fail(FHP_EINVAL, "empty frame");
-    if (p->get("useWAR") != 0) return fail(FHP_EINVAL, "useWAR=1 (vertex gradients) is not available in this build");
+    const bool war = p->get("useWAR") != 0;
+    if (war) {
+        if (p->get("maxBounces") != 0)
+            return fail(FHP_EINVAL, "useWAR=1 (vertex gradients) supports maxBounces = 0 (direct illumination) only");
+        if (!g->withMesh) return fail(FHP_ESTATE, "useWAR=1 needs mesh gradients (fhp_grads_set_mesh)");
+        const double aux = p->get("auxSampleCount");
+        if (aux < 1 || aux > 64 || double(uint32_t(aux)) != aux) return fail(FHP_EINVAL, "auxSampleCount must be in [1, 64]");
+        if (p->get("useAntitheticSampling") != 0 && (uint32_t(aux) & 1u))
+            return fail(FHP_EINVAL, "auxSampleCount must be even with useAntitheticSampling");
+    }
     hipStream_t st = hipStream_t(stream);
     int rc = ensurePassBuffers(p, 1, 1, 1);
     if (rc) return rc;
@@ -629,6 +712,21 @@ int fhp_pass_backward(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t sp
     a.rays = h_stats ? dstats : nullptr;
     if ((rc = prepareTimer(p))) return rc;
     HIP_OR_FAIL(launchWarAdjoint(s->dev, a, passTrace(s, p), st, &p->timer));
+    if (war) {
+        GeomArgs ga{};
+        ga.W = W; ga.H = H; ga.spp = spp; ga.frame = a.frame;
+        ga.useBSDFSampling = a.useBSDFSampling; ga.useNEE = a.useNEE; ga.useMIS = a.useMIS;
+        ga.useAntithetic = a.useAntithetic;
+        ga.aux = uint32_t(p->get("auxSampleCount"));
+        ga.kappa = float(std::pow(10.0, p->get("Log10vMFConcentration")));
+        ga.kappaScreen = float(std::pow(10.0, p->get("Log10vMFConcentrationScreen")));
+        ga.dLdI = d_dLdI;
+        ga.gradPos = g->posTmp.as<float>();
+        ga.gradNrm = g->nrmTmp.as<float>();
+        ga.hashSize = g->hashSize; ga.meshV0 = g->meshV0; ga.meshVerts = g->meshVerts;
+        ga.rays = h_stats ? dstats + 2 : nullptr;
+        HIP_OR_FAIL(launchWarGeom(s->dev, ga, passTrace(s, p), st, &p->timer));
+    }
     if ((rc = collectTimer(p, st))) return rc;
     if (h_stats) {
         unsigned long long hs[6];

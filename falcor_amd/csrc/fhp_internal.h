@@ -124,6 +124,19 @@ hipError_t launchWarPrimal(const SceneDev& s, const WarArgs& a, const TraceScrat
                            KernelTimer* timer);
 hipError_t launchWarAdjoint(const SceneDev& s, const WarArgs& a, const TraceScratch& ts, hipStream_t stream,
                             KernelTimer* timer);
+// Geometry gradients of WARDiffPathTracer (warped-area reparameterization, maxBounces = 0).
+struct GeomArgs {
+    uint32_t W, H, spp, frame;
+    uint32_t useBSDFSampling, useNEE, useMIS, useAntithetic;
+    uint32_t aux;              // auxiliary rays per reparameterization
+    float kappa, kappaScreen;  // vMF concentrations (10^Log10vMFConcentration[Screen])
+    const float* dLdI;
+    float* gradPos;            // MeshPosition gradients [hashSize][3 * meshVerts]
+    float* gradNrm;            // MeshNormal gradients, same layout
+    uint32_t hashSize, meshV0, meshVerts;
+    unsigned long long* rays;  // closest-hit ray counter (primary, scatter and auxiliary rays), or null
+};
+hipError_t launchWarGeom(const SceneDev& s, const GeomArgs& a, const TraceScratch& ts, hipStream_t stream, KernelTimer* timer);
 hipError_t launchAggregate(const float* tmp, uint32_t hashSize, uint32_t dim, float* out, hipStream_t stream);
 hipError_t launchFill(float* p, uint64_t n, float v, hipStream_t stream);
 

@@ -13,6 +13,7 @@ struct HostScene {
     uint32_t nV = 0, nT = 0, nM = 0, nTex = 0;
     std::vector<float> pos, nrm, uv;  // 3, 3, 2 floats per vertex
     std::vector<uint32_t> idx, triMat;
+    std::vector<uint32_t> meshOff;  // mesh m owns vertices [meshOff[m], meshOff[m + 1])
     std::vector<float> mats;
     std::vector<int32_t> texDesc;
     std::vector<float> texels;

@@ -44,6 +44,10 @@ class Scene:
             tm=np.ascontiguousarray(data.tri_material, np.uint32), mats=np.ascontiguousarray(data.materials, np.float32),
             td=np.ascontiguousarray(data.tex_desc, np.int32), tx=np.ascontiguousarray(data.texels, np.float32),
             cam=np.ascontiguousarray(data.camera_array(), np.float32))
+        off = data.mesh_vertex_offsets
+        if off is None:
+            off = np.array([0, data.positions.shape[0]], np.uint32)
+        self._arrays["moff"] = np.ascontiguousarray(off, np.uint32)
         a = self._arrays
         d = _lib.SceneDesc()
         d.num_vertices = a["pos"].shape[0]
@@ -57,6 +61,8 @@ class Scene:
         d.num_texels_floats = a["tx"].size
         d.h_texels = a["tx"].ctypes.data if a["tx"].size else None
         d.h_camera = a["cam"].ctypes.data
+        d.num_meshes = a["moff"].shape[0] - 1
+        d.h_mesh_vertex_offsets = a["moff"].ctypes.data
         h = C.c_void_p()
         _lib.check(L.fhp_scene_create(C.byref(d), C.byref(h)))
         self.handle = h
@@ -90,6 +96,38 @@ class Scene:
             raise ValueError(f"expected materials of shape {self.materials.shape}")
         _lib.check(_lib.load().fhp_scene_set_materials(self.handle, m.ctypes.data))
         self.materials = m.copy()
+
+    # Scene::getMeshVerticesAndIndices / setMeshVertices (Scene.cpp:4314-4336)
+    def mesh_vertex_range(self, mesh_id: int) -> tuple[int, int]:
+        off = self._arrays["moff"]
+        if not 0 <= mesh_id < off.shape[0] - 1:
+            raise ValueError(f"mesh id {mesh_id} out of range")
+        return int(off[mesh_id]), int(off[mesh_id + 1])
+
+    def get_mesh_vertices_and_indices(self, mesh_id: int):
+        """(positions [V, 3], normals [V, 3], indices [T, 3] local to the mesh) as numpy arrays."""
+        v0, v1 = self.mesh_vertex_range(mesh_id)
+        idx = self._arrays["idx"]
+        sel = np.all((idx >= v0) & (idx < v1), axis=1)
+        return (self._arrays["pos"][v0:v1].copy(), self._arrays["nrm"][v0:v1].copy(), (idx[sel] - v0).astype(np.uint32))
+
+    def set_mesh_vertices(self, mesh_id: int, positions, normals=None):
+        """Replaces the mesh's vertex positions (and normals) and rebuilds the BVH and lights."""
+        v0, v1 = self.mesh_vertex_range(mesh_id)
+        def host(x):
+            if hasattr(x, "detach"):
+                x = x.detach().float().cpu().numpy()
+            x = np.ascontiguousarray(x, np.float32).reshape(-1, 3)
+            if x.shape[0] != v1 - v0:
+                raise ValueError(f"mesh {mesh_id} has {v1 - v0} vertices, got {x.shape[0]}")
+            return x
+        p = host(positions)
+        n = host(normals) if normals is not None else None
+        _lib.check(_lib.load().fhp_scene_update_vertices(self.handle, mesh_id, p.ctypes.data,
+                                                         n.ctypes.data if n is not None else None))
+        self._arrays["pos"][v0:v1] = p
+        if n is not None:
+            self._arrays["nrm"][v0:v1] = n
 
     def set_texels(self, texels):
         """Copies a device tensor of texels (same layout as SceneData.texels) into the scene."""
@@ -238,12 +276,18 @@ class WARDiffPathTracer(_NativePass):
         self.mesh_to_optimize = 0
 
     def set_mesh_to_optimize(self, mesh_id: int):
+        """WARDiffPathTracer::setMeshToOptimize (WARDiffPathTracer.cpp:697-715): vertex gradients
+        (useWAR) flow into this mesh only."""
         self.mesh_to_optimize = int(mesh_id)
 
     def backward(self, width: int, height: int, spp: int = 1, frame: int | None = None, stats: bool = False):
-        """Accumulates d(sum dL_dI . color)/d(params) into scene_gradients (raw sum over samples)."""
+        """Accumulates d(sum dL_dI . color)/d(params) into scene_gradients (raw sum over samples).
+        With useWAR, vertex position/normal gradients of the mesh set by set_mesh_to_optimize
+        are accumulated too (maxBounces = 0)."""
         if self.scene_gradients is None or self.dL_dI is None:
             raise _lib.FhpError("backward needs scene_gradients and dL_dI")
+        if self.get_property("useWAR") != 0 and self.scene_gradients.mesh_id != self.mesh_to_optimize:
+            self.scene_gradients.enable_mesh(self.mesh_to_optimize)
         torch = _torch()
         d = self.dL_dI.detach().to(device=_device(), dtype=torch.float32).contiguous()
         if d.numel() != width * height * 3:
@@ -281,18 +325,28 @@ class AccumulatePass(RenderPass):
 
 
 class SceneGradients:
-    """SceneGradients (Source/Falcor/DiffRendering/SceneGradients.cpp:42-206) for material and
-    per-texel baseColor gradients."""
+    """SceneGradients (Source/Falcor/DiffRendering/SceneGradients.cpp:42-206): material, per-texel
+    baseColor, and the MeshPosition / MeshNormal gradients of one mesh."""
 
     MATERIAL = "Material"
     TEXTURE = "Texture"
+    MESH_POSITION = "MeshPosition"
+    MESH_NORMAL = "MeshNormal"
 
-    def __init__(self, scene: Scene, hash_size: int = 256, texture_grads: bool = True):
+    def __init__(self, scene: Scene, hash_size: int = 256, texture_grads: bool = True, mesh_id: int | None = None):
         h = C.c_void_p()
         _lib.check(_lib.load().fhp_grads_create(scene.handle, hash_size, int(texture_grads), C.byref(h)))
         self.handle = h
         self.scene = scene
         self.hash_size = hash_size
+        self.mesh_id = None
+        if mesh_id is not None:
+            self.enable_mesh(mesh_id)
+
+    def enable_mesh(self, mesh_id: int):
+        """Allocates the MeshPosition / MeshNormal gradients of mesh_id (3 floats per vertex each)."""
+        _lib.check(_lib.load().fhp_grads_set_mesh(self.handle, int(mesh_id)))
+        self.mesh_id = int(mesh_id)
 
     def __del__(self):
         if getattr(self, "handle", None):
@@ -309,17 +363,27 @@ class SceneGradients:
         _lib.check(_lib.load().fhp_grads_aggregate(self.handle, _lib.stream_ptr()))
 
     def get_grad_types(self):
-        return [self.MATERIAL, self.TEXTURE]
+        types = [self.MATERIAL, self.TEXTURE]
+        if self.mesh_id is not None:
+            types += [self.MESH_POSITION, self.MESH_NORMAL]
+        return types
 
     def get_grads_buffer(self, grad_type: str):
-        """Device tensor view of the aggregated gradients (copied to a new tensor)."""
+        """Device tensor of the aggregated gradients (copied to a new tensor)."""
         torch = _torch()
-        dm, nm, dt, nt = C.c_void_p(), C.c_uint64(), C.c_void_p(), C.c_uint64()
-        _lib.check(_lib.load().fhp_grads_get(self.handle, C.byref(dm), C.byref(nm), C.byref(dt), C.byref(nt)))
-        n = nm.value if grad_type == self.MATERIAL else nt.value
+        if grad_type in (self.MESH_POSITION, self.MESH_NORMAL):
+            if self.mesh_id is None:
+                raise _lib.FhpError("mesh gradients are not enabled (enable_mesh / set_mesh_to_optimize)")
+            v0, v1 = self.scene.mesh_vertex_range(self.mesh_id)
+            n = 3 * (v1 - v0)
+            which = 2 if grad_type == self.MESH_POSITION else 3
+        else:
+            dm, nm, dt, nt = C.c_void_p(), C.c_uint64(), C.c_void_p(), C.c_uint64()
+            _lib.check(_lib.load().fhp_grads_get(self.handle, C.byref(dm), C.byref(nm), C.byref(dt), C.byref(nt)))
+            n = nm.value if grad_type == self.MATERIAL else nt.value
+            which = 0 if grad_type == self.MATERIAL else 1
         out = torch.empty(n, dtype=torch.float32, device=_device())
         if n:
-            which = 0 if grad_type == self.MATERIAL else 1
             _lib.check(_lib.load().fhp_grads_copy(self.handle, which, out.data_ptr(), n, _lib.stream_ptr()))
         return out
 

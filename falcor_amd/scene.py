@@ -241,6 +241,7 @@ class SceneData:
     tex_desc: np.ndarray = field(default_factory=lambda: np.zeros((0, 4), np.int32))
     texels: np.ndarray = field(default_factory=lambda: np.zeros(0, np.float32))
     mesh_names: list = field(default_factory=list)
+    mesh_vertex_offsets: np.ndarray | None = None  # mesh m owns vertices [off[m], off[m + 1])
 
     @property
     def num_triangles(self) -> int:
@@ -259,6 +260,7 @@ class SceneBuilder:
         self.textures: list[np.ndarray] = []
         self.mesh_names: list[str] = []
         self._nv = 0
+        self._mesh_off = [0]
         self.camera = Camera()
 
     def add_material(self, rec: np.ndarray) -> int:
@@ -291,6 +293,7 @@ class SceneBuilder:
         self._mat.append(mat.copy())
         self._mesh.append(np.full(idx.shape[0], mesh_id, np.uint32))
         self._nv += p.shape[0]
+        self._mesh_off.append(self._nv)
         return mesh_id
 
     def build(self) -> SceneData:
@@ -304,7 +307,8 @@ class SceneBuilder:
             positions=np.concatenate(self._pos), normals=np.concatenate(self._nrm), texcoords=np.concatenate(self._uv),
             indices=np.concatenate(self._idx), tri_material=np.concatenate(self._mat),
             tri_mesh=np.concatenate(self._mesh), materials=np.stack(self.materials), camera=self.camera,
-            tex_desc=tex_desc, texels=texels.astype(np.float32), mesh_names=list(self.mesh_names))
+            tex_desc=tex_desc, texels=texels.astype(np.float32), mesh_names=list(self.mesh_names),
+            mesh_vertex_offsets=np.asarray(self._mesh_off, np.uint32))
 
 
 # --------------------------------------------------------------------------- benchmark scenes
@@ -333,6 +337,57 @@ def closed_box(seed: int = 0, aspect: float = 1.0) -> SceneData:
         sb.add_mesh_instance(cube, m, t, name)
     light = sb.add_material(standard_material(base=(0, 0, 0), roughness=0.5, emissive=(17, 17, 17)))
     sb.add_mesh_instance(quad, light, Transform.deg((0.13, 0.13, 0.13), (0, 0.549, 0), (180, 0, 0)), "Light")
+    sb.camera = Camera((0, 0.28, 1.2), (0, 0.28, 0), (0, 1, 0), math.radians(40.0), aspect)
+    return sb.build()
+
+
+def vertex_normals(positions: np.ndarray, indices: np.ndarray) -> np.ndarray:
+    """Area-weighted vertex normals (degenerate triangles contribute nothing)."""
+    p = np.asarray(positions, np.float64)
+    idx = np.asarray(indices, np.int64)
+    fn = np.cross(p[idx[:, 1]] - p[idx[:, 0]], p[idx[:, 2]] - p[idx[:, 0]])
+    n = np.zeros_like(p)
+    for k in range(3):
+        np.add.at(n, idx[:, k], fn)
+    n /= np.maximum(np.linalg.norm(n, axis=1, keepdims=True), 1e-30)
+    return n.astype(np.float32)
+
+
+def displaced_sphere_in_box(seed: int = 0, radius: float = 0.12, segments=(100, 100), sigma: float = 0.05,
+                            center=(0.0, 0.2, 0.0), aspect: float = 1.0) -> SceneData:
+    """A Lambertian UV sphere (createSphere(r, 100, 100): 10,201 vertices, 20,000 triangles) inside
+    the closed box's walls and ceiling light. Vertices are displaced along the normal by a smooth
+    seeded field with standard deviation sigma * radius (sigma = 0 gives the plain sphere); vertex
+    normals are recomputed from the displaced surface. The sphere is the mesh named "Sphere"."""
+    rng = np.random.default_rng(seed)
+    sb = SceneBuilder()
+    quad = TriangleMesh.create_quad()
+    walls = [
+        ("Floor", Transform.deg((0.55, 1.0, 0.56))),
+        ("Ceiling", Transform.deg((0.55, 1.0, 0.56), (0, 0.55, 0), (180, 0, 0))),
+        ("Back Wall", Transform.deg((0.55, 1.0, 0.56), (0, 0.275, -0.275), (90, 0, 0))),
+        ("Left Wall", Transform.deg((0.55, 1.0, 0.56), (-0.275, 0.275, 0), (0, 0, -90))),
+        ("Right Wall", Transform.deg((0.55, 1.0, 0.56), (0.275, 0.275, 0), (0, 0, 90))),
+    ]
+    for name, t in walls:
+        m = sb.add_material(pbrt_diffuse_material(rng.uniform(0.1, 0.8, 3)))
+        sb.add_mesh_instance(quad, m, t, name)
+    light = sb.add_material(standard_material(base=(0, 0, 0), roughness=0.5, emissive=(17, 17, 17)))
+    sb.add_mesh_instance(quad, light, Transform.deg((0.13, 0.13, 0.13), (0, 0.549, 0), (180, 0, 0)), "Light")
+    sph = TriangleMesh.create_sphere(1.0, segments[0], segments[1])
+    n0 = sph.positions.astype(np.float64)
+    d = np.zeros(n0.shape[0])
+    if sigma > 0:
+        centers = rng.normal(size=(8, 3))
+        centers /= np.linalg.norm(centers, axis=1, keepdims=True)
+        amp = rng.normal(size=8)
+        for c, a in zip(centers, amp):
+            d += a * np.exp(6.0 * (n0 @ c - 1.0))
+        d = (d - d.mean()) / max(d.std(), 1e-12) * sigma * radius
+    pos = (n0 * (radius + d)[:, None] + np.asarray(center)).astype(np.float32)
+    mesh = TriangleMesh(pos, vertex_normals(pos, sph.indices), sph.texcoords, sph.indices)
+    m = sb.add_material(pbrt_diffuse_material(rng.uniform(0.3, 0.8, 3)))
+    sb.add_mesh_instance(mesh, m, None, "Sphere")
     sb.camera = Camera((0, 0.28, 1.2), (0, 0.28, 0), (0, 1, 0), math.radians(40.0), aspect)
     return sb.build()
 

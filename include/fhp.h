@@ -62,12 +62,16 @@ typedef struct FhpSceneDesc {
     uint64_t num_texels_floats;
     const float* h_texels;
     const float* h_camera; /* 14 floats */
+    /* Meshes (optional): mesh m owns vertices [offsets[m], offsets[m + 1]) and the triangles
+     * indexing them (SceneBuilder's mesh ids, Scene.cpp:4314-4336). 0 = one mesh. */
+    uint32_t num_meshes;
+    const uint32_t* h_mesh_vertex_offsets; /* num_meshes + 1 */
 } FhpSceneDesc;
 
 typedef struct FhpSceneInfo {
     uint32_t num_triangles, num_vertices, num_materials, num_textures;
     uint32_t bvh_nodes, bvh_leaves, bvh_depth, stack_need;
-    uint32_t num_lights, num_active_lights, light_bvh_nodes;
+    uint32_t num_lights, num_active_lights, light_bvh_nodes, num_meshes;
     double bvh_sah_cost;
     uint64_t device_bytes;
     float camera_basis[9]; /* U, V, W as computed by Camera.cpp:187-193 */
@@ -76,6 +80,11 @@ typedef struct FhpSceneInfo {
 int fhp_scene_create(const FhpSceneDesc* desc, FhpScene* out);
 int fhp_scene_destroy(FhpScene scene);
 int fhp_scene_info(FhpScene scene, FhpSceneInfo* out);
+/* Replace the vertex positions (and normals, if h_normals is not null) of mesh mesh_id from
+ * host memory (3 floats per vertex of the mesh), then rebuild the BVH, the light collection and
+ * the light BVH. Replaces Scene::setMeshVertices / set_mesh_vertices (Scene.cpp:4314-4336) and
+ * the BLAS rebuild it triggers. */
+int fhp_scene_update_vertices(FhpScene scene, uint32_t mesh_id, const float* h_positions, const float* h_normals);
 /* Replace all material records (host memory, 16 * num_materials floats).
  * Replaces Scene::setMaterialParams (Source/Falcor/Scene/Scene.cpp:4620-4643). */
 int fhp_scene_set_materials(FhpScene scene, const float* h_materials);
@@ -146,8 +155,14 @@ int fhp_grads_aggregate(FhpGrads grads, FhpStream stream);
  * texture gradients (NULL / 0 when disabled). */
 int fhp_grads_get(FhpGrads grads, float** d_material, uint64_t* material_floats, float** d_texture,
                   uint64_t* texture_floats);
-/* Copies the aggregated material gradients (which = 0) or the texture gradients (which = 1)
- * into d_dst (num_floats must equal the size reported by fhp_grads_get).
+/* Enables the MeshPosition and MeshNormal gradients of mesh mesh_id (3 floats per vertex of
+ * the mesh each, same hash size as the material gradients); written by fhp_pass_backward when
+ * the pass property useWAR is 1. Replaces GradConfig(MeshPosition / MeshNormal, dim = 3 *
+ * vertex count) of SceneGradients::create (SceneGradients.cpp:42-102, shape_example.py:92-111)
+ * together with WARDiffPathTracer::setMeshToOptimize (WARDiffPathTracer.cpp:697-715). */
+int fhp_grads_set_mesh(FhpGrads grads, uint32_t mesh_id);
+/* Copies aggregated gradients into d_dst: which = 0 material, 1 texture, 2 mesh position,
+ * 3 mesh normal (num_floats must equal the gradient's size).
  * Replaces SceneGradients::getGradsBuffer + Buffer::copy_to_torch (common.py:103-110). */
 int fhp_grads_copy(FhpGrads grads, int which, float* d_dst, uint64_t num_floats, FhpStream stream);
 

@@ -1,6 +1,8 @@
 """C-ABI checks that need no GPU: the library loads, exports every entry point declared in
 include/fhp.h, and reports argument errors with codes and messages (no device calls)."""
 import ctypes as C
+
+import numpy as np
 import re
 from pathlib import Path
 
@@ -88,6 +90,33 @@ def test_traversal_launch_properties_are_validated():
     vals = (C.c_double * 2)(64.0, 40.0)
     assert lib.fhp_pass_create(b"WARDiffPathTracer", 2, keys, vals, C.byref(h)) == -1
     assert "traceBlocksPerCU" in lib.fhp_last_error().decode()
+
+
+def test_scene_mesh_offsets_are_validated():
+    """Mesh vertex ranges must partition the vertices (checked before any device work)."""
+    from falcor_amd import _lib
+    lib = _lib.load()
+    pos = np.zeros((3, 3), np.float32)
+    nrm = np.tile(np.array([0, 0, 1], np.float32), (3, 1))
+    uv = np.zeros((3, 2), np.float32)
+    idx = np.array([0, 1, 2], np.uint32)
+    tm = np.zeros(1, np.uint32)
+    mats = np.zeros((1, 16), np.float32)
+    mats[0, 12] = -1
+    cam = np.zeros(14, np.float32)
+    d = _lib.SceneDesc()
+    d.num_vertices, d.num_triangles, d.num_materials = 3, 1, 1
+    d.h_positions, d.h_normals, d.h_texcoords = pos.ctypes.data, nrm.ctypes.data, uv.ctypes.data
+    d.h_indices, d.h_tri_material, d.h_materials, d.h_camera = idx.ctypes.data, tm.ctypes.data, mats.ctypes.data, cam.ctypes.data
+    h = C.c_void_p()
+    for off in ([0, 2], [1, 3], [0, 2, 1, 3]):
+        o = np.array(off, np.uint32)
+        d.num_meshes = len(off) - 1
+        d.h_mesh_vertex_offsets = o.ctypes.data
+        assert lib.fhp_scene_create(C.byref(d), C.byref(h)) == -1
+        assert "mesh vertex offsets" in lib.fhp_last_error().decode()
+    assert lib.fhp_scene_update_vertices(None, 0, pos.ctypes.data, None) == -1
+    assert lib.fhp_grads_set_mesh(None, 0) == -1
 
 
 def test_product_never_imports_the_oracle():
