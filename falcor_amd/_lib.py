@@ -101,6 +101,12 @@ def load():
     if _lib is None:
         if not _LIB_PATH.exists():
             raise FhpError(f"{_LIB_PATH} is missing: build it with `python -m falcor_amd.build_lib`")
+        # Load torch's HIP runtime first so libfhp binds to the same libamdhip64 instance as
+        # the tensors and streams it is handed (a second runtime copy would not see them).
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         lib = C.CDLL(str(_LIB_PATH))
         for name, (res, args) in _SIGS.items():
             fn = getattr(lib, name)

@@ -459,18 +459,21 @@ __device__ void geomPath(const SceneDev& s, const GeomArgs& a, uint32_t px, uint
                 }
                 float gci, gco, gcio;
                 const float phi = bsdfGeomGrad(sd, m, wo, dL * Li * misN, gci, gco, gcio);
-                const float lnd = dot(ls.normalW, -wo);
-                if (phi != 0.f && lnd > 0.f) {
+                if (phi != 0.f) {
                     QN = phi;
                     gNs = gNs + sd.V * gci + wo * gco;
                     gWi = gWi + sd.N * gci + wo * gcio;
-                    f3 gDir = sd.N * gco + sd.V * gcio + ls.normalW * (QN / lnd);
-                    const float gDist = 2.f * QN / dist;
+                    // Change of variables on the light: y = T(y') is the light point hit from x1 in
+                    // direction w0(y') + V, with w0 the (detached) sampled direction. The area
+                    // factors of T and of the geometry term cancel, leaving the integrand
+                    //   Le f(x1, N, w0 + V) detJ / p(w0),
+                    // so only the BSDF (through the warp and the shading normal) and detJ depend
+                    // on the parameters; V follows the light's edges and the occluders as seen from
+                    // the moving x1 (origin term).
+                    const f3 gF = sd.N * gco + sd.V * gcio;  // d phi / d direction
                     f3 gO, gOm;
-                    warpAdjoint(s, a, st, rayOriginOffset(x1, faceO), wo, rng, gDir, QN, true, row, gO, gOm, nc);
+                    warpAdjoint(s, a, st, rayOriginOffset(x1, faceO), wo, rng, gF, QN, false, row, gO, gOm, nc);
                     gX = gX + gO;
-                    const f3 gWo = gDir + gOm;
-                    gX = gX - (gWo - wo * dot(wo, gWo)) / dist - wo * gDist;
                 }
             }
         }

@@ -9,8 +9,13 @@ gradient on a rigid or uniform-scale deformation of the sphere, which leaves ver
 unchanged (so only MeshPosition gradients enter), against a weight image dLdI chosen so the
 derivative is far from zero.
 
-Tolerance: relative error <= 0.1 of the finite-difference derivative (both sides are Monte Carlo
-estimates; the adjoint uses 16 auxiliary rays per warp, vMF concentration 10^3 and 10^3 screen).
+The scene is a closed box (walls, ceiling light flush with the ceiling) around the sphere: the
+warp's harmonic weights locate discontinuities at silhouettes (grazing aux hits), so edges of
+open surfaces floating in empty space are outside the estimator's assumptions.
+
+Tolerance: relative error <= 0.1 of the finite-difference derivative. Both sides are Monte Carlo
+estimates, and the warp itself is a ratio estimate whose bias shrinks with more auxiliary rays and
+a higher vMF concentration: 32 auxiliary rays and concentration 10^4 keep it at a few percent.
 """
 import numpy as np
 import pytest
@@ -28,8 +33,8 @@ def _setup(props=None):
     data = S.displaced_sphere_in_box(0, radius=0.12, segments=(24, 12), sigma=0.05)
     sc = Scene(data)
     mesh = data.mesh_names.index("Sphere")
-    p = WARDiffPathTracer(dict({"maxBounces": 0, "useWAR": True, "Log10vMFConcentration": 3,
-                                "Log10vMFConcentrationScreen": 3}, **(props or {})))
+    p = WARDiffPathTracer(dict({"maxBounces": 0, "useWAR": True, "Log10vMFConcentration": 4,
+                                "Log10vMFConcentrationScreen": 4, "auxSampleCount": 32}, **(props or {})))
     p.set_scene(sc)
     return sc, p, mesh
 
@@ -56,7 +61,7 @@ def _loss(p, dLdI, spp, frames):
     return tot / len(frames)
 
 
-def _fd(sc, p, mesh, deform, dLdI, eps, spp=256, frames=range(8)):
+def _fd(sc, p, mesh, deform, dLdI, eps, spp=512, frames=range(8)):
     pos0, nrm0, _ = sc.get_mesh_vertices_and_indices(mesh)
     vals = []
     for s in (eps, -eps):
@@ -80,7 +85,7 @@ def _adjoint_grad(sc, p, mesh, dLdI, spp=256, frames=range(100, 108)):
             g.get_grads_buffer("MeshNormal").view(-1, 3).cpu().numpy().astype(np.float64) / (spp * len(frames)))
 
 
-@pytest.mark.parametrize("axis,weights", [(0, "x"), (1, "y")])
+@pytest.mark.parametrize("axis,weights", [(0, "x"), (1, "y"), (2, "ones")])
 def test_translation_gradient_matches_fd(gpu, axis, weights):
     sc, p, mesh = _setup()
     dLdI = _weights(weights)
