@@ -15,7 +15,7 @@ open surfaces floating in empty space are outside the estimator's assumptions.
 
 Tolerance: relative error <= 0.1 of the finite-difference derivative. Both sides are Monte Carlo
 estimates, and the warp itself is a ratio estimate whose bias shrinks with more auxiliary rays and
-a higher vMF concentration: 32 auxiliary rays and concentration 10^4 keep it at a few percent.
+a higher vMF concentration: 64 auxiliary rays and concentration 10^4 keep it at a few percent.
 """
 import numpy as np
 import pytest
@@ -34,7 +34,7 @@ def _setup(props=None):
     sc = Scene(data)
     mesh = data.mesh_names.index("Sphere")
     p = WARDiffPathTracer(dict({"maxBounces": 0, "useWAR": True, "Log10vMFConcentration": 4,
-                                "Log10vMFConcentrationScreen": 4, "auxSampleCount": 32}, **(props or {})))
+                                "Log10vMFConcentrationScreen": 4, "auxSampleCount": 64}, **(props or {})))
     p.set_scene(sc)
     return sc, p, mesh
 
