@@ -183,6 +183,10 @@ def random_rotation(rng: np.random.Generator) -> np.ndarray:
 
 
 # --------------------------------------------------------------------------- materials
+# Material record fields (16 floats; see include/fhp.h).
+M_TYPE, M_BASE, M_ROUGH, M_METAL, M_IOR, M_EMIT, M_EMIT_FACTOR, M_DOUBLE_SIDED, M_TEX = 0, 1, 4, 5, 6, 7, 10, 11, 12
+
+
 def standard_material(base=(1, 1, 1), roughness=0.0, metallic=0.0, ior=1.5, emissive=(0, 0, 0), emissive_factor=1.0,
                       double_sided=False, texture=-1) -> np.ndarray:
     """StandardMaterial record (defaults of BasicMaterialData.slang:52-57: baseColor 1, specular 0)."""
