@@ -34,23 +34,25 @@ def test_autograd_material_gradients_equal_direct_pass(gpu):
     assert float(params.grad[:, 0:3].abs().sum()) > 0
 
 
-def test_material_optimization_reduces_loss(gpu):
+def test_material_optimization_recovers_albedos(gpu):
+    """Adam on the walls' and boxes' albedos from grey toward a target image (the light fixed):
+    the albedo error falls (the image loss itself is dominated by Monte Carlo noise)."""
     import torch
     from falcor_amd import autograd as A
     from falcor_amd.render_graph import Scene
     data = S.closed_box(2)
     sc = Scene(data)
-    ref = A.DiffRenderer(sc, 32, 32, spp_primal=64, spp_grad=4, props={"maxBounces": 2}, hash_size=16)
+    ref = A.DiffRenderer(sc, 32, 32, spp_primal=256, spp_grad=4, props={"maxBounces": 2}, hash_size=16)
     true_p = torch.tensor(A.material_params(sc.materials), device="cuda")
     with torch.no_grad():
         target = ref(materials=true_p)
     init = true_p.clone()
     init[:7, 0:3] = 0.5  # walls and boxes start grey
     params = init.clone().requires_grad_(True)
-    r = A.DiffRenderer(sc, 32, 32, spp_primal=16, spp_grad=16, props={"maxBounces": 2}, hash_size=16, seed=100)
-    opt = torch.optim.Adam([params], lr=0.05)
-    losses = []
-    for it in range(12):
+    r = A.DiffRenderer(sc, 32, 32, spp_primal=32, spp_grad=32, props={"maxBounces": 2}, hash_size=16, seed=100)
+    opt = torch.optim.Adam([params], lr=0.03)
+    errs = []
+    for it in range(30):
         opt.zero_grad()
         loss = ((r(materials=params) - target) ** 2).mean()
         loss.backward()
@@ -58,9 +60,9 @@ def test_material_optimization_reduces_loss(gpu):
         opt.step()
         with torch.no_grad():
             params[:, 0:3].clamp_(0.0, 1.0)
-        losses.append(float(loss))
-    print("losses", [round(x, 6) for x in losses], flush=True)
-    assert np.mean(losses[-3:]) < 0.5 * np.mean(losses[:3])
+        errs.append(float((params[:7, 0:3] - true_p[:7, 0:3]).abs().mean()))
+    print("albedo error", [round(x, 4) for x in errs[::5]], errs[-1], flush=True)
+    assert errs[-1] < 0.6 * float((init[:7, 0:3] - true_p[:7, 0:3]).abs().mean())
 
 
 def test_autograd_vertex_gradients_equal_direct_pass(gpu):
