@@ -53,6 +53,15 @@ class SceneInfo(C.Structure):
     ]
 
 
+class View(C.Structure):
+    _fields_ = [
+        ("scene", C.c_void_p),
+        ("h_camera", C.c_void_p),
+        ("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32), ("frame", C.c_uint32),
+        ("d_color", C.c_void_p),
+    ]
+
+
 # name -> (restype, argtypes)
 _SIGS = {
     "fhp_get_version": (C.c_int, []),
@@ -75,6 +84,7 @@ _SIGS = {
     "fhp_pass_get_property": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_double)]),
     "fhp_pass_execute": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p,
                                    C.c_void_p, C.c_void_p]),
+    "fhp_pass_execute_batch": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(View), C.c_void_p, C.c_void_p]),
     "fhp_pass_kernel_times": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
     "fhp_grads_create": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int, C.POINTER(C.c_void_p)]),
     "fhp_grads_destroy": (C.c_int, [C.c_void_p]),

@@ -72,6 +72,7 @@ struct FhpPass_ {
     std::map<std::string, double> props;
     // Wavefront buffers (forward pass).
     DevBuf thp, prevN, rng, Lp, Ls, rayQ0, rayQ1, pathQ0, pathQ1, hitQ, hitIdx, shQ, pixHit, counters, stats;
+    DevBuf viewTab, sceneTab;  // batch mode tables
     uint32_t cap = 0, capPix = 0;
     KernelTimer timer;
     // Captured bounce loop of the forward pass (property useGraph), keyed by everything the
@@ -154,8 +155,12 @@ TraceScratch passTrace(const FhpScene_* s, const FhpPass_* p) {
     return t;
 }
 
+int ensurePassBuffersN(FhpPass p, uint64_t n, uint64_t npix);
 int ensurePassBuffers(FhpPass p, uint32_t W, uint32_t H, uint32_t spp) {
-    const uint64_t n = uint64_t(W) * H * spp, npix = uint64_t(W) * H;
+    return ensurePassBuffersN(p, uint64_t(W) * H * spp, uint64_t(W) * H);
+}
+// Wavefront buffers for n paths over npix pixels.
+int ensurePassBuffersN(FhpPass p, uint64_t n, uint64_t npix) {
     if (n > 0xffffffffull / 3) return fail(FHP_EINVAL, "frame too large");
     if (n <= p->cap && npix <= p->capPix) return FHP_OK;
     p->dropGraph();
@@ -529,6 +534,7 @@ int fhp_pass_execute(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t spp
         a.separateShadow = p->get("splitVisibilityLaunch") != 0;
         if (a.nee && !a.computeDirect) return fail(FHP_EINVAL, "useEmissiveNEE requires computeDirect");
         a.nPaths = W * H * spp;
+        a.nPix = W * H;
         a.thp = p->thp.as<float4>(); a.prevN = p->prevN.as<float4>(); a.rng = p->rng.as<uint4>();
         a.Lp = p->Lp.as<float4>(); a.Ls = p->Ls.as<float4>();
         a.rayQ[0] = p->rayQ0.as<float4>(); a.rayQ[1] = p->rayQ1.as<float4>();
@@ -590,6 +596,92 @@ int fhp_pass_execute(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t spp
         HIP_OR_FAIL(hipMemcpyAsync(hs, dstats, 48, hipMemcpyDeviceToHost, st));
         HIP_OR_FAIL(hipStreamSynchronize(st));
         for (int i = 0; i < 6; i++) h_stats[i] = hs[i];
+    }
+    return FHP_OK;
+}
+
+int fhp_pass_execute_batch(FhpPass p, uint32_t nv, const FhpView* views, uint64_t* h_stats, FhpStream stream) {
+    if (!p || (nv && !views)) return fail(FHP_EINVAL, "null argument");
+    if (p->kind != "MinimalPathTracer") return fail(FHP_EINVAL, "batched execution is available for MinimalPathTracer");
+    if (nv == 0) return FHP_OK;
+    hipStream_t st = hipStream_t(stream);
+    std::vector<ViewDev> vt(nv);
+    std::vector<FhpScene> scenes;
+    uint64_t nPix = 0, nPaths = 0;
+    FhpScene deepest = nullptr;
+    for (uint32_t i = 0; i < nv; i++) {
+        const FhpView& v = views[i];
+        if (!v.scene || !v.d_color) return fail(FHP_EINVAL, "view " + std::to_string(i) + ": null scene or output");
+        if (v.width == 0 || v.height == 0 || v.spp == 0 || v.width > 65536 || v.height > 65536)
+            return fail(FHP_EINVAL, "view " + std::to_string(i) + ": bad frame shape");
+        uint32_t si = 0;
+        while (si < scenes.size() && scenes[si] != v.scene) si++;
+        if (si == scenes.size()) scenes.push_back(v.scene);
+        if (!deepest || v.scene->stackNeed > deepest->stackNeed) deepest = v.scene;
+        ViewDev& d = vt[i];
+        d.cam = v.scene->dev.cam;
+        if (v.h_camera) {
+            HostScene c;
+            std::memcpy(c.cam, v.h_camera, sizeof(c.cam));
+            computeCamera(c);
+            d.cam.pos = mk3(c.cam[0], c.cam[1], c.cam[2]);
+            d.cam.U = mk3(c.camU[0], c.camU[1], c.camU[2]);
+            d.cam.V = mk3(c.camV[0], c.camV[1], c.camV[2]);
+            d.cam.W = mk3(c.camW[0], c.camW[1], c.camW[2]);
+            d.cam.nearZ = c.cam[12];
+            d.cam.farZ = c.cam[13];
+        }
+        d.W = v.width; d.H = v.height; d.spp = v.spp; d.frame = v.frame;
+        d.scene = si;
+        d.pixBase = uint32_t(nPix);
+        d.pathBase = uint32_t(nPaths);
+        d.out = reinterpret_cast<float4*>(v.d_color);
+        nPix += uint64_t(v.width) * v.height;
+        nPaths += uint64_t(v.width) * v.height * v.spp;
+        if (nPaths > 0xffffffffull / 3) return fail(FHP_EINVAL, "batch too large");
+    }
+    std::vector<SceneDev> stab(scenes.size());
+    for (size_t i = 0; i < scenes.size(); i++) stab[i] = scenes[i]->dev;
+    int rc = ensurePassBuffersN(p, nPaths, nPix);
+    if (rc) return rc;
+    HIP_OR_FAIL(p->viewTab.alloc(vt.size() * sizeof(ViewDev)));
+    HIP_OR_FAIL(p->sceneTab.alloc(stab.size() * sizeof(SceneDev)));
+    HIP_OR_FAIL(hipMemcpyAsync(p->viewTab.p, vt.data(), vt.size() * sizeof(ViewDev), hipMemcpyHostToDevice, st));
+    HIP_OR_FAIL(hipMemcpyAsync(p->sceneTab.p, stab.data(), stab.size() * sizeof(SceneDev), hipMemcpyHostToDevice, st));
+    const bool count = p->get("countTraversal") != 0;
+    unsigned long long* dstats = p->stats.as<unsigned long long>();
+    if (h_stats) HIP_OR_FAIL(launchFill(reinterpret_cast<float*>(dstats), 12, 0.f, st));
+    MptArgs a{};
+    a.maxBounces = uint32_t(p->get("maxBounces"));
+    a.computeDirect = p->get("computeDirect") != 0;
+    a.importance = p->get("useImportanceSampling") != 0;
+    a.nee = p->get("useEmissiveNEE") != 0;
+    a.lightKind = uint32_t(p->get("emissiveSampler"));
+    if (a.nee && !a.computeDirect) return fail(FHP_EINVAL, "useEmissiveNEE requires computeDirect");
+    a.separateShadow = 0;
+    a.nPaths = uint32_t(nPaths);
+    a.nPix = uint32_t(nPix);
+    a.views = p->viewTab.as<ViewDev>();
+    a.scenes = p->sceneTab.as<SceneDev>();
+    a.nViews = nv;
+    a.thp = p->thp.as<float4>(); a.prevN = p->prevN.as<float4>(); a.rng = p->rng.as<uint4>();
+    a.Lp = p->Lp.as<float4>(); a.Ls = p->Ls.as<float4>();
+    a.rayQ[0] = p->rayQ0.as<float4>(); a.rayQ[1] = p->rayQ1.as<float4>();
+    a.pathQ[0] = p->pathQ0.as<uint32_t>(); a.pathQ[1] = p->pathQ1.as<uint32_t>();
+    a.hitQ = p->hitQ.as<uint4>(); a.hitIdx = p->hitIdx.as<uint32_t>(); a.shQ = p->shQ.as<float4>();
+    a.pixHit = p->pixHit.as<uint4>();
+    a.counters = p->counters.as<uint32_t>();
+    if ((rc = prepareTimer(p))) return rc;
+    HIP_OR_FAIL(launchMptBatch(a, passTrace(deepest, p), count, h_stats ? dstats : nullptr, st, &p->timer));
+    if ((rc = collectTimer(p, st))) return rc;
+    if (h_stats) {
+        unsigned long long hs[6];
+        HIP_OR_FAIL(hipMemcpyAsync(hs, dstats, 48, hipMemcpyDeviceToHost, st));
+        HIP_OR_FAIL(hipStreamSynchronize(st));
+        for (int i = 0; i < 6; i++) h_stats[i] = hs[i];
+    } else {
+        // The tables above are copied from this call's host vectors.
+        HIP_OR_FAIL(hipStreamSynchronize(st));
     }
     return FHP_OK;
 }

@@ -20,8 +20,18 @@ constexpr int kCounterStride = 32;
 constexpr int kHitCount = 3 * kCounterStride;
 constexpr int kCounterWords = 4 * kCounterStride;  // counters: [0, 32) ray queues by depth, [32, 64) shadow queues
 
+// One view of a batched frame: its scene (index into the batch's scene table), camera, frame
+// shape and the first pixel / path of the view in the batch's pixel and path index spaces.
+struct ViewDev {
+    CameraDev cam;
+    uint32_t W, H, spp, frame;
+    uint32_t scene;
+    uint32_t pixBase, pathBase;
+    float4* out;
+};
+
 // Per-path state and queues of the wavefront forward pass. Slots are indexed by path id
-// (pixel * spp + sample); queues are compacted each bounce.
+// (pixel * spp + sample, views concatenated in a batch); queues are compacted each bounce.
 struct MptArgs {
     uint32_t W, H, spp, frame, maxBounces;
     uint32_t computeDirect, importance, nee, lightKind;
@@ -41,6 +51,12 @@ struct MptArgs {
     uint4* pixHit;   // primary hits per pixel
     uint32_t* counters;   // [0, 64) queue counters, [64] frame number (read by the kernels so a
                           // captured graph can be replayed for any frame)
+    // Batch mode (several views, possibly of different scenes, in one launch sequence): views
+    // and scenes are device tables; W, H, spp and frame above are unused.
+    const ViewDev* views;
+    const SceneDev* scenes;
+    uint32_t nViews;
+    uint32_t nPix;        // pixels over all views (W * H for one view)
 };
 
 struct TraceScratch {
@@ -99,6 +115,8 @@ struct KernelTimer {
 hipError_t launchTraceRays(const SceneDev& s, const float4* rays, uint32_t n, uint4* hits, uint32_t* occluded, bool any,
                            const TraceScratch& ts, hipStream_t stream);
 // Full frame: frame-number store, bounce loop, resolve, optional ray counts.
+hipError_t launchMptBatch(const MptArgs& a, const TraceScratch& ts, bool count, unsigned long long* d_counts,
+                          hipStream_t stream, KernelTimer* timer);
 hipError_t launchMptFrame(const SceneDev& s, const MptArgs& a, const TraceScratch& ts, float4* out, bool count,
                           unsigned long long* d_counts, hipStream_t stream, KernelTimer* timer);
 // Pieces used with graph capture: the bounce loop (capturable, no host-varying arguments) and

@@ -73,6 +73,31 @@ __global__ void kZeroU32(uint32_t* p, uint32_t n) {
     if (i < n) p[i] = 0u;
 }
 
+// View of a pixel or path: one frame (the kernel's scene and MptArgs) or a batch (binary search
+// of the view table by first pixel / first path).
+struct ViewCtx {
+    const SceneDev* s;
+    const CameraDev* cam;
+    uint32_t W, H, spp, frame, pixBase, pathBase;
+    float4* out;
+};
+__device__ __forceinline__ uint32_t findView(const ViewDev* v, uint32_t n, uint32_t x, bool byPath) {
+    uint32_t lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        const uint32_t b = byPath ? v[mid].pathBase : v[mid].pixBase;
+        if (b <= x) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+template <bool MULTI>
+__device__ __forceinline__ ViewCtx viewCtx(const SceneDev& s0, const MptArgs& a, uint32_t x, bool byPath) {
+    if (!MULTI) return ViewCtx{&s0, &s0.cam, a.W, a.H, a.spp, a.counters[2 * kCounterStride], 0u, 0u, nullptr};
+    const ViewDev& v = a.views[findView(a.views, a.nViews, x, byPath)];
+    return ViewCtx{a.scenes + v.scene, &v.cam, v.W, v.H, v.spp, v.frame, v.pixBase, v.pathBase, v.out};
+}
+
 // Pinhole primary ray through the pixel centre (Camera.slang:46-77, VBufferRT.slang:56-66).
 __device__ __forceinline__ void primaryRay(const CameraDev& c, uint32_t px, uint32_t py, uint32_t W, uint32_t H, f3& o, f3& d,
                                            float& tmin, float& tmax) {
@@ -86,12 +111,15 @@ __device__ __forceinline__ void primaryRay(const CameraDev& c, uint32_t px, uint
     tmax = c.farZ * invCos;
 }
 
+template <bool MULTI>
 __global__ void kPrimaryRays(SceneDev s, MptArgs a) {
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= a.W * a.H) return;
+    if (p >= a.nPix) return;
+    const ViewCtx c = viewCtx<MULTI>(s, a, p, false);
+    const uint32_t q = p - c.pixBase;
     f3 o, d;
     float tmin, tmax;
-    primaryRay(s.cam, p % a.W, p / a.W, a.W, a.H, o, d, tmin, tmax);
+    primaryRay(*c.cam, q % c.W, q / c.W, c.W, c.H, o, d, tmin, tmax);
     a.rayQ[0][2 * p] = make_float4(o.x, o.y, o.z, tmin);
     a.rayQ[0][2 * p + 1] = make_float4(d.x, d.y, d.z, tmax);
 }
@@ -153,22 +181,26 @@ __device__ __forceinline__ void pushRay(const MptArgs& a, int q, uint32_t slot, 
 }
 
 // Primary-hit shading, one thread per path (MinimalPathTracer.rt.slang:392-449).
-__global__ void kShadePrimary(SceneDev s, MptArgs a) {
+template <bool MULTI>
+__global__ void kShadePrimary(SceneDev s0, MptArgs a) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= a.nPaths) return;
-    const uint32_t pix = p / a.spp, k = p % a.spp;
-    const uint32_t px = pix % a.W, py = pix / a.W;
-    const uint4 h = a.pixHit[pix];
+    const ViewCtx c = viewCtx<MULTI>(s0, a, p, true);
+    const SceneDev& s = *c.s;
+    const uint32_t lp = p - c.pathBase;
+    const uint32_t pix = lp / c.spp, k = lp % c.spp;
+    const uint32_t px = pix % c.W, py = pix / c.W;
+    const uint4 h = a.pixHit[c.pixBase + pix];
     bool cont = false, shadow = false;
     f3 newO, newD, shO, shD, shC;
     float shT = 0.f;
     if (h.x != kInvalid) {
         f3 o, d;
         float tmin, tmax;
-        primaryRay(s.cam, px, py, a.W, a.H, o, d, tmin, tmax);
+        primaryRay(*c.cam, px, py, c.W, c.H, o, d, tmin, tmax);
         Shading sd = shadingFromHit(s, h.x, u2f(h.z), u2f(h.w), d);
         Material m = loadMaterial(s, s.mats, s.texels, sd, sd, nullptr);
-        Rng rng = rngSeed(px, py, a.counters[2 * kCounterStride] * a.spp + k);
+        Rng rng = rngSeed(px, py, c.frame * c.spp + k);
         f3 org = rayOriginFromShading(sd);
         f3 Lp = mk3(0.f);
         if (a.computeDirect) {
@@ -204,7 +236,8 @@ __global__ void kShadePrimary(SceneDev s, MptArgs a) {
 // handleHit for scatter depth `depth` (MinimalPathTracer.rt.slang:324-376), one thread per queued ray.
 // Runs on the compacted hits of the bounce (entry k: hit hitQ[k] of ray hitIdx[k]) unless
 // visibility rays have launches of their own, then on every queued ray (hitQ[i] of ray i).
-__global__ void kShade(SceneDev s, MptArgs a, uint32_t depth) {
+template <bool MULTI>
+__global__ void kShade(SceneDev s0, MptArgs a, uint32_t depth) {
     const int qi = depth & 1, qo = qi ^ 1;
     const bool compact = !a.separateShadow;
     const uint32_t n = compact ? a.counters[kHitCount + depth] : a.counters[depth];
@@ -219,6 +252,7 @@ __global__ void kShade(SceneDev s, MptArgs a, uint32_t depth) {
             const uint32_t i = compact ? a.hitIdx[k] : k;
             pathId = a.pathQ[qi][i];
             const uint4 h = a.hitQ[k];
+            const SceneDev& s = *viewCtx<MULTI>(s0, a, pathId, true).s;
             if (h.x != kInvalid) {
                 const float4 ro = a.rayQ[qi][2 * size_t(i)], rd = a.rayQ[qi][2 * size_t(i) + 1];
                 const f3 o = mk3(ro.x, ro.y, ro.z), d = mk3(rd.x, rd.y, rd.z);
@@ -305,11 +339,15 @@ __global__ __launch_bounds__(B) void kShadow(SceneDev s, MptArgs a, uint32_t slo
 // pixel); otherwise only rays that hit something are appended (hits[k], hitIdx[k] = j, count at
 // *hitCount), so the shading launch runs on dense lanes. A miss needs no shading: the scenes have
 // no environment light (MinimalPathTracer.rt.slang:378-390 adds nothing for a black background).
-template <int B, int D, bool COUNT>
+//
+// In a batch (MULTI) each ray traverses the scene of its view: visibility and bounce rays find it
+// through their path id (pathIds[j] for bounce ray j), primary rays through their pixel.
+template <int B, int D, bool COUNT, bool MULTI>
 __global__ __launch_bounds__(B) void kTraceBounce(SceneDev s, MptArgs a, const float4* __restrict__ rays,
                                                   const uint32_t* nCPtr, uint32_t nCConst, uint4* __restrict__ hits,
                                                   uint32_t* __restrict__ hitIdx, uint32_t* hitCount, const uint32_t* nSPtr,
-                                                  TraceScratch ts, unsigned long long* stats) {
+                                                  const uint32_t* __restrict__ pathIds, TraceScratch ts,
+                                                  unsigned long long* stats) {
     __shared__ uint32_t lds[D * B];
     TraceStack st = makeStack<B, D>(lds, ts);
     const uint32_t nS = nSPtr ? *nSPtr : 0u;
@@ -322,10 +360,17 @@ __global__ __launch_bounds__(B) void kTraceBounce(SceneDev s, MptArgs a, const f
         const uint32_t j = vis ? i : i - nS;
         const float4* q = vis ? a.shQ + 3 * size_t(j) : rays + 2 * size_t(j);
         const float4 r0 = q[0], r1 = q[1];
+        GeomRef g{s.bvh, s.triPos};
+        if (MULTI) {
+            const bool byPath = vis || pathIds != nullptr;
+            const uint32_t key = vis ? f2u(r0.w) : (pathIds ? pathIds[j] : j);
+            const SceneDev& sv = a.scenes[a.views[findView(a.views, a.nViews, key, byPath)].scene];
+            g = GeomRef{sv.bvh, sv.triPos};
+        }
         TravState r;
         r.start(mk3(r0.x, r0.y, r0.z), mk3(r1.x, r1.y, r1.z), vis ? 0.f : r0.w, r1.w, vis);
         TraceCounters c{0, 0};
-        while (r.step<COUNT>(s, st, &c)) {
+        while (r.step<COUNT>(g, st, &c)) {
         }
         if (COUNT) {
             TraceCounters& t = vis ? cs : cc;
@@ -361,23 +406,32 @@ __global__ __launch_bounds__(B) void kTraceBounce(SceneDev s, MptArgs a, const f
 }
 
 // outColor + radiance per sample, averaged over samples like AccumulatePass Single mode.
+template <bool MULTI>
 __global__ void kResolve(MptArgs a, float4* out) {
-    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pix >= a.W * a.H) return;
+    const uint32_t bpix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (bpix >= a.nPix) return;
+    uint32_t pix = bpix, pathBase = 0, spp = a.spp;
+    if (MULTI) {
+        const ViewDev& v = a.views[findView(a.views, a.nViews, bpix, false)];
+        pix = bpix - v.pixBase;
+        pathBase = v.pathBase;
+        spp = v.spp;
+        out = v.out;
+    }
     f3 sum = mk3(0.f);
-    for (uint32_t k = 0; k < a.spp; k++) {
-        const uint32_t p = pix * a.spp + k;
+    for (uint32_t k = 0; k < spp; k++) {
+        const uint32_t p = pathBase + pix * spp + k;
         const float4 lp = a.Lp[p], ls = a.Ls[p];
         sum = sum + (mk3(lp.x, lp.y, lp.z) + mk3(ls.x, ls.y, ls.z));
     }
-    const float w = 1.0f / float(a.spp);
+    const float w = 1.0f / float(spp);
     out[pix] = make_float4(sum.x * w, sum.y * w, sum.z * w, 1.f);
 }
 
 __global__ void kRayCounts(const MptArgs a, uint32_t nb, unsigned long long* out) {
     // out[0] += closest-hit rays (primary + all scatter queues), out[1] += shadow rays.
     if (threadIdx.x == 0 && blockIdx.x == 0) {
-        unsigned long long c = (unsigned long long)a.W * a.H, v = 0;
+        unsigned long long c = (unsigned long long)a.nPix, v = 0;
         for (uint32_t b = 0; b < nb; b++) c += a.counters[b];
         for (uint32_t b = 0; b <= nb; b++) v += a.counters[kCounterStride + b];
         out[0] += c;
@@ -395,8 +449,8 @@ hipError_t launchMptPrologue(const MptArgs& a, hipStream_t st) {
 }
 
 hipError_t launchMptEpilogue(const MptArgs& a, float4* out, unsigned long long* d_counts, hipStream_t st) {
-    const uint32_t npix = a.W * a.H;
-    hipLaunchKernelGGL(kResolve, dim3((npix + 255) / 256), dim3(256), 0, st, a, out);
+    const uint32_t npix = a.nPix;
+    hipLaunchKernelGGL(kResolve<false>, dim3((npix + 255) / 256), dim3(256), 0, st, a, out);
     if (d_counts) hipLaunchKernelGGL(kRayCounts, dim3(1), dim3(64), 0, st, a, a.maxBounces + 1, d_counts);
     return hipGetLastError();
 }
@@ -410,15 +464,16 @@ hipError_t launchMptFrame(const SceneDev& s, const MptArgs& a, const TraceScratc
     return launchMptEpilogue(a, out, d_counts, st);
 }
 
-hipError_t launchMptBounces(const SceneDev& s, const MptArgs& a, const TraceScratch& ts, bool count,
-                            unsigned long long* d_counts, hipStream_t st, KernelTimer* tm) {
+template <bool MULTI>
+hipError_t mptBounces(const SceneDev& s, const MptArgs& a, const TraceScratch& ts, bool count, unsigned long long* d_counts,
+                      hipStream_t st, KernelTimer* tm) {
     KernelTimer dummy;
     if (!tm) tm = &dummy;
-    const uint32_t npix = a.W * a.H;
+    const uint32_t npix = a.nPix;
     const int T = 256;
     hipLaunchKernelGGL(kZeroU32, dim3(1), dim3(2 * kCounterStride), 0, st, a.counters, 2u * kCounterStride);
     hipLaunchKernelGGL(kZeroU32, dim3(1), dim3(kCounterStride), 0, st, a.counters + kHitCount, uint32_t(kCounterStride));
-    hipLaunchKernelGGL(kPrimaryRays, dim3((npix + T - 1) / T), dim3(T), 0, st, s, a);
+    hipLaunchKernelGGL(kPrimaryRays<MULTI>, dim3((npix + T - 1) / T), dim3(T), 0, st, s, a);
     unsigned long long* tstat = count ? d_counts + 2 : nullptr;   // closest-hit nodes, triangles
     unsigned long long* sstat = count ? d_counts + 4 : nullptr;   // shadow nodes, triangles
     const TraceScratch& tsc = ts;
@@ -446,17 +501,17 @@ hipError_t launchMptBounces(const SceneDev& s, const MptArgs& a, const TraceScra
         return e;
     };
     auto bounce = [&](const float4* rays, const uint32_t* nC, uint32_t nCConst, uint4* hits, uint32_t* hitIdx,
-                      uint32_t* hitCount, const uint32_t* nS, uint32_t maxBlocks) {
+                      uint32_t* hitCount, const uint32_t* nS, const uint32_t* pathIds, uint32_t maxBlocks) {
         tm->begin(KK_TRACE, st);
         hipError_t e = withTraceConfig(tsc, [&](auto Bc, auto Dc) {
             constexpr int B = decltype(Bc)::value, D = decltype(Dc)::value;
             const uint32_t blocks = std::min<uint32_t>(tsc.gridBlocks, maxBlocks);
             if (count)
-                hipLaunchKernelGGL((kTraceBounce<B, D, true>), dim3(blocks), dim3(B), 0, st, s, a, rays, nC, nCConst, hits,
-                                   hitIdx, hitCount, nS, tsc, tstat);
+                hipLaunchKernelGGL((kTraceBounce<B, D, true, MULTI>), dim3(blocks), dim3(B), 0, st, s, a, rays, nC, nCConst,
+                                   hits, hitIdx, hitCount, nS, pathIds, tsc, tstat);
             else
-                hipLaunchKernelGGL((kTraceBounce<B, D, false>), dim3(blocks), dim3(B), 0, st, s, a, rays, nC, nCConst, hits,
-                                   hitIdx, hitCount, nS, tsc, tstat);
+                hipLaunchKernelGGL((kTraceBounce<B, D, false, MULTI>), dim3(blocks), dim3(B), 0, st, s, a, rays, nC, nCConst,
+                                   hits, hitIdx, hitCount, nS, pathIds, tsc, tstat);
             return hipGetLastError();
         });
         tm->end(st);
@@ -464,24 +519,41 @@ hipError_t launchMptBounces(const SceneDev& s, const MptArgs& a, const TraceScra
     };
     // Primary rays: one per pixel (the reference's V-buffer), shared by the pixel's samples.
     hipError_t e = a.separateShadow ? traceQ(a.rayQ[0], nullptr, npix, a.pixHit, (npix + tsc.block - 1) / tsc.block)
-                                    : bounce(a.rayQ[0], nullptr, npix, a.pixHit, nullptr, nullptr, nullptr,
+                                    : bounce(a.rayQ[0], nullptr, npix, a.pixHit, nullptr, nullptr, nullptr, nullptr,
                                              (npix + tsc.block - 1) / tsc.block);
     if (e != hipSuccess) return e;
     tm->begin(KK_SHADE, st);
-    hipLaunchKernelGGL(kShadePrimary, dim3((a.nPaths + T - 1) / T), dim3(T), 0, st, s, a);
+    hipLaunchKernelGGL(kShadePrimary<MULTI>, dim3((a.nPaths + T - 1) / T), dim3(T), 0, st, s, a);
     tm->end(st);
     const uint32_t shadeBlocks = std::min<uint32_t>((a.nPaths + T - 1) / T, uint32_t(numCUs()) * 32);
     for (uint32_t b = 0; b <= a.maxBounces; b++) {
         // Closest-hit rays of bounce b with the visibility rays produced alongside them.
         if ((e = (a.separateShadow ? traceQ(a.rayQ[b & 1], a.counters + b, 0u, a.hitQ, 0xffffffffu)
                                    : bounce(a.rayQ[b & 1], a.counters + b, 0u, a.hitQ, a.hitIdx, a.counters + kHitCount + b,
-                                            a.nee ? a.counters + kCounterStride + b : nullptr, 0xffffffffu))) != hipSuccess)
+                                            a.nee ? a.counters + kCounterStride + b : nullptr, a.pathQ[b & 1], 0xffffffffu))) !=
+             hipSuccess)
             return e;
         if (a.separateShadow && a.nee && (e = shadow(b)) != hipSuccess) return e;
         tm->begin(KK_SHADE, st);
-        hipLaunchKernelGGL(kShade, dim3(shadeBlocks), dim3(T), 0, st, s, a, b);
+        hipLaunchKernelGGL(kShade<MULTI>, dim3(shadeBlocks), dim3(T), 0, st, s, a, b);
         tm->end(st);
     }
+    return hipGetLastError();
+}
+
+hipError_t launchMptBounces(const SceneDev& s, const MptArgs& a, const TraceScratch& ts, bool count,
+                            unsigned long long* d_counts, hipStream_t st, KernelTimer* tm) {
+    return mptBounces<false>(s, a, ts, count, d_counts, st, tm);
+}
+
+// A batch of views in one launch sequence (no graph: the view table changes per call).
+hipError_t launchMptBatch(const MptArgs& a, const TraceScratch& ts, bool count, unsigned long long* d_counts, hipStream_t st,
+                          KernelTimer* tm) {
+    const SceneDev none{};
+    hipError_t e = mptBounces<true>(none, a, ts, count, d_counts, st, tm);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(kResolve<true>, dim3((a.nPix + 255) / 256), dim3(256), 0, st, a, nullptr);
+    if (d_counts) hipLaunchKernelGGL(kRayCounts, dim3(1), dim3(64), 0, st, a, a.maxBounces + 1, d_counts);
     return hipGetLastError();
 }
 

@@ -41,6 +41,12 @@ struct TraceStack {
     __device__ __forceinline__ uint32_t capacity() const { return ldsDepth + ovfDepth; }
 };
 
+// The arrays traversal reads: a SceneDev, or this pair when the scene varies per ray.
+struct GeomRef {
+    const float4* __restrict__ bvh;
+    const float4* __restrict__ triPos;
+};
+
 struct TraceCounters {
     uint64_t nodes, tris;
 };
@@ -114,8 +120,8 @@ struct TravState {
         sp = 0;
         any = anyHit;
     }
-    template <bool COUNT>
-    __device__ __forceinline__ bool step(const SceneDev& s, TraceStack& st, TraceCounters* cnt) {
+    template <bool COUNT, typename SceneT>
+    __device__ __forceinline__ bool step(const SceneT& s, TraceStack& st, TraceCounters* cnt) {
         if (!childIsLeaf(cur)) {
             const float4* node = s.bvh + 8 * size_t(cur);
             float tn[4];
@@ -167,8 +173,8 @@ struct TravState {
 
 // Closest hit (ANY = false) or first hit (ANY = true) in [tmin, tmax]. Returns true on a hit.
 // `anyRt` selects first-hit per lane at run time (kernels that trace both kinds of ray).
-template <bool ANY, bool COUNT>
-__device__ __forceinline__ bool traceBVH4(const SceneDev& s, f3 o, f3 d, float tmin, float tmax, TraceStack& st,
+template <bool ANY, bool COUNT, typename SceneT>
+__device__ __forceinline__ bool traceBVH4(const SceneT& s, f3 o, f3 d, float tmin, float tmax, TraceStack& st,
                                           HitRec& hit, TraceCounters* cnt = nullptr, bool anyRt = false) {
     TravState r;
     r.start(o, d, tmin, tmax, ANY || anyRt);

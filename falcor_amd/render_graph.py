@@ -261,6 +261,33 @@ class MinimalPathTracer(_NativePass):
     kind = "MinimalPathTracer"
     outputs = ("color",)
 
+    def render_batch(self, views: list[dict], stats: bool = False):
+        """Renders several views in one launch sequence (fhp_pass_execute_batch). Each view is a dict
+        with scene, width, height and optional spp (1), frame (0), camera (a scene.Camera, default
+        the scene's) and out ([H, W, 4] float32 tensor). Returns the output tensors (and stats)."""
+        torch = _torch()
+        dev = _device()
+        arr = (_lib.View * len(views))()
+        outs, cams = [], []
+        for i, v in enumerate(views):
+            w, h = int(v["width"]), int(v["height"])
+            out = v.get("out")
+            if out is None:
+                out = torch.empty((h, w, 4), dtype=torch.float32, device=dev)
+            outs.append(out)
+            cam = v.get("camera")
+            if cam is not None:
+                cams.append(np.ascontiguousarray(cam.pack(), np.float32))
+            arr[i].scene = v["scene"].handle
+            arr[i].h_camera = cams[-1].ctypes.data if cam is not None else None
+            arr[i].width, arr[i].height = w, h
+            arr[i].spp, arr[i].frame = int(v.get("spp", 1)), int(v.get("frame", 0))
+            arr[i].d_color = out.data_ptr()
+        st = np.zeros(6, np.uint64) if stats else None
+        _lib.check(_lib.load().fhp_pass_execute_batch(self.handle, len(views), arr, st.ctypes.data if stats else None,
+                                                      _lib.stream_ptr()))
+        return (outs, st) if stats else outs
+
 
 class WARDiffPathTracer(_NativePass):
     """WARDiffPathTracer (WARDiffPathTracer.cpp). Outputs color and dColor."""

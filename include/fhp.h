@@ -134,6 +134,21 @@ int fhp_pass_get_property(FhpPass pass, const char* key, double* value);
 int fhp_pass_execute(FhpPass pass, FhpScene scene, uint32_t width, uint32_t height, uint32_t spp, uint32_t frame,
                      float* d_color, uint64_t* h_stats, FhpStream stream);
 
+/* A batch of views rendered in one launch sequence (MinimalPathTracer): every view's paths share
+ * the wavefront queues and the persistent traversal grid, each ray traversing its own view's
+ * scene. h_camera is 14 floats as in FhpSceneDesc, or NULL for the scene's camera; each view is
+ * resolved into its d_color (height * width * 4 floats) exactly as fhp_pass_execute would render
+ * it. No counterpart in the reference (one graph execution per view there,
+ * RenderGraph::execute, Source/Falcor/RenderGraph/RenderGraph.cpp). Synchronises the stream
+ * before returning (the view tables are staged from host memory). */
+typedef struct FhpView {
+    FhpScene scene;
+    const float* h_camera;
+    uint32_t width, height, spp, frame;
+    float* d_color;
+} FhpView;
+int fhp_pass_execute_batch(FhpPass pass, uint32_t num_views, const FhpView* views, uint64_t* h_stats, FhpStream stream);
+
 /* Accumulated kernel times (ms) and launch counts by kernel kind, recorded with HIP events on
  * the launch stream when the pass property "timeKernels" is 1: [0] closest-hit traversal,
  * [1] shadow traversal, [2] shading, [3] WAR primal, [4] WAR adjoint, [5] other.
