@@ -144,6 +144,49 @@ def forward_bench(steps: int, warmup: int, tune: bool, tune_out: str | None):
     return res
 
 
+def batch_bench(reps: int = 2):
+    """Batched forward: 64 views (512x512, spp in {1,2,4,8,16}, 6 bounces, LightBVH NEE) over 16
+    sphere fields of 10k-2M triangles, rendered in one launch sequence, against the same views
+    rendered one after the other (one launch sequence per view)."""
+    import torch
+    from falcor_amd import scene as S
+    from falcor_amd.render_graph import MinimalPathTracer, Scene
+
+    t0 = time.time()
+    datas, views = S.batch_workload()
+    scenes = [Scene(d) for d in datas]
+    torch.cuda.synchronize()
+    build = time.time() - t0
+    vs = [dict(v, scene=scenes[v["scene"]]) for v in views]
+    for v in vs:
+        v["out"] = torch.empty((v["height"], v["width"], 4), device="cuda")
+    p = MinimalPathTracer({"maxBounces": 6, "useEmissiveNEE": True, "emissiveSampler": "LightBVH"})
+    _, st = p.render_batch(vs, stats=True)  # warm-up; also allocates the batch buffers
+    rays = int(st[0]) + int(st[1])
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        p.render_batch(vs)
+    torch.cuda.synchronize()
+    batched = (time.perf_counter() - t) / reps * 1e3
+    q = MinimalPathTracer({"maxBounces": 6, "useEmissiveNEE": True, "emissiveSampler": "LightBVH"})
+    for v in vs:
+        q.render_batch([v])
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        for v in vs:
+            q.render_batch([v])
+    torch.cuda.synchronize()
+    seq = (time.perf_counter() - t) / reps * 1e3
+    paths = sum(v["width"] * v["height"] * v["spp"] for v in views)
+    log(f"batch: {len(views)} views over {len(datas)} scenes ({sum(d.num_triangles for d in datas)} tris, build "
+        f"{build:.1f}s), {paths / 1e6:.1f}M paths, {rays / 1e6:.1f}M rays: batched {batched:.1f} ms "
+        f"({rays / batched / 1e3:.1f} Mrays/s), per-view {seq:.1f} ms ({rays / seq / 1e3:.1f} Mrays/s)")
+    return dict(views=len(views), scenes=len(datas), paths=paths, rays=rays, ms_batched=batched, ms_per_view_loop=seq,
+                mrays_batched=rays / batched / 1e3, speedup=seq / batched)
+
+
 def cpu_baseline(data, W, H, spp, B):
     import oracle
     osc = oracle.OracleScene(data)
@@ -264,6 +307,8 @@ def main():
     ap.add_argument("--tune", action="store_true",
                     help="autotune the traversal launch on this workload (untimed) instead of using the tuned table")
     ap.add_argument("--tune-out", default=None, help="with --tune: JSON file to record the chosen launch in")
+    ap.add_argument("--batch", action="store_true",
+                    help="also measure the batched forward workload (64 views over 16 scenes; adds ~1 min)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy 3 times (scale for HBM byte counters)")
     args = ap.parse_args()
@@ -310,6 +355,8 @@ def main():
                      "avg_launch_ms": round(fwd["trace_avg_launch_ms"], 4),
                      "launches_per_step": fwd["trace_launches_per_step"]},
     }
+    if args.batch:
+        line["batch"] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in batch_bench().items()}
     if not args.skip_diff:
         line["diff"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in diff_bench().items()}
     if not args.skip_cpu:

@@ -432,6 +432,36 @@ def sphere_field(seed: int = 1, num_spheres: int = 1024, num_emissive: int = 409
     return data
 
 
+def batch_workload(seed: int = 7, num_scenes: int = 16, num_views: int = 64, res: int = 512,
+                   tri_range=(1e4, 2e6), spp_choices=(1, 2, 4, 8, 16)):
+    """Batched-forward workload: `num_scenes` sphere fields whose triangle counts are log-uniform in
+    `tri_range` (1280-triangle icospheres at the sphere-field density of 1024 spheres per 100^3,
+    at most 4096 emissive triangles), and `num_views` views, each with a scene, a camera pose
+    looking into the field, and spp drawn from `spp_choices`. Returns (scenes, views) with views as
+    dicts of scene index, width, height, spp, frame and camera."""
+    rng = np.random.default_rng(seed)
+    scenes = []
+    for k in range(num_scenes):
+        tris = math.exp(rng.uniform(math.log(tri_range[0]), math.log(tri_range[1])))
+        n = max(1, int(round(tris / 1280)))
+        box = 100.0 * (n / 1024.0) ** (1.0 / 3.0)
+        scenes.append(sphere_field(seed * 1000 + k, num_spheres=n, num_emissive=min(4096, n * 1280 // 4), box=box,
+                                   aspect=1.0))
+    views = []
+    for v in range(num_views):
+        k = int(rng.integers(num_scenes))
+        d = scenes[k]
+        lo, hi = d.positions.min(0), d.positions.max(0)
+        c = 0.5 * (lo + hi)
+        ext = float((hi - lo).max())
+        theta = rng.uniform(0, 2 * math.pi)
+        eye = c + np.array([math.cos(theta), rng.uniform(-0.3, 0.3), math.sin(theta)]) * 0.9 * ext
+        target = c + rng.normal(0, 0.05 * ext, 3)
+        cam = Camera(tuple(eye), tuple(target), (0, 1, 0), math.radians(60.0), 1.0)
+        views.append(dict(scene=k, width=res, height=res, spp=int(rng.choice(spp_choices)), frame=v, camera=cam))
+    return scenes, views
+
+
 def textured_spheres(seed: int = 3, num_objects: int = 64, tex_res: int = 512, subdivisions: int = 3,
                      target: bool = False, init_albedo: float = 0.5, init_roughness: float = 0.5) -> SceneData:
     """Differentiable-materials scene: an 8x8 grid of icospheres, each with its own RGB fp32 albedo
