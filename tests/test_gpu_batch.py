@@ -50,8 +50,10 @@ def test_batch_rejects_bad_views(gpu):
     from falcor_amd.render_graph import MinimalPathTracer, Scene, WARDiffPathTracer
     box = Scene(S.closed_box(0))
     p = MinimalPathTracer({})
-    with pytest.raises(_lib.FhpError, match="frame shape"):
+    with pytest.raises(_lib.FhpError, match="view 0"):
         p.render_batch([dict(scene=box, width=0, height=8)])
+    with pytest.raises(_lib.FhpError, match="frame shape"):
+        p.render_batch([dict(scene=box, width=8, height=8, spp=0)])
     w = WARDiffPathTracer({})
     with pytest.raises(_lib.FhpError, match="MinimalPathTracer"):
         MinimalPathTracer.render_batch(w, [dict(scene=box, width=8, height=8)])
