@@ -96,6 +96,12 @@ _SIGS = {
     "fhp_grads_copy": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_uint64, C.c_void_p]),
     "fhp_pass_backward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p,
                                     C.c_void_p, C.c_void_p, C.c_void_p]),
+    "fhp_accumulate": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_uint32, C.c_uint32, C.c_void_p, C.c_uint32,
+                                 C.c_void_p]),
+    "fhp_accumulate_reset": (C.c_int, [C.c_void_p]),
+    "fhp_accumulate_frame_count": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
+    "fhp_image_compare": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_int, C.c_void_p,
+                                    C.c_void_p, C.POINTER(C.c_double), C.c_void_p]),
 }
 
 EXPORTED = tuple(_SIGS)

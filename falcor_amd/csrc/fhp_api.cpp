@@ -75,6 +75,11 @@ struct FhpPass_ {
     DevBuf viewTab, sceneTab;  // batch mode tables
     uint32_t cap = 0, capPix = 0;
     KernelTimer timer;
+    // AccumulatePass state: running sums of the current precision mode, frames accumulated,
+    // frame shape (AccumulatePass.h:130-150).
+    DevBuf accSum, accCorr;
+    uint32_t accCount = 0, accW = 0, accH = 0;
+    int accMode = -1;  // precision mode the sums are laid out for (-1: none yet)
     // Captured bounce loop of the forward pass (property useGraph), keyed by everything the
     // captured kernels read by value.
     hipGraphExec_t graph = nullptr;
@@ -129,6 +134,22 @@ const std::map<std::string, double>& warDefaults() {
         {"countTraversal", 0},  {"timeKernels", 0},       {"traceBlockSize", kDefaultBlock},
         {"traceStackDepth", kDefaultLds},                 {"traceBlocksPerCU", kDefaultBlocksPerCU}};
     return d;
+}
+
+const std::map<std::string, double>& accDefaults() {
+    // AccumulatePass.h:135-145: enabled, autoReset, precisionMode (Single), maxFrameCount (0: no
+    // limit), overflowMode (Stop).
+    static const std::map<std::string, double> d = {
+        {"enabled", 1}, {"autoReset", 1}, {"precisionMode", kAccumSingle}, {"maxFrameCount", 0}, {"overflowMode", 0}};
+    return d;
+}
+// Validates one AccumulatePass property value; returns an error message or "".
+std::string accCheck(const std::string& k, double v) {
+    const bool integral = v >= 0 && v <= 4294967295.0 && double(uint64_t(v)) == v;
+    if (k == "precisionMode" && !(integral && v <= 2)) return "precisionMode: 0 Double, 1 Single, 2 SingleCompensated";
+    if (k == "overflowMode" && !(integral && v <= 2)) return "overflowMode: 0 Stop, 1 Reset, 2 EMA";
+    if (k == "maxFrameCount" && !integral) return "maxFrameCount: a non-negative integer";
+    return "";
 }
 
 constexpr uint32_t kMinLds = 16;
@@ -451,20 +472,40 @@ int fhp_pass_create(const char* kind, int np, const char* const* keys, const dou
     const std::map<std::string, double>* defs;
     if (k == "MinimalPathTracer") defs = &mptDefaults();
     else if (k == "WARDiffPathTracer") defs = &warDefaults();
+    else if (k == "AccumulatePass") defs = &accDefaults();
     else return fail(FHP_EINVAL, "unknown pass kind '" + k + "'");
     std::unique_ptr<FhpPass_> p(new FhpPass_());
     p->kind = k;
     p->props = *defs;
     g_warning.clear();
+    const bool acc = k == "AccumulatePass";
+    bool hasEnabled = false;
+    int legacyEnable = -1;
     for (int i = 0; i < np; i++) {
         if (!keys[i]) return fail(FHP_EINVAL, "null property key");
         std::string key(keys[i]);
+        if (acc && key == "enableAccumulation") {
+            // Deprecated alias of 'enabled' (AccumulatePass.cpp:87-92).
+            g_warning = "'enableAccumulation' is deprecated. Use 'enabled' instead.";
+            legacyEnable = i;
+            continue;
+        }
         if (!defs->count(key)) {
             g_warning = "Unknown property '" + key + "' in " + k + " properties.";
             continue;
         }
+        if (acc) {
+            const std::string msg = accCheck(key, values[i]);
+            if (!msg.empty()) return fail(FHP_EINVAL, msg);
+            hasEnabled |= key == "enabled";
+        }
         p->props[key] = values[i];
         if (key == "fixedSeed") p->props["useFixedSeed"] = 1;
+    }
+    if (acc) {
+        if (legacyEnable >= 0 && !hasEnabled) p->props["enabled"] = values[legacyEnable];
+        *out = p.release();
+        return FHP_OK;
     }
     if (p->get("maxBounces") < 0 || p->get("maxBounces") > kMaxBounceSlots - 2)
         return fail(FHP_EINVAL, "maxBounces out of range [0, 28]");
@@ -484,8 +525,18 @@ int fhp_pass_set_property(FhpPass p, const char* key, double v) {
         g_warning = std::string("Unknown property '") + key + "' in " + p->kind + " properties.";
         return FHP_OK;
     }
-    if (std::string(key) == "maxBounces" && (v < 0 || v > kMaxBounceSlots - 2)) return fail(FHP_EINVAL, "maxBounces out of range");
     const std::string k(key);
+    if (p->kind == "AccumulatePass") {
+        const std::string msg = accCheck(k, v);
+        if (!msg.empty()) return fail(FHP_EINVAL, msg);
+        // Toggling accumulation or changing the precision mode restarts it
+        // (AccumulatePass::setEnabled, :366-373; the new mode's sums are created cleared, :380-420).
+        if ((k == "enabled" && (v != 0) != (p->get("enabled") != 0)) || (k == "precisionMode" && v != p->get("precisionMode")))
+            p->accCount = 0;
+        p->props[k] = v;
+        return FHP_OK;
+    }
+    if (k == "maxBounces" && (v < 0 || v > kMaxBounceSlots - 2)) return fail(FHP_EINVAL, "maxBounces out of range");
     if (k == "traceBlockSize" || k == "traceStackDepth" || k == "traceBlocksPerCU") {
         const double b = k == "traceBlockSize" ? v : p->get("traceBlockSize");
         const double l = k == "traceStackDepth" ? v : p->get("traceStackDepth");
@@ -509,6 +560,7 @@ int fhp_pass_get_property(FhpPass p, const char* key, double* v) {
 int fhp_pass_execute(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t spp, uint32_t frame, float* d_color,
                      uint64_t* h_stats, FhpStream stream) {
     if (!p || !d_color) return fail(FHP_EINVAL, "null argument");
+    if (p->kind == "AccumulatePass") return fail(FHP_EINVAL, "AccumulatePass runs through fhp_accumulate");
     if (W == 0 || H == 0 || spp == 0) return fail(FHP_EINVAL, "empty frame");
     if (W > 65536 || H > 65536) return fail(FHP_EINVAL, "frame dimension above 65536");
     hipStream_t st = hipStream_t(stream);
@@ -826,6 +878,100 @@ int fhp_pass_backward(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t sp
         HIP_OR_FAIL(hipStreamSynchronize(st));
         for (int i = 0; i < 6; i++) h_stats[i] = hs[i];
     }
+    return FHP_OK;
+}
+
+// ------------------------------------------------------------------ AccumulatePass
+int fhp_accumulate(FhpPass p, const void* d_in, int fmt, uint32_t W, uint32_t H, float* d_out, uint32_t refresh,
+                   FhpStream stream) {
+    if (!p || !d_in || !d_out) return fail(FHP_EINVAL, "null argument");
+    if (p->kind != "AccumulatePass") return fail(FHP_EINVAL, "fhp_accumulate needs an AccumulatePass");
+    if (fmt < kAccumInputFloat || fmt > kAccumInputSint) return fail(FHP_EINVAL, "input format: 0 float, 1 uint, 2 int");
+    if (W == 0 || H == 0) return fail(FHP_EINVAL, "empty frame");
+    if (W > 65536 || H > 65536) return fail(FHP_EINVAL, "frame dimension above 65536");
+    hipStream_t st = hipStream_t(stream);
+    // AccumulatePass::execute (AccumulatePass.cpp:127-218).
+    if (p->get("autoReset") != 0 && refresh) p->accCount = 0;
+    const uint32_t maxFrames = uint32_t(p->get("maxFrameCount"));
+    const int overflow = int(p->get("overflowMode"));
+    if (maxFrames > 0 && p->accCount == maxFrames) {
+        if (overflow == 0) return FHP_OK;  // Stop: the output keeps the accumulated image
+        if (overflow == 1) p->accCount = 0;  // Reset; EMA continues with weight 1 / (maxFrameCount + 1)
+    }
+    if (W != p->accW || H != p->accH) {
+        p->accW = W;
+        p->accH = H;
+        p->accCount = 0;
+    }
+    const uint64_t n = uint64_t(W) * H;
+    const bool enabled = p->get("enabled") != 0;
+    if (!enabled && fmt == kAccumInputFloat) {
+        // Disabled with float input: the input is copied through (the reference blits it).
+        HIP_OR_FAIL(hipMemcpyAsync(d_out, d_in, n * 16, hipMemcpyDeviceToDevice, st));
+        return FHP_OK;
+    }
+    // AccumulatePass::accumulate (:220-292) and prepareAccumulation (:380-420): the sums of the
+    // current precision mode are (re)created on a mode or size change, which restarts the
+    // accumulation; a frame count of 0 treats them as cleared.
+    const int mode = int(p->get("precisionMode"));
+    const uint64_t sumBytes = n * (mode == kAccumDouble ? 32 : 16);
+    if (mode != p->accMode || p->accSum.bytes != sumBytes) {
+        p->accSum.release();
+        p->accCorr.release();
+        HIP_OR_FAIL(p->accSum.alloc(sumBytes));
+        if (mode == kAccumSingleCompensated) HIP_OR_FAIL(p->accCorr.alloc(n * 16));
+        p->accMode = mode;
+        p->accCount = 0;
+    }
+    AccumArgs a{};
+    a.in = d_in;
+    a.out = reinterpret_cast<float4*>(d_out);
+    a.sum = mode == kAccumDouble ? nullptr : p->accSum.as<float4>();
+    a.corr = mode == kAccumSingleCompensated ? p->accCorr.as<float4>() : nullptr;
+    a.sumD = mode == kAccumDouble ? p->accSum.as<double2>() : nullptr;
+    a.n = n;
+    a.count = p->accCount;
+    a.flags = (enabled ? kAccumEnabled : 0u) | (maxFrames > 0 ? kAccumMovingAverage : 0u) | (p->accCount == 0 ? kAccumFirst : 0u);
+    a.mode = mode;
+    a.inputFormat = fmt;
+    HIP_OR_FAIL(launchAccumulate(a, st));
+    // The frame limit does not apply to SingleCompensated (:279-284).
+    if (maxFrames == 0 || mode == kAccumSingleCompensated || p->accCount < maxFrames) p->accCount++;
+    return FHP_OK;
+}
+int fhp_accumulate_reset(FhpPass p) {
+    if (!p) return fail(FHP_EINVAL, "null argument");
+    if (p->kind != "AccumulatePass") return fail(FHP_EINVAL, "not an AccumulatePass");
+    p->accCount = 0;
+    return FHP_OK;
+}
+int fhp_accumulate_frame_count(FhpPass p, uint32_t* count) {
+    if (!p || !count) return fail(FHP_EINVAL, "null argument");
+    if (p->kind != "AccumulatePass") return fail(FHP_EINVAL, "not an AccumulatePass");
+    *count = p->accCount;
+    return FHP_OK;
+}
+
+// ------------------------------------------------------------------ ImageCompare
+int fhp_image_compare(int metric, const float* d_a, const float* d_b, uint32_t W, uint32_t H, int alpha, float* d_error_map,
+                      float* d_heat_map, double* h_error, FhpStream stream) {
+    if (!d_a || !d_b || !h_error) return fail(FHP_EINVAL, "null argument");
+    if (metric < kMetricMSE || metric > kMetricMAPE) return fail(FHP_EINVAL, "metric: 0 mse, 1 rmse, 2 mae, 3 mape");
+    if (W == 0 || H == 0) return fail(FHP_EINVAL, "empty image");
+    if (d_heat_map && !d_error_map) return fail(FHP_EINVAL, "a heat map needs the error map");
+    hipStream_t st = hipStream_t(stream);
+    const uint64_t n = uint64_t(W) * H;
+    DevBuf scratch;
+    HIP_OR_FAIL(scratch.alloc(size_t(comparePartials(n) + 1) * sizeof(CompareResult)));
+    CompareResult* part = scratch.as<CompareResult>();
+    CompareResult* res = part + comparePartials(n);
+    HIP_OR_FAIL(launchCompare(metric, reinterpret_cast<const float4*>(d_a), reinterpret_cast<const float4*>(d_b), n,
+                              alpha ? 4 : 3, d_error_map, part, res, st));
+    if (d_heat_map) HIP_OR_FAIL(launchHeatMap(d_error_map, n, res, reinterpret_cast<float4*>(d_heat_map), st));
+    CompareResult r{};
+    HIP_OR_FAIL(hipMemcpyAsync(&r, res, sizeof(r), hipMemcpyDeviceToHost, st));
+    HIP_OR_FAIL(hipStreamSynchronize(st));
+    *h_error = r.sum / double(n);
     return FHP_OK;
 }
 

@@ -158,6 +158,38 @@ hipError_t launchWarGeom(const SceneDev& s, const GeomArgs& a, const TraceScratc
 hipError_t launchAggregate(const float* tmp, uint32_t hashSize, uint32_t dim, float* out, hipStream_t stream);
 hipError_t launchFill(float* p, uint64_t n, float v, hipStream_t stream);
 
+// AccumulatePass (Accumulate.cs.slang, AccumulatePass.cpp). Precision modes use the values of
+// AccumulatePass::Precision (AccumulatePass.h:70-75); input formats those of _INPUT_FORMAT
+// (Accumulate.cs.slang:40-43).
+constexpr int kAccumDouble = 0, kAccumSingle = 1, kAccumSingleCompensated = 2;
+constexpr int kAccumInputFloat = 0, kAccumInputUint = 1, kAccumInputSint = 2;
+constexpr uint32_t kAccumEnabled = 1, kAccumMovingAverage = 2, kAccumFirst = 4;
+struct AccumArgs {
+    const void* in;   // n RGBA texels (float, uint32 or int32)
+    float4* out;      // n RGBA float
+    float4* sum;      // Single / SingleCompensated running sum (or moving average)
+    float4* corr;     // SingleCompensated correction term
+    double2* sumD;    // Double: 2 double2 per pixel
+    uint64_t n;
+    uint32_t count;   // frames accumulated before this one
+    uint32_t flags;   // kAccum* flags
+    int mode, inputFormat;
+};
+hipError_t launchAccumulate(const AccumArgs& a, hipStream_t stream);
+
+// ImageCompare metrics in the order of its errorMetrics table (ImageCompare.cpp:256-261).
+constexpr int kMetricMSE = 0, kMetricRMSE = 1, kMetricMAE = 2, kMetricMAPE = 3;
+constexpr uint32_t kComparePartialsMax = 2048;
+struct CompareResult {
+    double sum;       // sum of the per-pixel errors
+    float minErr, maxErr;  // min / max of the float per-pixel errors (NaN left out)
+};
+uint32_t comparePartials(uint64_t n);
+// part: comparePartials(n) entries; res: one entry. errMap may be null.
+hipError_t launchCompare(int metric, const float4* a, const float4* b, uint64_t n, int nch, float* errMap,
+                         CompareResult* part, CompareResult* res, hipStream_t stream);
+hipError_t launchHeatMap(const float* err, uint64_t n, const CompareResult* res, float4* out, hipStream_t stream);
+
 int numCUs();
 
 }  // namespace fhp

@@ -67,6 +67,9 @@ class Scene:
         _lib.check(L.fhp_scene_create(C.byref(d), C.byref(h)))
         self.handle = h
         self.materials = a["mats"].copy()
+        # Incremented by every parameter change (materials, textures, vertices, sampling freeze);
+        # AccumulatePass restarts when it moves (the scene updates of AccumulatePass.cpp:141-147).
+        self.version = 0
 
     def close(self):
         if getattr(self, "handle", None):
@@ -96,6 +99,7 @@ class Scene:
             raise ValueError(f"expected materials of shape {self.materials.shape}")
         _lib.check(_lib.load().fhp_scene_set_materials(self.handle, m.ctypes.data))
         self.materials = m.copy()
+        self.version += 1
 
     # Scene::getMeshVerticesAndIndices / setMeshVertices (Scene.cpp:4314-4336)
     def mesh_vertex_range(self, mesh_id: int) -> tuple[int, int]:
@@ -128,16 +132,19 @@ class Scene:
         self._arrays["pos"][v0:v1] = p
         if n is not None:
             self._arrays["nrm"][v0:v1] = n
+        self.version += 1
 
     def set_texels(self, texels):
         """Copies a device tensor of texels (same layout as SceneData.texels) into the scene."""
         torch = _torch()
         t = texels.detach().to(device=_device(), dtype=torch.float32).contiguous()
         _lib.check(_lib.load().fhp_scene_set_texels(self.handle, t.data_ptr(), t.numel(), _lib.stream_ptr()))
+        self.version += 1
 
     def freeze_sampling(self, enable: bool = True):
         """Finite-difference mode: keep sampling/pdfs at the current parameters (see fhp.h)."""
         _lib.check(_lib.load().fhp_scene_freeze_sampling(self.handle, int(enable), _lib.stream_ptr()))
+        self.version += 1
 
     def light_bvh(self):
         i = self.info()
@@ -329,26 +336,80 @@ class WARDiffPathTracer(_NativePass):
         return st
 
 
-class AccumulatePass(RenderPass):
-    """Running average over frames, Single precision mode (Accumulate.cs.slang:76-114)."""
+class AccumulatePass(_NativePass):
+    """AccumulatePass (Source/RenderPasses/AccumulatePass/AccumulatePass.cpp) on the GPU.
+
+    Properties: enabled, autoReset, precisionMode ('Double' | 'Single' | 'SingleCompensated'),
+    maxFrameCount (0 = no limit), overflowMode ('Stop' | 'Reset' | 'EMA'); the deprecated
+    'enableAccumulation' is accepted as the reference does. `accumulate(frame)` adds one
+    [H, W, 4] frame (float32, int32 or uint32) and returns the accumulated [H, W, 4] float32
+    image, a buffer the pass keeps (with overflowMode 'Stop' it keeps its last contents once
+    maxFrameCount frames are in). With autoReset, a change of the scene's parameters (materials,
+    textures, mesh vertices) since the last frame restarts the accumulation, as scene updates do
+    in AccumulatePass::execute (:129-155); `refresh=True` forces the same.
+    """
 
     kind = "AccumulatePass"
     outputs = ("output",)
+    _FORMATS = {"float32": 0, "uint32": 1, "int32": 2}
 
     def __init__(self, props=None, name=None):
         super().__init__(props, name)
+        self._out = None
+        self._scene_version = None
+
+    @staticmethod
+    def _encode(key, value):
+        if key == "precisionMode" and isinstance(value, str):
+            return {"Double": 0, "Single": 1, "SingleCompensated": 2}[value]
+        if key == "overflowMode" and isinstance(value, str):
+            return {"Stop": 0, "Reset": 1, "EMA": 2}[value]
+        return float(value)
+
+    def set_scene(self, scene):
+        """A new scene restarts accumulation (AccumulatePass::setScene, :351-357)."""
+        super().set_scene(scene)
+        self._scene_version = scene.version if scene is not None else None
         self.reset()
 
     def reset(self):
-        self._sum = None
-        self._n = 0
+        _lib.check(_lib.load().fhp_accumulate_reset(self.handle))
 
-    def accumulate(self, cur):
-        if not self.properties.get("enabled", True):
-            return cur
-        self._sum = cur.clone() if self._sum is None else self._sum + cur
-        self._n += 1
-        return self._sum * (1.0 / self._n)
+    @property
+    def enabled(self) -> bool:
+        return self.get_property("enabled") != 0
+
+    @enabled.setter
+    def enabled(self, value: bool):
+        self.set_property("enabled", int(bool(value)))
+
+    @property
+    def frame_count(self) -> int:
+        n = C.c_uint32()
+        _lib.check(_lib.load().fhp_accumulate_frame_count(self.handle, C.byref(n)))
+        return n.value
+
+    @frame_count.setter
+    def frame_count(self, value):
+        pass  # the count lives in the native pass; RenderPass.__init__ assigns it
+
+    def accumulate(self, cur, refresh: bool = False):
+        torch = _torch()
+        fmt = self._FORMATS.get(str(cur.dtype).replace("torch.", ""))
+        if fmt is None:
+            raise ValueError("AccumulatePass input must be float32, uint32 or int32")
+        if cur.dim() != 3 or cur.shape[2] != 4:
+            raise ValueError("AccumulatePass input must be [H, W, 4]")
+        cur = cur.contiguous()
+        h, w = cur.shape[0], cur.shape[1]
+        if self._out is None or self._out.shape[:2] != (h, w):
+            self._out = torch.zeros((h, w, 4), dtype=torch.float32, device=cur.device)
+        if self.scene is not None and self._scene_version != self.scene.version:
+            self._scene_version = self.scene.version
+            refresh = True
+        _lib.check(_lib.load().fhp_accumulate(self.handle, cur.data_ptr(), fmt, w, h, self._out.data_ptr(), int(bool(refresh)),
+                                              _lib.stream_ptr()))
+        return self._out
 
 
 class SceneGradients:
@@ -463,7 +524,13 @@ class RenderGraph:
         vals: dict[str, object] = {}
         for n in self._order():
             p = self.passes[n]
-            if isinstance(p, _NativePass):
+            if isinstance(p, AccumulatePass):
+                if p.scene is not scene:
+                    p.set_scene(scene)
+                srcs = [s for s, d in self.edges if d == f"{n}.input"]
+                if srcs:
+                    vals[f"{n}.output"] = p.accumulate(vals[srcs[0]])
+            elif isinstance(p, _NativePass):
                 if p.scene is not scene:
                     p.set_scene(scene)
                 if isinstance(p, WARDiffPathTracer) and p.run_backward:
@@ -474,10 +541,6 @@ class RenderGraph:
                     vals[f"{n}.color"] = p.render(width, height, 1)
                     if isinstance(p, WARDiffPathTracer):
                         vals[f"{n}.dColor"] = torch.zeros((height, width, 4), device=_device())
-            elif isinstance(p, AccumulatePass):
-                srcs = [s for s, d in self.edges if d == f"{n}.input"]
-                if srcs:
-                    vals[f"{n}.output"] = p.accumulate(vals[srcs[0]])
         self._outputs = vals
         return vals
 

@@ -189,6 +189,35 @@ int fhp_grads_copy(FhpGrads grads, int which, float* d_dst, uint64_t num_floats,
 int fhp_pass_backward(FhpPass pass, FhpScene scene, uint32_t width, uint32_t height, uint32_t spp, uint32_t frame,
                       const float* d_dLdI, FhpGrads grads, uint64_t* h_stats, FhpStream stream);
 
+/* AccumulatePass: a pass of kind "AccumulatePass" (fhp_pass_create) with the reference's
+ * properties enabled (1), autoReset (1), precisionMode (0 Double, 1 Single [default],
+ * 2 SingleCompensated), maxFrameCount (0 = no limit; above 0 the Single and Double modes
+ * average with moving-average updates), overflowMode (0 Stop, 1 Reset, 2 EMA), and the
+ * deprecated alias enableAccumulation. The output is RGBA32F at the input's size (the
+ * reference's outputFormat / outputSize / fixedOutputSize are not properties here).
+ * fhp_accumulate adds one frame d_input (height * width RGBA texels: input_format 0 float32,
+ * 1 uint32, 2 int32) into the running average and writes it to d_output (height * width * 4
+ * floats). refresh != 0 reports a change that restarts accumulation when autoReset is 1 (the
+ * render-pass refresh flags or scene updates of AccumulatePass.cpp:129-155). With
+ * overflowMode Stop and maxFrameCount frames accumulated, d_output is left untouched.
+ * Replaces AccumulatePass::execute / accumulate / reset (AccumulatePass.cpp:127-292, :375-378)
+ * and the accumulate* kernels (Accumulate.cs.slang:76-190). */
+int fhp_accumulate(FhpPass pass, const void* d_input, int input_format, uint32_t width, uint32_t height, float* d_output,
+                   uint32_t refresh, FhpStream stream);
+int fhp_accumulate_reset(FhpPass pass);
+/* Frames accumulated so far (mFrameCount, shown by AccumulatePass::renderUI, :346). */
+int fhp_accumulate_frame_count(FhpPass pass, uint32_t* count);
+
+/* ImageCompare: error of image b against image a (height * width RGBA floats each, device),
+ * metric 0 mse, 1 rmse (relative), 2 mae, 3 mape, over RGB or RGBA (alpha != 0): the mean over
+ * pixels of each pixel's mean channel error, returned in *h_error. d_error_map (optional,
+ * height * width floats) receives the per-pixel errors; d_heat_map (optional, needs the error
+ * map, height * width * 4 floats) their colour-ramp heat map. Synchronises the stream.
+ * Replaces the metrics, compare and generateHeatMap of Source/Tools/ImageCompare/ImageCompare.cpp
+ * (:191-270); file loading and the threshold test are the caller's (falcor_amd.image_compare). */
+int fhp_image_compare(int metric, const float* d_a, const float* d_b, uint32_t width, uint32_t height, int alpha,
+                      float* d_error_map, float* d_heat_map, double* h_error, FhpStream stream);
+
 #ifdef __cplusplus
 }
 #endif

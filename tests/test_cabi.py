@@ -123,3 +123,64 @@ def test_product_never_imports_the_oracle():
     for p in (ROOT / "falcor_amd").rglob("*.py"):
         src = p.read_text()
         assert not re.search(r"^\s*(import|from)\s+oracle\b", src, re.M), p
+
+
+def _acc_pass(lib, props):
+    keys = (C.c_char_p * max(len(props), 1))(*[k.encode() for k in props])
+    vals = (C.c_double * max(len(props), 1))(*[float(v) for v in props.values()])
+    h = C.c_void_p()
+    rc = lib.fhp_pass_create(b"AccumulatePass", len(props), keys, vals, C.byref(h))
+    return rc, h
+
+
+def test_accumulate_pass_properties():
+    """Defaults of AccumulatePass.h:135-145; enum ranges checked; enableAccumulation is the
+    deprecated alias of enabled (AccumulatePass.cpp:87-92)."""
+    from falcor_amd import _lib
+    lib = _lib.load()
+    rc, h = _acc_pass(lib, {})
+    assert rc == 0
+    v = C.c_double()
+    want = {b"enabled": 1, b"autoReset": 1, b"precisionMode": 1, b"maxFrameCount": 0, b"overflowMode": 0}
+    for k, x in want.items():
+        assert lib.fhp_pass_get_property(h, k, C.byref(v)) == 0 and v.value == x, k
+    n = C.c_uint32(7)
+    assert lib.fhp_accumulate_frame_count(h, C.byref(n)) == 0 and n.value == 0
+    assert lib.fhp_pass_set_property(h, b"precisionMode", 3.0) == -1
+    assert "precisionMode" in lib.fhp_last_error().decode()
+    assert lib.fhp_pass_set_property(h, b"overflowMode", -1.0) == -1
+    assert lib.fhp_pass_set_property(h, b"maxFrameCount", 2.5) == -1
+    assert lib.fhp_pass_set_property(h, b"precisionMode", 2.0) == 0
+    # An AccumulatePass does not render and a path tracer does not accumulate.
+    assert lib.fhp_pass_execute(h, None, 4, 4, 1, 0, C.c_void_p(16), None, None) == -1
+    assert "fhp_accumulate" in lib.fhp_last_error().decode()
+    lib.fhp_pass_destroy(h)
+
+    rc, h = _acc_pass(lib, {"enableAccumulation": 0})
+    assert rc == 0 and "deprecated" in lib.fhp_last_warning().decode()
+    assert lib.fhp_pass_get_property(h, b"enabled", C.byref(v)) == 0 and v.value == 0
+    lib.fhp_pass_destroy(h)
+    rc, h = _acc_pass(lib, {"enableAccumulation": 0, "enabled": 1})
+    assert lib.fhp_pass_get_property(h, b"enabled", C.byref(v)) == 0 and v.value == 1
+    lib.fhp_pass_destroy(h)
+    rc, h = _acc_pass(lib, {"overflowMode": 7})
+    assert rc == -1 and "overflowMode" in lib.fhp_last_error().decode()
+
+    keys = (C.c_char_p * 1)(b"maxBounces")
+    vals = (C.c_double * 1)(3.0)
+    assert lib.fhp_pass_create(b"MinimalPathTracer", 1, keys, vals, C.byref(h)) == 0
+    assert lib.fhp_accumulate(h, C.c_void_p(16), 0, 4, 4, C.c_void_p(16), 0, None) == -1
+    assert "AccumulatePass" in lib.fhp_last_error().decode()
+    lib.fhp_pass_destroy(h)
+
+
+def test_image_compare_argument_errors():
+    from falcor_amd import _lib
+    lib = _lib.load()
+    e = C.c_double()
+    p = C.c_void_p(16)
+    assert lib.fhp_image_compare(4, p, p, 4, 4, 0, None, None, C.byref(e), None) == -1
+    assert "metric" in lib.fhp_last_error().decode()
+    assert lib.fhp_image_compare(0, p, p, 0, 4, 0, None, None, C.byref(e), None) == -1
+    assert lib.fhp_image_compare(0, p, p, 4, 4, 0, None, p, C.byref(e), None) == -1
+    assert "heat map" in lib.fhp_last_error().decode()

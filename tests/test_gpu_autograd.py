@@ -60,7 +60,7 @@ def test_material_optimization_recovers_albedos(gpu):
         opt.step()
         with torch.no_grad():
             params[:, 0:3].clamp_(0.0, 1.0)
-        errs.append(float((params[:7, 0:3] - true_p[:7, 0:3]).abs().mean()))
+        errs.append(float((params.detach()[:7, 0:3] - true_p[:7, 0:3]).abs().mean()))
     print("albedo error", [round(x, 4) for x in errs[::5]], errs[-1], flush=True)
     assert errs[-1] < 0.6 * float((init[:7, 0:3] - true_p[:7, 0:3]).abs().mean())
 
