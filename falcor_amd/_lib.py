@@ -39,7 +39,11 @@ class SceneDesc(C.Structure):
         ("h_camera", C.c_void_p),
         ("num_meshes", C.c_uint32),
         ("h_mesh_vertex_offsets", C.c_void_p),
+        ("flags", C.c_uint32),
     ]
+
+
+SCENE_USE_COMPRESSED_HIT_INFO = 0x8000
 
 
 class SceneInfo(C.Structure):
@@ -50,6 +54,9 @@ class SceneInfo(C.Structure):
         ("num_active_lights", C.c_uint32), ("light_bvh_nodes", C.c_uint32), ("num_meshes", C.c_uint32),
         ("bvh_sah_cost", C.c_double),
         ("device_bytes", C.c_uint64), ("camera_basis", C.c_float * 9),
+        ("hit_info_compressed", C.c_uint32), ("hit_info_type_bits", C.c_uint32),
+        ("hit_info_instance_bits", C.c_uint32), ("hit_info_primitive_bits", C.c_uint32),
+        ("meshes_contiguous", C.c_uint32),
     ]
 
 
@@ -99,6 +106,9 @@ _SIGS = {
     "fhp_accumulate": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_uint32, C.c_uint32, C.c_void_p, C.c_uint32,
                                  C.c_void_p]),
     "fhp_accumulate_reset": (C.c_int, [C.c_void_p]),
+    "fhp_vbuffer_execute": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_void_p, C.c_void_p, C.c_void_p]),
+    "fhp_pass_set_input": (C.c_int, [C.c_void_p, C.c_char_p, C.c_void_p]),
     "fhp_accumulate_frame_count": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
     "fhp_image_compare": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_int, C.c_void_p,
                                     C.c_void_p, C.POINTER(C.c_double), C.c_void_p]),

@@ -3,10 +3,12 @@
 #include "fhp_internal.h"
 #include "scene_build.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <map>
 #include <memory>
+#include <random>
 #include <string>
 #include <vector>
 
@@ -59,6 +61,7 @@ struct DevBuf {
 struct FhpScene_ {
     HostScene h;
     DevBuf bvh, triPos, idx, vtx, triMat, mats, texels, texDesc, lights, active, triToLight, lbvh, lbvhTri, lbvhMask;
+    DevBuf meshTri;  // first triangle of each mesh (PackedHitInfo instance ids)
     DevBuf ovf;  // traversal overflow stack
     DevBuf matsS, texelsS;  // frozen sampling parameters (finite-difference mode)
     TraceScratch ts{};      // default traversal configuration (ray queries)
@@ -66,6 +69,83 @@ struct FhpScene_ {
     SceneDev dev{};
     uint64_t deviceBytes = 0;
 };
+
+namespace fhp {
+// Camera jitter patterns of VBufferRT (GBufferBase.cpp:202-218): none (Center), the 8 standard
+// Direct3D sample positions (DxSamplePattern.cpp), Halton bases 2 and 3
+// (HaltonSamplePattern.cpp), or stratified bins visited in a shuffled order per round
+// (StratifiedSamplePattern.cpp: std::mt19937 with its default seed, std::shuffle and
+// std::uniform_real_distribution<float>, so a libstdc++ build reproduces the reference's sequence).
+struct JitterPattern {
+    int kind = 0;  // 0 Center, 1 DirectX, 2 Halton, 3 Stratified
+    uint32_t count = 0, cur = 0, binsX = 1, binsY = 1;
+    std::vector<uint32_t> perm;
+    std::mt19937 rng;
+
+    // Returns the sample count the pattern uses (the reference stores it back into sampleCount).
+    uint32_t create(int k, uint32_t sampleCount) {
+        kind = k;
+        cur = 0;
+        rng = std::mt19937();
+        switch (k) {
+        case 1: count = 8; break;
+        case 2: count = sampleCount; break;
+        case 3: {
+            count = std::min(std::max(sampleCount, 1u), 1024u);
+            binsX = uint32_t(std::sqrt(double(count)));
+            binsY = count / binsX;
+            while (binsX * binsY != count) {
+                binsX++;
+                binsY = count / binsX;
+            }
+            perm.resize(count);
+            for (uint32_t i = 0; i < count; i++) perm[i] = i;
+            break;
+        }
+        default: count = sampleCount; break;
+        }
+        return count;
+    }
+    static float halton(uint32_t index, uint32_t base) {
+        float result = 0.f, factor = 1.f;
+        for (; index > 0; index /= base) {
+            factor /= float(base);
+            result += factor * float(index % base);
+        }
+        return result;
+    }
+    // Next jitter sample in [-0.5, 0.5)^2 (pixel units), or (0, 0) for Center.
+    void next(float& x, float& y) {
+        x = y = 0.f;
+        if (kind == 1) {
+            static const float kDx[8][2] = {{1.f / 16.f, -3.f / 16.f}, {-1.f / 16.f, 3.f / 16.f}, {5.f / 16.f, 1.f / 16.f},
+                                            {-3.f / 16.f, -5.f / 16.f}, {-5.f / 16.f, 5.f / 16.f}, {-7.f / 16.f, -1.f / 16.f},
+                                            {3.f / 16.f, 7.f / 16.f}, {7.f / 16.f, -7.f / 16.f}};
+            x = kDx[cur % 8][0];
+            y = kDx[cur % 8][1];
+            cur++;
+        } else if (kind == 2) {
+            const float hx = halton(cur, 2) + 0.5f, hy = halton(cur, 3) + 0.5f;
+            cur++;
+            if (count != 0) cur %= count;
+            x = (hx - std::floor(hx)) - 0.5f;
+            y = (hy - std::floor(hy)) - 0.5f;
+        } else if (kind == 3) {
+            std::uniform_real_distribution<float> dist;
+            if (cur == 0) std::shuffle(perm.begin(), perm.end(), rng);
+            const uint32_t bin = perm[cur];
+            const uint32_t i = bin % binsX, j = bin / binsX;
+            cur = (cur + 1) % count;
+            const float ux = dist(rng);
+            const float px = (float(i) + ux) / float(binsX);
+            const float uy = dist(rng);
+            const float py = (float(j) + uy) / float(binsY);
+            x = px - 0.5f;
+            y = py - 0.5f;
+        }
+    }
+};
+}  // namespace fhp
 
 struct FhpPass_ {
     std::string kind;
@@ -80,6 +160,11 @@ struct FhpPass_ {
     DevBuf accSum, accCorr;
     uint32_t accCount = 0, accW = 0, accH = 0;
     int accMode = -1;  // precision mode the sums are laid out for (-1: none yet)
+    // VBufferRT camera jitter pattern (created from samplePattern / sampleCount).
+    JitterPattern jitter;
+    // Bound input channels of MinimalPathTracer (MinimalPathTracer.cpp:48-53): "vbuffer", "viewW".
+    const uint32_t* inVbuf = nullptr;
+    const float4* inViewW = nullptr;
     // Captured bounce loop of the forward pass (property useGraph), keyed by everything the
     // captured kernels read by value.
     hipGraphExec_t graph = nullptr;
@@ -142,6 +227,29 @@ const std::map<std::string, double>& accDefaults() {
     static const std::map<std::string, double> d = {
         {"enabled", 1}, {"autoReset", 1}, {"precisionMode", kAccumSingle}, {"maxFrameCount", 0}, {"overflowMode", 0}};
     return d;
+}
+const std::map<std::string, double>& vbufDefaults() {
+    // GBufferBase.h:96-106 and VBufferRT.h:74-76: samplePattern (Center), sampleCount,
+    // useAlphaTest, adjustShadingNormals, forceCullMode, cull (Back), useTraceRayInline, useDOF,
+    // plus the traversal launch properties.
+    static const std::map<std::string, double> d = {
+        {"samplePattern", 0}, {"sampleCount", 16}, {"useAlphaTest", 1}, {"adjustShadingNormals", 1},
+        {"forceCullMode", 0}, {"cull", 2}, {"useTraceRayInline", 0}, {"useDOF", 1},
+        {"traceBlockSize", kDefaultBlock}, {"traceStackDepth", kDefaultLds}, {"traceBlocksPerCU", kDefaultBlocksPerCU}};
+    return d;
+}
+// VBufferRT property values this build renders: the four sample patterns and no forced face
+// culling.
+std::string vbufCheck(const std::string& k, double v, const std::map<std::string, double>& props) {
+    if (k == "samplePattern" && !(v == 0 || v == 1 || v == 2 || v == 3))
+        return "samplePattern: 0 Center, 1 DirectX, 2 Halton, 3 Stratified";
+    if (k == "sampleCount" && !(v >= 0 && v <= 4294967295.0 && double(uint64_t(v)) == v))
+        return "sampleCount: a non-negative integer";
+    const double force = k == "forceCullMode" ? v : props.at("forceCullMode");
+    const double cull = k == "cull" ? v : props.at("cull");
+    if ((k == "forceCullMode" || k == "cull") && force != 0 && cull != 0)
+        return "forceCullMode with cull Front/Back is not supported (rays hit both faces)";
+    return "";
 }
 // Validates one AccumulatePass property value; returns an error message or "".
 std::string accCheck(const std::string& k, double v) {
@@ -261,6 +369,7 @@ int buildAndUpload(FhpScene_* s) {
     HIP_OR_FAIL(s->lbvh.upload(h.lbvh));
     HIP_OR_FAIL(s->lbvhTri.upload(h.lbvhTri));
     HIP_OR_FAIL(s->lbvhMask.upload(h.lbvhMask));
+    HIP_OR_FAIL(s->meshTri.upload(h.meshTriOff));
     // Overflow stack for the persistent traversal grid, sized for the smallest LDS stack and the
     // largest grid any configuration launches.
     s->stackNeed = h.stackNeed + 1;
@@ -294,6 +403,11 @@ int buildAndUpload(FhpScene_* s) {
     dv.cam.W = mk3(h.camW[0], h.camW[1], h.camW[2]);
     dv.cam.nearZ = h.cam[12];
     dv.cam.farZ = h.cam[13];
+    dv.meshTriOff = h.meshTriContiguous ? s->meshTri.as<uint32_t>() : nullptr;
+    dv.nMeshes = uint32_t(h.meshOff.size() - 1);
+    dv.hit = HitInfoLayout{h.hitCompressed, h.hitTypeBits, h.hitInstBits, h.hitPrimBits};
+    dv.viewProjZ = make_float4(h.viewProj[8], h.viewProj[9], h.viewProj[10], h.viewProj[11]);
+    dv.viewProjW = make_float4(h.viewProj[12], h.viewProj[13], h.viewProj[14], h.viewProj[15]);
     dv.nTris = h.nT;
     dv.nLights = h.nLights;
     dv.nActive = uint32_t(h.activeLights.size());
@@ -365,8 +479,10 @@ int fhp_scene_create(const FhpSceneDesc* d, FhpScene* out) {
     } else {
         h.meshOff = {0u, h.nV};
     }
+    if (d->flags & ~uint32_t(FHP_SCENE_USE_COMPRESSED_HIT_INFO)) return fail(FHP_EINVAL, "unknown scene flags");
     try {
         computeCamera(h);
+        computeHitInfo(h, (d->flags & FHP_SCENE_USE_COMPRESSED_HIT_INFO) != 0);
     } catch (const std::exception& e) {
         return fail(FHP_EBUILD, e.what());
     }
@@ -400,6 +516,11 @@ int fhp_scene_info(FhpScene s, FhpSceneInfo* o) {
     o->bvh_nodes = h.bvhNodes; o->bvh_leaves = h.bvhLeaves; o->bvh_depth = h.bvhDepth; o->stack_need = h.stackNeed;
     o->num_lights = h.nLights; o->num_active_lights = uint32_t(h.activeLights.size()); o->light_bvh_nodes = h.nLbvh;
     o->num_meshes = uint32_t(h.meshOff.size() - 1);
+    o->hit_info_compressed = h.hitCompressed;
+    o->hit_info_type_bits = h.hitTypeBits;
+    o->hit_info_instance_bits = h.hitInstBits;
+    o->hit_info_primitive_bits = h.hitPrimBits;
+    o->meshes_contiguous = h.meshTriContiguous;
     o->bvh_sah_cost = h.bvhSah;
     o->device_bytes = s->deviceBytes;
     for (int i = 0; i < 3; i++) {
@@ -473,6 +594,7 @@ int fhp_pass_create(const char* kind, int np, const char* const* keys, const dou
     if (k == "MinimalPathTracer") defs = &mptDefaults();
     else if (k == "WARDiffPathTracer") defs = &warDefaults();
     else if (k == "AccumulatePass") defs = &accDefaults();
+    else if (k == "VBufferRT") defs = &vbufDefaults();
     else return fail(FHP_EINVAL, "unknown pass kind '" + k + "'");
     std::unique_ptr<FhpPass_> p(new FhpPass_());
     p->kind = k;
@@ -499,6 +621,10 @@ int fhp_pass_create(const char* kind, int np, const char* const* keys, const dou
             if (!msg.empty()) return fail(FHP_EINVAL, msg);
             hasEnabled |= key == "enabled";
         }
+        if (k == "VBufferRT") {
+            const std::string msg = vbufCheck(key, values[i], p->props);
+            if (!msg.empty()) return fail(FHP_EINVAL, msg);
+        }
         p->props[key] = values[i];
         if (key == "fixedSeed") p->props["useFixedSeed"] = 1;
     }
@@ -507,6 +633,8 @@ int fhp_pass_create(const char* kind, int np, const char* const* keys, const dou
         *out = p.release();
         return FHP_OK;
     }
+    if (k == "VBufferRT")
+        p->props["sampleCount"] = p->jitter.create(int(p->get("samplePattern")), uint32_t(p->get("sampleCount")));
     if (p->get("maxBounces") < 0 || p->get("maxBounces") > kMaxBounceSlots - 2)
         return fail(FHP_EINVAL, "maxBounces out of range [0, 28]");
     if (!validLaunch(p->get("traceBlockSize"), p->get("traceStackDepth"), p->get("traceBlocksPerCU")))
@@ -536,6 +664,16 @@ int fhp_pass_set_property(FhpPass p, const char* key, double v) {
         p->props[k] = v;
         return FHP_OK;
     }
+    if (p->kind == "VBufferRT") {
+        const std::string msg = vbufCheck(k, v, p->props);
+        if (!msg.empty()) return fail(FHP_EINVAL, msg);
+        if (k == "samplePattern" || k == "sampleCount") {
+            // A new pattern generator (GBufferBase::updateSamplePattern, GBufferBase.cpp:231-236).
+            p->props[k] = v;
+            p->props["sampleCount"] = p->jitter.create(int(p->get("samplePattern")), uint32_t(p->get("sampleCount")));
+            return FHP_OK;
+        }
+    }
     if (k == "maxBounces" && (v < 0 || v > kMaxBounceSlots - 2)) return fail(FHP_EINVAL, "maxBounces out of range");
     if (k == "traceBlockSize" || k == "traceStackDepth" || k == "traceBlocksPerCU") {
         const double b = k == "traceBlockSize" ? v : p->get("traceBlockSize");
@@ -561,6 +699,7 @@ int fhp_pass_execute(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t spp
                      uint64_t* h_stats, FhpStream stream) {
     if (!p || !d_color) return fail(FHP_EINVAL, "null argument");
     if (p->kind == "AccumulatePass") return fail(FHP_EINVAL, "AccumulatePass runs through fhp_accumulate");
+    if (p->kind == "VBufferRT") return fail(FHP_EINVAL, "VBufferRT runs through fhp_vbuffer_execute");
     if (W == 0 || H == 0 || spp == 0) return fail(FHP_EINVAL, "empty frame");
     if (W > 65536 || H > 65536) return fail(FHP_EINVAL, "frame dimension above 65536");
     hipStream_t st = hipStream_t(stream);
@@ -593,6 +732,10 @@ int fhp_pass_execute(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t spp
         a.pathQ[0] = p->pathQ0.as<uint32_t>(); a.pathQ[1] = p->pathQ1.as<uint32_t>();
         a.hitQ = p->hitQ.as<uint4>(); a.hitIdx = p->hitIdx.as<uint32_t>(); a.shQ = p->shQ.as<float4>(); a.pixHit = p->pixHit.as<uint4>();
         a.counters = p->counters.as<uint32_t>();
+        if (p->inVbuf && !s->h.meshTriContiguous)
+            return fail(FHP_ESTATE, "a V-buffer input needs each mesh's triangles to be contiguous");
+        a.vbuf = p->inVbuf;
+        a.viewW = p->inViewW;
         if ((rc = prepareTimer(p))) return rc;
         const bool graph = p->get("useGraph") != 0 && !count && !p->timer.on;
         if (!graph) {
@@ -972,6 +1115,38 @@ int fhp_image_compare(int metric, const float* d_a, const float* d_b, uint32_t W
     HIP_OR_FAIL(hipMemcpyAsync(&r, res, sizeof(r), hipMemcpyDeviceToHost, st));
     HIP_OR_FAIL(hipStreamSynchronize(st));
     *h_error = r.sum / double(n);
+    return FHP_OK;
+}
+
+// ------------------------------------------------------------------ V-buffer
+int fhp_pass_set_input(FhpPass p, const char* channel, const void* d_data) {
+    if (!p || !channel) return fail(FHP_EINVAL, "null argument");
+    if (p->kind != "MinimalPathTracer") return fail(FHP_EINVAL, "input channels are bound on a MinimalPathTracer");
+    const std::string c(channel);
+    if (c == "vbuffer") p->inVbuf = static_cast<const uint32_t*>(d_data);
+    else if (c == "viewW") p->inViewW = static_cast<const float4*>(d_data);
+    else return fail(FHP_EINVAL, "unknown input channel '" + c + "' (vbuffer, viewW)");
+    return FHP_OK;
+}
+
+int fhp_vbuffer_execute(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t* d_vbuffer, float* d_depth, float* d_viewW,
+                        float* d_mask, float* h_jitter, FhpStream stream) {
+    if (!p || !s || !d_vbuffer) return fail(FHP_EINVAL, "null argument");
+    if (p->kind != "VBufferRT") return fail(FHP_EINVAL, "fhp_vbuffer_execute needs a VBufferRT pass");
+    if (W == 0 || H == 0) return fail(FHP_EINVAL, "empty frame");
+    if (W > 65536 || H > 65536 || uint64_t(W) * H > 0xffffffffull) return fail(FHP_EINVAL, "frame too large");
+    if (!s->h.meshTriContiguous) return fail(FHP_ESTATE, "V-buffer encoding needs each mesh's triangles to be contiguous");
+    // Camera::beginFrame (Camera.cpp:51-58): one jitter sample per frame, scaled by 1 / frame size.
+    float jx, jy;
+    p->jitter.next(jx, jy);
+    jx *= 1.f / float(W);
+    jy *= 1.f / float(H);
+    if (h_jitter) {
+        h_jitter[0] = jx;
+        h_jitter[1] = jy;
+    }
+    HIP_OR_FAIL(launchVBuffer(s->dev, W, H, jx, jy, d_vbuffer, d_depth, reinterpret_cast<float4*>(d_viewW), d_mask,
+                              passTrace(s, p), hipStream_t(stream)));
     return FHP_OK;
 }
 

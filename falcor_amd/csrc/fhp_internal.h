@@ -57,6 +57,11 @@ struct MptArgs {
     const SceneDev* scenes;
     uint32_t nViews;
     uint32_t nPix;        // pixels over all views (W * H for one view)
+    // Bound inputs of a single-view frame (MinimalPathTracer.cpp:48-59): the primary hits as a
+    // V-buffer of PackedHitInfo in the scene's format (replaces the primary traversal) and the
+    // view directions (viewW = -ray direction). Null when not bound.
+    const uint32_t* vbuf;
+    const float4* viewW;
 };
 
 struct TraceScratch {
@@ -112,6 +117,11 @@ struct KernelTimer {
 };
 
 // Launchers (fhp_kernels.hip). All are asynchronous on `stream`.
+// VBufferRT (VBufferRT.rt.slang, VBufferRT.slang:56-143): primary hits as PackedHitInfo (2 or 4
+// words per pixel in the scene's format), and optionally NDC depth, viewW and the hit mask.
+// (jx, jy): camera jitter of the frame (Camera.slang:68-78).
+hipError_t launchVBuffer(const SceneDev& s, uint32_t W, uint32_t H, float jx, float jy, uint32_t* vbuf, float* depth,
+                         float4* viewW, float* mask, const TraceScratch& ts, hipStream_t stream);
 hipError_t launchTraceRays(const SceneDev& s, const float4* rays, uint32_t n, uint4* hits, uint32_t* occluded, bool any,
                            const TraceScratch& ts, hipStream_t stream);
 // Full frame: frame-number store, bounce loop, resolve, optional ray counts.

@@ -99,9 +99,13 @@ __device__ __forceinline__ ViewCtx viewCtx(const SceneDev& s0, const MptArgs& a,
 }
 
 // Pinhole primary ray through the pixel centre (Camera.slang:46-77, VBufferRT.slang:56-66).
+// Camera::computeRayPinhole (Camera.slang:46-78); (jx, jy) is the camera jitter in screen units
+// (jitterX, jitterY), zero without a sample pattern.
 __device__ __forceinline__ void primaryRay(const CameraDev& c, uint32_t px, uint32_t py, uint32_t W, uint32_t H, f3& o, f3& d,
-                                           float& tmin, float& tmax) {
+                                           float& tmin, float& tmax, float jx = 0.f, float jy = 0.f) {
     float p_x = (float(px) + 0.5f) / float(W), p_y = (float(py) + 0.5f) / float(H);
+    p_x = p_x + -jx;
+    p_y = p_y + jy;
     float ndc_x = 2.f * p_x + -1.f, ndc_y = -2.f * p_y + 1.f;
     f3 dir = c.U * ndc_x + c.V * ndc_y + c.W;
     d = normalize(dir);
@@ -109,6 +113,142 @@ __device__ __forceinline__ void primaryRay(const CameraDev& c, uint32_t px, uint
     float invCos = 1.f / dot(normalize(c.W), d);
     tmin = c.nearZ * invCos;
     tmax = c.farZ * invCos;
+}
+
+// ------------------------------------------------------------------ PackedHitInfo
+// packUnorm16_unsafe (FormatConversion.slang:292-295).
+__device__ __forceinline__ uint32_t packUnorm16(float v) { return uint32_t(truncf(v * 65535.f + 0.5f)); }
+__device__ __forceinline__ uint32_t lowMask(uint32_t bits) { return uint32_t((1ull << bits) - 1ull); }
+
+// TriangleHit::pack with HitInfo::packHeader (HitInfo.slang:94-121, :462-473): type Triangle (1).
+__device__ __forceinline__ void packTriangleHit(const HitInfoLayout& L, uint32_t inst, uint32_t prim, float b1, float b2,
+                                                uint32_t w[4]) {
+    const uint32_t typeWord = 1u << (32u - L.typeBits);
+    w[0] = w[1] = w[2] = w[3] = 0u;
+    if (L.typeBits + L.instBits + L.primBits <= 32) {
+        w[0] = typeWord | (inst << L.primBits) | prim;
+    } else {
+        w[0] = typeWord | inst;
+        w[1] = prim;
+    }
+    if (L.compressed) {
+        w[1] = (packUnorm16(b2) << 16) | packUnorm16(b1);
+    } else {
+        w[2] = f2u(b1);
+        w[3] = f2u(b2);
+    }
+}
+
+// TriangleHit(PackedHitInfo) with HitInfo::unpackHeader (HitInfo.slang:94-108, :475-487). Returns
+// false unless the record is a triangle hit of an existing mesh and primitive with finite
+// barycentrics (records that do not name a triangle of this scene read as misses).
+__device__ __forceinline__ bool unpackTriangleHit(const SceneDev& s, const uint32_t* w, uint32_t& tri, float& b1, float& b2) {
+    const HitInfoLayout& L = s.hit;
+    if ((w[0] >> (32u - L.typeBits)) != 1u || !s.meshTriOff) return false;
+    uint32_t inst, prim;
+    if (L.typeBits + L.instBits + L.primBits <= 32) {
+        inst = (w[0] >> L.primBits) & lowMask(L.instBits);
+        prim = w[0] & lowMask(L.primBits);
+    } else {
+        inst = w[0] & lowMask(L.instBits);
+        prim = w[1];
+    }
+    if (inst >= s.nMeshes) return false;
+    const uint32_t t0 = s.meshTriOff[inst], t1 = s.meshTriOff[inst + 1];
+    if (prim >= t1 - t0) return false;
+    tri = t0 + prim;
+    if (L.compressed) {
+        // unpackUnorm2x16 (FormatConversion.slang:338-341)
+        b1 = float(w[1] & 0xffffu) * (1.f / 65535);
+        b2 = float(w[1] >> 16) * (1.f / 65535);
+    } else {
+        b1 = u2f(w[2]);
+        b2 = u2f(w[3]);
+    }
+    return isfinite(b1) && isfinite(b2);
+}
+
+// Mesh (geometry instance) holding global triangle tri: the last m with meshTriOff[m] <= tri.
+__device__ __forceinline__ uint32_t meshOfTriangle(const SceneDev& s, uint32_t tri) {
+    uint32_t lo = 0, hi = s.nMeshes;  // answer in [lo, hi)
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (s.meshTriOff[mid] <= tri) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// VBufferRT::writeHit / writeMiss / writeAux (VBufferRT.slang:76-143) for the pinhole camera
+// without jitter (samplePattern Center). depth = NDC z / w of the interpolated hit position.
+template <int B, int D>
+__global__ __launch_bounds__(B) void kVBuffer(SceneDev s, uint32_t W, uint32_t H, float jx, float jy, uint32_t* __restrict__ vbuf,
+                                              float* __restrict__ depth, float4* __restrict__ viewW,
+                                              float* __restrict__ mask, TraceScratch ts) {
+    __shared__ uint32_t lds[D * B];
+    TraceStack st = makeStack<B, D>(lds, ts);
+    const uint32_t n = W * H;
+    const uint32_t G = gridDim.x * B;
+    const uint32_t words = s.hit.compressed ? 2u : 4u;
+    for (uint32_t i = blockIdx.x * B + threadIdx.x; i < n; i += G) {
+        f3 o, d;
+        float tmin, tmax;
+        primaryRay(s.cam, i % W, i / W, W, H, o, d, tmin, tmax, jx, jy);
+        HitRec h;
+        const bool hit = traceBVH4<false, false>(s, o, d, tmin, tmax, st, h);
+        uint32_t w[4] = {0u, 0u, 0u, 0u};
+        float dep = 1.f, msk = 0.f;
+        if (hit) {
+            const uint32_t m = meshOfTriangle(s, h.tri);
+            packTriangleHit(s.hit, m, h.tri - s.meshTriOff[m], h.b1, h.b2, w);
+            const uint32_t* tv = s.idx + 3 * size_t(h.tri);
+            const float4 p0 = s.vtx[2 * size_t(tv[0])], p1 = s.vtx[2 * size_t(tv[1])], p2 = s.vtx[2 * size_t(tv[2])];
+            const float b0 = 1.f - h.b1 - h.b2;
+            const f3 P = mk3(p0.x, p0.y, p0.z) * b0 + mk3(p1.x, p1.y, p1.z) * h.b1 + mk3(p2.x, p2.y, p2.z) * h.b2;
+            const float4 z = s.viewProjZ, ww = s.viewProjW;
+            dep = (z.x * P.x + z.y * P.y + z.z * P.z + z.w) / (ww.x * P.x + ww.y * P.y + ww.z * P.z + ww.w);
+            msk = 1.f;
+        }
+        if (words == 2u) {
+            reinterpret_cast<uint2*>(vbuf)[i] = make_uint2(w[0], w[1]);
+        } else {
+            reinterpret_cast<uint4*>(vbuf)[i] = make_uint4(w[0], w[1], w[2], w[3]);
+        }
+        if (depth) depth[i] = dep;
+        if (viewW) viewW[i] = make_float4(-d.x, -d.y, -d.z, 0.f);
+        if (mask) mask[i] = msk;
+    }
+}
+
+hipError_t launchVBuffer(const SceneDev& s, uint32_t W, uint32_t H, float jx, float jy, uint32_t* vbuf, float* depth,
+                         float4* viewW, float* mask, const TraceScratch& ts, hipStream_t stream) {
+    const uint32_t n = W * H;
+    if (n == 0) return hipSuccess;
+    return withTraceConfig(ts, [&](auto Bc, auto Dc) {
+        constexpr int B = decltype(Bc)::value, D = decltype(Dc)::value;
+        const uint32_t blocks = std::min<uint32_t>(ts.gridBlocks, (n + B - 1) / B);
+        hipLaunchKernelGGL((kVBuffer<B, D>), dim3(blocks), dim3(B), 0, stream, s, W, H, jx, jy, vbuf, depth, viewW, mask, ts);
+        return hipGetLastError();
+    });
+}
+
+// Primary hits of a bound V-buffer: PackedHitInfo -> (triangle, 0, b1, b2) per pixel.
+__global__ void kVBufferHits(SceneDev s, MptArgs a) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.nPix) return;
+    const uint32_t words = s.hit.compressed ? 2u : 4u;
+    uint32_t w[4] = {0u, 0u, 0u, 0u};
+    if (words == 2u) {
+        const uint2 v = reinterpret_cast<const uint2*>(a.vbuf)[p];
+        w[0] = v.x;
+        w[1] = v.y;
+    } else {
+        const uint4 v = reinterpret_cast<const uint4*>(a.vbuf)[p];
+        w[0] = v.x; w[1] = v.y; w[2] = v.z; w[3] = v.w;
+    }
+    uint32_t tri;
+    float b1, b2;
+    a.pixHit[p] = unpackTriangleHit(s, w, tri, b1, b2) ? make_uint4(tri, 0u, f2u(b1), f2u(b2)) : make_uint4(kInvalid, 0u, 0u, 0u);
 }
 
 template <bool MULTI>
@@ -197,7 +337,13 @@ __global__ void kShadePrimary(SceneDev s0, MptArgs a) {
     if (h.x != kInvalid) {
         f3 o, d;
         float tmin, tmax;
-        primaryRay(*c.cam, px, py, c.W, c.H, o, d, tmin, tmax);
+        if (!MULTI && a.viewW) {
+            // getPrimaryRayDir with a bound viewW (MinimalPathTracer.rt.slang: -viewW).
+            const float4 v = a.viewW[pix];
+            d = mk3(-v.x, -v.y, -v.z);
+        } else {
+            primaryRay(*c.cam, px, py, c.W, c.H, o, d, tmin, tmax);
+        }
         Shading sd = shadingFromHit(s, h.x, u2f(h.z), u2f(h.w), d);
         Material m = loadMaterial(s, s.mats, s.texels, sd, sd, nullptr);
         Rng rng = rngSeed(px, py, c.frame * c.spp + k);
@@ -431,7 +577,7 @@ __global__ void kResolve(MptArgs a, float4* out) {
 __global__ void kRayCounts(const MptArgs a, uint32_t nb, unsigned long long* out) {
     // out[0] += closest-hit rays (primary + all scatter queues), out[1] += shadow rays.
     if (threadIdx.x == 0 && blockIdx.x == 0) {
-        unsigned long long c = (unsigned long long)a.nPix, v = 0;
+        unsigned long long c = a.vbuf ? 0ull : (unsigned long long)a.nPix, v = 0;
         for (uint32_t b = 0; b < nb; b++) c += a.counters[b];
         for (uint32_t b = 0; b <= nb; b++) v += a.counters[kCounterStride + b];
         out[0] += c;
@@ -473,7 +619,9 @@ hipError_t mptBounces(const SceneDev& s, const MptArgs& a, const TraceScratch& t
     const int T = 256;
     hipLaunchKernelGGL(kZeroU32, dim3(1), dim3(2 * kCounterStride), 0, st, a.counters, 2u * kCounterStride);
     hipLaunchKernelGGL(kZeroU32, dim3(1), dim3(kCounterStride), 0, st, a.counters + kHitCount, uint32_t(kCounterStride));
-    hipLaunchKernelGGL(kPrimaryRays<MULTI>, dim3((npix + T - 1) / T), dim3(T), 0, st, s, a);
+    const bool vbuf = !MULTI && a.vbuf;
+    if (vbuf) hipLaunchKernelGGL(kVBufferHits, dim3((npix + T - 1) / T), dim3(T), 0, st, s, a);
+    else hipLaunchKernelGGL(kPrimaryRays<MULTI>, dim3((npix + T - 1) / T), dim3(T), 0, st, s, a);
     unsigned long long* tstat = count ? d_counts + 2 : nullptr;   // closest-hit nodes, triangles
     unsigned long long* sstat = count ? d_counts + 4 : nullptr;   // shadow nodes, triangles
     const TraceScratch& tsc = ts;
@@ -517,10 +665,13 @@ hipError_t mptBounces(const SceneDev& s, const MptArgs& a, const TraceScratch& t
         tm->end(st);
         return e;
     };
-    // Primary rays: one per pixel (the reference's V-buffer), shared by the pixel's samples.
-    hipError_t e = a.separateShadow ? traceQ(a.rayQ[0], nullptr, npix, a.pixHit, (npix + tsc.block - 1) / tsc.block)
-                                    : bounce(a.rayQ[0], nullptr, npix, a.pixHit, nullptr, nullptr, nullptr, nullptr,
-                                             (npix + tsc.block - 1) / tsc.block);
+    // Primary rays: one per pixel (the reference's V-buffer), shared by the pixel's samples;
+    // with a bound V-buffer its hits are decoded instead.
+    hipError_t e = hipSuccess;
+    if (!vbuf)
+        e = a.separateShadow ? traceQ(a.rayQ[0], nullptr, npix, a.pixHit, (npix + tsc.block - 1) / tsc.block)
+                             : bounce(a.rayQ[0], nullptr, npix, a.pixHit, nullptr, nullptr, nullptr, nullptr,
+                                      (npix + tsc.block - 1) / tsc.block);
     if (e != hipSuccess) return e;
     tm->begin(KK_SHADE, st);
     hipLaunchKernelGGL(kShadePrimary<MULTI>, dim3((a.nPaths + T - 1) / T), dim3(T), 0, st, s, a);

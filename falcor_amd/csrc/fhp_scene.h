@@ -30,6 +30,16 @@ struct CameraDev {
     float nearZ, farZ;
 };
 
+// PackedHitInfo bit allocation of a scene (HitInfo.cpp:52-106, HitInfo.slang:300-312): the type in
+// the top typeBits of word 0, then the instance id above primBits of primitive index when the
+// header fits in 32 bits; otherwise word 0 holds type and instance id and word 1 the primitive.
+// Compressed (64-bit) records carry the barycentrics as two unorm16 in word 1; uncompressed
+// (128-bit) records carry them as floats in words 2 and 3.
+struct HitInfoLayout {
+    uint32_t compressed;
+    uint32_t typeBits, instBits, primBits;
+};
+
 struct SceneDev {
     const float4* bvh;
     const float4* triPos;
@@ -49,6 +59,12 @@ struct SceneDev {
     const uint2* lbvhMask;
     CameraDev cam;
     uint32_t nTris, nLights, nActive, nLbvh;
+    // Meshes as geometry instances of PackedHitInfo: mesh m owns triangles
+    // [meshTriOff[m], meshTriOff[m + 1]) (null when some mesh's triangles are not contiguous).
+    const uint32_t* meshTriOff;
+    uint32_t nMeshes;
+    HitInfoLayout hit;
+    float4 viewProjZ, viewProjW;  // rows 2 and 3 of the camera's view-projection matrix (depth = z / w)
 };
 
 // ------------------------------------------------------------------ shading data

@@ -26,6 +26,62 @@ void computeCamera(HostScene& s) {
     s.camU[0] = U.x; s.camU[1] = U.y; s.camU[2] = U.z;
     s.camV[0] = V.x; s.camV[1] = V.y; s.camV[2] = V.z;
     s.camW[0] = W.x; s.camW[1] = W.y; s.camW[2] = W.z;
+
+    // viewProj = perspective(fovY, aspect, near, far) * lookAt(pos, target, up), in float.
+    const float nearZ = c[12], farZ = c[13];
+    const f3 f = normalize(pos - target);
+    const f3 r = normalize(cross(up, f));
+    const f3 u = cross(f, r);
+    const float view[16] = {r.x, r.y, r.z, -dot(r, pos), u.x, u.y, u.z, -dot(u, pos),
+                            f.x, f.y, f.z, -dot(f, pos), 0.f, 0.f, 0.f, 1.f};
+    const float t = std::tan(fovY / 2.f);
+    float proj[16] = {};
+    proj[0] = 1.f / (aspect * t);
+    proj[5] = 1.f / t;
+    proj[10] = farZ / (nearZ - farZ);
+    proj[11] = -(farZ * nearZ) / (farZ - nearZ);
+    proj[14] = -1.f;
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) {
+            float acc = 0.f;
+            for (int k = 0; k < 4; k++) acc += proj[4 * i + k] * view[4 * k + j];
+            s.viewProj[4 * i + j] = acc;
+        }
+}
+
+void computeHitInfo(HostScene& s, bool useCompression) {
+    // allocateBits (HitInfo.cpp:44-49): bits to store values 0 .. count - 1.
+    auto bits = [](uint64_t count) -> uint32_t {
+        uint32_t b = 0;
+        while (count > 1 && (uint64_t(1) << b) < count) b++;
+        return b;
+    };
+    const uint32_t nMesh = uint32_t(s.meshOff.size() - 1);
+    s.meshTriOff.assign(nMesh + 1, 0u);
+    s.meshTriContiguous = true;
+    std::vector<uint32_t> count(nMesh, 0u);
+    uint32_t prev = 0;
+    for (uint32_t t = 0; t < s.nT; t++) {
+        const uint32_t v = s.idx[3 * size_t(t)];
+        const uint32_t m = uint32_t(std::upper_bound(s.meshOff.begin(), s.meshOff.end(), v) - s.meshOff.begin()) - 1;
+        if (m < prev) s.meshTriContiguous = false;
+        prev = m;
+        count[m]++;
+    }
+    uint32_t maxPrim = 0;
+    for (uint32_t m = 0; m < nMesh; m++) {
+        s.meshTriOff[m + 1] = s.meshTriOff[m] + count[m];
+        maxPrim = std::max(maxPrim, count[m]);
+    }
+    // HitType::Count = 7 (None, Triangle, Volume, DisplacedTriangle, Curve, reserved, SDFGrid).
+    s.hitTypeBits = bits(7);
+    s.hitInstBits = bits(nMesh);
+    s.hitPrimBits = bits(maxPrim);
+    if (s.hitPrimBits > 32 || s.hitTypeBits + s.hitInstBits > 32)
+        throw std::runtime_error("Scene requires > 64 bits for encoding hit info header.");
+    const uint32_t kCompressedTypeBits = 2;
+    s.hitCompressed = useCompression && kCompressedTypeBits + s.hitInstBits + s.hitPrimBits <= 32;
+    if (s.hitCompressed) s.hitTypeBits = kCompressedTypeBits;
 }
 
 // ------------------------------------------------------------------ BVH4 build

@@ -20,6 +20,15 @@ struct HostScene {
     float cam[14] = {};  // pos, target, up, fovY, aspect, focalDistance, nearZ, farZ
     // Camera basis (Camera.cpp:187-193).
     float camU[3], camV[3], camW[3];
+    // View-projection matrix without jitter, row-major (Camera.cpp: viewMat = matrixFromLookAt,
+    // projMat = perspective with depth in [0, 1], MatrixMath.h:553-566, :699-725).
+    float viewProj[16];
+
+    // Geometry instances (meshes) for PackedHitInfo: first triangle of each mesh (meshes + 1
+    // entries), valid when every mesh's triangles are contiguous; bit allocation (HitInfo.cpp).
+    std::vector<uint32_t> meshTriOff;
+    bool meshTriContiguous = false;
+    uint32_t hitCompressed = 0, hitTypeBits = 0, hitInstBits = 0, hitPrimBits = 0;
 
     // BVH4 (8 float4 per node) and triangles in leaf order (12 floats per triangle).
     std::vector<float> bvh;
@@ -38,6 +47,9 @@ struct HostScene {
 };
 
 void computeCamera(HostScene& s);
+// Mesh triangle ranges and the PackedHitInfo bit allocation; throws when the header does not fit
+// in 64 bits (HitInfo.cpp:83-87).
+void computeHitInfo(HostScene& s, bool useCompression);
 void buildBVH4(HostScene& s, uint32_t maxLeaf = 4);
 void buildLights(HostScene& s);
 void buildLightBVH(HostScene& s);

@@ -35,7 +35,9 @@ def _device():
 class Scene:
     """Device-resident scene (BVH4, light collection, light BVH, materials, textures)."""
 
-    def __init__(self, data: SceneData):
+    def __init__(self, data: SceneData, use_compressed_hit_info: bool = False):
+        """use_compressed_hit_info: 64-bit PackedHitInfo when the header fits
+        (SceneBuilder::Flags::UseCompressedHitInfo)."""
         L = _lib.load()
         self.data = data
         self._arrays = dict(
@@ -63,6 +65,7 @@ class Scene:
         d.h_camera = a["cam"].ctypes.data
         d.num_meshes = a["moff"].shape[0] - 1
         d.h_mesh_vertex_offsets = a["moff"].ctypes.data
+        d.flags = _lib.SCENE_USE_COMPRESSED_HIT_INFO if use_compressed_hit_info else 0
         h = C.c_void_p()
         _lib.check(L.fhp_scene_create(C.byref(d), C.byref(h)))
         self.handle = h
@@ -268,6 +271,35 @@ class MinimalPathTracer(_NativePass):
     kind = "MinimalPathTracer"
     outputs = ("color",)
 
+    def set_input(self, channel: str, tensor):
+        """Binds the "vbuffer" (PackedHitInfo per pixel, as VBufferRT writes it) or "viewW" input
+        for later frames; None unbinds. The pass keeps a reference to the tensor."""
+        if not hasattr(self, "_inputs"):
+            self._inputs = {}
+        ptr = None
+        if tensor is not None:
+            if not tensor.is_contiguous():
+                raise ValueError(f"input '{channel}' must be contiguous")
+            ptr = tensor.data_ptr()
+        _lib.check(_lib.load().fhp_pass_set_input(self.handle, channel.encode(), ptr))
+        if tensor is None:
+            self._inputs.pop(channel, None)
+        else:
+            self._inputs[channel] = tensor
+
+    def render(self, width: int, height: int, spp: int = 1, frame: int | None = None, out=None, stats: bool = False):
+        for k, t in getattr(self, "_inputs", {}).items():
+            want = (height, width)
+            if tuple(t.shape[:2]) != want:
+                raise ValueError(f"input '{k}' is {tuple(t.shape[:2])}, frame is {want}")
+            if k == "vbuffer" and self.scene is not None:
+                words = 2 if self.scene.info()["hit_info_compressed"] else 4
+                if t.dim() != 3 or t.shape[2] != words or t.element_size() != 4:
+                    raise ValueError(f"vbuffer must be [H, W, {words}] 32-bit words for this scene")
+            if k == "viewW" and (t.dim() != 3 or t.shape[2] != 4 or t.dtype != _torch().float32):
+                raise ValueError("viewW must be [H, W, 4] float32")
+        return super().render(width, height, spp, frame=frame, out=out, stats=stats)
+
     def render_batch(self, views: list[dict], stats: bool = False):
         """Renders several views in one launch sequence (fhp_pass_execute_batch). Each view is a dict
         with scene, width, height and optional spp (1), frame (0), camera (a scene.Camera, default
@@ -294,6 +326,55 @@ class MinimalPathTracer(_NativePass):
         _lib.check(_lib.load().fhp_pass_execute_batch(self.handle, len(views), arr, st.ctypes.data if stats else None,
                                                       _lib.stream_ptr()))
         return (outs, st) if stats else outs
+
+
+class VBufferRT(_NativePass):
+    """VBufferRT (Source/RenderPasses/GBuffer/VBuffer/VBufferRT.cpp): per-pixel primary hits as
+    PackedHitInfo. Properties samplePattern ('Center' | 'DirectX' | 'Halton' | 'Stratified'),
+    sampleCount, useAlphaTest, adjustShadingNormals, forceCullMode, cull, useTraceRayInline, useDOF.
+    `execute_frame(width, height)` returns {"vbuffer": [H, W, 2 or 4] int32 (the uint32 words),
+    "depth": [H, W], "viewW": [H, W, 4], "mask": [H, W]} and advances the camera jitter pattern.
+    The output tensors belong to the pass and are overwritten by the next frame of the same size
+    (as render-pass output textures are), so bound consumers see stable device addresses."""
+
+    kind = "VBufferRT"
+    outputs = ("vbuffer", "depth", "viewW", "mask")
+    SAMPLE_PATTERNS = {"Center": 0, "DirectX": 1, "Halton": 2, "Stratified": 3}
+    CULL_MODES = {"None": 0, "Front": 1, "Back": 2}
+
+    @classmethod
+    def _encode(cls, key, value):
+        if key == "samplePattern" and isinstance(value, str):
+            return cls.SAMPLE_PATTERNS[value]
+        if key == "cull" and isinstance(value, str):
+            return cls.CULL_MODES[value]
+        return float(value)
+
+    def execute_frame(self, width: int, height: int, outputs=("vbuffer", "depth", "viewW", "mask")) -> dict:
+        torch = _torch()
+        dev = _device()
+        if self.scene is None:
+            raise _lib.FhpError("VBufferRT needs a scene")
+        words = 2 if self.scene.info()["hit_info_compressed"] else 4
+        shapes = {"vbuffer": ((height, width, words), torch.int32), "depth": ((height, width), torch.float32),
+                  "viewW": ((height, width, 4), torch.float32), "mask": ((height, width), torch.float32)}
+        if not hasattr(self, "_bufs"):
+            self._bufs = {}
+        out = {}
+        for k, (shape, dt) in shapes.items():
+            if k != "vbuffer" and k not in outputs:
+                continue
+            b = self._bufs.get(k)
+            if b is None or tuple(b.shape) != shape or b.device != dev:
+                b = self._bufs[k] = torch.empty(shape, dtype=dt, device=dev)
+            out[k] = b
+        jit = (C.c_float * 2)()
+        ptr = lambda k: out[k].data_ptr() if k in out else None
+        _lib.check(_lib.load().fhp_vbuffer_execute(self.handle, self.scene.handle, width, height, ptr("vbuffer"),
+                                                   ptr("depth"), ptr("viewW"), ptr("mask"), jit, _lib.stream_ptr()))
+        self.jitter = (jit[0], jit[1])
+        self.frame_count += 1
+        return out
 
 
 class WARDiffPathTracer(_NativePass):
@@ -478,7 +559,7 @@ class SceneGradients:
 
 # --------------------------------------------------------------------------- graph / testbed
 _PASS_TYPES = {"MinimalPathTracer": MinimalPathTracer, "WARDiffPathTracer": WARDiffPathTracer,
-               "AccumulatePass": AccumulatePass}
+               "AccumulatePass": AccumulatePass, "VBufferRT": VBufferRT}
 
 
 class RenderGraph:
@@ -530,9 +611,19 @@ class RenderGraph:
                 srcs = [s for s, d in self.edges if d == f"{n}.input"]
                 if srcs:
                     vals[f"{n}.output"] = p.accumulate(vals[srcs[0]])
+            elif isinstance(p, VBufferRT):
+                if p.scene is not scene:
+                    p.set_scene(scene)
+                for k, t in p.execute_frame(width, height).items():
+                    vals[f"{n}.{k}"] = t
             elif isinstance(p, _NativePass):
                 if p.scene is not scene:
                     p.set_scene(scene)
+                if isinstance(p, MinimalPathTracer):
+                    # Input channels (MinimalPathTracer.cpp:48-53) from the graph's edges.
+                    for ch in ("vbuffer", "viewW"):
+                        srcs = [s for s, d in self.edges if d == f"{n}.{ch}"]
+                        p.set_input(ch, vals[srcs[0]] if srcs else None)
                 if isinstance(p, WARDiffPathTracer) and p.run_backward:
                     p.backward(width, height, 1)
                     vals[f"{n}.dColor"] = torch.zeros((height, width, 4), device=_device())

@@ -66,7 +66,11 @@ typedef struct FhpSceneDesc {
      * indexing them (SceneBuilder's mesh ids, Scene.cpp:4314-4336). 0 = one mesh. */
     uint32_t num_meshes;
     const uint32_t* h_mesh_vertex_offsets; /* num_meshes + 1 */
+    /* FHP_SCENE_* flags: FHP_SCENE_USE_COMPRESSED_HIT_INFO selects the 64-bit PackedHitInfo when
+     * the header fits (SceneBuilder::Flags::UseCompressedHitInfo, SceneBuilder.h:78). */
+    uint32_t flags;
 } FhpSceneDesc;
+#define FHP_SCENE_USE_COMPRESSED_HIT_INFO 0x8000u
 
 typedef struct FhpSceneInfo {
     uint32_t num_triangles, num_vertices, num_materials, num_textures;
@@ -75,6 +79,12 @@ typedef struct FhpSceneInfo {
     double bvh_sah_cost;
     uint64_t device_bytes;
     float camera_basis[9]; /* U, V, W as computed by Camera.cpp:187-193 */
+    /* PackedHitInfo format (HitInfo::init, Source/Falcor/Scene/HitInfo.cpp:52-106): 1 = 64-bit
+     * records (2 words per pixel), 0 = 128-bit (4 words); bits of the type, instance id (mesh)
+     * and primitive index (triangle within the mesh). meshes_contiguous = 1 when each mesh's
+     * triangles are consecutive, which V-buffer encoding and decoding need. */
+    uint32_t hit_info_compressed, hit_info_type_bits, hit_info_instance_bits, hit_info_primitive_bits;
+    uint32_t meshes_contiguous;
 } FhpSceneInfo;
 
 int fhp_scene_create(const FhpSceneDesc* desc, FhpScene* out);
@@ -113,7 +123,7 @@ int fhp_trace_closest(FhpScene scene, const float* d_rays, uint32_t num_rays, ui
 int fhp_trace_any(FhpScene scene, const float* d_rays, uint32_t num_rays, uint32_t* d_occluded, FhpStream stream);
 
 /* ---------------------------------------------------------------- render passes
- * kind: "MinimalPathTracer" or "WARDiffPathTracer". Properties are (key, value) pairs with
+ * kind: "MinimalPathTracer", "WARDiffPathTracer", "AccumulatePass" or "VBufferRT". Properties are (key, value) pairs with
  * numeric values; enumerations are passed as their integer value. Unknown keys are ignored
  * with a warning (fhp_last_warning), as the reference's parseProperties does.
  * Replaces registerPlugin/RenderPass::create (MinimalPathTracer.cpp:32-35, :66-89;
@@ -217,6 +227,28 @@ int fhp_accumulate_frame_count(FhpPass pass, uint32_t* count);
  * (:191-270); file loading and the threshold test are the caller's (falcor_amd.image_compare). */
 int fhp_image_compare(int metric, const float* d_a, const float* d_b, uint32_t width, uint32_t height, int alpha,
                       float* d_error_map, float* d_heat_map, double* h_error, FhpStream stream);
+
+/* VBufferRT: a pass of kind "VBufferRT" (properties samplePattern [0 Center, 1 DirectX, 2 Halton,
+ * 3 Stratified], sampleCount, useAlphaTest, adjustShadingNormals, forceCullMode, cull [forced
+ * Front/Back culling is refused], useTraceRayInline, useDOF, and the traversal launch
+ * properties). Each fhp_vbuffer_execute call is one frame: it takes the next camera jitter
+ * sample of the pattern (scaled by 1 / frame size; returned in h_jitter[2] when not null), traces
+ * the pinhole camera's jittered primary ray of every pixel and writes d_vbuffer: one PackedHitInfo per pixel
+ * (2 or 4 words, hit_info_compressed of fhp_scene_info; a miss is all zeros), and when not null
+ * d_depth (NDC depth z/w of the hit, 1 on a miss), d_viewW (4 floats: -ray direction, 0) and
+ * d_mask (1 hit, 0 miss). The instance id of a hit is its mesh, the primitive index its triangle
+ * within the mesh. Replaces VBufferRT::execute and its ray generation / hit / miss programs
+ * (Source/RenderPasses/GBuffer/VBuffer/VBufferRT.cpp, VBufferRT.rt.slang, VBufferRT.slang:56-143)
+ * with TriangleHit::pack (Source/Falcor/Scene/HitInfo.slang:94-121, :457-473). */
+int fhp_vbuffer_execute(FhpPass pass, FhpScene scene, uint32_t width, uint32_t height, uint32_t* d_vbuffer, float* d_depth,
+                        float* d_viewW, float* d_mask, float* h_jitter, FhpStream stream);
+/* Binds (d_data != NULL) or unbinds an input channel of a MinimalPathTracer for later
+ * fhp_pass_execute calls: "vbuffer" (PackedHitInfo per pixel in the scene's format, as written
+ * by fhp_vbuffer_execute; its hits replace the primary traversal) or "viewW" (4 floats per pixel;
+ * the primary direction is -viewW). The buffers must hold width * height records at execution.
+ * Records that do not name a triangle of the scene read as misses. Replaces the "vbuffer" and
+ * "viewW" inputs of MinimalPathTracer (MinimalPathTracer.cpp:48-53, MinimalPathTracer.rt.slang). */
+int fhp_pass_set_input(FhpPass pass, const char* channel, const void* d_data);
 
 #ifdef __cplusplus
 }
