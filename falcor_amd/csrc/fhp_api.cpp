@@ -216,8 +216,8 @@ const std::map<std::string, double>& warDefaults() {
         {"fixedSeed", 0},       {"useFixedSeed", 0},      {"useBSDFSampling", 1},   {"useNEE", 1},
         {"useMIS", 1},          {"useWAR", 0},            {"auxSampleCount", 16},   {"Log10vMFConcentration", 5},
         {"Log10vMFConcentrationScreen", 5},               {"boundaryTermBeta", 0.01}, {"useAntitheticSampling", 1},
-        {"countTraversal", 0},  {"timeKernels", 0},       {"traceBlockSize", kDefaultBlock},
-        {"traceStackDepth", kDefaultLds},                 {"traceBlocksPerCU", kDefaultBlocksPerCU}};
+        {"countTraversal", 0},  {"timeKernels", 0},       {"traceBlockSize", 128},
+        {"traceStackDepth", 24},                          {"traceBlocksPerCU", 10}};  // 1280 threads per CU: 5 waves per SIMD
     return d;
 }
 

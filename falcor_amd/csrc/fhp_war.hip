@@ -285,8 +285,15 @@ __device__ f3 warPath(const SceneDev& s, const WarArgs& a, uint32_t px, uint32_t
     return radiance;
 }
 
+// The megakernels hold a whole path (and its adjoint) per lane: unconstrained they take 147 / 183
+// VGPRs (3 / 2 waves per SIMD) and latency hiding suffers. Capping them at 96 VGPRs (5 waves per
+// SIMD) spills some state to scratch but measured 1.47x faster per 512x512, 16 spp
+// forward+adjoint iteration (with 1280 threads per CU, the pass's default launch).
+constexpr int kWarWavesPerEU = 5;
+
 template <int B, int D, bool ADJ>
-__global__ __launch_bounds__(B) void kWar(SceneDev s, WarArgs a, TraceScratch ts) {
+__global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(kWarWavesPerEU))) void kWar(SceneDev s, WarArgs a,
+                                                                                                 TraceScratch ts) {
     __shared__ uint32_t lds[D * B];
     TraceStack st{(LdsU32*)(lds + threadIdx.x), uint32_t(B), uint32_t(D), (GlobalU32*)(ts.ovf + blockIdx.x * B + threadIdx.x),
                   ts.gridBlocks * B, ts.ovfDepth};
