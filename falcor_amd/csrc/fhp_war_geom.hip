@@ -568,8 +568,13 @@ __device__ void geomPath(const SceneDev& s, const GeomArgs& a, uint32_t px, uint
     if (Qt != 0.f || gVx != 0.f || gVy != 0.f) screenWarpAdjoint(s, a, st, psx, psy, rngScreen, gVx, gVy, Qt, row, nc);
 }
 
+// Unconstrained the kernel takes 248 VGPRs (2 waves per SIMD); capped at 128 (4 waves) it spills
+// more but runs the 256x256, 8 spp, 16-aux vertex-gradient pass about 1.2x faster.
+constexpr int kGeomWavesPerEU = 4;
+
 template <int B, int D>
-__global__ __launch_bounds__(B) void kWarGeom(SceneDev s, GeomArgs a, TraceScratch ts) {
+__global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(kGeomWavesPerEU))) void kWarGeom(SceneDev s, GeomArgs a,
+                                                                                                     TraceScratch ts) {
     __shared__ uint32_t lds[D * B];
     TraceStack st{(LdsU32*)(lds + threadIdx.x), uint32_t(B), uint32_t(D), (GlobalU32*)(ts.ovf + blockIdx.x * B + threadIdx.x),
                   ts.gridBlocks * B, ts.ovfDepth};
@@ -593,7 +598,9 @@ hipError_t launchWarGeom(const SceneDev& s, const GeomArgs& a, const TraceScratc
     if (tm) tm->begin(KK_WAR_ADJOINT, st);
     hipError_t e = withTraceConfig(ts, [&](auto Bc, auto Dc) {
         constexpr int B = decltype(Bc)::value, D = decltype(Dc)::value;
-        const uint32_t blocks = std::min<uint32_t>(ts.gridBlocks, (n + B - 1) / B);
+        // Persistent grid: no more blocks than are resident at kGeomWavesPerEU waves per SIMD.
+        const uint32_t resident = uint32_t(numCUs()) * std::max<uint32_t>(1u, kGeomWavesPerEU * 4 * 64 / B);
+        const uint32_t blocks = std::min<uint32_t>(std::min<uint32_t>(ts.gridBlocks, resident), (n + B - 1) / B);
         hipLaunchKernelGGL((kWarGeom<B, D>), dim3(blocks), dim3(B), 0, st, s, a, ts);
         return hipGetLastError();
     });
