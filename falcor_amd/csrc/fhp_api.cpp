@@ -62,6 +62,7 @@ struct FhpScene_ {
     HostScene h;
     DevBuf bvh, triPos, idx, vtx, triMat, mats, texels, texDesc, lights, active, triToLight, lbvh, lbvhTri, lbvhMask;
     DevBuf meshTri;  // first triangle of each mesh (PackedHitInfo instance ids)
+    DevBuf levels;   // BVH refit schedule (node ids by depth)
     DevBuf ovf;  // traversal overflow stack
     DevBuf matsS, texelsS;  // frozen sampling parameters (finite-difference mode)
     TraceScratch ts{};      // default traversal configuration (ray queries)
@@ -370,6 +371,7 @@ int buildAndUpload(FhpScene_* s) {
     HIP_OR_FAIL(s->lbvhTri.upload(h.lbvhTri));
     HIP_OR_FAIL(s->lbvhMask.upload(h.lbvhMask));
     HIP_OR_FAIL(s->meshTri.upload(h.meshTriOff));
+    HIP_OR_FAIL(s->levels.upload(h.levelNodes));
     // Overflow stack for the persistent traversal grid, sized for the smallest LDS stack and the
     // largest grid any configuration launches.
     s->stackNeed = h.stackNeed + 1;
@@ -502,6 +504,54 @@ int fhp_scene_update_vertices(FhpScene s, uint32_t mesh, const float* pos, const
     std::copy(pos, pos + 3 * size_t(v1 - v0), h.pos.begin() + 3 * size_t(v0));
     if (nrm) std::copy(nrm, nrm + 3 * size_t(v1 - v0), h.nrm.begin() + 3 * size_t(v0));
     return buildAndUpload(s);
+}
+
+int fhp_scene_update_vertices_device(FhpScene s, uint32_t mesh, const float* d_pos, const float* d_nrm, int rebuild,
+                                     FhpStream stream) {
+    if (!s || !d_pos) return fail(FHP_EINVAL, "null argument");
+    HostScene& h = s->h;
+    if (mesh + 1 >= h.meshOff.size()) return fail(FHP_EINVAL, "mesh id out of range");
+    const uint32_t v0 = h.meshOff[mesh], v1 = h.meshOff[mesh + 1], nv = v1 - v0;
+    hipStream_t st = hipStream_t(stream);
+    // Host mirrors of the vertices (light collection, later rebuilds) and the finiteness check.
+    std::vector<float> pos(3 * size_t(nv)), nrm(d_nrm ? 3 * size_t(nv) : 0);
+    if (nv) {
+        HIP_OR_FAIL(hipMemcpyAsync(pos.data(), d_pos, pos.size() * 4, hipMemcpyDeviceToHost, st));
+        if (d_nrm) HIP_OR_FAIL(hipMemcpyAsync(nrm.data(), d_nrm, nrm.size() * 4, hipMemcpyDeviceToHost, st));
+        HIP_OR_FAIL(hipStreamSynchronize(st));
+    }
+    for (float v : pos)
+        if (!std::isfinite(v)) return fail(FHP_EINVAL, "non-finite vertex data");
+    for (float v : nrm)
+        if (!std::isfinite(v)) return fail(FHP_EINVAL, "non-finite vertex data");
+    std::copy(pos.begin(), pos.end(), h.pos.begin() + 3 * size_t(v0));
+    if (d_nrm) std::copy(nrm.begin(), nrm.end(), h.nrm.begin() + 3 * size_t(v0));
+    // An emissive mesh also moves lights: rebuild the light collection and light BVH with the tree.
+    bool emissive = false;
+    for (uint32_t t = 0; t < h.nT && !emissive; t++) {
+        const uint32_t i0 = h.idx[3 * size_t(t)];
+        emissive = i0 >= v0 && i0 < v1 && h.triToLight[t] != kInvalid;
+    }
+    if (rebuild || emissive) return buildAndUpload(s);
+    float ext = 0.f;
+    for (float v : pos) ext = std::max(ext, std::fabs(v));
+    h.bvhAbsPad = std::max(h.bvhAbsPad, ext * 1e-6f + 1e-30f);
+    RefitArgs r{};
+    r.vtx = s->vtx.as<float4>();
+    r.triPos = s->triPos.as<float4>();
+    r.bvh = s->bvh.as<float>();
+    r.idx = s->idx.as<uint32_t>();
+    r.pos = d_pos;
+    r.nrm = d_nrm;
+    r.v0 = v0;
+    r.nVerts = nv;
+    r.nTris = h.nT;
+    r.levelNodes = s->levels.as<uint32_t>();
+    r.levelOff = h.levelOff.data();
+    r.nLevels = uint32_t(h.levelOff.size() - 1);
+    r.absPad = h.bvhAbsPad;
+    HIP_OR_FAIL(launchRefit(r, st));
+    return FHP_OK;
 }
 
 int fhp_scene_destroy(FhpScene s) {

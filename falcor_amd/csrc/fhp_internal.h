@@ -168,6 +168,22 @@ hipError_t launchWarGeom(const SceneDev& s, const GeomArgs& a, const TraceScratc
 hipError_t launchAggregate(const float* tmp, uint32_t hashSize, uint32_t dim, float* out, hipStream_t stream);
 hipError_t launchFill(float* p, uint64_t n, float v, hipStream_t stream);
 
+// BVH refit after vertex updates of one mesh (fhp_refit.hip).
+struct RefitArgs {
+    float4* vtx;             // vertex records of the scene
+    float4* triPos;          // leaf-order triangle records
+    float* bvh;              // BVH4 nodes (32 floats each)
+    const uint32_t* idx;     // triangle vertex indices
+    const float* pos;        // new positions of vertices [v0, v0 + nVerts) (3 floats each, device)
+    const float* nrm;        // new normals (or null)
+    uint32_t v0, nVerts, nTris;
+    const uint32_t* levelNodes;  // device: node ids by depth
+    const uint32_t* levelOff;    // host: nLevels + 1 offsets into levelNodes
+    uint32_t nLevels;
+    float absPad;
+};
+hipError_t launchRefit(const RefitArgs& r, hipStream_t stream);
+
 // AccumulatePass (Accumulate.cs.slang, AccumulatePass.cpp). Precision modes use the values of
 // AccumulatePass::Precision (AccumulatePass.h:70-75); input formats those of _INPUT_FORMAT
 // (Accumulate.cs.slang:40-43).

@@ -219,6 +219,7 @@ void buildBVH4(HostScene& s, uint32_t maxLeaf) {
     float ext = 0.f;
     for (int a = 0; a < 3; a++) ext = std::max(ext, std::max(std::fabs(sceneLo[a]), std::fabs(sceneHi[a])));
     const float absPad = ext * 1e-6f + 1e-30f;
+    s.bvhAbsPad = absPad;
     auto padLo = [&](float v) { return v - (std::fabs(v) * 1e-6f + absPad); };
     auto padHi = [&](float v) { return v + (std::fabs(v) * 1e-6f + absPad); };
 
@@ -292,6 +293,25 @@ void buildBVH4(HostScene& s, uint32_t maxLeaf) {
     s.bvhNodes = uint32_t(out.size() / 32);
     s.bvhDepth = root.depth;
     s.stackNeed = root.need;
+
+    // Nodes by depth (children are allocated after their parent, so one pass in index order
+    // assigns every depth).
+    std::vector<uint32_t> depth(s.bvhNodes, 0u);
+    uint32_t maxD = 0;
+    for (uint32_t n = 0; n < s.bvhNodes; n++) {
+        maxD = std::max(maxD, depth[n]);
+        for (int i = 0; i < 4; i++) {
+            uint32_t c;
+            std::memcpy(&c, &out[32 * size_t(n) + 24 + i], 4);
+            if (c != kInvalid && !childIsLeaf(c)) depth[c] = depth[n] + 1;
+        }
+    }
+    s.levelOff.assign(maxD + 2, 0u);
+    for (uint32_t n = 0; n < s.bvhNodes; n++) s.levelOff[depth[n] + 1]++;
+    for (uint32_t d = 0; d <= maxD; d++) s.levelOff[d + 1] += s.levelOff[d];
+    s.levelNodes.assign(s.bvhNodes, 0u);
+    std::vector<uint32_t> fill(s.levelOff.begin(), s.levelOff.end() - 1);
+    for (uint32_t n = 0; n < s.bvhNodes; n++) s.levelNodes[fill[depth[n]]++] = n;
 }
 
 // ------------------------------------------------------------------ light collection

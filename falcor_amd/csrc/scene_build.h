@@ -35,6 +35,9 @@ struct HostScene {
     std::vector<float> triPos;
     uint32_t bvhNodes = 0, bvhDepth = 0, stackNeed = 0, bvhLeaves = 0;
     double bvhSah = 0.0;
+    float bvhAbsPad = 0.f;  // absolute part of the outward box padding (scene extent * 1e-6)
+    // Refit schedule: node ids grouped by depth (levelOff[d] .. levelOff[d + 1]), root first.
+    std::vector<uint32_t> levelNodes, levelOff;
 
     // Lights: 20 floats per light, see SceneDev::lights.
     std::vector<float> lights;

@@ -12,6 +12,7 @@ from __future__ import annotations
 import ctypes as C
 import math
 import warnings
+from pathlib import Path
 
 import numpy as np
 
@@ -118,20 +119,41 @@ class Scene:
         sel = np.all((idx >= v0) & (idx < v1), axis=1)
         return (self._arrays["pos"][v0:v1].copy(), self._arrays["nrm"][v0:v1].copy(), (idx[sel] - v0).astype(np.uint32))
 
-    def set_mesh_vertices(self, mesh_id: int, positions, normals=None):
-        """Replaces the mesh's vertex positions (and normals) and rebuilds the BVH and lights."""
+    def set_mesh_vertices(self, mesh_id: int, positions, normals=None, update_mode: str = "refit"):
+        """Replaces the mesh's vertex positions (and normals) (Scene::setMeshVertices).
+
+        Device tensors are applied on the GPU: update_mode "refit" (the reference's default BLAS
+        update mode) refits the BVH in place, "rebuild" rebuilds it; a mesh with emissive
+        triangles always rebuilds (its lights move). Host arrays rebuild the BVH and lights."""
+        if update_mode not in ("refit", "rebuild"):
+            raise ValueError("update_mode is 'refit' or 'rebuild'")
         v0, v1 = self.mesh_vertex_range(mesh_id)
-        def host(x):
-            if hasattr(x, "detach"):
-                x = x.detach().float().cpu().numpy()
-            x = np.ascontiguousarray(x, np.float32).reshape(-1, 3)
-            if x.shape[0] != v1 - v0:
-                raise ValueError(f"mesh {mesh_id} has {v1 - v0} vertices, got {x.shape[0]}")
-            return x
-        p = host(positions)
-        n = host(normals) if normals is not None else None
-        _lib.check(_lib.load().fhp_scene_update_vertices(self.handle, mesh_id, p.ctypes.data,
-                                                         n.ctypes.data if n is not None else None))
+        def check(x):
+            if x.reshape(-1).shape[0] != 3 * (v1 - v0):
+                raise ValueError(f"mesh {mesh_id} has {v1 - v0} vertices, got {x.reshape(-1).shape[0] // 3}")
+        if hasattr(positions, "is_cuda") and positions.is_cuda:
+            torch = _torch()
+            p = positions.detach().to(torch.float32).contiguous()
+            check(p)
+            n = normals.detach().to(device=p.device, dtype=torch.float32).contiguous() if normals is not None else None
+            if n is not None:
+                check(n)
+            _lib.check(_lib.load().fhp_scene_update_vertices_device(
+                self.handle, mesh_id, p.data_ptr(), n.data_ptr() if n is not None else None,
+                int(update_mode == "rebuild"), _lib.stream_ptr()))
+            p = p.cpu().numpy().reshape(-1, 3)
+            n = n.cpu().numpy().reshape(-1, 3) if n is not None else None
+        else:
+            def host(x):
+                if hasattr(x, "detach"):
+                    x = x.detach().float().cpu().numpy()
+                x = np.ascontiguousarray(x, np.float32).reshape(-1, 3)
+                check(x)
+                return x
+            p = host(positions)
+            n = host(normals) if normals is not None else None
+            _lib.check(_lib.load().fhp_scene_update_vertices(self.handle, mesh_id, p.ctypes.data,
+                                                             n.ctypes.data if n is not None else None))
         self._arrays["pos"][v0:v1] = p
         if n is not None:
             self._arrays["nrm"][v0:v1] = n
@@ -645,8 +667,16 @@ class Testbed:
         self.render_graph: RenderGraph | None = None
         self.frame_count = 0
 
-    def load_scene(self, data: SceneData) -> Scene:
-        self.scene = Scene(data)
+    def load_scene(self, data, use_compressed_hit_info: bool = False) -> Scene:
+        """data: SceneData, or the path of a .pyscene file (Testbed::loadScene; the camera's aspect
+        ratio follows the frame size, as the reference sets it from the frame dimensions)."""
+        if isinstance(data, (str, Path)):
+            from .pyscene import load_pyscene
+            warns = []
+            data = load_pyscene(data, aspect=self.width / self.height, warnings=warns)
+            for w in warns:
+                warnings.warn(w)
+        self.scene = Scene(data, use_compressed_hit_info=use_compressed_hit_info)
         return self.scene
 
     def create_render_graph(self, name: str = "RenderGraph") -> RenderGraph:

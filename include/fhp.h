@@ -95,6 +95,14 @@ int fhp_scene_info(FhpScene scene, FhpSceneInfo* out);
  * the light BVH. Replaces Scene::setMeshVertices / set_mesh_vertices (Scene.cpp:4314-4336) and
  * the BLAS rebuild it triggers. */
 int fhp_scene_update_vertices(FhpScene scene, uint32_t mesh_id, const float* h_positions, const float* h_normals);
+/* The same update from device memory. rebuild = 0 refits the BVH on the GPU (the reference's
+ * default BLAS update mode, Scene.h:1366 UpdateMode::Refit): vertex and triangle records are
+ * rewritten and every node box recomputed bottom-up, keeping the tree's topology. rebuild = 1,
+ * or a mesh with emissive triangles (whose lights move), rebuilds the BVH, the light collection
+ * and the light BVH as fhp_scene_update_vertices does. Copies the vertices to host memory
+ * (synchronises the stream). */
+int fhp_scene_update_vertices_device(FhpScene scene, uint32_t mesh_id, const float* d_positions, const float* d_normals,
+                                     int rebuild, FhpStream stream);
 /* Replace all material records (host memory, 16 * num_materials floats).
  * Replaces Scene::setMaterialParams (Source/Falcor/Scene/Scene.cpp:4620-4643). */
 int fhp_scene_set_materials(FhpScene scene, const float* h_materials);

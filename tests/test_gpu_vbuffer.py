@@ -239,3 +239,25 @@ def test_vbuffer_errors(gpu):
     assert sc2.info()["meshes_contiguous"] == 0
     with pytest.raises(_lib.FhpError, match="contiguous"):
         _vbuffer(sc2).execute_frame(W, H)
+
+
+def test_testbed_loads_pyscene_and_renders(gpu, tmp_path):
+    """Testbed.load_scene with a .pyscene path: the scene renders through MinimalPathTracer."""
+    from falcor_amd.render_graph import MinimalPathTracer, Testbed
+    (tmp_path / "box.pyscene").write_text(
+        "m = StandardMaterial('White')\nm.baseColor = float4(0.7, 0.7, 0.7, 1)\n"
+        "lamp = StandardMaterial('Lamp')\nlamp.emissiveColor = float3(10, 10, 10)\n"
+        "quad = TriangleMesh.createQuad()\n"
+        "sceneBuilder.addMeshInstance(sceneBuilder.addNode('Floor', Transform(scaling=2)), sceneBuilder.addTriangleMesh(quad, m))\n"
+        "sceneBuilder.addMeshInstance(sceneBuilder.addNode('Lamp', Transform(translation=float3(0, 1, 0), rotationEulerDeg=float3(180, 0, 0))),"
+        " sceneBuilder.addTriangleMesh(quad, lamp))\n"
+        "c = Camera()\nc.position = float3(0, 0.5, 2.5)\nc.target = float3(0, 0.3, 0)\nc.up = float3(0, 1, 0)\nc.focalLength = 35\n"
+        "sceneBuilder.addCamera(c)\n")
+    tb = Testbed(48, 32)
+    sc = tb.load_scene(tmp_path / "box.pyscene")
+    assert sc.info()["num_triangles"] == 4
+    p = MinimalPathTracer({"maxBounces": 2})
+    p.set_scene(sc)
+    img = p.render(48, 32, 4, frame=0)
+    import torch
+    assert torch.isfinite(img).all() and float(img[..., :3].mean()) > 0

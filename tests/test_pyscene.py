@@ -1,0 +1,176 @@
+"""`.pyscene` loader (falcor_amd.pyscene): a restricted interpreter over the parsed script, no
+execution. Checked against the equivalent SceneBuilder calls, the reference's conventions
+(fovY from focal length and frame height, float16 material parameters, sRGB textures) and a
+refusal of everything outside the whitelist."""
+import math
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from falcor_amd import scene as S
+from falcor_amd.pyscene import PySceneError, load_obj, load_pyscene
+
+SCENE = """
+# A small scene in the .pyscene dialect.
+from falcor import *
+
+red = StandardMaterial('Red')
+red.baseColor = float4(0.8, 0.1, 0.1, 1.0)
+red.roughness = 0.3
+red.metallic = 0.25
+red.doubleSided = True
+
+lamp = StandardMaterial('Lamp')
+lamp.emissiveColor = float3(4, 3, 2)
+lamp.emissiveFactor = 2.5
+
+matte = PBRTDiffuseMaterial('Matte')
+matte.baseColor = float4(0.5, 0.6, 0.7, 1)
+
+tex = StandardMaterial('Textured')
+tex.loadTexture(MaterialTextureSlot.BaseColor, 'checker.png')
+
+quad = TriangleMesh.createQuad()
+cube = TriangleMesh.createCube(float3(2, 1, 1))
+ball = TriangleMesh.createSphere(radius=0.5, segmentsU=8, segmentsV=6)
+prism = TriangleMesh.createFromFile('prism.obj', smoothNormals=False)
+
+sceneBuilder.addMeshInstance(sceneBuilder.addNode('Floor', Transform(scaling=float3(4, 1, 4))),
+                             sceneBuilder.addTriangleMesh(quad, matte))
+for i in range(2):
+    sceneBuilder.addMeshInstance(
+        sceneBuilder.addNode('Box', Transform(translation=float3(i, 0.5, 0), rotationEulerDeg=float3(0, 30 * i, 0))),
+        sceneBuilder.addTriangleMesh(cube, red))
+sceneBuilder.addMeshInstance(sceneBuilder.addNode('Lamp', Transform(scaling=0.25, translation=float3(0, 2, 0), rotationEuler=float3(math.pi, 0, 0))),
+                             sceneBuilder.addTriangleMesh(quad, lamp))
+sceneBuilder.addMeshInstance(sceneBuilder.addNode('Ball', Transform(translation=float3(-1, 0.5, 0))),
+                             sceneBuilder.addTriangleMesh(ball, tex))
+sceneBuilder.addMeshInstance(sceneBuilder.addNode('Prism'), sceneBuilder.addTriangleMesh(prism, red))
+
+light = PointLight('p')
+light.intensity = float3(1, 1, 1)
+sceneBuilder.addLight(light)
+
+camera = Camera()
+camera.position = float3(0, 1, 5)
+camera.target = float3(0, 0.5, 0)
+camera.up = float3(0, 1, 0)
+camera.focalLength = 35.0
+sceneBuilder.addCamera(camera)
+"""
+
+OBJ = """# square pyramid without a base
+v 0 0 0
+v 1 0 0
+v 1 0 1
+v 0 0 1
+v 0.5 1 0.5
+vt 0 0
+vt 1 0
+vt 0.5 1
+f 1/1 2/2 5/3
+f 2/1 3/2 5/3
+f 3/1 4/2 5/3
+f 4/1 1/2 5/3
+f -5 -4 -3 -2
+"""
+
+
+def _write(tmp_path, scene=SCENE):
+    from PIL import Image
+    (tmp_path / "s.pyscene").write_text(scene)
+    (tmp_path / "prism.obj").write_text(OBJ)
+    chk = (np.indices((4, 4)).sum(0) % 2 * 255).astype(np.uint8)
+    Image.fromarray(np.stack([chk, chk // 2, np.full_like(chk, 64)], -1), "RGB").save(tmp_path / "checker.png")
+    return tmp_path / "s.pyscene"
+
+
+def test_scene_matches_builder_calls(tmp_path):
+    warns = []
+    d = load_pyscene(_write(tmp_path), aspect=1.5, warnings=warns)
+    assert d.mesh_names == ["Floor", "Box", "Box", "Lamp", "Ball", "Prism"]
+    assert any("PointLight" in w for w in warns)
+    # The same scene through SceneBuilder directly.
+    sb = S.SceneBuilder()
+    quad, cube = S.TriangleMesh.create_quad(), S.TriangleMesh.create_cube((2, 1, 1))
+    ball = S.TriangleMesh.create_sphere(0.5, 8, 6)
+    prism = load_obj(tmp_path / "prism.obj")
+    m = [sb.add_material(np.zeros(S.MAT_STRIDE, np.float32)) for _ in range(4)]
+    sb.add_mesh_instance(quad, m[0], S.Transform((4, 1, 4)), "Floor")
+    for i in range(2):
+        sb.add_mesh_instance(cube, m[1], S.Transform.deg((1, 1, 1), (i, 0.5, 0), (0, 30 * i, 0)), "Box")
+    sb.add_mesh_instance(quad, m[2], S.Transform((0.25,) * 3, (0, 2, 0), (math.pi, 0, 0)), "Lamp")
+    sb.add_mesh_instance(ball, m[3], S.Transform((1, 1, 1), (-1, 0.5, 0)), "Ball")
+    sb.add_mesh_instance(prism, m[1], None, "Prism")
+    ref = sb.build()
+    np.testing.assert_allclose(d.positions, ref.positions, atol=1e-6)
+    np.testing.assert_allclose(d.normals, ref.normals, atol=1e-6)
+    assert np.array_equal(d.indices, ref.indices)
+    assert np.array_equal(d.mesh_vertex_offsets, ref.mesh_vertex_offsets)
+    # Camera: fovY = 2 atan(frameHeight / (2 focalLength)), frame height 24 mm.
+    assert d.camera.fov_y == pytest.approx(2 * math.atan(12 / 35), rel=1e-12)
+    assert d.camera.aspect == 1.5 and d.camera.position == (0, 1, 5)
+
+
+def test_materials_follow_reference_storage(tmp_path):
+    d = load_pyscene(_write(tmp_path))
+    mats = d.materials
+    order = [int(d.tri_material[d.tri_mesh == k][0]) for k in range(len(d.mesh_names))]
+    matte, red, lamp, tex = mats[order[0]], mats[order[1]], mats[order[3]], mats[order[4]]
+    # Base colour and specular parameters are float16 in BasicMaterialData.
+    assert red[S.M_TYPE] == S.MAT_STANDARD
+    assert np.array_equal(red[S.M_BASE:S.M_BASE + 3], np.float16([0.8, 0.1, 0.1]).astype(np.float32))
+    assert red[S.M_ROUGH] == np.float16(0.3) and red[S.M_METAL] == 0.25 and red[S.M_DOUBLE_SIDED] == 1
+    assert matte[S.M_TYPE] == S.MAT_PBRT_DIFFUSE
+    assert np.array_equal(lamp[S.M_EMIT:S.M_EMIT + 3], [4, 3, 2]) and lamp[S.M_EMIT_FACTOR] == 2.5
+    assert np.array_equal(lamp[S.M_BASE:S.M_BASE + 3], [1, 1, 1])  # default base colour 1
+    assert tex[S.M_TEX] == 0 and red[S.M_TEX] == -1
+    # sRGB texture decoded to linear: 255 -> 1, 128 -> ((128/255 + 0.055) / 1.055)^2.4, 64 -> ...
+    t = d.texels.reshape(4, 4, 3)
+    assert t[0, 1, 0] == pytest.approx(1.0) and t[0, 0, 0] == 0.0
+    assert t[0, 1, 1] == pytest.approx(((127 / 255 + 0.055) / 1.055) ** 2.4, rel=1e-6)
+    assert t[0, 0, 2] == pytest.approx(((64 / 255 + 0.055) / 1.055) ** 2.4, rel=1e-6)
+
+
+def test_obj_loader(tmp_path):
+    (tmp_path / "p.obj").write_text(OBJ)
+    m = load_obj(tmp_path / "p.obj")
+    # 4 side triangles + the quad base fanned into 2, negative indices resolved.
+    assert m.indices.shape == (6, 3)
+    # Distinct (position, uv) corners: 9 on the sides, 4 on the untextured base.
+    assert m.positions.shape[0] == 13
+    base = m.positions[m.indices[4:]].reshape(-1, 3)
+    assert np.all(base[:, 1] == 0)
+    assert np.allclose(np.linalg.norm(m.normals, axis=1), 1, atol=1e-6)
+
+
+@pytest.mark.parametrize("bad", [
+    "import os\n",
+    "__import__('os')\n",
+    "x = (lambda: 1)()\n",
+    "while True:\n    pass\n",
+    "m = StandardMaterial('a')\nm.__class__ = 1\n",
+    "x = 'abc'.upper()\n",
+    "sceneBuilder.sb = None\n",
+    "t = Transform(lookAt=float3(0, 0, 0))\n",
+    "c = Camera()\nc.focalLenght = 3\n",
+])
+def test_refuses_everything_outside_the_subset(tmp_path, bad):
+    p = tmp_path / "bad.pyscene"
+    p.write_text(bad)
+    with pytest.raises(PySceneError, match="line"):
+        load_pyscene(p)
+
+
+def test_reference_cornell_box_if_present():
+    """The reference's Scene/Box/CornellBox.pyscene, parsed as data, gives the closed box layout."""
+    path = Path("/root/reference/Scene/Box/CornellBox.pyscene")
+    if not path.exists():
+        pytest.skip("reference scene not available")
+    warns = []
+    d = load_pyscene(path, warnings=warns)
+    ref = S.closed_box(0)
+    assert d.mesh_names == ref.mesh_names
+    np.testing.assert_allclose(d.positions, ref.positions, atol=1e-6)
+    assert any("PointLight" in w for w in warns)
