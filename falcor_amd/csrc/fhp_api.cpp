@@ -63,6 +63,7 @@ struct FhpScene_ {
     DevBuf bvh, triPos, idx, vtx, triMat, mats, texels, texDesc, lights, active, triToLight, lbvh, lbvhTri, lbvhMask;
     DevBuf meshTri;  // first triangle of each mesh (PackedHitInfo instance ids)
     DevBuf levels;   // BVH refit schedule (node ids by depth)
+    DevBuf envTex;   // environment map texels (RGB fp32)
     DevBuf ovf;  // traversal overflow stack
     DevBuf matsS, texelsS;  // frozen sampling parameters (finite-difference mode)
     TraceScratch ts{};      // default traversal configuration (ray queries)
@@ -554,6 +555,41 @@ int fhp_scene_update_vertices_device(FhpScene s, uint32_t mesh, const float* d_p
     return FHP_OK;
 }
 
+int fhp_scene_set_env_map(FhpScene s, const float* h_texels, uint32_t W, uint32_t H, const float* h_params, uint32_t flags) {
+    if (!s) return fail(FHP_EINVAL, "null argument");
+    SceneDev& dv = s->dev;
+    if (W == 0 || H == 0) {
+        dv.envTex = nullptr;
+        dv.envW = dv.envH = dv.envFlags = 0;
+        return FHP_OK;
+    }
+    if (!h_texels || !h_params) return fail(FHP_EINVAL, "null argument");
+    if (flags & ~(kEnvLight | kEnvBackground)) return fail(FHP_EINVAL, "env map flags: 1 light, 2 background");
+    const size_t n = 3 * size_t(W) * H;
+    for (size_t i = 0; i < n; i++)
+        if (!std::isfinite(h_texels[i])) return fail(FHP_EINVAL, "non-finite env map texel");
+    HIP_OR_FAIL(s->envTex.alloc(n * 4));
+    HIP_OR_FAIL(hipMemcpy(s->envTex.p, h_texels, n * 4, hipMemcpyHostToDevice));
+    // h_params: intensity, tint rgb, rotation in degrees xyz. EnvMap::setRotation (EnvMap.cpp:62-72):
+    // transform = matrixFromRotationXYZ (MatrixMath.h:649-679, angles negated); the world-to-local
+    // rotation is its inverse, the transpose.
+    const float kDeg = 3.14159265358979323846f / 180.f;
+    const float c1 = std::cos(-h_params[4] * kDeg), c2 = std::cos(-h_params[5] * kDeg), c3 = std::cos(-h_params[6] * kDeg);
+    const float s1 = std::sin(-h_params[4] * kDeg), s2 = std::sin(-h_params[5] * kDeg), s3 = std::sin(-h_params[6] * kDeg);
+    const float m[3][3] = {{c2 * c3, c2 * s3, -s2},
+                           {-c1 * s3 + s1 * s2 * c3, c1 * c3 + s1 * s2 * s3, s1 * c2},
+                           {s1 * s3 + c1 * s2 * c3, -s1 * c3 + c1 * s2 * s3, c1 * c2}};
+    dv.envInv0 = mk3(m[0][0], m[1][0], m[2][0]);
+    dv.envInv1 = mk3(m[0][1], m[1][1], m[2][1]);
+    dv.envInv2 = mk3(m[0][2], m[1][2], m[2][2]);
+    dv.envScale = mk3(h_params[0] * h_params[1], h_params[0] * h_params[2], h_params[0] * h_params[3]);
+    dv.envTex = s->envTex.as<float>();
+    dv.envW = W;
+    dv.envH = H;
+    dv.envFlags = flags;
+    return FHP_OK;
+}
+
 int fhp_scene_destroy(FhpScene s) {
     delete s;
     return FHP_OK;
@@ -786,6 +822,7 @@ int fhp_pass_execute(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t spp
             return fail(FHP_ESTATE, "a V-buffer input needs each mesh's triangles to be contiguous");
         a.vbuf = p->inVbuf;
         a.viewW = p->inViewW;
+        a.keepMisses = s->dev.envTex && (s->dev.envFlags & kEnvLight);
         if ((rc = prepareTimer(p))) return rc;
         const bool graph = p->get("useGraph") != 0 && !count && !p->timer.on;
         if (!graph) {
@@ -906,6 +943,7 @@ int fhp_pass_execute_batch(FhpPass p, uint32_t nv, const FhpView* views, uint64_
     a.separateShadow = 0;
     a.nPaths = uint32_t(nPaths);
     a.nPix = uint32_t(nPix);
+    for (FhpScene sc : scenes) a.keepMisses |= (sc->dev.envTex && (sc->dev.envFlags & kEnvLight)) ? 1u : 0u;
     a.views = p->viewTab.as<ViewDev>();
     a.scenes = p->sceneTab.as<SceneDev>();
     a.nViews = nv;

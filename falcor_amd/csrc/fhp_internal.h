@@ -62,6 +62,7 @@ struct MptArgs {
     // view directions (viewW = -ray direction). Null when not bound.
     const uint32_t* vbuf;
     const float4* viewW;
+    uint32_t keepMisses;  // 1: closest-hit misses are queued for shading too (environment light)
 };
 
 struct TraceScratch {

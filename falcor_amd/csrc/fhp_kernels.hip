@@ -370,7 +370,20 @@ __global__ void kShadePrimary(SceneDev s0, MptArgs a) {
         a.rng[p] = make_uint4(rng.s0, rng.s1, rng.s2, rng.s3);
         a.Lp[p] = make_float4(Lp.x, Lp.y, Lp.z, 0.f);
     } else {
-        a.Lp[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+        // Background pixel (MinimalPathTracer.rt.slang:462-465).
+        f3 bg = mk3(0.f);
+        if (s.envTex && (s.envFlags & kEnvBackground)) {
+            f3 o, d;
+            float tmin, tmax;
+            if (!MULTI && a.viewW) {
+                const float4 v = a.viewW[pix];
+                d = mk3(-v.x, -v.y, -v.z);
+            } else {
+                primaryRay(*c.cam, px, py, c.W, c.H, o, d, tmin, tmax);
+            }
+            bg = envEval(s, d);
+        }
+        a.Lp[p] = make_float4(bg.x, bg.y, bg.z, 0.f);
     }
     a.Ls[p] = make_float4(0.f, 0.f, 0.f, 0.f);
     uint32_t slot = waveAppend(cont, &a.counters[0]);
@@ -438,6 +451,12 @@ __global__ void kShade(SceneDev s0, MptArgs a, uint32_t depth) {
                     }
                     a.rng[pathId] = make_uint4(rng.s0, rng.s1, rng.s2, rng.s3);
                 }
+            } else if (s.envTex && (s.envFlags & kEnvLight) && (a.computeDirect || depth > 0)) {
+                // scatterMiss (MinimalPathTracer.rt.slang:475-486): the env map lights the missed ray.
+                const float4 rd = a.rayQ[qi][2 * size_t(i) + 1];
+                const float4 t4 = a.thp[pathId], ls4 = a.Ls[pathId];
+                const f3 Le = envEval(s, mk3(rd.x, rd.y, rd.z));
+                a.Ls[pathId] = make_float4(ls4.x + t4.x * Le.x, ls4.y + t4.y * Le.y, ls4.z + t4.z * Le.z, 0.f);
             }
         }
         uint32_t slot = waveAppend(cont, &a.counters[depth + 1]);
@@ -535,7 +554,7 @@ __global__ __launch_bounds__(B) void kTraceBounce(SceneDev s, MptArgs a, const f
         if (!hitCount) {
             if (!vis) hits[j] = hit ? make_uint4(h.tri, f2u(h.t), f2u(h.b1), f2u(h.b2)) : make_uint4(kInvalid, 0u, 0u, 0u);
         } else {
-            const bool keep = !vis && hit;
+            const bool keep = !vis && (hit || a.keepMisses);
             const uint32_t k = waveAppend(keep, hitCount);
             if (keep) {
                 hits[k] = make_uint4(h.tri, f2u(h.t), f2u(h.b1), f2u(h.b2));

@@ -65,7 +65,13 @@ struct SceneDev {
     uint32_t nMeshes;
     HitInfoLayout hit;
     float4 viewProjZ, viewProjW;  // rows 2 and 3 of the camera's view-projection matrix (depth = z / w)
+    // Environment map (EnvMap.slang): lat-long RGB texels (null: none), world-to-local rotation
+    // rows, intensity * tint, flags kEnvLight / kEnvBackground.
+    const float* envTex;
+    uint32_t envW, envH, envFlags;
+    f3 envInv0, envInv1, envInv2, envScale;
 };
+constexpr uint32_t kEnvLight = 1u, kEnvBackground = 2u;
 
 // ------------------------------------------------------------------ shading data
 // Scene.slang:440-476 and MaterialFactory.slang:78-104 with world-space geometry.
@@ -150,6 +156,31 @@ FHP_HD f3 texFetch(const SceneDev& s, const float* texels, int tex, float u, flo
         tap->w[0] = wx0 * wy0; tap->w[1] = fx * wy0; tap->w[2] = wx0 * fy; tap->w[3] = fx * fy;
     }
     return top * wy0 + bot * fy;
+}
+
+// EnvMap::eval(dir) (EnvMap.slang:58-68): direction to local space, lat-long coordinates
+// world_to_latlong_map (MathHelpers.slang:111-118), bilinear filter at mip 0 wrapping in u and
+// clamping in v (the env sampler of EnvMap.cpp:131-135), times intensity * tint.
+FHP_HD f3 envEval(const SceneDev& s, f3 dir) {
+    const f3 p = normalize(mk3(dot(s.envInv0, dir), dot(s.envInv1, dir), dot(s.envInv2, dir)));
+    const float u = atan2f(p.x, -p.z) * (0.5f * kInvPi) + 0.5f;
+    const float v = acosf(fminf(fmaxf(p.y, -1.f), 1.f)) * kInvPi;
+    const int W = int(s.envW), H = int(s.envH);
+    const float x = u * float(W) - 0.5f, y = v * float(H) - 0.5f;
+    const float fx0 = floorf(x), fy0 = floorf(y);
+    const float fx = x - fx0, fy = y - fy0;
+    const int x0 = int(fx0), y0 = int(fy0);
+    const int xa = imod(x0, W), xb = imod(x0 + 1, W);
+    const int ya = y0 < 0 ? 0 : (y0 >= H ? H - 1 : y0), yb = y0 + 1 < 0 ? 0 : (y0 + 1 >= H ? H - 1 : y0 + 1);
+    const float* q00 = s.envTex + 3 * (size_t(ya) * W + xa);
+    const float* q10 = s.envTex + 3 * (size_t(ya) * W + xb);
+    const float* q01 = s.envTex + 3 * (size_t(yb) * W + xa);
+    const float* q11 = s.envTex + 3 * (size_t(yb) * W + xb);
+    const float wx0 = 1.f - fx, wy0 = 1.f - fy;
+    const f3 top = mk3(q00[0], q00[1], q00[2]) * wx0 + mk3(q10[0], q10[1], q10[2]) * fx;
+    const f3 bot = mk3(q01[0], q01[1], q01[2]) * wx0 + mk3(q11[0], q11[1], q11[2]) * fx;
+    const f3 c = top * wy0 + bot * fy;
+    return mk3(c.x * s.envScale.x, c.y * s.envScale.y, c.z * s.envScale.z);
 }
 
 // ------------------------------------------------------------------ material instance

@@ -19,8 +19,9 @@ Objects available to a scene script, after Falcor's bindings:
   (Transform.cpp:151-213); Camera with position, target, up, focalLength, frameHeight,
   focalDistance, nearPlane, farPlane, aspectRatio (Camera.cpp:448-458; fovY from the focal
   length and frame height); sceneBuilder.addNode/addTriangleMesh/addMeshInstance/addCamera.
-Analytic lights (PointLight, DirectionalLight, ...), environment maps and other builder calls are
-accepted and ignored with a warning (this renderer's lights are emissive triangles).
+EnvMap(path) with intensity, tint and rotation, assigned to sceneBuilder.envMap, becomes the
+scene's environment map. Analytic lights (PointLight, DirectionalLight, ...) are accepted and
+ignored with a warning (this renderer's lights are emissive triangles and the environment map).
 """
 from __future__ import annotations
 
@@ -190,6 +191,16 @@ class _Ignored:
         object.__setattr__(self, k, v)
 
 
+class _EnvMap:
+    """EnvMap(path) with intensity, tint and rotation (degrees XYZ) (EnvMap.cpp:140-160)."""
+
+    def __init__(self, path):
+        self.path = str(path)
+        self.intensity = 1.0
+        self.tint = Vec(1, 1, 1)
+        self.rotation = Vec(0, 0, 0)
+
+
 class _TriangleMeshAPI:
     def __init__(self, base: Path, warnings: list):
         self.base = base
@@ -273,8 +284,8 @@ class _SceneBuilderAPI:
         return 0
 
     def __setattr__(self, k, v):
-        if k == "envMap" and v is not None:
-            self.warnings.append("environment map ignored (not modelled)")
+        if k == "envMap" and v is not None and not isinstance(v, _EnvMap):
+            raise PySceneError("sceneBuilder.envMap must be an EnvMap")
         object.__setattr__(self, k, v)
 
 
@@ -408,7 +419,7 @@ class _Interp:
             obj = self.expr(target.value)
             if isinstance(obj, _SceneBuilderAPI):
                 ok = target.attr == "envMap"
-            elif isinstance(obj, (_Material, _Camera, _Transform)):
+            elif isinstance(obj, (_Material, _Camera, _Transform, _EnvMap)):
                 # Only the existing data attributes of these bindings.
                 ok = hasattr(obj, target.attr) and not callable(getattr(obj, target.attr))
             else:
@@ -484,7 +495,8 @@ def load_pyscene(path, aspect: float = 1.0, warnings: list | None = None) -> S.S
                                   "cos": staticmethod(math.cos), "sqrt": staticmethod(math.sqrt)}),
         "True": True, "False": False, "None": None,
     }
-    for kind in ("PointLight", "DirectionalLight", "DistantLight", "RectLight", "DiscLight", "SphereLight", "EnvMap",
+    env["EnvMap"] = _EnvMap
+    for kind in ("PointLight", "DirectionalLight", "DistantLight", "RectLight", "DiscLight", "SphereLight",
                  "GridVolume", "Grid", "SDFGrid", "SDFSVS", "SDFSBS", "SDFSVO", "NDSDFGrid"):
         env[kind] = (lambda k: (lambda *a, **kw: _Ignored(k, *a, **kw)))(kind)
     _Interp(env).run(tree.body)
@@ -494,4 +506,12 @@ def load_pyscene(path, aspect: float = 1.0, warnings: list | None = None) -> S.S
     if not api.cameras:
         warns.append("no camera in the scene: default camera")
     sb.camera = cam.to_scene(aspect)
-    return sb.build()
+    data = sb.build()
+    if api.envMap is not None:
+        from .image_io import read_rgba
+        e = api.envMap
+        tint = e.tint.v if isinstance(e.tint, Vec) else (float(e.tint),) * 3
+        rot = e.rotation.v if isinstance(e.rotation, Vec) else (0.0, 0.0, 0.0)
+        data.env_map = {"texels": read_rgba(base / e.path)[..., :3], "intensity": float(e.intensity),
+                        "tint": tuple(tint[:3]), "rotation": tuple(rot[:3])}
+    return data

@@ -71,6 +71,9 @@ class Scene:
         _lib.check(L.fhp_scene_create(C.byref(d), C.byref(h)))
         self.handle = h
         self.materials = a["mats"].copy()
+        if data.env_map is not None:
+            e = data.env_map
+            self.set_env_map(e["texels"], e.get("intensity", 1.0), e.get("tint", (1, 1, 1)), e.get("rotation", (0, 0, 0)))
         # Incremented by every parameter change (materials, textures, vertices, sampling freeze);
         # AccumulatePass restarts when it moves (the scene updates of AccumulatePass.cpp:141-147).
         self.version = 0
@@ -157,6 +160,26 @@ class Scene:
         self._arrays["pos"][v0:v1] = p
         if n is not None:
             self._arrays["nrm"][v0:v1] = n
+        self.version += 1
+
+    def set_env_map(self, texels=None, intensity: float = 1.0, tint=(1.0, 1.0, 1.0), rotation=(0.0, 0.0, 0.0),
+                    use_light: bool = True, use_background: bool = True):
+        """Environment map (EnvMap): [H, W, 3 or 4] lat-long radiance (array or image path), intensity,
+        tint, rotation in degrees XYZ; it lights rays that leave the scene (use_light) and is the
+        background of primary misses (use_background). None removes it."""
+        if texels is None:
+            _lib.check(_lib.load().fhp_scene_set_env_map(self.handle, None, 0, 0, None, 0))
+        else:
+            if isinstance(texels, (str, Path)):
+                from .image_io import read_rgba
+                texels = read_rgba(texels)
+            t = np.ascontiguousarray(np.asarray(texels, np.float32)[..., :3])
+            if t.ndim != 3 or t.shape[2] != 3:
+                raise ValueError("env map texels must be [H, W, 3 or 4]")
+            prm = np.array([intensity, *tint, *rotation], np.float32)
+            flags = (1 if use_light else 0) | (2 if use_background else 0)
+            _lib.check(_lib.load().fhp_scene_set_env_map(self.handle, t.ctypes.data, t.shape[1], t.shape[0],
+                                                         prm.ctypes.data, flags))
         self.version += 1
 
     def set_texels(self, texels):

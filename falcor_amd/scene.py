@@ -246,6 +246,8 @@ class SceneData:
     texels: np.ndarray = field(default_factory=lambda: np.zeros(0, np.float32))
     mesh_names: list = field(default_factory=list)
     mesh_vertex_offsets: np.ndarray | None = None  # mesh m owns vertices [off[m], off[m + 1])
+    # Environment map: {"texels": [H, W, 3], "intensity", "tint", "rotation" (degrees XYZ)} or None.
+    env_map: dict | None = None
 
     @property
     def num_triangles(self) -> int:

@@ -114,6 +114,13 @@ int fhp_scene_set_texels(FhpScene scene, const float* d_texels, uint64_t num_flo
  * differentiates (sampling is detached in StandardBSDF.slang:155-178 and PTUtils.slang:136-249).
  * enable = 0 returns to normal rendering. No counterpart in the reference. */
 int fhp_scene_freeze_sampling(FhpScene scene, int enable, FhpStream stream);
+/* Environment map (EnvMap, Source/Falcor/Scene/Lights/EnvMap.cpp / EnvMap.slang): a lat-long
+ * RGB fp32 map of height * width texels (h_texels, row 0 at the top: v = acos(y) / pi), h_params =
+ * 7 floats [intensity, tint rgb, rotation in degrees xyz], flags 1 = light the scene (rays that
+ * miss it pick up its radiance, MinimalPathTracer.rt.slang:475-486), 2 = background of primary
+ * misses (:462-465), i.e. Scene::useEnvLight / useEnvBackground. width = 0 removes the map. */
+int fhp_scene_set_env_map(FhpScene scene, const float* h_texels, uint32_t width, uint32_t height, const float* h_params,
+                          uint32_t flags);
 /* Light BVH export (for checks): nodes 16 floats each, leaf light lists, per-light bitmasks
  * (2 words each). Sizes from fhp_scene_info. */
 int fhp_scene_light_bvh(FhpScene scene, float* h_nodes, uint32_t* h_tri, uint32_t* h_masks);

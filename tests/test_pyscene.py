@@ -174,3 +174,17 @@ def test_reference_cornell_box_if_present():
     assert d.mesh_names == ref.mesh_names
     np.testing.assert_allclose(d.positions, ref.positions, atol=1e-6)
     assert any("PointLight" in w for w in warns)
+
+
+def test_environment_map(tmp_path):
+    from falcor_amd.image_io import write_exr
+    env = np.random.default_rng(0).random((4, 8, 3)).astype(np.float32)
+    write_exr(tmp_path / "sky.exr", env)
+    (tmp_path / "e.pyscene").write_text(
+        "m = StandardMaterial('m')\nq = TriangleMesh.createQuad()\n"
+        "sceneBuilder.addMeshInstance(sceneBuilder.addNode('q'), sceneBuilder.addTriangleMesh(q, m))\n"
+        "e = EnvMap('sky.exr')\ne.intensity = 2.5\ne.tint = float3(1, 0.5, 0.5)\ne.rotation = float3(0, 90, 0)\n"
+        "sceneBuilder.envMap = e\n")
+    d = load_pyscene(tmp_path / "e.pyscene")
+    assert np.array_equal(d.env_map["texels"], env)
+    assert d.env_map["intensity"] == 2.5 and d.env_map["tint"] == (1, 0.5, 0.5) and d.env_map["rotation"] == (0, 90, 0)
