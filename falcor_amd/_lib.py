@@ -109,6 +109,7 @@ _SIGS = {
     "fhp_vbuffer_execute": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
                                       C.c_void_p, C.c_void_p, C.c_void_p]),
     "fhp_pass_set_input": (C.c_int, [C.c_void_p, C.c_char_p, C.c_void_p]),
+    "fhp_scene_lights": (C.c_int, [C.c_void_p, C.c_void_p]),
     "fhp_scene_set_env_map": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_uint32]),
     "fhp_scene_update_vertices_device": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
     "fhp_accumulate_frame_count": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),

@@ -555,6 +555,12 @@ int fhp_scene_update_vertices_device(FhpScene s, uint32_t mesh, const float* d_p
     return FHP_OK;
 }
 
+int fhp_scene_lights(FhpScene s, float* h_lights) {
+    if (!s || !h_lights) return fail(FHP_EINVAL, "null argument");
+    std::copy(s->h.lights.begin(), s->h.lights.end(), h_lights);
+    return FHP_OK;
+}
+
 int fhp_scene_set_env_map(FhpScene s, const float* h_texels, uint32_t W, uint32_t H, const float* h_params, uint32_t flags) {
     if (!s) return fail(FHP_EINVAL, "null argument");
     SceneDev& dv = s->dev;

@@ -15,7 +15,8 @@ namespace fhp {
 
 // Material record (16 floats), identical to the scene description the Python layer builds.
 enum MatField { M_TYPE = 0, M_BASE = 1, M_ROUGH = 4, M_METAL = 5, M_IOR = 6, M_EMIT = 7, M_EMIT_FACTOR = 10,
-                M_DOUBLE_SIDED = 11, M_TEX = 12, M_STRIDE = 16 };
+                M_DOUBLE_SIDED = 11, M_TEX = 12, M_EMIT_TEX = 13, M_STRIDE = 16 };
+// M_EMIT_TEX holds the emissive texture index + 1 (0: the emissive colour is used).
 enum MatType { MAT_STANDARD = 0, MAT_PBRT_DIFFUSE = 1 };
 
 // BVH4 child reference: bit 31 = leaf, bits 28..30 = count - 1, bits 0..27 = first triangle.
@@ -187,6 +188,7 @@ FHP_HD f3 envEval(const SceneDev& s, f3 dir) {
 struct Material {
     int type;
     f3 diffuse, specular, emission, base;
+    f3 emitColor;  // emissive colour before emissiveFactor (texture value when textured)
     float roughness, metallic, F0;
     bool doubleSided;
     int tex;
@@ -207,6 +209,7 @@ FHP_HD Material loadMaterial(const SceneDev& s, const float* mats, const float* 
     if (m.type == MAT_PBRT_DIFFUSE) {
         m.diffuse = base; m.specular = mk3(0.f); m.roughness = 0.f; m.metallic = 0.f; m.F0 = 0.f;
         m.emission = mk3(0.f);
+        m.emitColor = mk3(0.f);
         return m;
     }
     float ior = r[M_IOR];
@@ -216,8 +219,15 @@ FHP_HD Material loadMaterial(const SceneDev& s, const float* mats, const float* 
     m.diffuse = base + (mk3(0.f) - base) * m.metallic;                    // lerp(base, 0, metallic)
     m.specular = mk3(m.F0) + (base - mk3(m.F0)) * m.metallic;             // lerp(F0, base, metallic)
     m.roughness = r[M_ROUGH];
+    // MaterialSystem::evalEmissive (MaterialSystem.slang:201-213): the emissive texture (default
+    // sampler, mip 0) or colour, times emissiveFactor, on the front side only.
+    m.emitColor = mk3(0.f);
     m.emission = mk3(0.f);
-    if (sd.frontFacing) m.emission = mk3(r[M_EMIT], r[M_EMIT + 1], r[M_EMIT + 2]) * r[M_EMIT_FACTOR];
+    if (sd.frontFacing) {
+        const int etex = int(r[M_EMIT_TEX]) - 1;
+        m.emitColor = etex >= 0 ? texFetch(s, texels, etex, sd.uvx, sd.uvy, nullptr) : mk3(r[M_EMIT], r[M_EMIT + 1], r[M_EMIT + 2]);
+        m.emission = m.emitColor * r[M_EMIT_FACTOR];
+    }
     return m;
 }
 
@@ -431,6 +441,13 @@ FHP_HD bool sampleTri(const SceneDev& s, f3 posW, uint32_t li, float ux, float u
     float cosT = dot(ls.normalW, -ls.dir);
     if (cosT <= 0.f) return false;
     ls.Le = mk3(e.x, e.y, e.z);
+    if (e.w > 0.f) {
+        // Textured emitter: evalEmissive at the sample's texture coordinates; e.xyz = emissiveFactor.
+        const uint32_t* tv = s.idx + 3 * size_t(f2u(c.w));
+        const float4 t0 = s.vtx[2 * size_t(tv[0]) + 1], t1 = s.vtx[2 * size_t(tv[1]) + 1], t2 = s.vtx[2 * size_t(tv[2]) + 1];
+        const float u = t0.z * bc.x + t1.z * bc.y + t2.z * bc.z, v = t0.w * bc.x + t1.w * bc.y + t2.w * bc.z;
+        ls.Le = texFetch(s, s.texels, int(e.w) - 1, u, v, nullptr) * ls.Le;
+    }
     float denom = fmaxf_(kFltMin, cosT * a.w);
     ls.pdf = d2 / denom;
     return true;

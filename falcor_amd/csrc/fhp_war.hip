@@ -210,8 +210,8 @@ __device__ f3 warPath(const SceneDev& s, const WarArgs& a, uint32_t px, uint32_t
             if (ADJ && m.type == MAT_STANDARD && sd.frontFacing) {
                 const float* r = s.mats + size_t(sd.mat) * M_STRIDE;
                 f3 dE = dL * thp * mis;
-                g.emit = g.emit + dE * r[M_EMIT_FACTOR];
-                g.emitFactor += dot(dE, mk3(r[M_EMIT], r[M_EMIT + 1], r[M_EMIT + 2]));
+                if (r[M_EMIT_TEX] <= 0.f) g.emit = g.emit + dE * r[M_EMIT_FACTOR];  // textured: no colour slot
+                g.emitFactor += dot(dE, m.emitColor);
             }
         }
         bool stop = len >= a.maxBounces + 1;

@@ -319,6 +319,82 @@ void buildBVH4(HostScene& s, uint32_t maxLeaf) {
 // triangle order (LightCollection.cpp:220-262); world-space normal and area as in
 // Scene.slang:392-407; flux = luminance(Le) * area * pi (FinalizeIntegration.cs.slang:108);
 // active list = flux > 0 (LightCollection.cpp:494-548).
+namespace {
+// Area of the triangle (texel-space vertices) inside the unit square [x0, x0 + 1] x [y0, y0 + 1]:
+// the triangle clipped against the square's four half-planes, then the shoelace formula.
+double clippedArea(const double v[3][2], double x0, double y0) {
+    double poly[16][2], tmp[16][2];
+    int n = 3;
+    for (int i = 0; i < 3; i++) { poly[i][0] = v[i][0]; poly[i][1] = v[i][1]; }
+    for (int side = 0; side < 4 && n > 0; side++) {
+        const int axis = side >> 1;
+        const double lim = (side & 1) ? (axis ? y0 + 1.0 : x0 + 1.0) : (axis ? y0 : x0);
+        auto inside = [&](const double* p) { return (side & 1) ? p[axis] <= lim : p[axis] >= lim; };
+        int m = 0;
+        for (int i = 0; i < n; i++) {
+            const double* a = poly[i];
+            const double* b = poly[(i + 1) % n];
+            const bool ia = inside(a), ib = inside(b);
+            if (ia) { tmp[m][0] = a[0]; tmp[m][1] = a[1]; m++; }
+            if (ia != ib) {
+                const double t = (lim - a[axis]) / (b[axis] - a[axis]);
+                tmp[m][0] = a[0] + t * (b[0] - a[0]);
+                tmp[m][1] = a[1] + t * (b[1] - a[1]);
+                m++;
+            }
+        }
+        n = m;
+        for (int i = 0; i < n; i++) { poly[i][0] = tmp[i][0]; poly[i][1] = tmp[i][1]; }
+    }
+    double a2 = 0.0;
+    for (int i = 0; i < n; i++) a2 += poly[i][0] * poly[(i + 1) % n][1] - poly[(i + 1) % n][0] * poly[i][1];
+    return std::fabs(0.5 * a2);
+}
+
+// Average emissive texel value over a triangle's texture footprint (the emissive integrator of
+// EmissiveIntegrator.3d.slang and FinalizeIntegration.cs.slang): texel-space vertices
+// (uv - floor(uvMin)) * (W, H); every texel the triangle overlaps adds its nearest-sampled value
+// (wrap addressing) weighted by the covered area (at most 1); with no coverage (a triangle
+// degenerate in texture space) the mean of the texels at the three vertices. Summed in double
+// where the reference sums 29.35 fixed point.
+f3 averageEmission(const HostScene& s, int tex, const float uv[3][2]) {
+    const int32_t* td = &s.texDesc[4 * size_t(tex)];
+    const int W = td[1], H = td[2];
+    const float* T = &s.texels[size_t(td[0])];
+    auto texel = [&](long i, long j) {
+        const long x = ((i % W) + W) % W, y = ((j % H) + H) % H;
+        const float* q = T + 3 * (size_t(y) * W + x);
+        return mk3(q[0], q[1], q[2]);
+    };
+    const float umin = std::min(uv[0][0], std::min(uv[1][0], uv[2][0]));
+    const float vmin = std::min(uv[0][1], std::min(uv[1][1], uv[2][1]));
+    const float ou = std::floor(umin), ov = std::floor(vmin);
+    double v[3][2];
+    double lo[2] = {1e300, 1e300}, hi[2] = {-1e300, -1e300};
+    for (int k = 0; k < 3; k++) {
+        v[k][0] = double((uv[k][0] - ou) * float(W));
+        v[k][1] = double((uv[k][1] - ov) * float(H));
+        for (int a = 0; a < 2; a++) { lo[a] = std::min(lo[a], v[k][a]); hi[a] = std::max(hi[a], v[k][a]); }
+    }
+    double sum[3] = {0, 0, 0}, wsum = 0.0;
+    for (long j = long(std::floor(lo[1])); j < long(std::ceil(hi[1])); j++)
+        for (long i = long(std::floor(lo[0])); i < long(std::ceil(hi[0])); i++) {
+            const double w = std::min(clippedArea(v, double(i), double(j)), 1.0);
+            if (!(w > 0.0)) continue;
+            const f3 c = texel(i, j);
+            sum[0] += w * c.x; sum[1] += w * c.y; sum[2] += w * c.z;
+            wsum += w;
+        }
+    if (wsum > 0.0) return mk3(float(sum[0] / wsum), float(sum[1] / wsum), float(sum[2] / wsum));
+    f3 acc = mk3(0.f);
+    for (int k = 0; k < 3; k++) {
+        const float fu = uv[k][0] - std::floor(uv[k][0]), fv = uv[k][1] - std::floor(uv[k][1]);
+        acc = acc + texel(long(fu * float(W)), long(fv * float(H)));
+    }
+    return acc / 3.f;
+}
+}  // namespace
+
 void buildLights(HostScene& s) {
     s.lights.clear();
     s.activeLights.clear();
@@ -327,8 +403,26 @@ void buildLights(HostScene& s) {
     for (uint32_t t = 0; t < s.nT; t++) {
         const float* m = &s.mats[size_t(s.triMat[t]) * M_STRIDE];
         if (int(m[M_TYPE]) != MAT_STANDARD) continue;
+        const int etex = int(m[M_EMIT_TEX]) - 1;
         f3 Le = mk3(m[M_EMIT], m[M_EMIT + 1], m[M_EMIT + 2]) * m[M_EMIT_FACTOR];
-        if (!anyPos(Le)) continue;
+        // Light record radiance: Le, or for a textured emitter emissiveFactor (the texture is
+        // evaluated where the light is sampled) with the texture in the fourth word.
+        f3 recLe = Le;
+        float recTex = 0.f;
+        if (etex >= 0) {
+            if (!(m[M_EMIT_FACTOR] > 0.f)) continue;
+            float uvs[3][2];
+            for (int j = 0; j < 3; j++) {
+                const uint32_t vi = s.idx[3 * size_t(t) + j];
+                uvs[j][0] = s.uv[2 * size_t(vi)];
+                uvs[j][1] = s.uv[2 * size_t(vi) + 1];
+            }
+            Le = averageEmission(s, etex, uvs) * m[M_EMIT_FACTOR];
+            recLe = mk3(m[M_EMIT_FACTOR]);
+            recTex = float(etex + 1);
+        } else if (!anyPos(Le)) {
+            continue;
+        }
         f3 p[3];
         for (int j = 0; j < 3; j++) {
             const float* q = &s.pos[3 * size_t(s.idx[3 * size_t(t) + j])];
@@ -342,7 +436,7 @@ void buildLights(HostScene& s) {
         float tbf;
         std::memcpy(&tbf, &tb, 4);
         float rec[20] = {p[0].x, p[0].y, p[0].z, area, p[1].x, p[1].y, p[1].z, flux, p[2].x, p[2].y, p[2].z, tbf,
-                         nn.x, nn.y, nn.z, 0.f, Le.x, Le.y, Le.z, 0.f};
+                         nn.x, nn.y, nn.z, 0.f, recLe.x, recLe.y, recLe.z, recTex};
         s.lights.insert(s.lights.end(), rec, rec + 20);
         s.triToLight[t] = n;
         if (flux > 0.f) s.activeLights.push_back(n);

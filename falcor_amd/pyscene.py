@@ -109,18 +109,22 @@ class _Material:
         self.diffuseTransmission = 0.0
         self.texture_path = None
         self.texture_srgb = True
+        self.emissive_path = None
+        self.emissive_srgb = True
         self._warnings = []
 
     def loadTexture(self, slot, path, useSrgb=True):
-        if slot != MaterialTextureSlot.BaseColor:
-            self._warnings.append(f"material '{self.name}': texture slot {slot} ignored (base colour only)")
-            return True
-        self.texture_path, self.texture_srgb = str(path), bool(useSrgb)
+        if slot == MaterialTextureSlot.BaseColor:
+            self.texture_path, self.texture_srgb = str(path), bool(useSrgb)
+        elif slot == MaterialTextureSlot.Emissive:
+            self.emissive_path, self.emissive_srgb = str(path), bool(useSrgb)
+        else:
+            self._warnings.append(f"material '{self.name}': texture slot {slot} ignored (base colour and emissive only)")
         return True
 
     load_texture = loadTexture
 
-    def record(self, texture_index=-1) -> np.ndarray:
+    def record(self, texture_index=-1, emissive_index=-1) -> np.ndarray:
         bc = self.baseColor.v
         if self.kind == S.MAT_PBRT_DIFFUSE:
             return S.pbrt_diffuse_material([_half(c) for c in bc[:3]], self.doubleSided, texture_index)
@@ -129,7 +133,7 @@ class _Material:
         e = self.emissiveColor.v if isinstance(self.emissiveColor, Vec) else (self.emissiveColor,) * 3
         return S.standard_material([_half(c) for c in bc[:3]], _half(self.roughness), _half(self.metallic),
                                    _half(self.indexOfRefraction), e[:3], self.emissiveFactor, self.doubleSided,
-                                   texture_index)
+                                   texture_index, emissive_index)
 
 
 class StandardMaterial(_Material):
@@ -260,13 +264,16 @@ class _SceneBuilderAPI:
     def _material_id(self, m: _Material) -> int:
         key = id(m)
         if key not in self.materials:
-            tex = -1
-            if m.texture_path:
-                tp = self.base / m.texture_path
-                if tp not in self.textures:
-                    self.textures[tp] = self.sb.add_texture(_load_texture(tp, m.texture_srgb))
-                tex = self.textures[tp]
-            self.materials[key] = self.sb.add_material(m.record(tex))
+            def texture(path, srgb):
+                if not path:
+                    return -1
+                tp = self.base / path
+                if (tp, srgb) not in self.textures:
+                    self.textures[(tp, srgb)] = self.sb.add_texture(_load_texture(tp, srgb))
+                return self.textures[(tp, srgb)]
+            tex = texture(m.texture_path, m.texture_srgb)
+            etex = texture(m.emissive_path, m.emissive_srgb) if m.kind == S.MAT_STANDARD else -1
+            self.materials[key] = self.sb.add_material(m.record(tex, etex))
             self.warnings.extend(m._warnings)
         return self.materials[key]
 

@@ -194,6 +194,14 @@ class Scene:
         _lib.check(_lib.load().fhp_scene_freeze_sampling(self.handle, int(enable), _lib.stream_ptr()))
         self.version += 1
 
+    def lights(self) -> np.ndarray:
+        """Light collection records [num_lights, 20] (see fhp_scene_lights)."""
+        n = self.info()["num_lights"]
+        out = np.zeros((n, 20), np.float32)
+        if n:
+            _lib.check(_lib.load().fhp_scene_lights(self.handle, out.ctypes.data))
+        return out
+
     def light_bvh(self):
         i = self.info()
         nodes = np.zeros((i["light_bvh_nodes"], 16), np.float32)

@@ -185,10 +185,11 @@ def random_rotation(rng: np.random.Generator) -> np.ndarray:
 # --------------------------------------------------------------------------- materials
 # Material record fields (16 floats; see include/fhp.h).
 M_TYPE, M_BASE, M_ROUGH, M_METAL, M_IOR, M_EMIT, M_EMIT_FACTOR, M_DOUBLE_SIDED, M_TEX = 0, 1, 4, 5, 6, 7, 10, 11, 12
+M_EMIT_TEX = 13  # emissive texture index + 1 (0: the emissive colour)
 
 
 def standard_material(base=(1, 1, 1), roughness=0.0, metallic=0.0, ior=1.5, emissive=(0, 0, 0), emissive_factor=1.0,
-                      double_sided=False, texture=-1) -> np.ndarray:
+                      double_sided=False, texture=-1, emissive_texture=-1) -> np.ndarray:
     """StandardMaterial record (defaults of BasicMaterialData.slang:52-57: baseColor 1, specular 0)."""
     m = np.zeros(MAT_STRIDE, np.float32)
     m[0] = MAT_STANDARD
@@ -200,6 +201,7 @@ def standard_material(base=(1, 1, 1), roughness=0.0, metallic=0.0, ior=1.5, emis
     m[10] = emissive_factor
     m[11] = 1.0 if double_sided else 0.0
     m[12] = texture
+    m[M_EMIT_TEX] = emissive_texture + 1
     return m
 
 

@@ -40,7 +40,8 @@ const char* fhp_last_warning(void);
  * Scene description: world-space triangle geometry (instances flattened by the caller).
  * Materials are 16 floats each: [type (0 StandardMaterial, 1 PBRTDiffuseMaterial),
  * baseColor rgb, roughness, metallic, ior, emissive rgb, emissiveFactor, doubleSided,
- * baseColor texture index (-1 none), 0, 0, 0]. Textures are RGB fp32, texDesc = 4 ints
+ * baseColor texture index (-1 none), emissive texture index + 1 (0 none: the emissive colour is
+ * used; MaterialSystem.slang:201-213), 0, 0]. Textures are RGB fp32, texDesc = 4 ints
  * (float offset into texels, width, height, 0). Camera = 14 floats [position, target, up,
  * fovY (radians), aspect, focalDistance, nearZ, farZ].
  * Replaces Scene::create + SceneBuilder (Source/Falcor/Scene/Scene.cpp, SceneBuilder.cpp),
@@ -121,6 +122,11 @@ int fhp_scene_freeze_sampling(FhpScene scene, int enable, FhpStream stream);
  * misses (:462-465), i.e. Scene::useEnvLight / useEnvBackground. width = 0 removes the map. */
 int fhp_scene_set_env_map(FhpScene scene, const float* h_texels, uint32_t width, uint32_t height, const float* h_params,
                           uint32_t flags);
+/* Light collection export (for checks): num_lights records of 20 floats [p0 xyz, area, p1 xyz,
+ * flux, p2 xyz, triangle id (bits), normal xyz, 0, Le rgb (a textured emitter: emissiveFactor),
+ * emissive texture + 1 (0: constant)]. The flux of a textured emitter comes from the
+ * coverage-weighted texel average of EmissiveIntegrator.3d.slang / FinalizeIntegration.cs.slang. */
+int fhp_scene_lights(FhpScene scene, float* h_lights);
 /* Light BVH export (for checks): nodes 16 floats each, leaf light lists, per-light bitmasks
  * (2 words each). Sizes from fhp_scene_info. */
 int fhp_scene_light_bvh(FhpScene scene, float* h_nodes, uint32_t* h_tri, uint32_t* h_masks);

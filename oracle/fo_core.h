@@ -309,7 +309,7 @@ template <typename T> inline V3<T> sampleGGX_VNDF(T alpha, V3<T> wi, T ux, T uy,
 // Material record shared with the product's scene format (16 floats per material).
 enum MatField {
     M_TYPE = 0, M_BASE = 1, M_ROUGH = 4, M_METAL = 5, M_IOR = 6, M_EMIT = 7, M_EMIT_FACTOR = 10,
-    M_DOUBLE_SIDED = 11, M_TEX = 12, M_STRIDE = 16
+    M_DOUBLE_SIDED = 11, M_TEX = 12, M_EMIT_TEX = 13, M_STRIDE = 16  // M_EMIT_TEX: emissive texture + 1, 0 none
 };
 enum MatType { MAT_STANDARD = 0, MAT_PBRT_DIFFUSE = 1 };
 

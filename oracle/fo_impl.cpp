@@ -70,6 +70,72 @@ void buildOracleBVH(Scene& s) {
 // LightCollection.cpp:220-262 / UpdateTriangleVertices.cs.slang (world-space triangle data,
 // Scene.slang:392-407 for normal and area) / FinalizeIntegration.cs.slang:103-111 (flux for a
 // constant emissive colour). Triangles are listed in ascending global triangle order.
+// Texture-space coverage of one texel: the polygon triangle-inside-square obtained by clipping
+// against x >= i, x <= i + 1, y >= j, y <= j + 1 in turn, then its area.
+static double texelCoverage(const std::vector<std::array<double, 2>>& tri, double i, double j) {
+    std::vector<std::array<double, 2>> poly = tri, next;
+    const double bounds[4] = {i, i + 1.0, j, j + 1.0};
+    for (int e = 0; e < 4 && !poly.empty(); e++) {
+        const int ax = e / 2;
+        const bool keepAbove = (e % 2) == 0;
+        next.clear();
+        for (size_t k = 0; k < poly.size(); k++) {
+            const auto& P = poly[k];
+            const auto& Q = poly[(k + 1) % poly.size()];
+            const double dp = keepAbove ? P[ax] - bounds[e] : bounds[e] - P[ax];
+            const double dq = keepAbove ? Q[ax] - bounds[e] : bounds[e] - Q[ax];
+            if (dp >= 0) next.push_back(P);
+            if ((dp >= 0) != (dq >= 0)) {
+                const double t = dp / (dp - dq);
+                next.push_back({P[0] + t * (Q[0] - P[0]), P[1] + t * (Q[1] - P[1])});
+            }
+        }
+        poly.swap(next);
+    }
+    double twice = 0.0;
+    for (size_t k = 0; k < poly.size(); k++) {
+        const auto& P = poly[k];
+        const auto& Q = poly[(k + 1) % poly.size()];
+        twice += P[0] * Q[1] - Q[0] * P[1];
+    }
+    return std::fabs(twice) * 0.5;
+}
+
+// EmissiveIntegrator.3d.slang + FinalizeIntegration.cs.slang:63-111: the coverage-weighted average of
+// the nearest texels (wrap addressing) over the triangle's texture footprint, placed at
+// (uv - floor(min uv)) * (W, H); a triangle with no coverage averages the texels at its vertices.
+static V3f textureAverage(const Scene& s, int tex, const float uv[3][2]) {
+    const int32_t* td = &s.texDesc[4 * tex];
+    const long W = td[1], H = td[2];
+    auto fetch = [&](long x, long y) {
+        x = ((x % W) + W) % W;
+        y = ((y % H) + H) % H;
+        const float* q = &s.texels[size_t(td[0]) + 3 * size_t(y * W + x)];
+        return V3f(q[0], q[1], q[2]);
+    };
+    const float ou = std::floor(std::min({uv[0][0], uv[1][0], uv[2][0]}));
+    const float ov = std::floor(std::min({uv[0][1], uv[1][1], uv[2][1]}));
+    std::vector<std::array<double, 2>> tri(3);
+    for (int k = 0; k < 3; k++) tri[k] = {double((uv[k][0] - ou) * float(W)), double((uv[k][1] - ov) * float(H))};
+    const double x0 = std::min({tri[0][0], tri[1][0], tri[2][0]}), x1 = std::max({tri[0][0], tri[1][0], tri[2][0]});
+    const double y0 = std::min({tri[0][1], tri[1][1], tri[2][1]}), y1 = std::max({tri[0][1], tri[1][1], tri[2][1]});
+    double acc[3] = {0.0, 0.0, 0.0}, weight = 0.0;
+    for (long y = long(std::floor(y0)); y < long(std::ceil(y1)); y++)
+        for (long x = long(std::floor(x0)); x < long(std::ceil(x1)); x++) {
+            double w = texelCoverage(tri, double(x), double(y));
+            if (w > 1.0) w = 1.0;
+            if (!(w > 0.0)) continue;
+            const V3f c = fetch(x, y);
+            acc[0] += w * c.x; acc[1] += w * c.y; acc[2] += w * c.z;
+            weight += w;
+        }
+    if (weight > 0.0) return V3f(float(acc[0] / weight), float(acc[1] / weight), float(acc[2] / weight));
+    V3f sum(0.f);
+    for (int k = 0; k < 3; k++)
+        sum = sum + fetch(long(wrapCoord(uv[k][0]) * float(W)), long(wrapCoord(uv[k][1]) * float(H)));
+    return sum / 3.f;
+}
+
 void buildLightCollection(Scene& s) {
     s.lights.clear();
     s.activeLights.clear();
@@ -77,8 +143,10 @@ void buildLightCollection(Scene& s) {
     for (uint32_t t = 0; t < s.nT; t++) {
         const float* m = s.mat(s.triMat[t]);
         if (int(m[M_TYPE]) != MAT_STANDARD) continue;
+        const int etex = int(m[M_EMIT_TEX]) - 1;
         V3f Le = V3f(m[M_EMIT], m[M_EMIT + 1], m[M_EMIT + 2]) * m[M_EMIT_FACTOR];
-        if (!(Le.x > 0.f || Le.y > 0.f || Le.z > 0.f)) continue;
+        if (etex < 0 && !(Le.x > 0.f || Le.y > 0.f || Le.z > 0.f)) continue;
+        if (etex >= 0 && !(m[M_EMIT_FACTOR] > 0.f)) continue;
         EmissiveTri e;
         s.triVerts(t, e.p);
         for (int j = 0; j < 3; j++) {
@@ -89,6 +157,11 @@ void buildLightCollection(Scene& s) {
         e.area = 0.5f * length(N);
         e.normal = normalize(N);
         e.Le = Le;
+        if (etex >= 0) {
+            Le = textureAverage(s, etex, e.uv) * m[M_EMIT_FACTOR];
+            e.Le = V3f(m[M_EMIT_FACTOR]);
+            e.etex = etex;
+        }
         e.flux = luminance(Le) * e.area * Consts<float>::pi;
         e.tri = t;
         s.triToLight[t] = uint32_t(s.lights.size());

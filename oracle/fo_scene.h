@@ -47,7 +47,8 @@ struct EmissiveTri {
     float uv[3][2];
     V3f normal;
     float area;
-    V3f Le;
+    V3f Le;          // constant emitter: radiance; textured emitter: emissiveFactor (x3)
+    int etex = -1;   // emissive texture (MaterialSystem::evalEmissive at the sample's uv), -1 none
     float flux;
     uint32_t tri;
 };
@@ -231,6 +232,13 @@ struct MatInst {
     V3f emission;
     StdBsdf<float> bsdf;  // initialised per wi
 };
+// MaterialSystem::evalEmissive (MaterialSystem.slang:201-213): emissive texture (bilinear, mip 0) or
+// colour, times emissiveFactor.
+inline V3f evalEmissive(const Scene& s, const float* m, float u, float v) {
+    const int etex = int(m[M_EMIT_TEX]) - 1;
+    V3f c = etex >= 0 ? sampleTexture(s, etex, u, v) : V3f(m[M_EMIT], m[M_EMIT + 1], m[M_EMIT + 2]);
+    return c * m[M_EMIT_FACTOR];
+}
 inline BsdfData<float> bsdfDataFrom(const Scene& s, const float* m, const ShadingData& sd, bool sampling) {
     BsdfData<float> d;
     d.type = int(m[M_TYPE]);
@@ -257,8 +265,7 @@ inline MatInst makeMatInst(const Scene& s, const ShadingData& sd) {
         mi.sf = sd.frame;
         if (!sd.frontFacing && m[M_DOUBLE_SIDED] != 0.f) mi.sf.N = -mi.sf.N;
         mi.emission = V3f(0.f);
-        if (mi.d.type == MAT_STANDARD && sd.frontFacing)
-            mi.emission = V3f(m[M_EMIT], m[M_EMIT + 1], m[M_EMIT + 2]) * m[M_EMIT_FACTOR];
+        if (mi.d.type == MAT_STANDARD && sd.frontFacing) mi.emission = evalEmissive(s, m, sd.uvx, sd.uvy);
         return mi;
     }
     mi.d.type = int(m[M_TYPE]);
@@ -286,7 +293,7 @@ inline MatInst makeMatInst(const Scene& s, const ShadingData& sd) {
     mi.d.roughness = m[M_ROUGH];
     mi.d.metallic = metal;
     mi.emission = V3f(0.f);
-    if (sd.frontFacing) mi.emission = V3f(m[M_EMIT], m[M_EMIT + 1], m[M_EMIT + 2]) * m[M_EMIT_FACTOR];
+    if (sd.frontFacing) mi.emission = evalEmissive(s, m, sd.uvx, sd.uvy);
     mi.ds = mi.d;
     return mi;
 }
@@ -384,6 +391,11 @@ inline bool sampleTriangle(const Scene& s, V3f posW, uint32_t li, float ux, floa
     float cosTheta = dot(ls.normalW, -ls.dir);
     if (cosTheta <= 0.f) return false;
     ls.Le = tri.Le;
+    if (tri.etex >= 0) {
+        const float u = tri.uv[0][0] * b.x + tri.uv[1][0] * b.y + tri.uv[2][0] * b.z;
+        const float v = tri.uv[0][1] * b.x + tri.uv[1][1] * b.y + tri.uv[2][1] * b.z;
+        ls.Le = sampleTexture(s, tri.etex, u, v) * tri.Le;
+    }
     float denom = std::max(std::numeric_limits<float>::min(), cosTheta * tri.area);
     ls.pdf = distSqr / denom;
     return true;
