@@ -10,7 +10,8 @@ scripts/inv-rendering/shape_optimization/diff_render_module.py) and render_prima
 
 Material parameters use the reference's 20-float layout per material
 (StandardMaterialParamLayout.slang:32-41): baseColor 0-2, metallic 3, roughness 4, ior 5,
-emissive 11-13, emissiveFactor 14 (slots 6-10 are transmission terms, unused here).
+transmissionColor 6-8, diffuseTransmission 9, specularTransmission 10, emissive 11-13,
+emissiveFactor 14. The adjoint differentiates reflection lobes only (no gradients reach 6-10).
 """
 from __future__ import annotations
 
@@ -23,13 +24,16 @@ PARAM_COUNT = 20
 
 
 def material_params(records: np.ndarray) -> np.ndarray:
-    """Scene material records (16 floats each) -> reference parameter layout (20 floats each)."""
+    """Scene material records (20 floats each) -> reference parameter layout (20 floats each)."""
     r = np.asarray(records, np.float32)
     p = np.zeros((r.shape[0], PARAM_COUNT), np.float32)
     p[:, 0:3] = r[:, S.M_BASE:S.M_BASE + 3]
     p[:, 3] = r[:, S.M_METAL]
     p[:, 4] = r[:, S.M_ROUGH]
     p[:, 5] = r[:, S.M_IOR]
+    p[:, 6:9] = r[:, S.M_TRANS:S.M_TRANS + 3]
+    p[:, 9] = r[:, S.M_DIFF_TRANS]
+    p[:, 10] = r[:, S.M_SPEC_TRANS]
     p[:, 11:14] = r[:, S.M_EMIT:S.M_EMIT + 3]
     p[:, 14] = r[:, S.M_EMIT_FACTOR]
     return p
@@ -43,6 +47,9 @@ def material_records(params: np.ndarray, like: np.ndarray) -> np.ndarray:
     r[:, S.M_METAL] = p[:, 3]
     r[:, S.M_ROUGH] = p[:, 4]
     r[:, S.M_IOR] = p[:, 5]
+    r[:, S.M_TRANS:S.M_TRANS + 3] = p[:, 6:9]
+    r[:, S.M_DIFF_TRANS] = p[:, 9]
+    r[:, S.M_SPEC_TRANS] = p[:, 10]
     r[:, S.M_EMIT:S.M_EMIT + 3] = p[:, 11:14]
     r[:, S.M_EMIT_FACTOR] = p[:, 14]
     return r

@@ -471,6 +471,8 @@ int fhp_scene_create(const FhpSceneDesc* d, FhpScene* out) {
     for (uint32_t m = 0; m < h.nM; m++) {
         int tex = int(h.mats[size_t(m) * M_STRIDE + M_TEX]);
         if (tex >= int(h.nTex)) return fail(FHP_EINVAL, "material texture index out of range");
+        if (int(h.mats[size_t(m) * M_STRIDE + M_EMIT_TEX]) - 1 >= int(h.nTex))
+            return fail(FHP_EINVAL, "material emissive texture index out of range");
     }
     std::memcpy(h.cam, d->h_camera, sizeof(h.cam));
     if (d->num_meshes) {
@@ -626,6 +628,8 @@ int fhp_scene_set_materials(FhpScene s, const float* mats) {
     for (uint32_t m = 0; m < s->h.nM; m++) {
         int tex = int(mats[size_t(m) * M_STRIDE + M_TEX]);
         if (tex >= int(s->h.nTex)) return fail(FHP_EINVAL, "material texture index out of range");
+        if (int(mats[size_t(m) * M_STRIDE + M_EMIT_TEX]) - 1 >= int(s->h.nTex))
+            return fail(FHP_EINVAL, "material emissive texture index out of range");
     }
     s->h.mats.assign(mats, mats + size_t(s->h.nM) * M_STRIDE);
     HIP_OR_FAIL(hipMemcpy(s->mats.p, mats, size_t(s->h.nM) * M_STRIDE * 4, hipMemcpyHostToDevice));
@@ -1060,6 +1064,12 @@ int fhp_pass_backward(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t sp
                       FhpGrads g, uint64_t* h_stats, FhpStream stream) {
     if (!p || !s || !d_dLdI || !g) return fail(FHP_EINVAL, "null argument");
     if (p->kind != "WARDiffPathTracer") return fail(FHP_ESTATE, "backward needs a WARDiffPathTracer pass");
+    for (uint32_t m = 0; s && m < s->h.nM; m++) {
+        const float* r = &s->h.mats[size_t(m) * M_STRIDE];
+        if (int(r[M_TYPE]) == MAT_STANDARD && (r[M_SPEC_TRANS] > 0.f || r[M_DIFF_TRANS] > 0.f))
+            return fail(FHP_EINVAL, "the adjoint does not differentiate transmission lobes (material " + std::to_string(m) +
+                                        " has specular or diffuse transmission)");
+    }
     if (g->scene != s) return fail(FHP_EINVAL, "gradients belong to another scene");
     if (W == 0 || H == 0 || spp == 0) return fail(FHP_EINVAL, "empty frame");
     const bool war = p->get("useWAR") != 0;

@@ -360,7 +360,7 @@ __global__ void kShadePrimary(SceneDev s0, MptArgs a) {
         if (matSample(sd, m, rng, sc, a.importance != 0)) {
             thp = thp * sc.weight;
             cont = anyPos(thp);
-            newO = org;
+            newO = sc.transmission ? rayOriginFromShading(sd, false) : org;  // computeRayOrigin(false) for transmission
             newD = sc.wo;
             pdf = sc.pdf;
             flags = sc.delta ? 1u : 0u;
@@ -444,7 +444,7 @@ __global__ void kShade(SceneDev s0, MptArgs a, uint32_t depth) {
                     if (matSample(sd, m, rng, sc, a.importance != 0)) {
                         thp = thp * sc.weight;
                         cont = anyPos(thp);
-                        newO = org;
+                        newO = sc.transmission ? rayOriginFromShading(sd, false) : org;  // computeRayOrigin(false) for transmission
                         newD = sc.wo;
                         a.thp[pathId] = make_float4(thp.x, thp.y, thp.z, sc.pdf);
                         a.prevN[pathId] = make_float4(sd.N.x, sd.N.y, sd.N.z, u2f(sc.delta ? 1u : 0u));

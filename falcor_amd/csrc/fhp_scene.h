@@ -15,8 +15,11 @@ namespace fhp {
 
 // Material record (16 floats), identical to the scene description the Python layer builds.
 enum MatField { M_TYPE = 0, M_BASE = 1, M_ROUGH = 4, M_METAL = 5, M_IOR = 6, M_EMIT = 7, M_EMIT_FACTOR = 10,
-                M_DOUBLE_SIDED = 11, M_TEX = 12, M_EMIT_TEX = 13, M_STRIDE = 16 };
-// M_EMIT_TEX holds the emissive texture index + 1 (0: the emissive colour is used).
+                M_DOUBLE_SIDED = 11, M_TEX = 12, M_EMIT_TEX = 13, M_SPEC_TRANS = 14, M_DIFF_TRANS = 15, M_TRANS = 16,
+                M_THIN = 19, M_STRIDE = 20 };
+// M_EMIT_TEX holds the emissive texture index + 1 (0: the emissive colour is used). M_SPEC_TRANS,
+// M_DIFF_TRANS: specular / diffuse transmission; M_TRANS: transmission colour (rgb); M_THIN: thin
+// surface (the transmission albedo is the colour itself rather than its square root).
 enum MatType { MAT_STANDARD = 0, MAT_PBRT_DIFFUSE = 1 };
 
 // BVH4 child reference: bit 31 = leaf, bits 28..30 = count - 1, bits 0..27 = first triangle.
@@ -189,6 +192,8 @@ struct Material {
     int type;
     f3 diffuse, specular, emission, base;
     f3 emitColor;  // emissive colour before emissiveFactor (texture value when textured)
+    f3 transAlbedo;                  // transmission albedo (StandardBSDF.slang: sqrt(colour) unless thin)
+    float specTrans, diffTrans, eta;  // transmission weights; relative IoR (incident / transmitted side)
     float roughness, metallic, F0;
     bool doubleSided;
     int tex;
@@ -204,6 +209,22 @@ FHP_HD Material loadMaterial(const SceneDev& s, const float* mats, const float* 
     if (m.tex >= 0) base = texFetch(s, texels, m.tex, sd.uvx, sd.uvy, tap);
     m.base = base;
     m.doubleSided = r[M_DOUBLE_SIDED] != 0.f;
+    m.specTrans = 0.f;
+    m.diffTrans = 0.f;
+    m.transAlbedo = mk3(0.f);
+    m.eta = 1.f;
+    if (m.type == MAT_STANDARD) {
+        m.specTrans = r[M_SPEC_TRANS];
+        m.diffTrans = r[M_DIFF_TRANS];
+        // Transmissive materials are double-sided (BasicMaterial::adjustDoubleSidedFlag, :662-673).
+        if (m.specTrans > 0.f || m.diffTrans > 0.f) {
+            m.doubleSided = true;
+            const f3 t = mk3(r[M_TRANS], r[M_TRANS + 1], r[M_TRANS + 2]);
+            m.transAlbedo = r[M_THIN] != 0.f ? t : mk3(sqrtf(t.x), sqrtf(t.y), sqrtf(t.z));
+        }
+        // StandardMaterial.slang:56-57 with the incident medium's IoR 1.
+        m.eta = sd.frontFacing ? (1.f / r[M_IOR]) : (r[M_IOR] / 1.f);
+    }
     // ShadingUtils.slang:107-117 (flipShadingNormal)
     if (!sd.frontFacing && m.doubleSided) frameOut.N = -frameOut.N;
     if (m.type == MAT_PBRT_DIFFUSE) {
@@ -231,25 +252,55 @@ FHP_HD Material loadMaterial(const SceneDev& s, const float* mats, const float* 
     return m;
 }
 
-// StandardBSDF (StandardBSDF.slang:125-199, reflection lobes: Frostbite diffuse + GGX).
+// Fresnel.slang:68-92: dielectric Fresnel reflectance for relative IoR eta; cosT is the cosine
+// of the refracted direction (0 on total internal reflection, which returns 1).
+FHP_HD float fresnelDielectric(float eta, float cosI, float& cosT) {
+    if (cosI < 0.f) {
+        eta = 1.f / eta;
+        cosI = -cosI;
+    }
+    const float sin2T = eta * eta * (1.f - cosI * cosI);
+    if (sin2T > 1.f) {
+        cosT = 0.f;
+        return 1.f;
+    }
+    cosT = sqrtf(1.f - sin2T);
+    const float rs = (eta * cosI - cosT) / (eta * cosI + cosT);
+    const float rp = (eta * cosT - cosI) / (eta * cosT + cosI);
+    return 0.5f * (rs * rs + rp * rp);
+}
+
+// StandardBSDF (StandardBSDF.slang:125-313): Frostbite diffuse reflection, Lambert diffuse
+// transmission (LambertDiffuseBTDF.slang), GGX specular reflection (SpecularMicrofacetBRDF) and
+// GGX dielectric reflection + transmission (SpecularMicrofacetBSDF, SpecularMicrofacet.slang:206-471),
+// mixed by specularTransmission / diffuseTransmission and sampled one lobe at a time.
 struct Bsdf {
-    f3 diff, spec;
-    float rough, alpha, pDiff, pSpec;
+    f3 diff, spec, trans;
+    float rough, alpha, alphaT, eta, specTrans, diffTrans;
+    float pDiff, pDiffT, pSpec, pSpecT;
     FHP_HD void init(f3 wi, const Material& m) {
         diff = m.diffuse; spec = m.specular; rough = m.roughness;
+        trans = m.transAlbedo; eta = m.eta;
+        specTrans = m.specTrans; diffTrans = m.diffTrans;
         alpha = rough * rough;
         if (alpha < kMinGGXAlpha) alpha = 0.f;
-        float metallicBRDF = m.metallic * (1.f - 0.f);
-        float dielectricBSDF = (1.f - m.metallic) * (1.f - 0.f);
+        alphaT = eta == 1.f ? 0.f : alpha;
+        float metallicBRDF = m.metallic * (1.f - specTrans);
+        float dielectricBSDF = (1.f - m.metallic) * (1.f - specTrans);
+        float specularBSDF = specTrans;
         float dw = luminance(m.diffuse);
         float sw = luminance(schlick3(m.specular, 1.f, wi.z));
-        pDiff = dw * dielectricBSDF * (1.f - 0.f);
+        pDiff = dw * dielectricBSDF * (1.f - diffTrans);
+        pDiffT = dw * dielectricBSDF * diffTrans;
         pSpec = sw * (metallicBRDF + dielectricBSDF);
-        float norm = pDiff + 0.f + pSpec + 0.f;
+        pSpecT = specularBSDF;
+        float norm = pDiff + pDiffT + pSpec + pSpecT;
         if (norm > 0.f) {
             norm = 1.f / norm;
             pDiff *= norm;
+            pDiffT *= norm;
             pSpec *= norm;
+            pSpecT *= norm;
         }
     }
     // FrostbiteDiffuseBRDF.slang:84-96 (returns f * pi without the cosine)
@@ -267,6 +318,15 @@ struct Bsdf {
         if (fminf_(wi.z, wo.z) < kMinCosTheta) return mk3(0.f);
         return frostbite(wi, wo) * kInvPi * wo.z;
     }
+    // LambertDiffuseBTDF.slang:40-84
+    FHP_HD f3 evalDiffuseT(f3 wi, f3 wo) const {
+        if (fminf_(wi.z, -wo.z) < kMinCosTheta) return mk3(0.f);
+        return kInvPi * trans * -wo.z;
+    }
+    FHP_HD float pdfDiffuseT(f3 wi, f3 wo) const {
+        if (fminf_(wi.z, -wo.z) < kMinCosTheta) return 0.f;
+        return kInvPi * -wo.z;
+    }
     // SpecularMicrofacet.slang:58-83
     FHP_HD f3 evalSpecular(f3 wi, f3 wo) const {
         if (fminf_(wi.z, wo.z) < kMinCosTheta) return mk3(0.f);
@@ -277,6 +337,23 @@ struct Bsdf {
         float G = ggxG(alpha, wi.z, wo.z);
         f3 F = schlick3(spec, 1.f, wiDotH);
         return F * D * G * 0.25f / wi.z;
+    }
+    // SpecularMicrofacetBSDF::eval (both the reflection and the transmission lobe active)
+    FHP_HD f3 evalSpecularT(f3 wi, f3 wo) const {
+        if (fminf_(wi.z, fabsf(wo.z)) < kMinCosTheta) return mk3(0.f);
+        if (alphaT == 0.f) return mk3(0.f);
+        const bool isR = wo.z > 0.f;
+        f3 h = normalize(wo + wi * (isR ? 1.f : eta));
+        h = h * (h.z > 0.f ? 1.f : (h.z < 0.f ? -1.f : 0.f));
+        const float wiDotH = dot(wi, h), woDotH = dot(wo, h);
+        const float D = ggxD(alphaT, h.z);
+        const float G = ggxG(alphaT, wi.z, fabsf(wo.z));
+        float cosT;
+        const float F = fresnelDielectric(eta, wiDotH, cosT);
+        if (isR) return mk3(F * D * G * 0.25f / wi.z);
+        const float sqrtDenom = woDotH + eta * wiDotH;
+        const float t = eta * eta * wiDotH * woDotH / (wi.z * sqrtDenom * sqrtDenom);
+        return trans * (1.f - F) * D * G * fabsf(t);
     }
     FHP_HD float pdfDiffuse(f3 wi, f3 wo) const {
         if (fminf_(wi.z, wo.z) < kMinCosTheta) return 0.f;
@@ -289,16 +366,42 @@ struct Bsdf {
         float wiDotH = dot(wi, h);
         return ggxPdfVndf(alpha, wi, h) / (4.f * wiDotH);
     }
+    // SpecularMicrofacetBSDF::evalPdf
+    FHP_HD float pdfSpecularT(f3 wi, f3 wo) const {
+        if (fminf_(wi.z, fabsf(wo.z)) < kMinCosTheta) return 0.f;
+        if (alphaT == 0.f) return 0.f;
+        const bool isR = wo.z > 0.f;
+        f3 h = normalize(wo + wi * (isR ? 1.f : eta));
+        h = h * (h.z > 0.f ? 1.f : (h.z < 0.f ? -1.f : 0.f));
+        const float wiDotH = dot(wi, h), woDotH = dot(wo, h);
+        float cosT;
+        const float F = fresnelDielectric(eta, wiDotH, cosT);
+        float pdf = ggxPdfVndf(alphaT, wi, h);
+        if (isR) {
+            pdf /= 4.f * woDotH;
+        } else {
+            if (woDotH > 0.f) return 0.f;
+            const float sqrtDenom = woDotH + eta * wiDotH;
+            const float denom = sqrtDenom * sqrtDenom;
+            pdf = (denom > 0.f) ? pdf * fabsf(woDotH) / denom : kFltMax;
+        }
+        pdf *= isR ? F : 1.f - F;
+        return pdf;
+    }
     FHP_HD f3 eval(f3 wi, f3 wo) const {
         f3 r = mk3(0.f);
-        if (pDiff > 0.f) r = r + evalDiffuse(wi, wo) * ((1.f - 0.f) * (1.f - 0.f));
-        if (pSpec > 0.f) r = r + evalSpecular(wi, wo) * (1.f - 0.f);
+        if (pDiff > 0.f) r = r + evalDiffuse(wi, wo) * ((1.f - specTrans) * (1.f - diffTrans));
+        if (pDiffT > 0.f) r = r + evalDiffuseT(wi, wo) * ((1.f - specTrans) * diffTrans);
+        if (pSpec > 0.f) r = r + evalSpecular(wi, wo) * (1.f - specTrans);
+        if (pSpecT > 0.f) r = r + evalSpecularT(wi, wo) * specTrans;
         return r;
     }
     FHP_HD float evalPdf(f3 wi, f3 wo) const {
         float p = 0.f;
         if (pDiff > 0.f) p += pDiff * pdfDiffuse(wi, wo);
+        if (pDiffT > 0.f) p += pDiffT * pdfDiffuseT(wi, wo);
         if (pSpec > 0.f) p += pSpec * pdfSpecular(wi, wo);
+        if (pSpecT > 0.f) p += pSpecT * pdfSpecularT(wi, wo);
         return p;
     }
     // SpecularMicrofacet.slang:85-157
@@ -324,6 +427,46 @@ struct Bsdf {
         w = F * GOverG1wo;
         return true;
     }
+    // SpecularMicrofacetBSDF::sample (SpecularMicrofacet.slang:260-410): reflection or refraction
+    // chosen by the Fresnel term, VNDF half vectors, delta lobes when alpha is 0.
+    FHP_HD bool sampleSpecT(f3 wi, f3& wo, float& pdf, f3& w, bool& delta, Rng& rng) const {
+        wo = mk3(0.f); w = mk3(0.f); pdf = 0.f;
+        if (wi.z < kMinCosTheta) return false;
+        const float lobeSample = rng.u1();
+        if (alphaT == 0.f) {
+            float cosT;
+            const float F = fresnelDielectric(eta, wi.z, cosT);
+            const bool isR = lobeSample < F;
+            pdf = 0.f;
+            w = isR ? mk3(1.f) : trans;
+            wo = isR ? mk3(-wi.x, -wi.y, wi.z) : mk3(-wi.x * eta, -wi.y * eta, -cosT);
+            delta = true;
+            if (fabsf(wo.z) < kMinCosTheta || ((wo.z > 0.f) != isR)) return false;
+            return true;
+        }
+        const float ux = rng.u1(), uy = rng.u1();
+        const f3 h = ggxSampleVndf(alphaT, wi, ux, uy, pdf);
+        const float wiDotH = dot(wi, h);
+        float cosT;
+        const float F = fresnelDielectric(eta, wiDotH, cosT);
+        const bool isR = lobeSample < F;
+        wo = isR ? (2.f * wiDotH * h - wi) : ((eta * wiDotH - cosT) * h - eta * wi);
+        if (fabsf(wo.z) < kMinCosTheta || ((wo.z > 0.f) != isR)) return false;
+        const float woDotH = dot(wo, h);
+        const float G = ggxG(alphaT, wi.z, fabsf(wo.z));
+        const float GOverG1wo = G * (1.f + ggxLambda(alphaT * alphaT, wi.z));
+        w = mk3(GOverG1wo);
+        if (isR) {
+            pdf /= 4.f * woDotH;
+        } else {
+            const float sqrtDenom = woDotH + eta * wiDotH;
+            const float denom = sqrtDenom * sqrtDenom;
+            pdf = (denom > 0.f) ? pdf * fabsf(woDotH) / denom : kFltMax;
+            w = w * (trans * eta * eta);
+        }
+        pdf *= isR ? F : 1.f - F;
+        return true;
+    }
     // StandardBSDF.slang:242-313 (one-sample lobe selection)
     FHP_HD bool sample(f3 wi, f3& wo, float& pdf, f3& w, bool& delta, Rng& rng) const {
         wo = mk3(0.f); w = mk3(0.f); pdf = 0.f; delta = false;
@@ -335,16 +478,35 @@ struct Bsdf {
             if (fminf_(wi.z, wo.z) < kMinCosTheta) { w = mk3(0.f); valid = false; }
             else { w = frostbite(wi, wo); valid = true; }
             w = w / pDiff;
-            w = w * ((1.f - 0.f) * (1.f - 0.f));
+            w = w * ((1.f - specTrans) * (1.f - diffTrans));
             pdf *= pDiff;
             if (pSpec > 0.f) pdf += pSpec * pdfSpecular(wi, wo);
-        } else if (uSel < pDiff + 0.f) {
-        } else if (uSel < pDiff + 0.f + pSpec) {
+            if (pSpecT > 0.f) pdf += pSpecT * pdfSpecularT(wi, wo);
+        } else if (uSel < pDiff + pDiffT) {
+            float ux = rng.u1(), uy = rng.u1();
+            wo = cosineHemisphere(ux, uy, pdf);
+            wo.z = -wo.z;
+            if (fminf_(wi.z, -wo.z) < kMinCosTheta) { w = mk3(0.f); valid = false; }
+            else { w = trans; valid = true; }
+            w = w / pDiffT;
+            w = w * ((1.f - specTrans) * diffTrans);
+            pdf *= pDiffT;
+            if (pSpecT > 0.f) pdf += pSpecT * pdfSpecularT(wi, wo);
+        } else if (uSel < pDiff + pDiffT + pSpec) {
             valid = sampleSpec(wi, wo, pdf, w, delta, rng);
             w = w / pSpec;
-            w = w * (1.f - 0.f);
+            w = w * (1.f - specTrans);
             pdf *= pSpec;
             if (pDiff > 0.f) pdf += pDiff * pdfDiffuse(wi, wo);
+            if (pSpecT > 0.f) pdf += pSpecT * pdfSpecularT(wi, wo);
+        } else if (pSpecT > 0.f) {
+            valid = sampleSpecT(wi, wo, pdf, w, delta, rng);
+            w = w / pSpecT;
+            w = w * specTrans;
+            pdf *= pSpecT;
+            if (pDiff > 0.f) pdf += pDiff * pdfDiffuse(wi, wo);
+            if (pDiffT > 0.f) pdf += pDiffT * pdfDiffuseT(wi, wo);
+            if (pSpec > 0.f) pdf += pSpec * pdfSpecular(wi, wo);
         }
         return valid;
     }
@@ -386,7 +548,9 @@ FHP_HD f3 matEvalGated(const Shading& sd, const Material& m, const Material& mg,
     b.init(wiL, m);
     g.init(wiL, mg);
     b.pDiff = g.pDiff;
+    b.pDiffT = g.pDiffT;
     b.pSpec = g.pSpec;
+    b.pSpecT = g.pSpecT;
     return b.eval(wiL, woL);
 }
 FHP_HD float matPdf(const Shading& sd, const Material& m, f3 wo) {
@@ -397,11 +561,12 @@ FHP_HD float matPdf(const Shading& sd, const Material& m, f3 wo) {
     b.init(wiL, m);
     return b.evalPdf(wiL, woL);
 }
-struct Scatter { f3 wo, weight; float pdf; bool delta; };
+struct Scatter { f3 wo, weight; float pdf; bool delta, transmission; };
 // StandardMaterialInstance.slang:89-110 (+ sampleReference :202-231) / PBRTDiffuseMaterialInstance.slang:66-78
 FHP_HD bool matSample(const Shading& sd, const Material& m, Rng& rng, Scatter& s, bool importance) {
     f3 wiL = sd.toLocal(sd.V), woL;
     s.delta = false;
+    s.transmission = false;
     if (m.type == MAT_PBRT_DIFFUSE) {
         float ux = rng.u1(), uy = rng.u1();
         woL = cosineHemisphere(ux, uy, s.pdf);
@@ -411,15 +576,25 @@ FHP_HD bool matSample(const Shading& sd, const Material& m, Rng& rng, Scatter& s
     }
     Bsdf b;
     if (!importance) {
+        // sampleReference (StandardMaterialInstance.slang:202-231): the cosine hemisphere, mirrored
+        // below the surface with probability 1/2 when a transmission lobe is active.
+        const bool transmissive = m.specTrans > 0.f || (m.diffTrans > 0.f && m.specTrans < 1.f && anyPos(m.diffuse));
         float ux = rng.u1(), uy = rng.u1();
         woL = cosineHemisphere(ux, uy, s.pdf);
-        if (fminf_(wiL.z, woL.z) < kMinCosTheta || s.pdf == 0.f) return false;
+        if (transmissive) {
+            if (rng.u1() < 0.5f) woL.z = -woL.z;
+            s.pdf *= 0.5f;
+            if (fminf_(fabsf(wiL.z), fabsf(woL.z)) < kMinCosTheta || s.pdf == 0.f) return false;
+        } else if (fminf_(wiL.z, woL.z) < kMinCosTheta || s.pdf == 0.f) {
+            return false;
+        }
         b.init(wiL, m);
         s.weight = b.eval(wiL, woL) / s.pdf;
     } else {
         b.init(wiL, m);
         if (!b.sample(wiL, woL, s.pdf, s.weight, s.delta, rng)) return false;
     }
+    s.transmission = woL.z < 0.f;
     s.wo = sd.fromLocal(woL);
     return validRT(sd, wiL, woL, s.wo);
 }

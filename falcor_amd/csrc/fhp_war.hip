@@ -259,6 +259,7 @@ __device__ f3 warPath(const SceneDev& s, const WarArgs& a, uint32_t px, uint32_t
             Scatter sc;
             if (matSample(sd, mS, rng, sc, a.useBSDFSampling != 0) && sc.pdf > 0.f) {
                 rayD = sc.wo;
+                // PTUtils.slang:153: the view side, also for transmitted directions.
                 rayO = rayOriginOffset(posW, sd.frontFacing ? sd.faceN : -sd.faceN);
                 fS = matEvalGated(sd, m, mS, rayD);
                 woS = rayD;

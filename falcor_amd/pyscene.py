@@ -10,10 +10,12 @@ else raises `PySceneError` with the line number.
 
 Objects available to a scene script, after Falcor's bindings:
   float2/float3/float4; StandardMaterial(name) and PBRTDiffuseMaterial(name) with baseColor,
-  roughness, metallic, emissiveColor, emissiveFactor, indexOfRefraction, doubleSided and
+  roughness, metallic, emissiveColor, emissiveFactor, indexOfRefraction, doubleSided,
+  specularTransmission, diffuseTransmission, transmissionColor, thinSurface and
   loadTexture(MaterialTextureSlot.BaseColor, path, useSrgb) (StandardMaterial.cpp:261-268,
   BasicMaterial.cpp:681-691, Material.cpp:391-405; defaults of BasicMaterialData.slang:50-57,
-  base colour and specular parameters stored in float16 as the reference stores them);
+  base colour, specular and transmission parameters and the IoR stored in float16 as the
+  reference stores them);
   TriangleMesh.createQuad/createCube/createSphere/createDisk/createFromFile (.obj)
   (TriangleMesh.cpp:325-363); Transform(translation, scaling, rotationEuler, rotationEulerDeg)
   (Transform.cpp:151-213); Camera with position, target, up, focalLength, frameHeight,
@@ -107,6 +109,8 @@ class _Material:
         self.doubleSided = False
         self.specularTransmission = 0.0
         self.diffuseTransmission = 0.0
+        self.transmissionColor = Vec(1, 1, 1)
+        self.thinSurface = False
         self.texture_path = None
         self.texture_srgb = True
         self.emissive_path = None
@@ -128,12 +132,12 @@ class _Material:
         bc = self.baseColor.v
         if self.kind == S.MAT_PBRT_DIFFUSE:
             return S.pbrt_diffuse_material([_half(c) for c in bc[:3]], self.doubleSided, texture_index)
-        if self.specularTransmission or self.diffuseTransmission:
-            self._warnings.append(f"material '{self.name}': transmission ignored (reflection lobes only)")
         e = self.emissiveColor.v if isinstance(self.emissiveColor, Vec) else (self.emissiveColor,) * 3
+        t = self.transmissionColor.v if isinstance(self.transmissionColor, Vec) else (self.transmissionColor,) * 3
         return S.standard_material([_half(c) for c in bc[:3]], _half(self.roughness), _half(self.metallic),
                                    _half(self.indexOfRefraction), e[:3], self.emissiveFactor, self.doubleSided,
-                                   texture_index, emissive_index)
+                                   texture_index, emissive_index, _half(self.specularTransmission),
+                                   _half(self.diffuseTransmission), [_half(c) for c in t[:3]], bool(self.thinSurface))
 
 
 class StandardMaterial(_Material):

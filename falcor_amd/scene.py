@@ -1,14 +1,16 @@
 """Scene description and the synthetic benchmark scenes.
 
 A scene is world-space triangle geometry (mesh instances are flattened when added), a material
-table of 16-float records, optional RGB fp32 textures and a pinhole camera. This mirrors the
+table of 20-float records, optional RGB fp32 textures and a pinhole camera. This mirrors the
 parts of Falcor's SceneBuilder/TriangleMesh/Transform/Camera that the two passes read
 (Source/Falcor/Scene/SceneBuilder.cpp, TriangleMesh.cpp:58-195, Transform.cpp:80-100,
 Camera.cpp:112-193).
 
-Material record layout (float32[16]):
+Material record layout (float32[20]):
   0 type (0 StandardMaterial, 1 PBRTDiffuseMaterial)   1-3 baseColor   4 roughness   5 metallic
   6 ior   7-9 emissive   10 emissiveFactor   11 doubleSided   12 baseColor texture (-1 none)
+  13 emissive texture + 1 (0 none)   14 specularTransmission   15 diffuseTransmission
+  16-18 transmission colour   19 thinSurface
 """
 from __future__ import annotations
 
@@ -19,7 +21,7 @@ import numpy as np
 
 MAT_STANDARD = 0
 MAT_PBRT_DIFFUSE = 1
-MAT_STRIDE = 16
+MAT_STRIDE = 20
 
 
 # --------------------------------------------------------------------------- meshes
@@ -183,13 +185,15 @@ def random_rotation(rng: np.random.Generator) -> np.ndarray:
 
 
 # --------------------------------------------------------------------------- materials
-# Material record fields (16 floats; see include/fhp.h).
+# Material record fields (20 floats; see include/fhp.h).
 M_TYPE, M_BASE, M_ROUGH, M_METAL, M_IOR, M_EMIT, M_EMIT_FACTOR, M_DOUBLE_SIDED, M_TEX = 0, 1, 4, 5, 6, 7, 10, 11, 12
 M_EMIT_TEX = 13  # emissive texture index + 1 (0: the emissive colour)
+M_SPEC_TRANS, M_DIFF_TRANS, M_TRANS, M_THIN = 14, 15, 16, 19
 
 
 def standard_material(base=(1, 1, 1), roughness=0.0, metallic=0.0, ior=1.5, emissive=(0, 0, 0), emissive_factor=1.0,
-                      double_sided=False, texture=-1, emissive_texture=-1) -> np.ndarray:
+                      double_sided=False, texture=-1, emissive_texture=-1, specular_transmission=0.0,
+                      diffuse_transmission=0.0, transmission=(1.0, 1.0, 1.0), thin_surface=False) -> np.ndarray:
     """StandardMaterial record (defaults of BasicMaterialData.slang:52-57: baseColor 1, specular 0)."""
     m = np.zeros(MAT_STRIDE, np.float32)
     m[0] = MAT_STANDARD
@@ -202,6 +206,10 @@ def standard_material(base=(1, 1, 1), roughness=0.0, metallic=0.0, ior=1.5, emis
     m[11] = 1.0 if double_sided else 0.0
     m[12] = texture
     m[M_EMIT_TEX] = emissive_texture + 1
+    m[M_SPEC_TRANS] = specular_transmission
+    m[M_DIFF_TRANS] = diffuse_transmission
+    m[M_TRANS:M_TRANS + 3] = transmission
+    m[M_THIN] = 1.0 if thin_surface else 0.0
     return m
 
 
@@ -214,6 +222,7 @@ def pbrt_diffuse_material(albedo=(0.5, 0.5, 0.5), double_sided=False, texture=-1
     m[10] = 1.0
     m[11] = 1.0 if double_sided else 0.0
     m[12] = texture
+    m[M_TRANS:M_TRANS + 3] = 1.0
     return m
 
 
@@ -346,6 +355,23 @@ def closed_box(seed: int = 0, aspect: float = 1.0) -> SceneData:
     light = sb.add_material(standard_material(base=(0, 0, 0), roughness=0.5, emissive=(17, 17, 17)))
     sb.add_mesh_instance(quad, light, Transform.deg((0.13, 0.13, 0.13), (0, 0.549, 0), (180, 0, 0)), "Light")
     sb.camera = Camera((0, 0.28, 1.2), (0, 0.28, 0), (0, 1, 0), math.radians(40.0), aspect)
+    return sb.build()
+
+
+def glass_slab(colour=(1.0, 1.0, 1.0), roughness: float = 0.0, ior: float = 1.5, thin: bool = False,
+               slab: bool = True, radiance: float = 1.0, fov_deg: float = 2.0, aspect: float = 1.0) -> SceneData:
+    """A 2 x 2 x 0.2 slab of a specular-transmission StandardMaterial (declared single-sided) at the
+    origin, in front of a 4 x 4 emissive quad of `radiance` at z = -1 that faces the camera at
+    (0, 0, 5) with a narrow field of view. At normal incidence a smooth slab transmits
+    radiance * (1 - F) / (1 + F) * transmission albedo^2 (F the normal-incidence Fresnel term)."""
+    sb = SceneBuilder()
+    if slab:
+        g = sb.add_material(standard_material(base=(1, 1, 1), roughness=roughness, ior=ior, specular_transmission=1.0,
+                                              transmission=colour, thin_surface=thin))
+        sb.add_mesh_instance(TriangleMesh.create_cube((2.0, 2.0, 0.2)), g, None, "Slab")
+    e = sb.add_material(standard_material(base=(0, 0, 0), roughness=1.0, ior=1.0, emissive=(radiance,) * 3))
+    sb.add_mesh_instance(TriangleMesh.create_quad(), e, Transform.deg((4, 1, 4), (0, 0, -1), (90, 0, 0)), "Light")
+    sb.camera = Camera((0, 0, 5), (0, 0, 0), (0, 1, 0), math.radians(fov_deg), aspect)
     return sb.build()
 
 

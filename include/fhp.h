@@ -38,10 +38,12 @@ const char* fhp_last_warning(void);
 
 /* ---------------------------------------------------------------- scene
  * Scene description: world-space triangle geometry (instances flattened by the caller).
- * Materials are 16 floats each: [type (0 StandardMaterial, 1 PBRTDiffuseMaterial),
+ * Materials are 20 floats each: [type (0 StandardMaterial, 1 PBRTDiffuseMaterial),
  * baseColor rgb, roughness, metallic, ior, emissive rgb, emissiveFactor, doubleSided,
  * baseColor texture index (-1 none), emissive texture index + 1 (0 none: the emissive colour is
- * used; MaterialSystem.slang:201-213), 0, 0]. Textures are RGB fp32, texDesc = 4 ints
+ * used; MaterialSystem.slang:201-213), specularTransmission, diffuseTransmission, transmission
+ * colour rgb, thinSurface] (BasicMaterialData.slang:50-66; a material with transmission is
+ * double-sided, BasicMaterial.cpp:662-673). Textures are RGB fp32, texDesc = 4 ints
  * (float offset into texels, width, height, 0). Camera = 14 floats [position, target, up,
  * fovY (radians), aspect, focalDistance, nearZ, farZ].
  * Replaces Scene::create + SceneBuilder (Source/Falcor/Scene/Scene.cpp, SceneBuilder.cpp),
@@ -57,7 +59,7 @@ typedef struct FhpSceneDesc {
     const uint32_t* h_indices;     /* 3 * num_triangles */
     const uint32_t* h_tri_material; /* num_triangles */
     uint32_t num_materials;
-    const float* h_materials; /* 16 * num_materials */
+    const float* h_materials; /* 20 * num_materials */
     uint32_t num_textures;
     const int32_t* h_tex_desc; /* 4 * num_textures */
     uint64_t num_texels_floats;
@@ -104,7 +106,7 @@ int fhp_scene_update_vertices(FhpScene scene, uint32_t mesh_id, const float* h_p
  * (synchronises the stream). */
 int fhp_scene_update_vertices_device(FhpScene scene, uint32_t mesh_id, const float* d_positions, const float* d_normals,
                                      int rebuild, FhpStream stream);
-/* Replace all material records (host memory, 16 * num_materials floats).
+/* Replace all material records (host memory, 20 * num_materials floats).
  * Replaces Scene::setMaterialParams (Source/Falcor/Scene/Scene.cpp:4620-4643). */
 int fhp_scene_set_materials(FhpScene scene, const float* h_materials);
 /* Copy texels from device memory into the scene (same layout as the description). */

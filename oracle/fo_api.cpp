@@ -88,9 +88,10 @@ void fo_sample_vmf(int n, const float* u, float kappa, float* out) {
         out[3 * i] = w.x; out[3 * i + 1] = w.y; out[3 * i + 2] = w.z;
     }
 }
-// Standard BSDF in the local frame. mat = [type, base rgb, roughness, metallic, ior].
-// eval: out[4*i] = f rgb, pdf.
-void fo_bsdf_eval(int n, const float* mat, const float* wi, const float* wo, float* out) {
+// Standard BSDF in the local frame. mat = [type, base rgb, roughness, metallic, ior, eta,
+// specularTransmission, diffuseTransmission, transmission albedo rgb] (13 floats; the albedo is
+// used as given, after any thin-surface square root).
+static BsdfData<float> bsdfParams(const float* mat) {
     BsdfData<float> d;
     d.type = int(mat[0]);
     V3f base(mat[1], mat[2], mat[3]);
@@ -99,6 +100,15 @@ void fo_bsdf_eval(int n, const float* mat, const float* wi, const float* wo, flo
     d.specular = V3f(F0) + (base - V3f(F0)) * metal;
     d.roughness = mat[4];
     d.metallic = metal;
+    d.eta = mat[7];
+    d.specTrans = mat[8];
+    d.diffTrans = mat[9];
+    d.transmission = V3f(mat[10], mat[11], mat[12]);
+    return d;
+}
+// eval: out[4*i] = f rgb, pdf.
+void fo_bsdf_eval(int n, const float* mat, const float* wi, const float* wo, float* out) {
+    BsdfData<float> d = bsdfParams(mat);
     for (int i = 0; i < n; i++) {
         V3f a(wi[3 * i], wi[3 * i + 1], wi[3 * i + 2]), b(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]);
         StdBsdf<float> bs;
@@ -109,14 +119,7 @@ void fo_bsdf_eval(int n, const float* mat, const float* wi, const float* wo, flo
 }
 // sample: out[8*i] = wo xyz, weight rgb, pdf, valid (delta samples report valid = 2).
 void fo_bsdf_sample(int n, const float* mat, const float* wi, const uint32_t* seeds, float* out) {
-    BsdfData<float> d;
-    d.type = int(mat[0]);
-    V3f base(mat[1], mat[2], mat[3]);
-    float ior = mat[6], f = (ior - 1.f) / (ior + 1.f), F0 = f * f, metal = mat[5];
-    d.diffuse = base + (V3f(0.f) - base) * metal;
-    d.specular = V3f(F0) + (base - V3f(F0)) * metal;
-    d.roughness = mat[4];
-    d.metallic = metal;
+    BsdfData<float> d = bsdfParams(mat);
     for (int i = 0; i < n; i++) {
         V3f a(wi[3 * i], wi[3 * i + 1], wi[3 * i + 2]);
         SampleGen g = makeSampleGen(seeds[2 * i], 0, seeds[2 * i + 1]);
@@ -127,6 +130,15 @@ void fo_bsdf_sample(int n, const float* mat, const float* wi, const uint32_t* se
         float* o = out + 8 * i;
         o[0] = wo.x; o[1] = wo.y; o[2] = wo.z; o[3] = w.x; o[4] = w.y; o[5] = w.z; o[6] = pdf;
         o[7] = ok ? (delta ? 2.f : 1.f) : 0.f;
+    }
+}
+
+// Dielectric Fresnel: out[2*i] = F, cosT.
+void fo_fresnel_dielectric(int n, const float* eta, const float* cosI, float* out) {
+    for (int i = 0; i < n; i++) {
+        float ct;
+        out[2 * i] = fresnelDielectric(eta[i], cosI[i], ct);
+        out[2 * i + 1] = ct;
     }
 }
 

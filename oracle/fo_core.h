@@ -306,42 +306,71 @@ template <typename T> inline V3<T> sampleGGX_VNDF(T alpha, V3<T> wi, T ux, T uy,
     return h;
 }
 
-// Material record shared with the product's scene format (16 floats per material).
+// Material record shared with the product's scene format (20 floats per material).
 enum MatField {
     M_TYPE = 0, M_BASE = 1, M_ROUGH = 4, M_METAL = 5, M_IOR = 6, M_EMIT = 7, M_EMIT_FACTOR = 10,
-    M_DOUBLE_SIDED = 11, M_TEX = 12, M_EMIT_TEX = 13, M_STRIDE = 16  // M_EMIT_TEX: emissive texture + 1, 0 none
+    M_DOUBLE_SIDED = 11, M_TEX = 12, M_EMIT_TEX = 13,  // M_EMIT_TEX: emissive texture + 1, 0 none
+    M_SPEC_TRANS = 14, M_DIFF_TRANS = 15, M_TRANS = 16, M_THIN = 19, M_STRIDE = 20  // M_TRANS: rgb
 };
 enum MatType { MAT_STANDARD = 0, MAT_PBRT_DIFFUSE = 1 };
 
-// Parameters of one shading point's BSDF (reflection lobes only; transmission is zero).
+// Parameters of one shading point's BSDF (StandardBSDFData: diffuse/specular albedo, roughness,
+// metallic, relative IoR eta, transmission albedo and the two transmission weights).
 template <typename T> struct BsdfData {
     int type;            // MatType
     V3<T> diffuse, specular;
     T roughness, metallic;
+    T eta = T(1);
+    V3<T> transmission = V3<T>(T(0));  // after the thin-surface square root
+    T specTrans = T(0), diffTrans = T(0);
 };
 
-// StandardBSDF.slang:125-199 (lobe setup, reflection-only: specTrans = diffTrans = 0).
+// Fresnel.slang:68-92 (dielectric reflectance; cosT = 0 on total internal reflection).
+template <typename T> inline T fresnelDielectric(T eta, T cosI, T& cosT) {
+    if (cosI < T(0)) { eta = T(1) / eta; cosI = -cosI; }
+    T sin2 = eta * eta * (T(1) - cosI * cosI);
+    if (sin2 > T(1)) { cosT = T(0); return T(1); }
+    cosT = std::sqrt(T(1) - sin2);
+    T rs = (eta * cosI - cosT) / (eta * cosI + cosT);
+    T rp = (eta * cosT - cosI) / (eta * cosT + cosI);
+    return T(0.5f) * (rs * rs + rp * rp);
+}
+template <typename T> inline T signOf(T v) { return v > T(0) ? T(1) : (v < T(0) ? T(-1) : T(0)); }
+
+// StandardBSDF.slang:125-199 (lobe setup) with all four lobes: Frostbite diffuse reflection,
+// Lambert diffuse transmission, GGX specular reflection, GGX dielectric reflection/transmission.
+// Every lobe is active (Material.cpp:69 sets LobeType::All).
 template <typename T> struct StdBsdf {
-    V3<T> diffAlbedo, specAlbedo;
-    T roughness, alpha;
-    T pDiff, pSpec;
+    V3<T> diffAlbedo, specAlbedo, transAlbedo;
+    T roughness, alpha, alphaT, eta, specTrans, diffTrans;
+    T pDiff, pDiffT, pSpec, pSpecT;
     void init(V3<T> wi, const BsdfData<T>& d) {
         diffAlbedo = d.diffuse;
         roughness = d.roughness;
         specAlbedo = d.specular;
+        transAlbedo = d.transmission;
+        eta = d.eta;
         alpha = d.roughness * d.roughness;
         if (alpha < T(kMinGGXAlpha)) alpha = T(0);
-        T metallicBRDF = d.metallic * (T(1) - T(0));
-        T dielectricBSDF = (T(1) - d.metallic) * (T(1) - T(0));
+        alphaT = d.eta == T(1) ? T(0) : alpha;
+        diffTrans = d.diffTrans;
+        specTrans = d.specTrans;
+        T metallicBRDF = d.metallic * (T(1) - specTrans);
+        T dielectricBSDF = (T(1) - d.metallic) * (T(1) - specTrans);
+        T specularBSDF = specTrans;
         T diffuseWeight = luminance(d.diffuse);
         T specularWeight = luminance(fresnelSchlick3(d.specular, T(1), wi.z));
-        pDiff = diffuseWeight * dielectricBSDF * (T(1) - T(0));
+        pDiff = diffuseWeight * dielectricBSDF * (T(1) - diffTrans);
+        pDiffT = diffuseWeight * dielectricBSDF * diffTrans;
         pSpec = specularWeight * (metallicBRDF + dielectricBSDF);
-        T norm = pDiff + T(0) + pSpec + T(0);
+        pSpecT = specularBSDF;
+        T norm = pDiff + pDiffT + pSpec + pSpecT;
         if (norm > T(0)) {
             norm = T(1) / norm;
             pDiff *= norm;
+            pDiffT *= norm;
             pSpec *= norm;
+            pSpecT *= norm;
         }
     }
     // FrostbiteDiffuseBRDF.slang:84-96 (f * pi)
@@ -382,17 +411,72 @@ template <typename T> struct StdBsdf {
         T wiDotH = dot(wi, h);
         return evalPdfGGX_VNDF(alpha, wi, h) / (T(4) * wiDotH);
     }
+    // LambertDiffuseBTDF.slang:40-84
+    V3<T> evalDiffuseT(V3<T> wi, V3<T> wo) const {
+        if (std::min(wi.z, -wo.z) < T(kMinCosTheta)) return V3<T>(T(0));
+        return Consts<T>::inv_pi * transAlbedo * -wo.z;
+    }
+    T pdfDiffuseT(V3<T> wi, V3<T> wo) const {
+        if (std::min(wi.z, -wo.z) < T(kMinCosTheta)) return T(0);
+        return Consts<T>::inv_pi * -wo.z;
+    }
+    // Half vector of SpecularMicrofacetBSDF (SpecularMicrofacet.slang:230-233, :421-423), flipped
+    // into the upper hemisphere.
+    V3<T> bsdfHalf(V3<T> wi, V3<T> wo, bool isR) const {
+        V3<T> h = normalize(wo + wi * (isR ? T(1) : eta));
+        return h * signOf(h.z);
+    }
+    // SpecularMicrofacet.slang:215-256
+    V3<T> evalSpecularT(V3<T> wi, V3<T> wo) const {
+        if (std::min(wi.z, std::fabs(wo.z)) < T(kMinCosTheta)) return V3<T>(T(0));
+        if (alphaT == T(0)) return V3<T>(T(0));
+        bool isR = wo.z > T(0);
+        V3<T> h = bsdfHalf(wi, wo, isR);
+        T wiDotH = dot(wi, h), woDotH = dot(wo, h);
+        T D = evalNdfGGX(alphaT, h.z);
+        T G = evalMaskingSmithGGXCorrelated(alphaT, wi.z, std::fabs(wo.z));
+        T cosT;
+        T F = fresnelDielectric(eta, wiDotH, cosT);
+        if (isR) return V3<T>(F * D * G * T(0.25f) / wi.z);
+        T sqrtDenom = woDotH + eta * wiDotH;
+        T t = eta * eta * wiDotH * woDotH / (wi.z * sqrtDenom * sqrtDenom);
+        return transAlbedo * (T(1) - F) * D * G * std::fabs(t);
+    }
+    // SpecularMicrofacet.slang:412-458
+    T pdfSpecularT(V3<T> wi, V3<T> wo) const {
+        if (std::min(wi.z, std::fabs(wo.z)) < T(kMinCosTheta)) return T(0);
+        if (alphaT == T(0)) return T(0);
+        bool isR = wo.z > T(0);
+        V3<T> h = bsdfHalf(wi, wo, isR);
+        T wiDotH = dot(wi, h), woDotH = dot(wo, h);
+        T cosT;
+        T F = fresnelDielectric(eta, wiDotH, cosT);
+        T pdf = evalPdfGGX_VNDF(alphaT, wi, h);
+        if (isR) {
+            pdf /= T(4) * woDotH;
+        } else {
+            if (woDotH > T(0)) return T(0);
+            T sqrtDenom = woDotH + eta * wiDotH;
+            T denom = sqrtDenom * sqrtDenom;
+            pdf = (denom > T(0)) ? pdf * std::fabs(woDotH) / denom : T(std::numeric_limits<float>::max());
+        }
+        return pdf * (isR ? F : T(1) - F);
+    }
     // StandardBSDF.slang:228-240
     V3<T> eval(V3<T> wi, V3<T> wo) const {
         V3<T> r(T(0));
-        if (pDiff > T(0)) r = r + evalDiffuse(wi, wo) * ((T(1) - T(0)) * (T(1) - T(0)));
-        if (pSpec > T(0)) r = r + evalSpecular(wi, wo) * (T(1) - T(0));
+        if (pDiff > T(0)) r = r + evalDiffuse(wi, wo) * ((T(1) - specTrans) * (T(1) - diffTrans));
+        if (pDiffT > T(0)) r = r + evalDiffuseT(wi, wo) * ((T(1) - specTrans) * diffTrans);
+        if (pSpec > T(0)) r = r + evalSpecular(wi, wo) * (T(1) - specTrans);
+        if (pSpecT > T(0)) r = r + evalSpecularT(wi, wo) * specTrans;
         return r;
     }
     T evalPdf(V3<T> wi, V3<T> wo) const {
         T p = T(0);
         if (pDiff > T(0)) p += pDiff * pdfDiffuse(wi, wo);
+        if (pDiffT > T(0)) p += pDiffT * pdfDiffuseT(wi, wo);
         if (pSpec > T(0)) p += pSpec * pdfSpecular(wi, wo);
+        if (pSpecT > T(0)) p += pSpecT * pdfSpecularT(wi, wo);
         return p;
     }
     // StandardBSDF.slang:242-313 (one-sample lobe selection).
@@ -407,19 +491,77 @@ template <typename T> struct StdBsdf {
             if (std::min(wi.z, wo.z) < T(kMinCosTheta)) { weight = V3<T>(T(0)); valid = false; }
             else { weight = frostbiteWeight(wi, wo); valid = true; }
             weight = weight / pDiff;
-            weight = weight * ((T(1) - T(0)) * (T(1) - T(0)));
+            weight = weight * ((T(1) - specTrans) * (T(1) - diffTrans));
             pdf *= pDiff;
             if (pSpec > T(0)) pdf += pSpec * pdfSpecular(wi, wo);
-        } else if (uSelect < pDiff + T(0)) {
-            // diffuse transmission: never taken (probability zero).
-        } else if (uSelect < pDiff + T(0) + pSpec) {
+            if (pSpecT > T(0)) pdf += pSpecT * pdfSpecularT(wi, wo);
+        } else if (uSelect < pDiff + pDiffT) {
+            // LambertDiffuseBTDF.slang:49-69
+            T ux = T(sg.next1D()), uy = T(sg.next1D());
+            wo = sample_cosine_hemisphere_concentric(ux, uy, pdf);
+            wo.z = -wo.z;
+            if (std::min(wi.z, -wo.z) < T(kMinCosTheta)) { weight = V3<T>(T(0)); valid = false; }
+            else { weight = transAlbedo; valid = true; }
+            weight = weight / pDiffT;
+            weight = weight * ((T(1) - specTrans) * diffTrans);
+            pdf *= pDiffT;
+            if (pSpecT > T(0)) pdf += pSpecT * pdfSpecularT(wi, wo);
+        } else if (uSelect < pDiff + pDiffT + pSpec) {
             valid = sampleSpecular(wi, wo, pdf, weight, delta, sg);
             weight = weight / pSpec;
-            weight = weight * (T(1) - T(0));
+            weight = weight * (T(1) - specTrans);
             pdf *= pSpec;
             if (pDiff > T(0)) pdf += pDiff * pdfDiffuse(wi, wo);
+            if (pSpecT > T(0)) pdf += pSpecT * pdfSpecularT(wi, wo);
+        } else if (pSpecT > T(0)) {
+            valid = sampleSpecularT(wi, wo, pdf, weight, delta, sg);
+            weight = weight / pSpecT;
+            weight = weight * specTrans;
+            pdf *= pSpecT;
+            if (pDiff > T(0)) pdf += pDiff * pdfDiffuse(wi, wo);
+            if (pDiffT > T(0)) pdf += pDiffT * pdfDiffuseT(wi, wo);
+            if (pSpec > T(0)) pdf += pSpec * pdfSpecular(wi, wo);
         }
         return valid;
+    }
+    // SpecularMicrofacet.slang:260-410 with both reflection and transmission active: the Fresnel
+    // term picks the event; delta events when alpha is 0.
+    bool sampleSpecularT(V3<T> wi, V3<T>& wo, T& pdf, V3<T>& weight, bool& delta, SampleGen& sg) const {
+        wo = V3<T>(T(0)); weight = V3<T>(T(0)); pdf = T(0);
+        if (wi.z < T(kMinCosTheta)) return false;
+        T lobeSample = T(sg.next1D());
+        if (alphaT == T(0)) {
+            T cosT;
+            T F = fresnelDielectric(eta, wi.z, cosT);
+            bool isR = lobeSample < F;
+            pdf = T(0);
+            weight = isR ? V3<T>(T(1)) : transAlbedo;
+            wo = isR ? V3<T>(-wi.x, -wi.y, wi.z) : V3<T>(-wi.x * eta, -wi.y * eta, -cosT);
+            delta = true;
+            return !(std::fabs(wo.z) < T(kMinCosTheta) || ((wo.z > T(0)) != isR));
+        }
+        T ux = T(sg.next1D()), uy = T(sg.next1D());
+        V3<T> h = sampleGGX_VNDF(alphaT, wi, ux, uy, pdf);
+        T wiDotH = dot(wi, h);
+        T cosT;
+        T F = fresnelDielectric(eta, wiDotH, cosT);
+        bool isR = lobeSample < F;
+        wo = isR ? (T(2) * wiDotH * h - wi) : ((eta * wiDotH - cosT) * h - eta * wi);
+        if (std::fabs(wo.z) < T(kMinCosTheta) || ((wo.z > T(0)) != isR)) return false;
+        T woDotH = dot(wo, h);
+        T G = evalMaskingSmithGGXCorrelated(alphaT, wi.z, std::fabs(wo.z));
+        T GOverG1wo = G * (T(1) + evalLambdaGGX(alphaT * alphaT, wi.z));
+        weight = V3<T>(GOverG1wo);
+        if (isR) {
+            pdf /= T(4) * woDotH;
+        } else {
+            T sqrtDenom = woDotH + eta * wiDotH;
+            T denom = sqrtDenom * sqrtDenom;
+            pdf = (denom > T(0)) ? pdf * std::fabs(woDotH) / denom : T(std::numeric_limits<float>::max());
+            weight = weight * (transAlbedo * eta * eta);
+        }
+        pdf *= isR ? F : T(1) - F;
+        return true;
     }
     // SpecularMicrofacet.slang:85-157 (VNDF sampling; delta reflection when alpha == 0).
     bool sampleSpecular(V3<T> wi, V3<T>& wo, T& pdf, V3<T>& weight, bool& delta, SampleGen& sg) const {

@@ -224,7 +224,7 @@ inline V3f sampleTexture(const Scene& s, int tex, float u, float v, TexFetch* tf
     return c;
 }
 
-// Material instance: StandardMaterial.slang:50-126 / PBRTDiffuseMaterial.slang (reflection only).
+// Material instance: StandardMaterial.slang:50-126 / PBRTDiffuseMaterial.slang.
 struct MatInst {
     Frame<float> sf;
     BsdfData<float> d;   // evaluation parameters
@@ -239,6 +239,25 @@ inline V3f evalEmissive(const Scene& s, const float* m, float u, float v) {
     V3f c = etex >= 0 ? sampleTexture(s, etex, u, v) : V3f(m[M_EMIT], m[M_EMIT + 1], m[M_EMIT + 2]);
     return c * m[M_EMIT_FACTOR];
 }
+// BasicMaterial::adjustDoubleSidedFlag (BasicMaterial.cpp:662-673): transmissive materials are
+// double-sided.
+inline bool matDoubleSided(const float* m) {
+    return m[M_DOUBLE_SIDED] != 0.f ||
+           (int(m[M_TYPE]) == MAT_STANDARD && (m[M_DIFF_TRANS] > 0.f || m[M_SPEC_TRANS] > 0.f));
+}
+// StandardMaterial.slang:56-63 (eta with the incident medium's IoR 1, transmission colour) and
+// StandardBSDF.slang:131 (square root of the colour unless the surface is thin).
+inline void setTransmission(BsdfData<float>& d, const float* m, const ShadingData& sd) {
+    float ior = m[M_IOR];
+    d.eta = sd.frontFacing ? (1.f / ior) : (ior / 1.f);
+    d.diffTrans = m[M_DIFF_TRANS];
+    d.specTrans = m[M_SPEC_TRANS];
+    d.transmission = V3f(0.f);
+    if (d.diffTrans > 0.f || d.specTrans > 0.f) {
+        V3f t(m[M_TRANS], m[M_TRANS + 1], m[M_TRANS + 2]);
+        d.transmission = m[M_THIN] != 0.f ? t : V3f(std::sqrt(t.x), std::sqrt(t.y), std::sqrt(t.z));
+    }
+}
 inline BsdfData<float> bsdfDataFrom(const Scene& s, const float* m, const ShadingData& sd, bool sampling) {
     BsdfData<float> d;
     d.type = int(m[M_TYPE]);
@@ -249,6 +268,7 @@ inline BsdfData<float> bsdfDataFrom(const Scene& s, const float* m, const Shadin
         d.diffuse = base; d.specular = V3f(0.f); d.roughness = 0.f; d.metallic = 0.f;
         return d;
     }
+    setTransmission(d, m, sd);
     float ior = m[M_IOR], f = (ior - 1.f) / (ior + 1.f), F0 = f * f, metal = m[M_METAL];
     d.diffuse = base + (V3f(0.f) - base) * metal;
     d.specular = V3f(F0) + (base - V3f(F0)) * metal;
@@ -263,7 +283,7 @@ inline MatInst makeMatInst(const Scene& s, const ShadingData& sd) {
         mi.ds = bsdfDataFrom(s, &s.matsS[size_t(sd.mat) * M_STRIDE], sd, true);
         mi.d = bsdfDataFrom(s, m, sd, false);
         mi.sf = sd.frame;
-        if (!sd.frontFacing && m[M_DOUBLE_SIDED] != 0.f) mi.sf.N = -mi.sf.N;
+        if (!sd.frontFacing && matDoubleSided(m)) mi.sf.N = -mi.sf.N;
         mi.emission = V3f(0.f);
         if (mi.d.type == MAT_STANDARD && sd.frontFacing) mi.emission = evalEmissive(s, m, sd.uvx, sd.uvy);
         return mi;
@@ -274,7 +294,7 @@ inline MatInst makeMatInst(const Scene& s, const ShadingData& sd) {
     if (tex >= 0) base = sampleTexture(s, tex, sd.uvx, sd.uvy);
     mi.sf = sd.frame;
     // ShadingUtils.slang:107-117 (flipShadingNormal)
-    if (!sd.frontFacing && m[M_DOUBLE_SIDED] != 0.f) mi.sf.N = -mi.sf.N;
+    if (!sd.frontFacing && matDoubleSided(m)) mi.sf.N = -mi.sf.N;
     if (mi.d.type == MAT_PBRT_DIFFUSE) {
         mi.d.diffuse = base;
         mi.d.specular = V3f(0.f);
@@ -283,6 +303,7 @@ inline MatInst makeMatInst(const Scene& s, const ShadingData& sd) {
         mi.ds = mi.d;
         return mi;
     }
+    setTransmission(mi.d, m, sd);
     float ior = m[M_IOR];
     float f = (ior - 1.f) / (ior + 1.f);
     float F0 = f * f;
@@ -301,7 +322,8 @@ inline bool s_frozenGate(const MatInst& mi) {
     return mi.ds.type == MAT_STANDARD && (mi.ds.diffuse.x != mi.d.diffuse.x || mi.ds.diffuse.y != mi.d.diffuse.y ||
                                           mi.ds.diffuse.z != mi.d.diffuse.z || mi.ds.specular.x != mi.d.specular.x ||
                                           mi.ds.specular.y != mi.d.specular.y || mi.ds.specular.z != mi.d.specular.z ||
-                                          mi.ds.roughness != mi.d.roughness || mi.ds.metallic != mi.d.metallic);
+                                          mi.ds.roughness != mi.d.roughness || mi.ds.metallic != mi.d.metallic ||
+                                          mi.ds.specTrans != mi.d.specTrans || mi.ds.diffTrans != mi.d.diffTrans);
 }
 // ShadingUtils.slang:155-181 (reflection) and :214-240 (reflection or transmission, fullSphere=false)
 inline bool validHemisphere(const ShadingData& sd, V3f wiL, V3f woL, V3f wo) {
@@ -332,7 +354,9 @@ inline V3f miEval(const ShadingData& sd, MatInst& mi, V3f wo) {
         StdBsdf<float> g;
         g.init(wiL, mi.ds);
         mi.bsdf.pDiff = g.pDiff;
+        mi.bsdf.pDiffT = g.pDiffT;
         mi.bsdf.pSpec = g.pSpec;
+        mi.bsdf.pSpecT = g.pSpecT;
     }
     return mi.bsdf.eval(wiL, woL);
 }
@@ -343,7 +367,9 @@ inline float miEvalPdf(const ShadingData& sd, MatInst& mi, V3f wo) {
     mi.bsdf.init(wiL, mi.ds);
     return mi.bsdf.evalPdf(wiL, woL);
 }
-struct BsdfSample { V3f wo, weight; float pdf; bool delta; };
+// transmission: the sampled lobe is a transmission lobe (LobeType::Transmission), i.e. the local
+// direction lies below the shading hemisphere.
+struct BsdfSample { V3f wo, weight; float pdf; bool delta, transmission = false; };
 // StandardMaterialInstance.slang:89-110 (+ sampleReference :202-231) / PBRTDiffuseMaterialInstance.slang:66-78
 inline bool miSample(const ShadingData& sd, MatInst& mi, SampleGen& sg, BsdfSample& bs, bool importance = true) {
     V3f wiL = mi.sf.toLocal(sd.V);
@@ -358,9 +384,20 @@ inline bool miSample(const ShadingData& sd, MatInst& mi, SampleGen& sg, BsdfSamp
         return true;
     }
     if (!importance) {
+        // sampleReference (StandardMaterialInstance.slang:202-231): mirrored below the surface with
+        // probability 1/2 when the lobe set holds a transmission lobe (StandardBSDF.slang:205-225).
+        const BsdfData<float>& q = mi.ds;
+        bool anyDiffuse = q.diffuse.x > 0.f || q.diffuse.y > 0.f || q.diffuse.z > 0.f;
+        bool transmissive = q.specTrans > 0.f || (anyDiffuse && q.specTrans < 1.f && q.diffTrans > 0.f);
         float ux = sg.next1D(), uy = sg.next1D();
         woL = sample_cosine_hemisphere_concentric(ux, uy, bs.pdf);
-        if (std::min(wiL.z, woL.z) < kMinCosTheta || bs.pdf == 0.f) return false;
+        if (transmissive) {
+            if (sg.next1D() < 0.5f) woL.z = -woL.z;
+            bs.pdf *= 0.5f;
+            if (std::min(std::fabs(wiL.z), std::fabs(woL.z)) < kMinCosTheta || bs.pdf == 0.f) return false;
+        } else if (std::min(wiL.z, woL.z) < kMinCosTheta || bs.pdf == 0.f) {
+            return false;
+        }
         mi.bsdf.init(wiL, mi.ds);
         bs.weight = mi.bsdf.eval(wiL, woL) / bs.pdf;
     } else {
@@ -368,6 +405,7 @@ inline bool miSample(const ShadingData& sd, MatInst& mi, SampleGen& sg, BsdfSamp
         if (!mi.bsdf.sample(wiL, woL, bs.pdf, bs.weight, bs.delta, sg)) return false;
     }
     bs.wo = mi.sf.fromLocal(woL);
+    bs.transmission = woL.z < 0.f;
     if (!validHemisphere(sd, wiL, woL, bs.wo)) return false;
     return true;
 }
@@ -547,10 +585,11 @@ inline V3f mptTracePath(const Scene& s, const MptParams& P, uint32_t px, uint32_
     auto scatter = [&](const ShadingData& sdd, MatInst& mii, V3f org) -> bool {
         BsdfSample bs;
         if (miSample(sdd, mii, sg, bs, P.useImportanceSampling)) {
-            rOrigin = org;
+            // MinimalPathTracer.rt.slang:303-307: transmitted rays leave from the far side.
+            rOrigin = bs.transmission ? sdRayOrigin(sdd, false) : org;
             rDir = bs.wo;
             thp = thp * bs.weight;
-            prevPdf = bs.pdf; prevDelta = bs.delta; prevN = mii.sf.N; prevPos = org;
+            prevPdf = bs.pdf; prevDelta = bs.delta; prevN = mii.sf.N; prevPos = rOrigin;
             return anyPositive(thp);
         }
         return false;

@@ -44,6 +44,7 @@ def oracle_lib():
             "fo_sample_vmf": (None, [i32, vp, f32, vp]),
             "fo_bsdf_eval": (None, [i32, vp, vp, vp, vp]),
             "fo_bsdf_sample": (None, [i32, vp, vp, vp, vp]),
+            "fo_fresnel_dielectric": (None, [i32, vp, vp, vp]),
             "fo_scene_create": (vp, [u32, vp, vp, vp, u32, vp, vp, u32, vp, u32, vp, u64, vp, vp]),
             "fo_scene_destroy": (None, [vp]),
             "fo_scene_set_materials": (None, [vp, vp]),
@@ -98,7 +99,7 @@ class OracleScene:
 
     def set_materials(self, mats: np.ndarray):
         m = np.ascontiguousarray(mats, np.float32)
-        assert m.shape == (self.num_materials, 16)
+        assert m.shape == (self.num_materials, 20)
         self._mats = m
         oracle_lib().fo_scene_set_materials(self.h, _p(m))
 
@@ -239,8 +240,26 @@ class Oracle:
         return out
 
     @staticmethod
+    def bsdf_params(mat) -> np.ndarray:
+        """[type, base rgb, roughness, metallic, ior] (reflection lobes) or the 13-float form
+        [..., eta, specularTransmission, diffuseTransmission, transmission albedo rgb]."""
+        m = np.asarray(mat, np.float32).ravel()
+        if m.size == 7:
+            m = np.concatenate([m, np.float32([1.0 / m[6], 0, 0, 0, 0, 0])])
+        assert m.size == 13
+        return np.ascontiguousarray(m, np.float32)
+
+    @staticmethod
+    def fresnel_dielectric(eta, cos_i) -> np.ndarray:
+        e = np.ascontiguousarray(eta, np.float32).ravel()
+        c = np.ascontiguousarray(np.broadcast_to(cos_i, e.shape), np.float32)
+        out = np.zeros((e.size, 2), np.float32)
+        oracle_lib().fo_fresnel_dielectric(e.size, _p(e), _p(c), _p(out))
+        return out
+
+    @staticmethod
     def bsdf_eval(mat, wi, wo) -> np.ndarray:
-        m = np.ascontiguousarray(mat, np.float32)
+        m = Oracle.bsdf_params(mat)
         wi = np.ascontiguousarray(wi, np.float32)
         wo = np.ascontiguousarray(wo, np.float32)
         out = np.zeros((wi.shape[0], 4), np.float32)
@@ -249,7 +268,7 @@ class Oracle:
 
     @staticmethod
     def bsdf_sample(mat, wi, seeds) -> np.ndarray:
-        m = np.ascontiguousarray(mat, np.float32)
+        m = Oracle.bsdf_params(mat)
         wi = np.ascontiguousarray(wi, np.float32)
         s = np.ascontiguousarray(seeds, np.uint32)
         out = np.zeros((wi.shape[0], 8), np.float32)
