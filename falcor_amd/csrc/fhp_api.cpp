@@ -64,6 +64,9 @@ struct FhpScene_ {
     DevBuf meshTri;  // first triangle of each mesh (PackedHitInfo instance ids)
     DevBuf levels;   // BVH refit schedule (node ids by depth)
     DevBuf envTex;   // environment map texels (RGB fp32)
+    DevBuf lrefit;   // light BVH refit schedule
+    DevBuf restVtx;  // rest-pose vertex records for node transforms (uploaded on first use)
+    bool restValid = false;
     DevBuf ovf;  // traversal overflow stack
     DevBuf matsS, texelsS;  // frozen sampling parameters (finite-difference mode)
     TraceScratch ts{};      // default traversal configuration (ray queries)
@@ -373,6 +376,7 @@ int buildAndUpload(FhpScene_* s) {
     HIP_OR_FAIL(s->lbvhMask.upload(h.lbvhMask));
     HIP_OR_FAIL(s->meshTri.upload(h.meshTriOff));
     HIP_OR_FAIL(s->levels.upload(h.levelNodes));
+    HIP_OR_FAIL(s->lrefit.upload(h.lrefitNodes));
     // Overflow stack for the persistent traversal grid, sized for the smallest LDS stack and the
     // largest grid any configuration launches.
     s->stackNeed = h.stackNeed + 1;
@@ -417,9 +421,19 @@ int buildAndUpload(FhpScene_* s) {
     dv.nLbvh = h.nLbvh;
     s->deviceBytes = 0;
     for (DevBuf* b : {&s->bvh, &s->triPos, &s->idx, &s->vtx, &s->triMat, &s->mats, &s->texels, &s->texDesc, &s->lights,
-                      &s->active, &s->triToLight, &s->lbvh, &s->lbvhTri, &s->lbvhMask, &s->ovf, &s->matsS, &s->texelsS})
+                      &s->active, &s->triToLight, &s->lbvh, &s->lbvhTri, &s->lbvhMask, &s->ovf, &s->matsS, &s->texelsS,
+                      &s->lrefit, &s->restVtx})
         s->deviceBytes += b->bytes;
     return FHP_OK;
+}
+
+// A vertex update makes the new vertices the mesh's rest pose (its node transform is the identity
+// again); the device copy of the rest pose is refreshed on the next node transform.
+void setRestPose(FhpScene_* s, uint32_t v0, uint32_t v1) {
+    HostScene& h = s->h;
+    std::copy(h.pos.begin() + 3 * size_t(v0), h.pos.begin() + 3 * size_t(v1), h.restPos.begin() + 3 * size_t(v0));
+    std::copy(h.nrm.begin() + 3 * size_t(v0), h.nrm.begin() + 3 * size_t(v1), h.restNrm.begin() + 3 * size_t(v0));
+    s->restValid = false;
 }
 
 extern "C" {
@@ -451,6 +465,8 @@ int fhp_scene_create(const FhpSceneDesc* d, FhpScene* out) {
     h.nV = d->num_vertices; h.nT = d->num_triangles; h.nM = d->num_materials; h.nTex = d->num_textures;
     h.pos.assign(d->h_positions, d->h_positions + 3 * size_t(h.nV));
     h.nrm.assign(d->h_normals, d->h_normals + 3 * size_t(h.nV));
+    h.restPos = h.pos;
+    h.restNrm = h.nrm;
     h.uv.assign(d->h_texcoords, d->h_texcoords + 2 * size_t(h.nV));
     h.idx.assign(d->h_indices, d->h_indices + 3 * size_t(h.nT));
     h.triMat.assign(d->h_tri_material, d->h_tri_material + h.nT);
@@ -506,6 +522,7 @@ int fhp_scene_update_vertices(FhpScene s, uint32_t mesh, const float* pos, const
         if (!std::isfinite(pos[i]) || (nrm && !std::isfinite(nrm[i]))) return fail(FHP_EINVAL, "non-finite vertex data");
     std::copy(pos, pos + 3 * size_t(v1 - v0), h.pos.begin() + 3 * size_t(v0));
     if (nrm) std::copy(nrm, nrm + 3 * size_t(v1 - v0), h.nrm.begin() + 3 * size_t(v0));
+    setRestPose(s, v0, v1);
     return buildAndUpload(s);
 }
 
@@ -529,6 +546,7 @@ int fhp_scene_update_vertices_device(FhpScene s, uint32_t mesh, const float* d_p
         if (!std::isfinite(v)) return fail(FHP_EINVAL, "non-finite vertex data");
     std::copy(pos.begin(), pos.end(), h.pos.begin() + 3 * size_t(v0));
     if (d_nrm) std::copy(nrm.begin(), nrm.end(), h.nrm.begin() + 3 * size_t(v0));
+    setRestPose(s, v0, v1);
     // An emissive mesh also moves lights: rebuild the light collection and light BVH with the tree.
     bool emissive = false;
     for (uint32_t t = 0; t < h.nT && !emissive; t++) {
@@ -554,6 +572,108 @@ int fhp_scene_update_vertices_device(FhpScene s, uint32_t mesh, const float* d_p
     r.nLevels = uint32_t(h.levelOff.size() - 1);
     r.absPad = h.bvhAbsPad;
     HIP_OR_FAIL(launchRefit(r, st));
+    return FHP_OK;
+}
+
+int fhp_scene_update_node_transform(FhpScene s, uint32_t mesh, const float* h_matrix, FhpStream stream) {
+    if (!s || !h_matrix) return fail(FHP_EINVAL, "null argument");
+    HostScene& h = s->h;
+    if (mesh + 1 >= h.meshOff.size()) return fail(FHP_EINVAL, "mesh id out of range");
+    for (int i = 0; i < 12; i++)
+        if (!std::isfinite(h_matrix[i])) return fail(FHP_EINVAL, "non-finite transform");
+    // Normal matrix: inverse transpose of the 3x3 part (cofactors / determinant), in double.
+    double a[3][3];
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) a[r][c] = h_matrix[4 * r + c];
+    const double c00 = a[1][1] * a[2][2] - a[1][2] * a[2][1], c01 = a[1][2] * a[2][0] - a[1][0] * a[2][2],
+                 c02 = a[1][0] * a[2][1] - a[1][1] * a[2][0];
+    const double det = a[0][0] * c00 + a[0][1] * c01 + a[0][2] * c02;
+    if (!(det > 0.0))
+        return fail(FHP_EINVAL, "node transform must have a positive determinant (mirroring would flip the mesh's winding)");
+    const double cof[3][3] = {{c00, c01, c02},
+                              {a[0][2] * a[2][1] - a[0][1] * a[2][2], a[0][0] * a[2][2] - a[0][2] * a[2][0],
+                               a[0][1] * a[2][0] - a[0][0] * a[2][1]},
+                              {a[0][1] * a[1][2] - a[0][2] * a[1][1], a[0][2] * a[1][0] - a[0][0] * a[1][2],
+                               a[0][0] * a[1][1] - a[0][1] * a[1][0]}};
+    NodeXformArgs x{};
+    for (int i = 0; i < 12; i++) x.M[i] = h_matrix[i];
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) x.N[3 * r + c] = float(cof[r][c] / det);  // (A^-1)^T = cof(A) / det
+    const uint32_t v0 = h.meshOff[mesh], v1 = h.meshOff[mesh + 1], nv = v1 - v0;
+    hipStream_t st = hipStream_t(stream);
+    if (!s->restValid) {
+        std::vector<float> rest(8 * size_t(h.nV));
+        for (uint32_t i = 0; i < h.nV; i++) {
+            float* q = &rest[8 * size_t(i)];
+            q[0] = h.restPos[3 * i]; q[1] = h.restPos[3 * i + 1]; q[2] = h.restPos[3 * i + 2]; q[3] = h.restNrm[3 * i];
+            q[4] = h.restNrm[3 * i + 1]; q[5] = h.restNrm[3 * i + 2]; q[6] = h.uv[2 * i]; q[7] = h.uv[2 * i + 1];
+        }
+        HIP_OR_FAIL(s->restVtx.upload(rest));
+        s->restValid = true;
+    }
+    x.vtx = s->vtx.as<float4>();
+    x.rest = s->restVtx.as<float4>();
+    x.v0 = v0;
+    x.nVerts = nv;
+    HIP_OR_FAIL(launchNodeTransform(x, st));
+    // Host mirror of the moved vertices (later rebuilds, vertex reads) and the box padding.
+    std::vector<float> rec(8 * size_t(nv));
+    if (nv) HIP_OR_FAIL(hipMemcpyAsync(rec.data(), s->vtx.as<float4>() + 2 * size_t(v0), rec.size() * 4, hipMemcpyDeviceToHost, st));
+    HIP_OR_FAIL(hipStreamSynchronize(st));
+    float ext = 0.f;
+    for (uint32_t i = 0; i < nv; i++) {
+        const float* q = &rec[8 * size_t(i)];
+        float* p = &h.pos[3 * size_t(v0 + i)];
+        float* n = &h.nrm[3 * size_t(v0 + i)];
+        p[0] = q[0]; p[1] = q[1]; p[2] = q[2]; n[0] = q[3]; n[1] = q[4]; n[2] = q[5];
+        for (int k = 0; k < 3; k++) {
+            if (!std::isfinite(p[k])) return fail(FHP_EINVAL, "transformed vertices are not finite");
+            ext = std::max(ext, std::fabs(p[k]));
+        }
+    }
+    h.bvhAbsPad = std::max(h.bvhAbsPad, ext * 1e-6f + 1e-30f);
+    RefitArgs r{};
+    r.vtx = s->vtx.as<float4>();
+    r.triPos = s->triPos.as<float4>();
+    r.bvh = s->bvh.as<float>();
+    r.idx = s->idx.as<uint32_t>();
+    r.pos = nullptr;  // the vertex records already hold the new pose
+    r.v0 = v0;
+    r.nVerts = nv;
+    r.nTris = h.nT;
+    r.levelNodes = s->levels.as<uint32_t>();
+    r.levelOff = h.levelOff.data();
+    r.nLevels = uint32_t(h.levelOff.size() - 1);
+    r.absPad = h.bvhAbsPad;
+    HIP_OR_FAIL(launchRefit(r, st));
+    if (h.nLights) {
+        LightRefitArgs l{};
+        l.lights = s->lights.as<float>();
+        l.nLights = h.nLights;
+        l.vtx = s->vtx.as<float4>();
+        l.idx = s->idx.as<uint32_t>();
+        l.lbvh = s->lbvh.as<float>();
+        l.lbvhTri = s->lbvhTri.as<uint32_t>();
+        l.nodes = s->lrefit.as<uint32_t>();
+        l.off = h.lrefitOff.data();
+        l.depth = h.lrefitDepth;
+        HIP_OR_FAIL(launchLightRefit(l, st));
+        // Host mirrors of the light records and light BVH (fhp_scene_lights / fhp_scene_light_bvh).
+        HIP_OR_FAIL(hipMemcpyAsync(h.lights.data(), s->lights.p, h.lights.size() * 4, hipMemcpyDeviceToHost, st));
+        if (!h.lbvh.empty())
+            HIP_OR_FAIL(hipMemcpyAsync(h.lbvh.data(), s->lbvh.p, h.lbvh.size() * 4, hipMemcpyDeviceToHost, st));
+        HIP_OR_FAIL(hipStreamSynchronize(st));
+    }
+    return FHP_OK;
+}
+
+int fhp_scene_get_mesh_vertices(FhpScene s, uint32_t mesh, float* h_pos, float* h_nrm) {
+    if (!s) return fail(FHP_EINVAL, "null argument");
+    const HostScene& h = s->h;
+    if (mesh + 1 >= h.meshOff.size()) return fail(FHP_EINVAL, "mesh id out of range");
+    const uint32_t v0 = h.meshOff[mesh], v1 = h.meshOff[mesh + 1];
+    if (h_pos) std::copy(h.pos.begin() + 3 * size_t(v0), h.pos.begin() + 3 * size_t(v1), h_pos);
+    if (h_nrm) std::copy(h.nrm.begin() + 3 * size_t(v0), h.nrm.begin() + 3 * size_t(v1), h_nrm);
     return FHP_OK;
 }
 

@@ -175,7 +175,8 @@ struct RefitArgs {
     float4* triPos;          // leaf-order triangle records
     float* bvh;              // BVH4 nodes (32 floats each)
     const uint32_t* idx;     // triangle vertex indices
-    const float* pos;        // new positions of vertices [v0, v0 + nVerts) (3 floats each, device)
+    const float* pos;        // new positions of vertices [v0, v0 + nVerts) (3 floats each, device);
+                             // null when the vertex records already hold them
     const float* nrm;        // new normals (or null)
     uint32_t v0, nVerts, nTris;
     const uint32_t* levelNodes;  // device: node ids by depth
@@ -184,6 +185,33 @@ struct RefitArgs {
     float absPad;
 };
 hipError_t launchRefit(const RefitArgs& r, hipStream_t stream);
+
+// Node transform of one mesh (Scene::updateNodeTransform + the animation update): vertex records
+// [v0, v0 + nVerts) become M * rest position and N * rest normal (M: 3x4 row-major affine, N: the
+// inverse transpose of its 3x3 part).
+struct NodeXformArgs {
+    float4* vtx;
+    const float4* rest;  // rest-pose vertex records, same layout as vtx
+    uint32_t v0, nVerts;
+    float M[12], N[9];
+};
+hipError_t launchNodeTransform(const NodeXformArgs& a, hipStream_t stream);
+// LightCollection::updateTrianglePositions (LightCollection.cpp:550-574, UpdateTriangleVertices.cs.slang)
+// then LightBVH::refit (LightBVH.cpp:49-89, LightBVHRefit.cs.slang): every emissive triangle's
+// vertices, face normal and area from the vertex records (flux kept), then the light BVH's boxes
+// and normal cones, leaves first, then internal nodes from the deepest level up.
+struct LightRefitArgs {
+    float* lights;              // 20 floats per light
+    uint32_t nLights;
+    const float4* vtx;
+    const uint32_t* idx;
+    float* lbvh;                // 16 floats per node
+    const uint32_t* lbvhTri;
+    const uint32_t* nodes;      // device: refit schedule
+    const uint32_t* off;        // host: depth + 2 offsets (internal nodes by depth, then leaves)
+    uint32_t depth;
+};
+hipError_t launchLightRefit(const LightRefitArgs& a, hipStream_t stream);
 
 // AccumulatePass (Accumulate.cs.slang, AccumulatePass.cpp). Precision modes use the values of
 // AccumulatePass::Precision (AccumulatePass.h:70-75); input formats those of _INPUT_FORMAT

@@ -84,14 +84,144 @@ __global__ void kRefitLevel(float* __restrict__ bvh, const uint32_t* __restrict_
     }
 }
 
+// Node transform: rest-pose records (p.xyz, n.x) (n.yz, uv) -> (M p, N n); uvs kept. The normal
+// stays unnormalised, as the reference transforms interpolated normals by the inverse transpose
+// and normalises after interpolation (Scene.slang computeVertexData).
+struct Affine { float m[12]; };
+struct Linear3 { float m[9]; };
+__global__ void kTransformVertices(float4* __restrict__ vtx, const float4* __restrict__ rest, uint32_t v0, uint32_t n,
+                                   Affine M, Linear3 N) {
+    const uint32_t i = blockIdx.x * kRefitBlock + threadIdx.x;
+    if (i >= n) return;
+    const size_t v = size_t(v0) + i;
+    const float4 a = rest[2 * v], b = rest[2 * v + 1];
+    const float px = a.x, py = a.y, pz = a.z, nx = a.w, ny = b.x, nz = b.y;
+    float4 oa, ob;
+    oa.x = M.m[0] * px + M.m[1] * py + M.m[2] * pz + M.m[3];
+    oa.y = M.m[4] * px + M.m[5] * py + M.m[6] * pz + M.m[7];
+    oa.z = M.m[8] * px + M.m[9] * py + M.m[10] * pz + M.m[11];
+    oa.w = N.m[0] * nx + N.m[1] * ny + N.m[2] * nz;
+    ob.x = N.m[3] * nx + N.m[4] * ny + N.m[5] * nz;
+    ob.y = N.m[6] * nx + N.m[7] * ny + N.m[8] * nz;
+    ob.z = b.z;
+    ob.w = b.w;
+    vtx[2 * v] = oa;
+    vtx[2 * v + 1] = ob;
+}
+
+// UpdateTriangleVertices.cs.slang:42-63 with Scene::computeFaceNormalAndAreaW (Scene.slang:392-407):
+// light record (p0, area) (p1, flux) (p2, triangle) (normal, 0) (Le, texture); flux is kept.
+__global__ void kUpdateLights(float* __restrict__ lights, uint32_t n, const float4* __restrict__ vtx,
+                              const uint32_t* __restrict__ idx) {
+    const uint32_t i = blockIdx.x * kRefitBlock + threadIdx.x;
+    if (i >= n) return;
+    float* r = lights + 20 * size_t(i);
+    const uint32_t t = __float_as_uint(r[11]);
+    f3 p[3];
+    for (int j = 0; j < 3; j++) {
+        const float4 q = vtx[2 * size_t(idx[3 * size_t(t) + j])];
+        p[j] = mk3(q.x, q.y, q.z);
+    }
+    const f3 N = cross(p[1] - p[0], p[2] - p[0]);
+    const float area = 0.5f * length(N);
+    const f3 nn = normalize(N);
+    for (int j = 0; j < 3; j++) {
+        r[4 * j] = p[j].x;
+        r[4 * j + 1] = p[j].y;
+        r[4 * j + 2] = p[j].z;
+    }
+    r[3] = area;
+    r[12] = nn.x;
+    r[13] = nn.y;
+    r[14] = nn.z;
+}
+
+constexpr float kInvalidCone = -1.f;  // LightBVHTypes.slang:43 (kInvalidCosConeAngle)
+// Node record: (origin, flux) (extent, cosCone) (coneDir, 0) (right child | leaf info).
+__device__ __forceinline__ void setAABB(float* q, f3 lo, f3 hi) {
+    const f3 o = (hi + lo) * 0.5f, e = (hi - lo) * 0.5f;
+    q[0] = o.x; q[1] = o.y; q[2] = o.z;
+    q[4] = e.x; q[5] = e.y; q[6] = e.z;
+}
+
+// LightBVHRefit.cs.slang:46-93 (updateLeafNodes).
+__global__ void kLightRefitLeaves(float* __restrict__ lbvh, const uint32_t* __restrict__ nodes, uint32_t count,
+                                  const float* __restrict__ lights, const uint32_t* __restrict__ lbvhTri) {
+    const uint32_t k = blockIdx.x * kRefitBlock + threadIdx.x;
+    if (k >= count) return;
+    float* q = lbvh + 16 * size_t(nodes[k]);
+    const uint32_t first = __float_as_uint(q[13]), cnt = __float_as_uint(q[14]);
+    f3 lo = mk3(kFltMax), hi = mk3(-kFltMax), sum = mk3(0.f);
+    for (uint32_t i = 0; i < cnt; i++) {
+        const float* r = lights + 20 * size_t(lbvhTri[first + i]);
+        for (int j = 0; j < 3; j++) {
+            const f3 p = mk3(r[4 * j], r[4 * j + 1], r[4 * j + 2]);
+            lo = mk3(fminf(lo.x, p.x), fminf(lo.y, p.y), fminf(lo.z, p.z));
+            hi = mk3(fmaxf(hi.x, p.x), fmaxf(hi.y, p.y), fmaxf(hi.z, p.z));
+        }
+        sum = sum + mk3(r[12], r[13], r[14]);
+    }
+    setAABB(q, lo, hi);
+    const float len = length(sum);
+    const f3 dir = sum / len;
+    float cosCone = kInvalidCone;
+    if (len >= kFltMin) {
+        cosCone = 1.f;
+        for (uint32_t i = 0; i < cnt; i++) {
+            const float* r = lights + 20 * size_t(lbvhTri[first + i]);
+            cosCone = fminf(cosCone, dot(dir, mk3(r[12], r[13], r[14])));
+        }
+        cosCone = fmaxf(cosCone, -1.f);
+    }
+    q[7] = cosCone;
+    q[8] = dir.x; q[9] = dir.y; q[10] = dir.z;
+}
+
+__device__ __forceinline__ float sinFromCos(float c) { return sqrtf(fmaxf(0.f, 1.f - c * c)); }
+
+// LightBVHRefit.cs.slang:101-170 (updateInternalNodes): box union, and the normal cone that holds
+// both children's cones (each child's spread rotated away from the merged direction).
+__global__ void kLightRefitInternal(float* __restrict__ lbvh, const uint32_t* __restrict__ nodes, uint32_t count) {
+    const uint32_t k = blockIdx.x * kRefitBlock + threadIdx.x;
+    if (k >= count) return;
+    const uint32_t ni = nodes[k];
+    float* q = lbvh + 16 * size_t(ni);
+    const float* L = lbvh + 16 * size_t(ni + 1);
+    const float* R = lbvh + 16 * size_t(__float_as_uint(q[12]));
+    const f3 lo = mk3(fminf(L[0] - L[4], R[0] - R[4]), fminf(L[1] - L[5], R[1] - R[5]), fminf(L[2] - L[6], R[2] - R[6]));
+    const f3 hi = mk3(fmaxf(L[0] + L[4], R[0] + R[4]), fmaxf(L[1] + L[5], R[1] + R[5]), fmaxf(L[2] + L[6], R[2] + R[6]));
+    setAABB(q, lo, hi);
+    const f3 ld = mk3(L[8], L[9], L[10]), rd = mk3(R[8], R[9], R[10]);
+    const float lc = L[7], rc = R[7];
+    const f3 sum = ld + rd;
+    const float len = length(sum);
+    const f3 dir = sum / len;
+    float cosCone = kInvalidCone;
+    if (len >= kFltMin && lc != kInvalidCone && rc != kInvalidCone) {
+        const float cosL = dot(dir, ld), sinL = sinFromCos(cosL);
+        const float cosR = dot(dir, rd), sinR = sinFromCos(cosR);
+        const float sinLc = sinFromCos(lc), sinRc = sinFromCos(rc);
+        const float sinLt = sinLc * cosL + sinL * lc;
+        const float sinRt = sinRc * cosR + sinR * rc;
+        if (sinLt > 0.f && sinRt > 0.f) {
+            const float cosLt = lc * cosL - sinLc * sinL;
+            const float cosRt = rc * cosR - sinRc * sinR;
+            cosCone = fmaxf(fminf(cosLt, cosRt), -1.f);
+        }
+    }
+    q[7] = cosCone;
+    q[8] = dir.x; q[9] = dir.y; q[10] = dir.z;
+}
+
 uint32_t blocksFor(uint64_t n) { return uint32_t((n + kRefitBlock - 1) / kRefitBlock); }
 
 }  // namespace
 
 hipError_t launchRefit(const RefitArgs& r, hipStream_t st) {
     if (r.nVerts) {
-        hipLaunchKernelGGL(kUpdateVertices, dim3(blocksFor(r.nVerts)), dim3(kRefitBlock), 0, st, r.vtx, r.v0, r.nVerts, r.pos,
-                           r.nrm);
+        if (r.pos)
+            hipLaunchKernelGGL(kUpdateVertices, dim3(blocksFor(r.nVerts)), dim3(kRefitBlock), 0, st, r.vtx, r.v0, r.nVerts, r.pos,
+                               r.nrm);
         hipLaunchKernelGGL(kUpdateTriPos, dim3(blocksFor(r.nTris)), dim3(kRefitBlock), 0, st, r.triPos, r.nTris, r.idx, r.vtx,
                            r.v0, r.v0 + r.nVerts);
     }
@@ -100,6 +230,35 @@ hipError_t launchRefit(const RefitArgs& r, hipStream_t st) {
         if (e > b)
             hipLaunchKernelGGL(kRefitLevel, dim3(blocksFor(e - b)), dim3(kRefitBlock), 0, st, r.bvh, r.levelNodes + b, e - b,
                                r.triPos, r.absPad);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launchNodeTransform(const NodeXformArgs& a, hipStream_t st) {
+    if (a.nVerts) {
+        Affine M;
+        Linear3 N;
+        for (int i = 0; i < 12; i++) M.m[i] = a.M[i];
+        for (int i = 0; i < 9; i++) N.m[i] = a.N[i];
+        hipLaunchKernelGGL(kTransformVertices, dim3(blocksFor(a.nVerts)), dim3(kRefitBlock), 0, st, a.vtx, a.rest, a.v0,
+                           a.nVerts, M, N);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launchLightRefit(const LightRefitArgs& a, hipStream_t st) {
+    if (a.nLights)
+        hipLaunchKernelGGL(kUpdateLights, dim3(blocksFor(a.nLights)), dim3(kRefitBlock), 0, st, a.lights, a.nLights, a.vtx,
+                           a.idx);
+    const uint32_t lb = a.off[a.depth], le = a.off[a.depth + 1];
+    if (le > lb)
+        hipLaunchKernelGGL(kLightRefitLeaves, dim3(blocksFor(le - lb)), dim3(kRefitBlock), 0, st, a.lbvh, a.nodes + lb,
+                           le - lb, a.lights, a.lbvhTri);
+    for (int d = int(a.depth) - 1; d >= 0; d--) {
+        const uint32_t b = a.off[d], e = a.off[d + 1];
+        if (e > b)
+            hipLaunchKernelGGL(kLightRefitInternal, dim3(blocksFor(e - b)), dim3(kRefitBlock), 0, st, a.lbvh, a.nodes + b,
+                               e - b);
     }
     return hipGetLastError();
 }

@@ -655,6 +655,27 @@ void buildLightBVH(HostScene& s) {
         L.cones(0, c);
     }
     s.nLbvh = uint32_t(s.lbvh.size() / 16);
+    // Refit schedule: depth of every node (nodes are depth-first, left child = parent + 1).
+    s.lrefitNodes.clear();
+    s.lrefitOff.assign(2, 0u);  // no internal level, an empty leaf group
+    s.lrefitDepth = 0;
+    if (s.nLbvh == 0) return;
+    std::vector<uint32_t> depth(s.nLbvh, 0u);
+    std::vector<uint8_t> leaf(s.nLbvh, 0);
+    for (uint32_t n = 0; n < s.nLbvh; n++) {
+        uint32_t info;
+        std::memcpy(&info, &s.lbvh[16 * size_t(n) + 12], 4);
+        if (info & kLeafBit) { leaf[n] = 1; continue; }
+        depth[n + 1] = depth[n] + 1;
+        depth[info] = depth[n] + 1;
+        s.lrefitDepth = std::max(s.lrefitDepth, depth[n] + 1);
+    }
+    s.lrefitOff.assign(s.lrefitDepth + 2, 0u);
+    for (uint32_t n = 0; n < s.nLbvh; n++) s.lrefitOff[(leaf[n] ? s.lrefitDepth : depth[n]) + 1]++;
+    for (uint32_t d = 0; d <= s.lrefitDepth; d++) s.lrefitOff[d + 1] += s.lrefitOff[d];
+    s.lrefitNodes.assign(s.nLbvh, 0u);
+    std::vector<uint32_t> fill(s.lrefitOff.begin(), s.lrefitOff.end() - 1);
+    for (uint32_t n = 0; n < s.nLbvh; n++) s.lrefitNodes[fill[leaf[n] ? s.lrefitDepth : depth[n]]++] = n;
 }
 
 }  // namespace fhp

@@ -47,6 +47,14 @@ struct HostScene {
     std::vector<uint32_t> lbvhTri;
     std::vector<uint32_t> lbvhMask;  // 2 words per light
     uint32_t nLights = 0, nLbvh = 0;
+    // Light BVH refit schedule (LightBVH::updateNodeIndices, LightBVH.cpp:219-250): internal
+    // nodes of depth d at lrefitNodes[lrefitOff[d] .. lrefitOff[d + 1]) for d < lrefitDepth, then
+    // every leaf (any depth) at [lrefitOff[lrefitDepth] .. lrefitOff[lrefitDepth + 1]).
+    std::vector<uint32_t> lrefitNodes, lrefitOff;
+    uint32_t lrefitDepth = 0;
+    // Rest pose of every vertex (positions, normals): the vertices as created or as last set by a
+    // vertex update; a node transform maps a mesh's rest pose to its current pose.
+    std::vector<float> restPos, restNrm;
 };
 
 void computeCamera(HostScene& s);

@@ -41,8 +41,10 @@ class Scene:
         (SceneBuilder::Flags::UseCompressedHitInfo)."""
         L = _lib.load()
         self.data = data
+        # Own copies of the vertex arrays: vertex updates and node transforms write them, and the
+        # caller's SceneData stays as it was given.
         self._arrays = dict(
-            pos=np.ascontiguousarray(data.positions, np.float32), nrm=np.ascontiguousarray(data.normals, np.float32),
+            pos=np.array(data.positions, np.float32, order="C"), nrm=np.array(data.normals, np.float32, order="C"),
             uv=np.ascontiguousarray(data.texcoords, np.float32), idx=np.ascontiguousarray(data.indices, np.uint32),
             tm=np.ascontiguousarray(data.tri_material, np.uint32), mats=np.ascontiguousarray(data.materials, np.float32),
             td=np.ascontiguousarray(data.tex_desc, np.int32), tx=np.ascontiguousarray(data.texels, np.float32),
@@ -74,6 +76,9 @@ class Scene:
         if data.env_map is not None:
             e = data.env_map
             self.set_env_map(e["texels"], e.get("intensity", 1.0), e.get("tint", (1, 1, 1)), e.get("rotation", (0, 0, 0)))
+        nm = a["moff"].shape[0] - 1
+        xf = list(getattr(data, "mesh_transforms", None) or [])
+        self._rest_xf = [np.asarray(m, np.float64) for m in xf] if len(xf) == nm else [np.eye(4) for _ in range(nm)]
         # Incremented by every parameter change (materials, textures, vertices, sampling freeze);
         # AccumulatePass restarts when it moves (the scene updates of AccumulatePass.cpp:141-147).
         self.version = 0
@@ -160,6 +165,31 @@ class Scene:
         self._arrays["pos"][v0:v1] = p
         if n is not None:
             self._arrays["nrm"][v0:v1] = n
+        self._rest_xf[mesh_id] = np.eye(4)  # the new world-space vertices are the mesh's rest pose
+        self.version += 1
+
+    # Scene::updateNodeTransform (Scene.cpp:4282-4289) and the animation update it starts.
+    def update_node_transform(self, mesh_id: int, transform):
+        """Places mesh `mesh_id` with a new local-to-world transform (a 4x4 matrix or a
+        scene.Transform), as animating its scene-graph node does: the mesh's local vertices are
+        those it was built with (SceneBuilder transform undone), or the vertices last given to
+        set_mesh_vertices. On the GPU the BVH is refitted, the emissive triangles' positions,
+        normals and areas are recomputed (flux kept, LightCollection::updateTrianglePositions)
+        and the light BVH is refitted (LightBVH::refit). Mirroring transforms are refused."""
+        self.mesh_vertex_range(mesh_id)
+        T = transform.matrix() if hasattr(transform, "matrix") else np.asarray(transform, np.float64)
+        if T.shape != (4, 4):
+            raise ValueError("transform must be a 4x4 matrix or a Transform")
+        M = T @ np.linalg.inv(self._rest_xf[mesh_id])
+        m = np.ascontiguousarray(M[:3, :4], np.float32)
+        L = _lib.load()
+        _lib.check(L.fhp_scene_update_node_transform(self.handle, mesh_id, m.ctypes.data, _lib.stream_ptr()))
+        v0, v1 = self.mesh_vertex_range(mesh_id)
+        p = np.zeros((v1 - v0, 3), np.float32)
+        n = np.zeros((v1 - v0, 3), np.float32)
+        _lib.check(L.fhp_scene_get_mesh_vertices(self.handle, mesh_id, p.ctypes.data, n.ctypes.data))
+        self._arrays["pos"][v0:v1] = p
+        self._arrays["nrm"][v0:v1] = n
         self.version += 1
 
     def set_env_map(self, texels=None, intensity: float = 1.0, tint=(1.0, 1.0, 1.0), rotation=(0.0, 0.0, 0.0),

@@ -169,6 +169,14 @@ class Transform:
             r = _rot_z(az) @ _rot_y(ay) @ _rot_x(ax)
         return r, s, np.asarray(self.translation, np.float64)
 
+    def matrix(self) -> np.ndarray:
+        """4x4 local-to-world matrix T * R * S (float64)."""
+        r, s, t = self.matrices()
+        m = np.eye(4)
+        m[:3, :3] = r * s[None, :]
+        m[:3, 3] = t
+        return m
+
     @staticmethod
     def deg(scaling=(1, 1, 1), translation=(0, 0, 0), rotation_euler_deg=(0, 0, 0)) -> "Transform":
         return Transform(scaling, translation, tuple(math.radians(a) for a in rotation_euler_deg))
@@ -259,6 +267,8 @@ class SceneData:
     mesh_vertex_offsets: np.ndarray | None = None  # mesh m owns vertices [off[m], off[m + 1])
     # Environment map: {"texels": [H, W, 3], "intensity", "tint", "rotation" (degrees XYZ)} or None.
     env_map: dict | None = None
+    # Local-to-world matrix (4x4) each mesh was placed with (identity when built from world data).
+    mesh_transforms: list = field(default_factory=list)
 
     @property
     def num_triangles(self) -> int:
@@ -278,6 +288,7 @@ class SceneBuilder:
         self.mesh_names: list[str] = []
         self._nv = 0
         self._mesh_off = [0]
+        self._xf: list[np.ndarray] = []
         self.camera = Camera()
 
     def add_material(self, rec: np.ndarray) -> int:
@@ -309,6 +320,7 @@ class SceneBuilder:
         mat = np.broadcast_to(np.asarray(material, np.uint32), (idx.shape[0],))
         self._mat.append(mat.copy())
         self._mesh.append(np.full(idx.shape[0], mesh_id, np.uint32))
+        self._xf.append(t.matrix())
         self._nv += p.shape[0]
         self._mesh_off.append(self._nv)
         return mesh_id
@@ -325,7 +337,7 @@ class SceneBuilder:
             indices=np.concatenate(self._idx), tri_material=np.concatenate(self._mat),
             tri_mesh=np.concatenate(self._mesh), materials=np.stack(self.materials), camera=self.camera,
             tex_desc=tex_desc, texels=texels.astype(np.float32), mesh_names=list(self.mesh_names),
-            mesh_vertex_offsets=np.asarray(self._mesh_off, np.uint32))
+            mesh_vertex_offsets=np.asarray(self._mesh_off, np.uint32), mesh_transforms=[m.copy() for m in self._xf])
 
 
 # --------------------------------------------------------------------------- benchmark scenes

@@ -106,6 +106,20 @@ int fhp_scene_update_vertices(FhpScene scene, uint32_t mesh_id, const float* h_p
  * (synchronises the stream). */
 int fhp_scene_update_vertices_device(FhpScene scene, uint32_t mesh_id, const float* d_positions, const float* d_normals,
                                      int rebuild, FhpStream stream);
+/* Move mesh mesh_id as an animated scene-graph node: its vertices become M * rest position and
+ * M^-T * rest normal, where the rest pose is the mesh as created or as last set by a vertex
+ * update and h_matrix is the 3x4 row-major affine M (determinant > 0; the transform is not
+ * cumulative). On the GPU: the BVH is refitted, every emissive triangle's vertices, face normal
+ * and area are recomputed (flux kept), and the light BVH is refitted, leaves first, then internal
+ * nodes from the deepest level up. Synchronises the stream (host mirrors).
+ * Replaces Scene::updateNodeTransform (Scene.cpp:4282-4289) and the animation update it starts:
+ * LightCollection::update / updateTrianglePositions (LightCollection.cpp:78-114, :550-574,
+ * UpdateTriangleVertices.cs.slang) and LightBVHSampler::update -> LightBVH::refit
+ * (LightBVHSampler.cpp:36-75, LightBVH.cpp:49-89, LightBVHRefit.cs.slang). */
+int fhp_scene_update_node_transform(FhpScene scene, uint32_t mesh_id, const float* h_matrix, FhpStream stream);
+/* Current vertex positions and normals of mesh mesh_id (3 floats per vertex, host memory; either
+ * pointer may be null). Replaces Scene::getMeshVerticesAndIndices (Scene.cpp:4291-4312). */
+int fhp_scene_get_mesh_vertices(FhpScene scene, uint32_t mesh_id, float* h_positions, float* h_normals);
 /* Replace all material records (host memory, 20 * num_materials floats).
  * Replaces Scene::setMaterialParams (Source/Falcor/Scene/Scene.cpp:4620-4643). */
 int fhp_scene_set_materials(FhpScene scene, const float* h_materials);

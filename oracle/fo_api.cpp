@@ -172,6 +172,19 @@ void* fo_scene_create(uint32_t nV, const float* pos, const float* nrm, const flo
     return s;
 }
 void fo_scene_destroy(void* h) { delete static_cast<Scene*>(h); }
+// Animation update: new vertex positions and normals for every vertex (3 floats each), then
+// the traversal structure, the emissive triangles (positions, normals, areas; flux kept) and a
+// refit of the light BVH, as Scene::update does after a node transform.
+void fo_scene_animate(void* h, const float* pos, const float* nrm) {
+    Scene* s = static_cast<Scene*>(h);
+    for (uint32_t i = 0; i < s->nV; i++) {
+        s->pos[i] = V3f(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+        if (nrm) s->nrm[i] = V3f(nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]);
+    }
+    buildOracleBVH(*s);
+    updateLightPositions(*s);
+    refitLightBVH(*s);
+}
 void fo_scene_set_materials(void* h, const float* mats) {
     Scene* s = static_cast<Scene*>(h);
     s->mats.assign(mats, mats + size_t(s->nM) * M_STRIDE);

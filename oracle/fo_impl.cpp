@@ -426,6 +426,102 @@ void buildLightBVH(Scene& s) {
     B.cones(0, c);
 }
 
+// ---------------------------------------------------------------- animation update
+// UpdateTriangleVertices.cs.slang:42-63 (LightCollection::updateTrianglePositions,
+// LightCollection.cpp:550-574): every emissive triangle's world-space vertices, then
+// Scene::computeFaceNormalAndAreaW (Scene.slang:392-407); flux is left as built.
+void updateLightPositions(Scene& s) {
+    for (EmissiveTri& e : s.lights) {
+        s.triVerts(e.tri, e.p);
+        V3f e0 = e.p[1] - e.p[0], e1 = e.p[2] - e.p[0];
+        V3f N = cross(e0, e1);
+        e.area = 0.5f * length(N);
+        e.normal = normalize(N);
+    }
+}
+
+namespace {
+// LightBVHTypes.slang:56-69 (AABB stored as origin and half extent).
+void setNodeAABB(Scene::LNode& n, V3f lo, V3f hi) {
+    n.origin = (hi + lo) * 0.5f;
+    n.extent = (hi - lo) * 0.5f;
+}
+float sinOfCosine(float c) { return std::sqrt(std::max(0.0f, 1.0f - c * c)); }
+}  // namespace
+
+// LightBVH::refit (LightBVH.cpp:49-89) with LightBVHRefit.cs.slang: all leaves (updateLeafNodes,
+// :46-93), then the internal nodes level by level from the deepest (updateInternalNodes,
+// :101-170). Node depths follow LightBVH::updateNodeIndices (LightBVH.cpp:219-250).
+void refitLightBVH(Scene& s) {
+    const uint32_t n = uint32_t(s.lnodes.size());
+    if (n == 0) return;
+    std::vector<uint32_t> depth(n, 0);
+    uint32_t height = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        if (s.lnodes[i].leaf) continue;
+        depth[i + 1] = depth[i] + 1;
+        depth[s.lnodes[i].right] = depth[i] + 1;
+        height = std::max(height, depth[i] + 1);
+    }
+    const float kFltMaxF = std::numeric_limits<float>::max(), kFltMinF = std::numeric_limits<float>::min();
+    for (uint32_t i = 0; i < n; i++) {
+        Scene::LNode& node = s.lnodes[i];
+        if (!node.leaf) continue;
+        V3f lo(kFltMaxF), hi(-kFltMaxF), sum(0.f);
+        for (uint32_t k = 0; k < node.triCount; k++) {
+            const EmissiveTri& e = s.lights[s.lTriIdx[node.triOffset + k]];
+            for (int j = 0; j < 3; j++) {
+                lo = V3f(std::min(lo.x, e.p[j].x), std::min(lo.y, e.p[j].y), std::min(lo.z, e.p[j].z));
+                hi = V3f(std::max(hi.x, e.p[j].x), std::max(hi.y, e.p[j].y), std::max(hi.z, e.p[j].z));
+            }
+            sum = sum + e.normal;
+        }
+        setNodeAABB(node, lo, hi);
+        float len = length(sum);
+        V3f dir = sum / len;
+        float cosCone = -1.f;  // kInvalidCosConeAngle
+        if (len >= kFltMinF) {
+            cosCone = 1.f;
+            for (uint32_t k = 0; k < node.triCount; k++)
+                cosCone = std::min(cosCone, dot(dir, s.lights[s.lTriIdx[node.triOffset + k]].normal));
+            cosCone = std::max(cosCone, -1.f);
+        }
+        node.cosCone = cosCone;
+        node.coneDir = dir;
+    }
+    for (int d = int(height) - 1; d >= 0; d--) {
+        for (uint32_t i = 0; i < n; i++) {
+            Scene::LNode& node = s.lnodes[i];
+            if (node.leaf || depth[i] != uint32_t(d)) continue;
+            const Scene::LNode& L = s.lnodes[i + 1];
+            const Scene::LNode& R = s.lnodes[node.right];
+            V3f lmin = L.origin - L.extent, lmax = L.origin + L.extent;
+            V3f rmin = R.origin - R.extent, rmax = R.origin + R.extent;
+            setNodeAABB(node, V3f(std::min(lmin.x, rmin.x), std::min(lmin.y, rmin.y), std::min(lmin.z, rmin.z)),
+                        V3f(std::max(lmax.x, rmax.x), std::max(lmax.y, rmax.y), std::max(lmax.z, rmax.z)));
+            V3f sum = L.coneDir + R.coneDir;
+            float len = length(sum);
+            V3f dir = sum / len;
+            float cosCone = -1.f;
+            if (len >= kFltMinF && L.cosCone != -1.f && R.cosCone != -1.f) {
+                // Rotate each child's (cos, sin) of its offset from dir by its own spread.
+                float cosLD = dot(dir, L.coneDir), sinLD = sinOfCosine(cosLD);
+                float cosRD = dot(dir, R.coneDir), sinRD = sinOfCosine(cosRD);
+                float sinLC = sinOfCosine(L.cosCone), sinRC = sinOfCosine(R.cosCone);
+                float sinLT = sinLC * cosLD + sinLD * L.cosCone;
+                float sinRT = sinRC * cosRD + sinRD * R.cosCone;
+                if (sinLT > 0.f && sinRT > 0.f) {
+                    float cosLT = L.cosCone * cosLD - sinLC * sinLD;
+                    float cosRT = R.cosCone * cosRD - sinRC * sinRD;
+                    cosCone = std::max(std::min(cosLT, cosRT), -1.f);
+                }
+            }
+            node.cosCone = cosCone;
+            node.coneDir = dir;
+        }
+    }
+}
+
 // ---------------------------------------------------------------- LightBVH sampling
 // LightBVHSampler.slang:153-170
 static inline float cosSubClamped(float sinA, float cosA, float sinB, float cosB) {

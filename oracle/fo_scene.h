@@ -412,6 +412,8 @@ inline bool miSample(const ShadingData& sd, MatInst& mi, SampleGen& sg, BsdfSamp
 
 // ---------------------------------------------------------------- light collection
 void buildLightCollection(Scene& s);
+void updateLightPositions(Scene& s);
+void refitLightBVH(Scene& s);
 void buildLightBVH(Scene& s);
 
 struct LightSample { V3f posW, normalW, Le, dir; float pdf, distance; uint32_t light; };

@@ -45,6 +45,7 @@ def oracle_lib():
             "fo_bsdf_eval": (None, [i32, vp, vp, vp, vp]),
             "fo_bsdf_sample": (None, [i32, vp, vp, vp, vp]),
             "fo_fresnel_dielectric": (None, [i32, vp, vp, vp]),
+            "fo_scene_animate": (None, [vp, vp, vp]),
             "fo_scene_create": (vp, [u32, vp, vp, vp, u32, vp, vp, u32, vp, u32, vp, u64, vp, vp]),
             "fo_scene_destroy": (None, [vp]),
             "fo_scene_set_materials": (None, [vp, vp]),
@@ -87,6 +88,7 @@ class OracleScene:
                       np.ascontiguousarray(data.camera_array(), np.float32)]
         p, n, uv, idx, tm, mats, td, tx, cam = self._keep
         self.num_triangles = idx.shape[0]
+        self.num_vertices = p.shape[0]
         self.num_materials = mats.shape[0]
         self.h = L.fo_scene_create(p.shape[0], _p(p), _p(n), _p(uv), idx.shape[0], _p(idx), _p(tm), mats.shape[0],
                                    _p(mats), td.shape[0], _p(td) if td.size else None, tx.size, _p(tx) if tx.size else None,
@@ -96,6 +98,13 @@ class OracleScene:
         if getattr(self, "h", None):
             oracle_lib().fo_scene_destroy(self.h)
             self.h = None
+
+    def animate(self, positions: np.ndarray, normals: np.ndarray | None = None):
+        """All vertices move (3 floats each); emissive triangles are updated and the light BVH refitted."""
+        p = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+        assert p.shape[0] == self.num_vertices
+        n = np.ascontiguousarray(normals, np.float32).reshape(-1, 3) if normals is not None else None
+        oracle_lib().fo_scene_animate(self.h, _p(p), _p(n) if n is not None else None)
 
     def set_materials(self, mats: np.ndarray):
         m = np.ascontiguousarray(mats, np.float32)
