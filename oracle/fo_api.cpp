@@ -1,6 +1,7 @@
 // C entry points of the oracle library (loaded with ctypes by tests and bench.py's CPU leg).
 // TEST INFRASTRUCTURE ONLY (see fo_core.h).
 #include "fo_scene.h"
+#include "fo_f64.h"
 #include <atomic>
 #include <thread>
 
@@ -333,6 +334,21 @@ void fo_render_mpt(void* h, uint32_t W, uint32_t H, uint32_t spp, uint32_t frame
         for (auto& r : rowStats) { t.closest += r.closest; t.visibility += r.visibility; t.nodes += r.nodes; t.tris += r.tris; }
         stats[0] = t.closest; stats[1] = t.visibility; stats[2] = t.nodes; stats[3] = t.tris;
     }
+}
+// MinimalPathTracer in double precision (fo_f64.h): per-pixel mean over samples frame0 * spp + k,
+// out = H x W x 3 doubles. flags: 1 computeDirect, 2 useImportanceSampling.
+void fo_render_mpt_f64(void* h, uint32_t W, uint32_t H, uint32_t spp, uint32_t frame0, uint32_t maxBounces, uint32_t flags,
+                       int threads, double* out) {
+    Scene* s = static_cast<Scene*>(h);
+    parallelRows(H, hwThreads(threads), [&](uint32_t y) {
+        for (uint32_t x = 0; x < W; x++) {
+            f64::V3d sum(0.0);
+            for (uint32_t k = 0; k < spp; k++)
+                sum = sum + f64::tracePathD(*s, maxBounces, flags & 1, flags & 2, x, y, W, H, frame0 * spp + k);
+            double* q = out + 3 * (size_t(y) * W + x);
+            q[0] = sum.x / spp; q[1] = sum.y / spp; q[2] = sum.z / spp;
+        }
+    });
 }
 // Rays traced by MinimalPathTracer for pixels [p0, p1) (row-major), sample number `sampleNumber`.
 // rec: per ray 13 words: o xyz, tmin, d xyz, tmax, tri, t, b1, b2, kind (0 closest, 1 visibility). Returns the ray count

@@ -46,6 +46,7 @@ def oracle_lib():
             "fo_bsdf_sample": (None, [i32, vp, vp, vp, vp]),
             "fo_fresnel_dielectric": (None, [i32, vp, vp, vp]),
             "fo_scene_animate": (None, [vp, vp, vp]),
+            "fo_render_mpt_f64": (None, [vp, u32, u32, u32, u32, u32, u32, i32, vp]),
             "fo_scene_create": (vp, [u32, vp, vp, vp, u32, vp, vp, u32, vp, u32, vp, u64, vp, vp]),
             "fo_scene_destroy": (None, [vp]),
             "fo_scene_set_materials": (None, [vp, vp]),
@@ -162,6 +163,13 @@ class OracleScene:
         flags = int(compute_direct) | (int(importance) << 1) | (int(nee) << 2) | (int(brute) << 3)
         oracle_lib().fo_render_mpt(self.h, W, H, spp, frame, max_bounces, flags, light_sampler, threads, _p(out), _p(stats))
         return out, stats
+
+    def render_mpt_f64(self, W, H, spp=1, frame=0, max_bounces=3, compute_direct=True, importance=True, threads=0):
+        """MinimalPathTracer evaluated in double precision (float64 image [H, W, 3])."""
+        out = np.zeros((H, W, 3), np.float64)
+        flags = int(compute_direct) | (int(importance) << 1)
+        oracle_lib().fo_render_mpt_f64(self.h, W, H, spp, frame, max_bounces, flags, threads, _p(out))
+        return out
 
     def record_mpt(self, W, H, p0, p1, sample_number=0, max_bounces=3, compute_direct=True, importance=True, nee=False,
                    light_sampler=1, brute=False, cap=1 << 22):
