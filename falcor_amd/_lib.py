@@ -113,6 +113,13 @@ _SIGS = {
     "fhp_scene_set_env_map": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_uint32]),
     "fhp_scene_update_vertices_device": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
     "fhp_scene_update_node_transform": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
+    "fhp_session_create": (C.c_int, [C.c_int, C.c_void_p]),
+    "fhp_session_stream": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "fhp_session_destroy": (C.c_int, [C.c_void_p]),
+    "fhp_bvh_build": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "fhp_scene_get_materials": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "fhp_scene_get_material_params": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
+    "fhp_scene_set_material_params": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
     "fhp_scene_get_mesh_vertices": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
     "fhp_accumulate_frame_count": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
     "fhp_image_compare": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_int, C.c_void_p,

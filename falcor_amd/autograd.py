@@ -63,7 +63,7 @@ def _fn():
         def forward(ctx, renderer, materials, texels, positions, normals):
             r = renderer
             if materials is not None:
-                r.scene.set_material_params(material_records(materials.detach().cpu().numpy(), r.scene.materials))
+                r.scene.set_material_records(material_records(materials.detach().cpu().numpy(), r.scene.materials))
             if texels is not None:
                 r.scene.set_texels(texels.detach())
             if positions is not None:

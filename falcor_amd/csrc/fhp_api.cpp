@@ -58,6 +58,11 @@ struct DevBuf {
 };
 }  // namespace
 
+struct FhpSession_ {
+    int device;
+    hipStream_t stream;
+};
+
 struct FhpScene_ {
     HostScene h;
     DevBuf bvh, triPos, idx, vtx, triMat, mats, texels, texDesc, lights, active, triToLight, lbvh, lbvhTri, lbvhMask;
@@ -67,6 +72,7 @@ struct FhpScene_ {
     DevBuf lrefit;   // light BVH refit schedule
     DevBuf restVtx;  // rest-pose vertex records for node transforms (uploaded on first use)
     bool restValid = false;
+    uint32_t leafSize = 4;  // BVH4 leaf size (triangles)
     DevBuf ovf;  // traversal overflow stack
     DevBuf matsS, texelsS;  // frozen sampling parameters (finite-difference mode)
     TraceScratch ts{};      // default traversal configuration (ray queries)
@@ -347,7 +353,7 @@ int collectTimer(FhpPass p, hipStream_t st) {
 int buildAndUpload(FhpScene_* s) {
     HostScene& h = s->h;
     try {
-        buildBVH4(h, 4);
+        buildBVH4(h, s->leafSize);
         buildLights(h);
         buildLightBVH(h);
     } catch (const std::exception& e) {
@@ -439,6 +445,32 @@ void setRestPose(FhpScene_* s, uint32_t v0, uint32_t v1) {
 extern "C" {
 
 int fhp_get_version(void) { return 100; }
+
+int fhp_session_create(int device, FhpSession* out) {
+    if (!out) return fail(FHP_EINVAL, "null argument");
+    int count = 0;
+    HIP_OR_FAIL(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count) return fail(FHP_EINVAL, "device ordinal out of range");
+    HIP_OR_FAIL(hipSetDevice(device));
+    hipStream_t st = nullptr;
+    HIP_OR_FAIL(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));  // default priority
+    *out = new FhpSession_{device, st};
+    return FHP_OK;
+}
+
+int fhp_session_stream(FhpSession s, FhpStream* out) {
+    if (!s || !out) return fail(FHP_EINVAL, "null argument");
+    *out = FhpStream(s->stream);
+    return FHP_OK;
+}
+
+int fhp_session_destroy(FhpSession s) {
+    if (!s) return FHP_OK;
+    const hipError_t e = hipStreamDestroy(s->stream);
+    delete s;
+    if (e != hipSuccess) return fail(FHP_EHIP, hipGetErrorString(e));
+    return FHP_OK;
+}
 
 int fhp_pass_kernel_times(FhpPass p, double* ms, uint64_t* launches, int reset) {
     if (!p) return fail(FHP_EINVAL, "null argument");
@@ -667,6 +699,13 @@ int fhp_scene_update_node_transform(FhpScene s, uint32_t mesh, const float* h_ma
     return FHP_OK;
 }
 
+int fhp_bvh_build(FhpScene s, uint32_t max_leaf_triangles) {
+    if (!s) return fail(FHP_EINVAL, "null argument");
+    if (max_leaf_triangles > 8) return fail(FHP_EINVAL, "a BVH4 leaf holds at most 8 triangles");
+    s->leafSize = max_leaf_triangles ? max_leaf_triangles : 4u;
+    return buildAndUpload(s);
+}
+
 int fhp_scene_get_mesh_vertices(FhpScene s, uint32_t mesh, float* h_pos, float* h_nrm) {
     if (!s) return fail(FHP_EINVAL, "null argument");
     const HostScene& h = s->h;
@@ -743,6 +782,12 @@ int fhp_scene_info(FhpScene s, FhpSceneInfo* o) {
     return FHP_OK;
 }
 
+int fhp_scene_get_materials(FhpScene s, float* mats) {
+    if (!s || !mats) return fail(FHP_EINVAL, "null argument");
+    std::copy(s->h.mats.begin(), s->h.mats.end(), mats);
+    return FHP_OK;
+}
+
 int fhp_scene_set_materials(FhpScene s, const float* mats) {
     if (!s || !mats) return fail(FHP_EINVAL, "null argument");
     for (uint32_t m = 0; m < s->h.nM; m++) {
@@ -754,6 +799,59 @@ int fhp_scene_set_materials(FhpScene s, const float* mats) {
     s->h.mats.assign(mats, mats + size_t(s->h.nM) * M_STRIDE);
     HIP_OR_FAIL(hipMemcpy(s->mats.p, mats, size_t(s->h.nM) * M_STRIDE * 4, hipMemcpyHostToDevice));
     return FHP_OK;
+}
+
+// StandardMaterialParamLayout.slang:32-41: serialized offset, size, record field, deserialize
+// clamp range (MaterialParamLayout.slangh:40-45, defaults [1e-4, 1 - 1e-4]) and whether the
+// material stores the value in float16 (BasicMaterialData.slang:50-66, MaterialData.slang:183).
+struct ParamEntry { int offset, size, field; float lo, hi; bool half; };
+constexpr float kParamEps = 1e-4f;
+constexpr ParamEntry kParamLayout[] = {
+    {0, 3, M_BASE, kParamEps, 1.f - kParamEps, true},        // baseColor
+    {3, 1, M_METAL, kParamEps, 1.f - kParamEps, true},       // metallic
+    {4, 1, M_ROUGH, 0.05f, 1.f - kParamEps, true},           // roughness
+    {5, 1, M_IOR, 1e-3f, 1e3f, true},                        // ior
+    {6, 3, M_TRANS, kParamEps, 1.f - kParamEps, true},       // transmissionColor
+    {9, 1, M_DIFF_TRANS, 0.f, 1.f - kParamEps, true},        // diffuseTransmission
+    {10, 1, M_SPEC_TRANS, 0.f, 1.f - kParamEps, true},       // specularTransmission
+    {11, 3, M_EMIT, kParamEps, 1.f - kParamEps, false},      // emissiveColor
+    {14, 1, M_EMIT_FACTOR, 1e-8f, 1e8f, false},              // emissiveFactor
+};
+constexpr int kParamCount = 20;  // SerializedMaterialParams::kParamCount
+
+int fhp_scene_get_material_params(FhpScene s, uint32_t n, const uint32_t* ids, float* params) {
+    if (!s || (n && (!ids || !params))) return fail(FHP_EINVAL, "null argument");
+    for (uint32_t i = 0; i < n; i++) {
+        if (ids[i] >= s->h.nM) return fail(FHP_EINVAL, "material id out of range");
+        const float* r = &s->h.mats[size_t(ids[i]) * M_STRIDE];
+        if (int(r[M_TYPE]) != MAT_STANDARD)
+            return fail(FHP_EINVAL, "material " + std::to_string(ids[i]) + " does not support serializing parameters");
+        float* o = params + size_t(i) * kParamCount;
+        std::fill(o, o + kParamCount, 0.f);
+        for (const ParamEntry& e : kParamLayout)
+            for (int k = 0; k < e.size; k++) o[e.offset + k] = r[e.field + k];
+    }
+    return FHP_OK;
+}
+
+int fhp_scene_set_material_params(FhpScene s, uint32_t n, const uint32_t* ids, const float* params) {
+    if (!s || (n && (!ids || !params))) return fail(FHP_EINVAL, "null argument");
+    std::vector<float> mats = s->h.mats;
+    for (uint32_t i = 0; i < n; i++) {
+        if (ids[i] >= s->h.nM) return fail(FHP_EINVAL, "material id out of range");
+        float* r = &mats[size_t(ids[i]) * M_STRIDE];
+        if (int(r[M_TYPE]) != MAT_STANDARD)
+            return fail(FHP_EINVAL, "material " + std::to_string(ids[i]) + " does not support deserializing parameters");
+        const float* q = params + size_t(i) * kParamCount;
+        for (const ParamEntry& e : kParamLayout)
+            for (int k = 0; k < e.size; k++) {
+                const float v = q[e.offset + k];
+                if (std::isnan(v)) return fail(FHP_EINVAL, "NaN material parameter");
+                const float c = std::min(std::max(v, e.lo), e.hi);
+                r[e.field + k] = e.half ? float(_Float16(c)) : c;
+            }
+    }
+    return fhp_scene_set_materials(s, mats.data());
 }
 
 int fhp_scene_set_texels(FhpScene s, const float* d_texels, uint64_t n, FhpStream stream) {

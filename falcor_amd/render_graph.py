@@ -17,7 +17,7 @@ from pathlib import Path
 import numpy as np
 
 from . import _lib
-from .scene import MAT_STRIDE, SceneData
+from .scene import MAT_STANDARD as S_MAT_STANDARD, MAT_STRIDE, SceneData
 
 
 def _torch():
@@ -33,6 +33,23 @@ def _device():
 
 
 # --------------------------------------------------------------------------- scene
+class Material:
+    """Material parameter layout (Material.cpp:409-431: PARAM_COUNT, get_material_param_layout)."""
+
+    PARAM_COUNT = 20
+    # StandardMaterialParamLayout.slang:32-41 (python names, offsets and sizes).
+    LAYOUT = {"base_color": (0, 3), "metallic": (3, 1), "roughness": (4, 1), "ior": (5, 1),
+              "transmission_color": (6, 3), "diffuse_transmission": (9, 1), "specular_transmission": (10, 1),
+              "emissive_color": (11, 3), "emissive_factor": (14, 1)}
+
+
+def get_material_param_layout(material_type: str = "Standard") -> dict:
+    """{python name: {"offset", "size"}} of a material type's serialized parameters."""
+    if material_type not in ("Standard", 0, S_MAT_STANDARD):
+        raise ValueError(f"material type {material_type!r} has no parameter layout here")
+    return {k: {"offset": o, "size": n} for k, (o, n) in Material.LAYOUT.items()}
+
+
 class Scene:
     """Device-resident scene (BVH4, light collection, light BVH, materials, textures)."""
 
@@ -101,16 +118,51 @@ class Scene:
         out["camera_basis"] = np.array(i.camera_basis[:], np.float32)
         return out
 
-    # Scene::get/set_material_params (Scene.cpp:4620-4643)
-    def get_material_params(self) -> np.ndarray:
+    # Material records (the scene description's 20-float layout, see scene.py).
+    def get_material_records(self) -> np.ndarray:
         return self.materials.copy()
 
-    def set_material_params(self, mats: np.ndarray):
+    def set_material_records(self, mats: np.ndarray):
         m = np.ascontiguousarray(mats, np.float32)
         if m.shape != self.materials.shape:
             raise ValueError(f"expected materials of shape {self.materials.shape}")
         _lib.check(_lib.load().fhp_scene_set_materials(self.handle, m.ctypes.data))
         self.materials = m.copy()
+        self.version += 1
+
+    # Scene.get_material_params / set_material_params (Scene.cpp:4426-4476, 4620-4643): serialized
+    # StandardMaterialParamLayout values (Material.PARAM_COUNT floats per material) of a list of
+    # material ids. `params` may be a torch tensor or numpy array to fill, as the reference fills
+    # its parameter buffer.
+    @staticmethod
+    def _ids(material_ids) -> np.ndarray:
+        if hasattr(material_ids, "detach"):
+            material_ids = material_ids.detach().cpu().numpy()
+        return np.ascontiguousarray(np.asarray(material_ids).reshape(-1), np.uint32)
+
+    def get_material_params(self, material_ids, params=None):
+        ids = self._ids(material_ids)
+        out = np.zeros((ids.size, Material.PARAM_COUNT), np.float32)
+        _lib.check(_lib.load().fhp_scene_get_material_params(self.handle, ids.size, ids.ctypes.data, out.ctypes.data))
+        if params is None:
+            return out
+        if hasattr(params, "copy_"):
+            torch = _torch()
+            params.view(-1)[: out.size].copy_(torch.from_numpy(out.reshape(-1)))
+        else:
+            np.asarray(params).reshape(-1)[: out.size] = out.reshape(-1)
+        return params
+
+    def set_material_params(self, material_ids, params):
+        ids = self._ids(material_ids)
+        if hasattr(params, "detach"):
+            params = params.detach().float().cpu().numpy()
+        p = np.ascontiguousarray(np.asarray(params, np.float32).reshape(-1)[: ids.size * Material.PARAM_COUNT])
+        if p.size != ids.size * Material.PARAM_COUNT:
+            raise ValueError(f"expected {ids.size * Material.PARAM_COUNT} parameters")
+        L = _lib.load()
+        _lib.check(L.fhp_scene_set_material_params(self.handle, ids.size, ids.ctypes.data, p.ctypes.data))
+        _lib.check(L.fhp_scene_get_materials(self.handle, self.materials.ctypes.data))
         self.version += 1
 
     # Scene::getMeshVerticesAndIndices / setMeshVertices (Scene.cpp:4314-4336)

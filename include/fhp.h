@@ -27,6 +27,7 @@ extern "C" {
 typedef struct FhpScene_* FhpScene;
 typedef struct FhpPass_* FhpPass;
 typedef struct FhpGrads_* FhpGrads;
+typedef struct FhpSession_* FhpSession;
 typedef void* FhpStream; /* hipStream_t */
 
 /* Library version (major * 10000 + minor * 100 + patch). */
@@ -35,6 +36,15 @@ int fhp_get_version(void);
 const char* fhp_last_error(void);
 /* Message of the last warning (for example an unknown property), "" if none. */
 const char* fhp_last_warning(void);
+
+/* ---------------------------------------------------------------- session
+ * A device and a stream for callers without their own (torch callers pass torch's current
+ * stream instead). Selects device_ordinal (hipSetDevice) and creates a non-blocking stream of
+ * default priority; every FhpStream argument accepts it. Replaces the Device / RenderContext a
+ * Falcor host hands to its passes (Source/Falcor/Core/API/Device.h, RenderContext.h). */
+int fhp_session_create(int device_ordinal, FhpSession* out);
+int fhp_session_stream(FhpSession session, FhpStream* out);
+int fhp_session_destroy(FhpSession session);
 
 /* ---------------------------------------------------------------- scene
  * Scene description: world-space triangle geometry (instances flattened by the caller).
@@ -120,6 +130,26 @@ int fhp_scene_update_node_transform(FhpScene scene, uint32_t mesh_id, const floa
 /* Current vertex positions and normals of mesh mesh_id (3 floats per vertex, host memory; either
  * pointer may be null). Replaces Scene::getMeshVerticesAndIndices (Scene.cpp:4291-4312). */
 int fhp_scene_get_mesh_vertices(FhpScene scene, uint32_t mesh_id, float* h_positions, float* h_normals);
+/* Rebuild the BVH4 (leaves of at most max_leaf_triangles triangles, 1..8; 0 = 4), the light
+ * collection and the light BVH from the scene's current vertices. Replaces the BLAS/TLAS builds
+ * (Scene::buildBlas / buildTlas, Scene.cpp:3376, :3936-4040); creation and vertex updates build
+ * the same structures by themselves. */
+int fhp_bvh_build(FhpScene scene, uint32_t max_leaf_triangles);
+/* Serialized StandardMaterial parameters of materials h_ids[0..n): 20 floats each in
+ * StandardMaterialParamLayout (StandardMaterialParamLayout.slang:32-41: baseColor 0-2,
+ * metallic 3, roughness 4, ior 5, transmissionColor 6-8, diffuseTransmission 9,
+ * specularTransmission 10, emissiveColor 11-13, emissiveFactor 14, 15-19 zero). Other material
+ * types are refused. Replaces Scene get_material_params / Material::serializeParams
+ * (Scene.cpp:4426-4447, StandardMaterial.cpp:234-238). */
+int fhp_scene_get_material_params(FhpScene scene, uint32_t n, const uint32_t* h_ids, float* h_params);
+/* The inverse: each value is clamped to its layout range (MaterialParamLayout.slangh:40-45:
+ * [1e-4, 1 - 1e-4] by default, roughness >= 0.05, ior [1e-3, 1e3], transmissions >= 0,
+ * emissiveFactor [1e-8, 1e8]) and rounded to float16 where the reference stores float16
+ * (colours, metallic, roughness, ior, transmissions); type, double-sided, textures and thin
+ * surface are kept. Replaces Scene set_material_params (Scene.cpp:4453-4476). */
+int fhp_scene_set_material_params(FhpScene scene, uint32_t n, const uint32_t* h_ids, const float* h_params);
+/* Copy all material records into host memory (20 * num_materials floats). */
+int fhp_scene_get_materials(FhpScene scene, float* h_materials);
 /* Replace all material records (host memory, 20 * num_materials floats).
  * Replaces Scene::setMaterialParams (Source/Falcor/Scene/Scene.cpp:4620-4643). */
 int fhp_scene_set_materials(FhpScene scene, const float* h_materials);

@@ -430,12 +430,10 @@ inline bool sampleTriangle(const Scene& s, V3f posW, uint32_t li, float ux, floa
     ls.normalW = tri.normal;
     float cosTheta = dot(ls.normalW, -ls.dir);
     if (cosTheta <= 0.f) return false;
-    ls.Le = tri.Le;
-    if (tri.etex >= 0) {
-        const float u = tri.uv[0][0] * b.x + tri.uv[1][0] * b.y + tri.uv[2][0] * b.z;
-        const float v = tri.uv[0][1] * b.x + tri.uv[1][1] * b.y + tri.uv[2][1] * b.z;
-        ls.Le = sampleTexture(s, tri.etex, u, v) * tri.Le;
-    }
+    // EmissiveLightSamplerHelpers.slang:79: evalEmissive of the triangle's current material.
+    const float u = tri.uv[0][0] * b.x + tri.uv[1][0] * b.y + tri.uv[2][0] * b.z;
+    const float v = tri.uv[0][1] * b.x + tri.uv[1][1] * b.y + tri.uv[2][1] * b.z;
+    ls.Le = evalEmissive(s, s.mat(s.triMat[tri.tri]), u, v);
     float denom = std::max(std::numeric_limits<float>::min(), cosTheta * tri.area);
     ls.pdf = distSqr / denom;
     return true;

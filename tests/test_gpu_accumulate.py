@@ -140,8 +140,8 @@ def test_accumulate_after_path_tracer_in_graph(gpu):
     mean = (frames[0] + frames[1] + frames[2]) * np.float32(1.0 / 3.0)
     assert torch.allclose(out, mean, rtol=1e-6, atol=1e-7)
     assert acc.frame_count == 3
-    m = sc.get_material_params()
-    sc.set_material_params(m)
+    m = sc.get_material_records()
+    sc.set_material_records(m)
     v = g.execute(sc, 32, 24)
     assert acc.frame_count == 1
     assert torch.equal(v["AccumulatePass.output"], v["MinimalPathTracer.color"])

@@ -80,3 +80,25 @@ def test_war_primal_textured_emitters_match_oracle(gpu, oracle_mod):
     img = p.render(96, 96, spp=4, frame=0).cpu().numpy()[..., :3]
     ref, _ = osc.render_war(96, 96, spp=4, frame=0, max_bounces=2)
     _compare(img, ref, "WAR primal, textured emitters")
+
+
+def test_emission_updates_reach_light_samples(gpu, oracle_mod):
+    """Light samples evaluate the emitter's current material (evalEmissive at the sample,
+    EmissiveLightSamplerHelpers.slang:79): after a material update NEE sees the new emission,
+    while the light collection (flux, selection) stays as built, as in the reference."""
+    from falcor_amd.render_graph import MinimalPathTracer, Scene
+    data = S.closed_box(3)
+    sc, osc = Scene(data), oracle_mod.OracleScene(data)
+    p = MinimalPathTracer({"maxBounces": 2, "useEmissiveNEE": True, "emissiveSampler": "Uniform"})
+    p.set_scene(sc)
+    before = p.render(64, 64, 2, frame=1).cpu().numpy()[..., :3]
+    m = sc.get_material_records()
+    light = int(np.nonzero(m[:, S.M_EMIT:S.M_EMIT + 3].max(1) > 0)[0][0])
+    m[light, S.M_EMIT:S.M_EMIT + 3] = (5.0, 20.0, 40.0)
+    m[light, S.M_EMIT_FACTOR] = 0.5
+    sc.set_material_records(m)
+    osc.set_materials(m)
+    img = p.render(64, 64, 2, frame=1).cpu().numpy()[..., :3]
+    ref, _ = osc.render_mpt(64, 64, spp=2, frame=1, max_bounces=2, nee=True, light_sampler=0)
+    _compare(img, ref, "emission update, uniform NEE")
+    assert not np.allclose(img, before)
