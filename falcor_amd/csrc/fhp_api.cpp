@@ -798,6 +798,11 @@ int fhp_scene_set_materials(FhpScene s, const float* mats) {
     }
     s->h.mats.assign(mats, mats + size_t(s->h.nM) * M_STRIDE);
     HIP_OR_FAIL(hipMemcpy(s->mats.p, mats, size_t(s->h.nM) * M_STRIDE * 4, hipMemcpyHostToDevice));
+    // Light samples evaluate the emitters' current materials (EmissiveLightSamplerHelpers.slang:79).
+    if (s->h.nLights) {
+        refreshLightEmission(s->h);
+        HIP_OR_FAIL(hipMemcpy(s->lights.p, s->h.lights.data(), s->h.lights.size() * 4, hipMemcpyHostToDevice));
+    }
     return FHP_OK;
 }
 

@@ -604,7 +604,7 @@ struct LightSample { f3 posW, normalW, Le, dir; float pdf, distance; uint32_t li
 // EmissiveLightSamplerHelpers.slang:54-101
 FHP_HD bool sampleTri(const SceneDev& s, f3 posW, uint32_t li, float ux, float uy, LightSample& ls) {
     const float4* L = s.lights + 5 * size_t(li);
-    float4 a = L[0], b = L[1], c = L[2], n = L[3];
+    float4 a = L[0], b = L[1], c = L[2], n = L[3], e = L[4];
     ls.light = li;
     f3 bc = triangleSample(ux, uy);
     ls.posW = mk3(a.x, a.y, a.z) * bc.x + mk3(b.x, b.y, b.z) * bc.y + mk3(c.x, c.y, c.z) * bc.z;
@@ -615,20 +615,16 @@ FHP_HD bool sampleTri(const SceneDev& s, f3 posW, uint32_t li, float ux, float u
     ls.normalW = mk3(n.x, n.y, n.z);
     float cosT = dot(ls.normalW, -ls.dir);
     if (cosT <= 0.f) return false;
-    // MaterialSystem::evalEmissive of the triangle's current material at the sample
-    // (EmissiveLightSamplerHelpers.slang:79): emissive texture at the sample's uv, or the emissive
-    // colour, times emissiveFactor. Material updates after the build reach light samples; the
-    // record's flux (selection) stays as built, as in the reference.
-    const uint32_t tri = f2u(c.w);
-    const float* r = s.mats + size_t(s.triMat[tri]) * M_STRIDE;
-    const int etex = int(r[M_EMIT_TEX]) - 1;
-    if (etex >= 0) {
-        const uint32_t* tv = s.idx + 3 * size_t(tri);
+    // MaterialSystem::evalEmissive at the sample (EmissiveLightSamplerHelpers.slang:79): the
+    // record carries the material's current emission (refreshed on material updates; the flux
+    // used for selection stays as built, as in the reference): Le, or for a textured emitter
+    // emissiveFactor with the texture evaluated here at the sample's uv.
+    ls.Le = mk3(e.x, e.y, e.z);
+    if (e.w > 0.f) {
+        const uint32_t* tv = s.idx + 3 * size_t(f2u(c.w));
         const float4 t0 = s.vtx[2 * size_t(tv[0]) + 1], t1 = s.vtx[2 * size_t(tv[1]) + 1], t2 = s.vtx[2 * size_t(tv[2]) + 1];
         const float u = t0.z * bc.x + t1.z * bc.y + t2.z * bc.z, v = t0.w * bc.x + t1.w * bc.y + t2.w * bc.z;
-        ls.Le = texFetch(s, s.texels, etex, u, v, nullptr) * mk3(r[M_EMIT_FACTOR]);
-    } else {
-        ls.Le = mk3(r[M_EMIT], r[M_EMIT + 1], r[M_EMIT + 2]) * r[M_EMIT_FACTOR];
+        ls.Le = texFetch(s, s.texels, int(e.w) - 1, u, v, nullptr) * ls.Le;
     }
     float denom = fmaxf_(kFltMin, cosT * a.w);
     ls.pdf = d2 / denom;

@@ -395,6 +395,21 @@ f3 averageEmission(const HostScene& s, int tex, const float uv[3][2]) {
 }
 }  // namespace
 
+// Emission words of the light records (16-19) from the current materials: Le, or for a textured
+// emitter emissiveFactor and the texture + 1. Positions, normals, areas and flux are kept.
+void refreshLightEmission(HostScene& s) {
+    for (uint32_t i = 0; i < s.nLights; i++) {
+        float* rec = &s.lights[20 * size_t(i)];
+        uint32_t t;
+        std::memcpy(&t, &rec[11], 4);
+        const float* m = &s.mats[size_t(s.triMat[t]) * M_STRIDE];
+        const int etex = int(m[M_EMIT_TEX]) - 1;
+        const f3 Le = etex >= 0 ? mk3(m[M_EMIT_FACTOR]) : mk3(m[M_EMIT], m[M_EMIT + 1], m[M_EMIT + 2]) * m[M_EMIT_FACTOR];
+        rec[16] = Le.x; rec[17] = Le.y; rec[18] = Le.z;
+        rec[19] = etex >= 0 ? float(etex + 1) : 0.f;
+    }
+}
+
 void buildLights(HostScene& s) {
     s.lights.clear();
     s.activeLights.clear();

@@ -63,6 +63,7 @@ void computeCamera(HostScene& s);
 void computeHitInfo(HostScene& s, bool useCompression);
 void buildBVH4(HostScene& s, uint32_t maxLeaf = 4);
 void buildLights(HostScene& s);
+void refreshLightEmission(HostScene& s);
 void buildLightBVH(HostScene& s);
 
 }  // namespace fhp
