@@ -117,6 +117,7 @@ _SIGS = {
     "fhp_session_stream": (C.c_int, [C.c_void_p, C.c_void_p]),
     "fhp_session_destroy": (C.c_int, [C.c_void_p]),
     "fhp_bvh_build": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "fhp_scene_set_analytic_lights": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
     "fhp_scene_get_materials": (C.c_int, [C.c_void_p, C.c_void_p]),
     "fhp_scene_get_material_params": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
     "fhp_scene_set_material_params": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),

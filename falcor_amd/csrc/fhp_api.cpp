@@ -70,6 +70,7 @@ struct FhpScene_ {
     DevBuf levels;   // BVH refit schedule (node ids by depth)
     DevBuf envTex;   // environment map texels (RGB fp32)
     DevBuf lrefit;   // light BVH refit schedule
+    DevBuf alights;  // analytic lights
     DevBuf restVtx;  // rest-pose vertex records for node transforms (uploaded on first use)
     bool restValid = false;
     uint32_t leafSize = 4;  // BVH4 leaf size (triangles)
@@ -722,6 +723,47 @@ int fhp_scene_lights(FhpScene s, float* h_lights) {
     return FHP_OK;
 }
 
+int fhp_scene_set_analytic_lights(FhpScene s, uint32_t n, const float* h_lights) {
+    if (!s || (n && !h_lights)) return fail(FHP_EINVAL, "null argument");
+    SceneDev& dv = s->dev;
+    if (n == 0) {
+        dv.alights = nullptr;
+        dv.nALights = 0;
+        return FHP_OK;
+    }
+    constexpr float kPi = 3.14159265358979323846f;
+    std::vector<float> rec(16 * size_t(n), 0.f);
+    for (uint32_t i = 0; i < n; i++) {
+        const float* q = h_lights + 16 * size_t(i);
+        for (int k = 0; k < 12; k++)
+            if (!std::isfinite(q[k])) return fail(FHP_EINVAL, "non-finite analytic light " + std::to_string(i));
+        if (!(q[0] == 0.f || q[0] == 1.f))
+            return fail(FHP_EINVAL, "analytic light " + std::to_string(i) + ": type must be 0 (point) or 1 (directional)");
+        const uint32_t type = q[0] == 0.f ? kLightPoint : kLightDirectional;
+        // normalize() in float, one rounding per operation.
+        const float xx = q[8] * q[8], yy = q[9] * q[9], zz = q[10] * q[10];
+        float dd = xx + yy;
+        dd = dd + zz;
+        const float dl = std::sqrt(dd);
+        if (!(dl > 0.f)) return fail(FHP_EINVAL, "analytic light " + std::to_string(i) + ": zero direction");
+        // Light::setWorldDirection normalises; PointLight::setOpeningAngle / setPenumbraAngle clamp
+        // (Light.cpp:174-223) and keep cos(openingAngle).
+        const float opening = std::min(std::max(q[7], 0.f), kPi);
+        const float penumbra = std::min(std::max(q[11], 0.f), opening);
+        float* r = &rec[16 * size_t(i)];
+        uint32_t t = type;
+        std::memcpy(&r[0], &t, 4);
+        r[1] = q[1]; r[2] = q[2]; r[3] = q[3];
+        r[4] = q[4]; r[5] = q[5]; r[6] = q[6]; r[7] = opening;
+        r[8] = q[8] / dl; r[9] = q[9] / dl; r[10] = q[10] / dl; r[11] = std::cos(opening);
+        r[12] = penumbra;
+    }
+    HIP_OR_FAIL(s->alights.upload(rec));
+    dv.alights = s->alights.as<float4>();
+    dv.nALights = n;
+    return FHP_OK;
+}
+
 int fhp_scene_set_env_map(FhpScene s, const float* h_texels, uint32_t W, uint32_t H, const float* h_params, uint32_t flags) {
     if (!s) return fail(FHP_EINVAL, "null argument");
     SceneDev& dv = s->dev;
@@ -1043,6 +1085,9 @@ int fhp_pass_execute(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t spp
         a.lightKind = uint32_t(p->get("emissiveSampler"));
         a.separateShadow = p->get("splitVisibilityLaunch") != 0;
         if (a.nee && !a.computeDirect) return fail(FHP_EINVAL, "useEmissiveNEE requires computeDirect");
+        if (a.nee && s->dev.nALights)
+            return fail(FHP_EINVAL, "useEmissiveNEE cannot be combined with analytic lights (one shadow ray per vertex)");
+        a.shadowRays = (a.nee || s->dev.nALights) ? 1u : 0u;
         a.nPaths = W * H * spp;
         a.nPix = W * H;
         a.thp = p->thp.as<float4>(); a.prevN = p->prevN.as<float4>(); a.rng = p->rng.as<uint4>();
@@ -1173,6 +1218,11 @@ int fhp_pass_execute_batch(FhpPass p, uint32_t nv, const FhpView* views, uint64_
     a.nee = p->get("useEmissiveNEE") != 0;
     a.lightKind = uint32_t(p->get("emissiveSampler"));
     if (a.nee && !a.computeDirect) return fail(FHP_EINVAL, "useEmissiveNEE requires computeDirect");
+    for (FhpScene sc : scenes)
+        if (a.nee && sc->dev.nALights)
+            return fail(FHP_EINVAL, "useEmissiveNEE cannot be combined with analytic lights (one shadow ray per vertex)");
+    a.shadowRays = a.nee ? 1u : 0u;
+    for (FhpScene sc : scenes) a.shadowRays |= sc->dev.nALights ? 1u : 0u;
     a.separateShadow = 0;
     a.nPaths = uint32_t(nPaths);
     a.nPix = uint32_t(nPix);

@@ -36,6 +36,7 @@ struct MptArgs {
     uint32_t W, H, spp, frame, maxBounces;
     uint32_t computeDirect, importance, nee, lightKind;
     uint32_t nPaths;
+    uint32_t shadowRays;      // 1: shading emits visibility rays (emissive NEE or analytic lights)
     uint32_t separateShadow;  // 1: visibility rays in launches of their own (else shared with closest-hit)
                               // 0 also compacts each bounce's hits: hitQ[k] / hitIdx[k], count in counters[kHitCount + depth]
     float4* thp;     // thp.xyz, pdf of the last scatter

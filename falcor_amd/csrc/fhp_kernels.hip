@@ -352,6 +352,7 @@ __global__ void kShadePrimary(SceneDev s0, MptArgs a) {
         if (a.computeDirect) {
             Lp = Lp + m.emission;
             if (a.nee) shadow = neeSample(s, a.lightKind, sd, m, rng, shO, shD, shT, shC);
+            else if (s.nALights) shadow = analyticSample(s, sd, m, rng.u1(), shO, shD, shT, shC);
         }
         f3 thp = mk3(1.f);
         Scatter sc;
@@ -438,6 +439,9 @@ __global__ void kShade(SceneDev s0, MptArgs a, uint32_t depth) {
                     f3 org = rayOriginFromShading(sd);
                     if (a.nee) {
                         shadow = neeSample(s, a.lightKind, sd, m, rng, shO, shD, shT, shC);
+                        if (shadow) shC = thp * shC;
+                    } else if (s.nALights) {
+                        shadow = analyticSample(s, sd, m, rng.u1(), shO, shD, shT, shC);
                         if (shadow) shC = thp * shC;
                     }
                     Scatter sc;
@@ -700,10 +704,10 @@ hipError_t mptBounces(const SceneDev& s, const MptArgs& a, const TraceScratch& t
         // Closest-hit rays of bounce b with the visibility rays produced alongside them.
         if ((e = (a.separateShadow ? traceQ(a.rayQ[b & 1], a.counters + b, 0u, a.hitQ, 0xffffffffu)
                                    : bounce(a.rayQ[b & 1], a.counters + b, 0u, a.hitQ, a.hitIdx, a.counters + kHitCount + b,
-                                            a.nee ? a.counters + kCounterStride + b : nullptr, a.pathQ[b & 1], 0xffffffffu))) !=
+                                            a.shadowRays ? a.counters + kCounterStride + b : nullptr, a.pathQ[b & 1], 0xffffffffu))) !=
              hipSuccess)
             return e;
-        if (a.separateShadow && a.nee && (e = shadow(b)) != hipSuccess) return e;
+        if (a.separateShadow && a.shadowRays && (e = shadow(b)) != hipSuccess) return e;
         tm->begin(KK_SHADE, st);
         hipLaunchKernelGGL(kShade<MULTI>, dim3(shadeBlocks), dim3(T), 0, st, s, a, b);
         tm->end(st);

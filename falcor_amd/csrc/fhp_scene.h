@@ -74,7 +74,12 @@ struct SceneDev {
     const float* envTex;
     uint32_t envW, envH, envFlags;
     f3 envInv0, envInv1, envInv2, envScale;
+    // Analytic lights (LightData.slang:48-66 subset): 4 float4 per light, (type, intensity rgb)
+    // (posW, openingAngle) (dirW, cosOpeningAngle) (penumbraAngle, 0, 0, 0); null when none.
+    const float4* alights;
+    uint32_t nALights;
 };
+constexpr uint32_t kLightPoint = 0u, kLightDirectional = 1u;  // LightType::Point, ::Directional
 constexpr uint32_t kEnvLight = 1u, kEnvBackground = 2u;
 
 // ------------------------------------------------------------------ shading data
@@ -630,6 +635,46 @@ FHP_HD bool sampleTri(const SceneDev& s, f3 posW, uint32_t li, float ux, float u
     ls.pdf = d2 / denom;
     return true;
 }
+// MinimalPathTracer.rt.slang:253-286 (evalDirectAnalytic) with LightHelpers.slang:190-258: one
+// analytic light picked uniformly, its unshadowed contribution mi.eval * Li * lightCount, and the
+// shadow ray that gates it. Returns false when the sample is rejected before the shadow ray.
+FHP_HD float smoothstep01(float e0, float e1, float x) {
+    const float t = saturate((x - e0) / (e1 - e0));
+    return t * t * (3.f - 2.f * t);
+}
+FHP_HD bool analyticSample(const SceneDev& s, const Shading& sd, const Material& m, float u, f3& org, f3& dir, float& dist,
+                           f3& contrib) {
+    const uint32_t n = s.nALights;
+    const uint32_t li = min(uint32_t(u * float(n)), n - 1u);
+    const float4* L = s.alights + 4 * size_t(li);
+    const float4 q0 = L[0], q1 = L[1], q2 = L[2], q3 = L[3];
+    const f3 dirW = mk3(q2.x, q2.y, q2.z);
+    f3 Li;
+    if (f2u(q0.x) == kLightDirectional) {
+        dir = -dirW;  // sampleDirectionalLight (LightHelpers.slang:190-211)
+        dist = kFltMax;
+        Li = mk3(q0.y, q0.z, q0.w);
+    } else {
+        // samplePointLight (LightHelpers.slang:217-258), spot cone and penumbra.
+        const f3 toL = mk3(q1.x, q1.y, q1.z) - sd.posW;
+        const float d2 = fmaxf_(dot(toL, toL), 1e-9f);
+        dist = sqrtf(d2);
+        dir = toL / dist;
+        const float cosT = -dot(dir, dirW);
+        float falloff = 1.f;
+        if (cosT < q2.w) falloff = 0.f;
+        else if (q3.x > 0.f) falloff = smoothstep01(0.f, q3.x, q1.w - acosf(cosT));
+        Li = mk3(q0.y, q0.z, q0.w) * falloff / d2;
+    }
+    // Reject directions outside every lobe (getLobeTypes: a reflection lobe always exists).
+    const bool hasT = m.type == MAT_STANDARD && (m.specTrans > 0.f || (m.diffTrans > 0.f && m.specTrans < 1.f && anyPos(m.diffuse)));
+    const float NdotL = dot(sd.orientedFaceN(), dir);
+    if (NdotL <= kMinCosTheta && !hasT) return false;
+    org = rayOriginOffset(sd.posW, dot(sd.faceN, dir) >= 0.f ? sd.faceN : -sd.faceN);
+    contrib = matEval(sd, m, dir) * Li * float(n);
+    return true;
+}
+
 // EmissiveLightSamplerHelpers.slang:106-124
 FHP_HD float triPdf(const SceneDev& s, f3 posW, uint32_t li, f3 hitPos, f3 hitN) {
     f3 toL = hitPos - posW;

@@ -22,8 +22,9 @@ Objects available to a scene script, after Falcor's bindings:
   focalDistance, nearPlane, farPlane, aspectRatio (Camera.cpp:448-458; fovY from the focal
   length and frame height); sceneBuilder.addNode/addTriangleMesh/addMeshInstance/addCamera.
 EnvMap(path) with intensity, tint and rotation, assigned to sceneBuilder.envMap, becomes the
-scene's environment map. Analytic lights (PointLight, DirectionalLight, ...) are accepted and
-ignored with a warning (this renderer's lights are emissive triangles and the environment map).
+scene's environment map. PointLight (intensity, position, direction, openingAngle,
+penumbraAngle) and DirectionalLight (intensity, direction) passed to sceneBuilder.addLight become
+analytic lights; other light types are accepted and ignored with a warning.
 """
 from __future__ import annotations
 
@@ -243,6 +244,41 @@ class _TriangleMeshAPI:
         return load_obj(self.base / str(path), smooth_normals=bool(smoothNormals))
 
 
+class _AnalyticLight:
+    """PointLight / DirectionalLight with the reference's properties (Light.cpp:408-428: intensity,
+    position, direction, openingAngle, penumbraAngle; LightData.slang defaults)."""
+
+    def __init__(self, name=""):
+        self.name = str(name)
+        self.intensity = Vec(1, 1, 1)
+        self.direction = Vec(0, -1, 0)
+        self.active = True
+
+    def _v(self, x):
+        return tuple(x.v) if isinstance(x, Vec) else (float(x),) * 3
+
+
+class PointLight(_AnalyticLight):
+    kind = "PointLight"
+
+    def __init__(self, name=""):
+        super().__init__(name)
+        self.position = Vec(0, 0, 0)
+        self.openingAngle = math.pi
+        self.penumbraAngle = 0.0
+
+    def record(self):
+        return S.point_light(self._v(self.intensity), self._v(self.position), self._v(self.direction),
+                             float(self.openingAngle), float(self.penumbraAngle))
+
+
+class DirectionalLight(_AnalyticLight):
+    kind = "DirectionalLight"
+
+    def record(self):
+        return S.directional_light(self._v(self.intensity), self._v(self.direction))
+
+
 class _SceneBuilderAPI:
     def __init__(self, sb: S.SceneBuilder, base: Path, warnings: list):
         self.sb = sb
@@ -291,7 +327,9 @@ class _SceneBuilderAPI:
         return len(self.cameras) - 1
 
     def addLight(self, light):
-        self.warnings.append(f"{getattr(light, 'kind', 'light')} ignored (emissive triangles are the only lights)")
+        if isinstance(light, _AnalyticLight):
+            return self.sb.add_light(light.record())
+        self.warnings.append(f"{getattr(light, 'kind', 'light')} ignored (point and directional lights are supported)")
         return 0
 
     def __setattr__(self, k, v):
@@ -430,7 +468,7 @@ class _Interp:
             obj = self.expr(target.value)
             if isinstance(obj, _SceneBuilderAPI):
                 ok = target.attr == "envMap"
-            elif isinstance(obj, (_Material, _Camera, _Transform, _EnvMap)):
+            elif isinstance(obj, (_Material, _Camera, _Transform, _EnvMap, _AnalyticLight)):
                 # Only the existing data attributes of these bindings.
                 ok = hasattr(obj, target.attr) and not callable(getattr(obj, target.attr))
             else:
@@ -507,7 +545,9 @@ def load_pyscene(path, aspect: float = 1.0, warnings: list | None = None) -> S.S
         "True": True, "False": False, "None": None,
     }
     env["EnvMap"] = _EnvMap
-    for kind in ("PointLight", "DirectionalLight", "DistantLight", "RectLight", "DiscLight", "SphereLight",
+    env["PointLight"] = PointLight
+    env["DirectionalLight"] = DirectionalLight
+    for kind in ("DistantLight", "RectLight", "DiscLight", "SphereLight",
                  "GridVolume", "Grid", "SDFGrid", "SDFSVS", "SDFSBS", "SDFSVO", "NDSDFGrid"):
         env[kind] = (lambda k: (lambda *a, **kw: _Ignored(k, *a, **kw)))(kind)
     _Interp(env).run(tree.body)

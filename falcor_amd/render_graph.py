@@ -90,15 +90,17 @@ class Scene:
         _lib.check(L.fhp_scene_create(C.byref(d), C.byref(h)))
         self.handle = h
         self.materials = a["mats"].copy()
+        # Incremented by every parameter change (materials, textures, vertices, lights, sampling
+        # freeze); AccumulatePass restarts when it moves (the scene updates of AccumulatePass.cpp:141-147).
+        self.version = 0
+        if getattr(data, "analytic_lights", None):
+            self.set_analytic_lights(data.analytic_lights)
         if data.env_map is not None:
             e = data.env_map
             self.set_env_map(e["texels"], e.get("intensity", 1.0), e.get("tint", (1, 1, 1)), e.get("rotation", (0, 0, 0)))
         nm = a["moff"].shape[0] - 1
         xf = list(getattr(data, "mesh_transforms", None) or [])
         self._rest_xf = [np.asarray(m, np.float64) for m in xf] if len(xf) == nm else [np.eye(4) for _ in range(nm)]
-        # Incremented by every parameter change (materials, textures, vertices, sampling freeze);
-        # AccumulatePass restarts when it moves (the scene updates of AccumulatePass.cpp:141-147).
-        self.version = 0
 
     def close(self):
         if getattr(self, "handle", None):
@@ -242,6 +244,14 @@ class Scene:
         _lib.check(L.fhp_scene_get_mesh_vertices(self.handle, mesh_id, p.ctypes.data, n.ctypes.data))
         self._arrays["pos"][v0:v1] = p
         self._arrays["nrm"][v0:v1] = n
+        self.version += 1
+
+    def set_analytic_lights(self, records):
+        """Analytic point / directional lights (Scene::addLight): records of scene.point_light /
+        scene.directional_light, [n, 16]; an empty list removes them."""
+        r = np.ascontiguousarray(np.asarray(records, np.float32).reshape(-1, 16))
+        _lib.check(_lib.load().fhp_scene_set_analytic_lights(self.handle, r.shape[0], r.ctypes.data if r.size else None))
+        self.analytic_lights = r.copy()
         self.version += 1
 
     def set_env_map(self, texels=None, intensity: float = 1.0, tint=(1.0, 1.0, 1.0), rotation=(0.0, 0.0, 0.0),

@@ -234,6 +234,32 @@ def pbrt_diffuse_material(albedo=(0.5, 0.5, 0.5), double_sided=False, texture=-1
     return m
 
 
+# --------------------------------------------------------------------------- analytic lights
+LIGHT_POINT, LIGHT_DIRECTIONAL = 0, 1
+
+
+def point_light(intensity=(1.0, 1.0, 1.0), position=(0.0, 0.0, 0.0), direction=(0.0, -1.0, 0.0),
+                opening_angle=math.pi, penumbra_angle=0.0) -> np.ndarray:
+    """PointLight record (LightData.slang:48-66 defaults; a spot light when opening_angle < pi)."""
+    r = np.zeros(16, np.float32)
+    r[0] = LIGHT_POINT
+    r[1:4] = intensity
+    r[4:7] = position
+    r[7] = opening_angle
+    r[8:11] = direction
+    r[11] = penumbra_angle
+    return r
+
+
+def directional_light(intensity=(1.0, 1.0, 1.0), direction=(0.0, -1.0, 0.0)) -> np.ndarray:
+    """DirectionalLight record: irradiance `intensity` arriving along `direction`."""
+    r = np.zeros(16, np.float32)
+    r[0] = LIGHT_DIRECTIONAL
+    r[1:4] = intensity
+    r[8:11] = direction
+    return r
+
+
 # --------------------------------------------------------------------------- scene data
 @dataclass
 class Camera:
@@ -269,6 +295,8 @@ class SceneData:
     env_map: dict | None = None
     # Local-to-world matrix (4x4) each mesh was placed with (identity when built from world data).
     mesh_transforms: list = field(default_factory=list)
+    # Analytic lights: records of point_light() / directional_light() (16 floats each).
+    analytic_lights: list = field(default_factory=list)
 
     @property
     def num_triangles(self) -> int:
@@ -289,11 +317,17 @@ class SceneBuilder:
         self._nv = 0
         self._mesh_off = [0]
         self._xf: list[np.ndarray] = []
+        self.lights: list[np.ndarray] = []
         self.camera = Camera()
 
     def add_material(self, rec: np.ndarray) -> int:
         self.materials.append(np.asarray(rec, np.float32))
         return len(self.materials) - 1
+
+    def add_light(self, rec: np.ndarray) -> int:
+        """Adds an analytic light record (point_light / directional_light)."""
+        self.lights.append(np.asarray(rec, np.float32).copy())
+        return len(self.lights) - 1
 
     def add_texture(self, rgb: np.ndarray) -> int:
         rgb = np.ascontiguousarray(rgb, np.float32)
@@ -337,7 +371,8 @@ class SceneBuilder:
             indices=np.concatenate(self._idx), tri_material=np.concatenate(self._mat),
             tri_mesh=np.concatenate(self._mesh), materials=np.stack(self.materials), camera=self.camera,
             tex_desc=tex_desc, texels=texels.astype(np.float32), mesh_names=list(self.mesh_names),
-            mesh_vertex_offsets=np.asarray(self._mesh_off, np.uint32), mesh_transforms=[m.copy() for m in self._xf])
+            mesh_vertex_offsets=np.asarray(self._mesh_off, np.uint32), mesh_transforms=[m.copy() for m in self._xf],
+            analytic_lights=[r.copy() for r in self.lights])
 
 
 # --------------------------------------------------------------------------- benchmark scenes

@@ -168,6 +168,13 @@ int fhp_scene_freeze_sampling(FhpScene scene, int enable, FhpStream stream);
  * misses (:462-465), i.e. Scene::useEnvLight / useEnvBackground. width = 0 removes the map. */
 int fhp_scene_set_env_map(FhpScene scene, const float* h_texels, uint32_t width, uint32_t height, const float* h_params,
                           uint32_t flags);
+/* Analytic lights (Scene::addLight with PointLight / DirectionalLight, Source/Falcor/Scene/Lights/
+ * Light.cpp), n records of 16 floats: [type (0 point, 1 directional), intensity rgb, position xyz,
+ * openingAngle, direction xyz, penumbraAngle, 0, 0, 0, 0]. Directions are normalised, the opening
+ * angle clamped to [0, pi] and the penumbra to [0, opening] (Light.cpp:174-223). MinimalPathTracer
+ * samples one of them uniformly at every vertex with a shadow ray (evalDirectAnalytic,
+ * MinimalPathTracer.rt.slang:253-286; LightHelpers.slang:190-258). n = 0 removes them. */
+int fhp_scene_set_analytic_lights(FhpScene scene, uint32_t n, const float* h_lights);
 /* Light collection export (for checks): num_lights records of 20 floats [p0 xyz, area, p1 xyz,
  * flux, p2 xyz, triangle id (bits), normal xyz, 0, Le rgb (a textured emitter: emissiveFactor),
  * emissive texture + 1 (0: constant)]. The flux of a textured emitter comes from the

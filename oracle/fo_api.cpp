@@ -186,6 +186,24 @@ void fo_scene_animate(void* h, const float* pos, const float* nrm) {
     updateLightPositions(*s);
     refitLightBVH(*s);
 }
+// Analytic lights: records [type, intensity rgb, position, openingAngle, direction,
+// penumbraAngle, ...] (16 floats); Light::setWorldDirection normalises, PointLight's angle
+// setters clamp (Light.cpp:174-223).
+void fo_scene_set_analytic_lights(void* h, uint32_t n, const float* recs) {
+    Scene* s = static_cast<Scene*>(h);
+    s->aLights.assign(16 * size_t(n), 0.f);
+    for (uint32_t i = 0; i < n; i++) {
+        const float* q = recs + 16 * size_t(i);
+        float* r = &s->aLights[16 * size_t(i)];
+        const float pi = Consts<float>::pi;
+        float opening = std::min(std::max(q[7], 0.f), pi);
+        float penumbra = std::min(std::max(q[11], 0.f), opening);
+        V3f d = normalize(V3f(q[8], q[9], q[10]));
+        r[0] = q[0]; r[1] = q[1]; r[2] = q[2]; r[3] = q[3];
+        r[4] = q[4]; r[5] = q[5]; r[6] = q[6]; r[7] = opening;
+        r[8] = d.x; r[9] = d.y; r[10] = d.z; r[11] = std::cos(opening); r[12] = penumbra;
+    }
+}
 void fo_scene_set_materials(void* h, const float* mats) {
     Scene* s = static_cast<Scene*>(h);
     s->mats.assign(mats, mats + size_t(s->nM) * M_STRIDE);

@@ -87,6 +87,10 @@ struct Scene {
     std::vector<LNode> lnodes;
     std::vector<uint32_t> lTriIdx;    // leaf triangle lists (indices into `lights`)
     std::vector<uint64_t> lBitmask;   // per light: traversal path bitmask
+    // Analytic lights (LightData.slang:48-66 subset), 16 floats each after the setters' clamps:
+    // [type (0 point, 1 directional), intensity rgb, posW, openingAngle, dirW, cosOpeningAngle,
+    // penumbraAngle, 0, 0, 0].
+    std::vector<float> aLights;
 
     void triVerts(uint32_t t, V3f v[3]) const {
         v[0] = pos[idx[3 * t]]; v[1] = pos[idx[3 * t + 1]]; v[2] = pos[idx[3 * t + 2]];
@@ -561,6 +565,48 @@ inline V3f neeEmissive(const Scene& s, const MptParams& P, const ShadingData& sd
     return contrib;
 }
 
+// MinimalPathTracer.rt.slang:253-286 (evalDirectAnalytic) with samplePointLight /
+// sampleDirectionalLight (LightHelpers.slang:190-258).
+inline V3f evalDirectAnalytic(const Scene& s, const ShadingData& sd, MatInst& mi, SampleGen& sg, Tracer& tr) {
+    const uint32_t n = uint32_t(s.aLights.size() / 16);
+    if (n == 0) return V3f(0.f);
+    const uint32_t li = std::min(uint32_t(sg.next1D() * float(n)), n - 1);
+    const float invPdf = float(n);
+    const float* L = &s.aLights[16 * size_t(li)];
+    const V3f intensity(L[1], L[2], L[3]), dirW(L[8], L[9], L[10]);
+    V3f dir, Li;
+    float distance;
+    if (L[0] == 1.f) {
+        dir = -dirW;
+        distance = std::numeric_limits<float>::max();
+        Li = intensity;
+    } else {
+        V3f toLight = V3f(L[4], L[5], L[6]) - sd.posW;
+        float distSqr = std::max(dot(toLight, toLight), 1e-9f);  // kMinLightDistSqr
+        distance = std::sqrt(distSqr);
+        dir = toLight / distance;
+        float cosTheta = -dot(dir, dirW);
+        float falloff = 1.f;
+        if (cosTheta < L[11]) {
+            falloff = 0.f;
+        } else if (L[12] > 0.f) {
+            float delta = L[7] - std::acos(cosTheta);
+            float t = std::min(std::max((delta - 0.f) / (L[12] - 0.f), 0.f), 1.f);  // smoothstep(0, penumbra, delta)
+            falloff = t * t * (3.f - 2.f * t);
+        }
+        Li = intensity * falloff / distSqr;
+    }
+    // getLobeTypes: a reflection lobe always exists; transmission as StandardBSDF.slang:205-225.
+    const BsdfData<float>& q = mi.d;
+    const bool anyDiffuse = q.diffuse.x > 0.f || q.diffuse.y > 0.f || q.diffuse.z > 0.f;
+    const bool hasTransmission = q.type == MAT_STANDARD && (q.specTrans > 0.f || (anyDiffuse && q.specTrans < 1.f && q.diffTrans > 0.f));
+    const float NdotL = dot(orientedFaceN(sd), dir);
+    if (NdotL <= kMinCosTheta && !hasTransmission) return V3f(0.f);
+    const V3f origin = computeRayOrigin(sd.posW, dot(sd.faceN, dir) >= 0.f ? sd.faceN : -sd.faceN);
+    if (!tr.visible(origin, dir, distance)) return V3f(0.f);
+    return miEval(sd, mi, dir) * Li * invPdf;
+}
+
 // MinimalPathTracer.rt.slang:392-468 (one path for one pixel and sample number).
 inline V3f mptTracePath(const Scene& s, const MptParams& P, uint32_t px, uint32_t py, uint32_t W, uint32_t H,
                         uint32_t frameCount, Tracer& tr) {
@@ -576,6 +622,7 @@ inline V3f mptTracePath(const Scene& s, const MptParams& P, uint32_t px, uint32_
     if (P.computeDirect) {
         outColor = outColor + mi.emission;
         if (P.useEmissiveNEE) outColor = outColor + neeEmissive(s, P, sd, mi, sg, tr);
+        else if (!s.aLights.empty()) outColor = outColor + evalDirectAnalytic(s, sd, mi, sg, tr);
     }
     // ScatterRayData
     V3f radiance(0.f), thp(1.f), rOrigin, rDir;
@@ -613,6 +660,9 @@ inline V3f mptTracePath(const Scene& s, const MptParams& P, uint32_t px, uint32_
         V3f org = sdRayOrigin(sdh);
         if (P.useEmissiveNEE) {
             V3f Lr = neeEmissive(s, P, sdh, mih, sg, tr);
+            radiance = radiance + thp * Lr;
+        } else if (!s.aLights.empty()) {
+            V3f Lr = evalDirectAnalytic(s, sdh, mih, sg, tr);
             radiance = radiance + thp * Lr;
         }
         if (!scatter(sdh, mih, org)) { terminated = true; break; }

@@ -46,6 +46,7 @@ def oracle_lib():
             "fo_bsdf_sample": (None, [i32, vp, vp, vp, vp]),
             "fo_fresnel_dielectric": (None, [i32, vp, vp, vp]),
             "fo_scene_animate": (None, [vp, vp, vp]),
+            "fo_scene_set_analytic_lights": (None, [vp, u32, vp]),
             "fo_render_mpt_f64": (None, [vp, u32, u32, u32, u32, u32, u32, i32, vp]),
             "fo_scene_create": (vp, [u32, vp, vp, vp, u32, vp, vp, u32, vp, u32, vp, u64, vp, vp]),
             "fo_scene_destroy": (None, [vp]),
@@ -94,11 +95,18 @@ class OracleScene:
         self.h = L.fo_scene_create(p.shape[0], _p(p), _p(n), _p(uv), idx.shape[0], _p(idx), _p(tm), mats.shape[0],
                                    _p(mats), td.shape[0], _p(td) if td.size else None, tx.size, _p(tx) if tx.size else None,
                                    _p(cam))
+        if getattr(data, "analytic_lights", None):
+            self.set_analytic_lights(data.analytic_lights)
 
     def __del__(self):
         if getattr(self, "h", None):
             oracle_lib().fo_scene_destroy(self.h)
             self.h = None
+
+    def set_analytic_lights(self, records):
+        r = np.ascontiguousarray(np.asarray(records, np.float32).reshape(-1, 16))
+        self._alights = r
+        oracle_lib().fo_scene_set_analytic_lights(self.h, r.shape[0], _p(r) if r.size else None)
 
     def animate(self, positions: np.ndarray, normals: np.ndarray | None = None):
         """All vertices move (3 floats each); emissive triangles are updated and the light BVH refitted."""

@@ -90,7 +90,7 @@ def test_scene_matches_builder_calls(tmp_path):
     warns = []
     d = load_pyscene(_write(tmp_path), aspect=1.5, warnings=warns)
     assert d.mesh_names == ["Floor", "Box", "Box", "Lamp", "Ball", "Prism"]
-    assert any("PointLight" in w for w in warns)
+    assert len(d.analytic_lights) == 1 and d.analytic_lights[0][0] == S.LIGHT_POINT
     # The same scene through SceneBuilder directly.
     sb = S.SceneBuilder()
     quad, cube = S.TriangleMesh.create_quad(), S.TriangleMesh.create_cube((2, 1, 1))
@@ -173,7 +173,7 @@ def test_reference_cornell_box_if_present():
     ref = S.closed_box(0)
     assert d.mesh_names == ref.mesh_names
     np.testing.assert_allclose(d.positions, ref.positions, atol=1e-6)
-    assert any("PointLight" in w for w in warns)
+    assert len(d.analytic_lights) == 1  # the scene's PointLight
 
 
 def test_environment_map(tmp_path):
