@@ -111,6 +111,9 @@ _SIGS = {
     "fhp_pass_set_input": (C.c_int, [C.c_void_p, C.c_char_p, C.c_void_p]),
     "fhp_scene_lights": (C.c_int, [C.c_void_p, C.c_void_p]),
     "fhp_scene_set_env_map": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_uint32]),
+    "fhp_scene_get_env_importance": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "fhp_env_sample": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
+    "fhp_env_eval_pdf": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
     "fhp_scene_update_vertices_device": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
     "fhp_scene_update_node_transform": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
     "fhp_session_create": (C.c_int, [C.c_int, C.c_void_p]),

@@ -16,7 +16,7 @@ PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 OUT = PKG / "libfhp.so"
 BUILD = PKG / "_build"
-SOURCES = ["fhp_kernels.hip", "fhp_war.hip", "fhp_war_geom.hip", "fhp_image.hip", "fhp_refit.hip", "fhp_api.cpp", "scene_build.cpp"]
+SOURCES = ["fhp_kernels.hip", "fhp_war.hip", "fhp_war_geom.hip", "fhp_image.hip", "fhp_refit.hip", "fhp_envmap.hip", "fhp_api.cpp", "scene_build.cpp"]
 
 # -ffp-contract=off: no implicit FMA, so FP32 results follow the written operation order.
 # Packed FP32 instructions are disabled for the device code (kept out of generated code).

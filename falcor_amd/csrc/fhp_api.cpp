@@ -69,6 +69,7 @@ struct FhpScene_ {
     DevBuf meshTri;  // first triangle of each mesh (PackedHitInfo instance ids)
     DevBuf levels;   // BVH refit schedule (node ids by depth)
     DevBuf envTex;   // environment map texels (RGB fp32)
+    DevBuf envImp;   // env map importance levels (EnvMapSampler)
     DevBuf lrefit;   // light BVH refit schedule
     DevBuf alights;  // analytic lights
     DevBuf restVtx;  // rest-pose vertex records for node transforms (uploaded on first use)
@@ -213,9 +214,10 @@ struct FhpGrads_ {
 namespace {
 // Property tables: names and defaults of the reference passes.
 const std::map<std::string, double>& mptDefaults() {
-    // MinimalPathTracer.h:79-83; useEmissiveNEE/emissiveSampler/countTraversal are extensions.
+    // MinimalPathTracer.h:79-83; useEmissiveNEE/useEnvNEE/emissiveSampler/countTraversal are extensions.
     static const std::map<std::string, double> d = {{"maxBounces", 3}, {"computeDirect", 1}, {"useImportanceSampling", 1},
-                                                   {"useEmissiveNEE", 0}, {"emissiveSampler", 1}, {"countTraversal", 0},
+                                                   {"useEmissiveNEE", 0}, {"useEnvNEE", 0}, {"emissiveSampler", 1},
+                                                   {"countTraversal", 0},
                                                    {"timeKernels", 0}, {"useGraph", 1}, {"traceBlockSize", kDefaultBlock},
                                                    {"traceStackDepth", kDefaultLds}, {"traceBlocksPerCU", kDefaultBlocksPerCU},
                                                    {"splitVisibilityLaunch", 0}};
@@ -769,7 +771,8 @@ int fhp_scene_set_env_map(FhpScene s, const float* h_texels, uint32_t W, uint32_
     SceneDev& dv = s->dev;
     if (W == 0 || H == 0) {
         dv.envTex = nullptr;
-        dv.envW = dv.envH = dv.envFlags = 0;
+        dv.envImp = nullptr;
+        dv.envW = dv.envH = dv.envFlags = dv.envImpDim = 0;
         return FHP_OK;
     }
     if (!h_texels || !h_params) return fail(FHP_EINVAL, "null argument");
@@ -796,6 +799,42 @@ int fhp_scene_set_env_map(FhpScene s, const float* h_texels, uint32_t W, uint32_
     dv.envW = W;
     dv.envH = H;
     dv.envFlags = flags;
+    // EnvMapSampler (EnvMapSampler.cpp:39-65): 512 x 512 importance map, 64 samples per texel.
+    constexpr uint32_t kImpDim = 512, kImpSamplesPerAxis = 8;
+    const size_t impN = (4 * size_t(kImpDim) * kImpDim - 1) / 3;
+    HIP_OR_FAIL(s->envImp.alloc(impN * 4));
+    HIP_OR_FAIL(launchEnvImportance(dv, s->envImp.as<float>(), kImpDim, kImpSamplesPerAxis, nullptr));
+    float mean = 0.f;
+    HIP_OR_FAIL(hipMemcpy(&mean, s->envImp.p, 4, hipMemcpyDeviceToHost));
+    // A map whose luminance integrates to zero cannot be importance sampled (the descent would
+    // divide by zero): env NEE is then off for this scene.
+    const bool sampleable = mean > 0.f && std::isfinite(mean);
+    dv.envImp = sampleable ? s->envImp.as<float>() : nullptr;
+    dv.envImpDim = sampleable ? kImpDim : 0u;
+    return FHP_OK;
+}
+
+int fhp_scene_get_env_importance(FhpScene s, uint32_t* dim, float* h_levels) {
+    if (!s || !dim) return fail(FHP_EINVAL, "null argument");
+    *dim = s->dev.envImpDim;
+    if (h_levels && s->dev.envImp) {
+        const size_t n = (4 * size_t(*dim) * *dim - 1) / 3;
+        HIP_OR_FAIL(hipMemcpy(h_levels, s->dev.envImp, n * 4, hipMemcpyDeviceToHost));
+    }
+    return FHP_OK;
+}
+
+int fhp_env_sample(FhpScene s, const float* d_u, uint32_t n, float* d_out, FhpStream stream) {
+    if (!s || (n && (!d_u || !d_out))) return fail(FHP_EINVAL, "null argument");
+    if (!s->dev.envImp) return fail(FHP_ESTATE, "the scene has no env map that can be importance sampled");
+    if (n) HIP_OR_FAIL(launchEnvSampleBatch(s->dev, d_u, n, d_out, hipStream_t(stream)));
+    return FHP_OK;
+}
+
+int fhp_env_eval_pdf(FhpScene s, const float* d_dirs, uint32_t n, float* d_pdf, FhpStream stream) {
+    if (!s || (n && (!d_dirs || !d_pdf))) return fail(FHP_EINVAL, "null argument");
+    if (!s->dev.envImp) return fail(FHP_ESTATE, "the scene has no env map that can be importance sampled");
+    if (n) HIP_OR_FAIL(launchEnvPdfBatch(s->dev, d_dirs, n, d_pdf, hipStream_t(stream)));
     return FHP_OK;
 }
 
@@ -1084,10 +1123,10 @@ int fhp_pass_execute(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t spp
         a.nee = p->get("useEmissiveNEE") != 0;
         a.lightKind = uint32_t(p->get("emissiveSampler"));
         a.separateShadow = p->get("splitVisibilityLaunch") != 0;
+        a.envNee = p->get("useEnvNEE") != 0;
         if (a.nee && !a.computeDirect) return fail(FHP_EINVAL, "useEmissiveNEE requires computeDirect");
-        if (a.nee && s->dev.nALights)
-            return fail(FHP_EINVAL, "useEmissiveNEE cannot be combined with analytic lights (one shadow ray per vertex)");
-        a.shadowRays = (a.nee || s->dev.nALights) ? 1u : 0u;
+        if (a.envNee && !a.computeDirect) return fail(FHP_EINVAL, "useEnvNEE requires computeDirect");
+        a.shadowRays = (a.nee || s->dev.nALights || (a.envNee && s->dev.envImp && (s->dev.envFlags & kEnvLight))) ? 1u : 0u;
         a.nPaths = W * H * spp;
         a.nPix = W * H;
         a.thp = p->thp.as<float4>(); a.prevN = p->prevN.as<float4>(); a.rng = p->rng.as<uint4>();
@@ -1217,12 +1256,12 @@ int fhp_pass_execute_batch(FhpPass p, uint32_t nv, const FhpView* views, uint64_
     a.importance = p->get("useImportanceSampling") != 0;
     a.nee = p->get("useEmissiveNEE") != 0;
     a.lightKind = uint32_t(p->get("emissiveSampler"));
+    a.envNee = p->get("useEnvNEE") != 0;
     if (a.nee && !a.computeDirect) return fail(FHP_EINVAL, "useEmissiveNEE requires computeDirect");
-    for (FhpScene sc : scenes)
-        if (a.nee && sc->dev.nALights)
-            return fail(FHP_EINVAL, "useEmissiveNEE cannot be combined with analytic lights (one shadow ray per vertex)");
+    if (a.envNee && !a.computeDirect) return fail(FHP_EINVAL, "useEnvNEE requires computeDirect");
     a.shadowRays = a.nee ? 1u : 0u;
-    for (FhpScene sc : scenes) a.shadowRays |= sc->dev.nALights ? 1u : 0u;
+    for (FhpScene sc : scenes)
+        a.shadowRays |= (sc->dev.nALights || (a.envNee && sc->dev.envImp && (sc->dev.envFlags & kEnvLight))) ? 1u : 0u;
     a.separateShadow = 0;
     a.nPaths = uint32_t(nPaths);
     a.nPix = uint32_t(nPix);

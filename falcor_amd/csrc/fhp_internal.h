@@ -36,7 +36,8 @@ struct MptArgs {
     uint32_t W, H, spp, frame, maxBounces;
     uint32_t computeDirect, importance, nee, lightKind;
     uint32_t nPaths;
-    uint32_t shadowRays;      // 1: shading emits visibility rays (emissive NEE or analytic lights)
+    uint32_t envNee;          // 1: env map next-event estimation (useEnvNEE)
+    uint32_t shadowRays;      // 1: shading emits visibility rays (NEE of any light type)
     uint32_t separateShadow;  // 1: visibility rays in launches of their own (else shared with closest-hit)
                               // 0 also compacts each bounce's hits: hitQ[k] / hitIdx[k], count in counters[kHitCount + depth]
     float4* thp;     // thp.xyz, pdf of the last scatter
@@ -213,6 +214,11 @@ struct LightRefitArgs {
     uint32_t depth;
 };
 hipError_t launchLightRefit(const LightRefitArgs& a, hipStream_t stream);
+// Env map importance map (fhp_envmap.hip): dim x dim finest level (power of two) with
+// samplesPerAxis^2 luminance samples per texel, all levels into out ((4 dim^2 - 1) / 3 floats).
+hipError_t launchEnvImportance(const SceneDev& s, float* out, uint32_t dim, uint32_t samplesPerAxis, hipStream_t st);
+hipError_t launchEnvSampleBatch(const SceneDev& s, const float* u, uint32_t n, float* out, hipStream_t st);
+hipError_t launchEnvPdfBatch(const SceneDev& s, const float* dirs, uint32_t n, float* pdf, hipStream_t st);
 
 // AccumulatePass (Accumulate.cs.slang, AccumulatePass.cpp). Precision modes use the values of
 // AccumulatePass::Precision (AccumulatePass.h:70-75); input formats those of _INPUT_FORMAT

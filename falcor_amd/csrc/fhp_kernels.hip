@@ -288,13 +288,16 @@ __global__ __launch_bounds__(B) void kTraceQueue(SceneDev s, const float4* __res
 // Emissive next-event estimation at a shading point (extension of MinimalPathTracer; semantics of
 // PTUtils.slang:177-249 without reparameterization). Produces an unoccluded contribution and the
 // shadow ray that gates it.
+// `sel` is the probability of having chosen emissive lights among the light types.
 __device__ bool neeSample(const SceneDev& s, uint32_t kind, const Shading& sd, const Material& m, Rng& rng, f3& org, f3& dir,
-                          float& dist, f3& contrib) {
+                          float& dist, f3& contrib, float sel) {
     LightSample ls;
     if (!sampleLight(s, int(kind), sd.posW, sd.N, rng, ls)) return false;
     if (!(ls.pdf > 0.f)) return false;
     f3 Li = ls.Le / ls.pdf;
     if (!anyPos(Li)) return false;
+    ls.pdf *= sel;
+    Li = Li / sel;
     org = rayOriginOffset(sd.posW, dot(sd.faceN, ls.dir) >= 0.f ? sd.faceN : -sd.faceN);
     f3 lightPos = rayOriginOffset(ls.posW, ls.normalW);
     f3 toL = lightPos - org;
@@ -304,6 +307,72 @@ __device__ bool neeSample(const SceneDev& s, uint32_t kind, const Shading& sd, c
     if (!anyPos(f)) return false;
     float mis = misPower(ls.pdf, matPdf(sd, m, dir));
     contrib = f * Li * mis;
+    return true;
+}
+
+// Env map next-event estimation (extension of MinimalPathTracer; PathTracer.slang generateEnvMapSample
+// with the NEE contribution and MIS of PathTracer.slang:695-735): one EnvMapSampler sample, a
+// visibility ray to infinity from the side of the surface the direction leaves.
+__device__ bool envNeeSample(const SceneDev& s, const Shading& sd, const Material& m, Rng& rng, f3& org, f3& dir, float& dist,
+                             f3& contrib, float sel) {
+    const float ux = rng.u1(), uy = rng.u1();
+    f3 Le;
+    float pdf;
+    if (!envSample(s, ux, uy, dir, pdf, Le)) return false;
+    f3 Li = pdf > 0.f ? Le / pdf : mk3(0.f);
+    if (!anyPos(Li)) return false;
+    pdf *= sel;
+    Li = Li / sel;
+    org = rayOriginOffset(sd.posW, dot(sd.faceN, dir) >= 0.f ? sd.faceN : -sd.faceN);
+    dist = kFltMax;
+    const f3 f = matEval(sd, m, dir);
+    if (!anyPos(f)) return false;
+    contrib = f * Li * misPower(pdf, matPdf(sd, m, dir));
+    return true;
+}
+
+// Light types of next-event estimation at a vertex: env map (useEnvNEE and a sampleable env
+// light), emissive triangles (useEmissiveNEE), analytic lights (any in the scene), each chosen with
+// equal probability (PathTracer.slang:643-663).
+struct LightTypes {
+    float pEnv, pEmissive, pAnalytic;
+    uint32_t count;
+};
+__device__ __forceinline__ LightTypes lightTypes(const SceneDev& s, const MptArgs& a) {
+    const bool env = a.envNee && s.envImp && (s.envFlags & kEnvLight);
+    const bool em = a.nee != 0u, an = s.nALights != 0u;
+    LightTypes t;
+    t.count = uint32_t(env) + uint32_t(em) + uint32_t(an);
+    const float inv = t.count ? 1.f / float(t.count) : 0.f;
+    t.pEnv = env ? inv : 0.f;
+    t.pEmissive = em ? inv : 0.f;
+    t.pAnalytic = an ? inv : 0.f;
+    return t;
+}
+
+// Direct light at a vertex with one visibility ray: selectLightType (PathTracer.slang:671-693),
+// then the chosen type's sample. A number is drawn for the selection only when more than one
+// type is enabled, so one type alone keeps its own sample sequence (analytic lights alone are
+// MinimalPathTracer's evalDirectAnalytic, MinimalPathTracer.rt.slang:253-286).
+__device__ bool directSample(const SceneDev& s, const MptArgs& a, const Shading& sd, const Material& m, Rng& rng, f3& org,
+                             f3& dir, float& dist, f3& contrib) {
+    const LightTypes t = lightTypes(s, a);
+    if (t.count == 0u) return false;
+    uint32_t type;
+    float sel = 1.f;
+    if (t.count == 1u) {
+        type = t.pEnv > 0.f ? 0u : (t.pEmissive > 0.f ? 1u : 2u);
+    } else {
+        float u = rng.u1();
+        if (u < t.pEnv) { type = 0u; sel = t.pEnv; }
+        else if ((u -= t.pEnv) < t.pEmissive) { type = 1u; sel = t.pEmissive; }
+        else if ((u -= t.pEmissive) < t.pAnalytic) { type = 2u; sel = t.pAnalytic; }
+        else return false;
+    }
+    if (type == 0u) return envNeeSample(s, sd, m, rng, org, dir, dist, contrib, sel);
+    if (type == 1u) return neeSample(s, a.lightKind, sd, m, rng, org, dir, dist, contrib, sel);
+    if (!analyticSample(s, sd, m, rng.u1(), org, dir, dist, contrib)) return false;
+    contrib = contrib / sel;
     return true;
 }
 
@@ -351,8 +420,7 @@ __global__ void kShadePrimary(SceneDev s0, MptArgs a) {
         f3 Lp = mk3(0.f);
         if (a.computeDirect) {
             Lp = Lp + m.emission;
-            if (a.nee) shadow = neeSample(s, a.lightKind, sd, m, rng, shO, shD, shT, shC);
-            else if (s.nALights) shadow = analyticSample(s, sd, m, rng.u1(), shO, shD, shT, shC);
+            shadow = directSample(s, a, sd, m, rng, shO, shD, shT, shC);
         }
         f3 thp = mk3(1.f);
         Scatter sc;
@@ -430,20 +498,15 @@ __global__ void kShade(SceneDev s0, MptArgs a, uint32_t depth) {
                     if (a.nee && anyPos(e) && !prevDelta) {
                         float lp = lightPdf(s, int(a.lightKind), o, mk3(n4.x, n4.y, n4.z), s.triToLight[h.x], sd.posW,
                                             sd.orientedFaceN());
-                        e = e * misPower(t4.w, lp);
+                        e = e * misPower(t4.w, lp * lightTypes(s, a).pEmissive);
                     }
                     Ls = Ls + thp * e;
                     a.Ls[pathId] = make_float4(Ls.x, Ls.y, Ls.z, 0.f);
                 }
                 if (depth < a.maxBounces) {
                     f3 org = rayOriginFromShading(sd);
-                    if (a.nee) {
-                        shadow = neeSample(s, a.lightKind, sd, m, rng, shO, shD, shT, shC);
-                        if (shadow) shC = thp * shC;
-                    } else if (s.nALights) {
-                        shadow = analyticSample(s, sd, m, rng.u1(), shO, shD, shT, shC);
-                        if (shadow) shC = thp * shC;
-                    }
+                    shadow = directSample(s, a, sd, m, rng, shO, shD, shT, shC);
+                    if (shadow) shC = thp * shC;
                     Scatter sc;
                     if (matSample(sd, m, rng, sc, a.importance != 0)) {
                         thp = thp * sc.weight;
@@ -459,7 +522,11 @@ __global__ void kShade(SceneDev s0, MptArgs a, uint32_t depth) {
                 // scatterMiss (MinimalPathTracer.rt.slang:475-486): the env map lights the missed ray.
                 const float4 rd = a.rayQ[qi][2 * size_t(i) + 1];
                 const float4 t4 = a.thp[pathId], ls4 = a.Ls[pathId];
-                const f3 Le = envEval(s, mk3(rd.x, rd.y, rd.z));
+                const f3 rdir = mk3(rd.x, rd.y, rd.z);
+                f3 Le = envEval(s, rdir);
+                // MIS against env map sampling at the previous vertex (PathTracer.slang:1096-1108).
+                const float pEnv = lightTypes(s, a).pEnv;
+                if (pEnv > 0.f && !(f2u(a.prevN[pathId].w) & 1u)) Le = Le * misPower(t4.w, envPdf(s, rdir) * pEnv);
                 a.Ls[pathId] = make_float4(ls4.x + t4.x * Le.x, ls4.y + t4.y * Le.y, ls4.z + t4.z * Le.z, 0.f);
             }
         }

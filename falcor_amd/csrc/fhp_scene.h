@@ -5,7 +5,7 @@
 //            child[4] (uint bits) and one spare float4. One node is one 128-byte cache line.
 //   triPos   48 B per triangle in BVH leaf order: (v0, origTri) (v1, 0) (v2, 0).
 //   idx      3 x u32 per triangle (original order); vtx 32 B per vertex: (p.xyz, n.x) (n.yz, uv).
-//   triMat   u32 material id per triangle; mats 64 B per material (16 floats, see MatField).
+//   triMat   u32 material id per triangle; mats 80 B per material (20 floats, see MatField).
 //   texels   RGB fp32 texels of all textures; texDesc int4 (offset, width, height, 0).
 //   lights   80 B per emissive triangle; lbvh 64 B per light-BVH node.
 #pragma once
@@ -13,7 +13,7 @@
 
 namespace fhp {
 
-// Material record (16 floats), identical to the scene description the Python layer builds.
+// Material record (20 floats), identical to the scene description the Python layer builds.
 enum MatField { M_TYPE = 0, M_BASE = 1, M_ROUGH = 4, M_METAL = 5, M_IOR = 6, M_EMIT = 7, M_EMIT_FACTOR = 10,
                 M_DOUBLE_SIDED = 11, M_TEX = 12, M_EMIT_TEX = 13, M_SPEC_TRANS = 14, M_DIFF_TRANS = 15, M_TRANS = 16,
                 M_THIN = 19, M_STRIDE = 20 };
@@ -74,6 +74,11 @@ struct SceneDev {
     const float* envTex;
     uint32_t envW, envH, envFlags;
     f3 envInv0, envInv1, envInv2, envScale;
+    // Hierarchical importance map of the env map (EnvMapSampler): square levels of luminance,
+    // the r x r level (r = 1, 2, 4, ..., envImpDim) row-major at offset (r * r - 1) / 3; null when
+    // there is no env map or its radiance integrates to zero.
+    const float* envImp;
+    uint32_t envImpDim;
     // Analytic lights (LightData.slang:48-66 subset): 4 float4 per light, (type, intensity rgb)
     // (posW, openingAngle) (dirW, cosOpeningAngle) (penumbraAngle, 0, 0, 0); null when none.
     const float4* alights;
@@ -167,13 +172,9 @@ FHP_HD f3 texFetch(const SceneDev& s, const float* texels, int tex, float u, flo
     return top * wy0 + bot * fy;
 }
 
-// EnvMap::eval(dir) (EnvMap.slang:58-68): direction to local space, lat-long coordinates
-// world_to_latlong_map (MathHelpers.slang:111-118), bilinear filter at mip 0 wrapping in u and
-// clamping in v (the env sampler of EnvMap.cpp:131-135), times intensity * tint.
-FHP_HD f3 envEval(const SceneDev& s, f3 dir) {
-    const f3 p = normalize(mk3(dot(s.envInv0, dir), dot(s.envInv1, dir), dot(s.envInv2, dir)));
-    const float u = atan2f(p.x, -p.z) * (0.5f * kInvPi) + 0.5f;
-    const float v = acosf(fminf(fmaxf(p.y, -1.f), 1.f)) * kInvPi;
+// Bilinear lookup of the lat-long texels at (u, v), mip 0, wrapping in u and clamping in v (the
+// env sampler of EnvMap.cpp:131-135), without intensity and tint.
+FHP_HD f3 envLookup(const SceneDev& s, float u, float v) {
     const int W = int(s.envW), H = int(s.envH);
     const float x = u * float(W) - 0.5f, y = v * float(H) - 0.5f;
     const float fx0 = floorf(x), fy0 = floorf(y);
@@ -188,8 +189,100 @@ FHP_HD f3 envEval(const SceneDev& s, f3 dir) {
     const float wx0 = 1.f - fx, wy0 = 1.f - fy;
     const f3 top = mk3(q00[0], q00[1], q00[2]) * wx0 + mk3(q10[0], q10[1], q10[2]) * fx;
     const f3 bot = mk3(q01[0], q01[1], q01[2]) * wx0 + mk3(q11[0], q11[1], q11[2]) * fx;
-    const f3 c = top * wy0 + bot * fy;
+    return top * wy0 + bot * fy;
+}
+
+// world_to_latlong_map (MathHelpers.slang:111-118) of a unit direction.
+FHP_HD void latlongUV(f3 p, float& u, float& v) {
+    u = atan2f(p.x, -p.z) * (0.5f * kInvPi) + 0.5f;
+    v = acosf(fminf(fmaxf(p.y, -1.f), 1.f)) * kInvPi;
+}
+
+// EnvMap::eval(dir) (EnvMap.slang:58-68): direction to local space, lat-long coordinates,
+// bilinear lookup, times intensity * tint.
+FHP_HD f3 envEval(const SceneDev& s, f3 dir) {
+    const f3 p = normalize(mk3(dot(s.envInv0, dir), dot(s.envInv1, dir), dot(s.envInv2, dir)));
+    float u, v;
+    latlongUV(p, u, v);
+    const f3 c = envLookup(s, u, v);
     return mk3(c.x * s.envScale.x, c.y * s.envScale.y, c.z * s.envScale.z);
+}
+
+FHP_HD float signf(float x) { return x > 0.f ? 1.f : (x < 0.f ? -1.f : 0.f); }
+
+// Equal-area octahedral map (Clarberg 2008), oct_to_ndir_equal_area_unorm and
+// ndir_to_oct_equal_area_unorm (MathHelpers.slang:211-254): the centre of [0,1]^2 is +z, the
+// corners -z.
+FHP_HD f3 octToDirEqualArea(float px, float py) {
+    px = px * 2.f - 1.f;
+    py = py * 2.f - 1.f;
+    const float d = 1.f - (fabsf(px) + fabsf(py));
+    const float r = 1.f - fabsf(d);
+    const float phi = r > 0.f ? ((fabsf(py) - fabsf(px)) / r + 1.f) * (0.25f * kPi) : 0.f;
+    const float f = r * sqrtf(2.f - r * r);
+    return mk3(f * signf(px) * cosf(phi), f * signf(py) * sinf(phi), signf(d) * (1.f - r * r));
+}
+FHP_HD void dirToOctEqualArea(f3 n, float& ux, float& uy) {
+    const float r = sqrtf(1.f - fabsf(n.z));
+    const float phi = atan2f(fabsf(n.y), fabsf(n.x));
+    float py = r * phi * (2.f * kInvPi);
+    float px = r - py;
+    if (n.z < 0.f) {
+        const float t = px;
+        px = 1.f - py;
+        py = 1.f - t;
+    }
+    px *= signf(n.x);
+    py *= signf(n.y);
+    ux = px * 0.5f + 0.5f;
+    uy = py * 0.5f + 0.5f;
+}
+
+FHP_HD f3 envToWorld(const SceneDev& s, f3 l) { return s.envInv0 * l.x + s.envInv1 * l.y + s.envInv2 * l.z; }
+
+// EnvMapSampler::sample (EnvMapSampler.slang:73-145): descend the importance map from the 2x2
+// level, choosing a column by its two-texel sums and then a row, rescaling (ux, uy) at each step;
+// the final texel plus the remaining (ux, uy) is a point of the octahedral map. pdf is per solid
+// angle: texel value / mean / (4 pi).
+FHP_HD bool envSample(const SceneDev& s, float ux, float uy, f3& dir, float& pdf, f3& Le) {
+    const uint32_t dim = s.envImpDim;
+    const float avg = s.envImp[0];
+    uint32_t px = 0, py = 0;
+    for (uint32_t r = 2; r <= dim; r <<= 1) {
+        px *= 2u;
+        py *= 2u;
+        const float* L = s.envImp + (r * r - 1u) / 3u;
+        const float w0 = L[py * r + px], w1 = L[py * r + px + 1u], w2 = L[(py + 1u) * r + px], w3 = L[(py + 1u) * r + px + 1u];
+        const float q0 = w0 + w2, q1 = w1 + w3;
+        const float d = q0 / (q0 + q1);
+        uint32_t ox, oy;
+        if (ux < d) { ox = 0u; ux = ux / d; }
+        else { ox = 1u; ux = (ux - d) / (1.f - d); }
+        const float e = ox == 0u ? (w0 / q0) : (w1 / q1);
+        if (uy < e) { oy = 0u; uy = uy / e; }
+        else { oy = 1u; uy = (uy - e) / (1.f - e); }
+        px += ox;
+        py += oy;
+    }
+    const float inv = 1.f / float(dim);
+    const f3 l = octToDirEqualArea((float(px) + ux) * inv, (float(py) + uy) * inv);
+    const float* L0 = s.envImp + (dim * dim - 1u) / 3u;
+    pdf = L0[py * dim + px] / avg * (0.25f * kInvPi);
+    dir = envToWorld(s, l);
+    Le = envEval(s, dir);
+    return true;
+}
+
+// EnvMapSampler::evalPdf (EnvMapSampler.slang:154-163): the finest level at the direction's
+// octahedral texel (point sampling, clamped), over the mean, per solid angle.
+FHP_HD float envPdf(const SceneDev& s, f3 dir) {
+    const f3 l = mk3(dot(s.envInv0, dir), dot(s.envInv1, dir), dot(s.envInv2, dir));
+    float ux, uy;
+    dirToOctEqualArea(l, ux, uy);
+    const int dim = int(s.envImpDim);
+    const int x = min(max(int(floorf(ux * float(dim))), 0), dim - 1), y = min(max(int(floorf(uy * float(dim))), 0), dim - 1);
+    const float* L0 = s.envImp + (uint32_t(dim) * uint32_t(dim) - 1u) / 3u;
+    return L0[y * dim + x] / s.envImp[0] * (0.25f * kInvPi);
 }
 
 // ------------------------------------------------------------------ material instance

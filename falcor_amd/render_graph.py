@@ -274,6 +274,39 @@ class Scene:
                                                          prm.ctypes.data, flags))
         self.version += 1
 
+    def get_env_importance(self):
+        """The env map's importance levels (EnvMapSampler): a list of square float32 arrays from
+        1 x 1 (the mean) to the finest level; empty without a sampleable env map."""
+        L = _lib.load()
+        dim = C.c_uint32()
+        _lib.check(L.fhp_scene_get_env_importance(self.handle, C.byref(dim), None))
+        if dim.value == 0:
+            return []
+        buf = np.empty((4 * dim.value ** 2 - 1) // 3, np.float32)
+        _lib.check(L.fhp_scene_get_env_importance(self.handle, C.byref(dim), buf.ctypes.data))
+        out, r = [], 1
+        while r <= dim.value:
+            o = (r * r - 1) // 3
+            out.append(buf[o:o + r * r].reshape(r, r))
+            r *= 2
+        return out
+
+    def env_sample(self, u):
+        """EnvMapSampler::sample for [n, 2] uniforms (device tensor): [n, 8] (dir, pdf, Le, valid)."""
+        torch = _torch()
+        u = u.detach().to(device=_device(), dtype=torch.float32).contiguous()
+        out = torch.empty((u.shape[0], 8), dtype=torch.float32, device=u.device)
+        _lib.check(_lib.load().fhp_env_sample(self.handle, u.data_ptr(), u.shape[0], out.data_ptr(), _lib.stream_ptr()))
+        return out
+
+    def env_eval_pdf(self, dirs):
+        """EnvMapSampler::evalPdf for [n, 3] world directions (device tensor): [n]."""
+        torch = _torch()
+        d = dirs.detach().to(device=_device(), dtype=torch.float32).contiguous()
+        out = torch.empty((d.shape[0],), dtype=torch.float32, device=d.device)
+        _lib.check(_lib.load().fhp_env_eval_pdf(self.handle, d.data_ptr(), d.shape[0], out.data_ptr(), _lib.stream_ptr()))
+        return out
+
     def set_texels(self, texels):
         """Copies a device tensor of texels (same layout as SceneData.texels) into the scene."""
         torch = _torch()
@@ -411,7 +444,9 @@ class _NativePass(RenderPass):
 
 class MinimalPathTracer(_NativePass):
     """MinimalPathTracer (MinimalPathTracer.cpp). Properties: maxBounces, computeDirect,
-    useImportanceSampling; extensions useEmissiveNEE, emissiveSampler ('Uniform'|'LightBVH')."""
+    useImportanceSampling; extensions useEmissiveNEE, useEnvNEE (env map light sampling),
+    emissiveSampler ('Uniform'|'LightBVH'). With several light types enabled (env map, emissive,
+    analytic) one is chosen per vertex with equal probability."""
 
     kind = "MinimalPathTracer"
     outputs = ("color",)

@@ -168,6 +168,17 @@ int fhp_scene_freeze_sampling(FhpScene scene, int enable, FhpStream stream);
  * misses (:462-465), i.e. Scene::useEnvLight / useEnvBackground. width = 0 removes the map. */
 int fhp_scene_set_env_map(FhpScene scene, const float* h_texels, uint32_t width, uint32_t height, const float* h_params,
                           uint32_t flags);
+/* EnvMapSampler (Source/Falcor/Rendering/Lights/EnvMapSampler.cpp:39-115, EnvMapSampler.slang):
+ * fhp_scene_set_env_map builds the hierarchical importance map (512 x 512 equal-area octahedral
+ * luminance, 64 samples per texel, 2 x 2 means down to 1 x 1). get_env_importance writes its
+ * dimension (0: no map, or one whose luminance integrates to zero) and, when h_levels is not null,
+ * all levels coarse to fine ((4 dim^2 - 1) / 3 floats, the r x r level at (r^2 - 1) / 3). */
+int fhp_scene_get_env_importance(FhpScene scene, uint32_t* dim, float* h_levels);
+/* EnvMapSampler::sample for n pairs d_u (2 floats each, in [0,1)): d_out 8 floats per sample
+ * (world direction, pdf per solid angle, Le, valid). */
+int fhp_env_sample(FhpScene scene, const float* d_u, uint32_t n, float* d_out, FhpStream stream);
+/* EnvMapSampler::evalPdf for n world directions d_dirs (3 floats each): d_pdf n floats. */
+int fhp_env_eval_pdf(FhpScene scene, const float* d_dirs, uint32_t n, float* d_pdf, FhpStream stream);
 /* Analytic lights (Scene::addLight with PointLight / DirectionalLight, Source/Falcor/Scene/Lights/
  * Light.cpp), n records of 16 floats: [type (0 point, 1 directional), intensity rgb, position xyz,
  * openingAngle, direction xyz, penumbraAngle, 0, 0, 0, 0]. Directions are normalised, the opening

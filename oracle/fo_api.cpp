@@ -204,6 +204,78 @@ void fo_scene_set_analytic_lights(void* h, uint32_t n, const float* recs) {
         r[8] = d.x; r[9] = d.y; r[10] = d.z; r[11] = std::cos(opening); r[12] = penumbra;
     }
 }
+// Environment map: lat-long texels H x W x 3, params [intensity, tint rgb, rotation degrees xyz],
+// flags 1 light / 2 background; builds EnvMapSampler's importance map (dim x dim, ns x ns samples
+// per texel, EnvMapSampler.cpp:39-115). W = 0 removes the map.
+void fo_scene_set_env_map(void* h, const float* texels, uint32_t W, uint32_t H, const float* prm, uint32_t flags,
+                          uint32_t dim, uint32_t ns, int threads) {
+    Scene* s = static_cast<Scene*>(h);
+    s->envImp.clear();
+    s->envImpDim = 0;
+    if (W == 0 || H == 0) { s->envTex.clear(); s->envW = s->envH = s->envFlags = 0; return; }
+    s->envTex.assign(texels, texels + 3 * size_t(W) * H);
+    s->envW = W; s->envH = H; s->envFlags = flags;
+    // EnvMap::setRotation (EnvMap.cpp:62-72): matrixFromRotationXYZ of the negated angles
+    // (MatrixMath.h:649-679); world-to-local is its transpose.
+    const float k = Consts<float>::pi / 180.f;
+    const float c1 = std::cos(-prm[4] * k), c2 = std::cos(-prm[5] * k), c3 = std::cos(-prm[6] * k);
+    const float s1 = std::sin(-prm[4] * k), s2 = std::sin(-prm[5] * k), s3 = std::sin(-prm[6] * k);
+    const float m[3][3] = {{c2 * c3, c2 * s3, -s2},
+                           {-c1 * s3 + s1 * s2 * c3, c1 * c3 + s1 * s2 * s3, s1 * c2},
+                           {s1 * s3 + c1 * s2 * c3, -s1 * c3 + c1 * s2 * s3, c1 * c2}};
+    for (int r = 0; r < 3; r++) s->envInv[r] = V3f(m[0][r], m[1][r], m[2][r]);
+    s->envScale = V3f(prm[0] * prm[1], prm[0] * prm[2], prm[0] * prm[3]);
+    std::vector<float> imp((4 * size_t(dim) * dim - 1) / 3);
+    float* fine = &imp[(size_t(dim) * dim - 1) / 3];
+    parallelRows(dim, hwThreads(threads), [&](uint32_t y) {
+        for (uint32_t x = 0; x < dim; x++) fine[size_t(y) * dim + x] = envImportanceTexel(*s, x, y, dim, ns);
+    });
+    for (uint32_t r = dim / 2; r >= 1; r /= 2) {
+        const float* src = &imp[(4 * size_t(r) * r - 1) / 3];
+        float* dst = &imp[(size_t(r) * r - 1) / 3];
+        const uint32_t R = 2 * r;
+        for (uint32_t y = 0; y < r; y++)
+            for (uint32_t x = 0; x < r; x++) {
+                const float* q = src + size_t(2 * y) * R + 2 * x;
+                dst[size_t(y) * r + x] = ((q[0] + q[1]) + (q[R] + q[R + 1])) * 0.25f;  // generateMips: 2 x 2 mean
+            }
+    }
+    if (imp[0] > 0.f && std::isfinite(imp[0])) { s->envImp.swap(imp); s->envImpDim = dim; }
+}
+uint32_t fo_env_importance(void* h, float* out) {
+    Scene* s = static_cast<Scene*>(h);
+    if (out) std::copy(s->envImp.begin(), s->envImp.end(), out);
+    return s->envImpDim;
+}
+// EnvMapSampler::sample for n pairs u: out 8 floats per sample (dir, pdf, Le, valid).
+void fo_env_sample(void* h, const float* u, uint32_t n, float* out) {
+    Scene* s = static_cast<Scene*>(h);
+    for (uint32_t i = 0; i < n; i++) {
+        V3f d, Le; float pdf;
+        bool ok = envSample(*s, u[2 * i], u[2 * i + 1], d, pdf, Le);
+        float* o = out + 8 * size_t(i);
+        o[0] = d.x; o[1] = d.y; o[2] = d.z; o[3] = pdf; o[4] = Le.x; o[5] = Le.y; o[6] = Le.z; o[7] = ok ? 1.f : 0.f;
+    }
+}
+void fo_env_pdf(void* h, const float* dirs, uint32_t n, float* out) {
+    Scene* s = static_cast<Scene*>(h);
+    for (uint32_t i = 0; i < n; i++) out[i] = envPdf(*s, V3f(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]));
+}
+void fo_env_eval(void* h, const float* dirs, uint32_t n, float* out) {
+    Scene* s = static_cast<Scene*>(h);
+    for (uint32_t i = 0; i < n; i++) {
+        V3f L = envEval(*s, V3f(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]));
+        out[3 * i] = L.x; out[3 * i + 1] = L.y; out[3 * i + 2] = L.z;
+    }
+}
+// Equal-area octahedral map and its inverse for n points uv: dirs (3 floats), uvBack (2 floats).
+void fo_oct_equal_area(const float* uv, uint32_t n, float* dirs, float* uvBack) {
+    for (uint32_t i = 0; i < n; i++) {
+        V3f d = octToDirEqualArea(uv[2 * i], uv[2 * i + 1]);
+        dirs[3 * i] = d.x; dirs[3 * i + 1] = d.y; dirs[3 * i + 2] = d.z;
+        dirToOctEqualArea(d, uvBack[2 * i], uvBack[2 * i + 1]);
+    }
+}
 void fo_scene_set_materials(void* h, const float* mats) {
     Scene* s = static_cast<Scene*>(h);
     s->mats.assign(mats, mats + size_t(s->nM) * M_STRIDE);
@@ -324,7 +396,7 @@ void fo_trace(void* h, int n, const float* rays, uint32_t* out, int brute, int a
 // ---------------------------------------------------------------- integrators
 // MinimalPathTracer render: out rgb (H*W*3), spp samples per pixel with sample numbers
 // frame0*spp + s, averaged like AccumulatePass Single mode (Accumulate.cs.slang:76-114).
-// flags: bit0 computeDirect, bit1 useImportanceSampling, bit2 useEmissiveNEE, bit3 brute.
+// flags: bit0 computeDirect, bit1 useImportanceSampling, bit2 useEmissiveNEE, bit3 brute, bit4 useEnvNEE.
 // stats (optional): [closest, visibility, nodes, tris].
 void fo_render_mpt(void* h, uint32_t W, uint32_t H, uint32_t spp, uint32_t frame0, uint32_t maxBounces, uint32_t flags,
                    int lightSampler, int threads, float* out, uint64_t* stats) {
@@ -335,6 +407,7 @@ void fo_render_mpt(void* h, uint32_t W, uint32_t H, uint32_t spp, uint32_t frame
     P.useImportanceSampling = flags & 2;
     P.useEmissiveNEE = flags & 4;
     P.brute = flags & 8;
+    P.useEnvNEE = flags & 16;
     P.lightSampler = lightSampler;
     std::vector<TraceStats> rowStats(H);
     parallelRows(H, hwThreads(threads), [&](uint32_t y) {
@@ -380,6 +453,7 @@ uint64_t fo_record_mpt(void* h, uint32_t W, uint32_t H, uint32_t p0, uint32_t p1
     P.useImportanceSampling = flags & 2;
     P.useEmissiveNEE = flags & 4;
     P.brute = flags & 8;
+    P.useEnvNEE = flags & 16;
     P.lightSampler = lightSampler;
     std::vector<RayRecord> recs;
     Tracer tr{*s, P.brute, nullptr, &recs};

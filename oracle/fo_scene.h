@@ -91,6 +91,14 @@ struct Scene {
     // [type (0 point, 1 directional), intensity rgb, posW, openingAngle, dirW, cosOpeningAngle,
     // penumbraAngle, 0, 0, 0].
     std::vector<float> aLights;
+    // Environment map (EnvMap.slang): lat-long RGB texels, world-to-local rotation rows,
+    // intensity * tint, flags (1 light, 2 background); the importance levels of EnvMapSampler,
+    // r x r level at (r * r - 1) / 3 (empty: none, or zero luminance).
+    std::vector<float> envTex;
+    uint32_t envW = 0, envH = 0, envFlags = 0;
+    V3f envInv[3], envScale;
+    std::vector<float> envImp;
+    uint32_t envImpDim = 0;
 
     void triVerts(uint32_t t, V3f v[3]) const {
         v[0] = pos[idx[3 * t]]; v[1] = pos[idx[3 * t + 1]]; v[2] = pos[idx[3 * t + 2]];
@@ -492,6 +500,103 @@ inline float evalMIS(float n0, float p0, float n1, float p1) {
     return q0 / (q0 + q1);
 }
 
+// ---------------------------------------------------------------- environment map
+// Bilinear lat-long lookup at mip 0, wrap in u, clamp in v (EnvMap.cpp:131-135).
+inline V3f envLookup(const Scene& s, float u, float v) {
+    const int W = int(s.envW), H = int(s.envH);
+    const float x = u * float(W) - 0.5f, y = v * float(H) - 0.5f;
+    const float x0f = std::floor(x), y0f = std::floor(y);
+    const float fx = x - x0f, fy = y - y0f;
+    const int x0 = int(x0f), y0 = int(y0f);
+    auto wrap = [](int a, int m) { int r = a % m; return r < 0 ? r + m : r; };
+    auto clampi = [](int a, int m) { return a < 0 ? 0 : (a >= m ? m - 1 : a); };
+    const int xa = wrap(x0, W), xb = wrap(x0 + 1, W), ya = clampi(y0, H), yb = clampi(y0 + 1, H);
+    auto T = [&](int xx, int yy) { const float* q = &s.envTex[3 * (size_t(yy) * W + xx)]; return V3f(q[0], q[1], q[2]); };
+    const V3f top = T(xa, ya) * (1.f - fx) + T(xb, ya) * fx;
+    const V3f bot = T(xa, yb) * (1.f - fx) + T(xb, yb) * fx;
+    return top * (1.f - fy) + bot * fy;
+}
+// world_to_latlong_map (MathHelpers.slang:111-118).
+inline void latlongMap(V3f dir, float& u, float& v) {
+    const V3f p = normalize(dir);
+    u = std::atan2(p.x, -p.z) * (0.5f / Consts<float>::pi) + 0.5f;
+    v = std::acos(std::min(std::max(p.y, -1.f), 1.f)) / Consts<float>::pi;
+}
+inline V3f envToLocal(const Scene& s, V3f d) { return V3f(dot(s.envInv[0], d), dot(s.envInv[1], d), dot(s.envInv[2], d)); }
+inline V3f envToWorld(const Scene& s, V3f l) { return s.envInv[0] * l.x + s.envInv[1] * l.y + s.envInv[2] * l.z; }
+// EnvMap::eval (EnvMap.slang:58-68).
+inline V3f envEval(const Scene& s, V3f dir) {
+    float u, v;
+    latlongMap(envToLocal(s, dir), u, v);
+    const V3f c = envLookup(s, u, v);
+    return V3f(c.x * s.envScale.x, c.y * s.envScale.y, c.z * s.envScale.z);
+}
+inline float signOf(float x) { return x > 0.f ? 1.f : (x < 0.f ? -1.f : 0.f); }
+// oct_to_ndir_equal_area_unorm / ndir_to_oct_equal_area_unorm (MathHelpers.slang:211-254).
+inline V3f octToDirEqualArea(float px, float py) {
+    px = px * 2.f - 1.f;
+    py = py * 2.f - 1.f;
+    const float d = 1.f - (std::fabs(px) + std::fabs(py));
+    const float r = 1.f - std::fabs(d);
+    const float phi = r > 0.f ? ((std::fabs(py) - std::fabs(px)) / r + 1.f) * (Consts<float>::pi / 4.f) : 0.f;
+    const float f = r * std::sqrt(2.f - r * r);
+    return V3f(f * signOf(px) * std::cos(phi), f * signOf(py) * std::sin(phi), signOf(d) * (1.f - r * r));
+}
+inline void dirToOctEqualArea(V3f n, float& ux, float& uy) {
+    const float r = std::sqrt(1.f - std::fabs(n.z));
+    const float phi = std::atan2(std::fabs(n.y), std::fabs(n.x));
+    float py = r * phi * (2.f / Consts<float>::pi);
+    float px = r - py;
+    if (n.z < 0.f) { const float t = px; px = 1.f - py; py = 1.f - t; }
+    px *= signOf(n.x);
+    py *= signOf(n.y);
+    ux = px * 0.5f + 0.5f;
+    uy = py * 0.5f + 0.5f;
+}
+// EnvMapSamplerSetup.cs.slang:44-68 for one texel of the dim x dim level, ns x ns samples.
+inline float envImportanceTexel(const Scene& s, uint32_t x, uint32_t y, uint32_t dim, uint32_t ns) {
+    float L = 0.f;
+    for (uint32_t sy = 0; sy < ns; sy++)
+        for (uint32_t sx = 0; sx < ns; sx++) {
+            const float px = (float(x * ns + sx) + 0.5f) / float(dim * ns), py = (float(y * ns + sy) + 0.5f) / float(dim * ns);
+            float u, v;
+            latlongMap(octToDirEqualArea(px, py), u, v);
+            L += luminance(envLookup(s, u, v));
+        }
+    return L / float(ns * ns);
+}
+// EnvMapSampler::sample (EnvMapSampler.slang:73-145).
+inline bool envSample(const Scene& s, float ux, float uy, V3f& dir, float& pdf, V3f& Le) {
+    const uint32_t dim = s.envImpDim;
+    uint32_t px = 0, py = 0;
+    for (uint32_t r = 2; r <= dim; r *= 2) {
+        px *= 2; py *= 2;
+        const float* L = &s.envImp[(r * r - 1) / 3];
+        const float w0 = L[py * r + px], w1 = L[py * r + px + 1], w2 = L[(py + 1) * r + px], w3 = L[(py + 1) * r + px + 1];
+        const float q0 = w0 + w2, q1 = w1 + w3;
+        const float d = q0 / (q0 + q1);
+        uint32_t ox, oy;
+        if (ux < d) { ox = 0; ux = ux / d; } else { ox = 1; ux = (ux - d) / (1.f - d); }
+        const float e = ox == 0 ? w0 / q0 : w1 / q1;
+        if (uy < e) { oy = 0; uy = uy / e; } else { oy = 1; uy = (uy - e) / (1.f - e); }
+        px += ox; py += oy;
+    }
+    const V3f l = octToDirEqualArea((float(px) + ux) / float(dim), (float(py) + uy) / float(dim));
+    pdf = s.envImp[(dim * dim - 1) / 3 + py * dim + px] / s.envImp[0] * (0.25f / Consts<float>::pi);
+    dir = envToWorld(s, l);
+    Le = envEval(s, dir);
+    return true;
+}
+// EnvMapSampler::evalPdf (EnvMapSampler.slang:154-163): point sampling with clamping.
+inline float envPdf(const Scene& s, V3f dir) {
+    float ux, uy;
+    dirToOctEqualArea(envToLocal(s, dir), ux, uy);
+    const int dim = int(s.envImpDim);
+    const int x = std::min(std::max(int(std::floor(ux * float(dim))), 0), dim - 1);
+    const int y = std::min(std::max(int(std::floor(uy * float(dim))), 0), dim - 1);
+    return s.envImp[(size_t(dim) * dim - 1) / 3 + size_t(y) * dim + x] / s.envImp[0] * (0.25f / Consts<float>::pi);
+}
+
 // ---------------------------------------------------------------- integrators
 struct TraceStats { uint64_t closest = 0, visibility = 0, nodes = 0, tris = 0; };
 struct RayRecord { V3f o, d; float tmin, tmax; uint32_t tri; float t, u, v; uint32_t kind; };
@@ -501,6 +606,7 @@ struct MptParams {
     bool computeDirect = true;
     bool useImportanceSampling = true;
     bool useEmissiveNEE = false;  // extension: emissive next-event estimation with MIS
+    bool useEnvNEE = false;       // extension: env map next-event estimation with MIS
     int lightSampler = LS_LIGHTBVH;
     bool brute = false;           // brute-force traversal (small scenes)
 };
@@ -544,13 +650,17 @@ inline void primaryRay(const Camera& c, uint32_t px, uint32_t py, uint32_t W, ui
 
 // NEE extension (shared by both integrator modes of this oracle): one emissive light sample
 // with a shadow ray; MIS against the BSDF pdf (PTUtils.slang:177-249 without reparameterization).
-inline V3f neeEmissive(const Scene& s, const MptParams& P, const ShadingData& sd, MatInst& mi, SampleGen& sg, Tracer& tr) {
+// `sel` is the probability of having chosen emissive lights among the light types.
+inline V3f neeEmissive(const Scene& s, const MptParams& P, const ShadingData& sd, MatInst& mi, SampleGen& sg, Tracer& tr,
+                       float sel = 1.f) {
     LightSample ls;
     V3f nrm = mi.sf.N;
     if (!sampleLight(s, P.lightSampler, sd.posW, nrm, sg, ls)) return V3f(0.f);
     if (!(ls.pdf > 0.f)) return V3f(0.f);
     V3f Li = ls.Le / ls.pdf;
     if (!anyPositive(Li)) return V3f(0.f);
+    ls.pdf *= sel;  // generateLightSample (PathTracer.slang:729-732)
+    Li = Li / sel;
     V3f origin = computeRayOrigin(sd.posW, dot(sd.faceN, ls.dir) >= 0.f ? sd.faceN : -sd.faceN);
     V3f lightPos = computeRayOrigin(ls.posW, ls.normalW);
     V3f toLight = lightPos - origin;
@@ -607,13 +717,67 @@ inline V3f evalDirectAnalytic(const Scene& s, const ShadingData& sd, MatInst& mi
     return miEval(sd, mi, dir) * Li * invPdf;
 }
 
+// Env map NEE (PathTracer.slang generateEnvMapSample, NEE with MIS): one EnvMapSampler sample
+// and a visibility ray to infinity.
+inline V3f neeEnvMap(const Scene& s, const ShadingData& sd, MatInst& mi, SampleGen& sg, Tracer& tr, float sel) {
+    const float ux = sg.next1D(), uy = sg.next1D();
+    V3f dir, Le;
+    float pdf;
+    if (!envSample(s, ux, uy, dir, pdf, Le)) return V3f(0.f);
+    V3f Li = pdf > 0.f ? Le / pdf : V3f(0.f);
+    if (!anyPositive(Li)) return V3f(0.f);
+    pdf *= sel;
+    Li = Li / sel;
+    const V3f origin = computeRayOrigin(sd.posW, dot(sd.faceN, dir) >= 0.f ? sd.faceN : -sd.faceN);
+    const V3f f = miEval(sd, mi, dir);
+    if (!anyPositive(f)) return V3f(0.f);
+    const V3f contrib = f * Li * evalMIS(1.f, pdf, 1.f, miEvalPdf(sd, mi, dir));
+    if (!tr.visible(origin, dir, std::numeric_limits<float>::max())) return V3f(0.f);
+    return contrib;
+}
+
+// Light-type selection probabilities (PathTracer.slang:643-663): equal among env map (when
+// useEnvNEE and a sampleable env light exist), emissive (useEmissiveNEE), analytic (any).
+struct LightTypeProbs { float env = 0.f, emissive = 0.f, analytic = 0.f; uint32_t count = 0; };
+inline LightTypeProbs lightTypeProbs(const Scene& s, const MptParams& P) {
+    const bool env = P.useEnvNEE && s.envImpDim > 0 && (s.envFlags & 1u);
+    const bool em = P.useEmissiveNEE, an = !s.aLights.empty();
+    LightTypeProbs t;
+    t.count = uint32_t(env) + uint32_t(em) + uint32_t(an);
+    const float inv = t.count ? 1.f / float(t.count) : 0.f;
+    t.env = env ? inv : 0.f; t.emissive = em ? inv : 0.f; t.analytic = an ? inv : 0.f;
+    return t;
+}
+
+// Direct light at a vertex: selectLightType (PathTracer.slang:671-693) and the chosen type's
+// sample with one visibility ray. The selection draws a number only when more than one type is
+// enabled; analytic lights alone are evalDirectAnalytic.
+inline V3f directLight(const Scene& s, const MptParams& P, const ShadingData& sd, MatInst& mi, SampleGen& sg, Tracer& tr) {
+    const LightTypeProbs t = lightTypeProbs(s, P);
+    if (t.count == 0) return V3f(0.f);
+    int type;
+    float sel = 1.f;
+    if (t.count == 1) {
+        type = t.env > 0.f ? 0 : (t.emissive > 0.f ? 1 : 2);
+    } else {
+        float u = sg.next1D();
+        if (u < t.env) { type = 0; sel = t.env; }
+        else if ((u -= t.env) < t.emissive) { type = 1; sel = t.emissive; }
+        else if ((u -= t.emissive) < t.analytic) { type = 2; sel = t.analytic; }
+        else return V3f(0.f);
+    }
+    if (type == 0) return neeEnvMap(s, sd, mi, sg, tr, sel);
+    if (type == 1) return neeEmissive(s, P, sd, mi, sg, tr, sel);
+    return evalDirectAnalytic(s, sd, mi, sg, tr) / sel;
+}
+
 // MinimalPathTracer.rt.slang:392-468 (one path for one pixel and sample number).
 inline V3f mptTracePath(const Scene& s, const MptParams& P, uint32_t px, uint32_t py, uint32_t W, uint32_t H,
                         uint32_t frameCount, Tracer& tr) {
     V3f o, d; float tmin, tmax;
     primaryRay(s.cam, px, py, W, H, o, d, tmin, tmax);
     Hit hit = tr.closest(o, d, tmin, tmax);
-    if (!hit.valid()) return V3f(0.f);
+    if (!hit.valid()) return !s.envTex.empty() && (s.envFlags & 2u) ? envEval(s, d) : V3f(0.f);  // :462-465
     ShadingData sd = loadShadingData(s, hit, d);
     MatInst mi = makeMatInst(s, sd);
     SampleGen sg = makeSampleGen(px, py, frameCount);
@@ -621,8 +785,7 @@ inline V3f mptTracePath(const Scene& s, const MptParams& P, uint32_t px, uint32_
     V3f outColor(0.f);
     if (P.computeDirect) {
         outColor = outColor + mi.emission;
-        if (P.useEmissiveNEE) outColor = outColor + neeEmissive(s, P, sd, mi, sg, tr);
-        else if (!s.aLights.empty()) outColor = outColor + evalDirectAnalytic(s, sd, mi, sg, tr);
+        outColor = outColor + directLight(s, P, sd, mi, sg, tr);
     }
     // ScatterRayData
     V3f radiance(0.f), thp(1.f), rOrigin, rDir;
@@ -644,7 +807,18 @@ inline V3f mptTracePath(const Scene& s, const MptParams& P, uint32_t px, uint32_
     if (!scatter(sd, mi, rayOrigin)) terminated = true;
     for (uint32_t depth = 0; depth <= P.maxBounces && !terminated; depth++) {
         Hit h = tr.closest(rOrigin, rDir, 0.f, std::numeric_limits<float>::max());
-        if (!h.valid()) { terminated = true; break; }  // scatterMiss: no env map
+        if (!h.valid()) {
+            // scatterMiss (MinimalPathTracer.rt.slang:475-486), with MIS against env map sampling
+            // at the previous vertex (PathTracer.slang:1096-1108).
+            if (!s.envTex.empty() && (s.envFlags & 1u) && (P.computeDirect || pathLength > 0)) {
+                V3f Le = envEval(s, rDir);
+                const float pEnv = lightTypeProbs(s, P).env;
+                if (pEnv > 0.f && !prevDelta) Le = Le * evalMIS(1.f, prevPdf, 1.f, envPdf(s, rDir) * pEnv);
+                radiance = radiance + thp * Le;
+            }
+            terminated = true;
+            break;
+        }
         ShadingData sdh = loadShadingData(s, h, rDir);
         MatInst mih = makeMatInst(s, sdh);
         if (P.computeDirect || pathLength > 0) {
@@ -652,19 +826,13 @@ inline V3f mptTracePath(const Scene& s, const MptParams& P, uint32_t px, uint32_
             if (P.useEmissiveNEE && anyPositive(e) && !prevDelta) {
                 uint32_t li = s.triToLight[h.tri];
                 float lp = lightPdf(s, P.lightSampler, prevPos, prevN, li, sdh.posW, orientedFaceN(sdh));
-                e = e * evalMIS(1.f, prevPdf, 1.f, lp);
+                e = e * evalMIS(1.f, prevPdf, 1.f, lp * lightTypeProbs(s, P).emissive);
             }
             radiance = radiance + thp * e;
         }
         if (pathLength >= P.maxBounces) { terminated = true; break; }
         V3f org = sdRayOrigin(sdh);
-        if (P.useEmissiveNEE) {
-            V3f Lr = neeEmissive(s, P, sdh, mih, sg, tr);
-            radiance = radiance + thp * Lr;
-        } else if (!s.aLights.empty()) {
-            V3f Lr = evalDirectAnalytic(s, sdh, mih, sg, tr);
-            radiance = radiance + thp * Lr;
-        }
+        if (lightTypeProbs(s, P).count) radiance = radiance + thp * directLight(s, P, sdh, mih, sg, tr);
         if (!scatter(sdh, mih, org)) { terminated = true; break; }
         pathLength++;
     }

@@ -47,6 +47,12 @@ def oracle_lib():
             "fo_fresnel_dielectric": (None, [i32, vp, vp, vp]),
             "fo_scene_animate": (None, [vp, vp, vp]),
             "fo_scene_set_analytic_lights": (None, [vp, u32, vp]),
+            "fo_scene_set_env_map": (None, [vp, vp, u32, u32, vp, u32, u32, u32, i32]),
+            "fo_env_importance": (u32, [vp, vp]),
+            "fo_env_sample": (None, [vp, vp, u32, vp]),
+            "fo_env_pdf": (None, [vp, vp, u32, vp]),
+            "fo_env_eval": (None, [vp, vp, u32, vp]),
+            "fo_oct_equal_area": (None, [vp, u32, vp, vp]),
             "fo_render_mpt_f64": (None, [vp, u32, u32, u32, u32, u32, u32, i32, vp]),
             "fo_scene_create": (vp, [u32, vp, vp, vp, u32, vp, vp, u32, vp, u32, vp, u64, vp, vp]),
             "fo_scene_destroy": (None, [vp]),
@@ -97,6 +103,9 @@ class OracleScene:
                                    _p(cam))
         if getattr(data, "analytic_lights", None):
             self.set_analytic_lights(data.analytic_lights)
+        if getattr(data, "env_map", None) is not None:
+            e = data.env_map
+            self.set_env_map(e["texels"], e.get("intensity", 1.0), e.get("tint", (1, 1, 1)), e.get("rotation", (0, 0, 0)))
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -107,6 +116,48 @@ class OracleScene:
         r = np.ascontiguousarray(np.asarray(records, np.float32).reshape(-1, 16))
         self._alights = r
         oracle_lib().fo_scene_set_analytic_lights(self.h, r.shape[0], _p(r) if r.size else None)
+
+    def set_env_map(self, texels, intensity=1.0, tint=(1.0, 1.0, 1.0), rotation=(0.0, 0.0, 0.0), use_light=True,
+                    use_background=True, dim=512, samples_per_axis=8, threads=0):
+        """EnvMap + EnvMapSampler: [H, W, 3] lat-long texels; builds the importance map."""
+        t = np.ascontiguousarray(np.asarray(texels, np.float32)[..., :3])
+        prm = np.array([intensity, *tint, *rotation], np.float32)
+        self._env = (t, prm)
+        flags = (1 if use_light else 0) | (2 if use_background else 0)
+        oracle_lib().fo_scene_set_env_map(self.h, _p(t), t.shape[1], t.shape[0], _p(prm), flags, dim, samples_per_axis, threads)
+
+    def env_importance(self):
+        """Importance levels from 1 x 1 to the finest (list of square arrays); [] without a map."""
+        L = oracle_lib()
+        dim = L.fo_env_importance(self.h, None)
+        if dim == 0:
+            return []
+        buf = np.zeros((4 * dim * dim - 1) // 3, np.float32)
+        L.fo_env_importance(self.h, _p(buf))
+        out, r = [], 1
+        while r <= dim:
+            o = (r * r - 1) // 3
+            out.append(buf[o:o + r * r].reshape(r, r))
+            r *= 2
+        return out
+
+    def env_sample(self, u):
+        u = np.ascontiguousarray(u, np.float32).reshape(-1, 2)
+        out = np.zeros((u.shape[0], 8), np.float32)
+        oracle_lib().fo_env_sample(self.h, _p(u), u.shape[0], _p(out))
+        return out
+
+    def env_pdf(self, dirs):
+        d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        out = np.zeros(d.shape[0], np.float32)
+        oracle_lib().fo_env_pdf(self.h, _p(d), d.shape[0], _p(out))
+        return out
+
+    def env_eval(self, dirs):
+        d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        out = np.zeros((d.shape[0], 3), np.float32)
+        oracle_lib().fo_env_eval(self.h, _p(d), d.shape[0], _p(out))
+        return out
 
     def animate(self, positions: np.ndarray, normals: np.ndarray | None = None):
         """All vertices move (3 floats each); emissive triangles are updated and the light BVH refitted."""
@@ -165,10 +216,10 @@ class OracleScene:
         return out
 
     def render_mpt(self, W, H, spp=1, frame=0, max_bounces=3, compute_direct=True, importance=True, nee=False,
-                   light_sampler=1, brute=False, threads=0):
+                   light_sampler=1, brute=False, threads=0, env_nee=False):
         out = np.zeros((H, W, 3), np.float32)
         stats = np.zeros(4, np.uint64)
-        flags = int(compute_direct) | (int(importance) << 1) | (int(nee) << 2) | (int(brute) << 3)
+        flags = int(compute_direct) | (int(importance) << 1) | (int(nee) << 2) | (int(brute) << 3) | (int(env_nee) << 4)
         oracle_lib().fo_render_mpt(self.h, W, H, spp, frame, max_bounces, flags, light_sampler, threads, _p(out), _p(stats))
         return out, stats
 
@@ -299,3 +350,12 @@ class Oracle:
         out = np.zeros((wi.shape[0], 8), np.float32)
         oracle_lib().fo_bsdf_sample(wi.shape[0], _p(m), _p(wi), _p(s), _p(out))
         return out
+
+
+def oct_equal_area(uv):
+    """Equal-area octahedral map of [n, 2] points: (directions [n, 3], inverse-mapped points [n, 2])."""
+    uv = np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
+    d = np.zeros((uv.shape[0], 3), np.float32)
+    back = np.zeros_like(uv)
+    oracle_lib().fo_oct_equal_area(_p(uv), uv.shape[0], _p(d), _p(back))
+    return d, back

@@ -9,7 +9,8 @@ MinimalPathTracer.rt.slang:253-286) against closed forms and the oracle.
   1e-4 (1 + |ref|), relative L1 difference at most 1e-3), with importance and reference sampling.
 * Clearing the lights gives the image of the scene without them, bit for bit.
 * Batched views of scenes with analytic lights equal their single-view renders bit for bit.
-* Analytic lights together with emissive next-event estimation are refused.
+* Analytic lights together with emissive next-event estimation (one light type chosen per vertex)
+  match the oracle.
 """
 import math
 
@@ -143,10 +144,9 @@ def test_batch_with_analytic_lights_equals_single_views(gpu):
         np.testing.assert_array_equal(o.cpu().numpy(), single.cpu().numpy())
 
 
-def test_emissive_nee_with_analytic_lights_is_refused(gpu):
-    from falcor_amd import _lib
-    from falcor_amd.render_graph import MinimalPathTracer, Scene
-    p = MinimalPathTracer({"maxBounces": 2, "useEmissiveNEE": True})
-    p.set_scene(Scene(_box_with_lights()))
-    with pytest.raises(_lib.FhpError, match="analytic"):
-        p.render(16, 16, 1, frame=0)
+def test_emissive_nee_with_analytic_lights_matches_oracle(gpu, oracle_mod):
+    # Both light types enabled: one is chosen per vertex with probability 1/2 (PathTracer.slang:643-693).
+    data = _field_with_lights(34)
+    img = _render(data, {"maxBounces": 4, "useEmissiveNEE": True}, 96, 96, 2, frame=2)
+    ref, _ = oracle_mod.OracleScene(data).render_mpt(96, 96, spp=2, frame=2, max_bounces=4, nee=True)
+    _compare(img, ref, "emissive NEE + analytic lights")
