@@ -1126,7 +1126,9 @@ int fhp_pass_execute(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t spp
         a.envNee = p->get("useEnvNEE") != 0;
         if (a.nee && !a.computeDirect) return fail(FHP_EINVAL, "useEmissiveNEE requires computeDirect");
         if (a.envNee && !a.computeDirect) return fail(FHP_EINVAL, "useEnvNEE requires computeDirect");
-        a.shadowRays = (a.nee || s->dev.nALights || (a.envNee && s->dev.envImp && (s->dev.envFlags & kEnvLight))) ? 1u : 0u;
+        const bool envLight = a.envNee && s->dev.envImp && (s->dev.envFlags & kEnvLight);
+        a.shadowRays = (a.nee || s->dev.nALights || envLight) ? 1u : 0u;
+        a.lightMode = (s->dev.nALights || envLight) ? 2u : (a.nee ? 1u : 0u);
         a.nPaths = W * H * spp;
         a.nPix = W * H;
         a.thp = p->thp.as<float4>(); a.prevN = p->prevN.as<float4>(); a.rng = p->rng.as<uint4>();
@@ -1259,9 +1261,10 @@ int fhp_pass_execute_batch(FhpPass p, uint32_t nv, const FhpView* views, uint64_
     a.envNee = p->get("useEnvNEE") != 0;
     if (a.nee && !a.computeDirect) return fail(FHP_EINVAL, "useEmissiveNEE requires computeDirect");
     if (a.envNee && !a.computeDirect) return fail(FHP_EINVAL, "useEnvNEE requires computeDirect");
-    a.shadowRays = a.nee ? 1u : 0u;
-    for (FhpScene sc : scenes)
-        a.shadowRays |= (sc->dev.nALights || (a.envNee && sc->dev.envImp && (sc->dev.envFlags & kEnvLight))) ? 1u : 0u;
+    bool otherLights = false;
+    for (FhpScene sc : scenes) otherLights |= sc->dev.nALights || (a.envNee && sc->dev.envImp && (sc->dev.envFlags & kEnvLight));
+    a.shadowRays = (a.nee || otherLights) ? 1u : 0u;
+    a.lightMode = otherLights ? 2u : (a.nee ? 1u : 0u);
     a.separateShadow = 0;
     a.nPaths = uint32_t(nPaths);
     a.nPix = uint32_t(nPix);

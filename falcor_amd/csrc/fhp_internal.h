@@ -38,6 +38,7 @@ struct MptArgs {
     uint32_t nPaths;
     uint32_t envNee;          // 1: env map next-event estimation (useEnvNEE)
     uint32_t shadowRays;      // 1: shading emits visibility rays (NEE of any light type)
+    uint32_t lightMode;       // shading specialisation: 0 no light sampling, 1 emissive NEE only, 2 any mix
     uint32_t separateShadow;  // 1: visibility rays in launches of their own (else shared with closest-hit)
                               // 0 also compacts each bounce's hits: hitQ[k] / hitIdx[k], count in counters[kHitCount + depth]
     float4* thp;     // thp.xyz, pdf of the last scatter

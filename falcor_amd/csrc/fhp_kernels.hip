@@ -350,12 +350,19 @@ __device__ __forceinline__ LightTypes lightTypes(const SceneDev& s, const MptArg
     return t;
 }
 
+// Light configurations the shading kernels are specialised for (MptArgs.lightMode): no light
+// sampling, emissive NEE as the only type (the code of a single type, without selection), any mix.
+constexpr int kLightsNone = 0, kLightsEmissive = 1, kLightsAny = 2;
+
 // Direct light at a vertex with one visibility ray: selectLightType (PathTracer.slang:671-693),
 // then the chosen type's sample. A number is drawn for the selection only when more than one
 // type is enabled, so one type alone keeps its own sample sequence (analytic lights alone are
 // MinimalPathTracer's evalDirectAnalytic, MinimalPathTracer.rt.slang:253-286).
+template <int LM>
 __device__ bool directSample(const SceneDev& s, const MptArgs& a, const Shading& sd, const Material& m, Rng& rng, f3& org,
                              f3& dir, float& dist, f3& contrib) {
+    if (LM == kLightsNone) return false;
+    if (LM == kLightsEmissive) return neeSample(s, a.lightKind, sd, m, rng, org, dir, dist, contrib, 1.f);
     const LightTypes t = lightTypes(s, a);
     if (t.count == 0u) return false;
     uint32_t type;
@@ -390,7 +397,7 @@ __device__ __forceinline__ void pushRay(const MptArgs& a, int q, uint32_t slot, 
 }
 
 // Primary-hit shading, one thread per path (MinimalPathTracer.rt.slang:392-449).
-template <bool MULTI>
+template <bool MULTI, int LM>
 __global__ void kShadePrimary(SceneDev s0, MptArgs a) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= a.nPaths) return;
@@ -420,7 +427,7 @@ __global__ void kShadePrimary(SceneDev s0, MptArgs a) {
         f3 Lp = mk3(0.f);
         if (a.computeDirect) {
             Lp = Lp + m.emission;
-            shadow = directSample(s, a, sd, m, rng, shO, shD, shT, shC);
+            shadow = directSample<LM>(s, a, sd, m, rng, shO, shD, shT, shC);
         }
         f3 thp = mk3(1.f);
         Scatter sc;
@@ -464,7 +471,7 @@ __global__ void kShadePrimary(SceneDev s0, MptArgs a) {
 // handleHit for scatter depth `depth` (MinimalPathTracer.rt.slang:324-376), one thread per queued ray.
 // Runs on the compacted hits of the bounce (entry k: hit hitQ[k] of ray hitIdx[k]) unless
 // visibility rays have launches of their own, then on every queued ray (hitQ[i] of ray i).
-template <bool MULTI>
+template <bool MULTI, int LM>
 __global__ void kShade(SceneDev s0, MptArgs a, uint32_t depth) {
     const int qi = depth & 1, qo = qi ^ 1;
     const bool compact = !a.separateShadow;
@@ -498,14 +505,14 @@ __global__ void kShade(SceneDev s0, MptArgs a, uint32_t depth) {
                     if (a.nee && anyPos(e) && !prevDelta) {
                         float lp = lightPdf(s, int(a.lightKind), o, mk3(n4.x, n4.y, n4.z), s.triToLight[h.x], sd.posW,
                                             sd.orientedFaceN());
-                        e = e * misPower(t4.w, lp * lightTypes(s, a).pEmissive);
+                        e = e * misPower(t4.w, LM == kLightsAny ? lp * lightTypes(s, a).pEmissive : lp);
                     }
                     Ls = Ls + thp * e;
                     a.Ls[pathId] = make_float4(Ls.x, Ls.y, Ls.z, 0.f);
                 }
                 if (depth < a.maxBounces) {
                     f3 org = rayOriginFromShading(sd);
-                    shadow = directSample(s, a, sd, m, rng, shO, shD, shT, shC);
+                    shadow = directSample<LM>(s, a, sd, m, rng, shO, shD, shT, shC);
                     if (shadow) shC = thp * shC;
                     Scatter sc;
                     if (matSample(sd, m, rng, sc, a.importance != 0)) {
@@ -525,7 +532,7 @@ __global__ void kShade(SceneDev s0, MptArgs a, uint32_t depth) {
                 const f3 rdir = mk3(rd.x, rd.y, rd.z);
                 f3 Le = envEval(s, rdir);
                 // MIS against env map sampling at the previous vertex (PathTracer.slang:1096-1108).
-                const float pEnv = lightTypes(s, a).pEnv;
+                const float pEnv = LM == kLightsAny ? lightTypes(s, a).pEnv : 0.f;
                 if (pEnv > 0.f && !(f2u(a.prevN[pathId].w) & 1u)) Le = Le * misPower(t4.w, envPdf(s, rdir) * pEnv);
                 a.Ls[pathId] = make_float4(ls4.x + t4.x * Le.x, ls4.y + t4.y * Le.y, ls4.z + t4.z * Le.z, 0.f);
             }
@@ -764,7 +771,12 @@ hipError_t mptBounces(const SceneDev& s, const MptArgs& a, const TraceScratch& t
                                       (npix + tsc.block - 1) / tsc.block);
     if (e != hipSuccess) return e;
     tm->begin(KK_SHADE, st);
-    hipLaunchKernelGGL(kShadePrimary<MULTI>, dim3((a.nPaths + T - 1) / T), dim3(T), 0, st, s, a);
+    if (a.lightMode == kLightsNone)
+        hipLaunchKernelGGL((kShadePrimary<MULTI, kLightsNone>), dim3((a.nPaths + T - 1) / T), dim3(T), 0, st, s, a);
+    else if (a.lightMode == kLightsEmissive)
+        hipLaunchKernelGGL((kShadePrimary<MULTI, kLightsEmissive>), dim3((a.nPaths + T - 1) / T), dim3(T), 0, st, s, a);
+    else
+        hipLaunchKernelGGL((kShadePrimary<MULTI, kLightsAny>), dim3((a.nPaths + T - 1) / T), dim3(T), 0, st, s, a);
     tm->end(st);
     const uint32_t shadeBlocks = std::min<uint32_t>((a.nPaths + T - 1) / T, uint32_t(numCUs()) * 32);
     for (uint32_t b = 0; b <= a.maxBounces; b++) {
@@ -776,7 +788,10 @@ hipError_t mptBounces(const SceneDev& s, const MptArgs& a, const TraceScratch& t
             return e;
         if (a.separateShadow && a.shadowRays && (e = shadow(b)) != hipSuccess) return e;
         tm->begin(KK_SHADE, st);
-        hipLaunchKernelGGL(kShade<MULTI>, dim3(shadeBlocks), dim3(T), 0, st, s, a, b);
+        if (a.lightMode == kLightsNone) hipLaunchKernelGGL((kShade<MULTI, kLightsNone>), dim3(shadeBlocks), dim3(T), 0, st, s, a, b);
+        else if (a.lightMode == kLightsEmissive)
+            hipLaunchKernelGGL((kShade<MULTI, kLightsEmissive>), dim3(shadeBlocks), dim3(T), 0, st, s, a, b);
+        else hipLaunchKernelGGL((kShade<MULTI, kLightsAny>), dim3(shadeBlocks), dim3(T), 0, st, s, a, b);
         tm->end(st);
     }
     return hipGetLastError();
