@@ -175,6 +175,62 @@ def read_exr(path) -> tuple[np.ndarray, list[str]]:
     return np.stack([planes[n].astype(out_dt) for n in names], axis=-1), names
 
 
+def read_hdr(path) -> np.ndarray:
+    """Radiance RGBE (.hdr) as [H, W, 3] float32: run-length encoded (new-style) or flat scanlines,
+    standard orientation "-Y H +X W"; a texel is byte * 2^(e - 136), zero when e = 0 (the rgbe.c
+    conversion FreeImage's HDR loader uses, Bitmap.cpp)."""
+    b = Path(path).read_bytes()
+    pos = 0
+    first = True
+    while True:
+        end = b.index(b"\n", pos)
+        line = b[pos:end].decode("latin-1").strip()
+        pos = end + 1
+        if first and not line.startswith("#?"):
+            raise ValueError("not a Radiance HDR file")
+        first = False
+        if line.startswith("FORMAT=") and line != "FORMAT=32-bit_rle_rgbe":
+            raise ValueError(f"unsupported HDR format {line}")
+        if line == "":
+            break
+    end = b.index(b"\n", pos)
+    res = b[pos:end].decode("latin-1").split()
+    pos = end + 1
+    if len(res) != 4 or res[0] != "-Y" or res[2] != "+X":
+        raise ValueError(f"unsupported HDR orientation {' '.join(res)}")
+    H, W = int(res[1]), int(res[3])
+    rgbe = np.zeros((H, W, 4), np.uint8)
+    for y in range(H):
+        if 8 <= W < 0x8000 and b[pos] == 2 and b[pos + 1] == 2 and ((b[pos + 2] << 8) | b[pos + 3]) == W:
+            pos += 4
+            for c in range(4):
+                x = 0
+                while x < W:
+                    n = b[pos]
+                    pos += 1
+                    if n > 128:
+                        n -= 128
+                        rgbe[y, x:x + n, c] = b[pos]
+                        pos += 1
+                    else:
+                        if n == 0 or x + n > W:
+                            raise ValueError("corrupt HDR scanline")
+                        rgbe[y, x:x + n, c] = np.frombuffer(b, np.uint8, n, pos)
+                        pos += n
+                    x += n
+                if x != W:
+                    raise ValueError("corrupt HDR scanline")
+        else:
+            row = np.frombuffer(b, np.uint8, 4 * W, pos).reshape(W, 4)
+            if (row[:, :3] == 1).all(axis=1).any():
+                raise ValueError("old-style run-length HDR scanlines are not supported")
+            rgbe[y] = row
+            pos += 4 * W
+    e = rgbe[..., 3].astype(np.int32)
+    f = np.where(e > 0, np.ldexp(np.float32(1.0), e - 136).astype(np.float32), np.float32(0.0))
+    return (rgbe[..., :3].astype(np.float32) * f[..., None]).astype(np.float32)
+
+
 def read_rgba(path) -> np.ndarray:
     """Any supported file as [H, W, 4] float32 RGBA, missing alpha = 1 (FreeImage_ConvertToRGBAF,
     ImageCompare.cpp:68-110): EXR channels R, G, B, A (Y as grey), PFM, or 8-bit PNG / other
@@ -192,6 +248,11 @@ def read_rgba(path) -> np.ndarray:
         for c, n in enumerate("RGBA"):
             if n in idx:
                 out[..., c] = img[..., idx[n]]
+        return out
+    if suf == ".hdr":
+        img = read_hdr(p)
+        out = np.ones(img.shape[:2] + (4,), np.float32)
+        out[..., :3] = img
         return out
     if suf == ".pfm":
         img = read_pfm(p)

@@ -103,3 +103,62 @@ def test_image_compare_cli_without_device(capsys):
     assert "mse - Mean Squared Error" in out and "mape" in out
     assert main(["-m", "nope", "a.exr", "b.exr"]) == 1
     assert main([]) == 1
+
+
+def _rgbe_encode(rgbe, rle=True):
+    """Radiance scanlines of [H, W, 4] uint8 quadruples: new-style run-length or flat."""
+    H, W, _ = rgbe.shape
+    out = bytearray(b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n" + f"-Y {H} +X {W}\n".encode())
+    for y in range(H):
+        if not rle:
+            out += rgbe[y].tobytes()
+            continue
+        out += bytes([2, 2, W >> 8, W & 255])
+        for c in range(4):
+            ch = rgbe[y, :, c]
+            x = 0
+            while x < W:
+                r = 1
+                while x + r < W and r < 127 and ch[x + r] == ch[x]:
+                    r += 1
+                if r >= 3:
+                    out += bytes([128 + r, ch[x]])
+                    x += r
+                else:
+                    n = min(128, W - x)
+                    out += bytes([n]) + ch[x:x + n].tobytes()
+                    x += n
+    return bytes(out)
+
+
+@pytest.mark.parametrize("rle", [True, False])
+def test_hdr_reader_decodes_rgbe(tmp_path, rle):
+    from falcor_amd import image_io as I
+    rng = np.random.default_rng(1)
+    H, W = 5, 37
+    rgbe = rng.integers(0, 256, (H, W, 4), dtype=np.uint8)
+    rgbe[:, 10:20] = rgbe[:, 10:11]           # runs
+    rgbe[2, :, 3] = 0                         # zero exponent: black
+    rgbe[..., 0] |= 1                         # no (1, 1, 1, n) old-style run markers in flat files
+    (tmp_path / "a.hdr").write_bytes(_rgbe_encode(rgbe, rle))
+    got = I.read_hdr(tmp_path / "a.hdr")
+    e = rgbe[..., 3].astype(np.int64)
+    want = np.where(e[..., None] > 0, rgbe[..., :3] * np.ldexp(1.0, e - 136)[..., None], 0.0)
+    np.testing.assert_array_equal(got, want.astype(np.float32))
+    np.testing.assert_array_equal(I.read_rgba(tmp_path / "a.hdr")[..., :3], got)
+
+
+def test_reference_default_hdr_env_map():
+    from pathlib import Path
+    from falcor_amd import image_io as I
+    p = Path("/root/reference/Scene/EnvMap/Default.hdr")
+    if not p.exists():
+        pytest.skip("reference env map not present")
+    img = I.read_hdr(p)
+    assert img.shape == (512, 1024, 3) and np.isfinite(img).all() and (img >= 0).all()
+    # Every nonzero texel has a normalised mantissa: its largest channel is in [128, 256) * 2^(e - 136).
+    m = img.max(-1)
+    nz = m > 0
+    e = np.floor(np.log2(m[nz])) + 1
+    frac = m[nz] / np.exp2(e)
+    assert ((frac >= 0.5) & (frac < 1.0)).all()
