@@ -188,3 +188,24 @@ def test_environment_map(tmp_path):
     d = load_pyscene(tmp_path / "e.pyscene")
     assert np.array_equal(d.env_map["texels"], env)
     assert d.env_map["intensity"] == 2.5 and d.env_map["tint"] == (1, 0.5, 0.5) and d.env_map["rotation"] == (0, 90, 0)
+
+
+def test_pyscene_env_map_from_hdr(tmp_path):
+    # EnvMap('sky.hdr') next to the script: the Radiance texels, intensity, tint and rotation.
+    from falcor_amd.pyscene import load_pyscene
+    from tests.test_image_io import _rgbe_encode
+    rng = np.random.default_rng(3)
+    rgbe = rng.integers(1, 256, (4, 16, 4), dtype=np.uint8)
+    (tmp_path / "sky.hdr").write_bytes(_rgbe_encode(rgbe))
+    (tmp_path / "e.pyscene").write_text(
+        "m = StandardMaterial('m')\nq = TriangleMesh.createQuad()\n"
+        "sceneBuilder.addMeshInstance(sceneBuilder.addNode('q'), sceneBuilder.addTriangleMesh(q, m))\n"
+        "e = EnvMap('sky.hdr')\ne.intensity = 2.5\ne.tint = float3(1, 0.5, 0.25)\ne.rotation = float3(0, 90, 0)\n"
+        "sceneBuilder.envMap = e\n")
+    d = load_pyscene(tmp_path / "e.pyscene")
+    e = rgbe[..., 3].astype(np.int64)
+    want = (rgbe[..., :3] * np.ldexp(1.0, e - 136)[..., None]).astype(np.float32)
+    np.testing.assert_array_equal(d.env_map["texels"], want)
+    assert d.env_map["intensity"] == 2.5
+    np.testing.assert_allclose(d.env_map["tint"], (1, 0.5, 0.25))
+    np.testing.assert_allclose(d.env_map["rotation"], (0, 90, 0))
