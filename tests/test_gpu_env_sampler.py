@@ -11,6 +11,8 @@ MinimalPathTracer on the GPU, against the oracle.
   of the pixels off by more than 1e-4 (1 + |ref|), relative L1 difference at most 1e-3.
 * Batched views with env NEE equal their single-view renders bit for bit; useEnvNEE without
   computeDirect is refused.
+* A zero-luminance map has no importance map: env NEE is off (same image as without it) and the
+  sampler entry points refuse it; removing the map removes the importance map.
 """
 import math
 
@@ -120,3 +122,28 @@ def test_batch_with_env_nee_and_refusal(gpu):
     p.set_scene(a)
     with pytest.raises(_lib.FhpError, match="computeDirect"):
         p.render(8, 8, 1, frame=0)
+
+
+def test_unsampleable_and_removed_env_maps(gpu):
+    # A map whose luminance integrates to zero has no importance map: env NEE is off for the scene
+    # (the image equals the one without useEnvNEE) and the sampler entry points refuse it; removing
+    # the map removes the importance map.
+    import torch
+    from falcor_amd import _lib
+    from falcor_amd.render_graph import MinimalPathTracer, Scene
+    data = S.sphere_field(45, num_spheres=30, num_emissive=60, box=8.0, aspect=1.0)
+    data.env_map = {"texels": np.zeros((8, 16, 3), np.float32)}
+    sc = Scene(data)
+    assert sc.get_env_importance() == []
+    with pytest.raises(_lib.FhpError, match="importance sampled"):
+        sc.env_sample(torch.zeros((4, 2), device="cuda"))
+    imgs = []
+    for env_nee in (False, True):
+        p = MinimalPathTracer({"maxBounces": 2, "useEnvNEE": env_nee})
+        p.set_scene(sc)
+        imgs.append(p.render(32, 32, 1, frame=0).cpu().numpy())
+    np.testing.assert_array_equal(imgs[0], imgs[1])
+    sc.set_env_map(_sky())
+    assert len(sc.get_env_importance()) == 10
+    sc.set_env_map(None)
+    assert sc.get_env_importance() == []
