@@ -8,7 +8,9 @@ MinimalPathTracer.rt.slang:253-286) against closed forms and the oracle.
   image tolerance of the other forward tests (at most 0.5 % of the pixels off by more than
   1e-4 (1 + |ref|), relative L1 difference at most 1e-3), with importance and reference sampling.
 * Clearing the lights gives the image of the scene without them, bit for bit.
-* Batched views of scenes with analytic lights equal their single-view renders bit for bit.
+* Batched views of scenes with analytic lights equal their single-view renders bit for bit, also
+  when the batch mixes an emissive-NEE-only scene with one that has analytic lights (the shading
+  kernels' mixed-type specialisation reproduces the emissive-only one exactly).
 * Analytic lights together with emissive next-event estimation (one light type chosen per vertex)
   match the oracle.
 """
@@ -150,3 +152,22 @@ def test_emissive_nee_with_analytic_lights_matches_oracle(gpu, oracle_mod):
     img = _render(data, {"maxBounces": 4, "useEmissiveNEE": True}, 96, 96, 2, frame=2)
     ref, _ = oracle_mod.OracleScene(data).render_mpt(96, 96, spp=2, frame=2, max_bounces=4, nee=True)
     _compare(img, ref, "emissive NEE + analytic lights")
+
+
+def test_batch_mixing_light_configurations_equals_single_views(gpu):
+    # With emissive NEE on, a batch holding a scene with analytic lights runs the shading kernels
+    # specialised for mixed light types; the views of the emissive-only scene must still equal
+    # their single-view renders (emissive-only specialisation) bit for bit.
+    import torch
+    from falcor_amd.render_graph import MinimalPathTracer, Scene
+    plain = S.sphere_field(35, num_spheres=60, num_emissive=200, box=12.0, aspect=1.0)
+    a, b = Scene(plain), Scene(_box_with_lights())
+    views = [dict(scene=a, width=40, height=32, spp=2, frame=1), dict(scene=b, width=24, height=24, spp=1, frame=2),
+             dict(scene=a, width=19, height=21, spp=3, frame=5)]
+    props = {"maxBounces": 3, "useEmissiveNEE": True}
+    outs = MinimalPathTracer(props).render_batch(views)
+    torch.cuda.synchronize()
+    for v, o in zip(views, outs):
+        q = MinimalPathTracer(props)
+        q.set_scene(v["scene"])
+        np.testing.assert_array_equal(o.cpu().numpy(), q.render(v["width"], v["height"], v["spp"], frame=v["frame"]).cpu().numpy())
