@@ -5,5 +5,5 @@ include/fhp.h); this package is the Python render-graph surface over it.
 """
 from . import scene  # noqa: F401
 
-__all__ = ["scene", "render_graph", "diff"]
+__all__ = ["scene", "render_graph", "autograd", "image_io", "image_compare", "pyscene"]
 __version__ = "0.1.0"

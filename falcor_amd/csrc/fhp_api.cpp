@@ -76,6 +76,7 @@ struct FhpScene_ {
     bool restValid = false;
     uint32_t leafSize = 4;  // BVH4 leaf size (triangles)
     DevBuf ovf;  // traversal overflow stack
+    DevBuf err;  // trace error word (kTraceErr* bits set by the traversal kernels)
     DevBuf matsS, texelsS;  // frozen sampling parameters (finite-difference mode)
     TraceScratch ts{};      // default traversal configuration (ray queries)
     uint32_t stackNeed = 0; // traversal stack entries a ray can need (from the BVH build)
@@ -284,6 +285,7 @@ TraceScratch traceConfig(const FhpScene_* s, uint32_t block, uint32_t lds, uint3
     t.gridBlocks = uint32_t(numCUs()) * blocksPerCU;
     t.ovfDepth = s->stackNeed > lds ? s->stackNeed - lds : 0u;
     t.ovf = s->ovf.as<uint32_t>();
+    t.err = s->err.as<uint32_t>();
     return t;
 }
 // Validates the traversal launch properties (traceBlockSize, traceStackDepth, traceBlocksPerCU).
@@ -292,10 +294,16 @@ bool validLaunch(double block, double lds, double bpc) {
         return false;
     return bpc >= 1 && double(uint32_t(bpc)) == bpc && uint32_t(bpc) * uint32_t(block) <= kMaxThreadsPerCU;
 }
-TraceScratch passTrace(const FhpScene_* s, const FhpPass_* p) {
-    TraceScratch t = traceConfig(s, uint32_t(p->get("traceBlockSize")), uint32_t(p->get("traceStackDepth")),
-                                 uint32_t(p->get("traceBlocksPerCU")));
-    return t;
+// Launch configuration of a pass on scene s. The persistent grid indexes the overflow stack as
+// [entry][global thread] over gridBlocks * block threads and ovfDepth entries: refuse a
+// configuration whose indices would not fit the scene's overflow buffer.
+int passTrace(const FhpScene_* s, const FhpPass_* p, TraceScratch& t) {
+    t = traceConfig(s, uint32_t(p->get("traceBlockSize")), uint32_t(p->get("traceStackDepth")),
+                    uint32_t(p->get("traceBlocksPerCU")));
+    const uint64_t words = uint64_t(t.ovfDepth) * t.gridBlocks * t.block;
+    if (words && (!s->ovf.p || words * 4 > s->ovf.bytes))
+        return fail(FHP_ESTATE, "traversal overflow stack smaller than the launch needs");
+    return FHP_OK;
 }
 
 int ensurePassBuffersN(FhpPass p, uint64_t n, uint64_t npix);
@@ -350,11 +358,20 @@ int collectTimer(FhpPass p, hipStream_t st) {
     return FHP_OK;
 }
 
+// Host-array scene mutators are synchronous: before they overwrite or reallocate buffers that
+// kernels read, every queued launch on every stream of the device has finished (a caller may
+// render on any stream, the session's non-blocking one included).
+int waitForDevice() {
+    HIP_OR_FAIL(hipDeviceSynchronize());
+    return FHP_OK;
+}
+
 // Builds the BVH, light collection and light BVH of s->h and uploads everything the kernels
 // read. Called at creation and after vertex updates (buffers are reused when large enough; a
 // frozen sampling copy of the materials stays in place).
 int buildAndUpload(FhpScene_* s) {
     HostScene& h = s->h;
+    if (int rc = waitForDevice()) return rc;
     try {
         buildBVH4(h, s->leafSize);
         buildLights(h);
@@ -362,13 +379,7 @@ int buildAndUpload(FhpScene_* s) {
     } catch (const std::exception& e) {
         return fail(FHP_EBUILD, e.what());
     }
-    // Vertex records: (p.xyz, n.x) (n.yz, uv).
-    std::vector<float> vtx(8 * size_t(h.nV));
-    for (uint32_t i = 0; i < h.nV; i++) {
-        float* q = &vtx[8 * size_t(i)];
-        q[0] = h.pos[3 * i]; q[1] = h.pos[3 * i + 1]; q[2] = h.pos[3 * i + 2]; q[3] = h.nrm[3 * i];
-        q[4] = h.nrm[3 * i + 1]; q[5] = h.nrm[3 * i + 2]; q[6] = h.uv[2 * i]; q[7] = h.uv[2 * i + 1];
-    }
+    const std::vector<float> vtx = packVertexRecords(h);
     HIP_OR_FAIL(s->bvh.upload(h.bvh));
     HIP_OR_FAIL(s->triPos.upload(h.triPos));
     HIP_OR_FAIL(s->idx.upload(h.idx));
@@ -391,6 +402,10 @@ int buildAndUpload(FhpScene_* s) {
     s->stackNeed = h.stackNeed + 1;
     const uint32_t worst = s->stackNeed > kMinLds ? s->stackNeed - kMinLds : 0u;
     if (worst) HIP_OR_FAIL(s->ovf.alloc(size_t(worst) * uint32_t(numCUs()) * kMaxThreadsPerCU * 4));
+    if (!s->err.p) {
+        HIP_OR_FAIL(s->err.alloc(16));
+        HIP_OR_FAIL(hipMemset(s->err.p, 0, 16));
+    }
     s->ts = traceConfig(s, kDefaultBlock, kDefaultLds, kDefaultBlocksPerCU);
 
     SceneDev& dv = s->dev;
@@ -490,58 +505,9 @@ const char* fhp_last_warning(void) { return g_warning.c_str(); }
 // ------------------------------------------------------------------ scene
 int fhp_scene_create(const FhpSceneDesc* d, FhpScene* out) {
     if (!d || !out) return fail(FHP_EINVAL, "null argument");
-    if (!d->h_positions || !d->h_normals || !d->h_texcoords || !d->h_indices || !d->h_tri_material || !d->h_materials ||
-        !d->h_camera)
-        return fail(FHP_EINVAL, "scene description has a null array");
-    if (d->num_triangles == 0 || d->num_vertices == 0) return fail(FHP_EBUILD, "scene has no triangles");
-    if (d->num_textures && (!d->h_tex_desc || !d->h_texels)) return fail(FHP_EINVAL, "texture arrays missing");
     std::unique_ptr<FhpScene_> s(new FhpScene_());
-    HostScene& h = s->h;
-    h.nV = d->num_vertices; h.nT = d->num_triangles; h.nM = d->num_materials; h.nTex = d->num_textures;
-    h.pos.assign(d->h_positions, d->h_positions + 3 * size_t(h.nV));
-    h.nrm.assign(d->h_normals, d->h_normals + 3 * size_t(h.nV));
-    h.restPos = h.pos;
-    h.restNrm = h.nrm;
-    h.uv.assign(d->h_texcoords, d->h_texcoords + 2 * size_t(h.nV));
-    h.idx.assign(d->h_indices, d->h_indices + 3 * size_t(h.nT));
-    h.triMat.assign(d->h_tri_material, d->h_tri_material + h.nT);
-    h.mats.assign(d->h_materials, d->h_materials + size_t(h.nM) * M_STRIDE);
-    for (uint32_t i : h.idx)
-        if (i >= h.nV) return fail(FHP_EINVAL, "vertex index out of range");
-    for (uint32_t m : h.triMat)
-        if (m >= h.nM) return fail(FHP_EINVAL, "material index out of range");
-    if (h.nTex) {
-        h.texDesc.assign(d->h_tex_desc, d->h_tex_desc + 4 * size_t(h.nTex));
-        h.texels.assign(d->h_texels, d->h_texels + d->num_texels_floats);
-        for (uint32_t t = 0; t < h.nTex; t++) {
-            int64_t off = h.texDesc[4 * t], w = h.texDesc[4 * t + 1], hh = h.texDesc[4 * t + 2];
-            if (off < 0 || w <= 0 || hh <= 0 || uint64_t(off + 3 * w * hh) > d->num_texels_floats)
-                return fail(FHP_EINVAL, "texture descriptor out of range");
-        }
-    }
-    for (uint32_t m = 0; m < h.nM; m++) {
-        int tex = int(h.mats[size_t(m) * M_STRIDE + M_TEX]);
-        if (tex >= int(h.nTex)) return fail(FHP_EINVAL, "material texture index out of range");
-        if (int(h.mats[size_t(m) * M_STRIDE + M_EMIT_TEX]) - 1 >= int(h.nTex))
-            return fail(FHP_EINVAL, "material emissive texture index out of range");
-    }
-    std::memcpy(h.cam, d->h_camera, sizeof(h.cam));
-    if (d->num_meshes) {
-        if (!d->h_mesh_vertex_offsets) return fail(FHP_EINVAL, "mesh vertex offsets missing");
-        h.meshOff.assign(d->h_mesh_vertex_offsets, d->h_mesh_vertex_offsets + d->num_meshes + 1);
-        if (h.meshOff.front() != 0 || h.meshOff.back() != h.nV) return fail(FHP_EINVAL, "mesh vertex offsets must span [0, num_vertices]");
-        for (uint32_t m = 0; m < d->num_meshes; m++)
-            if (h.meshOff[m] > h.meshOff[m + 1]) return fail(FHP_EINVAL, "mesh vertex offsets must not decrease");
-    } else {
-        h.meshOff = {0u, h.nV};
-    }
-    if (d->flags & ~uint32_t(FHP_SCENE_USE_COMPRESSED_HIT_INFO)) return fail(FHP_EINVAL, "unknown scene flags");
-    try {
-        computeCamera(h);
-        computeHitInfo(h, (d->flags & FHP_SCENE_USE_COMPRESSED_HIT_INFO) != 0);
-    } catch (const std::exception& e) {
-        return fail(FHP_EBUILD, e.what());
-    }
+    std::string msg;
+    if (int rc = sceneFromDesc(*d, s->h, msg)) return fail(rc, msg);
     int rc = buildAndUpload(s.get());
     if (rc) return rc;
     *out = s.release();
@@ -760,6 +726,7 @@ int fhp_scene_set_analytic_lights(FhpScene s, uint32_t n, const float* h_lights)
         r[8] = q[8] / dl; r[9] = q[9] / dl; r[10] = q[10] / dl; r[11] = std::cos(opening);
         r[12] = penumbra;
     }
+    if (int rc = waitForDevice()) return rc;
     HIP_OR_FAIL(s->alights.upload(rec));
     dv.alights = s->alights.as<float4>();
     dv.nALights = n;
@@ -780,6 +747,7 @@ int fhp_scene_set_env_map(FhpScene s, const float* h_texels, uint32_t W, uint32_
     const size_t n = 3 * size_t(W) * H;
     for (size_t i = 0; i < n; i++)
         if (!std::isfinite(h_texels[i])) return fail(FHP_EINVAL, "non-finite env map texel");
+    if (int rc = waitForDevice()) return rc;
     HIP_OR_FAIL(s->envTex.alloc(n * 4));
     HIP_OR_FAIL(hipMemcpy(s->envTex.p, h_texels, n * 4, hipMemcpyHostToDevice));
     // h_params: intensity, tint rgb, rotation in degrees xyz. EnvMap::setRotation (EnvMap.cpp:62-72):
@@ -839,7 +807,20 @@ int fhp_env_eval_pdf(FhpScene s, const float* d_dirs, uint32_t n, float* d_pdf, 
 }
 
 int fhp_scene_destroy(FhpScene s) {
+    if (!s) return FHP_OK;
+    const int rc = waitForDevice();
     delete s;
+    return rc;
+}
+
+int fhp_scene_trace_errors(FhpScene s, uint32_t* bits, int reset, FhpStream stream) {
+    if (!s || !bits) return fail(FHP_EINVAL, "null argument");
+    hipStream_t st = hipStream_t(stream);
+    uint32_t v = 0;
+    HIP_OR_FAIL(hipMemcpyAsync(&v, s->err.p, 4, hipMemcpyDeviceToHost, st));
+    HIP_OR_FAIL(hipStreamSynchronize(st));
+    if (reset && v) HIP_OR_FAIL(launchFill(s->err.as<float>(), 1, 0.f, st));
+    *bits = v;
     return FHP_OK;
 }
 
@@ -877,6 +858,7 @@ int fhp_scene_set_materials(FhpScene s, const float* mats) {
         if (int(mats[size_t(m) * M_STRIDE + M_EMIT_TEX]) - 1 >= int(s->h.nTex))
             return fail(FHP_EINVAL, "material emissive texture index out of range");
     }
+    if (int rc = waitForDevice()) return rc;
     s->h.mats.assign(mats, mats + size_t(s->h.nM) * M_STRIDE);
     HIP_OR_FAIL(hipMemcpy(s->mats.p, mats, size_t(s->h.nM) * M_STRIDE * 4, hipMemcpyHostToDevice));
     // Light samples evaluate the emitters' current materials (EmissiveLightSamplerHelpers.slang:79).
@@ -955,6 +937,8 @@ int fhp_scene_freeze_sampling(FhpScene s, int enable, FhpStream stream) {
         s->dev.texelsS = s->dev.texels;
         return FHP_OK;
     }
+    if (s->matsS.bytes < s->mats.bytes || s->texelsS.bytes < s->texels.bytes)
+        if (int rc = waitForDevice()) return rc;  // the frozen copies are reallocated
     HIP_OR_FAIL(s->matsS.alloc(s->mats.bytes));
     HIP_OR_FAIL(hipMemcpyAsync(s->matsS.p, s->mats.p, s->mats.bytes, hipMemcpyDeviceToDevice, st));
     HIP_OR_FAIL(s->texelsS.alloc(s->texels.bytes));
@@ -1113,7 +1097,8 @@ int fhp_pass_execute(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t spp
     const bool count = p->get("countTraversal") != 0;
     unsigned long long* dstats = p->stats.as<unsigned long long>();
     if (h_stats) HIP_OR_FAIL(launchFill(reinterpret_cast<float*>(dstats), 12, 0.f, st));
-    const TraceScratch ts = passTrace(s, p);
+    TraceScratch ts;
+    if ((rc = passTrace(s, p, ts))) return rc;
     if (p->kind == "MinimalPathTracer") {
         MptArgs a{};
         a.W = W; a.H = H; a.spp = spp; a.frame = frame;
@@ -1189,7 +1174,7 @@ int fhp_pass_execute(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t spp
         a.tmp = p->Lp.as<float4>();
         a.rays = h_stats ? dstats : nullptr;
         if ((rc = prepareTimer(p))) return rc;
-        HIP_OR_FAIL(launchWarPrimal(s->dev, a, passTrace(s, p), st, &p->timer));
+        HIP_OR_FAIL(launchWarPrimal(s->dev, a, ts, st, &p->timer));
         if ((rc = collectTimer(p, st))) return rc;
     }
     if (h_stats) {
@@ -1280,7 +1265,9 @@ int fhp_pass_execute_batch(FhpPass p, uint32_t nv, const FhpView* views, uint64_
     a.pixHit = p->pixHit.as<uint4>();
     a.counters = p->counters.as<uint32_t>();
     if ((rc = prepareTimer(p))) return rc;
-    HIP_OR_FAIL(launchMptBatch(a, passTrace(deepest, p), count, h_stats ? dstats : nullptr, st, &p->timer));
+    TraceScratch ts;
+    if ((rc = passTrace(deepest, p, ts))) return rc;
+    HIP_OR_FAIL(launchMptBatch(a, ts, count, h_stats ? dstats : nullptr, st, &p->timer));
     if ((rc = collectTimer(p, st))) return rc;
     if (h_stats) {
         unsigned long long hs[6];
@@ -1387,6 +1374,8 @@ int fhp_pass_backward(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t sp
     }
     if (g->scene != s) return fail(FHP_EINVAL, "gradients belong to another scene");
     if (W == 0 || H == 0 || spp == 0) return fail(FHP_EINVAL, "empty frame");
+    if (W > 65536 || H > 65536) return fail(FHP_EINVAL, "frame dimension above 65536");
+    if (uint64_t(W) * H * spp > 0xffffffffull / 3) return fail(FHP_EINVAL, "frame too large (width * height * spp paths)");
     const bool war = p->get("useWAR") != 0;
     if (war) {
         if (p->get("maxBounces") != 0)
@@ -1417,7 +1406,9 @@ int fhp_pass_backward(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t sp
     a.gradTex = g->withTex ? g->tex.as<float>() : nullptr;
     a.rays = h_stats ? dstats : nullptr;
     if ((rc = prepareTimer(p))) return rc;
-    HIP_OR_FAIL(launchWarAdjoint(s->dev, a, passTrace(s, p), st, &p->timer));
+    TraceScratch ts;
+    if ((rc = passTrace(s, p, ts))) return rc;
+    HIP_OR_FAIL(launchWarAdjoint(s->dev, a, ts, st, &p->timer));
     if (war) {
         GeomArgs ga{};
         ga.W = W; ga.H = H; ga.spp = spp; ga.frame = a.frame;
@@ -1431,7 +1422,7 @@ int fhp_pass_backward(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t sp
         ga.gradNrm = g->nrmTmp.as<float>();
         ga.hashSize = g->hashSize; ga.meshV0 = g->meshV0; ga.meshVerts = g->meshVerts;
         ga.rays = h_stats ? dstats + 2 : nullptr;
-        HIP_OR_FAIL(launchWarGeom(s->dev, ga, passTrace(s, p), st, &p->timer));
+        HIP_OR_FAIL(launchWarGeom(s->dev, ga, ts, st, &p->timer));
     }
     if ((rc = collectTimer(p, st))) return rc;
     if (h_stats) {
@@ -1564,8 +1555,10 @@ int fhp_vbuffer_execute(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t*
         h_jitter[0] = jx;
         h_jitter[1] = jy;
     }
-    HIP_OR_FAIL(launchVBuffer(s->dev, W, H, jx, jy, d_vbuffer, d_depth, reinterpret_cast<float4*>(d_viewW), d_mask,
-                              passTrace(s, p), hipStream_t(stream)));
+    TraceScratch ts;
+    if (int rc = passTrace(s, p, ts)) return rc;
+    HIP_OR_FAIL(launchVBuffer(s->dev, W, H, jx, jy, d_vbuffer, d_depth, reinterpret_cast<float4*>(d_viewW), d_mask, ts,
+                              hipStream_t(stream)));
     return FHP_OK;
 }
 

@@ -68,8 +68,12 @@ struct MptArgs {
     uint32_t keepMisses;  // 1: closest-hit misses are queued for shading too (environment light)
 };
 
+// Bits of a scene's trace error word (TraceScratch::err), read back by fhp_scene_trace_errors.
+constexpr uint32_t kTraceErrStackOverflow = 1u;
+
 struct TraceScratch {
     uint32_t* ovf;     // overflow stack (ovfDepth words per thread of the persistent grid)
+    uint32_t* err;     // trace error word of the scene (kTraceErr* bits), or null
     uint32_t ovfDepth;
     uint32_t gridBlocks;  // persistent grid size in blocks
     uint32_t block;       // threads per block (64, 128, 256)

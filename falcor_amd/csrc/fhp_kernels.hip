@@ -23,11 +23,6 @@ __device__ __forceinline__ uint32_t waveAppend(bool pred, uint32_t* counter) {
     return base + uint32_t(__popcll(mask & __lanemask_lt()));
 }
 
-template <int B, int D> __device__ __forceinline__ TraceStack makeStack(uint32_t* lds, const TraceScratch& ts) {
-    const uint32_t gtid = blockIdx.x * B + threadIdx.x;
-    return TraceStack{(LdsU32*)(lds + threadIdx.x), uint32_t(B), uint32_t(D), (GlobalU32*)(ts.ovf + gtid), ts.gridBlocks * B,
-                      ts.ovfDepth};
-}
 
 int numCUs() {
     static int cus = 0;
@@ -44,7 +39,7 @@ template <int B, int D, bool ANY>
 __global__ __launch_bounds__(B) void kTraceRays(SceneDev s, const float4* __restrict__ rays, uint32_t n,
                                                 uint4* __restrict__ hits, uint32_t* __restrict__ occl, TraceScratch ts) {
     __shared__ uint32_t lds[D * B];
-    TraceStack st = makeStack<B, D>(lds, ts);
+    TraceStack st = makeStack<B, D>(lds, ts, blockIdx.x, threadIdx.x);
     const uint32_t G = gridDim.x * B;
     for (uint32_t i = blockIdx.x * B + threadIdx.x; i < n; i += G) {
         const float4 a = rays[2 * i], b = rays[2 * i + 1];
@@ -186,7 +181,7 @@ __global__ __launch_bounds__(B) void kVBuffer(SceneDev s, uint32_t W, uint32_t H
                                               float* __restrict__ depth, float4* __restrict__ viewW,
                                               float* __restrict__ mask, TraceScratch ts) {
     __shared__ uint32_t lds[D * B];
-    TraceStack st = makeStack<B, D>(lds, ts);
+    TraceStack st = makeStack<B, D>(lds, ts, blockIdx.x, threadIdx.x);
     const uint32_t n = W * H;
     const uint32_t G = gridDim.x * B;
     const uint32_t words = s.hit.compressed ? 2u : 4u;
@@ -269,7 +264,7 @@ __global__ __launch_bounds__(B) void kTraceQueue(SceneDev s, const float4* __res
                                                  uint32_t countConst, uint4* __restrict__ hits, TraceScratch ts,
                                                  unsigned long long* stats) {
     __shared__ uint32_t lds[D * B];
-    TraceStack st = makeStack<B, D>(lds, ts);
+    TraceStack st = makeStack<B, D>(lds, ts, blockIdx.x, threadIdx.x);
     const uint32_t n = countPtr ? *countPtr : countConst;
     const uint32_t G = gridDim.x * B;
     TraceCounters cnt{0, 0};
@@ -549,7 +544,7 @@ template <int B, int D, bool COUNT>
 __global__ __launch_bounds__(B) void kShadow(SceneDev s, MptArgs a, uint32_t slotIdx, TraceScratch ts,
                                              unsigned long long* stats) {
     __shared__ uint32_t lds[D * B];
-    TraceStack st = makeStack<B, D>(lds, ts);
+    TraceStack st = makeStack<B, D>(lds, ts, blockIdx.x, threadIdx.x);
     const uint32_t n = a.counters[kCounterStride + slotIdx];
     const uint32_t G = gridDim.x * B;
     TraceCounters cnt{0, 0};
@@ -592,7 +587,7 @@ __global__ __launch_bounds__(B) void kTraceBounce(SceneDev s, MptArgs a, const f
                                                   const uint32_t* __restrict__ pathIds, TraceScratch ts,
                                                   unsigned long long* stats) {
     __shared__ uint32_t lds[D * B];
-    TraceStack st = makeStack<B, D>(lds, ts);
+    TraceStack st = makeStack<B, D>(lds, ts, blockIdx.x, threadIdx.x);
     const uint32_t nS = nSPtr ? *nSPtr : 0u;
     const uint32_t n = nS + (nCPtr ? *nCPtr : nCConst);
     const uint32_t G = gridDim.x * B;

@@ -8,6 +8,17 @@
 
 #define FHP_HD __host__ __device__ __forceinline__
 
+// Per-path device code (traversal, the WARDiffPathTracer paths) is __device__ in the library.
+// The host check build (FHP_HOST_CHECK, a CPU test build under AddressSanitizer) compiles the
+// same code for the CPU, with single-lane traversal stacks and plain gradient adds.
+#ifdef FHP_HOST_CHECK
+#define FHP_DEV __host__ __device__
+#define FHP_DEVI __host__ __device__ __forceinline__
+#else
+#define FHP_DEV __device__
+#define FHP_DEVI __device__ __forceinline__
+#endif
+
 namespace fhp {
 
 struct f3 {
@@ -46,6 +57,16 @@ constexpr float kFltMin = 1.175494351e-38f;
 constexpr uint32_t kInvalid = 0xffffffffu;
 
 FHP_HD uint32_t f2u(float f) { return __builtin_bit_cast(uint32_t, f); }
+FHP_HD uint32_t popc32(uint32_t x) { return uint32_t(__builtin_popcount(x)); }
+// Gradient accumulation into a hash-grid row: a float atomic on the GPU, a plain add in the
+// single-threaded host check build.
+FHP_DEVI void gradAdd(float* p, float v) {
+#ifdef FHP_HOST_CHECK
+    *p += v;
+#else
+    atomicAdd(p, v);
+#endif
+}
 FHP_HD float u2f(uint32_t u) { return __builtin_bit_cast(float, u); }
 
 // ------------------------------------------------------------------ PRNG

@@ -9,7 +9,7 @@
 // the smallest t wins, equal t resolves to the lowest triangle id. Triangles use the watertight
 // test, so the answer is the same whatever the tree shape.
 #pragma once
-#include "fhp_scene.h"
+#include "fhp_internal.h"
 
 namespace fhp {
 
@@ -21,9 +21,18 @@ struct HitRec {
 // Address-space-qualified pointers keep stack traffic on ds_read/ds_write (LDS) and
 // global_load/global_store (overflow); through generic pointers the compiler merges the two
 // paths into flat accesses, which are slower for LDS.
+#ifdef FHP_HOST_CHECK
+typedef uint32_t LdsU32;
+typedef uint32_t GlobalU32;
+#else
 typedef __attribute__((address_space(3))) uint32_t LdsU32;
 typedef __attribute__((address_space(1))) uint32_t GlobalU32;
+#endif
 
+// A traversal stack of capacity() = ldsDepth + ovfDepth entries. The scene builder sizes it to
+// the tree's exact worst case, and every push is checked against the capacity anyway: a ray
+// that would overflow ends without a hit and sets the scene's trace error word, so a stale
+// bound fails loudly instead of writing out of bounds.
 struct TraceStack {
     LdsU32* lds;         // this lane's slot 0; entry i at lds[i * ldsStride]
     uint32_t ldsStride;  // threads per block
@@ -31,15 +40,37 @@ struct TraceStack {
     GlobalU32* ovf;      // this thread's overflow slot 0; entry i at ovf[i * ovfStride]
     uint32_t ovfStride;
     uint32_t ovfDepth;
-    __device__ __forceinline__ void put(uint32_t i, uint32_t v) {
+    uint32_t* err;       // trace error word (bit 0: stack overflow), or null
+#ifdef FHP_HOST_CHECK
+    uint32_t peak = 0;   // entries in use at the deepest push
+#endif
+    FHP_DEVI void put(uint32_t i, uint32_t v) {
+#ifdef FHP_HOST_CHECK
+        if (i + 1 > peak) peak = i + 1;
+#endif
         if (i < ldsDepth) lds[i * ldsStride] = v;
         else ovf[(i - ldsDepth) * ovfStride] = v;
     }
-    __device__ __forceinline__ uint32_t get(uint32_t i) const {
+    FHP_DEVI uint32_t get(uint32_t i) const {
         return i < ldsDepth ? lds[i * ldsStride] : ovf[(i - ldsDepth) * ovfStride];
     }
-    __device__ __forceinline__ uint32_t capacity() const { return ldsDepth + ovfDepth; }
+    FHP_DEVI uint32_t capacity() const { return ldsDepth + ovfDepth; }
+    FHP_DEVI void overflowed() const {
+#ifdef FHP_HOST_CHECK
+        if (err) *err |= kTraceErrStackOverflow;
+#else
+        if (err) atomicOr(err, kTraceErrStackOverflow);
+#endif
+    }
 };
+
+// Stack of one lane of a persistent traversal grid: LDS entries [entry][lane] in the block's
+// array, overflow entries [entry][global thread] in the scene's overflow buffer.
+template <int B, int D> FHP_DEVI TraceStack makeStack(uint32_t* lds, const TraceScratch& ts, uint32_t block, uint32_t lane) {
+    const uint32_t gtid = block * B + lane;
+    return TraceStack{(LdsU32*)(lds + lane), uint32_t(B), uint32_t(D), (GlobalU32*)(ts.ovf + gtid), ts.gridBlocks * B,
+                      ts.ovfDepth, ts.err};
+}
 
 // The arrays traversal reads: a SceneDev, or this pair when the scene varies per ray.
 struct GeomRef {
@@ -53,7 +84,7 @@ struct TraceCounters {
 
 // Slab test of the 4 children of one node. Returns a 4-bit hit mask and entry distances.
 // Conservative: the exit distance is scaled by (1 + 4 ulp) and boxes are padded at build time.
-__device__ __forceinline__ uint32_t testNode4(const float4* __restrict__ node, f3 o, f3 inv, float tmin, float tmax,
+FHP_DEVI uint32_t testNode4(const float4* __restrict__ node, f3 o, f3 inv, float tmin, float tmax,
                                               float tn[4], uint4& child) {
     const float4 lx = node[0], hx = node[1], ly = node[2], hy = node[3], lz = node[4], hz = node[5];
     const float4 cb = node[6];
@@ -78,7 +109,7 @@ __device__ __forceinline__ uint32_t testNode4(const float4* __restrict__ node, f
 }
 
 // Compare-exchange on (distance, child) pairs: afterwards ta <= tb.
-__device__ __forceinline__ void sortPair(float& ta, uint32_t& ca, float& tb, uint32_t& cb) {
+FHP_DEVI void sortPair(float& ta, uint32_t& ca, float& tb, uint32_t& cb) {
     const bool sw = tb < ta;
     const float t = sw ? tb : ta;
     const uint32_t c = sw ? cb : ca;
@@ -108,7 +139,7 @@ struct TravState {
     HitRec hit;
     uint32_t cur, sp;
     bool any;
-    __device__ __forceinline__ void start(f3 org, f3 dir, float t0, float t1, bool anyHit) {
+    FHP_DEVI void start(f3 org, f3 dir, float t0, float t1, bool anyHit) {
         o = org;
         d = dir;
         inv = safeInv(dir);
@@ -121,7 +152,7 @@ struct TravState {
         any = anyHit;
     }
     template <bool COUNT, typename SceneT>
-    __device__ __forceinline__ bool step(const SceneT& s, TraceStack& st, TraceCounters* cnt) {
+    FHP_DEVI bool step(const SceneT& s, TraceStack& st, TraceCounters* cnt) {
         if (!childIsLeaf(cur)) {
             const float4* node = s.bvh + 8 * size_t(cur);
             float tn[4];
@@ -141,8 +172,15 @@ struct TravState {
                 sortPair(t1, c1, t3, c3);
                 sortPair(t1, c1, t2, c2);
                 // The scene builder sizes the stack to the tree's exact worst case
-                // (sum of (children - 1) along any root-to-leaf path), so sp stays in range.
-                const uint32_t k = __popc(mask);
+                // (sum of (children - 1) along any root-to-leaf path); the check below only
+                // fires when that bound is broken (stale tree or launch configuration).
+                const uint32_t k = popc32(mask);
+                if (sp + k - 1u > st.capacity()) {
+                    st.overflowed();
+                    hit.tri = kInvalid;
+                    sp = 0;
+                    return false;
+                }
                 if (k > 3u) st.put(sp++, c3);
                 if (k > 2u) st.put(sp++, c2);
                 if (k > 1u) st.put(sp++, c1);
@@ -174,7 +212,7 @@ struct TravState {
 // Closest hit (ANY = false) or first hit (ANY = true) in [tmin, tmax]. Returns true on a hit.
 // `anyRt` selects first-hit per lane at run time (kernels that trace both kinds of ray).
 template <bool ANY, bool COUNT, typename SceneT>
-__device__ __forceinline__ bool traceBVH4(const SceneT& s, f3 o, f3 d, float tmin, float tmax, TraceStack& st,
+FHP_DEVI bool traceBVH4(const SceneT& s, f3 o, f3 d, float tmin, float tmax, TraceStack& st,
                                           HitRec& hit, TraceCounters* cnt = nullptr, bool anyRt = false) {
     TravState r;
     r.start(o, d, tmin, tmax, ANY || anyRt);

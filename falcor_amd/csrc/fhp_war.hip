@@ -1,290 +1,10 @@
-// WARDiffPathTracer primal and adjoint kernels (one path per thread).
-//
-// Primal: WARDiffPathTracer.rt.slang:75-143 with PTUtils.slang:97-341 (uniform emissive NEE,
-// MIS, BSDF sampling, Gaussian pixel filter with antithetic jitter), useWAR = false.
-//
-// Adjoint: path replay backpropagation. Each thread first traces its path to get the path
-// radiance L, then replays it from the same seed. Material parameters enter the path only
-// through BSDF evaluations f and emission E, with sampling, pdfs and MIS weights detached as in
-// the reference (PTUtils.slang:136-249, StandardBSDF.slang:155-178 `detach`). With
-// S = dLdI . L and R_j = radiance gathered after vertex j,
-//   dS/df_j(w_j) = dLdI * R_j / f_j(w_j)          (scatter throughput factor)
-//   dS/df_j(w_l) = dLdI * thp_j * Li * mis        (NEE)
-//   dS/dE_j      = dLdI * thp_j * mis             (emission)
-// R_j is obtained as L - (radiance accumulated up to vertex j), so no vertex is stored.
-// Parameter gradients are routed like GradientIOWrapper.slang:48-116: 20 floats per material
-// (StandardMaterialParamLayout.slang:32-41), hash row jenkinsHash(pixelID) % hashSize; the
-// per-texel baseColor gradient (texture extension) uses the bilinear tap weights.
-#include "fhp_internal.h"
-#include "fhp_trace.h"
+// WARDiffPathTracer primal and adjoint kernels (one path per thread); the per-path code is in
+// fhp_war_path.h.
+#include "fhp_war_path.h"
 
 namespace fhp {
 
 namespace {
-
-// Gaussian pixel jitter (WARDiffPathTracer.rt.slang:60-63).
-__device__ __forceinline__ void gaussian2(float ux, float uy, float sigma, float& gx, float& gy) {
-    float r = sqrtf(-2.f * logf(ux));
-    gx = (r * cosf(kTwoPi * uy)) * sigma;
-    gy = (r * sinf(kTwoPi * uy)) * sigma;
-}
-
-// Local gradient of one vertex's material parameters.
-struct MatGrad {
-    f3 base;
-    float metal, rough, ior;
-    f3 emit;
-    float emitFactor;
-};
-__device__ __forceinline__ void zero(MatGrad& g) {
-    g.base = mk3(0.f); g.metal = g.rough = g.ior = 0.f; g.emit = mk3(0.f); g.emitFactor = 0.f;
-}
-
-// Backpropagates adj (dS/df, RGB) through f = material BSDF eval(wi, wo) into material params.
-__device__ void bsdfAdjoint(const Shading& sd, const Material& m, f3 wo, f3 adj, MatGrad& g) {
-    f3 wiL = sd.toLocal(sd.V), woL = sd.toLocal(wo);
-    if (m.type == MAT_PBRT_DIFFUSE) {
-        if (!validR(sd, wiL, woL, wo)) return;
-        float k = fmaxf_(woL.z, 0.f) * kInvPi;
-        g.base = g.base + adj * k;
-        return;
-    }
-    if (!validRT(sd, wiL, woL, wo)) return;
-    Bsdf b;
-    b.init(wiL, m);
-    f3 dDiff = mk3(0.f), dSpec = mk3(0.f);
-    float dRough = 0.f;
-    if (b.pDiff > 0.f && fminf_(wiL.z, woL.z) >= kMinCosTheta) {
-        // Frostbite: f = diff * wiS * woS * eF / pi * wo.z
-        f3 h = normalize(wiL + woL);
-        float woDotH = dot(woL, h);
-        float r = b.rough;
-        float eF = 1.f + (1.f / 1.51f - 1.f) * r;
-        float fd90 = 0.5f * r + 2.f * woDotH * woDotH * r;
-        float wi5 = pow5(fmaxf_(1.f - wiL.z, 0.f)), wo5 = pow5(fmaxf_(1.f - woL.z, 0.f));
-        float wiS = 1.f + (fd90 - 1.f) * wi5, woS = 1.f + (fd90 - 1.f) * wo5;
-        float c = kInvPi * woL.z;
-        float K = wiS * woS * eF * c;
-        dDiff = adj * K;
-        float dfd90 = 0.5f + 2.f * woDotH * woDotH;
-        float dProd = (dfd90 * wi5) * woS * eF + wiS * (dfd90 * wo5) * eF + wiS * woS * (1.f / 1.51f - 1.f);
-        dRough += dot(adj, b.diff) * dProd * c;
-    }
-    if (b.pSpec > 0.f && b.alpha > 0.f && fminf_(wiL.z, woL.z) >= kMinCosTheta) {
-        f3 h = normalize(wiL + woL);
-        float wiDotH = dot(wiL, h);
-        float D = ggxD(b.alpha, h.z), G = ggxG(b.alpha, wiL.z, woL.z);
-        float w5 = pow5(fmaxf_(1.f - wiDotH, 0.f));
-        float s = D * G * 0.25f / wiL.z;
-        dSpec = adj * ((1.f - w5) * s);
-        // d/d(alpha^2) of D and G.
-        float a2 = b.alpha * b.alpha, cz = h.z;
-        float d = (cz * a2 - cz) * cz + 1.f;
-        float dD = (d - 2.f * a2 * cz * cz) / (kPi * d * d * d);
-        auto dLam = [&](float c0) {
-            float c2 = c0 * c0;
-            float t2 = fmaxf_(1.f - c2, 0.f) / c2;
-            return t2 / (4.f * sqrtf(1.f + a2 * t2));
-        };
-        float dG = -G * G * (dLam(wiL.z) + dLam(woL.z));
-        f3 F = schlick3(b.spec, 1.f, wiDotH);
-        float dA2 = (dD * G + D * dG) * 0.25f / wiL.z;
-        float r = b.rough;
-        dRough += dot(adj, F) * dA2 * (4.f * r * r * r);
-    }
-    // diffuse = base (1 - metal); specular = F0 + (base - F0) metal.
-    g.base = g.base + dDiff * (1.f - m.metallic) + dSpec * m.metallic;
-    g.metal += -dot(dDiff, m.base) + dot(dSpec, m.base - mk3(m.F0));
-    g.rough += dRough;
-    // dS/dF0, converted to dS/dior = dS/dF0 * dF0/dior when the vertex is flushed.
-    g.ior += (dSpec.x + dSpec.y + dSpec.z) * (1.f - m.metallic);
-}
-
-__device__ __forceinline__ void atomAdd(float* p, float v) {
-    if (v != 0.f) atomicAdd(p, v);
-}
-
-__device__ void flushGrad(const SceneDev& s, const WarArgs& a, uint32_t hashRow, const Material& m, uint32_t mat,
-                          const TexTap& tap, const MatGrad& g) {
-    float* row = a.gradMat + size_t(hashRow) * a.matGradDim + size_t(mat) * 20;
-    atomAdd(row + 0, g.base.x);
-    atomAdd(row + 1, g.base.y);
-    atomAdd(row + 2, g.base.z);
-    if (m.type == MAT_STANDARD) {
-        atomAdd(row + 3, g.metal);
-        atomAdd(row + 4, g.rough);
-        const float ior = s.mats[size_t(mat) * M_STRIDE + M_IOR];
-        const float f = (ior - 1.f) / (ior + 1.f);
-        const float dfdior = 2.f / ((ior + 1.f) * (ior + 1.f));
-        atomAdd(row + 5, g.ior * 2.f * f * dfdior);
-        atomAdd(row + 11, g.emit.x);
-        atomAdd(row + 12, g.emit.y);
-        atomAdd(row + 13, g.emit.z);
-        atomAdd(row + 14, g.emitFactor);
-    }
-    if (a.gradTex && m.tex >= 0) {
-        float* t = a.gradTex + tap.base;
-        for (int k = 0; k < 4; k++) {
-            float w = tap.w[k];
-            if (w == 0.f) continue;
-            float* q = t + 3 * size_t(tap.t[k]);
-            atomAdd(q + 0, g.base.x * w);
-            atomAdd(q + 1, g.base.y * w);
-            atomAdd(q + 2, g.base.z * w);
-        }
-    }
-}
-
-// One path of WARDiffPathTracer (tracePaths + tracePath). In ADJ mode, Ltot is the path radiance
-// from a previous primal run and gradients are accumulated; returns the path radiance.
-template <bool ADJ>
-__device__ f3 warPath(const SceneDev& s, const WarArgs& a, uint32_t px, uint32_t py, uint32_t frameSeed, TraceStack& st,
-                      f3 Ltot, f3 dL, uint32_t hashRow, unsigned long long& nc, unsigned long long& nv) {
-    Rng rng = rngSeed(px, py, frameSeed / 2);
-    for (int i = 0; i < 6; i++) rng.u1();
-    const float pfx = float(px) + 0.5f, pfy = float(py) + 0.5f;
-    float ux = rng.u1(), uy = rng.u1();
-    float jx, jy;
-    gaussian2(ux, uy, 0.5f, jx, jy);
-    if (a.useAntithetic && frameSeed % 2 == 0) { jx = -jx; jy = -jy; }
-    const float sx = pfx + jx, sy = pfy + jy;
-    f3 radiance = mk3(0.f), thp = mk3(1.f);
-    uint32_t len = 0;
-    f3 prevN = mk3(0.f);
-    float prevPdf = 0.f;
-    // computeCameraRayDirection (DiffSceneQuery.slang:252-261)
-    const float p_x = sx / float(a.W), p_y = sy / float(a.H);
-    const float ndc_x = 2.f * p_x + -1.f, ndc_y = -2.f * p_y + 1.f;
-    f3 dir = s.cam.U * ndc_x + s.cam.V * ndc_y + s.cam.W;
-    const float dl = length(dir);
-    dir = dl > 0.f ? dir / dl : mk3(0.f);
-    f3 rayO = s.cam.pos, rayD = dir;
-    HitRec h;
-    nc++;
-    if (!traceBVH4<false, false>(s, rayO, rayD, 0.f, kFltMax, st, h)) return radiance;
-    for (;;) {
-        // computeIntersectionAD, AttachToRay (DiffSceneQuery.slang:131-219).
-        const uint32_t i0 = s.idx[3 * h.tri], i1 = s.idx[3 * h.tri + 1], i2 = s.idx[3 * h.tri + 2];
-        f3 v0, v1, v2, n0, n1, n2;
-        float tu, tv;
-        loadVertex(s, i0, v0, n0, tu, tv);
-        loadVertex(s, i1, v1, n1, tu, tv);
-        loadVertex(s, i2, v2, n2, tu, tv);
-        f3 e1 = v1 - v0, e2 = v2 - v0;
-        f3 pvec = cross(rayD, e2);
-        float det = dot(e1, pvec);
-        f3 tvec = rayO - v0;
-        f3 qvec = cross(tvec, e1);
-        float tt = dot(e2, qvec) / det;
-        f3 p = rayO + rayD * tt;
-        f3 ep0 = v0 - p, ep1 = v1 - p, ep2 = v2 - p;
-        float A0 = length(cross(ep1, ep2)), A1 = length(cross(ep2, ep0)), A2 = length(cross(ep0, ep1));
-        float At = A0 + A1 + A2;
-        f3 bary = mk3(A0 / At, A1 / At, A2 / At);
-        f3 posW = v0 * bary.x + v1 * bary.y + v2 * bary.z;
-        f3 normalW = n0 * bary.x + n1 * bary.y + n2 * bary.z;
-        // PTUtils.slang:97-123
-        Shading sd = shadingFromHit(s, h.tri, bary.y, bary.z, rayD);
-        sd.V = -rayD;
-        setFrame(sd, normalW);
-        TexTap tap;
-        const Shading sd0 = sd;
-        Material m = loadMaterial(s, s.mats, s.texels, sd0, sd, &tap);
-        // Sampling parameters differ from the evaluated ones only in the finite-difference mode.
-        Material mS = m;
-        if (s.matsS != s.mats || s.texelsS != s.texels) {
-            Shading unused = sd0;
-            mS = loadMaterial(s, s.matsS, s.texelsS, sd0, unused, nullptr);
-        }
-        MatGrad g;
-        if (ADJ) zero(g);
-        const bool isPrimary = len == 0;
-        const bool computeEmissive = isPrimary || (!a.useNEE || a.useMIS);
-        if (computeEmissive && anyPos(m.emission)) {
-            float mis = 1.f;
-            if (!isPrimary && a.useNEE && a.useMIS) {
-                float lp = lightPdf(s, LS_UNIFORM, rayO, prevN, s.triToLight[h.tri], sd.posW, sd.orientedFaceN());
-                mis = misPower(prevPdf, lp);
-            }
-            radiance = radiance + thp * m.emission * mis;
-            if (ADJ && m.type == MAT_STANDARD && sd.frontFacing) {
-                const float* r = s.mats + size_t(sd.mat) * M_STRIDE;
-                f3 dE = dL * thp * mis;
-                if (r[M_EMIT_TEX] <= 0.f) g.emit = g.emit + dE * r[M_EMIT_FACTOR];  // textured: no colour slot
-                g.emitFactor += dot(dE, m.emitColor);
-            }
-        }
-        bool stop = len >= a.maxBounces + 1;
-        if (!stop && a.useNEE) {
-            LightSample ls;
-            if (sampleLight(s, LS_UNIFORM, posW, normalW, rng, ls)) {
-                f3 Li = ls.pdf > 0.f ? ls.Le / ls.pdf : mk3(0.f);
-                f3 lightPos = rayOriginOffset(ls.posW, ls.normalW);
-                f3 origin = rayOriginOffset(posW, normalW);
-                f3 toL = lightPos - origin;
-                float distance = length(toL);
-                f3 ldir = normalize(toL);
-                if (anyPos(Li)) {
-                    HitRec sh;
-                    nv++;
-                    if (!traceBVH4<true, false>(s, origin, ldir, 0.f, distance, st, sh)) {
-                        // computeNEE (PTUtils.slang:211-249)
-                        f3 wo = ls.posW - posW;
-                        float dist = length(wo);
-                        wo = wo / dist;
-                        float a2s = (dist * dist) / dot(ls.normalW, -wo);
-                        f3 f = matEvalGated(sd, m, mS, wo);
-                        f3 w = Li * f * a2s / a2s;
-                        float misN = 1.f;
-                        if (a.useMIS) {
-                            float sp;
-                            if (a.useBSDFSampling || m.type == MAT_PBRT_DIFFUSE) sp = matPdf(sd, mS, ldir);
-                            else {
-                                f3 wiL = sd.toLocal(sd.V), woL = sd.toLocal(ldir);
-                                sp = (validRT(sd, wiL, woL, ldir) && fminf_(wiL.z, woL.z) >= kMinCosTheta) ? woL.z * kInvPi : 0.f;
-                            }
-                            misN = misPower(ls.pdf, sp);
-                            w = w * misN;
-                        }
-                        radiance = radiance + thp * w;
-                        if (ADJ) bsdfAdjoint(sd, m, wo, dL * thp * Li * (a2s / a2s) * misN, g);
-                    }
-                }
-            }
-        }
-        bool cont = false;
-        f3 fS = mk3(0.f), woS = mk3(0.f);
-        if (!stop) {
-            // generateScatterRay (PTUtils.slang:136-172)
-            Scatter sc;
-            if (matSample(sd, mS, rng, sc, a.useBSDFSampling != 0) && sc.pdf > 0.f) {
-                rayD = sc.wo;
-                // PTUtils.slang:153: the view side, also for transmitted directions.
-                rayO = rayOriginOffset(posW, sd.frontFacing ? sd.faceN : -sd.faceN);
-                fS = matEvalGated(sd, m, mS, rayD);
-                woS = rayD;
-                thp = thp * (fS / sc.pdf);
-                prevN = sd.orientedFaceN();
-                prevPdf = sc.pdf;
-                cont = anyPos(thp);
-            }
-        }
-        if (ADJ) {
-            if (cont) {
-                f3 R = Ltot - radiance;
-                f3 adj = mk3(fS.x != 0.f ? dL.x * R.x / fS.x : 0.f, fS.y != 0.f ? dL.y * R.y / fS.y : 0.f,
-                             fS.z != 0.f ? dL.z * R.z / fS.z : 0.f);
-                bsdfAdjoint(sd, m, woS, adj, g);
-            }
-            flushGrad(s, a, hashRow, m, sd.mat, tap, g);
-        }
-        if (!cont) break;
-        len++;
-        nc++;
-        if (!traceBVH4<false, false>(s, rayO, rayD, 0.f, kFltMax, st, h)) break;
-    }
-    return radiance;
-}
 
 // The megakernels hold a whole path (and its adjoint) per lane: unconstrained they take 147 / 183
 // VGPRs (3 / 2 waves per SIMD) and latency hiding suffers. Capping them at 96 VGPRs (5 waves per
@@ -296,29 +16,11 @@ template <int B, int D, bool ADJ>
 __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(kWarWavesPerEU))) void kWar(SceneDev s, WarArgs a,
                                                                                                  TraceScratch ts) {
     __shared__ uint32_t lds[D * B];
-    TraceStack st{(LdsU32*)(lds + threadIdx.x), uint32_t(B), uint32_t(D), (GlobalU32*)(ts.ovf + blockIdx.x * B + threadIdx.x),
-                  ts.gridBlocks * B, ts.ovfDepth};
+    TraceStack st = makeStack<B, D>(lds, ts, blockIdx.x, threadIdx.x);
     const uint32_t n = a.W * a.H * a.spp;
     const uint32_t G = gridDim.x * B;
     unsigned long long nc = 0, nv = 0;
-    for (uint32_t id = blockIdx.x * B + threadIdx.x; id < n; id += G) {
-        const uint32_t pix = id / a.spp, k = id % a.spp;
-        const uint32_t px = pix % a.W, py = pix / a.W;
-        const uint32_t seed = a.frame + k;
-        if (!ADJ) {
-            f3 L = warPath<false>(s, a, px, py, seed, st, mk3(0.f), mk3(0.f), 0, nc, nv);
-            a.out[id] = make_float4(L.x, L.y, L.z, 0.f);
-        } else {
-            const float* dl = a.dLdI + 3 * size_t(pix);
-            f3 dL = mk3(dl[0], dl[1], dl[2]);
-            if (dL.x == 0.f && dL.y == 0.f && dL.z == 0.f) continue;
-            unsigned long long c0 = 0, c1 = 0;
-            f3 L = warPath<false>(s, a, px, py, seed, st, mk3(0.f), mk3(0.f), 0, c0, c1);
-            warPath<true>(s, a, px, py, seed, st, L, dL, jenkins(pix) % a.hashSize, nc, nv);
-            nc += c0;
-            nv += c1;
-        }
-    }
+    for (uint32_t id = blockIdx.x * B + threadIdx.x; id < n; id += G) warItem<ADJ>(s, a, id, st, nc, nv);
     if (a.rays) {
         atomicAdd(&a.rays[0], nc);
         atomicAdd(&a.rays[1], nv);

@@ -1,572 +1,10 @@
-// WARDiffPathTracer geometry gradients: warped-area reparameterization (useWAR = 1), direct
-// illumination paths (maxBounces = 0, the configuration of the reference's shape optimization,
-// scripts/inv-rendering/shape_optimization/shape_example.py:23).
-//
-// The path (WARDiffPathTracer.rt.slang:75-143, PTUtils.slang:253-341) is: a screen sample ps with
-// Gaussian jitter; the camera ray hits x1 (AttachToRay, DiffSceneQuery.slang:131-219); x1 adds its
-// emission; NEE toward a light sample (computeNEE, PTUtils.slang:211-249); a BSDF-sampled ray hits
-// x2, which adds its MIS-weighted emission. With useWAR the screen sample and the NEE and scatter
-// directions are warped (WarpedAreaReparam.slang:199-332): each gets a warp V (zero in value)
-// and a Jacobian determinant detJ (one in value) whose parameter derivatives carry the boundary
-// terms. Sampled directions, light samples, pdfs and MIS weights are detached as in the reference.
-//
-// Writing S = dLdI . radiance and Q_E, Q_N, Q_S for the three contributions of the path,
-//   S = thp0 [Q_E + w_N (a2s/a2s^) (f_N/f_N^) Q_N + detJ_S (f_S/f_S^) Q_S],
-//   thp0 = detJ_screen G(ps + V_screen - pc) / G^,
-// so the adjoint is a short reverse sweep: BSDF partials at x1 (isotropic BSDF: a function of
-// N.wi, N.wo and wi.wo), the NEE geometry factor, the two secondary warps, the interpolated shading
-// normal, the barycentrics and the ray/plane intersection of x1, and finally the screen warp.
-//
-// Warp adjoint. With aux directions a_i (vMF around the base direction, antithetic pairs), aux
-// hit directions d_i (AttachToGeometry: fixed barycentrics, so d_i moves with the aux triangle's
-// vertices and the ray origin), harmonic weights w_i and their direction derivatives g_i (the
-// custom forward derivative of computeHarmonicWeight), m = sum w_i d_i / sum w_i:
-//   dV    = N(m) dm,                 N(m) = (I - u u^T) / |m|, u = m / |m|
-//   detJ' = trace of the warp Jacobian's parameter derivative
-//         = d/dtheta [ sum_i g_i^T N(m) d_i / sum w_i ]   (the m G^T term vanishes as N(m) m = 0).
-// The screen warp is the same construction projected to screen space (computeCameraRayScreenPos),
-// which adds the projection's gradient P and Hessian. The closed forms below were checked
-// against numerical differentiation of the defining expressions.
-#include "fhp_internal.h"
-#include "fhp_trace.h"
+// WARDiffPathTracer geometry-gradient kernel (warped-area reparameterization, maxBounces = 0);
+// the per-path code is in fhp_war_geom.h.
+#include "fhp_war_geom.h"
 
 namespace fhp {
 
 namespace {
-
-// ------------------------------------------------------------------ dual numbers over (ci, co, cio)
-struct DF {
-    float v, d0, d1, d2;
-};
-__device__ __forceinline__ DF dconst(float c) { return DF{c, 0.f, 0.f, 0.f}; }
-__device__ __forceinline__ DF operator+(DF a, DF b) { return DF{a.v + b.v, a.d0 + b.d0, a.d1 + b.d1, a.d2 + b.d2}; }
-__device__ __forceinline__ DF operator-(DF a, DF b) { return DF{a.v - b.v, a.d0 - b.d0, a.d1 - b.d1, a.d2 - b.d2}; }
-__device__ __forceinline__ DF operator*(DF a, DF b) {
-    return DF{a.v * b.v, a.d0 * b.v + a.v * b.d0, a.d1 * b.v + a.v * b.d1, a.d2 * b.v + a.v * b.d2};
-}
-__device__ __forceinline__ DF operator*(DF a, float s) { return DF{a.v * s, a.d0 * s, a.d1 * s, a.d2 * s}; }
-__device__ __forceinline__ DF operator*(float s, DF a) { return a * s; }
-__device__ __forceinline__ DF operator+(DF a, float s) { return DF{a.v + s, a.d0, a.d1, a.d2}; }
-__device__ __forceinline__ DF operator+(float s, DF a) { return a + s; }
-__device__ __forceinline__ DF operator-(float s, DF a) { return DF{s - a.v, -a.d0, -a.d1, -a.d2}; }
-__device__ __forceinline__ DF operator/(DF a, DF b) {
-    const float iv = 1.f / b.v, q = a.v * iv;
-    return DF{q, (a.d0 - q * b.d0) * iv, (a.d1 - q * b.d1) * iv, (a.d2 - q * b.d2) * iv};
-}
-__device__ __forceinline__ DF dsqrt(DF a) {
-    const float r = sqrtf(a.v), k = r > 0.f ? 0.5f / r : 0.f;
-    return DF{r, a.d0 * k, a.d1 * k, a.d2 * k};
-}
-__device__ __forceinline__ DF dpow5(DF a) {
-    const float a2 = a.v * a.v, a4 = a2 * a2, k = 5.f * a4;
-    return DF{a4 * a.v, a.d0 * k, a.d1 * k, a.d2 * k};
-}
-__device__ __forceinline__ DF dlambda(float a2, DF c) {
-    // IsotropicGGX.slang:163 evalLambdaGGX: (-1 + sqrt(1 + a2 tan^2)) / 2, tan^2 = (1 - c^2) / c^2
-    const DF c2 = c * c;
-    const DF t2 = (1.f - c2) / c2;
-    return (dsqrt(1.f + a2 * t2) + (-1.f)) * 0.5f;
-}
-
-// phi = adj . f(wi, wo) and its partials with respect to ci = N.wi, co = N.wo, cio = wi.wo for
-// the reflection BSDFs of StandardMaterial (Frostbite diffuse + GGX, StandardBSDF.slang:228-240)
-// and PBRTDiffuseMaterial (Lambert). Returns 0 where the BSDF is zero or invalid.
-__device__ float bsdfGeomGrad(const Shading& sd, const Material& m, f3 wo, f3 adj, float& gci, float& gco, float& gcio) {
-    gci = gco = gcio = 0.f;
-    const f3 wiL = sd.toLocal(sd.V), woL = sd.toLocal(wo);
-    if (m.type == MAT_PBRT_DIFFUSE) {
-        if (!validR(sd, wiL, woL, wo)) return 0.f;
-        const float k = dot(adj, m.diffuse) * kInvPi;
-        gco = k;
-        return k * woL.z;
-    }
-    if (!validRT(sd, wiL, woL, wo)) return 0.f;
-    Bsdf b;
-    b.init(wiL, m);
-    if (fminf_(wiL.z, woL.z) < kMinCosTheta) return 0.f;
-    const DF ci{wiL.z, 1.f, 0.f, 0.f}, co{woL.z, 0.f, 1.f, 0.f}, cio{dot(sd.V, wo), 0.f, 0.f, 1.f};
-    DF phi = dconst(0.f);
-    if (b.pDiff > 0.f) {
-        // (wo.h)^2 = (1 + wi.wo) / 2 for h = normalize(wi + wo)
-        const float r = b.rough;
-        const DF fd90 = (0.f + (0.5f - 0.f) * r) + (2.f * r) * ((1.f + cio) * 0.5f);
-        const DF wiS = 1.f + (fd90 + (-1.f)) * dpow5(1.f - ci);
-        const DF woS = 1.f + (fd90 + (-1.f)) * dpow5(1.f - co);
-        const float eF = 1.f + (1.f / 1.51f - 1.f) * r;
-        phi = phi + (wiS * woS * co) * (dot(adj, b.diff) * eF * kInvPi);
-    }
-    if (b.pSpec > 0.f && b.alpha > 0.f) {
-        const float a2 = b.alpha * b.alpha;
-        const DF len = dsqrt(2.f + 2.f * cio);
-        const DF hz = (ci + co) / len, wiH = (1.f + cio) / len;
-        const DF dd = hz * hz * (a2 - 1.f) + 1.f;
-        const DF D = dconst(a2 * kInvPi) / (dd * dd);
-        const DF G = dconst(1.f) / (1.f + dlambda(a2, ci) + dlambda(a2, co));
-        const DF sfac = D * G * 0.25f / ci;
-        const DF w5 = dpow5(1.f - wiH);
-        const float as = dot(adj, b.spec), a1 = adj.x + adj.y + adj.z;
-        phi = phi + sfac * (1.f - w5) * as + sfac * w5 * a1;
-    }
-    gci = phi.d0;
-    gco = phi.d1;
-    gcio = phi.d2;
-    return phi.v;
-}
-
-// ------------------------------------------------------------------ auxiliary rays
-// Orthonormal frame of a direction (ShadingFrame(N, perp_stark(N)), ShadingFrame.slang:45-51).
-struct Frame3 {
-    f3 T, B, N;
-};
-__device__ __forceinline__ Frame3 dirFrame(f3 w) {
-    Frame3 f;
-    f.N = normalize(w);
-    const f3 t = perpStark(f.N);
-    f.T = normalize(t - f.N * dot(t, f.N));
-    f.B = cross(f.N, f.T);
-    return f;
-}
-
-// Regenerates the aux directions of one traceAsymptoticWeightedMeanIntersection call
-// (WarpedAreaReparam.slang:111-193) from a copy of the path's sample generator: aux 2j and 2j + 1
-// share the uniform pair (rnd[2j], rnd[2j + 1]), the odd one reflected about the base direction;
-// without antithetic sampling the N rnd values are drawn and discarded and each aux draws a pair.
-struct AuxGen {
-    Rng rng;
-    Frame3 f;
-    float kappa;
-    bool anti;
-    float u0, u1;
-    __device__ void start(Rng r, f3 w, float k, bool antithetic, uint32_t n) {
-        rng = r;
-        f = dirFrame(w);
-        kappa = k;
-        anti = antithetic;
-        u0 = u1 = 0.f;
-        if (!anti)
-            for (uint32_t i = 0; i < n; i++) rng.u1();
-    }
-    __device__ f3 next(uint32_t i, float& uy) {
-        if (!anti || (i & 1u) == 0u) {
-            u0 = rng.u1();
-            u1 = rng.u1();
-        }
-        uy = u1;
-        // sampleVonMisesFisher (MathHelpers.slang:403-411)
-        const float sy = fmaxf_(1.f - u1, 1e-6f);
-        const float phi = kTwoPi * u0;
-        const float cosT = 1.f + logf(expf(-2.f * kappa) * (1.f - sy) + sy) / kappa;
-        const float sinT = sqrtf(fmaxf_(1.f - cosT * cosT, 0.f));
-        const f3 dl = mk3(cosf(phi) * sinT, sinf(phi) * sinT, cosT);
-        f3 a = normalize(f.T * dl.x + f.B * dl.y + f.N * dl.z);
-        if (anti && (i & 1u)) a = a * -1.f + f.N * (2.f * dot(f.N, a));  // reflect(-a, N)
-        return a;
-    }
-};
-
-struct AuxHit {
-    bool hit;
-    uint32_t tri;
-    float b[3];
-    f3 p, d;       // hit point and normalized direction from the origin (or the aux direction)
-    float w, c;    // harmonic weight and its direction-derivative coefficient
-};
-
-// Traces one aux ray (AttachToGeometry) and evaluates computeHarmonicWeight and the coefficient of
-// its custom forward derivative (WarpedAreaReparam.slang:50-98).
-__device__ AuxHit auxTrace(const SceneDev& s, TraceStack& st, f3 o, f3 a, float uy, float kappa) {
-    AuxHit r;
-    HitRec h;
-    r.hit = traceBVH4<false, false>(s, o, a, 0.f, kFltMax, st, h);
-    float boundary = 0.f;
-    r.d = a;
-    r.tri = kInvalid;
-    if (r.hit) {
-        r.tri = h.tri;
-        r.b[1] = h.b1;
-        r.b[2] = h.b2;
-        r.b[0] = 1.f - h.b1 - h.b2;
-        f3 p[3], n[3];
-        float tu, tv;
-        for (int k = 0; k < 3; k++) loadVertex(s, s.idx[3 * h.tri + k], p[k], n[k], tu, tv);
-        r.p = p[0] * r.b[0] + p[1] * r.b[1] + p[2] * r.b[2];
-        const f3 nrm = n[0] * r.b[0] + n[1] * r.b[1] + n[2] * r.b[2];  // computeShadingFrame: not normalized
-        const f3 v = r.p - o;
-        const float l = length(v);
-        r.d = l > 0.f ? v / l : mk3(0.f);
-        const float dn = dot(nrm, a);
-        boundary = dn * dn;
-    }
-    const float sy = fmaxf_(1.f - uy, 1e-6f);
-    const float invVMF = 1.f / ((1.f - sy) * expf(-2.f * kappa) + sy);
-    const float wDen = invVMF - 1.f + boundary;
-    const float rcp = wDen > 1e-4f ? 1.f / wDen : 0.f;
-    r.w = rcp * rcp * rcp * invVMF;
-    r.c = invVMF * r.w * rcp * kappa * 3.f;
-    return r;
-}
-
-// Position / normal gradient of the three vertices of triangle tri, if it belongs to the mesh.
-__device__ void addVertexGrad(const SceneDev& s, float* grad, const GeomArgs& a, uint32_t row, uint32_t tri, const float b[3],
-                              f3 g) {
-    if (!(g.x != 0.f || g.y != 0.f || g.z != 0.f)) return;
-    for (int k = 0; k < 3; k++) {
-        const uint32_t v = s.idx[3 * tri + k];
-        if (v < a.meshV0 || v - a.meshV0 >= a.meshVerts) continue;
-        float* q = grad + size_t(row) * 3 * a.meshVerts + 3 * size_t(v - a.meshV0);
-        const f3 gk = g * b[k];
-        if (gk.x != 0.f) atomicAdd(q + 0, gk.x);
-        if (gk.y != 0.f) atomicAdd(q + 1, gk.y);
-        if (gk.z != 0.f) atomicAdd(q + 2, gk.z);
-    }
-}
-__device__ void addVertexGrads3(const SceneDev& s, float* grad, const GeomArgs& a, uint32_t row, uint32_t tri, const f3 g[3]) {
-    for (int k = 0; k < 3; k++) {
-        const uint32_t v = s.idx[3 * tri + k];
-        if (v < a.meshV0 || v - a.meshV0 >= a.meshVerts) continue;
-        float* q = grad + size_t(row) * 3 * a.meshVerts + 3 * size_t(v - a.meshV0);
-        if (g[k].x != 0.f) atomicAdd(q + 0, g[k].x);
-        if (g[k].y != 0.f) atomicAdd(q + 1, g[k].y);
-        if (g[k].z != 0.f) atomicAdd(q + 2, g[k].z);
-    }
-}
-
-// Adjoint of reparameterizeRay (WarpedAreaReparam.slang:226-263) for a ray (o, w): given dS/dV
-// (gV) and dS/d detJ (gDet), adds the aux-triangle vertex gradients and returns dS/do; with
-// wantOmega also dS/dw through the harmonic weights' direction derivatives.
-__device__ void warpAdjoint(const SceneDev& s, const GeomArgs& a, TraceStack& st, f3 o, f3 w, Rng rng, f3 gV, float gDet,
-                            bool wantOmega, uint32_t row, f3& gO, f3& gW, unsigned long long& nc) {
-    gO = mk3(0.f);
-    gW = mk3(0.f);
-    const float ww = dot(w, w);
-    AuxGen gen;
-    gen.start(rng, w, a.kappa, a.useAntithetic != 0, a.aux);
-    float Sw = 0.f;
-    f3 Sd = mk3(0.f);
-    f3 Mc0 = mk3(0.f), Mc1 = mk3(0.f), Mc2 = mk3(0.f);  // M = sum d g^T, stored by columns
-    for (uint32_t i = 0; i < a.aux; i++) {
-        float uy;
-        const f3 ai = gen.next(i, uy);
-        const AuxHit h = auxTrace(s, st, o, ai, uy, a.kappa);
-        nc++;
-        const f3 g = (ai * ww - w * dot(w, ai)) * h.c;
-        Sw += h.w;
-        Sd = Sd + h.d * h.w;
-        Mc0 = Mc0 + h.d * g.x;
-        Mc1 = Mc1 + h.d * g.y;
-        Mc2 = Mc2 + h.d * g.z;
-    }
-    const float Swc = fmaxf_(Sw, 1e-8f);
-    const f3 m = Sd / Swc;
-    const float r = length(m);
-    if (!(r > 0.f)) return;
-    const f3 u = m / r;
-    auto Nm = [&](f3 v) { return (v - u * dot(u, v)) / r; };
-    const f3 Mu = Mc0 * u.x + Mc1 * u.y + Mc2 * u.z;      // M u
-    const f3 MTu = mk3(dot(Mc0, u), dot(Mc1, u), dot(Mc2, u));  // M^T u
-    const float trM = Mc0.x + Mc1.y + Mc2.z;
-    const float uMu = dot(u, Mu);
-    const f3 qV = Nm(gV);
-    const f3 common = Nm(Mu + MTu) * (-1.f / r) - u * ((trM - uMu) / (r * r));
-    gen.start(rng, w, a.kappa, a.useAntithetic != 0, a.aux);
-    for (uint32_t i = 0; i < a.aux; i++) {
-        float uy;
-        const f3 ai = gen.next(i, uy);
-        const AuxHit h = auxTrace(s, st, o, ai, uy, a.kappa);
-        nc++;
-        const f3 g = (ai * ww - w * dot(w, ai)) * h.c;
-        const float sk = h.w / Swc;
-        const f3 dT = Nm(g) + common * sk;
-        const f3 gd = qV * sk + dT * (gDet / Swc);
-        if (h.hit) {
-            const f3 v = h.p - o;
-            const float l = length(v);
-            if (l > 0.f) {
-                const f3 gp = (gd - h.d * dot(h.d, gd)) / l;
-                gO = gO - gp;
-                addVertexGrad(s, a.gradPos, a, row, h.tri, h.b, gp);
-            }
-        }
-        if (wantOmega) {
-            const f3 qk = Nm(h.d);
-            gW = gW + (w * (2.f * dot(ai, qk)) - ai * dot(w, qk) - qk * dot(w, ai)) * (gDet / Swc * h.c);
-        }
-    }
-}
-
-// Camera ray direction for a screen position (computeCameraRayDirection, DiffSceneQuery.slang:252-261).
-__device__ __forceinline__ f3 camRaw(const CameraDev& c, float sx, float sy, uint32_t W, uint32_t H) {
-    const float p_x = sx / float(W), p_y = sy / float(H);
-    const float ndc_x = 2.f * p_x + -1.f, ndc_y = -2.f * p_y + 1.f;
-    return c.U * ndc_x + c.V * ndc_y + c.W;
-}
-
-// Adjoint of reparameterizeScreenSample (WarpedAreaReparam.slang:269-332) at screen sample ps:
-// gVs = dS/dV (screen units), gDet = dS/d detJ. Adds aux-triangle vertex gradients.
-__device__ void screenWarpAdjoint(const SceneDev& s, const GeomArgs& a, TraceStack& st, float psx, float psy, Rng rng,
-                                  float gVx, float gVy, float gDet, uint32_t row, unsigned long long& nc) {
-    const CameraDev& cam = s.cam;
-    const f3 o = cam.pos;
-    const f3 r0 = camRaw(cam, psx, psy, a.W, a.H);
-    const float rl = length(r0);
-    if (!(rl > 0.f)) return;
-    const f3 w = r0 / rl;
-    auto Nw = [&](f3 v) { return (v - w * dot(w, v)) / rl; };
-    const f3 cx = Nw(cam.U * (2.f / float(a.W))), cy = Nw(cam.V * (-2.f / float(a.H)));  // columns of d w / d ps
-    AuxGen gen;
-    gen.start(rng, w, a.kappaScreen, a.useAntithetic != 0, a.aux);
-    float Sw = 0.f;
-    f3 Sd = mk3(0.f), Mh0 = mk3(0.f), Mh1 = mk3(0.f);  // M_h = sum d h^T (3 x 2)
-    for (uint32_t i = 0; i < a.aux; i++) {
-        float uy;
-        const f3 ai = gen.next(i, uy);
-        const AuxHit h = auxTrace(s, st, o, ai, uy, a.kappaScreen);
-        nc++;
-        const f3 g = (ai - w * dot(w, ai)) * h.c;
-        Sw += h.w;
-        Sd = Sd + h.d * h.w;
-        Mh0 = Mh0 + h.d * dot(cx, g);
-        Mh1 = Mh1 + h.d * dot(cy, g);
-    }
-    const float Swc = fmaxf_(Sw, 1e-8f);
-    const f3 m = Sd / Swc;
-    const float r = length(m);
-    if (!(r > 0.f)) return;
-    const f3 u = m / r;
-    auto Nm = [&](f3 v) { return (v - u * dot(u, v)) / r; };
-    // Screen projection proj_c(u) = beta_c (A_c . u) / (W . u) (computeCameraRayScreenPos with an
-    // orthogonal camera basis), its gradient rows P_c and Hessians H_c.
-    const f3 Wc = cam.W;
-    const float WW = dot(Wc, Wc);
-    const float bx = float(a.W) * WW / (2.f * dot(cam.U, cam.U));
-    const float by = -float(a.H) * WW / (2.f * dot(cam.V, cam.V));
-    const float Wu = dot(Wc, u);
-    const f3 Px = (cam.U / Wu - Wc * (dot(cam.U, u) / (Wu * Wu))) * bx;
-    const f3 Py = (cam.V / Wu - Wc * (dot(cam.V, u) / (Wu * Wu))) * by;
-    auto Hx = [&](f3 v) {
-        return ((cam.U * dot(Wc, v) + Wc * dot(cam.U, v)) * (-1.f / (Wu * Wu)) + Wc * (2.f * dot(cam.U, u) * dot(Wc, v) / (Wu * Wu * Wu))) * bx;
-    };
-    auto Hy = [&](f3 v) {
-        return ((cam.V * dot(Wc, v) + Wc * dot(cam.V, v)) * (-1.f / (Wu * Wu)) + Wc * (2.f * dot(cam.V, u) * dot(Wc, v) / (Wu * Wu * Wu))) * by;
-    };
-    const f3 NM0 = Nm(Mh0), NM1 = Nm(Mh1);
-    const f3 PtMtu = Px * dot(Mh0, u) + Py * dot(Mh1, u);
-    const f3 MhPu = Mh0 * dot(Px, u) + Mh1 * dot(Py, u);
-    const float trPMh = dot(Px, Mh0) + dot(Py, Mh1);
-    const float uMhPu = dot(u, MhPu);
-    const f3 hv = Nm(Hx(NM0) + Hy(NM1));
-    const f3 qV = Nm(Px * gVx + Py * gVy);
-    const f3 common = Nm(PtMtu + MhPu) * (-1.f / r) - u * ((trPMh - uMhPu) / (r * r)) + hv;
-    gen.start(rng, w, a.kappaScreen, a.useAntithetic != 0, a.aux);
-    for (uint32_t i = 0; i < a.aux; i++) {
-        float uy;
-        const f3 ai = gen.next(i, uy);
-        const AuxHit h = auxTrace(s, st, o, ai, uy, a.kappaScreen);
-        nc++;
-        if (!h.hit) continue;
-        const f3 g = (ai - w * dot(w, ai)) * h.c;
-        const float hx = dot(cx, g), hy = dot(cy, g);
-        const float sk = h.w / Swc;
-        const f3 dT = Nm(Px * hx + Py * hy) + common * sk;
-        const f3 gd = qV * sk + dT * (gDet / Swc);
-        const f3 v = h.p - o;
-        const float l = length(v);
-        if (l > 0.f) addVertexGrad(s, a.gradPos, a, row, h.tri, h.b, (gd - h.d * dot(h.d, gd)) / l);
-    }
-}
-
-// Gaussian pixel jitter (WARDiffPathTracer.rt.slang:60-63).
-__device__ __forceinline__ void gaussianJitter(float ux, float uy, float sigma, float& gx, float& gy) {
-    float r = sqrtf(-2.f * logf(ux));
-    gx = (r * cosf(kTwoPi * uy)) * sigma;
-    gy = (r * sinf(kTwoPi * uy)) * sigma;
-}
-
-// One path: primal replay of tracePaths/tracePath for maxBounces = 0, then the reverse sweep.
-__device__ void geomPath(const SceneDev& s, const GeomArgs& a, uint32_t px, uint32_t py, uint32_t seed, f3 dL, uint32_t row,
-                         TraceStack& st, unsigned long long& nc) {
-    Rng rng = rngSeed(px, py, seed / 2);
-    for (int i = 0; i < 6; i++) rng.u1();
-    const float pcx = float(px) + 0.5f, pcy = float(py) + 0.5f;
-    float ux = rng.u1(), uy = rng.u1();
-    float jx, jy;
-    gaussianJitter(ux, uy, 0.5f, jx, jy);
-    if (a.useAntithetic && seed % 2 == 0) { jx = -jx; jy = -jy; }
-    const float psx = pcx + jx, psy = pcy + jy;
-    const Rng rngScreen = rng;  // reparameterizeScreenSample takes the generator by value
-    const f3 r0 = camRaw(s.cam, psx, psy, a.W, a.H);
-    const float rl = length(r0);
-    const f3 w0 = rl > 0.f ? r0 / rl : mk3(0.f);
-    const f3 o = s.cam.pos;
-    HitRec h;
-    nc++;
-    if (!traceBVH4<false, false>(s, o, w0, 0.f, kFltMax, st, h)) return;
-
-    // AttachToRay (DiffSceneQuery.slang:192-219): ray/plane distance, area barycentrics.
-    const uint32_t tri = h.tri;
-    f3 v[3], nv[3];
-    float tu, tv;
-    for (int k = 0; k < 3; k++) loadVertex(s, s.idx[3 * tri + k], v[k], nv[k], tu, tv);
-    const f3 e1 = v[1] - v[0], e2 = v[2] - v[0];
-    const f3 pvec = cross(w0, e2);
-    const float det = dot(e1, pvec);
-    const f3 qvec = cross(o - v[0], e1);
-    const float t = dot(e2, qvec) / det;
-    const f3 p = o + w0 * t;
-    const f3 ep0 = v[0] - p, ep1 = v[1] - p, ep2 = v[2] - p;
-    const float A0 = length(cross(ep1, ep2)), A1 = length(cross(ep2, ep0)), A2 = length(cross(ep0, ep1));
-    const float At = A0 + A1 + A2;
-    const float bary[3] = {A0 / At, A1 / At, A2 / At};
-    const f3 x1 = v[0] * bary[0] + v[1] * bary[1] + v[2] * bary[2];
-    const f3 normalW = nv[0] * bary[0] + nv[1] * bary[1] + nv[2] * bary[2];
-    Shading sd = shadingFromHit(s, tri, bary[1], bary[2], w0);
-    sd.V = -w0;
-    setFrame(sd, normalW);
-    const Shading sd0 = sd;
-    const Material m = loadMaterial(s, s.mats, s.texels, sd0, sd, nullptr);
-    const float nl = length(normalW);
-    const f3 nh = normalW / nl;
-    const float sgn = dot(sd.N, nh) < 0.f ? -1.f : 1.f;  // double-sided back-face flip
-
-    // Primal contributions and the adjoint of x1's shading (S = Q_E + Q_N + Q_S at the base point).
-    const float QE = anyPos(m.emission) ? dot(dL, m.emission) : 0.f;
-    float QN = 0.f, QS = 0.f;
-    f3 gX = mk3(0.f), gNs = mk3(0.f), gWi = mk3(0.f);
-    const f3 faceO = sd.frontFacing ? sd.faceN : -sd.faceN;
-    if (a.useNEE) {
-        LightSample ls;
-        if (sampleLight(s, LS_UNIFORM, x1, normalW, rng, ls)) {
-            const f3 Li = ls.pdf > 0.f ? ls.Le / ls.pdf : mk3(0.f);
-            const f3 lightPos = rayOriginOffset(ls.posW, ls.normalW);
-            const f3 origin = rayOriginOffset(x1, normalW);
-            const f3 toL = lightPos - origin;
-            const float distance = length(toL);
-            const f3 ldir = normalize(toL);
-            HitRec sh;
-            if (anyPos(Li) && !traceBVH4<true, false>(s, origin, ldir, 0.f, distance, st, sh)) {
-                f3 wo = ls.posW - x1;
-                const float dist = length(wo);
-                wo = wo / dist;
-                float misN = 1.f;
-                if (a.useMIS) {
-                    float sp;
-                    if (a.useBSDFSampling || m.type == MAT_PBRT_DIFFUSE) sp = matPdf(sd, m, ldir);
-                    else {
-                        f3 wiL = sd.toLocal(sd.V), woL = sd.toLocal(ldir);
-                        sp = (validRT(sd, wiL, woL, ldir) && fminf_(wiL.z, woL.z) >= kMinCosTheta) ? woL.z * kInvPi : 0.f;
-                    }
-                    misN = misPower(ls.pdf, sp);
-                }
-                float gci, gco, gcio;
-                const float phi = bsdfGeomGrad(sd, m, wo, dL * Li * misN, gci, gco, gcio);
-                if (phi != 0.f) {
-                    QN = phi;
-                    gNs = gNs + sd.V * gci + wo * gco;
-                    gWi = gWi + sd.N * gci + wo * gcio;
-                    // Change of variables on the light: y = T(y') is the light point hit from x1 in
-                    // direction w0(y') + V, with w0 the (detached) sampled direction. The area
-                    // factors of T and of the geometry term cancel, leaving the integrand
-                    //   Le f(x1, N, w0 + V) detJ / p(w0),
-                    // so only the BSDF (through the warp and the shading normal) and detJ depend
-                    // on the parameters; V follows the light's edges and the occluders as seen from
-                    // the moving x1 (origin term).
-                    const f3 gF = sd.N * gco + sd.V * gcio;  // d phi / d direction
-                    f3 gO, gOm;
-                    warpAdjoint(s, a, st, rayOriginOffset(x1, faceO), wo, rng, gF, QN, false, row, gO, gOm, nc);
-                    gX = gX + gO;
-                }
-            }
-        }
-    }
-    Scatter sc;
-    if (matSample(sd, m, rng, sc, a.useBSDFSampling != 0) && sc.pdf > 0.f) {
-        const f3 woS = sc.wo;
-        const f3 scO = rayOriginOffset(x1, faceO);
-        const f3 fS = matEval(sd, m, woS);
-        HitRec h2;
-        nc++;
-        if (anyPos(fS / sc.pdf) && (!a.useNEE || a.useMIS) && traceBVH4<false, false>(s, scO, woS, 0.f, kFltMax, st, h2)) {
-            Shading sd2 = shadingFromHit(s, h2.tri, h2.b1, h2.b2, woS);
-            const Shading sd20 = sd2;
-            const Material m2 = loadMaterial(s, s.mats, s.texels, sd20, sd2, nullptr);
-            if (anyPos(m2.emission)) {
-                float mis = 1.f;
-                if (a.useNEE && a.useMIS) {
-                    const float lp = lightPdf(s, LS_UNIFORM, scO, sd.orientedFaceN(), s.triToLight[h2.tri], sd2.posW,
-                                              sd2.orientedFaceN());
-                    mis = misPower(sc.pdf, lp);
-                }
-                float gci, gco, gcio;
-                const float phi = bsdfGeomGrad(sd, m, woS, dL * m2.emission * (mis / sc.pdf), gci, gco, gcio);
-                if (phi != 0.f) {
-                    QS = phi;
-                    gNs = gNs + sd.V * gci + woS * gco;
-                    gWi = gWi + sd.N * gci + woS * gcio;
-                    const f3 gDir = sd.N * gco + sd.V * gcio;
-                    f3 gO, gOm;
-                    warpAdjoint(s, a, st, scO, woS, rng, gDir, QS, false, row, gO, gOm, nc);
-                    gX = gX + gO;
-                }
-            }
-        }
-    }
-
-    // Shading normal N = sgn normalize(sum b_k n_k) -> vertex normals and barycentrics.
-    const f3 gNw = (gNs - nh * dot(nh, gNs)) * (sgn / nl);
-    const float gB[3] = {dot(gNw, nv[0]), dot(gNw, nv[1]), dot(gNw, nv[2])};
-    {
-        const f3 gn[3] = {gNw * bary[0], gNw * bary[1], gNw * bary[2]};
-        addVertexGrads3(s, a.gradNrm, a, row, tri, gn);
-    }
-    // Barycentrics of x1 (b1, b2 solve the 2x2 Gram system of the edges; b0 = 1 - b1 - b2).
-    f3 gV[3] = {mk3(0.f), mk3(0.f), mk3(0.f)};
-    {
-        const float g1 = gB[1] - gB[0], g2 = gB[2] - gB[0];
-        const f3 q = x1 - v[0];
-        const float a11 = dot(e1, e1), a12 = dot(e1, e2), a22 = dot(e2, e2);
-        const float dg = a11 * a22 - a12 * a12;
-        if (dg != 0.f) {
-            const float gr1 = (a22 * g1 - a12 * g2) / dg, gr2 = (-a12 * g1 + a11 * g2) / dg;
-            const float b1 = bary[1], b2 = bary[2];
-            const float gA11 = -gr1 * b1, gA12 = -(gr1 * b2 + gr2 * b1), gA22 = -gr2 * b2;
-            const f3 gE1 = q * gr1 + e1 * (2.f * gA11) + e2 * gA12;
-            const f3 gE2 = q * gr2 + e1 * gA12 + e2 * (2.f * gA22);
-            const f3 gQ = e1 * gr1 + e2 * gr2;
-            gX = gX + gQ;
-            gV[0] = gV[0] - gQ - gE1 - gE2;
-            gV[1] = gV[1] + gE1;
-            gV[2] = gV[2] + gE2;
-        }
-    }
-    // x1 = o + t w0 with t = nf . (v0 - o) / (nf . w0).
-    f3 gW0 = gWi * -1.f;  // wi = -w0
-    {
-        const f3 nf = cross(e1, e2);
-        const float den = dot(nf, w0);
-        if (den != 0.f) {
-            const float gT = dot(gX, w0);
-            gW0 = gW0 + gX * t - nf * (gT * t / den);
-            gV[0] = gV[0] + nf * (gT / den);
-            const f3 gNf = (v[0] - x1) * (gT / den);
-            const f3 gE1 = cross(e2, gNf), gE2 = cross(gNf, e1);
-            gV[1] = gV[1] + gE1;
-            gV[2] = gV[2] + gE2;
-            gV[0] = gV[0] - gE1 - gE2;
-        }
-    }
-    addVertexGrads3(s, a.gradPos, a, row, tri, gV);
-
-    // Screen warp: thp0 = detJ * G(ps + V - pc) / G^ and w0 = camDir(ps + V).
-    const float Qt = QE + QN + QS;
-    const float inv2s2 = 1.f / (0.5f * 0.5f);
-    float gVx = -Qt * (psx - pcx) * inv2s2, gVy = -Qt * (psy - pcy) * inv2s2;
-    {
-        const f3 q = (gW0 - w0 * dot(w0, gW0)) / rl;
-        gVx += dot(s.cam.U, q) * (2.f / float(a.W));
-        gVy += dot(s.cam.V, q) * (-2.f / float(a.H));
-    }
-    if (Qt != 0.f || gVx != 0.f || gVy != 0.f) screenWarpAdjoint(s, a, st, psx, psy, rngScreen, gVx, gVy, Qt, row, nc);
-}
 
 // Unconstrained the kernel takes 248 VGPRs (2 waves per SIMD); capped at 128 (4 waves) it spills
 // more but runs the 256x256, 8 spp, 16-aux vertex-gradient pass about 1.2x faster.
@@ -576,18 +14,11 @@ template <int B, int D>
 __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(kGeomWavesPerEU))) void kWarGeom(SceneDev s, GeomArgs a,
                                                                                                      TraceScratch ts) {
     __shared__ uint32_t lds[D * B];
-    TraceStack st{(LdsU32*)(lds + threadIdx.x), uint32_t(B), uint32_t(D), (GlobalU32*)(ts.ovf + blockIdx.x * B + threadIdx.x),
-                  ts.gridBlocks * B, ts.ovfDepth};
+    TraceStack st = makeStack<B, D>(lds, ts, blockIdx.x, threadIdx.x);
     const uint32_t n = a.W * a.H * a.spp;
     const uint32_t G = gridDim.x * B;
     unsigned long long nc = 0;
-    for (uint32_t id = blockIdx.x * B + threadIdx.x; id < n; id += G) {
-        const uint32_t pix = id / a.spp, k = id % a.spp;
-        const float* dl = a.dLdI + 3 * size_t(pix);
-        const f3 dL = mk3(dl[0], dl[1], dl[2]);
-        if (dL.x == 0.f && dL.y == 0.f && dL.z == 0.f) continue;
-        geomPath(s, a, pix % a.W, pix / a.W, a.frame + k, dL, jenkins(pix) % a.hashSize, st, nc);
-    }
+    for (uint32_t id = blockIdx.x * B + threadIdx.x; id < n; id += G) geomItem(s, a, id, st, nc);
     if (a.rays) atomicAdd(&a.rays[0], nc);
 }
 

@@ -693,4 +693,78 @@ void buildLightBVH(HostScene& s) {
     for (uint32_t n = 0; n < s.nLbvh; n++) s.lrefitNodes[fill[leaf[n] ? s.lrefitDepth : depth[n]]++] = n;
 }
 
+namespace {
+int sceneFail(std::string& err, int code, const std::string& msg) {
+    err = msg;
+    return code;
+}
+}  // namespace
+
+// fhp_scene_create's validation and host copy of a scene description (FHP_OK, or FHP_EINVAL /
+// FHP_EBUILD with the message in err).
+int sceneFromDesc(const FhpSceneDesc& d, HostScene& h, std::string& err) {
+    if (!d.h_positions || !d.h_normals || !d.h_texcoords || !d.h_indices || !d.h_tri_material || !d.h_materials ||
+        !d.h_camera)
+        return sceneFail(err, FHP_EINVAL, "scene description has a null array");
+    if (d.num_triangles == 0 || d.num_vertices == 0) return sceneFail(err, FHP_EBUILD, "scene has no triangles");
+    if (d.num_textures && (!d.h_tex_desc || !d.h_texels)) return sceneFail(err, FHP_EINVAL, "texture arrays missing");
+    h.nV = d.num_vertices; h.nT = d.num_triangles; h.nM = d.num_materials; h.nTex = d.num_textures;
+    h.pos.assign(d.h_positions, d.h_positions + 3 * size_t(h.nV));
+    h.nrm.assign(d.h_normals, d.h_normals + 3 * size_t(h.nV));
+    h.restPos = h.pos;
+    h.restNrm = h.nrm;
+    h.uv.assign(d.h_texcoords, d.h_texcoords + 2 * size_t(h.nV));
+    h.idx.assign(d.h_indices, d.h_indices + 3 * size_t(h.nT));
+    h.triMat.assign(d.h_tri_material, d.h_tri_material + h.nT);
+    h.mats.assign(d.h_materials, d.h_materials + size_t(h.nM) * M_STRIDE);
+    for (uint32_t i : h.idx)
+        if (i >= h.nV) return sceneFail(err, FHP_EINVAL, "vertex index out of range");
+    for (uint32_t m : h.triMat)
+        if (m >= h.nM) return sceneFail(err, FHP_EINVAL, "material index out of range");
+    if (h.nTex) {
+        h.texDesc.assign(d.h_tex_desc, d.h_tex_desc + 4 * size_t(h.nTex));
+        h.texels.assign(d.h_texels, d.h_texels + d.num_texels_floats);
+        for (uint32_t t = 0; t < h.nTex; t++) {
+            int64_t off = h.texDesc[4 * t], w = h.texDesc[4 * t + 1], hh = h.texDesc[4 * t + 2];
+            if (off < 0 || w <= 0 || hh <= 0 || uint64_t(off + 3 * w * hh) > d.num_texels_floats)
+                return sceneFail(err, FHP_EINVAL, "texture descriptor out of range");
+        }
+    }
+    for (uint32_t m = 0; m < h.nM; m++) {
+        int tex = int(h.mats[size_t(m) * M_STRIDE + M_TEX]);
+        if (tex >= int(h.nTex)) return sceneFail(err, FHP_EINVAL, "material texture index out of range");
+        if (int(h.mats[size_t(m) * M_STRIDE + M_EMIT_TEX]) - 1 >= int(h.nTex))
+            return sceneFail(err, FHP_EINVAL, "material emissive texture index out of range");
+    }
+    std::memcpy(h.cam, d.h_camera, sizeof(h.cam));
+    if (d.num_meshes) {
+        if (!d.h_mesh_vertex_offsets) return sceneFail(err, FHP_EINVAL, "mesh vertex offsets missing");
+        h.meshOff.assign(d.h_mesh_vertex_offsets, d.h_mesh_vertex_offsets + d.num_meshes + 1);
+        if (h.meshOff.front() != 0 || h.meshOff.back() != h.nV) return sceneFail(err, FHP_EINVAL, "mesh vertex offsets must span [0, num_vertices]");
+        for (uint32_t m = 0; m < d.num_meshes; m++)
+            if (h.meshOff[m] > h.meshOff[m + 1]) return sceneFail(err, FHP_EINVAL, "mesh vertex offsets must not decrease");
+    } else {
+        h.meshOff = {0u, h.nV};
+    }
+    if (d.flags & ~uint32_t(FHP_SCENE_USE_COMPRESSED_HIT_INFO)) return sceneFail(err, FHP_EINVAL, "unknown scene flags");
+    try {
+        computeCamera(h);
+        computeHitInfo(h, (d.flags & FHP_SCENE_USE_COMPRESSED_HIT_INFO) != 0);
+    } catch (const std::exception& e) {
+        return sceneFail(err, FHP_EBUILD, e.what());
+    }
+    return FHP_OK;
+}
+
+
+std::vector<float> packVertexRecords(const HostScene& h) {
+    std::vector<float> vtx(8 * size_t(h.nV));
+    for (uint32_t i = 0; i < h.nV; i++) {
+        float* q = &vtx[8 * size_t(i)];
+        q[0] = h.pos[3 * i]; q[1] = h.pos[3 * i + 1]; q[2] = h.pos[3 * i + 2]; q[3] = h.nrm[3 * i];
+        q[4] = h.nrm[3 * i + 1]; q[5] = h.nrm[3 * i + 2]; q[6] = h.uv[2 * i]; q[7] = h.uv[2 * i + 1];
+    }
+    return vtx;
+}
+
 }  // namespace fhp

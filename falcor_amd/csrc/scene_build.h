@@ -6,6 +6,8 @@
 #include <string>
 #include <vector>
 
+#include "../../include/fhp.h"
+
 namespace fhp {
 
 struct HostScene {
@@ -57,6 +59,11 @@ struct HostScene {
     std::vector<float> restPos, restNrm;
 };
 
+// fhp_scene_create's validation and host copy of a scene description: FHP_OK, or FHP_EINVAL /
+// FHP_EBUILD with the message in err. Computes the camera basis and the PackedHitInfo layout.
+int sceneFromDesc(const FhpSceneDesc& d, HostScene& h, std::string& err);
+// Vertex records as the kernels read them: (p.xyz, n.x) (n.yz, u, v), 8 floats per vertex.
+std::vector<float> packVertexRecords(const HostScene& h);
 void computeCamera(HostScene& s);
 // Mesh triangle ranges and the PackedHitInfo bit allocation; throws when the header does not fit
 // in 64 bits (HitInfo.cpp:83-87).

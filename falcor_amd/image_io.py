@@ -178,12 +178,21 @@ def read_exr(path) -> tuple[np.ndarray, list[str]]:
 def read_hdr(path) -> np.ndarray:
     """Radiance RGBE (.hdr) as [H, W, 3] float32: run-length encoded (new-style) or flat scanlines,
     standard orientation "-Y H +X W"; a texel is byte * 2^(e - 136), zero when e = 0 (the rgbe.c
-    conversion FreeImage's HDR loader uses, Bitmap.cpp)."""
+    conversion FreeImage's HDR loader uses, Bitmap.cpp). Flat scanlines are read as literal
+    pixels, as FreeImage's rgbe_ReadPixels reads them (no old-style run markers). Truncated or
+    inconsistent data raises ValueError."""
     b = Path(path).read_bytes()
     pos = 0
     first = True
+
+    def line_end(at):
+        end = b.find(b"\n", at)
+        if end < 0:
+            raise ValueError("truncated HDR header")
+        return end
+
     while True:
-        end = b.index(b"\n", pos)
+        end = line_end(pos)
         line = b[pos:end].decode("latin-1").strip()
         pos = end + 1
         if first and not line.startswith("#?"):
@@ -193,38 +202,51 @@ def read_hdr(path) -> np.ndarray:
             raise ValueError(f"unsupported HDR format {line}")
         if line == "":
             break
-    end = b.index(b"\n", pos)
+    end = line_end(pos)
     res = b[pos:end].decode("latin-1").split()
     pos = end + 1
     if len(res) != 4 or res[0] != "-Y" or res[2] != "+X":
         raise ValueError(f"unsupported HDR orientation {' '.join(res)}")
-    H, W = int(res[1]), int(res[3])
+    try:
+        H, W = int(res[1]), int(res[3])
+    except ValueError:
+        raise ValueError(f"bad HDR resolution {' '.join(res)}") from None
+    if H <= 0 or W <= 0:
+        raise ValueError(f"bad HDR resolution {' '.join(res)}")
+    n_bytes = len(b)
+
+    def need(k):
+        if pos + k > n_bytes:
+            raise ValueError("corrupt HDR scanline (truncated data)")
+
     rgbe = np.zeros((H, W, 4), np.uint8)
     for y in range(H):
+        need(4)
         if 8 <= W < 0x8000 and b[pos] == 2 and b[pos + 1] == 2 and ((b[pos + 2] << 8) | b[pos + 3]) == W:
             pos += 4
             for c in range(4):
                 x = 0
                 while x < W:
+                    need(1)
                     n = b[pos]
                     pos += 1
                     if n > 128:
                         n -= 128
+                        if x + n > W:
+                            raise ValueError("corrupt HDR scanline (run past the row)")
+                        need(1)
                         rgbe[y, x:x + n, c] = b[pos]
                         pos += 1
                     else:
                         if n == 0 or x + n > W:
                             raise ValueError("corrupt HDR scanline")
+                        need(n)
                         rgbe[y, x:x + n, c] = np.frombuffer(b, np.uint8, n, pos)
                         pos += n
                     x += n
-                if x != W:
-                    raise ValueError("corrupt HDR scanline")
         else:
-            row = np.frombuffer(b, np.uint8, 4 * W, pos).reshape(W, 4)
-            if (row[:, :3] == 1).all(axis=1).any():
-                raise ValueError("old-style run-length HDR scanlines are not supported")
-            rgbe[y] = row
+            need(4 * W)
+            rgbe[y] = np.frombuffer(b, np.uint8, 4 * W, pos).reshape(W, 4)
             pos += 4 * W
     e = rgbe[..., 3].astype(np.int32)
     f = np.where(e > 0, np.ldexp(np.float32(1.0), e - 136).astype(np.float32), np.float32(0.0))

@@ -336,6 +336,15 @@ class Scene:
         return nodes, tri[: i["num_active_lights"]], masks[: i["num_lights"]]
 
     # Ray queries (TraceRay / RayQuery replacements).
+    TRACE_ERR_STACK_OVERFLOW = 1
+
+    def trace_errors(self, reset: bool = True) -> int:
+        """Trace error bits raised by this scene's traversal kernels since the last reset (1: a
+        ray needed more stack entries than the tree's worst case and ended without a hit)."""
+        v = C.c_uint32()
+        _lib.check(_lib.load().fhp_scene_trace_errors(self.handle, C.byref(v), int(reset), _lib.stream_ptr()))
+        return int(v.value)
+
     def trace_closest(self, rays):
         """rays: float32 tensor [N, 8] on the device -> int32-bit tensor [N, 4] (tri, t, b1, b2 bits)."""
         torch = _torch()

@@ -101,8 +101,18 @@ typedef struct FhpSceneInfo {
 } FhpSceneInfo;
 
 int fhp_scene_create(const FhpSceneDesc* desc, FhpScene* out);
+/* Scene mutators that take host arrays (fhp_scene_destroy, fhp_scene_update_vertices,
+ * fhp_bvh_build, fhp_scene_set_materials, fhp_scene_set_material_params, fhp_scene_set_env_map,
+ * fhp_scene_set_analytic_lights) are synchronous: they wait for all queued work of the device,
+ * on every stream, before they overwrite or free buffers that kernels read. */
 int fhp_scene_destroy(FhpScene scene);
 int fhp_scene_info(FhpScene scene, FhpSceneInfo* out);
+/* Trace error bits accumulated by the scene's traversal kernels since the last reset (bit 0:
+ * a ray needed more stack entries than the tree's worst case; that ray ended without a hit).
+ * Zero when the scene's acceleration structure and its launch bounds agree. Synchronises the
+ * stream; reset = 1 clears the bits. No counterpart in the reference (a driver-built
+ * acceleration structure reports no such error). */
+int fhp_scene_trace_errors(FhpScene scene, uint32_t* h_bits, int reset, FhpStream stream);
 /* Replace the vertex positions (and normals, if h_normals is not null) of mesh mesh_id from
  * host memory (3 floats per vertex of the mesh), then rebuild the BVH, the light collection and
  * the light BVH. Replaces Scene::setMeshVertices / set_mesh_vertices (Scene.cpp:4314-4336) and

@@ -22,13 +22,16 @@ import pytest
 
 from falcor_amd import scene as S
 
-# The full GPU suite faulted the GPU inside test_translation_gradient_matches_fd[0-x] (the first
-# test of this module; every earlier test had passed in that process). Reading the code that
-# changed since the module last passed (the light sample's emission read from the current
-# material) and the vertex-gradient kernel's indexing, stacks and launch bounds has not shown a
-# cause, so these tests stay off the GPU until one is shown.
-pytestmark = [pytest.mark.gpu,
-              pytest.mark.skip(reason="faulted the GPU once with no cause found yet; kept off the GPU until one is shown")]
+# The full GPU suite once faulted the GPU inside test_translation_gradient_matches_fd[0-x]. No
+# cause has been shown: reading the vertex-gradient kernel's indexing, stacks, grid and launch
+# bounds found none, and the host check build (tests/test_hostcheck_war.py) runs this exact
+# configuration -- both 512-spp finite-difference renders on the rebuilt scenes over 8 frames,
+# then the 256-spp adjoint and vertex-gradient passes with 64 auxiliary rays over 8 frames --
+# with every array access checked by AddressSanitizer and no error (stack peak 11 of 24
+# entries). The finite-difference tests of that configuration stay off the GPU until a cause is
+# shown; the tests below them never launch the vertex-gradient kernel and run.
+_FAULTED = pytest.mark.skip(reason="this configuration faulted the GPU once; no cause shown yet (CPU replay clean)")
+pytestmark = pytest.mark.gpu
 
 W = H = 64
 TOL = 0.1
@@ -91,6 +94,7 @@ def _adjoint_grad(sc, p, mesh, dLdI, spp=256, frames=range(100, 108)):
             g.get_grads_buffer("MeshNormal").view(-1, 3).cpu().numpy().astype(np.float64) / (spp * len(frames)))
 
 
+@_FAULTED
 @pytest.mark.parametrize("axis,weights", [(0, "x"), (1, "y"), (2, "ones")])
 def test_translation_gradient_matches_fd(gpu, axis, weights):
     sc, p, mesh = _setup()
@@ -104,6 +108,7 @@ def test_translation_gradient_matches_fd(gpu, axis, weights):
     assert abs(ad - fd) <= TOL * abs(fd)
 
 
+@_FAULTED
 def test_scale_gradient_matches_fd(gpu):
     sc, p, mesh = _setup()
     dLdI = _weights("ones")
@@ -158,3 +163,14 @@ def test_war_backward_argument_errors(gpu):
     p.set_property("auxSampleCount", 15)
     with pytest.raises(_lib.FhpError, match="even"):
         p.backward(W, H, 1, frame=0)
+    p.set_property("auxSampleCount", 16)
+    # Frames whose path count does not fit the kernels' 32-bit path ids are refused before any
+    # launch (65536 x 65536 x 1 paths; dL_dI is checked against the frame first, so it is
+    # bypassed through the C ABI with a small buffer that is never read).
+    L = _lib.load()
+    rc = L.fhp_pass_backward(p.handle, sc.handle, 65536, 65536, 1, 0, p.dL_dI.data_ptr(), g.handle, None,
+                             _lib.stream_ptr())
+    assert rc != 0 and "frame too large" in L.fhp_last_error().decode()
+    rc = L.fhp_pass_backward(p.handle, sc.handle, 70000, 1, 1, 0, p.dL_dI.data_ptr(), g.handle, None,
+                             _lib.stream_ptr())
+    assert rc != 0 and "above 65536" in L.fhp_last_error().decode()

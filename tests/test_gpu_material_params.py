@@ -10,6 +10,9 @@ session (device + stream) handles.
   resulting records directly.
 * fhp_bvh_build with another leaf size gives the same closest hits, bit for bit.
 * A session's stream runs the ray queries like the default stream.
+* Host-array mutators wait for work queued on other streams: frames queued on a session stream
+  before a material and vertex update equal renders of the old scene, the next one a render of
+  a fresh scene built with the new materials and vertices.
 """
 import ctypes as C
 
@@ -133,3 +136,53 @@ def test_bvh_build_and_session_stream(gpu):
     _lib.check(L.fhp_session_destroy(sess))
     np.testing.assert_array_equal(hits.cpu().numpy(), h0)
     assert L.fhp_session_create(1 << 20, C.byref(sess)) == -1
+
+
+def test_host_mutators_wait_for_session_stream_work(gpu):
+    """Host-array scene mutators (materials, vertices) are synchronous with every stream: frames
+    queued on a session's non-blocking stream before an update render the old scene, the frame
+    queued after it renders the new one, each bit-identical to a fresh scene's render."""
+    import dataclasses
+    import torch
+    from falcor_amd import _lib
+    from falcor_amd.render_graph import MinimalPathTracer, Scene
+    data = S.displaced_sphere_in_box(1, radius=0.12, segments=(48, 24), sigma=0.05)
+    mesh = data.mesh_names.index("Sphere")
+    W, H, spp, frames = 192, 192, 16, 4
+    props = {"maxBounces": 3, "useEmissiveNEE": True, "emissiveSampler": "LightBVH"}
+    L = _lib.load()
+    sess, st = C.c_void_p(), C.c_void_p()
+    _lib.check(L.fhp_session_create(0, C.byref(sess)))
+    _lib.check(L.fhp_session_stream(sess, C.byref(st)))
+    try:
+        sc = Scene(data)
+        p = MinimalPathTracer(props)
+        outs = [torch.empty((H, W, 4), device="cuda") for _ in range(frames + 1)]
+        torch.cuda.synchronize()
+        for f in range(frames):  # queued, not waited for
+            _lib.check(L.fhp_pass_execute(p.handle, sc.handle, W, H, spp, f, outs[f].data_ptr(), None, st))
+        mats = np.array(data.materials, np.float32)
+        mats[:, S.M_BASE:S.M_BASE + 3] = mats[:, S.M_BASE:S.M_BASE + 3][::-1]  # permute the base colours
+        _lib.check(L.fhp_scene_set_materials(sc.handle, mats.ctypes.data))
+        v0, v1 = sc.mesh_vertex_range(mesh)
+        pos = np.array(data.positions[v0:v1] + np.array([0.03, -0.02, 0.01], np.float32), np.float32)
+        nrm = np.ascontiguousarray(data.normals[v0:v1], np.float32)
+        _lib.check(L.fhp_scene_update_vertices(sc.handle, mesh, pos.ctypes.data, nrm.ctypes.data))
+        _lib.check(L.fhp_pass_execute(p.handle, sc.handle, W, H, spp, frames, outs[frames].data_ptr(), None, st))
+        # The trace error query synchronises the stream it is given.
+        bits = C.c_uint32()
+        _lib.check(L.fhp_scene_trace_errors(sc.handle, C.byref(bits), 1, st))
+        assert bits.value == 0
+        got = [o.cpu().numpy() for o in outs]
+    finally:
+        _lib.check(L.fhp_session_destroy(sess))
+    positions = np.array(data.positions, np.float32)
+    positions[v0:v1] = pos
+    moved = dataclasses.replace(data, materials=mats, positions=positions)
+    ref_old, ref_new = MinimalPathTracer(props), MinimalPathTracer(props)
+    ref_old.set_scene(Scene(data))
+    ref_new.set_scene(Scene(moved))
+    for f in range(frames):
+        np.testing.assert_array_equal(got[f], ref_old.render(W, H, spp, frame=f).cpu().numpy())
+    np.testing.assert_array_equal(got[frames], ref_new.render(W, H, spp, frame=frames).cpu().numpy())
+    assert not np.array_equal(got[frames - 1], got[frames])

@@ -1,0 +1,96 @@
+"""Host check build of the WARDiffPathTracer kernels' per-path code (CPU, AddressSanitizer).
+
+The kernels' grid-loop items (warItem for the primal and the path-replay adjoint, geomItem for
+the vertex gradients) are compiled for the CPU together with the library's host scene builder
+(tests/hostcheck). Every array the kernels read is an exact-size heap allocation, each traversal
+stack is one lane's LDS part plus its overflow part, and gradient rows are exact-size too, so any
+out-of-bounds access aborts the program with the sanitizer's report. The checks:
+
+* the configuration of the vertex-gradient finite-difference test (displaced sphere in the
+  closed box, 64 x 64, 64 auxiliary rays, vMF concentration 10^4): host vertex updates that
+  rebuild the scene, primal renders on the moved scenes, then the adjoint and vertex-gradient
+  passes on the restored scene; a bounded sample of its frames here, all of them (512 spp
+  renders and 256 spp backward passes over 8 frames each) with FHP_HOSTCHECK_FULL=1;
+* the same passes at the vertex-gradient workload's settings (16 auxiliary rays, 10^3);
+* shallow LDS stacks (2 and 4 entries) so traversal runs through the overflow part of the stack;
+* multi-bounce primal and adjoint paths on a sphere field and on textured spheres.
+Each step must also report no trace error (no ray needed more stack than the tree's bound).
+"""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from falcor_amd import scene as S
+
+hostcheck = pytest.importorskip("tests.hostcheck.run")
+
+
+def _run(data, steps, dLdI, timeout=900):
+    rc, out, err = hostcheck.run(data, steps, dLdI=dLdI, timeout=timeout)
+    assert "ERROR: AddressSanitizer" not in err and "runtime error" not in err, err[-4000:]
+    assert rc == 0, f"host check failed ({rc}):\n{out}\n{err[-4000:]}"
+    for line in out.splitlines():
+        m = re.search(r"stack_peak=(\d+) stack_capacity=(\d+) trace_errors=(\d+)", line)
+        if m:
+            assert int(m.group(3)) == 0 and int(m.group(1)) <= int(m.group(2)), line
+    return out
+
+
+def _ramp(W, H):
+    x = (np.arange(W) + 0.5) / W
+    return np.repeat(np.broadcast_to(x[None, :], (H, W))[..., None], 3, axis=2).astype(np.float32)
+
+
+def _fd_case_steps(mesh, W, H, primal_spp, bwd_spp, frame, aux, log_kappa):
+    return [f"translate {mesh} 0.004 0 0", f"primal {W} {H} {primal_spp} {frame} 0 24",
+            f"translate {mesh} -0.008 0 0", f"primal {W} {H} {primal_spp} {frame} 0 24",
+            f"translate {mesh} 0.004 0 0",
+            f"adjoint {W} {H} {bwd_spp} {100 + frame} 0 24 64",
+            f"geom {W} {H} {bwd_spp} {100 + frame} {aux} {log_kappa} {log_kappa} {mesh} 64 24"]
+
+
+@pytest.fixture(scope="module")
+def box_sphere():
+    data = S.displaced_sphere_in_box(0, radius=0.12, segments=(24, 12), sigma=0.05)
+    return data, data.mesh_names.index("Sphere")
+
+
+def test_vertex_gradient_fd_configuration_in_bounds(box_sphere):
+    data, mesh = box_sphere
+    W = H = 64
+    full = os.environ.get("FHP_HOSTCHECK_FULL") == "1"
+    frames = range(8) if full else (0, 5)
+    steps = []
+    for f in frames:
+        steps += _fd_case_steps(mesh, W, H, 512 if full else 8, 256 if full else 4, f, 64, 4)
+    out = _run(data, steps, _ramp(W, H), timeout=7200 if full else 900)
+    g = [float(v) for v in re.findall(r"grad_abs_sum=([0-9.e+-]+)", out)]
+    assert len(g) == len(frames) and all(v > 0 for v in g)
+    assert "finite=0" not in out
+
+
+def test_vertex_gradient_workload_settings_in_bounds(box_sphere):
+    data, mesh = box_sphere
+    W = H = 48
+    _run(data, [f"geom {W} {H} 4 7 16 3 3 {mesh} 128 16", f"geom {W} {H} 2 9 16 3 3 {mesh} 1 32"], _ramp(W, H))
+
+
+@pytest.mark.parametrize("lds", [2, 4])
+def test_overflow_stack_in_bounds(box_sphere, lds):
+    data, mesh = box_sphere
+    W = H = 32
+    out = _run(data, [f"primal {W} {H} 2 0 2 {lds}", f"adjoint {W} {H} 2 1 2 {lds} 8",
+                      f"geom {W} {H} 1 2 8 3 3 {mesh} 8 {lds}"], _ramp(W, H))
+    peaks = [int(v) for v in re.findall(r"stack_peak=(\d+)", out)]
+    assert max(peaks) > lds  # the overflow part of the stack was used
+
+
+@pytest.mark.parametrize("which", ["sphere_field", "textured_spheres"])
+def test_multi_bounce_paths_in_bounds(which):
+    data = S.sphere_field(1, num_spheres=64, num_emissive=256) if which == "sphere_field" else \
+        S.textured_spheres(3, num_objects=4, tex_res=16)
+    W = H = 48
+    _run(data, [f"primal {W} {H} 4 0 4 16", f"adjoint {W} {H} 4 1 4 16 16", f"adjoint {W} {H} 2 2 1 4 3"],
+         np.ones((H, W, 3), np.float32))

@@ -139,13 +139,29 @@ def test_hdr_reader_decodes_rgbe(tmp_path, rle):
     rgbe = rng.integers(0, 256, (H, W, 4), dtype=np.uint8)
     rgbe[:, 10:20] = rgbe[:, 10:11]           # runs
     rgbe[2, :, 3] = 0                         # zero exponent: black
-    rgbe[..., 0] |= 1                         # no (1, 1, 1, n) old-style run markers in flat files
+    rgbe[3, 4:6, :3] = 1                      # (1, 1, 1, e) texels are literal pixels, as FreeImage reads them
     (tmp_path / "a.hdr").write_bytes(_rgbe_encode(rgbe, rle))
     got = I.read_hdr(tmp_path / "a.hdr")
     e = rgbe[..., 3].astype(np.int64)
     want = np.where(e[..., None] > 0, rgbe[..., :3] * np.ldexp(1.0, e - 136)[..., None], 0.0)
     np.testing.assert_array_equal(got, want.astype(np.float32))
     np.testing.assert_array_equal(I.read_rgba(tmp_path / "a.hdr")[..., :3], got)
+
+
+@pytest.mark.parametrize("rle", [True, False])
+def test_hdr_reader_rejects_truncated_files(tmp_path, rle):
+    from falcor_amd import image_io as I
+    rng = np.random.default_rng(2)
+    rgbe = rng.integers(1, 256, (4, 16, 4), dtype=np.uint8)
+    data = _rgbe_encode(rgbe, rle)
+    header = data.index(b"+X 16\n") + len(b"+X 16\n")
+    for cut in (header + 1, header + 3, header + 17, len(data) - 1):
+        (tmp_path / "t.hdr").write_bytes(data[:cut])
+        with pytest.raises(ValueError, match="corrupt HDR scanline"):
+            I.read_hdr(tmp_path / "t.hdr")
+    (tmp_path / "h.hdr").write_bytes(b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n")
+    with pytest.raises(ValueError, match="truncated HDR header"):
+        I.read_hdr(tmp_path / "h.hdr")
 
 
 def test_reference_default_hdr_env_map():
