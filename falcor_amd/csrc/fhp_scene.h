@@ -28,6 +28,10 @@ FHP_HD bool childIsLeaf(uint32_t c) { return (c & kLeafBit) != 0u; }
 FHP_HD uint32_t leafFirst(uint32_t c) { return c & 0x0fffffffu; }
 FHP_HD uint32_t leafCount(uint32_t c) { return ((c >> 28) & 7u) + 1u; }
 FHP_HD uint32_t makeLeaf(uint32_t first, uint32_t count) { return kLeafBit | ((count - 1u) << 28) | first; }
+// What a pop from beyond a traversal stack's capacity returns (the overflow is already reported):
+// the leaf of triangle 0 alone, which every scene has. Testing it reads in-bounds memory and can
+// only report a true intersection.
+constexpr uint32_t kOverflowLeaf = kLeafBit | 0u;
 
 struct CameraDev {
     f3 pos, U, V, W;

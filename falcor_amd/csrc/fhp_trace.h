@@ -30,9 +30,10 @@ typedef __attribute__((address_space(1))) uint32_t GlobalU32;
 #endif
 
 // A traversal stack of capacity() = ldsDepth + ovfDepth entries. The scene builder sizes it to
-// the tree's exact worst case, and every push is checked against the capacity anyway: a ray
-// that would overflow ends without a hit and sets the scene's trace error word, so a stale
-// bound fails loudly instead of writing out of bounds.
+// the tree's exact worst case; the overflow part is bounds checked anyway (the LDS part needs no
+// check: its index test is the branch that selects it). A push beyond the capacity is dropped and
+// sets the scene's trace error word, and a pop beyond it returns the leaf of triangle 0, so a
+// stale bound fails loudly (the ray may miss hits) instead of reading or writing out of bounds.
 struct TraceStack {
     LdsU32* lds;         // this lane's slot 0; entry i at lds[i * ldsStride]
     uint32_t ldsStride;  // threads per block
@@ -48,20 +49,23 @@ struct TraceStack {
 #ifdef FHP_HOST_CHECK
         if (i + 1 > peak) peak = i + 1;
 #endif
-        if (i < ldsDepth) lds[i * ldsStride] = v;
-        else ovf[(i - ldsDepth) * ovfStride] = v;
+        if (i < ldsDepth) {
+            lds[i * ldsStride] = v;
+        } else if (i - ldsDepth < ovfDepth) {
+            ovf[(i - ldsDepth) * ovfStride] = v;
+        } else {
+#ifdef FHP_HOST_CHECK
+            if (err) *err |= kTraceErrStackOverflow;
+#else
+            if (err) atomicOr(err, kTraceErrStackOverflow);
+#endif
+        }
     }
     FHP_DEVI uint32_t get(uint32_t i) const {
-        return i < ldsDepth ? lds[i * ldsStride] : ovf[(i - ldsDepth) * ovfStride];
+        if (i < ldsDepth) return lds[i * ldsStride];
+        return i - ldsDepth < ovfDepth ? ovf[(i - ldsDepth) * ovfStride] : kOverflowLeaf;
     }
     FHP_DEVI uint32_t capacity() const { return ldsDepth + ovfDepth; }
-    FHP_DEVI void overflowed() const {
-#ifdef FHP_HOST_CHECK
-        if (err) *err |= kTraceErrStackOverflow;
-#else
-        if (err) atomicOr(err, kTraceErrStackOverflow);
-#endif
-    }
 };
 
 // Stack of one lane of a persistent traversal grid: LDS entries [entry][lane] in the block's
@@ -172,15 +176,8 @@ struct TravState {
                 sortPair(t1, c1, t3, c3);
                 sortPair(t1, c1, t2, c2);
                 // The scene builder sizes the stack to the tree's exact worst case
-                // (sum of (children - 1) along any root-to-leaf path); the check below only
-                // fires when that bound is broken (stale tree or launch configuration).
+                // (sum of (children - 1) along any root-to-leaf path).
                 const uint32_t k = popc32(mask);
-                if (sp + k - 1u > st.capacity()) {
-                    st.overflowed();
-                    hit.tri = kInvalid;
-                    sp = 0;
-                    return false;
-                }
                 if (k > 3u) st.put(sp++, c3);
                 if (k > 2u) st.put(sp++, c2);
                 if (k > 1u) st.put(sp++, c1);

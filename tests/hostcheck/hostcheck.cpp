@@ -14,12 +14,15 @@
 //     primal <W> <H> <spp> <frame> <maxBounces> <lds>
 //     adjoint <W> <H> <spp> <frame> <maxBounces> <lds> <hash>
 //     geom <W> <H> <spp> <frame> <aux> <log10kappa> <log10kappaScreen> <mesh> <hash> <lds>
+// A primal step may end with cap=<n>: the lane's stack then holds n entries in total (fewer than
+// the tree needs), to check that overflow is reported without any out-of-bounds access.
 // Prints one line per step: rays, the deepest stack entry used, the stack capacity and the
 // trace error bits; exits non-zero when a step sets a trace error bit.
 #include "../../falcor_amd/csrc/fhp_war_geom.h"
 #include "../../falcor_amd/csrc/fhp_war_path.h"
 #include "../../falcor_amd/csrc/scene_build.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -111,8 +114,12 @@ struct LaneStack {
     std::unique_ptr<uint32_t[]> lds, ovf;
     uint32_t err = 0;
     TraceStack st{};
-    LaneStack(uint32_t ldsDepth, uint32_t stackNeed) {
-        const uint32_t need = stackNeed + 1;  // as buildAndUpload sizes it
+    LaneStack(uint32_t ldsDepth, uint32_t stackNeed, uint32_t cap = 0) {
+        uint32_t need = stackNeed + 1;  // as buildAndUpload sizes it
+        if (cap) {
+            need = cap;
+            ldsDepth = std::min(ldsDepth, cap);
+        }
         const uint32_t ovfDepth = need > ldsDepth ? need - ldsDepth : 0u;
         lds.reset(new uint32_t[ldsDepth]);
         if (ovfDepth) ovf.reset(new uint32_t[ovfDepth]);
@@ -216,6 +223,12 @@ int main(int argc, char** argv) {
                 const bool adj = op == "adjoint";
                 if (adj) in >> hash;
                 if (!in) throw std::runtime_error("bad step: " + line);
+                uint32_t cap = 0;
+                std::string opt;
+                if (!adj && (in >> opt)) {
+                    if (opt.rfind("cap=", 0) != 0) throw std::runtime_error("bad step option: " + opt);
+                    cap = uint32_t(std::stoul(opt.substr(4)));
+                }
                 if (adj && ndl != 3 * size_t(W) * H) throw std::runtime_error("dLdI size does not match the frame");
                 std::vector<float4> out(adj ? 0 : size_t(W) * H * spp);
                 std::vector<float> grad(size_t(hash) * ck.h.nM * 20, 0.f);
@@ -229,7 +242,7 @@ int main(int argc, char** argv) {
                 a.hashSize = hash;
                 a.matGradDim = ck.h.nM * 20;
                 a.gradTex = gradTex.empty() ? nullptr : gradTex.data();
-                LaneStack ls(lds, ck.h.stackNeed);
+                LaneStack ls(lds, ck.h.stackNeed, cap);
                 unsigned long long nc = 0, nv = 0;
                 uint32_t peak = 0;
                 const uint64_t n = uint64_t(W) * H * spp;

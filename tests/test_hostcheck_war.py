@@ -15,6 +15,8 @@ out-of-bounds access aborts the program with the sanitizer's report. The checks:
 * shallow LDS stacks (2 and 4 entries) so traversal runs through the overflow part of the stack;
 * multi-bounce primal and adjoint paths on a sphere field and on textured spheres.
 Each step must also report no trace error (no ray needed more stack than the tree's bound).
+A stack made smaller than the tree needs must report the overflow, still without any
+out-of-bounds access.
 """
 import os
 import re
@@ -94,3 +96,13 @@ def test_multi_bounce_paths_in_bounds(which):
     W = H = 48
     _run(data, [f"primal {W} {H} 4 0 4 16", f"adjoint {W} {H} 4 1 4 16 16", f"adjoint {W} {H} 2 2 1 4 3"],
          np.ones((H, W, 3), np.float32))
+
+
+def test_undersized_stack_reports_overflow_in_bounds(box_sphere):
+    data, _ = box_sphere
+    W = H = 32
+    rc, out, err = hostcheck.run(data, [f"primal {W} {H} 2 0 2 2 cap=3"], dLdI=_ramp(W, H))
+    assert "ERROR: AddressSanitizer" not in err and "runtime error" not in err, err[-4000:]
+    assert rc == 1, (rc, out, err[-2000:])
+    m = re.search(r"stack_peak=(\d+) stack_capacity=(\d+) trace_errors=(\d+)", out)
+    assert m and int(m.group(2)) == 3 and int(m.group(3)) == 1 and int(m.group(1)) > 3, out

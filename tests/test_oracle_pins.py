@@ -25,9 +25,13 @@ def test_splitmix64_published_vector(oracle_mod):
     assert [int(v) for v in out] == GOLDEN["splitmix64_state0"]
 
 
-def test_xoshiro128ss_hand_derived(oracle_mod):
-    out = oracle_mod.Oracle.xoshiro128ss([1, 2, 3, 4], 2)
-    assert [int(v) for v in out] == GOLDEN["xoshiro128ss_state1234"]
+def test_xoshiro128ss_known_answers(oracle_mod):
+    """Eight outputs from state {1, 2, 3, 4}: the third is the first to depend on rotl(s3, 11), the
+    fifth the first through s2 ^= s1 << 9 (tests/golden/make_known_answers.py)."""
+    want = GOLDEN["xoshiro128ss_state1234"]
+    assert len(want) == 8
+    out = oracle_mod.Oracle.xoshiro128ss([1, 2, 3, 4], len(want))
+    assert [int(v) for v in out] == want
 
 
 def test_interleave_known_answers(oracle_mod):
@@ -67,6 +71,12 @@ def test_sample_generator_statistics(oracle_mod):
     assert abs(vals.mean() - 0.5) < 0.01
     c = np.corrcoef(vals[:, 0], vals[:, 1])[0, 1]
     assert abs(c) < 0.1
+
+
+def test_jenkins_hash_known_answers(oracle_mod):
+    k = GOLDEN["jenkins"]
+    got = oracle_mod.Oracle.jenkins(np.array(k["inputs"], np.uint32))
+    assert [int(v) for v in got] == k["hashes"]
 
 
 def test_jenkins_hash_is_a_permutation(oracle_mod):
@@ -232,8 +242,8 @@ def test_pbrt_diffuse_known_answer(oracle_mod):
         f1 = oracle_mod.Oracle.bsdf_eval(m2, wi[None], wo[None])[0, :3]
         g = (f1.sum() - f0.sum()) / eps
         assert abs(g - k["grad_albedo"][c]) < k["tol"]
-    # d/d wo.z of sum f = sum(albedo) / pi
-    assert abs(np.sum(k["base_color"]) / math.pi - k["grad_wo_z"]) < k["tol"]
+    # Lambert: sum f = sum(albedo) / pi * wo.z, so its wo.z derivative is the reference's 0.5411268.
+    assert abs(f0.sum() / wo[2] - k["grad_wo_z"]) < k["tol"]
 
 
 # --------------------------------------------------------------------------- lights
@@ -356,6 +366,95 @@ def test_war_primal_furnace_mean(oracle_mod):
     img, st = sc.render_war(16, 16, spp=64, max_bounces=0, brute=True)
     assert abs(img.mean() - 0.6 * 17.0) < 0.02 * 0.6 * 17.0
     assert st[1] > 0
+
+
+def _emissive_ramp_scene():
+    """A camera-facing emissive quad (x, y in [-1, 1] at z = 0) whose emissive texture is the ramp
+    Le = 1 + 4u = 3 + 2x, seen from (0, 0, 3) with a 20 degree vertical field of view: every camera
+    ray hits it inside u in [0.24, 0.76], where bilinear filtering reproduces the ramp exactly.
+    Nothing else is in the scene and the base colour is black, so a pixel sample's radiance is the
+    emission at its primary hit (with NEE off: a light sample on the quad itself is coplanar with
+    the shading point, where computeNEE's areaToSolidAngle / detach(areaToSolidAngle) is inf / inf
+    in the reference, PTUtils.slang:236-238)."""
+    sb = S.SceneBuilder()
+    tw = 64
+    ramp = 1.0 + 4.0 * (np.arange(tw) + 0.5) / tw
+    tex = sb.add_texture(np.repeat(np.repeat(ramp[None, :, None], 2, axis=0), 3, axis=2).astype(np.float32))
+    m = sb.add_material(S.standard_material(base=(0, 0, 0), roughness=1.0, emissive=(1, 1, 1), emissive_texture=tex))
+    sb.add_mesh_instance(S.TriangleMesh.create_quad((2.0, 2.0)), m, S.Transform.deg((1, 1, 1), (0, 0, 0), (90, 0, 0)))
+    sb.camera = S.Camera((0, 0, 3), (0, 0, 0), (0, 1, 0), math.radians(20.0), 1.0)
+    return sb.build()
+
+
+def test_war_primal_pixel_jitter_known_answer(oracle_mod):
+    """tracePaths (WARDiffPathTracer.rt.slang:114-143): SampleGenerator(pixel, frameSeed / 2), three
+    2D samples discarded, Gaussian jitter sigma 0.5 from the next two (sampleGaussian :60-63,
+    Box-Muller), negated when frameSeed is even (antithetic). On the emissive ramp a sample's value
+    is 3 + 6 tan(10 deg) ndc_x of its jittered screen position, predicted here from the independently
+    pinned sample generator; a wrong discard count, sigma, flip parity or axis fails it."""
+    sc = oracle_mod.OracleScene(_emissive_ramp_scene())
+    W = H = 24
+    k = 6.0 * math.tan(math.radians(10.0))
+    for frame in range(4):  # spp = 1: frameSeed = frame
+        img, _ = sc.render_war(W, H, spp=1, frame=frame, max_bounces=0, nee=False)
+        want = np.zeros((H, W))
+        for py in range(H):
+            for px in range(W):
+                u = oracle_mod.Oracle.sample_gen(px, py, frame // 2, 8).astype(np.float64)
+                jx = math.sqrt(-2.0 * math.log(u[6])) * math.cos(2.0 * math.pi * u[7]) * 0.5
+                if frame % 2 == 0:
+                    jx = -jx
+                want[py, px] = 3.0 + k * (2.0 * (px + 0.5 + jx) / W - 1.0)
+        np.testing.assert_allclose(img[..., 0], want, rtol=2e-5, atol=0)
+        assert np.array_equal(img[..., 0], img[..., 1]) and np.array_equal(img[..., 0], img[..., 2])
+    # The antithetic pair of a seed (frames 2j, 2j + 1) averages to the pixel centre's value.
+    a, _ = sc.render_war(W, H, spp=1, frame=0, max_bounces=0, nee=False)
+    b, _ = sc.render_war(W, H, spp=1, frame=1, max_bounces=0, nee=False)
+    centre = 3.0 + k * (2.0 * (np.arange(W) + 0.5) / W - 1.0)
+    np.testing.assert_allclose(0.5 * (a[..., 0] + b[..., 0]), np.broadcast_to(centre, (H, W)), rtol=2e-5)
+
+
+def test_war_primal_converges_to_fp64_minimal_path_tracer(oracle_mod):
+    """The WAR primal and MinimalPathTracer at the same maxBounces (1) estimate the same light
+    transport on the closed box: both bound a path at the primary hit plus maxBounces + 1 scatter
+    vertices (PTUtils.slang:302-307 terminates at length >= maxBounces + 1 after the length++ of
+    each scatter; MinimalPathTracer.rt.slang:342-346, :451 at pathLength >= maxBounces with the
+    primary scatter not counted). WAR uses NEE + MIS at both shading vertices, the FP64
+    MinimalPathTracer BSDF sampling only. WAR's pixels are Gaussian-filtered (sigma 0.5 px), so
+    the FP64 image is rendered at 4 x 4 sample positions per pixel and filtered with the same
+    Gaussian; compared are the interior pixels whose filter support does not reach the directly
+    visible light (whose edge the 4 x 4 quadrature cannot resolve at 17x the wall radiance). A
+    missing NEE or MIS term at the second vertex biases the mean by several percent; a wrong
+    jitter width moves the wall edges."""
+    data = S.closed_box(0)
+    sc = oracle_mod.OracleScene(data)
+    W = H = 24
+    SS = 4
+    war, _ = sc.render_war(W, H, spp=2048, frame=0, max_bounces=1)
+    his = [sc.render_mpt_f64(SS * W, SS * H, spp=128, frame=100 + f, max_bounces=1) for f in range(2)]
+    c = (np.arange(SS * W) + 0.5) / SS
+    bright = np.maximum(his[0], his[1]).max(-1) > 2.0
+    refs = np.zeros((2, H, W, 3))
+    mask = np.zeros((H, W), bool)
+    for py in range(2, H - 2):
+        wy = np.exp(-((c - (py + 0.5)) ** 2) / (2 * 0.25))
+        near_y = np.abs(c - (py + 0.5)) < 2.0
+        for px in range(2, W - 2):
+            wx = np.exp(-((c - (px + 0.5)) ** 2) / (2 * 0.25))
+            w = wy[:, None] * wx[None, :]
+            for k in range(2):
+                refs[k, py, px] = (his[k] * w[..., None]).sum((0, 1)) / w.sum()
+            mask[py, px] = not (bright & near_y[:, None] & (np.abs(c - (px + 0.5)) < 2.0)[None, :]).any()
+    assert mask.sum() > 250
+    a = war[mask].astype(np.float64)
+    ref = 0.5 * (refs[0] + refs[1])
+    b = ref[mask]
+    assert abs(a.mean() - b.mean()) < 0.01 * b.mean(), (a.mean(), b.mean())
+    # Per pixel, the difference is the two estimates' noise: the FP64 average's own noise is half
+    # the RMS of its two independent halves' difference (WAR's is much smaller).
+    noise = 0.5 * np.sqrt(((refs[0][mask] - refs[1][mask]) ** 2).mean())
+    rms = np.sqrt(((a - b) ** 2).mean())
+    assert rms < 1.5 * noise, (rms, noise)
 
 
 def test_oracle_bvh_matches_brute_force(oracle_mod):
