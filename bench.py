@@ -6,19 +6,27 @@ A step is one forward frame of the sphere-field workload: 1024 icospheres (1,310
 MinimalPathTracer with emissive next-event estimation (FP32). `value` is closest-hit plus
 visibility rays per second over the timed steps (Mrays/s), with all inputs resident in HBM.
 
+The timed steps run the product path: the captured bounce loop, no per-kernel events and no host
+synchronisation inside the timed region. The same frames are rendered untimed beforehand for
+their exact ray counts and afterwards with HIP events on the launch stream for kernel durations.
+
 Also reported on the same JSON line:
   * roofline of the dominant kernel (kTraceBounce, the BVH4 traversal of every bounce's
     closest-hit and visibility rays): algorithmic bytes per closest-hit ray (32 B ray + 16 B hit
     + 128 B per visited node + 48 B per tested triangle) and per visibility ray (48 B record +
-    nodes + triangles), node/triangle counts measured on the same frame, over the kernel time
-    from HIP events on its stream;
+    nodes + triangles), node/triangle counts measured on the same frame, over the kernel's average
+    launch duration; beside it the rocprofv3-measured HBM traffic per launch and L2 hit rate
+    (committed profile summary) and the measured HBM fraction of peak;
+  * batch: the batched forward workload (64 views of 512x512, spp in {1, 2, 4, 8, 16}, over 16
+    sphere fields of 10 k - 2 M triangles) in one launch sequence against one sequence per view;
   * diff: ms per forward+adjoint iteration of the differentiable-materials workload (64
     textured spheres, 512x512 RGB fp32 albedo textures, 512x512 image, 16 spp forward +
-    16 spp adjoint), batched vs the plain per-frame loop, and the gradient relative error
-    against central finite differences (eps 1e-3) on 256 texels;
+    16 spp adjoint), batched vs the plain per-frame loop, at maxBounces 1 (and batched at 4),
+    and the gradient relative error against central finite differences (eps 1e-3) on 256 texels;
   * cpu_baseline: the oracle (C++ restatement, all host cores) on the same forward workload.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--skip-cpu] [--skip-diff] [--tune [--tune-out F]]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--skip-cpu] [--skip-diff] [--skip-batch]
+                       [--only forward|batch|diff] [--tune [--tune-out F]]
        python bench.py --copy-calibration   (1 GiB copies: scale for rocprofv3 FETCH_SIZE/WRITE_SIZE)
 """
 from __future__ import annotations
@@ -61,7 +69,22 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def profiled_l2_hit_rate():
+    """L2 hit rate over the traversal launches from the committed rocprofv3 TCC_HIT / TCC_MISS summary."""
+    try:
+        for line in TRAFFIC_PROFILE.read_text().splitlines():
+            if line.startswith("l2_hit_rate:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def forward_bench(steps: int, warmup: int, tune: bool, tune_out: str | None):
+    """Times `steps` frames of the product path (captured bounce loop, no per-kernel events, no
+    host synchronisation inside the timed region). The same frames are rendered untimed before:
+    once with ray statistics (exact counts: a frame's rays depend only on its seeds) and once with
+    per-launch HIP events on the launch stream (kernel durations for the roofline)."""
     import torch
     from falcor_amd import autotune
     from falcor_amd import scene as S
@@ -78,7 +101,7 @@ def forward_bench(steps: int, warmup: int, tune: bool, tune_out: str | None):
     log(f"scene: {info['num_triangles']} tris, {info['num_active_lights']} lights, bvh {info['bvh_nodes']} nodes "
         f"depth {info['bvh_depth']} stack {info['stack_need']}; build {t2 - t1:.2f}s (geometry {t1 - t0:.2f}s)")
     props = {"maxBounces": B, "useEmissiveNEE": True, "emissiveSampler": "LightBVH"}
-    p = MinimalPathTracer(dict(props, timeKernels=1))
+    p = MinimalPathTracer(dict(props))
     p.set_scene(sc)
     out = torch.empty((H, W, 4), device="cuda")
     launch = autotune.load("MinimalPathTracer", TUNE_TAG)
@@ -87,10 +110,7 @@ def forward_bench(steps: int, warmup: int, tune: bool, tune_out: str | None):
     launch = {k: int(p.get_property(k)) for k in autotune.LAUNCH_KEYS}
     if tune:
         # Untimed: pick the traversal launch configuration on this frame shape.
-        p.set_property("timeKernels", 0)
-        tr = autotune.tune(p, lambda i: p.render(W, H, SPP, frame=1000 + i, out=out),
-                           reps=2, warmup=1)
-        p.set_property("timeKernels", 1)
+        tr = autotune.tune(p, lambda i: p.render(W, H, SPP, frame=1000 + i, out=out), reps=2, warmup=1)
         launch = dict(tr["best"])
         table = sorted(tr["table"], key=lambda r: r[1])
         log("autotune (fastest first): " + "; ".join(
@@ -98,22 +118,32 @@ def forward_bench(steps: int, warmup: int, tune: bool, tune_out: str | None):
             for c, ms in table))
         if tune_out:
             autotune.save("MinimalPathTracer", TUNE_TAG, launch, path=tune_out)
-    for _ in range(warmup):
-        p.render(W, H, SPP, out=out)
-    torch.cuda.synchronize()
-    p.kernel_times(reset=True)
+        autotune.apply(p, launch)
+    frames = [100 + i for i in range(steps)]
+    for i in range(warmup):
+        p.render(W, H, SPP, frame=10 + i, out=out)
+    # Ray counts of the timed frames (untimed).
     rays_c = rays_v = 0
-    torch.cuda.synchronize()
-    ts = time.perf_counter()
-    for _ in range(steps):
-        _, st = p.render(W, H, SPP, out=out, stats=True)
+    for f in frames:
+        _, st = p.render(W, H, SPP, frame=f, out=out, stats=True)
         rays_c += int(st[0])
         rays_v += int(st[1])
     torch.cuda.synchronize()
+    ts = time.perf_counter()
+    for f in frames:
+        p.render(W, H, SPP, frame=f, out=out)
+    torch.cuda.synchronize()
     te = time.perf_counter()
-    kt = p.kernel_times(reset=True)
     sec = te - ts
     mrays = (rays_c + rays_v) / sec / 1e6
+    # Kernel durations of the same frames (untimed): HIP events around every launch on its stream.
+    pt = MinimalPathTracer(dict(props, timeKernels=1, **launch))
+    pt.set_scene(sc)
+    pt.render(W, H, SPP, frame=10, out=out)
+    pt.kernel_times(reset=True)
+    for f in frames:
+        pt.render(W, H, SPP, frame=f, out=out)
+    kt = pt.kernel_times(reset=True)
     # Traversal counts of one frame (separate, untimed run).
     pc = MinimalPathTracer(dict(props, countTraversal=1, **launch))
     pc.set_scene(sc)
@@ -141,6 +171,9 @@ def forward_bench(steps: int, warmup: int, tune: bool, tune_out: str | None):
     log(f"forward: {mrays:.1f} Mrays/s, {res['ms_per_step']:.2f} ms/frame, {rays_c / steps / 1e6:.2f}M closest + "
         f"{rays_v / steps / 1e6:.2f}M visibility rays/frame; nodes/ray {n_node:.1f} tris/ray {n_tri:.1f} "
         f"(visibility {s_node:.1f} / {s_tri:.1f}); launch {launch}; kernels {kt}")
+    errs = sc.trace_errors()
+    if errs:
+        raise RuntimeError(f"traversal reported errors {errs:#x}")
     return res
 
 
@@ -187,76 +220,103 @@ def batch_bench(reps: int = 2):
                 mrays_batched=rays / batched / 1e3, speedup=seq / batched)
 
 
-def cpu_baseline(data, W, H, spp, B):
+def cpu_baseline(data, W, H, spp, B, budget_s: float = 12.0, max_frames: int = 12):
+    """The oracle (C++ restatement of the same algorithms, scalar BVH traversal) on the host cores:
+    whole frames of the forward workload until about budget_s of wall time."""
     import oracle
     osc = oracle.OracleScene(data)
     threads = os.cpu_count() or 1
+    rays = 0
+    frames = 0
     t = time.perf_counter()
-    _, st = osc.render_mpt(W, H, spp=spp, frame=0, max_bounces=B, nee=True, light_sampler=1, threads=threads)
+    while frames < max_frames and (frames == 0 or time.perf_counter() - t < budget_s):
+        _, st = osc.render_mpt(W, H, spp=spp, frame=frames, max_bounces=B, nee=True, light_sampler=1, threads=threads)
+        rays += int(st[0]) + int(st[1])
+        frames += 1
     dt = time.perf_counter() - t
-    return dict(value=(int(st[0]) + int(st[1])) / dt / 1e6, unit="Mrays/s", cores=threads, kind="port",
-                sample=f"one {W}x{H} frame of the same workload ({spp} spp, {B} bounces, LightBVH NEE), "
-                       f"oracle C++ restatement on a std::thread pool, {dt:.1f} s")
+    return dict(value=rays / dt / 1e6, unit="Mrays/s", cores=threads, kind="port",
+                sample=f"{frames} {W}x{H} frames of the same workload ({spp} spp, {B} bounces, LightBVH NEE), "
+                       f"oracle C++ restatement on a std::thread pool of {threads} threads, {dt:.1f} s")
 
 
-def diff_bench(iters: int = 3, fd_texels: int = 256):
+def diff_bench(iters: int = 3, fd_texels: int = 256, bounces=(1, 4), plain_loop: bool = True):
+    """ms per forward + adjoint iteration of the differentiable-materials workload (64 textured
+    spheres, 512x512 RGB albedo textures, 512x512 image, 16 spp forward + 16 spp adjoint, L2 image
+    loss): one 16-spp launch each way ("batched") against the reference's per-frame loop of 1-spp
+    launches with a host synchronisation per frame ("plain"), at maxBounces 1 and 4 (the
+    reference material driver's setting, sphere_materials_example.py:47). Timed without per-kernel
+    events; kernel times come from a separate untimed iteration. Then the gradient's relative
+    error against central finite differences on 256 texels."""
     import torch
     from falcor_amd import scene as S
     from falcor_amd.render_graph import Scene, SceneGradients, WARDiffPathTracer
 
     W = H = 512
-    SPP, B = 16, 1
+    SPP = 16
     tgt = S.textured_spheres(3, target=True)
     init = S.textured_spheres(3, target=False)
     sc_t = Scene(tgt)
     sc = Scene(init)
-    props = {"maxBounces": B, "useWAR": False}
-    pt = WARDiffPathTracer(props)
-    pt.set_scene(sc_t)
-    target = pt.render(W, H, SPP, frame=1000)[..., :3].clone()
-    p = WARDiffPathTracer(dict(props, timeKernels=1))
-    p.set_scene(sc)
-    g = SceneGradients(sc, hash_size=256, texture_grads=True)
-    p.scene_gradients = g
     n_pix = W * H * 3
-
-    def batched(frame):
-        img = p.render(W, H, SPP, frame=2 * frame)[..., :3]
-        p.dL_dI = (2.0 / n_pix) * (img - target)
-        g.clear_all_grads()
-        p.backward(W, H, SPP, frame=2 * frame + 1)
-        g.aggregate_all_grads()
-        return img
-
-    def plain(frame):
-        # The reference's per-frame loop (scripts/inv-rendering/common.py:80-114): one launch per
-        # sample, host synchronisation after each frame, forward then adjoint.
-        acc = None
-        for k in range(SPP):
-            img = p.render(W, H, 1, frame=frame * 2 * SPP + k)[..., :3]
-            torch.cuda.synchronize()
-            acc = img.clone() if acc is None else acc + img
-        img = acc * (1.0 / SPP)
-        p.dL_dI = (2.0 / n_pix) * (img - target)
-        g.clear_all_grads()
-        for k in range(SPP):
-            p.backward(W, H, 1, frame=frame * 2 * SPP + SPP + k)
-            torch.cuda.synchronize()
-        g.aggregate_all_grads()
-        return img
-
-    for fn in (batched, plain):
-        fn(0)
-    torch.cuda.synchronize()
     res = {}
-    for name, fn in (("batched", batched), ("plain", plain)):
-        t = time.perf_counter()
-        for i in range(iters):
-            fn(i + 1)
+    kt = {}
+    for B in bounces:
+        props = {"maxBounces": B, "useWAR": False}
+        pt = WARDiffPathTracer(props)
+        pt.set_scene(sc_t)
+        target = pt.render(W, H, SPP, frame=1000)[..., :3].clone()
+        p = WARDiffPathTracer(props)
+        p.set_scene(sc)
+        g = SceneGradients(sc, hash_size=256, texture_grads=True)
+        p.scene_gradients = g
+
+        def batched(frame, p=p, g=g, target=target):
+            img = p.render(W, H, SPP, frame=2 * frame)[..., :3]
+            p.dL_dI = (2.0 / n_pix) * (img - target)
+            g.clear_all_grads()
+            p.backward(W, H, SPP, frame=2 * frame + 1)
+            g.aggregate_all_grads()
+            return img
+
+        def plain(frame, p=p, g=g, target=target):
+            # The reference's per-frame loop (scripts/inv-rendering/common.py:80-114): one launch per
+            # sample, host synchronisation after each frame, forward then adjoint.
+            acc = None
+            for k in range(SPP):
+                img = p.render(W, H, 1, frame=frame * 2 * SPP + k)[..., :3]
+                torch.cuda.synchronize()
+                acc = img.clone() if acc is None else acc + img
+            img = acc * (1.0 / SPP)
+            p.dL_dI = (2.0 / n_pix) * (img - target)
+            g.clear_all_grads()
+            for k in range(SPP):
+                p.backward(W, H, 1, frame=frame * 2 * SPP + SPP + k)
+                torch.cuda.synchronize()
+            g.aggregate_all_grads()
+            return img
+
+        fns = (("batched", batched), ("plain", plain)) if B == 1 and plain_loop else (("batched", batched),)
+        for _, fn in fns:
+            fn(0)
         torch.cuda.synchronize()
-        res[name] = (time.perf_counter() - t) / iters * 1e3
-    kt = p.kernel_times(reset=True)
-    log(f"diff: batched {res['batched']:.1f} ms/iter, plain {res['plain']:.1f} ms/iter; kernels {kt}")
+        for name, fn in fns:
+            t = time.perf_counter()
+            for i in range(iters):
+                fn(i + 1)
+            torch.cuda.synchronize()
+            res[f"{name}_b{B}"] = (time.perf_counter() - t) / iters * 1e3
+        # Kernel times of one batched iteration (untimed): HIP events on the launch stream.
+        p.set_property("timeKernels", 1)
+        p.kernel_times(reset=True)
+        batched(iters + 1)
+        kt[f"b{B}"] = p.kernel_times(reset=True)
+        p.set_property("timeKernels", 0)
+        if B == 1:
+            keep = (p, g, target)
+    p, g, target = keep
+    log(f"diff: ms/iter {res}; kernels {kt}")
+    if not fd_texels:
+        return dict(res, kernel_ms=kt)
 
     # Gradient check: central differences of L = sum(dLdI * I) on random texels with nonzero
     # gradient, sampling frozen at the current parameters (the function the adjoint differentiates).
@@ -292,9 +352,9 @@ def diff_bench(iters: int = 3, fd_texels: int = 256):
     fd = np.array(fd)
     rel = float(np.linalg.norm(ad - fd) / max(np.linalg.norm(fd), 1e-30))
     log(f"diff: grad rel-err vs FD on {pick.size} texels = {rel:.2e} ({time.perf_counter() - t:.1f}s)")
-    return dict(ms_per_iter_batched=res["batched"], ms_per_iter_plain=res["plain"],
-                speedup_batched_vs_plain=res["plain"] / res["batched"], grad_rel_err_fd=rel, fd_texels=int(pick.size),
-                kernel_ms=kt)
+    return dict(ms_per_iter_batched=res["batched_b1"], ms_per_iter_plain=res["plain_b1"],
+                speedup_batched_vs_plain=res["plain_b1"] / res["batched_b1"], ms_per_iter_batched_b4=res["batched_b4"],
+                grad_rel_err_fd=rel, fd_texels=int(pick.size), kernel_ms=kt)
 
 
 def main():
@@ -307,8 +367,10 @@ def main():
     ap.add_argument("--tune", action="store_true",
                     help="autotune the traversal launch on this workload (untimed) instead of using the tuned table")
     ap.add_argument("--tune-out", default=None, help="with --tune: JSON file to record the chosen launch in")
-    ap.add_argument("--batch", action="store_true",
-                    help="also measure the batched forward workload (64 views over 16 scenes; adds ~1 min)")
+    ap.add_argument("--skip-batch", action="store_true",
+                    help="skip the batched forward workload (64 views over 16 scenes)")
+    ap.add_argument("--only", choices=("forward", "batch", "diff", "war-iter"), default=None,
+                    help="run one leg only and print its result (profiling runs); no JSON line")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy 3 times (scale for HBM byte counters)")
     args = ap.parse_args()
@@ -330,8 +392,16 @@ def main():
         return
     from falcor_amd import _lib
     _lib.load()
+    if args.only:
+        r = {"forward": lambda: forward_bench(args.steps, args.warmup, args.tune, args.tune_out),
+             "batch": batch_bench, "diff": diff_bench,
+             "war-iter": lambda: diff_bench(iters=1, fd_texels=0, bounces=(1,), plain_loop=False)}[args.only]()
+        log(f"{args.only}: " + json.dumps({k: v for k, v in r.items() if isinstance(v, (int, float, str))}))
+        return
 
     fwd = forward_bench(args.steps, args.warmup, args.tune, args.tune_out)
+    traffic = profiled_traffic()
+    launch_s = fwd["trace_avg_launch_ms"] * 1e-3
     line = {
         "metric": METRIC, "value": round(fwd["mrays"], 3), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(fwd["ms_per_step"], 3), "higher_is_better": True,
@@ -345,8 +415,14 @@ def main():
                    "visibility_tris_per_ray": round(fwd["shadow_tris_per_ray"], 2),
                    "bvh_build_s": round(fwd["build_s"], 2)},
         "roofline": {"bound": "hbm", "achieved": round(fwd["achieved_gbs"], 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(fwd["achieved_gbs"] / HBM_PEAK_GBS, 5), "traffic": profiled_traffic(),
+                     "frac": round(fwd["achieved_gbs"] / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "traffic_source": str(TRAFFIC_PROFILE.relative_to(ROOT)),
+                     "limiter": "L2 / Infinity Cache latency of the BVH pointer chase: the working set is cache "
+                                "resident, so measured HBM traffic is a fraction of the algorithmic bytes",
+                     "measured_hbm_gbs": round(traffic / launch_s / 1e9, 2) if traffic and launch_s > 0 else None,
+                     "measured_hbm_frac": round(traffic / launch_s / 1e9 / HBM_PEAK_GBS, 5)
+                     if traffic and launch_s > 0 else None,
+                     "l2_hit_rate": profiled_l2_hit_rate(),
                      "kernel": "kTraceBounce (BVH4 traversal of the primary, closest-hit and visibility rays)",
                      "bytes_per_launch": round(fwd["bytes_per_launch"]),
                      "bytes_per_ray": round(fwd["bytes_per_ray"], 1), "nodes_per_ray": round(fwd["nodes_per_ray"], 2),
@@ -355,7 +431,7 @@ def main():
                      "avg_launch_ms": round(fwd["trace_avg_launch_ms"], 4),
                      "launches_per_step": fwd["trace_launches_per_step"]},
     }
-    if args.batch:
+    if not args.skip_batch:
         line["batch"] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in batch_bench().items()}
     if not args.skip_diff:
         line["diff"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in diff_bench().items()}

@@ -6,13 +6,15 @@ namespace fhp {
 
 namespace {
 
-// The megakernels hold a whole path (and its adjoint) per lane: unconstrained they take 147 / 183
-// VGPRs (3 / 2 waves per SIMD) and latency hiding suffers. Capping them at 96 VGPRs (5 waves per
-// SIMD) spills some state to scratch but measured 1.47x faster per 512x512, 16 spp
-// forward+adjoint iteration (with 1280 threads per CU, the pass's default launch).
-constexpr int kWarWavesPerEU = 5;
+// The megakernels hold a whole path (and its adjoint) per lane. Capped at 96 VGPRs (5 waves per
+// SIMD) they spill some state to scratch; unconstrained (3 / 2 waves per SIMD) latency hiding
+// suffers more (the cap measured 1.47x faster per 512x512, 16 spp forward+adjoint iteration).
+#ifndef FHP_WAR_WAVES_PER_EU
+#define FHP_WAR_WAVES_PER_EU 5
+#endif
+constexpr int kWarWavesPerEU = FHP_WAR_WAVES_PER_EU;
 
-template <int B, int D, bool ADJ>
+template <int B, int D, bool ADJ, bool FROZEN>
 __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(kWarWavesPerEU))) void kWar(SceneDev s, WarArgs a,
                                                                                                  TraceScratch ts) {
     __shared__ uint32_t lds[D * B];
@@ -20,7 +22,7 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(kWarWavesPerE
     const uint32_t n = a.W * a.H * a.spp;
     const uint32_t G = gridDim.x * B;
     unsigned long long nc = 0, nv = 0;
-    for (uint32_t id = blockIdx.x * B + threadIdx.x; id < n; id += G) warItem<ADJ>(s, a, id, st, nc, nv);
+    for (uint32_t id = blockIdx.x * B + threadIdx.x; id < n; id += G) warItem<ADJ, FROZEN>(s, a, id, st, nc, nv);
     if (a.rays) {
         atomicAdd(&a.rays[0], nc);
         atomicAdd(&a.rays[1], nv);
@@ -50,7 +52,8 @@ hipError_t launchWarPrimal(const SceneDev& s, const WarArgs& a, const TraceScrat
     hipError_t e = withTraceConfig(ts, [&](auto Bc, auto Dc) {
         constexpr int B = decltype(Bc)::value, D = decltype(Dc)::value;
         const uint32_t blocks = std::min<uint32_t>(ts.gridBlocks, (n + B - 1) / B);
-        hipLaunchKernelGGL((kWar<B, D, false>), dim3(blocks), dim3(B), 0, st, s, b, ts);
+        if (samplingFrozen(s)) hipLaunchKernelGGL((kWar<B, D, false, true>), dim3(blocks), dim3(B), 0, st, s, b, ts);
+        else hipLaunchKernelGGL((kWar<B, D, false, false>), dim3(blocks), dim3(B), 0, st, s, b, ts);
         return hipGetLastError();
     });
     if (tm) tm->end(st);
@@ -66,12 +69,12 @@ hipError_t launchWarAdjoint(const SceneDev& s, const WarArgs& a, const TraceScra
     hipError_t e = withTraceConfig(ts, [&](auto Bc, auto Dc) {
         constexpr int B = decltype(Bc)::value, D = decltype(Dc)::value;
         const uint32_t blocks = std::min<uint32_t>(ts.gridBlocks, (n + B - 1) / B);
-        hipLaunchKernelGGL((kWar<B, D, true>), dim3(blocks), dim3(B), 0, st, s, a, ts);
+        if (samplingFrozen(s)) hipLaunchKernelGGL((kWar<B, D, true, true>), dim3(blocks), dim3(B), 0, st, s, a, ts);
+        else hipLaunchKernelGGL((kWar<B, D, true, false>), dim3(blocks), dim3(B), 0, st, s, a, ts);
         return hipGetLastError();
     });
     if (tm) tm->end(st);
-    if (e != hipSuccess) return e;
-    return hipGetLastError();
+    return e;
 }
 
 }  // namespace fhp

@@ -139,9 +139,18 @@ FHP_DEV void flushGrad(const SceneDev& s, const WarArgs& a, uint32_t hashRow, co
 
 // One path of WARDiffPathTracer (tracePaths + tracePath). In ADJ mode, Ltot is the path radiance
 // from a previous primal run and gradients are accumulated; returns the path radiance.
-template <bool ADJ>
+//
+// Each vertex first computes the differentiable intersection, draws the light sample and traces
+// its visibility ray, and only then loads the shading data and material and evaluates emission,
+// NEE and the scatter sample: the heavy per-vertex state (shading frame, material, gradient
+// accumulator, texture taps) is never live across a traversal, which keeps the kernels within
+// their register budget. The random numbers are drawn and the radiance terms summed in the
+// reference's order (emission, NEE, scatter), so the results do not depend on this order of
+// evaluation. FROZEN: the finite-difference mode's separate sampling parameters (matsS /
+// texelsS) are in use; otherwise one material serves evaluation and sampling.
+template <bool ADJ, bool FROZEN>
 FHP_DEV f3 warPath(const SceneDev& s, const WarArgs& a, uint32_t px, uint32_t py, uint32_t frameSeed, TraceStack& st,
-                      f3 Ltot, f3 dL, uint32_t hashRow, unsigned long long& nc, unsigned long long& nv) {
+                   f3 Ltot, f3 dL, uint32_t hashRow, unsigned long long& nc, unsigned long long& nv) {
     Rng rng = rngSeed(px, py, frameSeed / 2);
     for (int i = 0; i < 6; i++) rng.u1();
     const float pfx = float(px) + 0.5f, pfy = float(py) + 0.5f;
@@ -166,38 +175,62 @@ FHP_DEV f3 warPath(const SceneDev& s, const WarArgs& a, uint32_t px, uint32_t py
     if (!traceBVH4<false, false>(s, rayO, rayD, 0.f, kFltMax, st, h)) return radiance;
     for (;;) {
         // computeIntersectionAD, AttachToRay (DiffSceneQuery.slang:131-219).
-        const uint32_t i0 = s.idx[3 * h.tri], i1 = s.idx[3 * h.tri + 1], i2 = s.idx[3 * h.tri + 2];
-        f3 v0, v1, v2, n0, n1, n2;
-        float tu, tv;
-        loadVertex(s, i0, v0, n0, tu, tv);
-        loadVertex(s, i1, v1, n1, tu, tv);
-        loadVertex(s, i2, v2, n2, tu, tv);
-        f3 e1 = v1 - v0, e2 = v2 - v0;
-        f3 pvec = cross(rayD, e2);
-        float det = dot(e1, pvec);
-        f3 tvec = rayO - v0;
-        f3 qvec = cross(tvec, e1);
-        float tt = dot(e2, qvec) / det;
-        f3 p = rayO + rayD * tt;
-        f3 ep0 = v0 - p, ep1 = v1 - p, ep2 = v2 - p;
-        float A0 = length(cross(ep1, ep2)), A1 = length(cross(ep2, ep0)), A2 = length(cross(ep0, ep1));
-        float At = A0 + A1 + A2;
-        f3 bary = mk3(A0 / At, A1 / At, A2 / At);
-        f3 posW = v0 * bary.x + v1 * bary.y + v2 * bary.z;
-        f3 normalW = n0 * bary.x + n1 * bary.y + n2 * bary.z;
+        f3 posW, normalW, bary;
+        {
+            const uint32_t i0 = s.idx[3 * h.tri], i1 = s.idx[3 * h.tri + 1], i2 = s.idx[3 * h.tri + 2];
+            f3 v0, v1, v2, n0, n1, n2;
+            float tu, tv;
+            loadVertex(s, i0, v0, n0, tu, tv);
+            loadVertex(s, i1, v1, n1, tu, tv);
+            loadVertex(s, i2, v2, n2, tu, tv);
+            f3 e1 = v1 - v0, e2 = v2 - v0;
+            f3 pvec = cross(rayD, e2);
+            float det = dot(e1, pvec);
+            f3 tvec = rayO - v0;
+            f3 qvec = cross(tvec, e1);
+            float tt = dot(e2, qvec) / det;
+            f3 p = rayO + rayD * tt;
+            f3 ep0 = v0 - p, ep1 = v1 - p, ep2 = v2 - p;
+            float A0 = length(cross(ep1, ep2)), A1 = length(cross(ep2, ep0)), A2 = length(cross(ep0, ep1));
+            float At = A0 + A1 + A2;
+            bary = mk3(A0 / At, A1 / At, A2 / At);
+            posW = v0 * bary.x + v1 * bary.y + v2 * bary.z;
+            normalW = n0 * bary.x + n1 * bary.y + n2 * bary.z;
+        }
+        const bool stop = len >= a.maxBounces + 1;
+        // generateEmissiveSample + traceVisibilityRay (PTUtils.slang:177-207, :310-319).
+        LightSample ls;
+        bool lit = false;
+        f3 Li = mk3(0.f), ldir = mk3(0.f);
+        if (!stop && a.useNEE && sampleLight(s, LS_UNIFORM, posW, normalW, rng, ls)) {
+            Li = ls.pdf > 0.f ? ls.Le / ls.pdf : mk3(0.f);
+            const f3 lightPos = rayOriginOffset(ls.posW, ls.normalW);
+            const f3 origin = rayOriginOffset(posW, normalW);
+            const f3 toL = lightPos - origin;
+            const float distance = length(toL);
+            ldir = normalize(toL);
+            if (anyPos(Li)) {
+                HitRec sh;
+                nv++;
+                lit = !traceBVH4<true, false>(s, origin, ldir, 0.f, distance, st, sh);
+            }
+        }
         // PTUtils.slang:97-123
         Shading sd = shadingFromHit(s, h.tri, bary.y, bary.z, rayD);
         sd.V = -rayD;
         setFrame(sd, normalW);
         TexTap tap;
-        const Shading sd0 = sd;
-        Material m = loadMaterial(s, s.mats, s.texels, sd0, sd, &tap);
-        // Sampling parameters differ from the evaluated ones only in the finite-difference mode.
-        Material mS = m;
-        if (s.matsS != s.mats || s.texelsS != s.texels) {
-            Shading unused = sd0;
-            mS = loadMaterial(s, s.matsS, s.texelsS, sd0, unused, nullptr);
+        Material m, mS;
+        {
+            const Shading sd0 = sd;
+            m = loadMaterial(s, s.mats, s.texels, sd0, sd, ADJ ? &tap : nullptr);
+            // Sampling parameters differ from the evaluated ones only in the finite-difference mode.
+            if (FROZEN) {
+                Shading unused = sd0;
+                mS = loadMaterial(s, s.matsS, s.texelsS, sd0, unused, nullptr);
+            }
         }
+        const Material& mSamp = FROZEN ? mS : m;
         MatGrad g;
         if (ADJ) zero(g);
         const bool isPrimary = len == 0;
@@ -216,54 +249,38 @@ FHP_DEV f3 warPath(const SceneDev& s, const WarArgs& a, uint32_t px, uint32_t py
                 g.emitFactor += dot(dE, m.emitColor);
             }
         }
-        bool stop = len >= a.maxBounces + 1;
-        if (!stop && a.useNEE) {
-            LightSample ls;
-            if (sampleLight(s, LS_UNIFORM, posW, normalW, rng, ls)) {
-                f3 Li = ls.pdf > 0.f ? ls.Le / ls.pdf : mk3(0.f);
-                f3 lightPos = rayOriginOffset(ls.posW, ls.normalW);
-                f3 origin = rayOriginOffset(posW, normalW);
-                f3 toL = lightPos - origin;
-                float distance = length(toL);
-                f3 ldir = normalize(toL);
-                if (anyPos(Li)) {
-                    HitRec sh;
-                    nv++;
-                    if (!traceBVH4<true, false>(s, origin, ldir, 0.f, distance, st, sh)) {
-                        // computeNEE (PTUtils.slang:211-249)
-                        f3 wo = ls.posW - posW;
-                        float dist = length(wo);
-                        wo = wo / dist;
-                        float a2s = (dist * dist) / dot(ls.normalW, -wo);
-                        f3 f = matEvalGated(sd, m, mS, wo);
-                        f3 w = Li * f * a2s / a2s;
-                        float misN = 1.f;
-                        if (a.useMIS) {
-                            float sp;
-                            if (a.useBSDFSampling || m.type == MAT_PBRT_DIFFUSE) sp = matPdf(sd, mS, ldir);
-                            else {
-                                f3 wiL = sd.toLocal(sd.V), woL = sd.toLocal(ldir);
-                                sp = (validRT(sd, wiL, woL, ldir) && fminf_(wiL.z, woL.z) >= kMinCosTheta) ? woL.z * kInvPi : 0.f;
-                            }
-                            misN = misPower(ls.pdf, sp);
-                            w = w * misN;
-                        }
-                        radiance = radiance + thp * w;
-                        if (ADJ) bsdfAdjoint(sd, m, wo, dL * thp * Li * (a2s / a2s) * misN, g);
-                    }
+        if (lit) {
+            // computeNEE (PTUtils.slang:211-249)
+            f3 wo = ls.posW - posW;
+            float dist = length(wo);
+            wo = wo / dist;
+            float a2s = (dist * dist) / dot(ls.normalW, -wo);
+            f3 f = FROZEN ? matEvalGated(sd, m, mS, wo) : matEval(sd, m, wo);
+            f3 w = Li * f * a2s / a2s;
+            float misN = 1.f;
+            if (a.useMIS) {
+                float sp;
+                if (a.useBSDFSampling || m.type == MAT_PBRT_DIFFUSE) sp = matPdf(sd, mSamp, ldir);
+                else {
+                    f3 wiL = sd.toLocal(sd.V), woL = sd.toLocal(ldir);
+                    sp = (validRT(sd, wiL, woL, ldir) && fminf_(wiL.z, woL.z) >= kMinCosTheta) ? woL.z * kInvPi : 0.f;
                 }
+                misN = misPower(ls.pdf, sp);
+                w = w * misN;
             }
+            radiance = radiance + thp * w;
+            if (ADJ) bsdfAdjoint(sd, m, wo, dL * thp * Li * (a2s / a2s) * misN, g);
         }
         bool cont = false;
         f3 fS = mk3(0.f), woS = mk3(0.f);
         if (!stop) {
             // generateScatterRay (PTUtils.slang:136-172)
             Scatter sc;
-            if (matSample(sd, mS, rng, sc, a.useBSDFSampling != 0) && sc.pdf > 0.f) {
+            if (matSample(sd, mSamp, rng, sc, a.useBSDFSampling != 0) && sc.pdf > 0.f) {
                 rayD = sc.wo;
                 // PTUtils.slang:153: the view side, also for transmitted directions.
                 rayO = rayOriginOffset(posW, sd.frontFacing ? sd.faceN : -sd.faceN);
-                fS = matEvalGated(sd, m, mS, rayD);
+                fS = FROZEN ? matEvalGated(sd, m, mS, rayD) : matEval(sd, m, rayD);
                 woS = rayD;
                 thp = thp * (fS / sc.pdf);
                 prevN = sd.orientedFaceN();
@@ -289,27 +306,31 @@ FHP_DEV f3 warPath(const SceneDev& s, const WarArgs& a, uint32_t px, uint32_t py
 }
 
 // One item of the kernels' grid loop (path id = pixel * spp + sample): the primal writes the
-// path's radiance; the adjoint traces the path, then replays it from the same seed.
-template <bool ADJ>
+// path's radiance; the adjoint traces the path, then replays it from the same seed (one launch:
+// splitting the two passes into launches measured slower).
+template <bool ADJ, bool FROZEN>
 FHP_DEVI void warItem(const SceneDev& s, const WarArgs& a, uint32_t id, TraceStack& st, unsigned long long& nc,
                       unsigned long long& nv) {
     const uint32_t pix = id / a.spp, k = id % a.spp;
     const uint32_t px = pix % a.W, py = pix / a.W;
     const uint32_t seed = a.frame + k;
     if (!ADJ) {
-        f3 L = warPath<false>(s, a, px, py, seed, st, mk3(0.f), mk3(0.f), 0, nc, nv);
+        f3 L = warPath<false, FROZEN>(s, a, px, py, seed, st, mk3(0.f), mk3(0.f), 0, nc, nv);
         a.out[id] = make_float4(L.x, L.y, L.z, 0.f);
     } else {
         const float* dl = a.dLdI + 3 * size_t(pix);
         f3 dL = mk3(dl[0], dl[1], dl[2]);
         if (dL.x == 0.f && dL.y == 0.f && dL.z == 0.f) return;
         unsigned long long c0 = 0, c1 = 0;
-        f3 L = warPath<false>(s, a, px, py, seed, st, mk3(0.f), mk3(0.f), 0, c0, c1);
-        warPath<true>(s, a, px, py, seed, st, L, dL, jenkins(pix) % a.hashSize, nc, nv);
+        f3 L = warPath<false, FROZEN>(s, a, px, py, seed, st, mk3(0.f), mk3(0.f), 0, c0, c1);
+        warPath<true, FROZEN>(s, a, px, py, seed, st, L, dL, jenkins(pix) % a.hashSize, nc, nv);
         nc += c0;
         nv += c1;
     }
 }
+
+// Whether a scene renders with the finite-difference mode's separate sampling parameters.
+FHP_HD bool samplingFrozen(const SceneDev& s) { return s.matsS != s.mats || s.texelsS != s.texels; }
 
 }  // namespace
 }  // namespace fhp

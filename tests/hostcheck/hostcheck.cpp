@@ -247,8 +247,8 @@ int main(int argc, char** argv) {
                 uint32_t peak = 0;
                 const uint64_t n = uint64_t(W) * H * spp;
                 for (uint32_t id = 0; id < n; id++) {
-                    if (adj) warItem<true>(s, a, id, ls.st, nc, nv);
-                    else warItem<false>(s, a, id, ls.st, nc, nv);
+                    if (adj) warItem<true, false>(s, a, id, ls.st, nc, nv);
+                    else warItem<false, false>(s, a, id, ls.st, nc, nv);
                     peak = std::max(peak, ls.st.peak);
                 }
                 report(op.c_str(), n, nc + nv, ls, peak);
