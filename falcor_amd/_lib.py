@@ -78,6 +78,7 @@ _SIGS = {
     "fhp_scene_destroy": (C.c_int, [C.c_void_p]),
     "fhp_scene_info": (C.c_int, [C.c_void_p, C.POINTER(SceneInfo)]),
     "fhp_scene_trace_errors": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.c_int, C.c_void_p]),
+    "fhp_scene_set_camera": (C.c_int, [C.c_void_p, C.c_void_p]),
     "fhp_scene_update_vertices": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
     "fhp_scene_set_materials": (C.c_int, [C.c_void_p, C.c_void_p]),
     "fhp_scene_set_texels": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),

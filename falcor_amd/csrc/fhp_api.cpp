@@ -824,6 +824,25 @@ int fhp_scene_trace_errors(FhpScene s, uint32_t* bits, int reset, FhpStream stre
     return FHP_OK;
 }
 
+int fhp_scene_set_camera(FhpScene s, const float* h_camera) {
+    if (!s || !h_camera) return fail(FHP_EINVAL, "null argument");
+    for (int i = 0; i < 14; i++)
+        if (!std::isfinite(h_camera[i])) return fail(FHP_EINVAL, "non-finite camera parameter");
+    HostScene& h = s->h;
+    std::memcpy(h.cam, h_camera, sizeof(h.cam));
+    computeCamera(h);
+    SceneDev& dv = s->dev;
+    dv.cam.pos = mk3(h.cam[0], h.cam[1], h.cam[2]);
+    dv.cam.U = mk3(h.camU[0], h.camU[1], h.camU[2]);
+    dv.cam.V = mk3(h.camV[0], h.camV[1], h.camV[2]);
+    dv.cam.W = mk3(h.camW[0], h.camW[1], h.camW[2]);
+    dv.cam.nearZ = h.cam[12];
+    dv.cam.farZ = h.cam[13];
+    dv.viewProjZ = make_float4(h.viewProj[8], h.viewProj[9], h.viewProj[10], h.viewProj[11]);
+    dv.viewProjW = make_float4(h.viewProj[12], h.viewProj[13], h.viewProj[14], h.viewProj[15]);
+    return FHP_OK;
+}
+
 int fhp_scene_info(FhpScene s, FhpSceneInfo* o) {
     if (!s || !o) return fail(FHP_EINVAL, "null argument");
     const HostScene& h = s->h;

@@ -336,6 +336,12 @@ class Scene:
         return nodes, tri[: i["num_active_lights"]], masks[: i["num_lights"]]
 
     # Ray queries (TraceRay / RayQuery replacements).
+    def set_camera(self, camera):
+        """Replaces the scene's camera (a scene.Camera) for later renders."""
+        c = np.ascontiguousarray(camera.pack(), np.float32)
+        _lib.check(_lib.load().fhp_scene_set_camera(self.handle, c.ctypes.data))
+        self.version += 1
+
     TRACE_ERR_STACK_OVERFLOW = 1
 
     def trace_errors(self, reset: bool = True) -> int:

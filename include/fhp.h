@@ -107,6 +107,12 @@ int fhp_scene_create(const FhpSceneDesc* desc, FhpScene* out);
  * on every stream, before they overwrite or free buffers that kernels read. */
 int fhp_scene_destroy(FhpScene scene);
 int fhp_scene_info(FhpScene scene, FhpSceneInfo* out);
+/* Replace the scene's camera (14 floats, the layout of FhpSceneDesc.h_camera: position, target,
+ * up, vertical FOV in radians, aspect, focal distance, near and far plane). Later launches use
+ * it; the camera reaches the kernels by value, so work already queued is unaffected. Replaces
+ * the camera updates of Scene::getCamera() (Camera::setPosition / setTarget / setUpVector /
+ * setFocalLength / setAspectRatio / setDepthRange, Camera.cpp:112-193). */
+int fhp_scene_set_camera(FhpScene scene, const float* h_camera);
 /* Trace error bits accumulated by the scene's traversal kernels since the last reset (bit 0:
  * a ray needed more stack entries than the tree's worst case; that ray ended without a hit).
  * Zero when the scene's acceleration structure and its launch bounds agree. Synchronises the

@@ -468,8 +468,10 @@ uint64_t fo_record_mpt(void* h, uint32_t W, uint32_t H, uint32_t p0, uint32_t p1
 }
 // WARDiffPathTracer primal (useWAR = false), one frame per sample: frameSeed = frame0 + k.
 // flags: bit0 useBSDFSampling, bit1 useNEE, bit2 useMIS, bit3 useAntithetic, bit4 brute.
-void fo_render_war(void* h, uint32_t W, uint32_t H, uint32_t spp, uint32_t frame0, uint32_t maxBounces, uint32_t flags,
-                   int threads, float* out, uint64_t* stats) {
+// WAR primal (WARDiffPathTracer.rt.slang:114-143) of the pixels [x0, x1) x [y0, y1) of a W x H frame
+// (the camera and the pixel seeds are those of the full frame); out holds the rectangle row-major.
+void fo_render_war_rect(void* h, uint32_t W, uint32_t H, uint32_t x0, uint32_t y0, uint32_t x1, uint32_t y1, uint32_t spp,
+                        uint32_t frame0, uint32_t maxBounces, uint32_t flags, int threads, float* out, uint64_t* stats) {
     Scene* s = static_cast<Scene*>(h);
     WarParams P;
     P.maxBounces = maxBounces;
@@ -478,22 +480,29 @@ void fo_render_war(void* h, uint32_t W, uint32_t H, uint32_t spp, uint32_t frame
     P.useMIS = flags & 4;
     P.useAntithetic = flags & 8;
     P.brute = flags & 16;
-    std::vector<TraceStats> rowStats(H);
-    parallelRows(H, hwThreads(threads), [&](uint32_t y) {
-        Tracer tr{*s, P.brute, stats ? &rowStats[y] : nullptr, nullptr};
-        for (uint32_t x = 0; x < W; x++) {
+    const uint32_t RW = x1 - x0, RH = y1 - y0;
+    std::vector<TraceStats> rowStats(RH);
+    parallelRows(RH, hwThreads(threads), [&](uint32_t r) {
+        const uint32_t y = y0 + r;
+        Tracer tr{*s, P.brute, stats ? &rowStats[r] : nullptr, nullptr};
+        for (uint32_t x = x0; x < x1; x++) {
             V3f sum(0.f);
             for (uint32_t k = 0; k < spp; k++) sum = sum + warTracePaths(*s, P, x, y, W, H, frame0 + k, tr);
             V3f o = sum * (1.0f / float(spp));
-            float* q = out + 3 * (size_t(y) * W + x);
+            float* q = out + 3 * (size_t(r) * RW + (x - x0));
             q[0] = o.x; q[1] = o.y; q[2] = o.z;
         }
     });
     if (stats) {
         TraceStats t;
-        for (auto& r : rowStats) { t.closest += r.closest; t.visibility += r.visibility; t.nodes += r.nodes; t.tris += r.tris; }
+        for (auto& rs : rowStats) { t.closest += rs.closest; t.visibility += rs.visibility; t.nodes += rs.nodes; t.tris += rs.tris; }
         stats[0] = t.closest; stats[1] = t.visibility; stats[2] = t.nodes; stats[3] = t.tris;
     }
+}
+
+void fo_render_war(void* h, uint32_t W, uint32_t H, uint32_t spp, uint32_t frame0, uint32_t maxBounces, uint32_t flags,
+                   int threads, float* out, uint64_t* stats) {
+    fo_render_war_rect(h, W, H, 0, 0, W, H, spp, frame0, maxBounces, flags, threads, out, stats);
 }
 
 }  // extern "C"

@@ -71,6 +71,7 @@ def oracle_lib():
             "fo_render_mpt": (None, [vp, u32, u32, u32, u32, u32, u32, i32, i32, vp, vp]),
             "fo_record_mpt": (u64, [vp, u32, u32, u32, u32, u32, u32, u32, i32, u64, vp]),
             "fo_render_war": (None, [vp, u32, u32, u32, u32, u32, u32, i32, vp, vp]),
+            "fo_render_war_rect": (None, [vp, u32, u32, u32, u32, u32, u32, u32, u32, u32, u32, i32, vp, vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -236,6 +237,18 @@ class OracleScene:
         flags = int(compute_direct) | (int(importance) << 1) | (int(nee) << 2) | (int(brute) << 3)
         n = oracle_lib().fo_record_mpt(self.h, W, H, p0, p1, sample_number, max_bounces, flags, light_sampler, cap, _p(rec))
         return rec[: min(n, cap)]
+
+    def render_war_rect(self, W, H, rect, spp=1, frame=0, max_bounces=0, bsdf_sampling=True, nee=True, mis=True,
+                        antithetic=True, brute=False, threads=0):
+        """WAR primal of the pixels rect = (x0, y0, x1, y1) of a W x H frame: [y1 - y0, x1 - x0, 3]."""
+        x0, y0, x1, y1 = (int(v) for v in rect)
+        assert 0 <= x0 < x1 <= W and 0 <= y0 < y1 <= H
+        out = np.zeros((y1 - y0, x1 - x0, 3), np.float32)
+        stats = np.zeros(4, np.uint64)
+        flags = int(bsdf_sampling) | (int(nee) << 1) | (int(mis) << 2) | (int(antithetic) << 3) | (int(brute) << 4)
+        oracle_lib().fo_render_war_rect(self.h, W, H, x0, y0, x1, y1, spp, frame, max_bounces, flags, threads, _p(out),
+                                        _p(stats))
+        return out, stats
 
     def render_war(self, W, H, spp=1, frame=0, max_bounces=0, bsdf_sampling=True, nee=True, mis=True, antithetic=True,
                    brute=False, threads=0):
