@@ -94,6 +94,8 @@ _SIGS = {
     "fhp_pass_execute": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p,
                                    C.c_void_p, C.c_void_p]),
     "fhp_pass_execute_batch": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(View), C.c_void_p, C.c_void_p]),
+    "fhp_pass_execute_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                           C.c_void_p, C.c_void_p]),
     "fhp_pass_kernel_times": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
     "fhp_grads_create": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int, C.POINTER(C.c_void_p)]),
     "fhp_grads_destroy": (C.c_int, [C.c_void_p]),

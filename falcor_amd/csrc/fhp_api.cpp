@@ -8,6 +8,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <random>
 #include <string>
 #include <vector>
@@ -335,6 +336,33 @@ int ensurePassBuffersN(FhpPass p, uint64_t n, uint64_t npix) {
     return FHP_OK;
 }
 }  // namespace
+
+// MinimalPathTracer arguments of a frame from the pass properties and the scene (everything but
+// the wavefront buffers). Returns FHP_OK or an argument error.
+int mptFrameArgs(const FhpPass_* p, const FhpScene_* s, uint32_t W, uint32_t H, uint32_t spp, uint32_t frame, MptArgs& a) {
+    a = MptArgs{};
+    a.W = W; a.H = H; a.spp = spp; a.frame = frame;
+    a.maxBounces = uint32_t(p->get("maxBounces"));
+    a.computeDirect = p->get("computeDirect") != 0;
+    a.importance = p->get("useImportanceSampling") != 0;
+    a.nee = p->get("useEmissiveNEE") != 0;
+    a.lightKind = uint32_t(p->get("emissiveSampler"));
+    a.separateShadow = p->get("splitVisibilityLaunch") != 0;
+    a.envNee = p->get("useEnvNEE") != 0;
+    if (a.nee && !a.computeDirect) return fail(FHP_EINVAL, "useEmissiveNEE requires computeDirect");
+    if (a.envNee && !a.computeDirect) return fail(FHP_EINVAL, "useEnvNEE requires computeDirect");
+    const bool envLight = a.envNee && s->dev.envImp && (s->dev.envFlags & kEnvLight);
+    a.shadowRays = (a.nee || s->dev.nALights || envLight) ? 1u : 0u;
+    a.lightMode = (s->dev.nALights || envLight) ? 2u : (a.nee ? 1u : 0u);
+    a.nPaths = W * H * spp;
+    a.nPix = W * H;
+    if (p->inVbuf && !s->h.meshTriContiguous)
+        return fail(FHP_ESTATE, "a V-buffer input needs each mesh's triangles to be contiguous");
+    a.vbuf = p->inVbuf;
+    a.viewW = p->inViewW;
+    a.keepMisses = s->dev.envTex && (s->dev.envFlags & kEnvLight);
+    return FHP_OK;
+}
 
 int prepareTimer(FhpPass p) {
     p->timer.on = p->get("timeKernels") != 0;
@@ -1119,33 +1147,14 @@ int fhp_pass_execute(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t spp
     TraceScratch ts;
     if ((rc = passTrace(s, p, ts))) return rc;
     if (p->kind == "MinimalPathTracer") {
-        MptArgs a{};
-        a.W = W; a.H = H; a.spp = spp; a.frame = frame;
-        a.maxBounces = uint32_t(p->get("maxBounces"));
-        a.computeDirect = p->get("computeDirect") != 0;
-        a.importance = p->get("useImportanceSampling") != 0;
-        a.nee = p->get("useEmissiveNEE") != 0;
-        a.lightKind = uint32_t(p->get("emissiveSampler"));
-        a.separateShadow = p->get("splitVisibilityLaunch") != 0;
-        a.envNee = p->get("useEnvNEE") != 0;
-        if (a.nee && !a.computeDirect) return fail(FHP_EINVAL, "useEmissiveNEE requires computeDirect");
-        if (a.envNee && !a.computeDirect) return fail(FHP_EINVAL, "useEnvNEE requires computeDirect");
-        const bool envLight = a.envNee && s->dev.envImp && (s->dev.envFlags & kEnvLight);
-        a.shadowRays = (a.nee || s->dev.nALights || envLight) ? 1u : 0u;
-        a.lightMode = (s->dev.nALights || envLight) ? 2u : (a.nee ? 1u : 0u);
-        a.nPaths = W * H * spp;
-        a.nPix = W * H;
+        MptArgs a;
+        if ((rc = mptFrameArgs(p, s, W, H, spp, frame, a))) return rc;
         a.thp = p->thp.as<float4>(); a.prevN = p->prevN.as<float4>(); a.rng = p->rng.as<uint4>();
         a.Lp = p->Lp.as<float4>(); a.Ls = p->Ls.as<float4>();
         a.rayQ[0] = p->rayQ0.as<float4>(); a.rayQ[1] = p->rayQ1.as<float4>();
         a.pathQ[0] = p->pathQ0.as<uint32_t>(); a.pathQ[1] = p->pathQ1.as<uint32_t>();
         a.hitQ = p->hitQ.as<uint4>(); a.hitIdx = p->hitIdx.as<uint32_t>(); a.shQ = p->shQ.as<float4>(); a.pixHit = p->pixHit.as<uint4>();
         a.counters = p->counters.as<uint32_t>();
-        if (p->inVbuf && !s->h.meshTriContiguous)
-            return fail(FHP_ESTATE, "a V-buffer input needs each mesh's triangles to be contiguous");
-        a.vbuf = p->inVbuf;
-        a.viewW = p->inViewW;
-        a.keepMisses = s->dev.envTex && (s->dev.envFlags & kEnvLight);
         if ((rc = prepareTimer(p))) return rc;
         const bool graph = p->get("useGraph") != 0 && !count && !p->timer.on;
         if (!graph) {
@@ -1202,6 +1211,91 @@ int fhp_pass_execute(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t spp
         HIP_OR_FAIL(hipStreamSynchronize(st));
         for (int i = 0; i < 6; i++) h_stats[i] = hs[i];
     }
+    return FHP_OK;
+}
+
+// ------------------------------------------------------------------ allocation-free forward frame
+// fhp_pass_execute_noalloc keeps a MinimalPathTracer frame's wavefront state (path state, ray,
+// hit and visibility queues, primary hits, queue counters) in this file-scope device array
+// instead of pass-owned buffers. Each device holds its own copy (kNoAllocScratchBytes, about
+// 124 MB) from module load until the process ends.
+constexpr uint64_t kNoAllocMaxPaths = 1ull << 19, kNoAllocMaxPixels = 1ull << 19;
+constexpr uint64_t align256(uint64_t x) { return (x + 255u) & ~uint64_t(255u); }
+// thp, prevN, rng, Lp, Ls (16 B each), two ray queues (32 B), two path queues (4 B), hits (16 B),
+// visibility records (48 B), hit indices (4 B) per path; primary hits (16 B) per pixel; counters.
+constexpr uint64_t kNoAllocScratchBytes =
+    5 * align256(16 * kNoAllocMaxPaths) + 2 * align256(32 * kNoAllocMaxPaths) + 2 * align256(4 * kNoAllocMaxPaths) +
+    align256(16 * kNoAllocMaxPaths) + align256(48 * kNoAllocMaxPaths) + align256(4 * kNoAllocMaxPaths) +
+    align256(16 * kNoAllocMaxPixels) + align256(4 * kCounterWords);
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
+namespace {
+std::mutex g_scratchMutex;
+constexpr int kMaxDevices = 64;
+uint8_t* g_scratchPtr[kMaxDevices] = {};
+
+// Device address of this device's scratch array, looked up once per device (with the device
+// current) on the entry point's first call there. In host code the array's name is not a device
+// address; only this pointer is passed to kernels.
+int scratchAddress(uint8_t** out) {
+    int dev = 0;
+    HIP_OR_FAIL(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(FHP_EINVAL, "device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_scratchMutex);
+    if (!g_scratchPtr[dev]) {
+        void* ptr = nullptr;
+        HIP_OR_FAIL(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) return fail(FHP_EHIP, "scratch array has no device address");
+        g_scratchPtr[dev] = static_cast<uint8_t*>(ptr);
+    }
+    *out = g_scratchPtr[dev];
+    return FHP_OK;
+}
+}  // namespace
+
+int fhp_pass_execute_noalloc(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t spp, uint32_t frame, float* d_color,
+                             FhpStream stream) {
+    if (!p || !s || !d_color) return fail(FHP_EINVAL, "null argument");
+    if (p->kind != "MinimalPathTracer") return fail(FHP_EINVAL, "fhp_pass_execute_noalloc runs MinimalPathTracer passes");
+    if (W == 0 || H == 0 || spp == 0) return fail(FHP_EINVAL, "empty frame");
+    const uint64_t nPix = uint64_t(W) * H, nPaths = nPix * spp;
+    if (nPaths > kNoAllocMaxPaths || nPix > kNoAllocMaxPixels)
+        return fail(FHP_EINVAL, "frame too large for fhp_pass_execute_noalloc (at most " + std::to_string(kNoAllocMaxPaths) +
+                                    " paths and " + std::to_string(kNoAllocMaxPixels) + " pixels)");
+    if (p->get("countTraversal") != 0) return fail(FHP_EINVAL, "fhp_pass_execute_noalloc does not count traversal");
+    MptArgs a;
+    int rc = mptFrameArgs(p, s, W, H, spp, frame, a);
+    if (rc) return rc;
+    TraceScratch ts;
+    if ((rc = passTrace(s, p, ts))) return rc;
+    uint8_t* base = nullptr;
+    if ((rc = scratchAddress(&base))) return rc;
+    uint64_t off = 0;
+    auto take = [&](uint64_t bytes) {
+        uint8_t* q = base + off;
+        off += align256(bytes);
+        return q;
+    };
+    const uint64_t n = kNoAllocMaxPaths;
+    a.thp = reinterpret_cast<float4*>(take(16 * n));
+    a.prevN = reinterpret_cast<float4*>(take(16 * n));
+    a.rng = reinterpret_cast<uint4*>(take(16 * n));
+    a.Lp = reinterpret_cast<float4*>(take(16 * n));
+    a.Ls = reinterpret_cast<float4*>(take(16 * n));
+    a.rayQ[0] = reinterpret_cast<float4*>(take(32 * n));
+    a.rayQ[1] = reinterpret_cast<float4*>(take(32 * n));
+    a.pathQ[0] = reinterpret_cast<uint32_t*>(take(4 * n));
+    a.pathQ[1] = reinterpret_cast<uint32_t*>(take(4 * n));
+    a.hitQ = reinterpret_cast<uint4*>(take(16 * n));
+    a.shQ = reinterpret_cast<float4*>(take(48 * n));
+    a.hitIdx = reinterpret_cast<uint32_t*>(take(4 * n));
+    a.pixHit = reinterpret_cast<uint4*>(take(16 * kNoAllocMaxPixels));
+    a.counters = reinterpret_cast<uint32_t*>(take(4 * kCounterWords));
+    if (off > kNoAllocScratchBytes) return fail(FHP_EHIP, "scratch layout exceeds the scratch array");
+    // The frame clears its own counters and writes every path slot before reading it, so the
+    // previous call's contents need no clearing.
+    HIP_OR_FAIL(launchMptFrame(s->dev, a, ts, reinterpret_cast<float4*>(d_color), false, nullptr, hipStream_t(stream),
+                               nullptr));
     return FHP_OK;
 }
 

@@ -245,6 +245,17 @@ int fhp_pass_get_property(FhpPass pass, const char* key, double* value);
 int fhp_pass_execute(FhpPass pass, FhpScene scene, uint32_t width, uint32_t height, uint32_t spp, uint32_t frame,
                      float* d_color, uint64_t* h_stats, FhpStream stream);
 
+/* The fhp_pass_execute frame of a MinimalPathTracer pass without any device allocation: no
+ * counterpart in the reference; for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). The frame's wavefront state lives in a
+ * static device array of the library (about 124 MB per device, held until the process ends), so
+ * a frame holds at most 524288 paths (width * height * spp) and 524288 pixels; larger frames are
+ * refused with FHP_EINVAL. Output, seeds and properties are those of fhp_pass_execute (the bounce
+ * loop is launched directly, not through a captured graph); no statistics, no traversal counting,
+ * no kernel timing. Calls on one device share that device's array: calls that overlap in time on
+ * the same device (any streams) race and must be ordered by the caller. */
+int fhp_pass_execute_noalloc(FhpPass pass, FhpScene scene, uint32_t width, uint32_t height, uint32_t spp, uint32_t frame,
+                             float* d_color, FhpStream stream);
 /* A batch of views rendered in one launch sequence (MinimalPathTracer): every view's paths share
  * the wavefront queues and the persistent traversal grid, each ray traversing its own view's
  * scene. h_camera is 14 floats as in FhpSceneDesc, or NULL for the scene's camera; each view is
