@@ -265,7 +265,7 @@ def diff_bench(iters: int = 3, fd_texels: int = 256, bounces=(1, 4), plain_loop:
         pt = WARDiffPathTracer(props)
         pt.set_scene(sc_t)
         target = pt.render(W, H, SPP, frame=1000)[..., :3].clone()
-        p = WARDiffPathTracer(props)
+        p = WARDiffPathTracer(dict(props, diffMode="BackwardDiff"))
         p.set_scene(sc)
         g = SceneGradients(sc, hash_size=256, texture_grads=True)
         p.scene_gradients = g

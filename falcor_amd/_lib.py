@@ -113,6 +113,8 @@ _SIGS = {
     "fhp_vbuffer_execute": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
                                       C.c_void_p, C.c_void_p, C.c_void_p]),
     "fhp_pass_set_input": (C.c_int, [C.c_void_p, C.c_char_p, C.c_void_p]),
+    "fhp_pass_set_output": (C.c_int, [C.c_void_p, C.c_char_p, C.c_void_p]),
+    "fhp_pass_set_diff_debug": (C.c_int, [C.c_void_p, C.c_int, C.c_uint32, C.c_uint32, C.c_void_p]),
     "fhp_scene_lights": (C.c_int, [C.c_void_p, C.c_void_p]),
     "fhp_scene_set_env_map": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_uint32]),
     "fhp_scene_get_env_importance": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),

@@ -106,7 +106,7 @@ class DiffRenderer:
         self.scene = scene
         self.width, self.height = width, height
         self.spp_primal, self.spp_grad = spp_primal, spp_grad
-        self.pass_ = WARDiffPathTracer(props or {})
+        self.pass_ = WARDiffPathTracer(dict({"diffMode": "BackwardDiff"}, **(props or {})))
         self.pass_.set_scene(scene)
         self.mesh_id = mesh_id
         self.grads = SceneGradients(scene, hash_size=hash_size, texture_grads=texture_grads, mesh_id=mesh_id)

@@ -180,6 +180,12 @@ struct FhpPass_ {
     // Bound input channels of MinimalPathTracer (MinimalPathTracer.cpp:48-53): "vbuffer", "viewW".
     const uint32_t* inVbuf = nullptr;
     const float4* inViewW = nullptr;
+    // WARDiffPathTracer: bound dColor output (WARDiffPathTracer.cpp:50) and the gradient-image
+    // debug parameters (setDiffDebugParams, WARDiffPathTracer.cpp:689-695).
+    float4* outDColor = nullptr;
+    int dbgVar = 0;
+    uint32_t dbgId = 0, dbgOffset = 0;
+    float dbgGrad[4] = {0.f, 0.f, 0.f, 0.f};
     // Captured bounce loop of the forward pass (property useGraph), keyed by everything the
     // captured kernels read by value.
     hipGraphExec_t graph = nullptr;
@@ -225,10 +231,14 @@ const std::map<std::string, double>& mptDefaults() {
                                                    {"splitVisibilityLaunch", 0}};
     return d;
 }
+// WARDiffPathTracer's DiffMode (WARDiffPathTracer.h:56-62) and DiffVariableType (SharedTypes.slang:78-84).
+constexpr int kDiffPrimal = 0, kDiffBackward = 1, kDiffForwardDebug = 2, kDiffBackwardDebug = 3;
+constexpr int kDiffVarNone = 0, kDiffVarMaterial = 1, kDiffVarGeometryTranslation = 2;
 const std::map<std::string, double>& warDefaults() {
     // WARDiffPathTracer.h:120-170 (StaticParams) and Params.slang.
     static const std::map<std::string, double> d = {
-        {"samplesPerPixel", 1}, {"maxBounces", 0},        {"diffMode", 1},          {"sampleGenerator", 0},
+        {"samplesPerPixel", 1}, {"maxBounces", 0},        {"diffMode", kDiffForwardDebug}, {"sampleGenerator", 0},
+        {"diffVarName", 0},     {"assertThreshold", 1e9},
         {"fixedSeed", 0},       {"useFixedSeed", 0},      {"useBSDFSampling", 1},   {"useNEE", 1},
         {"useMIS", 1},          {"useWAR", 0},            {"auxSampleCount", 16},   {"Log10vMFConcentration", 5},
         {"Log10vMFConcentrationScreen", 5},               {"boundaryTermBeta", 0.01}, {"useAntitheticSampling", 1},
@@ -305,6 +315,26 @@ int passTrace(const FhpScene_* s, const FhpPass_* p, TraceScratch& t) {
     if (words && (!s->ovf.p || words * 4 > s->ovf.bytes))
         return fail(FHP_ESTATE, "traversal overflow stack smaller than the launch needs");
     return FHP_OK;
+}
+
+// The debug parameters the reference's constructor sets for its two named debug variables
+// (WARDiffPathTracer.cpp:84-93): CBOX_BUNNY_MATERIAL = baseColor of material 0 along (1, 1, 1),
+// CBOX_BUNNY_TRANSLATION = translation of mesh 0 along +y.
+void applyDiffVarName(FhpPass_* p) {
+    const int name = int(p->get("diffVarName"));
+    if (name == 1) {
+        p->dbgVar = kDiffVarMaterial;
+        p->dbgId = 0;
+        p->dbgOffset = 0;
+        const float g[4] = {1.f, 1.f, 1.f, 0.f};
+        std::memcpy(p->dbgGrad, g, sizeof(g));
+    } else if (name == 2) {
+        p->dbgVar = kDiffVarGeometryTranslation;
+        p->dbgId = 0;
+        p->dbgOffset = 0;
+        const float g[4] = {0.f, 1.f, 0.f, 0.f};
+        std::memcpy(p->dbgGrad, g, sizeof(g));
+    }
 }
 
 int ensurePassBuffersN(FhpPass p, uint64_t n, uint64_t npix);
@@ -1066,6 +1096,14 @@ int fhp_pass_create(const char* kind, int np, const char* const* keys, const dou
     }
     if (k == "VBufferRT")
         p->props["sampleCount"] = p->jitter.create(int(p->get("samplePattern")), uint32_t(p->get("sampleCount")));
+    if (k == "WARDiffPathTracer") {
+        const double dm = p->get("diffMode"), dv = p->get("diffVarName");
+        if (!(dm == 0 || dm == 1 || dm == 2 || dm == 3))
+            return fail(FHP_EINVAL, "diffMode: 0 Primal, 1 BackwardDiff, 2 ForwardDiffDebug, 3 BackwardDiffDebug");
+        if (!(dv == 0 || dv == 1 || dv == 2))
+            return fail(FHP_EINVAL, "diffVarName: 0 none, 1 CBOX_BUNNY_MATERIAL, 2 CBOX_BUNNY_TRANSLATION");
+        applyDiffVarName(p.get());
+    }
     if (p->get("maxBounces") < 0 || p->get("maxBounces") > kMaxBounceSlots - 2)
         return fail(FHP_EINVAL, "maxBounces out of range [0, 28]");
     if (!validLaunch(p->get("traceBlockSize"), p->get("traceStackDepth"), p->get("traceBlocksPerCU")))
@@ -1106,6 +1144,15 @@ int fhp_pass_set_property(FhpPass p, const char* key, double v) {
         }
     }
     if (k == "maxBounces" && (v < 0 || v > kMaxBounceSlots - 2)) return fail(FHP_EINVAL, "maxBounces out of range");
+    if (p->kind == "WARDiffPathTracer" && k == "diffMode" && !(v == 0 || v == 1 || v == 2 || v == 3))
+        return fail(FHP_EINVAL, "diffMode: 0 Primal, 1 BackwardDiff, 2 ForwardDiffDebug, 3 BackwardDiffDebug");
+    if (p->kind == "WARDiffPathTracer" && k == "diffVarName") {
+        if (!(v == 0 || v == 1 || v == 2))
+            return fail(FHP_EINVAL, "diffVarName: 0 none, 1 CBOX_BUNNY_MATERIAL, 2 CBOX_BUNNY_TRANSLATION");
+        p->props[k] = v;
+        applyDiffVarName(p);
+        return FHP_OK;
+    }
     if (k == "traceBlockSize" || k == "traceStackDepth" || k == "traceBlocksPerCU") {
         const double b = k == "traceBlockSize" ? v : p->get("traceBlockSize");
         const double l = k == "traceStackDepth" ? v : p->get("traceStackDepth");
@@ -1201,8 +1248,29 @@ int fhp_pass_execute(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t spp
         a.out = reinterpret_cast<float4*>(d_color);
         a.tmp = p->Lp.as<float4>();
         a.rays = h_stats ? dstats : nullptr;
+        const int mode = int(p->get("diffMode"));
+        const uint64_t npix = uint64_t(W) * H;
         if ((rc = prepareTimer(p))) return rc;
-        HIP_OR_FAIL(launchWarPrimal(s->dev, a, ts, st, &p->timer));
+        if (mode == kDiffBackwardDebug) {
+            // BackwardDiffDebug renders no colour (the reference clears its outputs); its gradients
+            // come from fhp_pass_backward.
+            HIP_OR_FAIL(launchFill(d_color, npix * 4, 0.f, st));
+            if (p->outDColor) HIP_OR_FAIL(launchFill(reinterpret_cast<float*>(p->outDColor), npix * 4, 0.f, st));
+        } else if (mode == kDiffForwardDebug && p->dbgVar == kDiffVarMaterial) {
+            a.dbgMat = p->dbgId;
+            a.dbgOffset = p->dbgOffset;
+            std::memcpy(a.dbgGrad, p->dbgGrad, sizeof(a.dbgGrad));
+            a.dbgThreshold = float(p->get("assertThreshold"));
+            a.dColor = p->outDColor;
+            HIP_OR_FAIL(launchWarDebugImage(s->dev, a, ts, st, &p->timer));
+        } else {
+            if (mode == kDiffForwardDebug && p->dbgVar == kDiffVarGeometryTranslation)
+                return fail(FHP_EINVAL, "ForwardDiffDebug images of a GeometryTranslation variable are not supported");
+            HIP_OR_FAIL(launchWarPrimal(s->dev, a, ts, st, &p->timer));
+            // ForwardDiffDebug without a debug variable: every derivative is zero (written, as the
+            // reference writes (0, 0, 0, 1) for a finite zero derivative).
+            if (mode == kDiffForwardDebug && p->outDColor) HIP_OR_FAIL(launchFillRGBA(p->outDColor, npix, 0.f, 0.f, 0.f, 1.f, st));
+        }
         if ((rc = collectTimer(p, st))) return rc;
     }
     if (h_stats) {
@@ -1477,8 +1545,14 @@ int fhp_grads_copy(FhpGrads g, int which, float* dst, uint64_t n, FhpStream stre
 
 int fhp_pass_backward(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t spp, uint32_t frame, const float* d_dLdI,
                       FhpGrads g, uint64_t* h_stats, FhpStream stream) {
-    if (!p || !s || !d_dLdI || !g) return fail(FHP_EINVAL, "null argument");
+    if (!p || !s || !g) return fail(FHP_EINVAL, "null argument");
     if (p->kind != "WARDiffPathTracer") return fail(FHP_ESTATE, "backward needs a WARDiffPathTracer pass");
+    // runBackward runs the adjoint in BackwardDiff mode (WARDiffPathTracer.rt.slang:161-180) and, with
+    // dL/dI = (1, 0, 0) at every pixel, in BackwardDiffDebug mode (:201-211).
+    const int mode = int(p->get("diffMode"));
+    if (mode != kDiffBackward && mode != kDiffBackwardDebug)
+        return fail(FHP_ESTATE, "backward needs diffMode BackwardDiff (1) or BackwardDiffDebug (3)");
+    if (mode == kDiffBackward && !d_dLdI) return fail(FHP_EINVAL, "null argument");
     for (uint32_t m = 0; s && m < s->h.nM; m++) {
         const float* r = &s->h.mats[size_t(m) * M_STRIDE];
         if (int(r[M_TYPE]) == MAT_STANDARD && (r[M_SPEC_TRANS] > 0.f || r[M_DIFF_TRANS] > 0.f))
@@ -1512,7 +1586,7 @@ int fhp_pass_backward(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t sp
     a.useNEE = p->get("useNEE") != 0;
     a.useMIS = p->get("useMIS") != 0;
     a.useAntithetic = p->get("useAntitheticSampling") != 0;
-    a.dLdI = d_dLdI;
+    a.dLdI = mode == kDiffBackwardDebug ? nullptr : d_dLdI;
     a.gradMat = g->tmp.as<float>();
     a.hashSize = g->hashSize;
     a.matGradDim = g->dim;
@@ -1530,7 +1604,7 @@ int fhp_pass_backward(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t sp
         ga.aux = uint32_t(p->get("auxSampleCount"));
         ga.kappa = float(std::pow(10.0, p->get("Log10vMFConcentration")));
         ga.kappaScreen = float(std::pow(10.0, p->get("Log10vMFConcentrationScreen")));
-        ga.dLdI = d_dLdI;
+        ga.dLdI = a.dLdI;
         ga.gradPos = g->posTmp.as<float>();
         ga.gradNrm = g->nrmTmp.as<float>();
         ga.hashSize = g->hashSize; ga.meshV0 = g->meshV0; ga.meshVerts = g->meshVerts;
@@ -1642,6 +1716,28 @@ int fhp_image_compare(int metric, const float* d_a, const float* d_b, uint32_t W
 }
 
 // ------------------------------------------------------------------ V-buffer
+int fhp_pass_set_output(FhpPass p, const char* channel, void* d_data) {
+    if (!p || !channel) return fail(FHP_EINVAL, "null argument");
+    if (p->kind != "WARDiffPathTracer") return fail(FHP_EINVAL, "output channels are bound on a WARDiffPathTracer");
+    const std::string c(channel);
+    if (c == "dColor") p->outDColor = static_cast<float4*>(d_data);
+    else return fail(FHP_EINVAL, "unknown output channel '" + c + "' (dColor)");
+    return FHP_OK;
+}
+
+int fhp_pass_set_diff_debug(FhpPass p, int var_type, uint32_t id, uint32_t offset, const float* h_grad) {
+    if (!p || (var_type != kDiffVarNone && !h_grad)) return fail(FHP_EINVAL, "null argument");
+    if (p->kind != "WARDiffPathTracer") return fail(FHP_EINVAL, "debug parameters belong to a WARDiffPathTracer");
+    if (var_type < kDiffVarNone || var_type > kDiffVarGeometryTranslation)
+        return fail(FHP_EINVAL, "var_type: 0 None, 1 Material, 2 GeometryTranslation");
+    if (var_type == kDiffVarMaterial && offset >= 20) return fail(FHP_EINVAL, "material parameter offset must be below 20");
+    p->dbgVar = var_type;
+    p->dbgId = id;
+    p->dbgOffset = offset;
+    for (int i = 0; i < 4; i++) p->dbgGrad[i] = var_type == kDiffVarNone ? 0.f : h_grad[i];
+    return FHP_OK;
+}
+
 int fhp_pass_set_input(FhpPass p, const char* channel, const void* d_data) {
     if (!p || !channel) return fail(FHP_EINVAL, "null argument");
     if (p->kind != "MinimalPathTracer") return fail(FHP_EINVAL, "input channels are bound on a MinimalPathTracer");

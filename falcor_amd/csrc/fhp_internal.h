@@ -155,11 +155,23 @@ struct WarArgs {
     uint32_t hashSize, matGradDim;
     float* gradTex;         // per-texel baseColor gradients (texel layout), or null
     unsigned long long* rays;  // [closest, visibility] counters, or null
+    // Gradient-image debug modes (DiffDebugParams, SharedTypes.slang:98-104): material dbgMat's
+    // parameter entry at dbgOffset of the 20-float layout, direction dbgGrad; per-sample
+    // derivatives beyond dbgThreshold in magnitude are dropped (assertThreshold, Params.slang:61).
+    uint32_t dbgMat, dbgOffset;
+    float dbgGrad[4];
+    float dbgThreshold;
+    float4* dColor;         // debug image (RGBA), or null
 };
 hipError_t launchWarPrimal(const SceneDev& s, const WarArgs& a, const TraceScratch& ts, hipStream_t stream,
                            KernelTimer* timer);
 hipError_t launchWarAdjoint(const SceneDev& s, const WarArgs& a, const TraceScratch& ts, hipStream_t stream,
                             KernelTimer* timer);
+// ForwardDiffDebug for a material variable: the primal image into out and, into dColor, the red
+// channel's derivative along the debug direction (red, 0, 0, 1), both averaged over spp; tmp holds
+// W * H * spp per-sample (radiance, derivative).
+hipError_t launchWarDebugImage(const SceneDev& s, const WarArgs& a, const TraceScratch& ts, hipStream_t stream,
+                               KernelTimer* timer);
 // Geometry gradients of WARDiffPathTracer (warped-area reparameterization, maxBounces = 0).
 struct GeomArgs {
     uint32_t W, H, spp, frame;
@@ -175,6 +187,7 @@ struct GeomArgs {
 hipError_t launchWarGeom(const SceneDev& s, const GeomArgs& a, const TraceScratch& ts, hipStream_t stream, KernelTimer* timer);
 hipError_t launchAggregate(const float* tmp, uint32_t hashSize, uint32_t dim, float* out, hipStream_t stream);
 hipError_t launchFill(float* p, uint64_t n, float v, hipStream_t stream);
+hipError_t launchFillRGBA(float4* p, uint64_t n, float r, float g, float b, float a, hipStream_t stream);
 
 // BVH refit after vertex updates of one mesh (fhp_refit.hip).
 struct RefitArgs {

@@ -814,6 +814,17 @@ __global__ void kFill(float* p, uint64_t n, float v) {
     uint64_t G = uint64_t(gridDim.x) * blockDim.x;
     for (; i < n; i += G) p[i] = v;
 }
+__global__ void kFillRGBA(float4* p, uint64_t n, float4 v) {
+    uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    uint64_t G = uint64_t(gridDim.x) * blockDim.x;
+    for (; i < n; i += G) p[i] = v;
+}
+hipError_t launchFillRGBA(float4* p, uint64_t n, float r, float g, float b, float a, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    const uint64_t blocks = std::min<uint64_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(kFillRGBA, dim3(uint32_t(blocks)), dim3(256), 0, st, p, n, make_float4(r, g, b, a));
+    return hipGetLastError();
+}
 hipError_t launchFill(float* p, uint64_t n, float v, hipStream_t st) {
     if (n == 0) return hipSuccess;
     uint64_t blocks = std::min<uint64_t>((n + 255) / 256, 65536);

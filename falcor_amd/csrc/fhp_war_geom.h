@@ -572,8 +572,11 @@ FHP_DEV void geomPath(const SceneDev& s, const GeomArgs& a, uint32_t px, uint32_
 // One item of the kernel's grid loop: path id = pixel * spp + sample.
 FHP_DEVI void geomItem(const SceneDev& s, const GeomArgs& a, uint32_t id, TraceStack& st, unsigned long long& nc) {
     const uint32_t pix = id / a.spp, k = id % a.spp;
-    const float* dl = a.dLdI + 3 * size_t(pix);
-    const f3 dL = mk3(dl[0], dl[1], dl[2]);
+    f3 dL = mk3(1.f, 0.f, 0.f);  // dLdI null: the BackwardDiffDebug mode's dL/dI = (1, 0, 0)
+    if (a.dLdI) {
+        const float* dl = a.dLdI + 3 * size_t(pix);
+        dL = mk3(dl[0], dl[1], dl[2]);
+    }
     if (dL.x == 0.f && dL.y == 0.f && dL.z == 0.f) return;
     geomPath(s, a, pix % a.W, pix / a.W, a.frame + k, dL, jenkins(pix) % a.hashSize, st, nc);
 }

@@ -106,6 +106,32 @@ FHP_DEVI void atomAdd(float* p, float v) {
     if (v != 0.f) gradAdd(p, v);
 }
 
+// The gradient-image projection of one vertex's gradient (the debug modes): the entry of the
+// 20-float layout that starts at a.dbgOffset of material a.dbgMat, dotted with a.dbgGrad, as
+// GradientIOWrapper seeds or reads it (GradientIOWrapper.slang:75-106); zero for other materials
+// and for offsets that start no entry.
+FHP_DEVI float debugProjection(const SceneDev& s, const WarArgs& a, const Material& m, uint32_t mat, const MatGrad& g) {
+    if (mat != a.dbgMat) return 0.f;
+    const float* v = a.dbgGrad;
+    switch (a.dbgOffset) {
+        case 0: return g.base.x * v[0] + g.base.y * v[1] + g.base.z * v[2];
+        default: break;
+    }
+    if (m.type != MAT_STANDARD) return 0.f;
+    switch (a.dbgOffset) {
+        case 3: return g.metal * v[0];
+        case 4: return g.rough * v[0];
+        case 5: {
+            const float ior = s.mats[size_t(mat) * M_STRIDE + M_IOR];
+            const float f = (ior - 1.f) / (ior + 1.f);
+            return g.ior * 2.f * f * (2.f / ((ior + 1.f) * (ior + 1.f))) * v[0];
+        }
+        case 11: return g.emit.x * v[0] + g.emit.y * v[1] + g.emit.z * v[2];
+        case 14: return g.emitFactor * v[0];
+        default: return 0.f;
+    }
+}
+
 FHP_DEV void flushGrad(const SceneDev& s, const WarArgs& a, uint32_t hashRow, const Material& m, uint32_t mat,
                           const TexTap& tap, const MatGrad& g) {
     float* row = a.gradMat + size_t(hashRow) * a.matGradDim + size_t(mat) * 20;
@@ -148,9 +174,12 @@ FHP_DEV void flushGrad(const SceneDev& s, const WarArgs& a, uint32_t hashRow, co
 // reference's order (emission, NEE, scatter), so the results do not depend on this order of
 // evaluation. FROZEN: the finite-difference mode's separate sampling parameters (matsS /
 // texelsS) are in use; otherwise one material serves evaluation and sampling.
-template <bool ADJ, bool FROZEN>
+// DEBUG (with ADJ): instead of accumulating the gradients, each vertex adds its debug projection
+// to *dproj.
+template <bool ADJ, bool FROZEN, bool DEBUG = false>
 FHP_DEV f3 warPath(const SceneDev& s, const WarArgs& a, uint32_t px, uint32_t py, uint32_t frameSeed, TraceStack& st,
-                   f3 Ltot, f3 dL, uint32_t hashRow, unsigned long long& nc, unsigned long long& nv) {
+                   f3 Ltot, f3 dL, uint32_t hashRow, unsigned long long& nc, unsigned long long& nv,
+                   float* dproj = nullptr) {
     Rng rng = rngSeed(px, py, frameSeed / 2);
     for (int i = 0; i < 6; i++) rng.u1();
     const float pfx = float(px) + 0.5f, pfy = float(py) + 0.5f;
@@ -223,7 +252,7 @@ FHP_DEV f3 warPath(const SceneDev& s, const WarArgs& a, uint32_t px, uint32_t py
         Material m, mS;
         {
             const Shading sd0 = sd;
-            m = loadMaterial(s, s.mats, s.texels, sd0, sd, ADJ ? &tap : nullptr);
+            m = loadMaterial(s, s.mats, s.texels, sd0, sd, ADJ && !DEBUG ? &tap : nullptr);
             // Sampling parameters differ from the evaluated ones only in the finite-difference mode.
             if (FROZEN) {
                 Shading unused = sd0;
@@ -295,7 +324,8 @@ FHP_DEV f3 warPath(const SceneDev& s, const WarArgs& a, uint32_t px, uint32_t py
                              fS.z != 0.f ? dL.z * R.z / fS.z : 0.f);
                 bsdfAdjoint(sd, m, woS, adj, g);
             }
-            flushGrad(s, a, hashRow, m, sd.mat, tap, g);
+            if (DEBUG) *dproj += debugProjection(s, a, m, sd.mat, g);
+            else flushGrad(s, a, hashRow, m, sd.mat, tap, g);
         }
         if (!cont) break;
         len++;
@@ -318,8 +348,12 @@ FHP_DEVI void warItem(const SceneDev& s, const WarArgs& a, uint32_t id, TraceSta
         f3 L = warPath<false, FROZEN>(s, a, px, py, seed, st, mk3(0.f), mk3(0.f), 0, nc, nv);
         a.out[id] = make_float4(L.x, L.y, L.z, 0.f);
     } else {
-        const float* dl = a.dLdI + 3 * size_t(pix);
-        f3 dL = mk3(dl[0], dl[1], dl[2]);
+        // dLdI null: the BackwardDiffDebug mode's dL/dI = (1, 0, 0) at every pixel.
+        f3 dL = mk3(1.f, 0.f, 0.f);
+        if (a.dLdI) {
+            const float* dl = a.dLdI + 3 * size_t(pix);
+            dL = mk3(dl[0], dl[1], dl[2]);
+        }
         if (dL.x == 0.f && dL.y == 0.f && dL.z == 0.f) return;
         unsigned long long c0 = 0, c1 = 0;
         f3 L = warPath<false, FROZEN>(s, a, px, py, seed, st, mk3(0.f), mk3(0.f), 0, c0, c1);
@@ -327,6 +361,23 @@ FHP_DEVI void warItem(const SceneDev& s, const WarArgs& a, uint32_t id, TraceSta
         nc += c0;
         nv += c1;
     }
+}
+
+// One item of the ForwardDiffDebug image kernel: the path's radiance, then its replay with
+// dL/dI = (1, 0, 0) projected on the debug direction: the forward-mode derivative of the red
+// channel (the reverse-mode gradient of one path estimate, dotted with a direction, is its
+// directional derivative). tmp[id] = (radiance, derivative).
+template <bool FROZEN>
+FHP_DEVI void warDebugItem(const SceneDev& s, const WarArgs& a, uint32_t id, TraceStack& st, unsigned long long& nc,
+                           unsigned long long& nv) {
+    const uint32_t pix = id / a.spp, k = id % a.spp;
+    const uint32_t px = pix % a.W, py = pix / a.W;
+    const uint32_t seed = a.frame + k;
+    unsigned long long c0 = 0, c1 = 0;
+    const f3 L = warPath<false, FROZEN>(s, a, px, py, seed, st, mk3(0.f), mk3(0.f), 0, nc, nv);
+    float d = 0.f;
+    warPath<true, FROZEN, true>(s, a, px, py, seed, st, L, mk3(1.f, 0.f, 0.f), 0, c0, c1, &d);
+    a.tmp[id] = make_float4(L.x, L.y, L.z, d);
 }
 
 // Whether a scene renders with the finite-difference mode's separate sampling parameters.

@@ -404,6 +404,8 @@ class _NativePass(RenderPass):
     def _encode(key, value):
         if key == "diffMode" and isinstance(value, str):
             return {"Primal": 0, "BackwardDiff": 1, "ForwardDiffDebug": 2, "BackwardDiffDebug": 3}[value]
+        if key == "diffVarName" and isinstance(value, str):
+            return {"": 0, "CBOX_BUNNY_MATERIAL": 1, "CBOX_BUNNY_TRANSLATION": 2}[value]
         if key == "emissiveSampler" and isinstance(value, str):
             return {"Uniform": 0, "LightBVH": 1}[value]
         return float(value)
@@ -585,6 +587,28 @@ class WARDiffPathTracer(_NativePass):
         self.dL_dI = None
         self.mesh_to_optimize = 0
 
+    DIFF_VAR_NONE, DIFF_VAR_MATERIAL, DIFF_VAR_GEOMETRY_TRANSLATION = 0, 1, 2
+
+    def set_diff_debug_params(self, var_type: int, id: int = 0, offset: int = 0, grad=(0.0, 0.0, 0.0, 0.0)):
+        """WARDiffPathTracer::setDiffDebugParams (WARDiffPathTracer.cpp:689-695): the variable of the
+        gradient-image debug modes (DIFF_VAR_MATERIAL: the parameter entry at `offset` of material
+        `id`'s 20-float layout along `grad`; DIFF_VAR_GEOMETRY_TRANSLATION: mesh `id` moved along
+        grad[:3])."""
+        g = np.zeros(4, np.float32)
+        g[:len(grad)] = grad
+        _lib.check(_lib.load().fhp_pass_set_diff_debug(self.handle, int(var_type), int(id), int(offset), g.ctypes.data))
+
+    def render(self, width: int, height: int, spp: int = 1, frame: int | None = None, out=None, stats: bool = False,
+               dcolor=None):
+        """Renders color ([H, W, 4] float32); with `dcolor` (a [H, W, 4] float32 tensor) bound as the
+        dColor output, the gradient-image debug modes write it as well."""
+        L = _lib.load()
+        _lib.check(L.fhp_pass_set_output(self.handle, b"dColor", dcolor.data_ptr() if dcolor is not None else None))
+        try:
+            return super().render(width, height, spp, frame, out, stats)
+        finally:
+            _lib.check(L.fhp_pass_set_output(self.handle, b"dColor", None))
+
     def set_mesh_to_optimize(self, mesh_id: int):
         """WARDiffPathTracer::setMeshToOptimize (WARDiffPathTracer.cpp:697-715): vertex gradients
         (useWAR) flow into this mesh only."""
@@ -594,17 +618,21 @@ class WARDiffPathTracer(_NativePass):
         """Accumulates d(sum dL_dI . color)/d(params) into scene_gradients (raw sum over samples).
         With useWAR, vertex position/normal gradients of the mesh set by set_mesh_to_optimize
         are accumulated too (maxBounces = 0)."""
-        if self.scene_gradients is None or self.dL_dI is None:
+        debug = self.get_property("diffMode") == 3  # BackwardDiffDebug: dL/dI = (1, 0, 0) everywhere
+        if self.scene_gradients is None or (self.dL_dI is None and not debug):
             raise _lib.FhpError("backward needs scene_gradients and dL_dI")
         if self.get_property("useWAR") != 0 and self.scene_gradients.mesh_id != self.mesh_to_optimize:
             self.scene_gradients.enable_mesh(self.mesh_to_optimize)
         torch = _torch()
-        d = self.dL_dI.detach().to(device=_device(), dtype=torch.float32).contiguous()
-        if d.numel() != width * height * 3:
-            raise ValueError("dL_dI must hold width * height * 3 floats")
+        d = None
+        if not debug:
+            d = self.dL_dI.detach().to(device=_device(), dtype=torch.float32).contiguous()
+            if d.numel() != width * height * 3:
+                raise ValueError("dL_dI must hold width * height * 3 floats")
         f = self.frame_count if frame is None else frame
         st = np.zeros(6, np.uint64) if stats else None
-        _lib.check(_lib.load().fhp_pass_backward(self.handle, self.scene.handle, width, height, spp, f, d.data_ptr(),
+        _lib.check(_lib.load().fhp_pass_backward(self.handle, self.scene.handle, width, height, spp, f,
+                                                 d.data_ptr() if d is not None else None,
                                                  self.scene_gradients.handle, st.ctypes.data if stats else None,
                                                  _lib.stream_ptr()))
         if frame is None:
@@ -823,10 +851,12 @@ class RenderGraph:
                     p.backward(width, height, 1)
                     vals[f"{n}.dColor"] = torch.zeros((height, width, 4), device=_device())
                     vals[f"{n}.color"] = torch.zeros((height, width, 4), device=_device())
+                elif isinstance(p, WARDiffPathTracer):
+                    dcolor = torch.zeros((height, width, 4), device=_device())
+                    vals[f"{n}.color"] = p.render(width, height, 1, dcolor=dcolor)
+                    vals[f"{n}.dColor"] = dcolor
                 else:
                     vals[f"{n}.color"] = p.render(width, height, 1)
-                    if isinstance(p, WARDiffPathTracer):
-                        vals[f"{n}.dColor"] = torch.zeros((height, width, 4), device=_device())
         self._outputs = vals
         return vals
 

@@ -305,7 +305,9 @@ int fhp_grads_copy(FhpGrads grads, int which, float* d_dst, uint64_t num_floats,
 
 /* Adjoint of WARDiffPathTracer's image: accumulates d(sum_pixels dLdI . I)/d(params) into
  * grads by path replay (recomputes each path from its seed; no path storage). d_dLdI is
- * height * width * 3 floats. Frame/sample numbering as in fhp_pass_execute.
+ * height * width * 3 floats. Frame/sample numbering as in fhp_pass_execute. Needs diffMode
+ * BackwardDiff (1), or BackwardDiffDebug (3), where dLdI = (1, 0, 0) at every pixel and d_dLdI
+ * is ignored (WARDiffPathTracer.rt.slang:201-211); other modes are refused.
  * Replaces rayGen's bwd_diff(tracePaths) branch (WARDiffPathTracer.rt.slang:161-180) and
  * the host loop of render_grad (scripts/inv-rendering/common.py:90-114). */
 int fhp_pass_backward(FhpPass pass, FhpScene scene, uint32_t width, uint32_t height, uint32_t spp, uint32_t frame,
@@ -360,6 +362,21 @@ int fhp_vbuffer_execute(FhpPass pass, FhpScene scene, uint32_t width, uint32_t h
  * the primary direction is -viewW). The buffers must hold width * height records at execution.
  * Records that do not name a triangle of the scene read as misses. Replaces the "vbuffer" and
  * "viewW" inputs of MinimalPathTracer (MinimalPathTracer.cpp:48-53, MinimalPathTracer.rt.slang). */
+/* Bind an output channel of a WARDiffPathTracer pass (WARDiffPathTracer.cpp:46-51): "dColor",
+ * height * width * 4 floats (RGBA), written by fhp_pass_execute in the gradient-image debug modes
+ * (NULL unbinds). ForwardDiffDebug writes (d red / d theta . v, 0, 0, 1) per pixel, averaged over
+ * the frame's samples, for the debug variable set below; samples whose derivative exceeds the
+ * property assertThreshold (1e9) in magnitude contribute zero (WARDiffPathTracer.rt.slang:182-199);
+ * with no debug variable it is (0, 0, 0, 1); BackwardDiffDebug clears it (and color). */
+int fhp_pass_set_output(FhpPass pass, const char* channel, void* d_data);
+/* The gradient-image debug variable of a WARDiffPathTracer pass (setDiffDebugParams,
+ * WARDiffPathTracer.cpp:689-695; DiffDebugParams, SharedTypes.slang:98-104): var_type 0 None,
+ * 1 Material (parameter entry starting at `offset` of material `id`'s 20-float layout, direction
+ * h_grad[0..3]), 2 GeometryTranslation (mesh `id` translated along h_grad[0..2]). The property
+ * diffVarName (1 CBOX_BUNNY_MATERIAL, 2 CBOX_BUNNY_TRANSLATION) sets the reference's presets
+ * (WARDiffPathTracer.cpp:84-93). ForwardDiffDebug images are produced for Material variables;
+ * a GeometryTranslation variable is refused there. */
+int fhp_pass_set_diff_debug(FhpPass pass, int var_type, uint32_t id, uint32_t offset, const float* h_grad);
 int fhp_pass_set_input(FhpPass pass, const char* channel, const void* d_data);
 
 #ifdef __cplusplus

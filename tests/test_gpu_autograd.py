@@ -21,7 +21,7 @@ def test_autograd_material_gradients_equal_direct_pass(gpu):
     loss = ((img - target) ** 2).mean()
     loss.backward()
     # Direct: same parameters, adjoint frame 11 (the renderer's second frame), dL/dI of the loss.
-    p = WARDiffPathTracer({"maxBounces": 2})
+    p = WARDiffPathTracer({"maxBounces": 2, "diffMode": "BackwardDiff"})
     p.set_scene(sc)
     g = SceneGradients(sc, hash_size=16, texture_grads=False)
     p.scene_gradients = g
@@ -80,7 +80,7 @@ def test_autograd_vertex_gradients_equal_direct_pass(gpu):
     nrm = torch.tensor(nrm0, device="cuda", requires_grad=True)
     img = r(positions=pos, normals=nrm)
     img.sum().backward()
-    p = WARDiffPathTracer(props)
+    p = WARDiffPathTracer(dict(props, diffMode="BackwardDiff"))
     p.set_scene(sc)
     g = SceneGradients(sc, hash_size=16, texture_grads=False, mesh_id=mesh)
     p.scene_gradients = g

@@ -42,7 +42,7 @@ def _setup(props=None):
     data = S.displaced_sphere_in_box(0, radius=0.12, segments=(24, 12), sigma=0.05)
     sc = Scene(data)
     mesh = data.mesh_names.index("Sphere")
-    p = WARDiffPathTracer(dict({"maxBounces": 0, "useWAR": True, "Log10vMFConcentration": 4,
+    p = WARDiffPathTracer(dict({"maxBounces": 0, "useWAR": True, "diffMode": "BackwardDiff", "Log10vMFConcentration": 4,
                                 "Log10vMFConcentrationScreen": 4, "auxSampleCount": 64}, **(props or {})))
     p.set_scene(sc)
     return sc, p, mesh

@@ -165,7 +165,7 @@ def _loss_weights(W, H, seed=0):
 def _gpu_grad(sc, props, W, H, spp, frame, dLdI, texture=False):
     import torch
     from falcor_amd.render_graph import SceneGradients, WARDiffPathTracer
-    p = WARDiffPathTracer(props)
+    p = WARDiffPathTracer(dict(props, diffMode="BackwardDiff"))
     p.set_scene(sc)
     g = SceneGradients(sc, hash_size=16, texture_grads=texture)
     p.scene_gradients = g

@@ -123,7 +123,7 @@ def test_adjoint_refuses_transmission(gpu):
     from falcor_amd import _lib
     from falcor_amd.render_graph import Scene, SceneGradients, WARDiffPathTracer
     sc = Scene(_transmissive_field(24))
-    p = WARDiffPathTracer({"maxBounces": 2})
+    p = WARDiffPathTracer({"maxBounces": 2, "diffMode": "BackwardDiff"})
     p.set_scene(sc)
     p.scene_gradients = SceneGradients(sc, hash_size=16)
     p.dL_dI = torch.ones((16, 16, 3), device="cuda")

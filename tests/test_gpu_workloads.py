@@ -84,7 +84,7 @@ def test_differentiable_materials_workload_matches_oracle_fd(gpu, oracle_mod):
     pt.set_scene(Scene(tgt))
     target = pt.render(W, H, SPP, frame=1000)[..., :3].clone()
     sc = Scene(init)
-    p = WARDiffPathTracer({"maxBounces": B})
+    p = WARDiffPathTracer({"maxBounces": B, "diffMode": "BackwardDiff"})
     p.set_scene(sc)
     img = p.render(W, H, SPP, frame=0)[..., :3]
     # Forward parity on a crop (the oracle renders the crop's pixels of the full frame).
