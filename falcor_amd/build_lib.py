@@ -16,7 +16,7 @@ PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 OUT = PKG / "libfhp.so"
 BUILD = PKG / "_build"
-SOURCES = ["fhp_kernels.hip", "fhp_war.hip", "fhp_war_geom.hip", "fhp_image.hip", "fhp_refit.hip", "fhp_envmap.hip", "fhp_api.cpp", "scene_build.cpp"]
+SOURCES = ["fhp_kernels.hip", "fhp_war.hip", "fhp_war_trans.hip", "fhp_war_geom.hip", "fhp_image.hip", "fhp_refit.hip", "fhp_envmap.hip", "fhp_api.cpp", "scene_build.cpp"]
 
 # -ffp-contract=off: no implicit FMA, so FP32 results follow the written operation order.
 # Packed FP32 instructions are disabled for the device code (kept out of generated code).
@@ -48,7 +48,7 @@ def _compile(src: str) -> Path:
 def build(verbose: bool = True) -> Path:
     """Compiles every source and links libfhp.so (always rebuilds)."""
     BUILD.mkdir(exist_ok=True)
-    with cf.ThreadPoolExecutor(max_workers=min(4, os.cpu_count() or 1)) as ex:
+    with cf.ThreadPoolExecutor(max_workers=min(6, os.cpu_count() or 1)) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = OUT.with_suffix(".so.tmp")
     cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)]

@@ -498,6 +498,7 @@ int buildAndUpload(FhpScene_* s) {
     dv.viewProjZ = make_float4(h.viewProj[8], h.viewProj[9], h.viewProj[10], h.viewProj[11]);
     dv.viewProjW = make_float4(h.viewProj[12], h.viewProj[13], h.viewProj[14], h.viewProj[15]);
     dv.nTris = h.nT;
+    dv.hasTransmission = hasTransmission(h) ? 1u : 0u;
     dv.nLights = h.nLights;
     dv.nActive = uint32_t(h.activeLights.size());
     dv.nLbvh = h.nLbvh;
@@ -938,6 +939,7 @@ int fhp_scene_set_materials(FhpScene s, const float* mats) {
     if (int rc = waitForDevice()) return rc;
     s->h.mats.assign(mats, mats + size_t(s->h.nM) * M_STRIDE);
     HIP_OR_FAIL(hipMemcpy(s->mats.p, mats, size_t(s->h.nM) * M_STRIDE * 4, hipMemcpyHostToDevice));
+    s->dev.hasTransmission = hasTransmission(s->h) ? 1u : 0u;
     // Light samples evaluate the emitters' current materials (EmissiveLightSamplerHelpers.slang:79).
     if (s->h.nLights) {
         refreshLightEmission(s->h);
@@ -1553,12 +1555,6 @@ int fhp_pass_backward(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t sp
     if (mode != kDiffBackward && mode != kDiffBackwardDebug)
         return fail(FHP_ESTATE, "backward needs diffMode BackwardDiff (1) or BackwardDiffDebug (3)");
     if (mode == kDiffBackward && !d_dLdI) return fail(FHP_EINVAL, "null argument");
-    for (uint32_t m = 0; s && m < s->h.nM; m++) {
-        const float* r = &s->h.mats[size_t(m) * M_STRIDE];
-        if (int(r[M_TYPE]) == MAT_STANDARD && (r[M_SPEC_TRANS] > 0.f || r[M_DIFF_TRANS] > 0.f))
-            return fail(FHP_EINVAL, "the adjoint does not differentiate transmission lobes (material " + std::to_string(m) +
-                                        " has specular or diffuse transmission)");
-    }
     if (g->scene != s) return fail(FHP_EINVAL, "gradients belong to another scene");
     if (W == 0 || H == 0 || spp == 0) return fail(FHP_EINVAL, "empty frame");
     if (W > 65536 || H > 65536) return fail(FHP_EINVAL, "frame dimension above 65536");

@@ -172,6 +172,10 @@ hipError_t launchWarAdjoint(const SceneDev& s, const WarArgs& a, const TraceScra
 // W * H * spp per-sample (radiance, derivative).
 hipError_t launchWarDebugImage(const SceneDev& s, const WarArgs& a, const TraceScratch& ts, hipStream_t stream,
                                KernelTimer* timer);
+// The adjoint (debug = false) or ForwardDiffDebug sample kernel (debug = true) instances with
+// the transmission-lobe adjoint, on `blocks` blocks of ts's configuration.
+hipError_t launchWarTransKernel(const SceneDev& s, const WarArgs& a, const TraceScratch& ts, hipStream_t stream,
+                                uint32_t blocks, bool debug);
 // Geometry gradients of WARDiffPathTracer (warped-area reparameterization, maxBounces = 0).
 struct GeomArgs {
     uint32_t W, H, spp, frame;

@@ -87,6 +87,7 @@ struct SceneDev {
     // (posW, openingAngle) (dirW, cosOpeningAngle) (penumbraAngle, 0, 0, 0); null when none.
     const float4* alights;
     uint32_t nALights;
+    uint32_t hasTransmission;  // 1: some StandardMaterial has specular or diffuse transmission
 };
 constexpr uint32_t kLightPoint = 0u, kLightDirectional = 1u;  // LightType::Point, ::Directional
 constexpr uint32_t kEnvLight = 1u, kEnvBackground = 2u;

@@ -1,48 +1,10 @@
-// WARDiffPathTracer primal and adjoint kernels (one path per thread); the per-path code is in
-// fhp_war_path.h.
-#include "fhp_war_path.h"
+// WARDiffPathTracer primal and adjoint launches (one path per thread); the per-path code is in
+// fhp_war_path.h, the kernels in fhp_war_kernels.h.
+#include "fhp_war_kernels.h"
 
 namespace fhp {
 
 namespace {
-
-// The megakernels hold a whole path (and its adjoint) per lane. Capped at 96 VGPRs (5 waves per
-// SIMD) they spill some state to scratch; unconstrained (3 / 2 waves per SIMD) latency hiding
-// suffers more (the cap measured 1.47x faster per 512x512, 16 spp forward+adjoint iteration).
-#ifndef FHP_WAR_WAVES_PER_EU
-#define FHP_WAR_WAVES_PER_EU 5
-#endif
-constexpr int kWarWavesPerEU = FHP_WAR_WAVES_PER_EU;
-
-template <int B, int D, bool ADJ, bool FROZEN>
-__global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(kWarWavesPerEU))) void kWar(SceneDev s, WarArgs a,
-                                                                                                 TraceScratch ts) {
-    __shared__ uint32_t lds[D * B];
-    TraceStack st = makeStack<B, D>(lds, ts, blockIdx.x, threadIdx.x);
-    const uint32_t n = a.W * a.H * a.spp;
-    const uint32_t G = gridDim.x * B;
-    unsigned long long nc = 0, nv = 0;
-    for (uint32_t id = blockIdx.x * B + threadIdx.x; id < n; id += G) warItem<ADJ, FROZEN>(s, a, id, st, nc, nv);
-    if (a.rays) {
-        atomicAdd(&a.rays[0], nc);
-        atomicAdd(&a.rays[1], nv);
-    }
-}
-
-template <int B, int D, bool FROZEN>
-__global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(kWarWavesPerEU))) void kWarDebug(SceneDev s, WarArgs a,
-                                                                                                      TraceScratch ts) {
-    __shared__ uint32_t lds[D * B];
-    TraceStack st = makeStack<B, D>(lds, ts, blockIdx.x, threadIdx.x);
-    const uint32_t n = a.W * a.H * a.spp;
-    const uint32_t G = gridDim.x * B;
-    unsigned long long nc = 0, nv = 0;
-    for (uint32_t id = blockIdx.x * B + threadIdx.x; id < n; id += G) warDebugItem<FROZEN>(s, a, id, st, nc, nv);
-    if (a.rays) {
-        atomicAdd(&a.rays[0], nc);
-        atomicAdd(&a.rays[1], nv);
-    }
-}
 
 // Per-pixel averages of the ForwardDiffDebug samples: the primal colour, and the red derivative
 // of the samples within the threshold (WARDiffPathTracer.rt.slang:193-198: a sample whose
@@ -104,8 +66,9 @@ hipError_t launchWarDebugImage(const SceneDev& s, const WarArgs& a, const TraceS
     hipError_t e = withTraceConfig(ts, [&](auto Bc, auto Dc) {
         constexpr int B = decltype(Bc)::value, D = decltype(Dc)::value;
         const uint32_t blocks = std::min<uint32_t>(ts.gridBlocks, (n + B - 1) / B);
-        if (samplingFrozen(s)) hipLaunchKernelGGL((kWarDebug<B, D, true>), dim3(blocks), dim3(B), 0, st, s, a, ts);
-        else hipLaunchKernelGGL((kWarDebug<B, D, false>), dim3(blocks), dim3(B), 0, st, s, a, ts);
+        if (s.hasTransmission) return launchWarTransKernel(s, a, ts, st, blocks, true);
+        if (samplingFrozen(s)) hipLaunchKernelGGL((kWarDebug<B, D, true, false>), dim3(blocks), dim3(B), 0, st, s, a, ts);
+        else hipLaunchKernelGGL((kWarDebug<B, D, false, false>), dim3(blocks), dim3(B), 0, st, s, a, ts);
         return hipGetLastError();
     });
     if (tm) tm->end(st);
@@ -122,8 +85,10 @@ hipError_t launchWarAdjoint(const SceneDev& s, const WarArgs& a, const TraceScra
     hipError_t e = withTraceConfig(ts, [&](auto Bc, auto Dc) {
         constexpr int B = decltype(Bc)::value, D = decltype(Dc)::value;
         const uint32_t blocks = std::min<uint32_t>(ts.gridBlocks, (n + B - 1) / B);
-        if (samplingFrozen(s)) hipLaunchKernelGGL((kWar<B, D, true, true>), dim3(blocks), dim3(B), 0, st, s, a, ts);
-        else hipLaunchKernelGGL((kWar<B, D, true, false>), dim3(blocks), dim3(B), 0, st, s, a, ts);
+        // Scenes with transmissive materials use the instances that carry the transmission adjoint.
+        if (s.hasTransmission) return launchWarTransKernel(s, a, ts, st, blocks, false);
+        if (samplingFrozen(s)) hipLaunchKernelGGL((kWar<B, D, true, true, false>), dim3(blocks), dim3(B), 0, st, s, a, ts);
+        else hipLaunchKernelGGL((kWar<B, D, true, false, false>), dim3(blocks), dim3(B), 0, st, s, a, ts);
         return hipGetLastError();
     });
     if (tm) tm->end(st);

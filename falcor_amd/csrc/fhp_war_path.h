@@ -31,18 +31,189 @@ FHP_DEVI void gaussian2(float ux, float uy, float sigma, float& gx, float& gy) {
     gy = (r * sinf(kTwoPi * uy)) * sigma;
 }
 
-// Local gradient of one vertex's material parameters.
+// Local gradient of one vertex's material parameters. ior holds dS/dF0 (converted to dS/dior when
+// the vertex is flushed); iorEta is dS/dior through the relative IoR of the transmission lobes.
 struct MatGrad {
     f3 base;
     float metal, rough, ior;
     f3 emit;
     float emitFactor;
+    f3 trans;                          // transmissionColor
+    float diffTrans, specTrans, iorEta;
 };
 FHP_DEVI void zero(MatGrad& g) {
     g.base = mk3(0.f); g.metal = g.rough = g.ior = 0.f; g.emit = mk3(0.f); g.emitFactor = 0.f;
+    g.trans = mk3(0.f); g.diffTrans = g.specTrans = g.iorEta = 0.f;
 }
 
-// Backpropagates adj (dS/df, RGB) through f = material BSDF eval(wi, wo) into material params.
+// ------------------------------------------------------------------ transmission lobes
+// Forward-mode dual numbers over (alpha, eta): value and the two partial derivatives.
+struct D2 {
+    float v, a, e;
+};
+FHP_DEVI D2 d2c(float c) { return D2{c, 0.f, 0.f}; }
+FHP_DEVI D2 operator+(D2 x, D2 y) { return D2{x.v + y.v, x.a + y.a, x.e + y.e}; }
+FHP_DEVI D2 operator-(D2 x, D2 y) { return D2{x.v - y.v, x.a - y.a, x.e - y.e}; }
+FHP_DEVI D2 operator-(D2 x) { return D2{-x.v, -x.a, -x.e}; }
+FHP_DEVI D2 operator*(D2 x, D2 y) { return D2{x.v * y.v, x.a * y.v + x.v * y.a, x.e * y.v + x.v * y.e}; }
+FHP_DEVI D2 operator*(D2 x, float c) { return D2{x.v * c, x.a * c, x.e * c}; }
+FHP_DEVI D2 operator*(float c, D2 x) { return x * c; }
+FHP_DEVI D2 operator+(D2 x, float c) { return D2{x.v + c, x.a, x.e}; }
+FHP_DEVI D2 operator+(float c, D2 x) { return x + c; }
+FHP_DEVI D2 operator-(float c, D2 x) { return D2{c - x.v, -x.a, -x.e}; }
+FHP_DEVI D2 operator/(D2 x, D2 y) {
+    const float q = x.v / y.v;
+    return D2{q, (x.a - q * y.a) / y.v, (x.e - q * y.e) / y.v};
+}
+FHP_DEVI D2 d2sqrt(D2 x) {
+    const float r = sqrtf(x.v), k = r > 0.f ? 0.5f / r : 0.f;
+    return D2{r, x.a * k, x.e * k};
+}
+FHP_DEVI D2 d2abs(D2 x) { return x.v < 0.f ? -x : x; }
+// IsotropicGGX.slang:46-51, :163-169, :200-206 with alpha a dual number (cosines ci, co fixed).
+FHP_DEVI D2 ggxDd(D2 alpha, D2 c) {
+    const D2 a2 = alpha * alpha;
+    const D2 d = (c * a2 - c) * c + 1.f;
+    return a2 / (d * d * kPi);
+}
+FHP_DEVI D2 ggxLambdad(D2 a2, float c) {
+    const float c2 = c * c;
+    const float t2 = fmaxf_(1.f - c2, 0.f) / c2;
+    return (-1.f + d2sqrt(1.f + a2 * t2)) * 0.5f;
+}
+// Fresnel.slang:68-92 with eta and cosI dual numbers (total internal reflection: a constant 1).
+FHP_DEVI D2 fresnelDielectricd(D2 eta, D2 cosI) {
+    if (cosI.v < 0.f) {
+        eta = d2c(1.f) / eta;
+        cosI = -cosI;
+    }
+    const D2 sin2T = eta * eta * (1.f - cosI * cosI);
+    if (sin2T.v > 1.f) return d2c(1.f);
+    const D2 cosT = d2sqrt(1.f - sin2T);
+    const D2 rs = (eta * cosI - cosT) / (eta * cosI + cosT);
+    const D2 rp = (eta * cosT - cosI) / (eta * cosT + cosI);
+    return 0.5f * (rs * rs + rp * rp);
+}
+// The GGX dielectric lobe of Bsdf::evalSpecularT (SpecularMicrofacetBSDF::eval) as a dual number
+// in (alphaT, eta), without the transmission albedo: on the reflection side F D G / (4 wi.z), on
+// the transmission side (1 - F) D G |t| (times the albedo per channel). Same operations as the
+// float evaluation; the caller has checked the lobe's cosine and alpha conditions.
+FHP_DEVI D2 specTDual(f3 wi, f3 wo, float alphaT, float eta, bool& isR) {
+    isR = wo.z > 0.f;
+    const D2 A{alphaT, 1.f, 0.f}, E{eta, 0.f, 1.f};
+    const D2 k = isR ? d2c(1.f) : E;
+    D2 hx = d2c(wo.x) + k * wi.x, hy = d2c(wo.y) + k * wi.y, hz = d2c(wo.z) + k * wi.z;
+    const D2 len = d2sqrt(hx * hx + hy * hy + hz * hz);
+    hx = hx / len;
+    hy = hy / len;
+    hz = hz / len;
+    const float sgn = hz.v > 0.f ? 1.f : (hz.v < 0.f ? -1.f : 0.f);
+    hx = hx * sgn;
+    hy = hy * sgn;
+    hz = hz * sgn;
+    const D2 wiDotH = hx * wi.x + hy * wi.y + hz * wi.z, woDotH = hx * wo.x + hy * wo.y + hz * wo.z;
+    const D2 D = ggxDd(A, hz);
+    const D2 a2 = A * A;
+    const D2 G = d2c(1.f) / (1.f + ggxLambdad(a2, wi.z) + ggxLambdad(a2, fabsf(wo.z)));
+    const D2 F = fresnelDielectricd(E, wiDotH);
+    if (isR) return F * D * G * (0.25f / wi.z);
+    const D2 sqrtDenom = woDotH + E * wiDotH;
+    const D2 t = E * E * wiDotH * woDotH / (sqrtDenom * sqrtDenom * wi.z);
+    return (1.f - F) * D * G * d2abs(t);
+}
+
+// Adjoint of the four-lobe StandardBSDF (StandardBSDF.slang:125-240) of a material with
+// transmission, f = (1 - sT)(1 - dT) f_dr + (1 - sT) dT f_dt + (1 - sT) f_sr + sT f_st with the
+// lobes gated by their (detached) selection probabilities, into baseColor, metallic, roughness,
+// IoR (through F0 and through the relative IoR of f_st), transmissionColor (the albedo is its
+// square root unless the surface is thin), diffuseTransmission and specularTransmission.
+FHP_DEV void bsdfAdjointTrans(const Shading& sd, const Material& m, bool thin, f3 wo, f3 adj, MatGrad& g) {
+    f3 wiL = sd.toLocal(sd.V), woL = sd.toLocal(wo);
+    if (!validRT(sd, wiL, woL, wo)) return;
+    Bsdf b;
+    b.init(wiL, m);
+    const float sT = b.specTrans, dT = b.diffTrans;
+    const float wDR = (1.f - sT) * (1.f - dT), wDT = (1.f - sT) * dT, wSR = 1.f - sT, wST = sT;
+    f3 fDR = mk3(0.f), fDT = mk3(0.f), fSR = mk3(0.f), fST = mk3(0.f);
+    f3 dDiff = mk3(0.f), dSpec = mk3(0.f), dTA = mk3(0.f);
+    float dRough = 0.f, dEta = 0.f;
+    if (b.pDiff > 0.f && fminf_(wiL.z, woL.z) >= kMinCosTheta) {
+        // Frostbite: f = diff * wiS * woS * eF / pi * wo.z
+        f3 h = normalize(wiL + woL);
+        float woDotH = dot(woL, h);
+        float r = b.rough;
+        float eF = 1.f + (1.f / 1.51f - 1.f) * r;
+        float fd90 = 0.5f * r + 2.f * woDotH * woDotH * r;
+        float wi5 = pow5(fmaxf_(1.f - wiL.z, 0.f)), wo5 = pow5(fmaxf_(1.f - woL.z, 0.f));
+        float wiS = 1.f + (fd90 - 1.f) * wi5, woS = 1.f + (fd90 - 1.f) * wo5;
+        float c = kInvPi * woL.z;
+        float K = wiS * woS * eF * c;
+        fDR = b.diff * K;
+        dDiff = adj * (K * wDR);
+        float dfd90 = 0.5f + 2.f * woDotH * woDotH;
+        float dProd = (dfd90 * wi5) * woS * eF + wiS * (dfd90 * wo5) * eF + wiS * woS * (1.f / 1.51f - 1.f);
+        dRough += dot(adj, b.diff) * dProd * c * wDR;
+    }
+    if (b.pDiffT > 0.f && fminf_(wiL.z, -woL.z) >= kMinCosTheta) {
+        // LambertDiffuseBTDF: f = albedo / pi * |wo.z|
+        const float k = kInvPi * -woL.z;
+        fDT = b.trans * k;
+        dTA = dTA + adj * (k * wDT);
+    }
+    if (b.pSpec > 0.f && b.alpha > 0.f && fminf_(wiL.z, woL.z) >= kMinCosTheta) {
+        f3 h = normalize(wiL + woL);
+        float wiDotH = dot(wiL, h);
+        float D = ggxD(b.alpha, h.z), G = ggxG(b.alpha, wiL.z, woL.z);
+        float w5 = pow5(fmaxf_(1.f - wiDotH, 0.f));
+        float sc = D * G * 0.25f / wiL.z;
+        f3 F = schlick3(b.spec, 1.f, wiDotH);
+        fSR = F * sc;
+        dSpec = adj * ((1.f - w5) * sc * wSR);
+        float a2 = b.alpha * b.alpha, cz = h.z;
+        float d = (cz * a2 - cz) * cz + 1.f;
+        float dD = (d - 2.f * a2 * cz * cz) / (kPi * d * d * d);
+        auto dLam = [&](float c0) {
+            float c2 = c0 * c0;
+            float t2 = fmaxf_(1.f - c2, 0.f) / c2;
+            return t2 / (4.f * sqrtf(1.f + a2 * t2));
+        };
+        float dG = -G * G * (dLam(wiL.z) + dLam(woL.z));
+        float dA2 = (dD * G + D * dG) * 0.25f / wiL.z;
+        float r = b.rough;
+        dRough += dot(adj, F) * dA2 * (4.f * r * r * r) * wSR;
+    }
+    if (b.pSpecT > 0.f && b.alphaT > 0.f && fminf_(wiL.z, fabsf(woL.z)) >= kMinCosTheta) {
+        bool isR;
+        const D2 v = specTDual(wiL, woL, b.alphaT, b.eta, isR);
+        const f3 col = isR ? mk3(1.f) : b.trans;
+        fST = col * v.v;
+        const float ac = dot(adj, col) * wST;
+        dRough += ac * v.a * (2.f * b.rough);  // alphaT = roughness^2 (eta != 1, alpha >= the delta bound)
+        dEta += ac * v.e;
+        if (!isR) dTA = dTA + adj * (v.v * wST);
+    }
+    // Mixing weights (the selection probabilities that gate the lobes are detached).
+    g.diffTrans += dot(adj, (fDT - fDR) * (1.f - sT));
+    g.specTrans += dot(adj, fST - fDR * (1.f - dT) - fDT * dT - fSR);
+    // diffuse = base (1 - metal); specular = F0 + (base - F0) metal.
+    g.base = g.base + dDiff * (1.f - m.metallic) + dSpec * m.metallic;
+    g.metal += -dot(dDiff, m.base) + dot(dSpec, m.base - mk3(m.F0));
+    g.rough += dRough;
+    g.ior += (dSpec.x + dSpec.y + dSpec.z) * (1.f - m.metallic);
+    // eta = 1 / IoR on the front side, IoR on the back side (StandardMaterial.slang:147).
+    g.iorEta += sd.frontFacing ? -dEta * b.eta * b.eta : dEta;
+    // Transmission albedo: sqrt(colour), or the colour on thin surfaces (StandardBSDF.slang:131).
+    const f3 tA = b.trans;
+    if (thin) g.trans = g.trans + dTA;
+    else g.trans = g.trans + mk3(tA.x > 0.f ? dTA.x * 0.5f / tA.x : 0.f, tA.y > 0.f ? dTA.y * 0.5f / tA.y : 0.f,
+                                 tA.z > 0.f ? dTA.z * 0.5f / tA.z : 0.f);
+}
+
+// Backpropagates adj (dS/df, RGB) through f = material BSDF eval(wi, wo) into material params
+// (reflection lobes; bsdfAdjointTrans above handles materials with transmission). The mixing
+// weights are differentiated at specularTransmission = diffuseTransmission = 0 too, as the
+// reference reads both through its gradient wrapper for every StandardMaterial
+// (StandardMaterial.slang:148-149): df/d(diffTrans) = -f_dr, df/d(specTrans) = -(f_dr + f_sr).
 FHP_DEV void bsdfAdjoint(const Shading& sd, const Material& m, f3 wo, f3 adj, MatGrad& g) {
     f3 wiL = sd.toLocal(sd.V), woL = sd.toLocal(wo);
     if (m.type == MAT_PBRT_DIFFUSE) {
@@ -55,7 +226,7 @@ FHP_DEV void bsdfAdjoint(const Shading& sd, const Material& m, f3 wo, f3 adj, Ma
     Bsdf b;
     b.init(wiL, m);
     f3 dDiff = mk3(0.f), dSpec = mk3(0.f);
-    float dRough = 0.f;
+    float dRough = 0.f, fDR = 0.f, fSR = 0.f;  // adj . f_dr, adj . f_sr
     if (b.pDiff > 0.f && fminf_(wiL.z, woL.z) >= kMinCosTheta) {
         // Frostbite: f = diff * wiS * woS * eF / pi * wo.z
         f3 h = normalize(wiL + woL);
@@ -68,6 +239,7 @@ FHP_DEV void bsdfAdjoint(const Shading& sd, const Material& m, f3 wo, f3 adj, Ma
         float c = kInvPi * woL.z;
         float K = wiS * woS * eF * c;
         dDiff = adj * K;
+        fDR = dot(adj, b.diff) * K;
         float dfd90 = 0.5f + 2.f * woDotH * woDotH;
         float dProd = (dfd90 * wi5) * woS * eF + wiS * (dfd90 * wo5) * eF + wiS * woS * (1.f / 1.51f - 1.f);
         dRough += dot(adj, b.diff) * dProd * c;
@@ -93,6 +265,7 @@ FHP_DEV void bsdfAdjoint(const Shading& sd, const Material& m, f3 wo, f3 adj, Ma
         float dA2 = (dD * G + D * dG) * 0.25f / wiL.z;
         float r = b.rough;
         dRough += dot(adj, F) * dA2 * (4.f * r * r * r);
+        fSR = dot(adj, F) * s;
     }
     // diffuse = base (1 - metal); specular = F0 + (base - F0) metal.
     g.base = g.base + dDiff * (1.f - m.metallic) + dSpec * m.metallic;
@@ -100,10 +273,22 @@ FHP_DEV void bsdfAdjoint(const Shading& sd, const Material& m, f3 wo, f3 adj, Ma
     g.rough += dRough;
     // dS/dF0, converted to dS/dior = dS/dF0 * dF0/dior when the vertex is flushed.
     g.ior += (dSpec.x + dSpec.y + dSpec.z) * (1.f - m.metallic);
+    g.diffTrans -= fDR;
+    g.specTrans -= fDR + fSR;
 }
 
 FHP_DEVI void atomAdd(float* p, float v) {
     if (v != 0.f) gradAdd(p, v);
+}
+
+// BSDF adjoint of a vertex: the transmission lobes' adjoint for a StandardMaterial with
+// transmission (compiled in the TRANS kernels only), the reflection lobes' otherwise.
+template <bool TRANS>
+FHP_DEVI void matAdjoint(const SceneDev& s, const Shading& sd, const Material& m, f3 wo, f3 adj, MatGrad& g) {
+    if (TRANS && m.type == MAT_STANDARD && (m.specTrans > 0.f || m.diffTrans > 0.f))
+        bsdfAdjointTrans(sd, m, s.mats[size_t(sd.mat) * M_STRIDE + M_THIN] != 0.f, wo, adj, g);
+    else
+        bsdfAdjoint(sd, m, wo, adj, g);
 }
 
 // The gradient-image projection of one vertex's gradient (the debug modes): the entry of the
@@ -124,8 +309,11 @@ FHP_DEVI float debugProjection(const SceneDev& s, const WarArgs& a, const Materi
         case 5: {
             const float ior = s.mats[size_t(mat) * M_STRIDE + M_IOR];
             const float f = (ior - 1.f) / (ior + 1.f);
-            return g.ior * 2.f * f * (2.f / ((ior + 1.f) * (ior + 1.f))) * v[0];
+            return (g.ior * 2.f * f * (2.f / ((ior + 1.f) * (ior + 1.f))) + g.iorEta) * v[0];
         }
+        case 6: return g.trans.x * v[0] + g.trans.y * v[1] + g.trans.z * v[2];
+        case 9: return g.diffTrans * v[0];
+        case 10: return g.specTrans * v[0];
         case 11: return g.emit.x * v[0] + g.emit.y * v[1] + g.emit.z * v[2];
         case 14: return g.emitFactor * v[0];
         default: return 0.f;
@@ -144,7 +332,12 @@ FHP_DEV void flushGrad(const SceneDev& s, const WarArgs& a, uint32_t hashRow, co
         const float ior = s.mats[size_t(mat) * M_STRIDE + M_IOR];
         const float f = (ior - 1.f) / (ior + 1.f);
         const float dfdior = 2.f / ((ior + 1.f) * (ior + 1.f));
-        atomAdd(row + 5, g.ior * 2.f * f * dfdior);
+        atomAdd(row + 5, g.ior * 2.f * f * dfdior + g.iorEta);
+        atomAdd(row + 6, g.trans.x);
+        atomAdd(row + 7, g.trans.y);
+        atomAdd(row + 8, g.trans.z);
+        atomAdd(row + 9, g.diffTrans);
+        atomAdd(row + 10, g.specTrans);
         atomAdd(row + 11, g.emit.x);
         atomAdd(row + 12, g.emit.y);
         atomAdd(row + 13, g.emit.z);
@@ -176,7 +369,7 @@ FHP_DEV void flushGrad(const SceneDev& s, const WarArgs& a, uint32_t hashRow, co
 // texelsS) are in use; otherwise one material serves evaluation and sampling.
 // DEBUG (with ADJ): instead of accumulating the gradients, each vertex adds its debug projection
 // to *dproj.
-template <bool ADJ, bool FROZEN, bool DEBUG = false>
+template <bool ADJ, bool FROZEN, bool DEBUG = false, bool TRANS = false>
 FHP_DEV f3 warPath(const SceneDev& s, const WarArgs& a, uint32_t px, uint32_t py, uint32_t frameSeed, TraceStack& st,
                    f3 Ltot, f3 dL, uint32_t hashRow, unsigned long long& nc, unsigned long long& nv,
                    float* dproj = nullptr) {
@@ -298,7 +491,7 @@ FHP_DEV f3 warPath(const SceneDev& s, const WarArgs& a, uint32_t px, uint32_t py
                 w = w * misN;
             }
             radiance = radiance + thp * w;
-            if (ADJ) bsdfAdjoint(sd, m, wo, dL * thp * Li * (a2s / a2s) * misN, g);
+            if (ADJ) matAdjoint<TRANS>(s, sd, m, wo, dL * thp * Li * (a2s / a2s) * misN, g);
         }
         bool cont = false;
         f3 fS = mk3(0.f), woS = mk3(0.f);
@@ -322,7 +515,7 @@ FHP_DEV f3 warPath(const SceneDev& s, const WarArgs& a, uint32_t px, uint32_t py
                 f3 R = Ltot - radiance;
                 f3 adj = mk3(fS.x != 0.f ? dL.x * R.x / fS.x : 0.f, fS.y != 0.f ? dL.y * R.y / fS.y : 0.f,
                              fS.z != 0.f ? dL.z * R.z / fS.z : 0.f);
-                bsdfAdjoint(sd, m, woS, adj, g);
+                matAdjoint<TRANS>(s, sd, m, woS, adj, g);
             }
             if (DEBUG) *dproj += debugProjection(s, a, m, sd.mat, g);
             else flushGrad(s, a, hashRow, m, sd.mat, tap, g);
@@ -338,7 +531,7 @@ FHP_DEV f3 warPath(const SceneDev& s, const WarArgs& a, uint32_t px, uint32_t py
 // One item of the kernels' grid loop (path id = pixel * spp + sample): the primal writes the
 // path's radiance; the adjoint traces the path, then replays it from the same seed (one launch:
 // splitting the two passes into launches measured slower).
-template <bool ADJ, bool FROZEN>
+template <bool ADJ, bool FROZEN, bool TRANS = false>
 FHP_DEVI void warItem(const SceneDev& s, const WarArgs& a, uint32_t id, TraceStack& st, unsigned long long& nc,
                       unsigned long long& nv) {
     const uint32_t pix = id / a.spp, k = id % a.spp;
@@ -357,7 +550,7 @@ FHP_DEVI void warItem(const SceneDev& s, const WarArgs& a, uint32_t id, TraceSta
         if (dL.x == 0.f && dL.y == 0.f && dL.z == 0.f) return;
         unsigned long long c0 = 0, c1 = 0;
         f3 L = warPath<false, FROZEN>(s, a, px, py, seed, st, mk3(0.f), mk3(0.f), 0, c0, c1);
-        warPath<true, FROZEN>(s, a, px, py, seed, st, L, dL, jenkins(pix) % a.hashSize, nc, nv);
+        warPath<true, FROZEN, false, TRANS>(s, a, px, py, seed, st, L, dL, jenkins(pix) % a.hashSize, nc, nv);
         nc += c0;
         nv += c1;
     }
@@ -367,7 +560,7 @@ FHP_DEVI void warItem(const SceneDev& s, const WarArgs& a, uint32_t id, TraceSta
 // dL/dI = (1, 0, 0) projected on the debug direction: the forward-mode derivative of the red
 // channel (the reverse-mode gradient of one path estimate, dotted with a direction, is its
 // directional derivative). tmp[id] = (radiance, derivative).
-template <bool FROZEN>
+template <bool FROZEN, bool TRANS>
 FHP_DEVI void warDebugItem(const SceneDev& s, const WarArgs& a, uint32_t id, TraceStack& st, unsigned long long& nc,
                            unsigned long long& nv) {
     const uint32_t pix = id / a.spp, k = id % a.spp;
@@ -376,7 +569,7 @@ FHP_DEVI void warDebugItem(const SceneDev& s, const WarArgs& a, uint32_t id, Tra
     unsigned long long c0 = 0, c1 = 0;
     const f3 L = warPath<false, FROZEN>(s, a, px, py, seed, st, mk3(0.f), mk3(0.f), 0, nc, nv);
     float d = 0.f;
-    warPath<true, FROZEN, true>(s, a, px, py, seed, st, L, mk3(1.f, 0.f, 0.f), 0, c0, c1, &d);
+    warPath<true, FROZEN, true, TRANS>(s, a, px, py, seed, st, L, mk3(1.f, 0.f, 0.f), 0, c0, c1, &d);
     a.tmp[id] = make_float4(L.x, L.y, L.z, d);
 }
 

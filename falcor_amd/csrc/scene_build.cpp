@@ -757,6 +757,14 @@ int sceneFromDesc(const FhpSceneDesc& d, HostScene& h, std::string& err) {
 }
 
 
+bool hasTransmission(const HostScene& h) {
+    for (uint32_t m = 0; m < h.nM; m++) {
+        const float* r = &h.mats[size_t(m) * M_STRIDE];
+        if (int(r[M_TYPE]) == MAT_STANDARD && (r[M_SPEC_TRANS] > 0.f || r[M_DIFF_TRANS] > 0.f)) return true;
+    }
+    return false;
+}
+
 std::vector<float> packVertexRecords(const HostScene& h) {
     std::vector<float> vtx(8 * size_t(h.nV));
     for (uint32_t i = 0; i < h.nV; i++) {

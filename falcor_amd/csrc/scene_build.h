@@ -64,6 +64,9 @@ struct HostScene {
 int sceneFromDesc(const FhpSceneDesc& d, HostScene& h, std::string& err);
 // Vertex records as the kernels read them: (p.xyz, n.x) (n.yz, u, v), 8 floats per vertex.
 std::vector<float> packVertexRecords(const HostScene& h);
+// True when some StandardMaterial has specular or diffuse transmission (such scenes run the
+// adjoint instances that differentiate the transmission lobes).
+bool hasTransmission(const HostScene& h);
 void computeCamera(HostScene& s);
 // Mesh triangle ranges and the PackedHitInfo bit allocation; throws when the header does not fit
 // in 64 bits (HitInfo.cpp:83-87).

@@ -15,6 +15,7 @@ ROOT = HERE.parents[1]
 CSRC = ROOT / "falcor_amd" / "csrc"
 OUT_DIR = HERE / "_build"
 EXE = OUT_DIR / "hostcheck"
+BSDF_EXE = OUT_DIR / "bsdfcheck"
 
 
 def hipcc() -> str:
@@ -24,23 +25,32 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
-def build(force: bool = False) -> Path:
-    srcs = [HERE / "hostcheck.cpp", CSRC / "scene_build.cpp"]
-    deps = srcs + sorted(CSRC.glob("*.h")) + [ROOT / "include" / "fhp.h"]
-    if EXE.exists() and not force and all(d.stat().st_mtime <= EXE.stat().st_mtime for d in deps):
-        return EXE
+def _compile(srcs, exe: Path, force: bool) -> Path:
+    deps = list(srcs) + sorted(CSRC.glob("*.h")) + [ROOT / "include" / "fhp.h"]
+    if exe.exists() and not force and all(d.stat().st_mtime <= exe.stat().st_mtime for d in deps):
+        return exe
     OUT_DIR.mkdir(exist_ok=True)
     # Host-only compilation: no device code is generated, and the sanitizer applies to the host
     # code (-Xarch_host), the only code in this program.
     cmd = [hipcc(), "-x", "hip", "--offload-host-only", "-DFHP_HOST_CHECK", "-std=c++17", "-O1", "-g",
            "-fno-omit-frame-pointer", "-ffp-contract=off", "-Xarch_host", "-fsanitize=address",
            "-Xarch_host", "-fsanitize=undefined", "-Wno-unused-command-line-argument",
-           *map(str, srcs), "-o", str(EXE)]
+           *map(str, srcs), "-o", str(exe)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"host check build failed:\n{' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
-    return EXE
+    return exe
+
+
+def build(force: bool = False) -> Path:
+    return _compile([HERE / "hostcheck.cpp", CSRC / "scene_build.cpp"], EXE, force)
+
+
+def build_bsdfcheck(force: bool = False) -> Path:
+    """The BSDF adjoint check (bsdfcheck.cpp): adjoints against central differences of the evaluation."""
+    return _compile([HERE / "bsdfcheck.cpp"], BSDF_EXE, force)
 
 
 if __name__ == "__main__":
     print(build(force=True))
+    print(build_bsdfcheck(force=True))

@@ -102,6 +102,7 @@ SceneDev bindScene(const HostScene& h, Arena& ar) {
     dv.nMeshes = uint32_t(h.meshOff.size() - 1);
     dv.hit = HitInfoLayout{h.hitCompressed, h.hitTypeBits, h.hitInstBits, h.hitPrimBits};
     dv.nTris = h.nT;
+    dv.hasTransmission = hasTransmission(h) ? 1u : 0u;
     dv.nLights = h.nLights;
     dv.nActive = uint32_t(h.activeLights.size());
     dv.nLbvh = h.nLbvh;
@@ -247,7 +248,8 @@ int main(int argc, char** argv) {
                 uint32_t peak = 0;
                 const uint64_t n = uint64_t(W) * H * spp;
                 for (uint32_t id = 0; id < n; id++) {
-                    if (adj) warItem<true, false>(s, a, id, ls.st, nc, nv);
+                    if (adj && s.hasTransmission) warItem<true, false, true>(s, a, id, ls.st, nc, nv);
+                    else if (adj) warItem<true, false>(s, a, id, ls.st, nc, nv);
                     else warItem<false, false>(s, a, id, ls.st, nc, nv);
                     peak = std::max(peak, ls.st.peak);
                 }

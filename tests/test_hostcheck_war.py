@@ -13,7 +13,8 @@ out-of-bounds access aborts the program with the sanitizer's report. The checks:
   renders and 256 spp backward passes over 8 frames each) with FHP_HOSTCHECK_FULL=1;
 * the same passes at the vertex-gradient workload's settings (16 auxiliary rays, 10^3);
 * shallow LDS stacks (2 and 4 entries) so traversal runs through the overflow part of the stack;
-* multi-bounce primal and adjoint paths on a sphere field and on textured spheres.
+* multi-bounce primal and adjoint paths on a sphere field, on textured spheres and on a sphere
+  field of transmissive materials (the adjoint instances with the transmission lobes).
 Each step must also report no trace error (no ray needed more stack than the tree's bound).
 A stack made smaller than the tree needs must report the overflow, still without any
 out-of-bounds access.
@@ -89,10 +90,22 @@ def test_overflow_stack_in_bounds(box_sphere, lds):
     assert max(peaks) > lds  # the overflow part of the stack was used
 
 
-@pytest.mark.parametrize("which", ["sphere_field", "textured_spheres"])
+def _transmissive_field():
+    data = S.sphere_field(2, num_spheres=64, num_emissive=256)
+    m = data.materials
+    for i in range(64):
+        m[i, S.M_SPEC_TRANS] = (0.0, 1.0, 0.6, 0.3)[i % 4]
+        m[i, S.M_DIFF_TRANS] = (0.0, 0.0, 0.5, 0.8)[i % 4]
+        m[i, S.M_TRANS:S.M_TRANS + 3] = (0.9, 0.7, 0.5)
+        m[i, S.M_THIN] = float(i % 8 == 3)
+    return data
+
+
+@pytest.mark.parametrize("which", ["sphere_field", "textured_spheres", "transmissive_field"])
 def test_multi_bounce_paths_in_bounds(which):
-    data = S.sphere_field(1, num_spheres=64, num_emissive=256) if which == "sphere_field" else \
-        S.textured_spheres(3, num_objects=4, tex_res=16)
+    data = {"sphere_field": lambda: S.sphere_field(1, num_spheres=64, num_emissive=256),
+            "textured_spheres": lambda: S.textured_spheres(3, num_objects=4, tex_res=16),
+            "transmissive_field": _transmissive_field}[which]()
     W = H = 48
     _run(data, [f"primal {W} {H} 4 0 4 16", f"adjoint {W} {H} 4 1 4 16 16", f"adjoint {W} {H} 2 2 1 4 3"],
          np.ones((H, W, 3), np.float32))
