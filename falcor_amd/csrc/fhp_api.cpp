@@ -242,6 +242,7 @@ const std::map<std::string, double>& warDefaults() {
         {"fixedSeed", 0},       {"useFixedSeed", 0},      {"useBSDFSampling", 1},   {"useNEE", 1},
         {"useMIS", 1},          {"useWAR", 0},            {"auxSampleCount", 16},   {"Log10vMFConcentration", 5},
         {"Log10vMFConcentrationScreen", 5},               {"boundaryTermBeta", 0.01}, {"useAntitheticSampling", 1},
+        {"useSolidAngleNEEWarp", 0},  // extension: NEE vertex gradients with the sampled direction held
         {"countTraversal", 0},  {"timeKernels", 0},       {"traceBlockSize", 128},
         {"traceStackDepth", 24},                          {"traceBlocksPerCU", 10}};  // 1280 threads per CU: 5 waves per SIMD
     return d;
@@ -1600,6 +1601,7 @@ int fhp_pass_backward(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t sp
         ga.aux = uint32_t(p->get("auxSampleCount"));
         ga.kappa = float(std::pow(10.0, p->get("Log10vMFConcentration")));
         ga.kappaScreen = float(std::pow(10.0, p->get("Log10vMFConcentrationScreen")));
+        ga.neeHeld = p->get("useSolidAngleNEEWarp") != 0;
         ga.dLdI = a.dLdI;
         ga.gradPos = g->posTmp.as<float>();
         ga.gradNrm = g->nrmTmp.as<float>();

@@ -182,6 +182,7 @@ struct GeomArgs {
     uint32_t useBSDFSampling, useNEE, useMIS, useAntithetic;
     uint32_t aux;              // auxiliary rays per reparameterization
     float kappa, kappaScreen;  // vMF concentrations (10^Log10vMFConcentration[Screen])
+    uint32_t neeHeld;          // 1: NEE direction held, no area factor (useSolidAngleNEEWarp); 0: reference
     const float* dLdI;
     float* gradPos;            // MeshPosition gradients [hashSize][3 * meshVerts]
     float* gradNrm;            // MeshNormal gradients, same layout

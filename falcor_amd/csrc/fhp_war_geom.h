@@ -232,6 +232,14 @@ FHP_DEV void addVertexGrads3(const SceneDev& s, float* grad, const GeomArgs& a, 
     }
 }
 
+// The harmonic weight's direction derivative dW = c dot(-w x (w x d), a) is clamped to
+// [-1e10, 1e10] (fwd_computeHarmonicWeight, WarpedAreaReparam.slang:93). For the tangents
+// d = e_k of a ray warp it is the k-th component of g = c (a (w.w) - w (w.a)); a clamped component
+// is constant in the parameters.
+constexpr float kDWClamp = 1e10f;
+FHP_DEVI float clampDW(float v) { return fminf_(fmaxf_(v, -kDWClamp), kDWClamp); }
+FHP_DEVI f3 clampDW3(f3 g) { return mk3(clampDW(g.x), clampDW(g.y), clampDW(g.z)); }
+
 // Adjoint of reparameterizeRay (WarpedAreaReparam.slang:226-263) for a ray (o, w): given dS/dV
 // (gV) and dS/d detJ (gDet), adds the aux-triangle vertex gradients and returns dS/do; with
 // wantOmega also dS/dw through the harmonic weights' direction derivatives.
@@ -250,7 +258,7 @@ FHP_DEV void warpAdjoint(const SceneDev& s, const GeomArgs& a, TraceStack& st, f
         const f3 ai = gen.next(i, uy);
         const AuxHit h = auxTrace(s, st, o, ai, uy, a.kappa);
         nc++;
-        const f3 g = (ai * ww - w * dot(w, ai)) * h.c;
+        const f3 g = clampDW3((ai * ww - w * dot(w, ai)) * h.c);
         Sw += h.w;
         Sd = Sd + h.d * h.w;
         Mc0 = Mc0 + h.d * g.x;
@@ -275,7 +283,8 @@ FHP_DEV void warpAdjoint(const SceneDev& s, const GeomArgs& a, TraceStack& st, f
         const f3 ai = gen.next(i, uy);
         const AuxHit h = auxTrace(s, st, o, ai, uy, a.kappa);
         nc++;
-        const f3 g = (ai * ww - w * dot(w, ai)) * h.c;
+        const f3 gu = (ai * ww - w * dot(w, ai)) * h.c;
+        const f3 g = clampDW3(gu);
         const float sk = h.w / Swc;
         const f3 dT = Nm(g) + common * sk;
         const f3 gd = qV * sk + dT * (gDet / Swc);
@@ -289,7 +298,10 @@ FHP_DEV void warpAdjoint(const SceneDev& s, const GeomArgs& a, TraceStack& st, f
             }
         }
         if (wantOmega) {
-            const f3 qk = Nm(h.d);
+            // d/dw of g . qk over the components of g that are not clamped.
+            f3 qk = Nm(h.d);
+            qk = mk3(fabsf(gu.x) <= kDWClamp ? qk.x : 0.f, fabsf(gu.y) <= kDWClamp ? qk.y : 0.f,
+                     fabsf(gu.z) <= kDWClamp ? qk.z : 0.f);
             gW = gW + (w * (2.f * dot(ai, qk)) - ai * dot(w, qk) - qk * dot(w, ai)) * (gDet / Swc * h.c);
         }
     }
@@ -326,8 +338,8 @@ FHP_DEV void screenWarpAdjoint(const SceneDev& s, const GeomArgs& a, TraceStack&
         const f3 g = (ai - w * dot(w, ai)) * h.c;
         Sw += h.w;
         Sd = Sd + h.d * h.w;
-        Mh0 = Mh0 + h.d * dot(cx, g);
-        Mh1 = Mh1 + h.d * dot(cy, g);
+        Mh0 = Mh0 + h.d * clampDW(dot(cx, g));
+        Mh1 = Mh1 + h.d * clampDW(dot(cy, g));
     }
     const float Swc = fmaxf_(Sw, 1e-8f);
     const f3 m = Sd / Swc;
@@ -366,7 +378,7 @@ FHP_DEV void screenWarpAdjoint(const SceneDev& s, const GeomArgs& a, TraceStack&
         nc++;
         if (!h.hit) continue;
         const f3 g = (ai - w * dot(w, ai)) * h.c;
-        const float hx = dot(cx, g), hy = dot(cy, g);
+        const float hx = clampDW(dot(cx, g)), hy = clampDW(dot(cy, g));
         const float sk = h.w / Swc;
         const f3 dT = Nm(Px * hx + Py * hy) + common * sk;
         const f3 gd = qV * sk + dT * (gDet / Swc);
@@ -460,21 +472,32 @@ FHP_DEV void geomPath(const SceneDev& s, const GeomArgs& a, uint32_t px, uint32_
                 }
                 float gci, gco, gcio;
                 const float phi = bsdfGeomGrad(sd, m, wo, dL * Li * misN, gci, gco, gcio);
-                if (phi != 0.f) {
+                const float cosL = dot(ls.normalW, -wo);
+                if (phi != 0.f && a.neeHeld) {
+                    // useSolidAngleNEEWarp: the sampled direction wo is held (a solid-angle sample)
+                    // and only the warp moves it, d = wo + V; no area factor. The warp then follows
+                    // the light point seen from the moving x1 and detJ carries its solid-angle
+                    // change, which is the derivative of the area-measure integrand.
                     QN = phi;
                     gNs = gNs + sd.V * gci + wo * gco;
                     gWi = gWi + sd.N * gci + wo * gcio;
-                    // Change of variables on the light: y = T(y') is the light point hit from x1 in
-                    // direction w0(y') + V, with w0 the (detached) sampled direction. The area
-                    // factors of T and of the geometry term cancel, leaving the integrand
-                    //   Le f(x1, N, w0 + V) detJ / p(w0),
-                    // so only the BSDF (through the warp and the shading normal) and detJ depend
-                    // on the parameters; V follows the light's edges and the occluders as seen from
-                    // the moving x1 (origin term).
-                    const f3 gF = sd.N * gco + sd.V * gcio;  // d phi / d direction
+                    const f3 gF = sd.N * gco + sd.V * gcio;
                     f3 gO, gOm;
                     warpAdjoint(s, a, st, rayOriginOffset(x1, faceO), wo, rng, gF, QN, false, row, gO, gOm, nc);
                     gX = gX + gO;
+                } else if (phi != 0.f && cosL > 0.f) {
+                    // computeNEE (PTUtils.slang:211-249): Q_N = phi(x1, N, d) (a2s / a2s^) detJ with
+                    // wo = normalize(y - x1) toward the held light point y, d = wo + V and
+                    // a2s = |y - x1|^2 / dot(n_y, -d); the warp's rays start at x1's view-side
+                    // offset and its Jacobian's harmonic weights differentiate along wo.
+                    QN = phi;
+                    gNs = gNs + sd.V * gci + wo * gco;
+                    gWi = gWi + sd.N * gci + wo * gcio;
+                    const f3 gD = sd.N * gco + sd.V * gcio + ls.normalW * (phi / cosL);  // dS/dd
+                    f3 gO, gOm;
+                    warpAdjoint(s, a, st, rayOriginOffset(x1, faceO), wo, rng, gD, QN, true, row, gO, gOm, nc);
+                    const f3 gWo = gD + gOm;
+                    gX = gX + gO - (gWo - wo * dot(wo, gWo)) / dist - wo * (2.f * phi / dist);
                 }
             }
         }

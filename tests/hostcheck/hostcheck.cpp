@@ -13,7 +13,9 @@
 //     translate <mesh> <dx> <dy> <dz>   move the mesh's vertices (host vertex update + rebuild)
 //     primal <W> <H> <spp> <frame> <maxBounces> <lds>
 //     adjoint <W> <H> <spp> <frame> <maxBounces> <lds> <hash>
-//     geom <W> <H> <spp> <frame> <aux> <log10kappa> <log10kappaScreen> <mesh> <hash> <lds>
+//     geom <W> <H> <spp> <frame> <aux> <log10kappa> <log10kappaScreen> <mesh> <hash> <lds> [out=<file>] [nee=0] [neeheld=1]
+// (out: the mesh's position then normal gradients, hash rows summed, as float64; nee=0: no NEE;
+// neeheld=1: the held NEE direction of useSolidAngleNEEWarp)
 // A primal step may end with cap=<n>: the lane's stack then holds n entries in total (fewer than
 // the tree needs), to check that overflow is reported without any out-of-bounds access.
 // Prints one line per step: rays, the deepest stack entry used, the stack capacity and the
@@ -260,6 +262,14 @@ int main(int argc, char** argv) {
                 float lk, lks;
                 in >> W >> H >> spp >> frame >> aux >> lk >> lks >> mesh >> hash >> lds;
                 if (!in) throw std::runtime_error("bad step: " + line);
+                std::string gradOut;
+                uint32_t useNEE = 1, neeHeld = 0;
+                for (std::string opt; in >> opt;) {
+                    if (opt.rfind("out=", 0) == 0) gradOut = opt.substr(4);
+                    else if (opt == "nee=0") useNEE = 0;
+                    else if (opt == "neeheld=1") neeHeld = 1;
+                    else throw std::runtime_error("bad step option: " + opt);
+                }
                 if (ndl != 3 * size_t(W) * H) throw std::runtime_error("dLdI size does not match the frame");
                 if (mesh + 1 >= ck.h.meshOff.size()) throw std::runtime_error("mesh id out of range");
                 const uint32_t v0 = ck.h.meshOff[mesh], nvtx = ck.h.meshOff[mesh + 1] - v0;
@@ -268,6 +278,8 @@ int main(int argc, char** argv) {
                 GeomArgs a{};
                 a.W = W; a.H = H; a.spp = spp; a.frame = frame * spp;
                 a.useBSDFSampling = a.useNEE = a.useMIS = a.useAntithetic = 1;
+                a.useNEE = useNEE;
+                a.neeHeld = neeHeld;
                 a.aux = aux;
                 a.kappa = float(std::pow(10.0, double(lk)));
                 a.kappaScreen = float(std::pow(10.0, double(lks)));
@@ -293,6 +305,17 @@ int main(int argc, char** argv) {
                 }
                 report("geom", n, nc, ls, peak);
                 std::printf("geom grad_abs_sum=%.6e finite=%d\n", sum, int(finite));
+                if (!gradOut.empty()) {
+                    // Position then normal gradients of the mesh's vertices, hash rows summed (float64).
+                    std::vector<double> g(6 * size_t(nvtx), 0.0);
+                    for (uint32_t r = 0; r < hash; r++)
+                        for (size_t i = 0; i < 3 * size_t(nvtx); i++) {
+                            g[i] += gp[size_t(r) * 3 * nvtx + i];
+                            g[3 * size_t(nvtx) + i] += gn[size_t(r) * 3 * nvtx + i];
+                        }
+                    std::ofstream f(gradOut, std::ios::binary);
+                    f.write(reinterpret_cast<const char*>(g.data()), std::streamsize(g.size() * sizeof(double)));
+                }
                 bad |= ls.err != 0;
             } else {
                 throw std::runtime_error("unknown step: " + line);

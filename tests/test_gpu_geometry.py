@@ -1,21 +1,24 @@
-"""WARDiffPathTracer vertex gradients (warped-area reparameterization, useWAR = 1, maxBounces = 0)
-against central finite differences of the primal image.
+"""WARDiffPathTracer vertex gradients (warped-area reparameterization, useWAR = 1, maxBounces = 0).
 
-The image expectation is smooth in the vertex positions although every sample is discontinuous
-at silhouettes and shadow edges; the reparameterized estimator differentiates the expectation.
-The finite differences use the same sample seeds on both sides (common random numbers) and
-enough samples that their own noise stays well below the tolerance. Each check projects the
-gradient on a rigid or uniform-scale deformation of the sphere, which leaves vertex normals
-unchanged (so only MeshPosition gradients enter), against a weight image dLdI chosen so the
-derivative is far from zero.
+* Parity: the kernel's MeshPosition / MeshNormal gradients equal the oracle's restatement of the
+  reference's estimator on the same seeds (test_vertex_gradients_match_oracle_estimator; the
+  oracle's own pins against finite differences are in tests/test_oracle_war_geom.py).
+* Finite differences: the image expectation is smooth in the vertex positions although every
+  sample is discontinuous at silhouettes and shadow edges; the reparameterized estimator
+  differentiates the expectation. The checks use the held NEE direction (useSolidAngleNEEWarp),
+  the consistent form of the NEE term: the reference's own form (PTUtils.slang:211-249) estimates
+  the vertical translation of this scene at about twice its finite difference. The finite
+  differences use the same sample seeds on both sides (common random numbers) and enough samples
+  that their own noise stays well below the tolerance. Each check projects the gradient on a
+  rigid or uniform-scale deformation of the sphere, which leaves vertex normals unchanged (so only
+  MeshPosition gradients enter), against a weight image dLdI chosen so the derivative is far from
+  zero. Tolerance: 0.1 of the finite-difference derivative. Both sides are Monte Carlo estimates,
+  and the warp itself is a ratio estimate whose bias shrinks with more auxiliary rays and a higher
+  vMF concentration: 64 auxiliary rays and concentration 10^4 keep it at a few percent.
 
 The scene is a closed box (walls, ceiling light flush with the ceiling) around the sphere: the
 warp's harmonic weights locate discontinuities at silhouettes (grazing aux hits), so edges of
 open surfaces floating in empty space are outside the estimator's assumptions.
-
-Tolerance: relative error <= 0.1 of the finite-difference derivative. Both sides are Monte Carlo
-estimates, and the warp itself is a ratio estimate whose bias shrinks with more auxiliary rays and
-a higher vMF concentration: 64 auxiliary rays and concentration 10^4 keep it at a few percent.
 """
 import numpy as np
 import pytest
@@ -43,7 +46,8 @@ def _setup(props=None):
     sc = Scene(data)
     mesh = data.mesh_names.index("Sphere")
     p = WARDiffPathTracer(dict({"maxBounces": 0, "useWAR": True, "diffMode": "BackwardDiff", "Log10vMFConcentration": 4,
-                                "Log10vMFConcentrationScreen": 4, "auxSampleCount": 64}, **(props or {})))
+                                "Log10vMFConcentrationScreen": 4, "auxSampleCount": 64, "useSolidAngleNEEWarp": 1},
+                               **(props or {})))
     p.set_scene(sc)
     return sc, p, mesh
 
@@ -119,6 +123,58 @@ def test_scale_gradient_matches_fd(gpu):
     ad = float((gp * (pos0 - c)).sum())
     print(f"scale: adjoint {ad:.5f}  fd {fd:.5f}  rel {abs(ad - fd) / abs(fd):.3f}", flush=True)
     assert abs(ad - fd) <= TOL * abs(fd)
+
+
+@pytest.mark.parametrize("aux,log_kappa,held", [(16, 3, False), (64, 4, False), (16, 3, True)])
+def test_vertex_gradients_match_oracle_estimator(gpu, oracle_mod, aux, log_kappa, held):
+    """The kernel's MeshPosition / MeshNormal gradients against the oracle's restatement of the
+    same estimator (oracle/fo_war_geom.cpp: the path radiance with every sampled decision held,
+    differentiated by central differences in double precision), same seeds: projections on three
+    translations, a uniform scale, a random position field and a random normal field, each within
+    2e-3 of the oracle's value (FP32 atomics on the GPU against FP64). 32 x 32, 8 spp, with the
+    reference's NEE formulation and with the held NEE direction (useSolidAngleNEEWarp)."""
+    import torch
+    from falcor_amd.render_graph import Scene, SceneGradients, WARDiffPathTracer
+    W2 = H2 = 32
+    spp, frame = 8, 5
+    data = S.displaced_sphere_in_box(0, radius=0.12, segments=(24, 12), sigma=0.05)
+    mesh = data.mesh_names.index("Sphere")
+    off = data.mesh_vertex_offsets
+    v0, v1 = int(off[mesh]), int(off[mesh + 1])
+    sc = Scene(data)
+    p = WARDiffPathTracer({"maxBounces": 0, "useWAR": True, "diffMode": "BackwardDiff", "auxSampleCount": aux,
+                           "Log10vMFConcentration": log_kappa, "Log10vMFConcentrationScreen": log_kappa,
+                           "useSolidAngleNEEWarp": int(held)})
+    p.set_scene(sc)
+    x = (np.arange(W2) + 0.5) / W2
+    d = np.ascontiguousarray(np.repeat(np.broadcast_to(x[None, :], (H2, W2))[..., None], 3, axis=2).astype(np.float32))
+    g = SceneGradients(sc, hash_size=16, texture_grads=False, mesh_id=mesh)
+    p.scene_gradients = g
+    p.set_mesh_to_optimize(mesh)
+    p.dL_dI = torch.from_numpy(d).cuda()
+    g.clear_all_grads()
+    p.backward(W2, H2, spp, frame=frame)
+    g.aggregate_all_grads()
+    assert sc.trace_errors() == 0
+    gp = g.get_grads_buffer("MeshPosition").view(-1, 3).cpu().numpy().astype(np.float64)
+    gn = g.get_grads_buffer("MeshNormal").view(-1, 3).cpu().numpy().astype(np.float64)
+    assert np.isfinite(gp).all() and np.isfinite(gn).all()
+    osc = oracle_mod.OracleScene(data)
+    rng = np.random.default_rng(11)
+    pr = np.zeros((6, osc.num_vertices, 6), np.float32)
+    for a in range(3):
+        pr[a, v0:v1, a] = 1.0
+    pr[3, v0:v1, 0:3] = data.positions[v0:v1] - data.positions[v0:v1].mean(0)
+    pr[4, v0:v1, 0:3] = rng.normal(size=(v1 - v0, 3))
+    pr[5, v0:v1, 3:6] = rng.normal(size=(v1 - v0, 3))
+    ref, _ = osc.war_geom_probe(W2, H2, spp, frame * spp, d, pr, aux=aux, log10_kappa=log_kappa,
+                                log10_kappa_screen=log_kappa, nee_direction_held=held)
+    got = np.array([(gp * pr[k, v0:v1, 0:3]).sum() + (gn * pr[k, v0:v1, 3:6]).sum() for k in range(6)])
+    rel = np.abs(got - ref) / np.abs(ref)
+    print(f"aux {aux} kappa 1e{log_kappa} held {held}: kernel {np.round(got, 4)} oracle {np.round(ref, 4)} "
+          f"rel {np.array2string(rel, precision=2)}", flush=True)
+    assert np.all(np.abs(ref) > 0)
+    assert np.all(rel <= 2e-3)
 
 
 def test_primal_does_not_depend_on_war(gpu):
