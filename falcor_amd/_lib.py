@@ -69,6 +69,10 @@ class View(C.Structure):
     ]
 
 
+class DiffView(C.Structure):
+    _fields_ = [("h_camera", C.c_void_p), ("frame", C.c_uint32), ("d_dLdI", C.c_void_p)]
+
+
 # name -> (restype, argtypes)
 _SIGS = {
     "fhp_get_version": (C.c_int, []),
@@ -107,6 +111,8 @@ _SIGS = {
     "fhp_grads_copy": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_uint64, C.c_void_p]),
     "fhp_pass_backward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p,
                                     C.c_void_p, C.c_void_p, C.c_void_p]),
+    "fhp_pass_backward_batch": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(DiffView), C.c_uint32, C.c_uint32,
+                                          C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "fhp_accumulate": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_uint32, C.c_uint32, C.c_void_p, C.c_uint32,
                                  C.c_void_p]),
     "fhp_accumulate_reset": (C.c_int, [C.c_void_p]),

@@ -168,6 +168,7 @@ struct FhpPass_ {
     // Wavefront buffers (forward pass).
     DevBuf thp, prevN, rng, Lp, Ls, rayQ0, rayQ1, pathQ0, pathQ1, hitQ, hitIdx, shQ, pixHit, counters, stats;
     DevBuf viewTab, sceneTab;  // batch mode tables
+    DevBuf warViewTab;         // batched WARDiffPathTracer views
     uint32_t cap = 0, capPix = 0;
     KernelTimer timer;
     // AccumulatePass state: running sums of the current precision mode, frames accumulated,
@@ -1370,10 +1371,14 @@ int fhp_pass_execute_noalloc(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint
     return FHP_OK;
 }
 
+static int warExecuteBatch(FhpPass p, uint32_t nv, const FhpView* views, uint64_t* h_stats, FhpStream stream);
+
 int fhp_pass_execute_batch(FhpPass p, uint32_t nv, const FhpView* views, uint64_t* h_stats, FhpStream stream) {
     if (!p || (nv && !views)) return fail(FHP_EINVAL, "null argument");
-    if (p->kind != "MinimalPathTracer") return fail(FHP_EINVAL, "batched execution is available for MinimalPathTracer");
     if (nv == 0) return FHP_OK;
+    if (p->kind == "WARDiffPathTracer") return warExecuteBatch(p, nv, views, h_stats, stream);
+    if (p->kind != "MinimalPathTracer")
+        return fail(FHP_EINVAL, "batched execution is available for MinimalPathTracer and WARDiffPathTracer");
     hipStream_t st = hipStream_t(stream);
     std::vector<ViewDev> vt(nv);
     std::vector<FhpScene> scenes;
@@ -1546,8 +1551,97 @@ int fhp_grads_copy(FhpGrads g, int which, float* dst, uint64_t n, FhpStream stre
     return FHP_OK;
 }
 
-int fhp_pass_backward(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t spp, uint32_t frame, const float* d_dLdI,
-                      FhpGrads g, uint64_t* h_stats, FhpStream stream) {
+namespace {
+// A batched view resolved for the kernels (camera from 14 floats, or the scene's).
+WarView warViewFrom(const FhpScene_* s, const float* h_camera, uint32_t frame, const float* dLdI, float* out) {
+    WarView v{};
+    v.cam = s->dev.cam;
+    if (h_camera) {
+        HostScene c;
+        std::memcpy(c.cam, h_camera, sizeof(c.cam));
+        computeCamera(c);
+        v.cam.pos = mk3(c.cam[0], c.cam[1], c.cam[2]);
+        v.cam.U = mk3(c.camU[0], c.camU[1], c.camU[2]);
+        v.cam.V = mk3(c.camV[0], c.camV[1], c.camV[2]);
+        v.cam.W = mk3(c.camW[0], c.camW[1], c.camW[2]);
+        v.cam.nearZ = c.cam[12];
+        v.cam.farZ = c.cam[13];
+    }
+    v.frame = frame;
+    v.dLdI = dLdI;
+    v.out = reinterpret_cast<float4*>(out);
+    return v;
+}
+
+}  // namespace
+
+// WARDiffPathTracer views of one scene and one frame shape rendered in one launch sequence: the
+// primal of every view (Primal, BackwardDiff and ForwardDiffDebug without a debug variable),
+// resolved into each view's d_color; BackwardDiffDebug clears them.
+static int warExecuteBatch(FhpPass p, uint32_t nv, const FhpView* views, uint64_t* h_stats, FhpStream stream) {
+    FhpScene s = views[0].scene;
+    const uint32_t W = views[0].width, H = views[0].height, spp = views[0].spp;
+    for (uint32_t i = 0; i < nv; i++) {
+        const FhpView& v = views[i];
+        if (!v.scene || !v.d_color) return fail(FHP_EINVAL, "view " + std::to_string(i) + ": null scene or output");
+        if (v.scene != s || v.width != W || v.height != H || v.spp != spp)
+            return fail(FHP_EINVAL, "batched WARDiffPathTracer views share one scene, width, height and spp");
+    }
+    if (W == 0 || H == 0 || spp == 0 || W > 65536 || H > 65536) return fail(FHP_EINVAL, "bad frame shape");
+    const uint64_t n = uint64_t(W) * H * spp * nv;
+    if (n > 0xffffffffull / 3) return fail(FHP_EINVAL, "batch too large");
+    const int mode = int(p->get("diffMode"));
+    if (mode == kDiffForwardDebug && p->dbgVar != kDiffVarNone)
+        return fail(FHP_EINVAL, "batched ForwardDiffDebug gradient images are not supported");
+    hipStream_t st = hipStream_t(stream);
+    const uint64_t npix = uint64_t(W) * H;
+    if (mode == kDiffBackwardDebug) {
+        for (uint32_t i = 0; i < nv; i++) HIP_OR_FAIL(launchFill(views[i].d_color, npix * 4, 0.f, st));
+        return FHP_OK;
+    }
+    int rc = ensurePassBuffersN(p, n, npix);
+    if (rc) return rc;
+    unsigned long long* dstats = p->stats.as<unsigned long long>();
+    if (h_stats) HIP_OR_FAIL(launchFill(reinterpret_cast<float*>(dstats), 12, 0.f, st));
+    const bool fixed = p->get("useFixedSeed") != 0;
+    std::vector<WarView> vt(nv);
+    for (uint32_t i = 0; i < nv; i++)
+        vt[i] = warViewFrom(s, views[i].h_camera, fixed ? uint32_t(p->get("fixedSeed")) : views[i].frame * spp, nullptr,
+                            views[i].d_color);
+    HIP_OR_FAIL(p->warViewTab.alloc(vt.size() * sizeof(WarView)));
+    HIP_OR_FAIL(hipMemcpyAsync(p->warViewTab.p, vt.data(), vt.size() * sizeof(WarView), hipMemcpyHostToDevice, st));
+    HIP_OR_FAIL(hipStreamSynchronize(st));
+    TraceScratch ts;
+    if ((rc = passTrace(s, p, ts))) return rc;
+    WarArgs a{};
+    a.W = W; a.H = H; a.spp = spp;
+    a.maxBounces = uint32_t(p->get("maxBounces"));
+    a.useBSDFSampling = p->get("useBSDFSampling") != 0;
+    a.useNEE = p->get("useNEE") != 0;
+    a.useMIS = p->get("useMIS") != 0;
+    a.useAntithetic = p->get("useAntitheticSampling") != 0;
+    a.tmp = p->Lp.as<float4>();
+    a.rays = h_stats ? dstats : nullptr;
+    a.views = p->warViewTab.as<WarView>();
+    a.nViews = nv;
+    if ((rc = prepareTimer(p))) return rc;
+    HIP_OR_FAIL(launchWarPrimal(s->dev, a, ts, st, &p->timer));
+    if ((rc = collectTimer(p, st))) return rc;
+    if (h_stats) {
+        unsigned long long hs[6];
+        HIP_OR_FAIL(hipMemcpyAsync(hs, dstats, 48, hipMemcpyDeviceToHost, st));
+        HIP_OR_FAIL(hipStreamSynchronize(st));
+        for (int i = 0; i < 6; i++) h_stats[i] = hs[i];
+    }
+    return FHP_OK;
+}
+
+namespace {
+// The WARDiffPathTracer adjoint (fhp_pass_backward) of one frame or, with views, of a batch of
+// views of the same shape in one launch sequence. The view table is staged from host memory; the
+// stream is synchronised before returning when views are given.
+int warBackward(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t spp, uint32_t frame, const float* d_dLdI,
+                const std::vector<WarView>* views, FhpGrads g, uint64_t* h_stats, FhpStream stream) {
     if (!p || !s || !g) return fail(FHP_EINVAL, "null argument");
     if (p->kind != "WARDiffPathTracer") return fail(FHP_ESTATE, "backward needs a WARDiffPathTracer pass");
     // runBackward runs the adjoint in BackwardDiff mode (WARDiffPathTracer.rt.slang:161-180) and, with
@@ -1555,11 +1649,16 @@ int fhp_pass_backward(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t sp
     const int mode = int(p->get("diffMode"));
     if (mode != kDiffBackward && mode != kDiffBackwardDebug)
         return fail(FHP_ESTATE, "backward needs diffMode BackwardDiff (1) or BackwardDiffDebug (3)");
-    if (mode == kDiffBackward && !d_dLdI) return fail(FHP_EINVAL, "null argument");
+    const uint32_t nviews = views ? uint32_t(views->size()) : 1u;
+    if (mode == kDiffBackward && !views && !d_dLdI) return fail(FHP_EINVAL, "null argument");
+    if (mode == kDiffBackward && views)
+        for (uint32_t i = 0; i < nviews; i++)
+            if (!(*views)[i].dLdI) return fail(FHP_EINVAL, "view " + std::to_string(i) + ": null dL/dI");
     if (g->scene != s) return fail(FHP_EINVAL, "gradients belong to another scene");
     if (W == 0 || H == 0 || spp == 0) return fail(FHP_EINVAL, "empty frame");
     if (W > 65536 || H > 65536) return fail(FHP_EINVAL, "frame dimension above 65536");
-    if (uint64_t(W) * H * spp > 0xffffffffull / 3) return fail(FHP_EINVAL, "frame too large (width * height * spp paths)");
+    if (uint64_t(W) * H * spp * nviews > 0xffffffffull / 3)
+        return fail(FHP_EINVAL, "frame too large (width * height * spp paths)");
     const bool war = p->get("useWAR") != 0;
     if (war) {
         if (p->get("maxBounces") != 0)
@@ -1584,6 +1683,17 @@ int fhp_pass_backward(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t sp
     a.useMIS = p->get("useMIS") != 0;
     a.useAntithetic = p->get("useAntitheticSampling") != 0;
     a.dLdI = mode == kDiffBackwardDebug ? nullptr : d_dLdI;
+    if (views) {
+        std::vector<WarView> vt = *views;
+        if (mode == kDiffBackwardDebug)
+            for (WarView& v : vt) v.dLdI = nullptr;
+        HIP_OR_FAIL(p->warViewTab.alloc(vt.size() * sizeof(WarView)));
+        HIP_OR_FAIL(hipMemcpyAsync(p->warViewTab.p, vt.data(), vt.size() * sizeof(WarView), hipMemcpyHostToDevice, st));
+        // The staging copy reads vt: wait for it before vt goes out of scope.
+        HIP_OR_FAIL(hipStreamSynchronize(st));
+        a.views = p->warViewTab.as<WarView>();
+        a.nViews = nviews;
+    }
     a.gradMat = g->tmp.as<float>();
     a.hashSize = g->hashSize;
     a.matGradDim = g->dim;
@@ -1607,6 +1717,8 @@ int fhp_pass_backward(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t sp
         ga.gradNrm = g->nrmTmp.as<float>();
         ga.hashSize = g->hashSize; ga.meshV0 = g->meshV0; ga.meshVerts = g->meshVerts;
         ga.rays = h_stats ? dstats + 2 : nullptr;
+        ga.views = a.views;
+        ga.nViews = a.nViews;
         HIP_OR_FAIL(launchWarGeom(s->dev, ga, ts, st, &p->timer));
     }
     if ((rc = collectTimer(p, st))) return rc;
@@ -1617,6 +1729,24 @@ int fhp_pass_backward(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t sp
         for (int i = 0; i < 6; i++) h_stats[i] = hs[i];
     }
     return FHP_OK;
+}
+}  // namespace
+
+int fhp_pass_backward(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t spp, uint32_t frame, const float* d_dLdI,
+                      FhpGrads g, uint64_t* h_stats, FhpStream stream) {
+    return warBackward(p, s, W, H, spp, frame, d_dLdI, nullptr, g, h_stats, stream);
+}
+
+int fhp_pass_backward_batch(FhpPass p, FhpScene s, uint32_t nv, const FhpDiffView* views, uint32_t W, uint32_t H,
+                            uint32_t spp, FhpGrads g, uint64_t* h_stats, FhpStream stream) {
+    if (!p || !s || (nv && !views)) return fail(FHP_EINVAL, "null argument");
+    if (nv == 0) return FHP_OK;
+    std::vector<WarView> vt(nv);
+    const bool fixed = p->get("useFixedSeed") != 0;
+    for (uint32_t i = 0; i < nv; i++)
+        vt[i] = warViewFrom(s, views[i].h_camera, fixed ? uint32_t(p->get("fixedSeed")) : views[i].frame * spp,
+                            views[i].d_dLdI, nullptr);
+    return warBackward(p, s, W, H, spp, 0, nullptr, &vt, g, h_stats, stream);
 }
 
 // ------------------------------------------------------------------ AccumulatePass

@@ -145,6 +145,14 @@ hipError_t launchMptPrologue(const MptArgs& a, hipStream_t stream);
 hipError_t launchMptEpilogue(const MptArgs& a, float4* out, unsigned long long* d_counts, hipStream_t stream);
 
 // WARDiffPathTracer primal and adjoint (megakernel, one path per thread).
+// One view of a batched WARDiffPathTracer launch: its camera, frame seed base, dL/dI image
+// (3 floats per pixel) and output image (RGBA). Every view of a batch has the launch's W, H, spp.
+struct WarView {
+    CameraDev cam;
+    uint32_t frame;
+    const float* dLdI;
+    float4* out;
+};
 struct WarArgs {
     uint32_t W, H, spp, frame, maxBounces;
     uint32_t useBSDFSampling, useNEE, useMIS, useAntithetic;
@@ -162,6 +170,10 @@ struct WarArgs {
     float dbgGrad[4];
     float dbgThreshold;
     float4* dColor;         // debug image (RGBA), or null
+    // Batched views (null: one view with the scene's camera, frame, dLdI and out above). Path id
+    // v * W * H * spp + p is path p of view v; tmp holds every view's samples in that order.
+    const WarView* views;
+    uint32_t nViews;
 };
 hipError_t launchWarPrimal(const SceneDev& s, const WarArgs& a, const TraceScratch& ts, hipStream_t stream,
                            KernelTimer* timer);
@@ -183,6 +195,8 @@ struct GeomArgs {
     uint32_t aux;              // auxiliary rays per reparameterization
     float kappa, kappaScreen;  // vMF concentrations (10^Log10vMFConcentration[Screen])
     uint32_t neeHeld;          // 1: NEE direction held, no area factor (useSolidAngleNEEWarp); 0: reference
+    const WarView* views;      // batched views (camera, frame, dLdI per view), or null
+    uint32_t nViews;
     const float* dLdI;
     float* gradPos;            // MeshPosition gradients [hashSize][3 * meshVerts]
     float* gradNrm;            // MeshNormal gradients, same layout

@@ -314,11 +314,16 @@ FHP_DEVI f3 camRaw(const CameraDev& c, float sx, float sy, uint32_t W, uint32_t 
     return c.U * ndc_x + c.V * ndc_y + c.W;
 }
 
+// The camera of view `view` of a batched launch (the scene's camera without batched views).
+FHP_DEVI CameraDev geomCamera(const SceneDev& s, const GeomArgs& a, uint32_t view) {
+    return a.views ? a.views[view].cam : s.cam;
+}
+
 // Adjoint of reparameterizeScreenSample (WarpedAreaReparam.slang:269-332) at screen sample ps:
 // gVs = dS/dV (screen units), gDet = dS/d detJ. Adds aux-triangle vertex gradients.
-FHP_DEV void screenWarpAdjoint(const SceneDev& s, const GeomArgs& a, TraceStack& st, float psx, float psy, Rng rng,
-                                  float gVx, float gVy, float gDet, uint32_t row, unsigned long long& nc) {
-    const CameraDev& cam = s.cam;
+FHP_DEV void screenWarpAdjoint(const SceneDev& s, const GeomArgs& a, uint32_t view, TraceStack& st, float psx, float psy,
+                                  Rng rng, float gVx, float gVy, float gDet, uint32_t row, unsigned long long& nc) {
+    const CameraDev cam = geomCamera(s, a, view);
     const f3 o = cam.pos;
     const f3 r0 = camRaw(cam, psx, psy, a.W, a.H);
     const float rl = length(r0);
@@ -396,8 +401,8 @@ FHP_DEVI void gaussianJitter(float ux, float uy, float sigma, float& gx, float& 
 }
 
 // One path: primal replay of tracePaths/tracePath for maxBounces = 0, then the reverse sweep.
-FHP_DEV void geomPath(const SceneDev& s, const GeomArgs& a, uint32_t px, uint32_t py, uint32_t seed, f3 dL, uint32_t row,
-                         TraceStack& st, unsigned long long& nc) {
+FHP_DEV void geomPath(const SceneDev& s, const GeomArgs& a, uint32_t view, uint32_t px, uint32_t py, uint32_t seed, f3 dL,
+                         uint32_t row, TraceStack& st, unsigned long long& nc) {
     Rng rng = rngSeed(px, py, seed / 2);
     for (int i = 0; i < 6; i++) rng.u1();
     const float pcx = float(px) + 0.5f, pcy = float(py) + 0.5f;
@@ -407,10 +412,11 @@ FHP_DEV void geomPath(const SceneDev& s, const GeomArgs& a, uint32_t px, uint32_
     if (a.useAntithetic && seed % 2 == 0) { jx = -jx; jy = -jy; }
     const float psx = pcx + jx, psy = pcy + jy;
     const Rng rngScreen = rng;  // reparameterizeScreenSample takes the generator by value
-    const f3 r0 = camRaw(s.cam, psx, psy, a.W, a.H);
+    const CameraDev cam0 = geomCamera(s, a, view);
+    const f3 r0 = camRaw(cam0, psx, psy, a.W, a.H);
     const float rl = length(r0);
     const f3 w0 = rl > 0.f ? r0 / rl : mk3(0.f);
-    const f3 o = s.cam.pos;
+    const f3 o = cam0.pos;
     HitRec h;
     nc++;
     if (!traceBVH4<false, false>(s, o, w0, 0.f, kFltMax, st, h)) return;
@@ -586,22 +592,32 @@ FHP_DEV void geomPath(const SceneDev& s, const GeomArgs& a, uint32_t px, uint32_
     float gVx = -Qt * (psx - pcx) * inv2s2, gVy = -Qt * (psy - pcy) * inv2s2;
     {
         const f3 q = (gW0 - w0 * dot(w0, gW0)) / rl;
-        gVx += dot(s.cam.U, q) * (2.f / float(a.W));
-        gVy += dot(s.cam.V, q) * (-2.f / float(a.H));
+        const CameraDev cam1 = geomCamera(s, a, view);
+        gVx += dot(cam1.U, q) * (2.f / float(a.W));
+        gVy += dot(cam1.V, q) * (-2.f / float(a.H));
     }
-    if (Qt != 0.f || gVx != 0.f || gVy != 0.f) screenWarpAdjoint(s, a, st, psx, psy, rngScreen, gVx, gVy, Qt, row, nc);
+    if (Qt != 0.f || gVx != 0.f || gVy != 0.f) screenWarpAdjoint(s, a, view, st, psx, psy, rngScreen, gVx, gVy, Qt, row, nc);
 }
 
-// One item of the kernel's grid loop: path id = pixel * spp + sample.
+// One item of the kernel's grid loop: path id = (view * W * H + pixel) * spp + sample.
 FHP_DEVI void geomItem(const SceneDev& s, const GeomArgs& a, uint32_t id, TraceStack& st, unsigned long long& nc) {
-    const uint32_t pix = id / a.spp, k = id % a.spp;
+    uint32_t view = 0, local = id, frame = a.frame;
+    const float* dLdI = a.dLdI;
+    if (a.views) {
+        const uint32_t per = a.W * a.H * a.spp;
+        view = id / per;
+        local = id - view * per;
+        frame = a.views[view].frame;
+        dLdI = a.views[view].dLdI;
+    }
+    const uint32_t pix = local / a.spp, k = local % a.spp;
     f3 dL = mk3(1.f, 0.f, 0.f);  // dLdI null: the BackwardDiffDebug mode's dL/dI = (1, 0, 0)
-    if (a.dLdI) {
-        const float* dl = a.dLdI + 3 * size_t(pix);
+    if (dLdI) {
+        const float* dl = dLdI + 3 * size_t(pix);
         dL = mk3(dl[0], dl[1], dl[2]);
     }
     if (dL.x == 0.f && dL.y == 0.f && dL.z == 0.f) return;
-    geomPath(s, a, pix % a.W, pix / a.W, a.frame + k, dL, jenkins(pix) % a.hashSize, st, nc);
+    geomPath(s, a, view, pix % a.W, pix / a.W, frame + k, dL, jenkins(pix) % a.hashSize, st, nc);
 }
 
 }  // namespace

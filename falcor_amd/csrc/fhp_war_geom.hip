@@ -15,7 +15,7 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(kGeomWavesPer
                                                                                                      TraceScratch ts) {
     __shared__ uint32_t lds[D * B];
     TraceStack st = makeStack<B, D>(lds, ts, blockIdx.x, threadIdx.x);
-    const uint32_t n = a.W * a.H * a.spp;
+    const uint32_t n = a.W * a.H * a.spp * (a.views ? a.nViews : 1u);
     const uint32_t G = gridDim.x * B;
     unsigned long long nc = 0;
     for (uint32_t id = blockIdx.x * B + threadIdx.x; id < n; id += G) geomItem(s, a, id, st, nc);
@@ -25,7 +25,7 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(kGeomWavesPer
 }  // namespace
 
 hipError_t launchWarGeom(const SceneDev& s, const GeomArgs& a, const TraceScratch& ts, hipStream_t st, KernelTimer* tm) {
-    const uint32_t n = a.W * a.H * a.spp;
+    const uint32_t n = a.W * a.H * a.spp * (a.views ? a.nViews : 1u);
     if (tm) tm->begin(KK_WAR_ADJOINT, st);
     hipError_t e = withTraceConfig(ts, [&](auto Bc, auto Dc) {
         constexpr int B = decltype(Bc)::value, D = decltype(Dc)::value;

@@ -20,7 +20,7 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(kWarWavesPerE
                                                                                                  TraceScratch ts) {
     __shared__ uint32_t lds[D * B];
     TraceStack st = makeStack<B, D>(lds, ts, blockIdx.x, threadIdx.x);
-    const uint32_t n = a.W * a.H * a.spp;
+    const uint32_t n = a.W * a.H * a.spp * (a.views ? a.nViews : 1u);
     const uint32_t G = gridDim.x * B;
     unsigned long long nc = 0, nv = 0;
     for (uint32_t id = blockIdx.x * B + threadIdx.x; id < n; id += G) warItem<ADJ, FROZEN, TRANS>(s, a, id, st, nc, nv);
@@ -35,7 +35,7 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(kWarWavesPerE
                                                                                                       TraceScratch ts) {
     __shared__ uint32_t lds[D * B];
     TraceStack st = makeStack<B, D>(lds, ts, blockIdx.x, threadIdx.x);
-    const uint32_t n = a.W * a.H * a.spp;
+    const uint32_t n = a.W * a.H * a.spp * (a.views ? a.nViews : 1u);
     const uint32_t G = gridDim.x * B;
     unsigned long long nc = 0, nv = 0;
     for (uint32_t id = blockIdx.x * B + threadIdx.x; id < n; id += G) warDebugItem<FROZEN, TRANS>(s, a, id, st, nc, nv);

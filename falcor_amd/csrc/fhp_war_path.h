@@ -370,9 +370,9 @@ FHP_DEV void flushGrad(const SceneDev& s, const WarArgs& a, uint32_t hashRow, co
 // DEBUG (with ADJ): instead of accumulating the gradients, each vertex adds its debug projection
 // to *dproj.
 template <bool ADJ, bool FROZEN, bool DEBUG = false, bool TRANS = false>
-FHP_DEV f3 warPath(const SceneDev& s, const WarArgs& a, uint32_t px, uint32_t py, uint32_t frameSeed, TraceStack& st,
-                   f3 Ltot, f3 dL, uint32_t hashRow, unsigned long long& nc, unsigned long long& nv,
-                   float* dproj = nullptr) {
+FHP_DEV f3 warPath(const SceneDev& s, const WarArgs& a, uint32_t view, uint32_t px, uint32_t py,
+                   uint32_t frameSeed, TraceStack& st, f3 Ltot, f3 dL, uint32_t hashRow, unsigned long long& nc,
+                   unsigned long long& nv, float* dproj = nullptr) {
     Rng rng = rngSeed(px, py, frameSeed / 2);
     for (int i = 0; i < 6; i++) rng.u1();
     const float pfx = float(px) + 0.5f, pfy = float(py) + 0.5f;
@@ -385,13 +385,14 @@ FHP_DEV f3 warPath(const SceneDev& s, const WarArgs& a, uint32_t px, uint32_t py
     uint32_t len = 0;
     f3 prevN = mk3(0.f);
     float prevPdf = 0.f;
-    // computeCameraRayDirection (DiffSceneQuery.slang:252-261)
+    // computeCameraRayDirection (DiffSceneQuery.slang:252-261), with the view's camera.
+    const CameraDev cam = a.views ? a.views[view].cam : s.cam;
     const float p_x = sx / float(a.W), p_y = sy / float(a.H);
     const float ndc_x = 2.f * p_x + -1.f, ndc_y = -2.f * p_y + 1.f;
-    f3 dir = s.cam.U * ndc_x + s.cam.V * ndc_y + s.cam.W;
+    f3 dir = cam.U * ndc_x + cam.V * ndc_y + cam.W;
     const float dl = length(dir);
     dir = dl > 0.f ? dir / dl : mk3(0.f);
-    f3 rayO = s.cam.pos, rayD = dir;
+    f3 rayO = cam.pos, rayD = dir;
     HitRec h;
     nc++;
     if (!traceBVH4<false, false>(s, rayO, rayD, 0.f, kFltMax, st, h)) return radiance;
@@ -531,26 +532,39 @@ FHP_DEV f3 warPath(const SceneDev& s, const WarArgs& a, uint32_t px, uint32_t py
 // One item of the kernels' grid loop (path id = pixel * spp + sample): the primal writes the
 // path's radiance; the adjoint traces the path, then replays it from the same seed (one launch:
 // splitting the two passes into launches measured slower).
+// Path id -> (view, path within the view); the view's frame seed base and dL/dI.
+struct WarViewRef {
+    uint32_t view, frame, local;
+    const float* dLdI;
+};
+template <typename Args> FHP_DEVI WarViewRef warView(const Args& a, uint32_t id) {
+    if (!a.views) return WarViewRef{0u, a.frame, id, a.dLdI};
+    const uint32_t per = a.W * a.H * a.spp, v = id / per;
+    const WarView& w = a.views[v];
+    return WarViewRef{v, w.frame, id - v * per, w.dLdI};
+}
+
 template <bool ADJ, bool FROZEN, bool TRANS = false>
 FHP_DEVI void warItem(const SceneDev& s, const WarArgs& a, uint32_t id, TraceStack& st, unsigned long long& nc,
                       unsigned long long& nv) {
-    const uint32_t pix = id / a.spp, k = id % a.spp;
+    const WarViewRef vr = warView(a, id);
+    const uint32_t pix = vr.local / a.spp, k = vr.local % a.spp;
     const uint32_t px = pix % a.W, py = pix / a.W;
-    const uint32_t seed = a.frame + k;
+    const uint32_t seed = vr.frame + k;
     if (!ADJ) {
-        f3 L = warPath<false, FROZEN>(s, a, px, py, seed, st, mk3(0.f), mk3(0.f), 0, nc, nv);
+        f3 L = warPath<false, FROZEN>(s, a, vr.view, px, py, seed, st, mk3(0.f), mk3(0.f), 0, nc, nv);
         a.out[id] = make_float4(L.x, L.y, L.z, 0.f);
     } else {
         // dLdI null: the BackwardDiffDebug mode's dL/dI = (1, 0, 0) at every pixel.
         f3 dL = mk3(1.f, 0.f, 0.f);
-        if (a.dLdI) {
-            const float* dl = a.dLdI + 3 * size_t(pix);
+        if (vr.dLdI) {
+            const float* dl = vr.dLdI + 3 * size_t(pix);
             dL = mk3(dl[0], dl[1], dl[2]);
         }
         if (dL.x == 0.f && dL.y == 0.f && dL.z == 0.f) return;
         unsigned long long c0 = 0, c1 = 0;
-        f3 L = warPath<false, FROZEN>(s, a, px, py, seed, st, mk3(0.f), mk3(0.f), 0, c0, c1);
-        warPath<true, FROZEN, false, TRANS>(s, a, px, py, seed, st, L, dL, jenkins(pix) % a.hashSize, nc, nv);
+        f3 L = warPath<false, FROZEN>(s, a, vr.view, px, py, seed, st, mk3(0.f), mk3(0.f), 0, c0, c1);
+        warPath<true, FROZEN, false, TRANS>(s, a, vr.view, px, py, seed, st, L, dL, jenkins(pix) % a.hashSize, nc, nv);
         nc += c0;
         nv += c1;
     }
@@ -567,9 +581,9 @@ FHP_DEVI void warDebugItem(const SceneDev& s, const WarArgs& a, uint32_t id, Tra
     const uint32_t px = pix % a.W, py = pix / a.W;
     const uint32_t seed = a.frame + k;
     unsigned long long c0 = 0, c1 = 0;
-    const f3 L = warPath<false, FROZEN>(s, a, px, py, seed, st, mk3(0.f), mk3(0.f), 0, nc, nv);
+    const f3 L = warPath<false, FROZEN>(s, a, 0u, px, py, seed, st, mk3(0.f), mk3(0.f), 0, nc, nv);
     float d = 0.f;
-    warPath<true, FROZEN, true, TRANS>(s, a, px, py, seed, st, L, mk3(1.f, 0.f, 0.f), 0, c0, c1, &d);
+    warPath<true, FROZEN, true, TRANS>(s, a, 0u, px, py, seed, st, L, mk3(1.f, 0.f, 0.f), 0, c0, c1, &d);
     a.tmp[id] = make_float4(L.x, L.y, L.z, d);
 }
 

@@ -458,6 +458,34 @@ class _NativePass(RenderPass):
         h, w = out.shape[0], out.shape[1]
         self.render(w, h, 1, out=out)
 
+    def render_batch(self, views: list[dict], stats: bool = False):
+        """Renders several views in one launch sequence (fhp_pass_execute_batch). Each view is a dict
+        with scene, width, height and optional spp (1), frame (0), camera (a scene.Camera, default
+        the scene's) and out ([H, W, 4] float32 tensor). Returns the output tensors (and stats).
+        WARDiffPathTracer views share one scene, width, height and spp."""
+        torch = _torch()
+        dev = _device()
+        arr = (_lib.View * len(views))()
+        outs, cams = [], []
+        for i, v in enumerate(views):
+            w, h = int(v["width"]), int(v["height"])
+            out = v.get("out")
+            if out is None:
+                out = torch.empty((h, w, 4), dtype=torch.float32, device=dev)
+            outs.append(out)
+            cam = v.get("camera")
+            if cam is not None:
+                cams.append(np.ascontiguousarray(cam.pack(), np.float32))
+            arr[i].scene = v["scene"].handle
+            arr[i].h_camera = cams[-1].ctypes.data if cam is not None else None
+            arr[i].width, arr[i].height = w, h
+            arr[i].spp, arr[i].frame = int(v.get("spp", 1)), int(v.get("frame", 0))
+            arr[i].d_color = out.data_ptr()
+        st = np.zeros(6, np.uint64) if stats else None
+        _lib.check(_lib.load().fhp_pass_execute_batch(self.handle, len(views), arr, st.ctypes.data if stats else None,
+                                                      _lib.stream_ptr()))
+        return (outs, st) if stats else outs
+
 
 class MinimalPathTracer(_NativePass):
     """MinimalPathTracer (MinimalPathTracer.cpp). Properties: maxBounces, computeDirect,
@@ -496,34 +524,6 @@ class MinimalPathTracer(_NativePass):
             if k == "viewW" and (t.dim() != 3 or t.shape[2] != 4 or t.dtype != _torch().float32):
                 raise ValueError("viewW must be [H, W, 4] float32")
         return super().render(width, height, spp, frame=frame, out=out, stats=stats)
-
-    def render_batch(self, views: list[dict], stats: bool = False):
-        """Renders several views in one launch sequence (fhp_pass_execute_batch). Each view is a dict
-        with scene, width, height and optional spp (1), frame (0), camera (a scene.Camera, default
-        the scene's) and out ([H, W, 4] float32 tensor). Returns the output tensors (and stats)."""
-        torch = _torch()
-        dev = _device()
-        arr = (_lib.View * len(views))()
-        outs, cams = [], []
-        for i, v in enumerate(views):
-            w, h = int(v["width"]), int(v["height"])
-            out = v.get("out")
-            if out is None:
-                out = torch.empty((h, w, 4), dtype=torch.float32, device=dev)
-            outs.append(out)
-            cam = v.get("camera")
-            if cam is not None:
-                cams.append(np.ascontiguousarray(cam.pack(), np.float32))
-            arr[i].scene = v["scene"].handle
-            arr[i].h_camera = cams[-1].ctypes.data if cam is not None else None
-            arr[i].width, arr[i].height = w, h
-            arr[i].spp, arr[i].frame = int(v.get("spp", 1)), int(v.get("frame", 0))
-            arr[i].d_color = out.data_ptr()
-        st = np.zeros(6, np.uint64) if stats else None
-        _lib.check(_lib.load().fhp_pass_execute_batch(self.handle, len(views), arr, st.ctypes.data if stats else None,
-                                                      _lib.stream_ptr()))
-        return (outs, st) if stats else outs
-
 
 class VBufferRT(_NativePass):
     """VBufferRT (Source/RenderPasses/GBuffer/VBuffer/VBufferRT.cpp): per-pixel primary hits as
@@ -637,6 +637,39 @@ class WARDiffPathTracer(_NativePass):
                                                  _lib.stream_ptr()))
         if frame is None:
             self.frame_count += 1
+        return st
+
+    def backward_batch(self, views: list[dict], width: int, height: int, spp: int = 1, stats: bool = False):
+        """backward() for several views of the scene in one launch sequence (fhp_pass_backward_batch):
+        each view is a dict with dL_dI ([H, W, 3] float32), optional frame (0) and camera (a
+        scene.Camera, default the scene's). All views accumulate into scene_gradients."""
+        debug = self.get_property("diffMode") == 3
+        if self.scene_gradients is None:
+            raise _lib.FhpError("backward needs scene_gradients")
+        if self.get_property("useWAR") != 0 and self.scene_gradients.mesh_id != self.mesh_to_optimize:
+            self.scene_gradients.enable_mesh(self.mesh_to_optimize)
+        torch = _torch()
+        arr = (_lib.DiffView * len(views))()
+        keep, cams = [], []
+        for i, v in enumerate(views):
+            d = v.get("dL_dI")
+            if d is None and not debug:
+                raise _lib.FhpError("backward needs dL_dI for every view")
+            if d is not None:
+                d = d.detach().to(device=_device(), dtype=torch.float32).contiguous()
+                if d.numel() != width * height * 3:
+                    raise ValueError("dL_dI must hold width * height * 3 floats")
+                keep.append(d)
+            cam = v.get("camera")
+            if cam is not None:
+                cams.append(np.ascontiguousarray(cam.pack(), np.float32))
+            arr[i].h_camera = cams[-1].ctypes.data if cam is not None else None
+            arr[i].frame = int(v.get("frame", 0))
+            arr[i].d_dLdI = d.data_ptr() if d is not None else None
+        st = np.zeros(6, np.uint64) if stats else None
+        _lib.check(_lib.load().fhp_pass_backward_batch(self.handle, self.scene.handle, len(views), arr, width, height, spp,
+                                                       self.scene_gradients.handle, st.ctypes.data if stats else None,
+                                                       _lib.stream_ptr()))
         return st
 
 

@@ -473,6 +473,21 @@ def displaced_sphere_in_box(seed: int = 0, radius: float = 0.12, segments=(100, 
     return sb.build()
 
 
+def orbit_cameras(n: int = 32, center=(0.0, 0.2, 0.0), radius: float = 0.22, heights=(-0.08, 0.08),
+                  fov_y: float = math.radians(70.0), aspect: float = 1.0) -> list:
+    """n pinhole cameras on rings around `center` (len(heights) rings of n / len(heights) cameras
+    at the given heights above it), each looking at the centre: the views of the batched
+    vertex-gradient workload around the sphere of displaced_sphere_in_box, inside the box."""
+    per = n // len(heights)
+    cams = []
+    for r, dy in enumerate(heights):
+        for k in range(per):
+            a = 2.0 * math.pi * (k + 0.5 * r) / per
+            pos = (center[0] + radius * math.sin(a), center[1] + dy, center[2] + radius * math.cos(a))
+            cams.append(Camera(pos, tuple(center), (0.0, 1.0, 0.0), fov_y, aspect))
+    return cams
+
+
 def sphere_field(seed: int = 1, num_spheres: int = 1024, num_emissive: int = 4096, box: float = 100.0,
                  subdivisions: int = 3, aspect: float = 16 / 9, camera: Camera | None = None) -> SceneData:
     """Icosphere field: log-normal(0, 0.5) scales, Haar rotations, uniform positions in [0, box]^3,

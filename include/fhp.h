@@ -256,9 +256,12 @@ int fhp_pass_execute(FhpPass pass, FhpScene scene, uint32_t width, uint32_t heig
  * the same device (any streams) race and must be ordered by the caller. */
 int fhp_pass_execute_noalloc(FhpPass pass, FhpScene scene, uint32_t width, uint32_t height, uint32_t spp, uint32_t frame,
                              float* d_color, FhpStream stream);
-/* A batch of views rendered in one launch sequence (MinimalPathTracer): every view's paths share
+/* A batch of views rendered in one launch sequence. MinimalPathTracer: every view's paths share
  * the wavefront queues and the persistent traversal grid, each ray traversing its own view's
- * scene. h_camera is 14 floats as in FhpSceneDesc, or NULL for the scene's camera; each view is
+ * scene. WARDiffPathTracer: the views share one scene, width, height and spp (frames and cameras
+ * differ) and run in one primal launch (Primal and BackwardDiff modes, and ForwardDiffDebug
+ * without a debug variable; BackwardDiffDebug clears the outputs; gradient images are refused).
+ * h_camera is 14 floats as in FhpSceneDesc, or NULL for the scene's camera; each view is
  * resolved into its d_color (height * width * 4 floats) exactly as fhp_pass_execute would render
  * it. No counterpart in the reference (one graph execution per view there,
  * RenderGraph::execute, Source/Falcor/RenderGraph/RenderGraph.cpp). Synchronises the stream
@@ -312,6 +315,22 @@ int fhp_grads_copy(FhpGrads grads, int which, float* d_dst, uint64_t num_floats,
  * the host loop of render_grad (scripts/inv-rendering/common.py:90-114). */
 int fhp_pass_backward(FhpPass pass, FhpScene scene, uint32_t width, uint32_t height, uint32_t spp, uint32_t frame,
                       const float* d_dLdI, FhpGrads grads, uint64_t* h_stats, FhpStream stream);
+/* fhp_pass_backward for a batch of views of one scene and one frame shape in one launch
+ * sequence (the material adjoint and, with useWAR, the vertex gradients): view v contributes the
+ * gradient of its own loss with dL/dI d_dLdI (height * width * 3 floats; ignored in
+ * BackwardDiffDebug mode), seeds frame * spp + k (or fixedSeed) and camera h_camera (14 floats as
+ * in FhpSceneDesc, or NULL for the scene's camera); all views accumulate into the same gradient
+ * rows. Equal to the sum of fhp_pass_backward over the views up to the order of the float
+ * atomics. No counterpart in the reference (one runBackward per view,
+ * WARDiffPathTracer.cpp:221-...). Synchronises the stream before its launches (the view table is
+ * staged from host memory). */
+typedef struct FhpDiffView {
+    const float* h_camera;
+    uint32_t frame;
+    const float* d_dLdI;
+} FhpDiffView;
+int fhp_pass_backward_batch(FhpPass pass, FhpScene scene, uint32_t num_views, const FhpDiffView* views, uint32_t width,
+                            uint32_t height, uint32_t spp, FhpGrads grads, uint64_t* h_stats, FhpStream stream);
 
 /* AccumulatePass: a pass of kind "AccumulatePass" (fhp_pass_create) with the reference's
  * properties enabled (1), autoReset (1), precisionMode (0 Double, 1 Single [default],

@@ -1,6 +1,10 @@
-"""Batched MinimalPathTracer (fhp_pass_execute_batch): views of different scenes, frame shapes,
-sample counts and cameras in one launch sequence. Each view's path state, seeds and traversal are
-those of a single-view frame, so every view must equal its single-view render bit for bit."""
+"""Batched views (fhp_pass_execute_batch, fhp_pass_backward_batch).
+
+MinimalPathTracer: views of different scenes, frame shapes, sample counts and cameras in one
+launch sequence. Each view's path state, seeds and traversal are those of a single-view frame, so
+every view must equal its single-view render bit for bit. WARDiffPathTracer: views of one scene
+and frame shape with their own cameras and frames; the primal equals single-view renders bit for
+bit and the batched adjoint equals the sum of single-view adjoints."""
 import math
 
 import numpy as np
@@ -54,6 +58,80 @@ def test_batch_rejects_bad_views(gpu):
         p.render_batch([dict(scene=box, width=0, height=8)])
     with pytest.raises(_lib.FhpError, match="frame shape"):
         p.render_batch([dict(scene=box, width=8, height=8, spp=0)])
+    from falcor_amd.render_graph import AccumulatePass
+    with pytest.raises(_lib.FhpError, match="MinimalPathTracer and WARDiffPathTracer"):
+        MinimalPathTracer.render_batch(AccumulatePass({}), [dict(scene=box, width=8, height=8)])
     w = WARDiffPathTracer({})
-    with pytest.raises(_lib.FhpError, match="MinimalPathTracer"):
-        MinimalPathTracer.render_batch(w, [dict(scene=box, width=8, height=8)])
+    with pytest.raises(_lib.FhpError, match="share one scene"):
+        w.render_batch([dict(scene=box, width=8, height=8), dict(scene=Scene(S.closed_box(1)), width=8, height=8)])
+
+
+def _war_setup(props, segments=(24, 12)):
+    from falcor_amd.render_graph import Scene, WARDiffPathTracer
+    data = S.displaced_sphere_in_box(2, radius=0.12, segments=segments, sigma=0.05)
+    sc = Scene(data)
+    p = WARDiffPathTracer(props)
+    p.set_scene(sc)
+    return data, sc, p
+
+
+def test_war_batch_views_equal_single_view_renders(gpu):
+    """Batched WARDiffPathTracer primal (one launch over every view): each view equals the
+    single-view render with the same camera and frame bit for bit."""
+    import torch
+    cams = S.orbit_cameras(4)
+    data, sc, p = _war_setup({"maxBounces": 2})
+    W = H = 32
+    views = [dict(scene=sc, width=W, height=H, spp=3, frame=7 + i, camera=c) for i, c in enumerate(cams)]
+    outs = p.render_batch(views)
+    torch.cuda.synchronize()
+    for i, c in enumerate(cams):
+        sc.set_camera(c)
+        ref = p.render(W, H, 3, frame=7 + i)
+        torch.cuda.synchronize()
+        assert torch.equal(outs[i], ref), f"view {i}"
+    with pytest.raises(Exception, match="share one scene"):
+        p.render_batch([dict(scene=sc, width=W, height=H, spp=3), dict(scene=sc, width=W, height=H, spp=2)])
+
+
+@pytest.mark.parametrize("war", [False, True])
+def test_war_backward_batch_equals_sum_of_views(gpu, war):
+    """fhp_pass_backward_batch over four views equals the sum of fhp_pass_backward over the views
+    (material gradients; with useWAR also vertex positions and normals) to 1e-4 of the largest
+    entry (float atomics in another order)."""
+    import torch
+    from falcor_amd.render_graph import SceneGradients
+    props = {"maxBounces": 0 if war else 2, "diffMode": "BackwardDiff", "useWAR": war, "auxSampleCount": 16,
+             "Log10vMFConcentration": 3, "Log10vMFConcentrationScreen": 3}
+    data, sc, p = _war_setup(props)
+    mesh = data.mesh_names.index("Sphere")
+    W = H = 24
+    spp = 2
+    cams = S.orbit_cameras(4)
+    rng = np.random.default_rng(4)
+    dl = [torch.from_numpy(rng.uniform(0.5, 1.5, (H, W, 3)).astype(np.float32)).cuda() for _ in cams]
+
+    def grads(batched):
+        g = SceneGradients(sc, hash_size=8, texture_grads=False, mesh_id=mesh if war else None)
+        p.scene_gradients = g
+        if war:
+            p.set_mesh_to_optimize(mesh)
+        g.clear_all_grads()
+        if batched:
+            p.backward_batch([dict(dL_dI=d, frame=3 + i, camera=c) for i, (d, c) in enumerate(zip(dl, cams))], W, H, spp)
+        else:
+            for i, (d, c) in enumerate(zip(dl, cams)):
+                sc.set_camera(c)
+                p.dL_dI = d
+                p.backward(W, H, spp, frame=3 + i)
+        g.aggregate_all_grads()
+        out = [g.get_grads_buffer(g.MATERIAL).cpu().numpy()]
+        if war:
+            out += [g.get_grads_buffer("MeshPosition").cpu().numpy(), g.get_grads_buffer("MeshNormal").cpu().numpy()]
+        return out
+
+    a, b = grads(True), grads(False)
+    assert sc.trace_errors() == 0
+    for x, y in zip(a, b):
+        assert np.isfinite(x).all() and np.abs(y).max() > 0
+        np.testing.assert_allclose(x, y, rtol=0, atol=1e-4 * np.abs(y).max())
