@@ -23,10 +23,15 @@ Also reported on the same JSON line:
     textured spheres, 512x512 RGB fp32 albedo textures, 512x512 image, 16 spp forward +
     16 spp adjoint), batched vs the plain per-frame loop, at maxBounces 1 (and batched at 4),
     and the gradient relative error against central finite differences (eps 1e-3) on 256 texels;
+  * geom: the batched differentiable-geometry workload (20,000-triangle displaced sphere in the
+    closed box, 32 views of 256x256 at 8 spp, 16 auxiliary rays, kappa 1e3): ms per iteration of
+    the displacement-recovery loop (one batched primal and one batched adjoint), the gradient's
+    relative error against central differences, and the displacement error after the iterations;
   * cpu_baseline: the oracle (C++ restatement, all host cores) on the same forward workload.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--skip-cpu] [--skip-diff] [--skip-batch]
-                       [--only forward|batch|diff] [--tune [--tune-out F]]
+                       [--skip-geom] [--geom-iters N] [--only forward|batch|diff|war-iter|geom]
+                       [--tune [--tune-out F]]
        python bench.py --copy-calibration   (1 GiB copies: scale for rocprofv3 FETCH_SIZE/WRITE_SIZE)
 """
 from __future__ import annotations
@@ -357,6 +362,47 @@ def diff_bench(iters: int = 3, fd_texels: int = 256, bounces=(1, 4), plain_loop:
                 grad_rel_err_fd=rel, fd_texels=int(pick.size), kernel_ms=kt)
 
 
+def geom_bench(iters: int = 20):
+    """Batched differentiable geometry: a 20,000-triangle sphere (seeded normal displacement,
+    sigma 0.05 radius) in the closed box, 32 views from inside the box (orbit cameras), 256x256,
+    8 spp forward and 8 spp adjoint, vertex gradients by warped-area reparameterization with 16
+    auxiliary rays (vMF kappa 1e3): one batched primal launch and one batched adjoint per
+    iteration of the displacement-recovery loop (falcor_amd.shape_optimization). Reports ms per
+    iteration, the adjoint's directional derivative against central differences of the batched
+    primal at the initial shape (uniform inflation and the remaining-error direction, with the
+    reference's NEE term and with the held NEE direction), and the relative displacement error
+    after `iters` iterations."""
+    import torch
+    from falcor_amd import scene as S
+    from falcor_amd.shape_optimization import DisplacementRecovery
+
+    tgt = S.displaced_sphere_in_box(0, radius=0.12, segments=(100, 100), sigma=0.05, front_wall=True)
+    ini = S.displaced_sphere_in_box(0, radius=0.12, segments=(100, 100), sigma=0.0, front_wall=True)
+    out = {"views": 32, "width": 256, "height": 256, "spp": 8, "aux": 16, "kappa": 1000.0,
+           "sphere_triangles": 20000, "scene_triangles": int(np.asarray(ini.indices).size // 3)}
+    fd = {}
+    for name, props in (("reference_nee", {}), ("held_nee", {"useSolidAngleNEEWarp": 1})):
+        rec = DisplacementRecovery(tgt, ini, props=props, lr=5e-4, lam=20.0, lr_decay=0.99)
+        for dname in ("uniform", "true"):
+            ad, f, rel = rec.gradient_fd_check(direction=dname)
+            fd[f"{name}_{dname}"] = {"adjoint": ad, "fd": f, "rel": rel}
+        if name == "held_nee":
+            keep = rec
+    rec = keep
+    rec.step()  # warm-up (allocations)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(iters):
+        rec.step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t) / iters * 1e3
+    out.update(ms_per_iter=ms, iters=iters + 1, rmse_start=1.0, rmse_end=rec.rmse(rec.displacement()),
+               loss_start=rec.history[0]["loss"], loss_end=rec.history[-1]["loss"], fd=fd)
+    log(f"geom: {ms:.1f} ms/iter (32 views 256x256 8 spp fwd + adjoint), displacement error 1.0 -> "
+        f"{out['rmse_end']:.3f} after {iters + 1} iterations; fd {json.dumps(fd)}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -369,7 +415,10 @@ def main():
     ap.add_argument("--tune-out", default=None, help="with --tune: JSON file to record the chosen launch in")
     ap.add_argument("--skip-batch", action="store_true",
                     help="skip the batched forward workload (64 views over 16 scenes)")
-    ap.add_argument("--only", choices=("forward", "batch", "diff", "war-iter"), default=None,
+    ap.add_argument("--skip-geom", action="store_true",
+                    help="skip the batched differentiable-geometry workload (32 views, vertex gradients)")
+    ap.add_argument("--geom-iters", type=int, default=20, help="displacement-recovery iterations of the geometry leg")
+    ap.add_argument("--only", choices=("forward", "batch", "diff", "war-iter", "geom"), default=None,
                     help="run one leg only and print its result (profiling runs); no JSON line")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy 3 times (scale for HBM byte counters)")
@@ -395,7 +444,8 @@ def main():
     if args.only:
         r = {"forward": lambda: forward_bench(args.steps, args.warmup, args.tune, args.tune_out),
              "batch": batch_bench, "diff": diff_bench,
-             "war-iter": lambda: diff_bench(iters=1, fd_texels=0, bounces=(1,), plain_loop=False)}[args.only]()
+             "war-iter": lambda: diff_bench(iters=1, fd_texels=0, bounces=(1,), plain_loop=False),
+             "geom": lambda: geom_bench(args.geom_iters)}[args.only]()
         log(f"{args.only}: " + json.dumps({k: v for k, v in r.items() if isinstance(v, (int, float, str))}))
         return
 
@@ -435,6 +485,10 @@ def main():
         line["batch"] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in batch_bench().items()}
     if not args.skip_diff:
         line["diff"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in diff_bench().items()}
+    if not args.skip_geom:
+        g = geom_bench(args.geom_iters)
+        line["geom"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in g.items() if k != "fd"}
+        line["geom"]["fd"] = {k: {kk: round(vv, 6) for kk, vv in v.items()} for k, v in g["fd"].items()}
     if not args.skip_cpu:
         line["cpu_baseline"] = cpu_baseline(fwd["data"], fwd["W"], fwd["H"], fwd["spp"], fwd["B"])
     print(json.dumps(line), flush=True)

@@ -435,11 +435,13 @@ def vertex_normals(positions: np.ndarray, indices: np.ndarray) -> np.ndarray:
 
 
 def displaced_sphere_in_box(seed: int = 0, radius: float = 0.12, segments=(100, 100), sigma: float = 0.05,
-                            center=(0.0, 0.2, 0.0), aspect: float = 1.0) -> SceneData:
+                            center=(0.0, 0.2, 0.0), aspect: float = 1.0, front_wall: bool = False) -> SceneData:
     """A Lambertian UV sphere (createSphere(r, 100, 100): 10,201 vertices, 20,000 triangles) inside
     the closed box's walls and ceiling light. Vertices are displaced along the normal by a smooth
     seeded field with standard deviation sigma * radius (sigma = 0 gives the plain sphere); vertex
-    normals are recomputed from the displaced surface. The sphere is the mesh named "Sphere"."""
+    normals are recomputed from the displaced surface. The sphere is the mesh named "Sphere".
+    front_wall adds a sixth wall closing the box's open side, so that views from inside the box
+    (orbit_cameras) see walls in every direction and no ray leaves the scene."""
     rng = np.random.default_rng(seed)
     sb = SceneBuilder()
     quad = TriangleMesh.create_quad()
@@ -453,6 +455,9 @@ def displaced_sphere_in_box(seed: int = 0, radius: float = 0.12, segments=(100, 
     for name, t in walls:
         m = sb.add_material(pbrt_diffuse_material(rng.uniform(0.1, 0.8, 3)))
         sb.add_mesh_instance(quad, m, t, name)
+    if front_wall:
+        sb.add_mesh_instance(quad, sb.add_material(pbrt_diffuse_material((0.5, 0.5, 0.5))),
+                             Transform.deg((0.55, 1.0, 0.56), (0, 0.275, 0.275), (-90, 0, 0)), "Front Wall")
     light = sb.add_material(standard_material(base=(0, 0, 0), roughness=0.5, emissive=(17, 17, 17)))
     sb.add_mesh_instance(quad, light, Transform.deg((0.13, 0.13, 0.13), (0, 0.549, 0), (180, 0, 0)), "Light")
     sph = TriangleMesh.create_sphere(1.0, segments[0], segments[1])

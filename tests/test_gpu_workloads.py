@@ -133,3 +133,59 @@ def test_differentiable_materials_workload_matches_oracle_fd(gpu, oracle_mod):
     rel = float(np.linalg.norm(tex[pick] - fd) / np.linalg.norm(fd))
     print(f"textured spheres: texel grad rel err vs oracle FD on {pick.size} texels = {rel:.2e}", flush=True)
     assert rel < 1e-2
+
+
+def test_batched_geometry_workload(gpu):
+    """Batched differentiable geometry at its full size: the 20,000-triangle displaced sphere in the
+    closed box, 32 views from inside the box at 256x256, 8 spp, 16 auxiliary rays at kappa 1e3.
+
+    * The batched adjoint (one launch over the 32 views) equals the sum of the 32 single-view
+      adjoints: vertex position and normal gradients to 1e-4 of their largest entry.
+    * With the held NEE direction the adjoint's derivative along a uniform inflation of the sphere
+      matches central differences of the batched primal (common random numbers, eps 5e-4) within
+      10 %.
+    * 30 iterations of the displacement-recovery loop bring the relative displacement error from 1
+      below 0.5, and the loss decreases.
+    """
+    import torch
+    from falcor_amd.render_graph import SceneGradients
+    from falcor_amd.shape_optimization import DisplacementRecovery
+    tgt = S.displaced_sphere_in_box(0, radius=0.12, segments=(100, 100), sigma=0.05, front_wall=True)
+    ini = S.displaced_sphere_in_box(0, radius=0.12, segments=(100, 100), sigma=0.0, front_wall=True)
+    m = ini.mesh_names.index("Sphere")
+    off = ini.mesh_vertex_offsets
+    idx = np.asarray(ini.indices).reshape(-1, 3)
+    assert int(((idx >= off[m]) & (idx < off[m + 1])).all(1).sum()) == 20000
+    rec = DisplacementRecovery(tgt, ini, props={"useSolidAngleNEEWarp": 1}, lr=5e-4, lam=20.0)
+    r = rec.renderer
+    assert len(r.cameras) == 32 and (r.width, r.height, r.spp_grad) == (256, 256, 8)
+    rng = np.random.default_rng(2)
+    dl = torch.from_numpy(rng.uniform(0.0, 1e-6, (32, 256, 256, 3)).astype(np.float32)).cuda()
+    gp_b, gn_b = r.backward(dl, frame=9)
+    g = r.grads
+    sc = rec.scene
+    acc_p = torch.zeros_like(gp_b)
+    acc_n = torch.zeros_like(gn_b)
+    p = r.pass_
+    for i, c in enumerate(r.cameras):
+        g.clear_all_grads()
+        sc.set_camera(c)
+        p.dL_dI = dl[i]
+        p.backward(256, 256, 8, frame=9 * 32 + i)
+        g.aggregate_all_grads()
+        acc_p += g.get_grads_buffer(g.MESH_POSITION).view(-1, 3)
+        acc_n += g.get_grads_buffer(g.MESH_NORMAL).view(-1, 3)
+    assert sc.trace_errors() == 0
+    for a, b in ((gp_b, acc_p), (gn_b, acc_n)):
+        a, b = a.cpu().numpy(), b.cpu().numpy()
+        np.testing.assert_allclose(a, b, rtol=0, atol=1e-4 * np.abs(b).max())
+    ad, fd, rel = rec.gradient_fd_check(direction="uniform")
+    print(f"batched geometry: uniform inflation adjoint {ad:.5e} fd {fd:.5e} rel {rel:.3f}", flush=True)
+    assert rel <= 0.1
+    for _ in range(30):
+        rec.step()
+    err = rec.rmse(rec.displacement())
+    print(f"batched geometry: loss {rec.history[0]['loss']:.4e} -> {rec.history[-1]['loss']:.4e}, "
+          f"displacement error 1 -> {err:.3f}", flush=True)
+    assert rec.history[-1]["loss"] < rec.history[0]["loss"]
+    assert err < 0.5
