@@ -22,7 +22,8 @@ Also reported on the same JSON line:
   * diff: ms per forward+adjoint iteration of the differentiable-materials workload (64
     textured spheres, 512x512 RGB fp32 albedo textures, 512x512 image, 16 spp forward +
     16 spp adjoint), batched vs the plain per-frame loop, at maxBounces 1 (and batched at 4),
-    and the gradient relative error against central finite differences (eps 1e-3) on 256 texels;
+    the gradient relative error against central finite differences (eps 1e-3) on 256 texels, and
+    100 Adam iterations of the texture / roughness optimization (loss and errors before and after);
   * geom: the batched differentiable-geometry workload (20,000-triangle displaced sphere in the
     closed box, 32 views of 256x256 at 8 spp, 16 auxiliary rays, kappa 1e3): ms per iteration of
     the displacement-recovery loop (one batched primal and one batched adjoint), the gradient's
@@ -357,9 +358,37 @@ def diff_bench(iters: int = 3, fd_texels: int = 256, bounces=(1, 4), plain_loop:
     fd = np.array(fd)
     rel = float(np.linalg.norm(ad - fd) / max(np.linalg.norm(fd), 1e-30))
     log(f"diff: grad rel-err vs FD on {pick.size} texels = {rel:.2e} ({time.perf_counter() - t:.1f}s)")
-    return dict(ms_per_iter_batched=res["batched_b1"], ms_per_iter_plain=res["plain_b1"],
+    loop = material_loop()
+    return dict(loop=loop, ms_per_iter_batched=res["batched_b1"], ms_per_iter_plain=res["plain_b1"],
                 speedup_batched_vs_plain=res["plain_b1"] / res["batched_b1"], ms_per_iter_batched_b4=res["batched_b4"],
                 grad_rel_err_fd=rel, fd_texels=int(pick.size), kernel_ms=kt)
+
+
+def material_loop(iters: int = 100) -> dict:
+    """The differentiable-materials workload's optimization: `iters` Adam iterations on the 64
+    albedo textures and roughness values (falcor_amd.material_optimization), with the loss and the
+    errors of the texels seen in the first iteration and of the roughness before and after."""
+    import torch
+    from falcor_amd import scene as S
+    from falcor_amd.material_optimization import TextureRecovery
+
+    rec = TextureRecovery(S.textured_spheres(3, target=True), S.textured_spheres(3, target=False))
+    rec.step()
+    seen = rec.seen.clone()
+    e1 = rec.errors(seen)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(iters - 1):
+        rec.step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t) / max(iters - 1, 1) * 1e3
+    e = rec.errors(seen)
+    out = dict(iters=iters, ms_per_iter=ms, loss_start=rec.history[0]["loss"], loss_end=rec.history[-1]["loss"],
+               seen_texels=float(seen.float().mean()), seen_texel_rmse_start=e1["seen_texel_rmse"],
+               seen_texel_rmse_end=e["seen_texel_rmse"], roughness_rmse_start=e1["roughness_rmse"],
+               roughness_rmse_end=e["roughness_rmse"])
+    log(f"diff loop: {json.dumps(out)}")
+    return out
 
 
 def geom_bench(iters: int = 20):
@@ -484,7 +513,9 @@ def main():
     if not args.skip_batch:
         line["batch"] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in batch_bench().items()}
     if not args.skip_diff:
-        line["diff"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in diff_bench().items()}
+        d = diff_bench()
+        line["diff"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in d.items() if k != "loop"}
+        line["diff"]["loop"] = {k: (round(v, 5) if isinstance(v, float) else v) for k, v in d["loop"].items()}
     if not args.skip_geom:
         g = geom_bench(args.geom_iters)
         line["geom"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in g.items() if k != "fd"}

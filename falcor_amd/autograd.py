@@ -11,7 +11,8 @@ scripts/inv-rendering/shape_optimization/diff_render_module.py) and render_prima
 Material parameters use the reference's 20-float layout per material
 (StandardMaterialParamLayout.slang:32-41): baseColor 0-2, metallic 3, roughness 4, ior 5,
 transmissionColor 6-8, diffuseTransmission 9, specularTransmission 10, emissive 11-13,
-emissiveFactor 14. The adjoint differentiates reflection lobes only (no gradients reach 6-10).
+emissiveFactor 14 (the transmission entries 6-10 through the transmission lobes of materials with
+transmission, and 9-10 through the lobe weights of every StandardMaterial).
 """
 from __future__ import annotations
 

@@ -189,3 +189,28 @@ def test_batched_geometry_workload(gpu):
           f"displacement error 1 -> {err:.3f}", flush=True)
     assert rec.history[-1]["loss"] < rec.history[0]["loss"]
     assert err < 0.5
+
+
+def test_differentiable_materials_optimization_loop(gpu):
+    """The differentiable-materials workload's optimization at its full size: 100 Adam iterations
+    (the reference's learning rates) on the 64 albedo textures (512x512 RGB fp32) and roughness
+    values from uniform 0.5, 512x512 image, 16 spp forward + 16 spp adjoint, maxBounces 1, L2 loss
+    against a 64-spp target. A 512x512 image sees each object's texture only where the object faces
+    the camera, at about 16 texels per pixel, so the texels are judged where they received a
+    gradient: their error, the roughness error and the loss all decrease (measured on MI355X: loss
+    -22 %, roughness error -39 %, error of the seen texels -4 %: a texel is covered by a few noisy
+    16-spp pixels per iteration)."""
+    from falcor_amd.material_optimization import TextureRecovery
+    rec = TextureRecovery(S.textured_spheres(3, target=True), S.textured_spheres(3, target=False))
+    rec.step()
+    seen = rec.seen.clone()
+    e1 = rec.errors(seen)
+    for _ in range(99):
+        rec.step()
+    h = rec.history
+    e = rec.errors(seen)
+    print(f"materials loop: loss {h[0]['loss']:.4e} -> {h[-1]['loss']:.4e}; seen texels {float(seen.float().mean()):.3f}, "
+          f"their error {e1['seen_texel_rmse']:.4f} -> {e['seen_texel_rmse']:.4f}; roughness error "
+          f"{e1['roughness_rmse']:.4f} -> {e['roughness_rmse']:.4f}", flush=True)
+    assert h[-1]["loss"] < 0.9 * h[0]["loss"]
+    assert e["seen_texel_rmse"] < e1["seen_texel_rmse"] and e["roughness_rmse"] < 0.8 * e1["roughness_rmse"]
