@@ -169,6 +169,7 @@ struct FhpPass_ {
     DevBuf thp, prevN, rng, Lp, Ls, rayQ0, rayQ1, pathQ0, pathQ1, hitQ, hitIdx, shQ, pixHit, counters, stats;
     DevBuf viewTab, sceneTab;  // batch mode tables
     DevBuf warViewTab;         // batched WARDiffPathTracer views
+    DevBuf warLog;             // WARDiffPathTracer adjoint hit log
     uint32_t cap = 0, capPix = 0;
     KernelTimer timer;
     // AccumulatePass state: running sums of the current precision mode, frames accumulated,
@@ -244,6 +245,7 @@ const std::map<std::string, double>& warDefaults() {
         {"useMIS", 1},          {"useWAR", 0},            {"auxSampleCount", 16},   {"Log10vMFConcentration", 5},
         {"Log10vMFConcentrationScreen", 5},               {"boundaryTermBeta", 0.01}, {"useAntitheticSampling", 1},
         {"useSolidAngleNEEWarp", 0},  // extension: NEE vertex gradients with the sampled direction held
+        {"useHitLog", 1},  // extension: the adjoint's replay reads its first pass's traversal answers
         {"countTraversal", 0},  {"timeKernels", 0},       {"traceBlockSize", 128},
         {"traceStackDepth", 24},                          {"traceBlocksPerCU", 10}};  // 1280 threads per CU: 5 waves per SIMD
     return d;
@@ -316,6 +318,20 @@ int passTrace(const FhpScene_* s, const FhpPass_* p, TraceScratch& t) {
     const uint64_t words = uint64_t(t.ovfDepth) * t.gridBlocks * t.block;
     if (words && (!s->ovf.p || words * 4 > s->ovf.bytes))
         return fail(FHP_ESTATE, "traversal overflow stack smaller than the launch needs");
+    return FHP_OK;
+}
+
+// Hit log of the WARDiffPathTracer adjoint and debug-image kernels (useHitLog): one word per
+// closest-hit trace of a path (at most maxBounces + 2) for every thread of the persistent grid,
+// so the replay pass traces nothing. Triangle ids must leave bit 31 free for the visibility bit.
+int prepareHitLog(FhpPass_* p, const FhpScene_* s, const TraceScratch& ts, WarArgs& a) {
+    a.hitLog = nullptr;
+    a.logDepth = 0;
+    const uint64_t depth = uint64_t(a.maxBounces) + 2;
+    if (p->get("useHitLog") == 0 || depth > 64 || s->h.nT >= 0x7fffffffu) return FHP_OK;
+    HIP_OR_FAIL(p->warLog.alloc(depth * ts.gridBlocks * ts.block * 4));
+    a.hitLog = p->warLog.as<uint32_t>();
+    a.logDepth = uint32_t(depth);
     return FHP_OK;
 }
 
@@ -1266,6 +1282,7 @@ int fhp_pass_execute(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t spp
             std::memcpy(a.dbgGrad, p->dbgGrad, sizeof(a.dbgGrad));
             a.dbgThreshold = float(p->get("assertThreshold"));
             a.dColor = p->outDColor;
+            if ((rc = prepareHitLog(p, s, ts, a))) return rc;
             HIP_OR_FAIL(launchWarDebugImage(s->dev, a, ts, st, &p->timer));
         } else {
             if (mode == kDiffForwardDebug && p->dbgVar == kDiffVarGeometryTranslation)
@@ -1702,6 +1719,7 @@ int warBackward(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t spp, uin
     if ((rc = prepareTimer(p))) return rc;
     TraceScratch ts;
     if ((rc = passTrace(s, p, ts))) return rc;
+    if ((rc = prepareHitLog(p, s, ts, a))) return rc;
     HIP_OR_FAIL(launchWarAdjoint(s->dev, a, ts, st, &p->timer));
     if (war) {
         GeomArgs ga{};

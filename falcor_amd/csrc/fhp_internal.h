@@ -174,6 +174,10 @@ struct WarArgs {
     // v * W * H * spp + p is path p of view v; tmp holds every view's samples in that order.
     const WarView* views;
     uint32_t nViews;
+    // Adjoint and debug-image kernels: per-thread hit log of the first pass, read by the replay
+    // (logDepth words per thread of the persistent grid, [entry][thread]); null: the replay traces.
+    uint32_t* hitLog;
+    uint32_t logDepth;
 };
 hipError_t launchWarPrimal(const SceneDev& s, const WarArgs& a, const TraceScratch& ts, hipStream_t stream,
                            KernelTimer* timer);

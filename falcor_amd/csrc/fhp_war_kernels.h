@@ -15,6 +15,13 @@ namespace {
 #endif
 constexpr int kWarWavesPerEU = FHP_WAR_WAVES_PER_EU;
 
+// Hit log of grid thread gtid (of G): a.logDepth entries [entry][thread] in a.hitLog, which holds
+// at least G * logDepth words; none without a log.
+FHP_DEVI HitLog warHitLog(const WarArgs& a, uint32_t gtid, uint32_t G) {
+    if (!a.hitLog || a.logDepth == 0) return noLog();
+    return HitLog{(GlobalU32*)(a.hitLog + gtid), G, a.logDepth};
+}
+
 template <int B, int D, bool ADJ, bool FROZEN, bool TRANS = false>
 __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(kWarWavesPerEU))) void kWar(SceneDev s, WarArgs a,
                                                                                                  TraceScratch ts) {
@@ -23,7 +30,8 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(kWarWavesPerE
     const uint32_t n = a.W * a.H * a.spp * (a.views ? a.nViews : 1u);
     const uint32_t G = gridDim.x * B;
     unsigned long long nc = 0, nv = 0;
-    for (uint32_t id = blockIdx.x * B + threadIdx.x; id < n; id += G) warItem<ADJ, FROZEN, TRANS>(s, a, id, st, nc, nv);
+    const HitLog hl = warHitLog(a, blockIdx.x * B + threadIdx.x, G);
+    for (uint32_t id = blockIdx.x * B + threadIdx.x; id < n; id += G) warItem<ADJ, FROZEN, TRANS>(s, a, id, st, nc, nv, hl);
     if (a.rays) {
         atomicAdd(&a.rays[0], nc);
         atomicAdd(&a.rays[1], nv);
@@ -38,7 +46,8 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(kWarWavesPerE
     const uint32_t n = a.W * a.H * a.spp * (a.views ? a.nViews : 1u);
     const uint32_t G = gridDim.x * B;
     unsigned long long nc = 0, nv = 0;
-    for (uint32_t id = blockIdx.x * B + threadIdx.x; id < n; id += G) warDebugItem<FROZEN, TRANS>(s, a, id, st, nc, nv);
+    const HitLog hl = warHitLog(a, blockIdx.x * B + threadIdx.x, G);
+    for (uint32_t id = blockIdx.x * B + threadIdx.x; id < n; id += G) warDebugItem<FROZEN, TRANS>(s, a, id, st, nc, nv, hl);
     if (a.rays) {
         atomicAdd(&a.rays[0], nc);
         atomicAdd(&a.rays[1], nv);

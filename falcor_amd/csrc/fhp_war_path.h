@@ -12,7 +12,10 @@
 //   dS/df_j(w_j) = dLdI * R_j / f_j(w_j)          (scatter throughput factor)
 //   dS/df_j(w_l) = dLdI * thp_j * Li * mis        (NEE)
 //   dS/dE_j      = dLdI * thp_j * mis             (emission)
-// R_j is obtained as L - (radiance accumulated up to vertex j), so no vertex is stored.
+// R_j is obtained as L - (radiance accumulated up to vertex j), so no vertex is stored. The
+// first pass records each traversal's answer in a per-thread hit log (4 bytes per path vertex),
+// and the replay reads it instead of tracing again: the replay recomputes every ray bit for bit,
+// so the answers are the ones a second traversal would give.
 // Parameter gradients are routed like GradientIOWrapper.slang:48-116: 20 floats per material
 // (StandardMaterialParamLayout.slang:32-41), hash row jenkinsHash(pixelID) % hashSize; the
 // per-texel baseColor gradient (texture extension) uses the bilinear tap weights.
@@ -23,6 +26,22 @@
 namespace fhp {
 namespace {
 
+
+// Traversal answers of one path, written by the adjoint's first pass and read by its replay.
+// Entry k (at p[k * stride]) is the triangle of closest-hit trace k (trace 0 is the primary ray,
+// trace k the scatter ray of vertex k - 1), or kLogMiss; bit 31 is the visibility of vertex k's
+// light sample. Entries at k >= depth are not kept: those rays are traced in both passes.
+constexpr uint32_t kLogMiss = 0x7fffffffu;
+constexpr uint32_t kLogLit = 0x80000000u;
+struct HitLog {
+    GlobalU32* p;     // this thread's entry 0, or null (both passes trace every ray)
+    uint32_t stride;  // threads of the persistent grid
+    uint32_t depth;   // entries per thread
+    FHP_DEVI bool has(uint32_t k) const { return p != nullptr && k < depth; }
+    FHP_DEVI void put(uint32_t k, uint32_t v) const { p[k * stride] = v; }
+    FHP_DEVI uint32_t get(uint32_t k) const { return p[k * stride]; }
+};
+FHP_DEVI HitLog noLog() { return HitLog{nullptr, 0u, 0u}; }
 
 // Gaussian pixel jitter (WARDiffPathTracer.rt.slang:60-63).
 FHP_DEVI void gaussian2(float ux, float uy, float sigma, float& gx, float& gy) {
@@ -368,11 +387,12 @@ FHP_DEV void flushGrad(const SceneDev& s, const WarArgs& a, uint32_t hashRow, co
 // evaluation. FROZEN: the finite-difference mode's separate sampling parameters (matsS /
 // texelsS) are in use; otherwise one material serves evaluation and sampling.
 // DEBUG (with ADJ): instead of accumulating the gradients, each vertex adds its debug projection
-// to *dproj.
+// to *dproj. With a hit log, the pass without ADJ records its traversal answers and the ADJ pass
+// replays them (tracing only the rays beyond the log's depth).
 template <bool ADJ, bool FROZEN, bool DEBUG = false, bool TRANS = false>
 FHP_DEV f3 warPath(const SceneDev& s, const WarArgs& a, uint32_t view, uint32_t px, uint32_t py,
                    uint32_t frameSeed, TraceStack& st, f3 Ltot, f3 dL, uint32_t hashRow, unsigned long long& nc,
-                   unsigned long long& nv, float* dproj = nullptr) {
+                   unsigned long long& nv, const HitLog& hl, float* dproj = nullptr) {
     Rng rng = rngSeed(px, py, frameSeed / 2);
     for (int i = 0; i < 6; i++) rng.u1();
     const float pfx = float(px) + 0.5f, pfy = float(py) + 0.5f;
@@ -394,8 +414,19 @@ FHP_DEV f3 warPath(const SceneDev& s, const WarArgs& a, uint32_t view, uint32_t 
     dir = dl > 0.f ? dir / dl : mk3(0.f);
     f3 rayO = cam.pos, rayD = dir;
     HitRec h;
-    nc++;
-    if (!traceBVH4<false, false>(s, rayO, rayD, 0.f, kFltMax, st, h)) return radiance;
+    // Closest-hit trace k: from the log in the replay, recorded in the first pass.
+    auto closest = [&](uint32_t k) -> bool {
+        if (ADJ && hl.has(k)) {
+            const uint32_t e = hl.get(k) & ~kLogLit;
+            h.tri = e;
+            return e != kLogMiss;
+        }
+        nc++;
+        const bool hit = traceBVH4<false, false>(s, rayO, rayD, 0.f, kFltMax, st, h);
+        if (!ADJ && hl.has(k) && !hit) hl.put(k, kLogMiss);
+        return hit;
+    };
+    if (!closest(0)) return radiance;
     for (;;) {
         // computeIntersectionAD, AttachToRay (DiffSceneQuery.slang:131-219).
         f3 posW, normalW, bary;
@@ -433,11 +464,16 @@ FHP_DEV f3 warPath(const SceneDev& s, const WarArgs& a, uint32_t view, uint32_t 
             const float distance = length(toL);
             ldir = normalize(toL);
             if (anyPos(Li)) {
-                HitRec sh;
-                nv++;
-                lit = !traceBVH4<true, false>(s, origin, ldir, 0.f, distance, st, sh);
+                if (ADJ && hl.has(len)) {
+                    lit = (hl.get(len) & kLogLit) != 0u;
+                } else {
+                    HitRec sh;
+                    nv++;
+                    lit = !traceBVH4<true, false>(s, origin, ldir, 0.f, distance, st, sh);
+                }
             }
         }
+        if (!ADJ && hl.has(len)) hl.put(len, h.tri | (lit ? kLogLit : 0u));
         // PTUtils.slang:97-123
         Shading sd = shadingFromHit(s, h.tri, bary.y, bary.z, rayD);
         sd.V = -rayD;
@@ -523,8 +559,7 @@ FHP_DEV f3 warPath(const SceneDev& s, const WarArgs& a, uint32_t view, uint32_t 
         }
         if (!cont) break;
         len++;
-        nc++;
-        if (!traceBVH4<false, false>(s, rayO, rayD, 0.f, kFltMax, st, h)) break;
+        if (!closest(len)) break;
     }
     return radiance;
 }
@@ -546,13 +581,13 @@ template <typename Args> FHP_DEVI WarViewRef warView(const Args& a, uint32_t id)
 
 template <bool ADJ, bool FROZEN, bool TRANS = false>
 FHP_DEVI void warItem(const SceneDev& s, const WarArgs& a, uint32_t id, TraceStack& st, unsigned long long& nc,
-                      unsigned long long& nv) {
+                      unsigned long long& nv, const HitLog& hl = noLog()) {
     const WarViewRef vr = warView(a, id);
     const uint32_t pix = vr.local / a.spp, k = vr.local % a.spp;
     const uint32_t px = pix % a.W, py = pix / a.W;
     const uint32_t seed = vr.frame + k;
     if (!ADJ) {
-        f3 L = warPath<false, FROZEN>(s, a, vr.view, px, py, seed, st, mk3(0.f), mk3(0.f), 0, nc, nv);
+        f3 L = warPath<false, FROZEN>(s, a, vr.view, px, py, seed, st, mk3(0.f), mk3(0.f), 0, nc, nv, noLog());
         a.out[id] = make_float4(L.x, L.y, L.z, 0.f);
     } else {
         // dLdI null: the BackwardDiffDebug mode's dL/dI = (1, 0, 0) at every pixel.
@@ -562,11 +597,9 @@ FHP_DEVI void warItem(const SceneDev& s, const WarArgs& a, uint32_t id, TraceSta
             dL = mk3(dl[0], dl[1], dl[2]);
         }
         if (dL.x == 0.f && dL.y == 0.f && dL.z == 0.f) return;
-        unsigned long long c0 = 0, c1 = 0;
-        f3 L = warPath<false, FROZEN>(s, a, vr.view, px, py, seed, st, mk3(0.f), mk3(0.f), 0, c0, c1);
-        warPath<true, FROZEN, false, TRANS>(s, a, vr.view, px, py, seed, st, L, dL, jenkins(pix) % a.hashSize, nc, nv);
-        nc += c0;
-        nv += c1;
+        f3 L = warPath<false, FROZEN>(s, a, vr.view, px, py, seed, st, mk3(0.f), mk3(0.f), 0, nc, nv, hl);
+        warPath<true, FROZEN, false, TRANS>(s, a, vr.view, px, py, seed, st, L, dL, jenkins(pix) % a.hashSize, nc, nv,
+                                            hl);
     }
 }
 
@@ -576,14 +609,16 @@ FHP_DEVI void warItem(const SceneDev& s, const WarArgs& a, uint32_t id, TraceSta
 // directional derivative). tmp[id] = (radiance, derivative).
 template <bool FROZEN, bool TRANS>
 FHP_DEVI void warDebugItem(const SceneDev& s, const WarArgs& a, uint32_t id, TraceStack& st, unsigned long long& nc,
-                           unsigned long long& nv) {
+                           unsigned long long& nv, const HitLog& hl = noLog()) {
     const uint32_t pix = id / a.spp, k = id % a.spp;
     const uint32_t px = pix % a.W, py = pix / a.W;
     const uint32_t seed = a.frame + k;
-    unsigned long long c0 = 0, c1 = 0;
-    const f3 L = warPath<false, FROZEN>(s, a, 0u, px, py, seed, st, mk3(0.f), mk3(0.f), 0, nc, nv);
+    const f3 L = warPath<false, FROZEN>(s, a, 0u, px, py, seed, st, mk3(0.f), mk3(0.f), 0, nc, nv, hl);
     float d = 0.f;
-    warPath<true, FROZEN, true, TRANS>(s, a, 0u, px, py, seed, st, L, mk3(1.f, 0.f, 0.f), 0, c0, c1, &d);
+    // The replay's rays are the first pass's: count them once.
+    unsigned long long c0 = 0, c1 = 0;
+    warPath<true, FROZEN, true, TRANS>(s, a, 0u, px, py, seed, st, L, mk3(1.f, 0.f, 0.f), 0, hl.p ? nc : c0,
+                                       hl.p ? nv : c1, hl, &d);
     a.tmp[id] = make_float4(L.x, L.y, L.z, d);
 }
 

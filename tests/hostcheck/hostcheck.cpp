@@ -227,10 +227,11 @@ int main(int argc, char** argv) {
                 if (adj) in >> hash;
                 if (!in) throw std::runtime_error("bad step: " + line);
                 uint32_t cap = 0;
-                std::string opt;
-                if (!adj && (in >> opt)) {
-                    if (opt.rfind("cap=", 0) != 0) throw std::runtime_error("bad step option: " + opt);
-                    cap = uint32_t(std::stoul(opt.substr(4)));
+                bool useLog = false;
+                for (std::string opt; in >> opt;) {
+                    if (!adj && opt.rfind("cap=", 0) == 0) cap = uint32_t(std::stoul(opt.substr(4)));
+                    else if (adj && opt == "log=1") useLog = true;
+                    else throw std::runtime_error("bad step option: " + opt);
                 }
                 if (adj && ndl != 3 * size_t(W) * H) throw std::runtime_error("dLdI size does not match the frame");
                 std::vector<float4> out(adj ? 0 : size_t(W) * H * spp);
@@ -246,16 +247,27 @@ int main(int argc, char** argv) {
                 a.matGradDim = ck.h.nM * 20;
                 a.gradTex = gradTex.empty() ? nullptr : gradTex.data();
                 LaneStack ls(lds, ck.h.stackNeed, cap);
+                // The adjoint's hit log of one thread (the only thread here): exact size, so an
+                // entry beyond its depth is an out-of-bounds access.
+                std::unique_ptr<uint32_t[]> logBuf(useLog ? new uint32_t[B + 2]() : nullptr);
+                const HitLog hl = useLog ? HitLog{logBuf.get(), 1u, B + 2} : noLog();
                 unsigned long long nc = 0, nv = 0;
                 uint32_t peak = 0;
                 const uint64_t n = uint64_t(W) * H * spp;
                 for (uint32_t id = 0; id < n; id++) {
-                    if (adj && s.hasTransmission) warItem<true, false, true>(s, a, id, ls.st, nc, nv);
-                    else if (adj) warItem<true, false>(s, a, id, ls.st, nc, nv);
+                    if (adj && s.hasTransmission) warItem<true, false, true>(s, a, id, ls.st, nc, nv, hl);
+                    else if (adj) warItem<true, false>(s, a, id, ls.st, nc, nv, hl);
                     else warItem<false, false>(s, a, id, ls.st, nc, nv);
                     peak = std::max(peak, ls.st.peak);
                 }
                 report(op.c_str(), n, nc + nv, ls, peak);
+                if (adj) {
+                    // Order-independent fingerprint of the gradients (bit patterns summed).
+                    uint64_t fp = 0;
+                    for (float v : grad) fp += uint64_t(f2u(v)) * 2654435761u;
+                    for (float v : gradTex) fp += uint64_t(f2u(v)) * 2246822519u;
+                    std::printf("adjoint grad_fingerprint=%016llx log=%d\n", (unsigned long long)fp, int(useLog));
+                }
                 bad |= ls.err != 0;
             } else if (op == "geom") {
                 uint32_t W, H, spp, frame, aux, mesh, hash, lds;

@@ -260,3 +260,21 @@ def test_adjoint_standard_params_and_texels_match_fd(gpu, oracle_mod):
         off, w, h = td[t, 0], td[t, 1], td[t, 2]
         s = tex[off:off + 3 * w * h].reshape(-1, 3).sum(0)
         np.testing.assert_allclose(s, mat[t, 0:3], rtol=1e-3, atol=1e-5)
+
+
+@pytest.mark.parametrize("bounces", [1, 4])
+def test_adjoint_hit_log_replay_equals_retrace(gpu, bounces):
+    """The adjoint replays its first pass's traversal answers from the hit log (useHitLog, the
+    default) instead of tracing again: same gradients as the replay that traces (up to the order
+    of the float atomics), on a StandardMaterial sphere field with textures, and half the rays."""
+    from falcor_amd.render_graph import Scene
+    data = S.textured_spheres(3, num_objects=16, tex_res=16, subdivisions=2, target=True)
+    W = H = 64
+    sc = Scene(data)
+    dLdI = _loss_weights(W, H, 1)
+    got = {}
+    for log in (0, 1):
+        got[log] = _gpu_grad(sc, {"maxBounces": bounces, "useHitLog": log}, W, H, 4, 3, dLdI, texture=True)
+    for a, b in zip(got[0], got[1]):
+        assert np.abs(a).max() > 0
+        np.testing.assert_allclose(b, a, rtol=1e-4, atol=1e-6 * np.abs(a).max())

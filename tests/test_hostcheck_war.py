@@ -111,6 +111,28 @@ def test_multi_bounce_paths_in_bounds(which):
          np.ones((H, W, 3), np.float32))
 
 
+@pytest.mark.parametrize("which", ["sphere_field", "textured_spheres", "transmissive_field"])
+def test_hit_log_replay_equals_retrace(which):
+    """The adjoint's replay from the first pass's hit log (one word per closest-hit trace, at most
+    maxBounces + 2, in an exact-size array) gives the gradients of a replay that traces again, bit
+    for bit (sequential host run: no atomics reorder the sums), and traces half the rays."""
+    data = {"sphere_field": lambda: S.sphere_field(1, num_spheres=64, num_emissive=256),
+            "textured_spheres": lambda: S.textured_spheres(3, num_objects=4, tex_res=16),
+            "transmissive_field": _transmissive_field}[which]()
+    W = H = 40
+    steps = []
+    for B in (0, 1, 4):
+        steps += [f"adjoint {W} {H} 3 {B + 1} {B} 16 8", f"adjoint {W} {H} 3 {B + 1} {B} 16 8 log=1"]
+    out = _run(data, steps, np.ones((H, W, 3), np.float32))
+    fps = re.findall(r"grad_fingerprint=([0-9a-f]+) log=(\d)", out)
+    rays = [int(v) for v in re.findall(r"adjoint paths=\d+ rays=(\d+)", out)]
+    assert len(fps) == 6 and len(rays) == 6
+    for i in range(0, 6, 2):
+        assert fps[i][1] == "0" and fps[i + 1][1] == "1"
+        assert fps[i][0] == fps[i + 1][0], out
+        assert rays[i] == 2 * rays[i + 1], out
+
+
 def test_undersized_stack_reports_overflow_in_bounds(box_sphere):
     data, _ = box_sphere
     W = H = 32
