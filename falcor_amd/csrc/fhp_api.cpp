@@ -170,6 +170,7 @@ struct FhpPass_ {
     DevBuf viewTab, sceneTab;  // batch mode tables
     DevBuf warViewTab;         // batched WARDiffPathTracer views
     DevBuf warLog;             // WARDiffPathTracer adjoint hit log
+    DevBuf auxLog;             // WARDiffPathTracer vertex-gradient aux-hit log
     uint32_t cap = 0, capPix = 0;
     KernelTimer timer;
     // AccumulatePass state: running sums of the current precision mode, frames accumulated,
@@ -245,7 +246,7 @@ const std::map<std::string, double>& warDefaults() {
         {"useMIS", 1},          {"useWAR", 0},            {"auxSampleCount", 16},   {"Log10vMFConcentration", 5},
         {"Log10vMFConcentrationScreen", 5},               {"boundaryTermBeta", 0.01}, {"useAntitheticSampling", 1},
         {"useSolidAngleNEEWarp", 0},  // extension: NEE vertex gradients with the sampled direction held
-        {"useHitLog", 1},  // extension: the adjoint's replay reads its first pass's traversal answers
+        {"useHitLog", 1},  // extension: adjoint replays and second aux loops read the first pass's hits
         {"countTraversal", 0},  {"timeKernels", 0},       {"traceBlockSize", 128},
         {"traceStackDepth", 24},                          {"traceBlocksPerCU", 10}};  // 1280 threads per CU: 5 waves per SIMD
     return d;
@@ -1737,6 +1738,11 @@ int warBackward(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t spp, uin
         ga.rays = h_stats ? dstats + 2 : nullptr;
         ga.views = a.views;
         ga.nViews = a.nViews;
+        if (p->get("useHitLog") != 0) {
+            // The kernel's grid has at most ts.gridBlocks blocks of ts.block threads.
+            HIP_OR_FAIL(p->auxLog.alloc(size_t(3) * ga.aux * ts.gridBlocks * ts.block * 4));
+            ga.auxLog = p->auxLog.as<uint32_t>();
+        }
         HIP_OR_FAIL(launchWarGeom(s->dev, ga, ts, st, &p->timer));
     }
     if ((rc = collectTimer(p, st))) return rc;

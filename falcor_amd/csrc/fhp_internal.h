@@ -206,6 +206,9 @@ struct GeomArgs {
     float* gradNrm;            // MeshNormal gradients, same layout
     uint32_t hashSize, meshV0, meshVerts;
     unsigned long long* rays;  // closest-hit ray counter (primary, scatter and auxiliary rays), or null
+    // Aux-hit log: 3 * aux words per thread of the persistent grid ([word][thread]), so each
+    // warp's second loop over its aux rays reads the first loop's hits; null: both loops trace.
+    uint32_t* auxLog;
 };
 hipError_t launchWarGeom(const SceneDev& s, const GeomArgs& a, const TraceScratch& ts, hipStream_t stream, KernelTimer* timer);
 hipError_t launchAggregate(const float* tmp, uint32_t hashSize, uint32_t dim, float* out, hipStream_t stream);

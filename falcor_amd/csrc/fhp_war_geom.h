@@ -173,12 +173,42 @@ struct AuxHit {
     float w, c;    // harmonic weight and its direction-derivative coefficient
 };
 
-// Traces one aux ray (AttachToGeometry) and evaluates computeHarmonicWeight and the coefficient of
-// its custom forward derivative (WarpedAreaReparam.slang:50-98).
-FHP_DEV AuxHit auxTrace(const SceneDev& s, TraceStack& st, f3 o, f3 a, float uy, float kappa) {
+// Aux-ray hits of one warp's first loop over its aux rays, read back by its second loop, which
+// then traces nothing: entry i is (triangle or kInvalid, b1, b2) at p[(3 i + k) * stride]. The
+// second loop regenerates the same aux directions from the same origin, so these are the answers
+// a second traversal would give.
+struct AuxLog {
+    GlobalU32* p;     // this thread's word 0, or null (both loops trace)
+    uint32_t stride;  // threads of the persistent grid
+    FHP_DEVI void put(uint32_t i, bool hit, const HitRec& h) const {
+        p[(3 * i) * stride] = hit ? h.tri : kInvalid;
+        p[(3 * i + 1) * stride] = f2u(h.b1);
+        p[(3 * i + 2) * stride] = f2u(h.b2);
+    }
+    FHP_DEVI bool get(uint32_t i, HitRec& h) const {
+        h.tri = p[(3 * i) * stride];
+        h.b1 = u2f(p[(3 * i + 1) * stride]);
+        h.b2 = u2f(p[(3 * i + 2) * stride]);
+        return h.tri != kInvalid;
+    }
+};
+FHP_DEVI AuxLog noAuxLog() { return AuxLog{nullptr, 0u}; }
+enum AuxMode { kAuxTrace, kAuxRecord, kAuxReplay };
+
+// Traces aux ray i of a warp (AttachToGeometry) and evaluates computeHarmonicWeight and the
+// coefficient of its custom forward derivative (WarpedAreaReparam.slang:50-98). With a log, the
+// first loop records the traversal's answer (kAuxRecord) and the second reads it (kAuxReplay).
+FHP_DEV AuxHit auxTrace(const SceneDev& s, TraceStack& st, f3 o, f3 a, float uy, float kappa, const AuxLog& lg,
+                        uint32_t i, AuxMode mode, unsigned long long& nc) {
     AuxHit r;
     HitRec h;
-    r.hit = traceBVH4<false, false>(s, o, a, 0.f, kFltMax, st, h);
+    if (mode == kAuxReplay && lg.p) {
+        r.hit = lg.get(i, h);
+    } else {
+        nc++;
+        r.hit = traceBVH4<false, false>(s, o, a, 0.f, kFltMax, st, h);
+        if (mode == kAuxRecord && lg.p) lg.put(i, r.hit, h);
+    }
     float boundary = 0.f;
     r.d = a;
     r.tri = kInvalid;
@@ -243,8 +273,8 @@ FHP_DEVI f3 clampDW3(f3 g) { return mk3(clampDW(g.x), clampDW(g.y), clampDW(g.z)
 // Adjoint of reparameterizeRay (WarpedAreaReparam.slang:226-263) for a ray (o, w): given dS/dV
 // (gV) and dS/d detJ (gDet), adds the aux-triangle vertex gradients and returns dS/do; with
 // wantOmega also dS/dw through the harmonic weights' direction derivatives.
-FHP_DEV void warpAdjoint(const SceneDev& s, const GeomArgs& a, TraceStack& st, f3 o, f3 w, Rng rng, f3 gV, float gDet,
-                            bool wantOmega, uint32_t row, f3& gO, f3& gW, unsigned long long& nc) {
+FHP_DEV void warpAdjoint(const SceneDev& s, const GeomArgs& a, TraceStack& st, const AuxLog& lg, f3 o, f3 w, Rng rng,
+                         f3 gV, float gDet, bool wantOmega, uint32_t row, f3& gO, f3& gW, unsigned long long& nc) {
     gO = mk3(0.f);
     gW = mk3(0.f);
     const float ww = dot(w, w);
@@ -256,8 +286,7 @@ FHP_DEV void warpAdjoint(const SceneDev& s, const GeomArgs& a, TraceStack& st, f
     for (uint32_t i = 0; i < a.aux; i++) {
         float uy;
         const f3 ai = gen.next(i, uy);
-        const AuxHit h = auxTrace(s, st, o, ai, uy, a.kappa);
-        nc++;
+        const AuxHit h = auxTrace(s, st, o, ai, uy, a.kappa, lg, i, kAuxRecord, nc);
         const f3 g = clampDW3((ai * ww - w * dot(w, ai)) * h.c);
         Sw += h.w;
         Sd = Sd + h.d * h.w;
@@ -281,8 +310,7 @@ FHP_DEV void warpAdjoint(const SceneDev& s, const GeomArgs& a, TraceStack& st, f
     for (uint32_t i = 0; i < a.aux; i++) {
         float uy;
         const f3 ai = gen.next(i, uy);
-        const AuxHit h = auxTrace(s, st, o, ai, uy, a.kappa);
-        nc++;
+        const AuxHit h = auxTrace(s, st, o, ai, uy, a.kappa, lg, i, kAuxReplay, nc);
         const f3 gu = (ai * ww - w * dot(w, ai)) * h.c;
         const f3 g = clampDW3(gu);
         const float sk = h.w / Swc;
@@ -321,8 +349,9 @@ FHP_DEVI CameraDev geomCamera(const SceneDev& s, const GeomArgs& a, uint32_t vie
 
 // Adjoint of reparameterizeScreenSample (WarpedAreaReparam.slang:269-332) at screen sample ps:
 // gVs = dS/dV (screen units), gDet = dS/d detJ. Adds aux-triangle vertex gradients.
-FHP_DEV void screenWarpAdjoint(const SceneDev& s, const GeomArgs& a, uint32_t view, TraceStack& st, float psx, float psy,
-                                  Rng rng, float gVx, float gVy, float gDet, uint32_t row, unsigned long long& nc) {
+FHP_DEV void screenWarpAdjoint(const SceneDev& s, const GeomArgs& a, uint32_t view, TraceStack& st, const AuxLog& lg,
+                               float psx, float psy, Rng rng, float gVx, float gVy, float gDet, uint32_t row,
+                               unsigned long long& nc) {
     const CameraDev cam = geomCamera(s, a, view);
     const f3 o = cam.pos;
     const f3 r0 = camRaw(cam, psx, psy, a.W, a.H);
@@ -338,8 +367,7 @@ FHP_DEV void screenWarpAdjoint(const SceneDev& s, const GeomArgs& a, uint32_t vi
     for (uint32_t i = 0; i < a.aux; i++) {
         float uy;
         const f3 ai = gen.next(i, uy);
-        const AuxHit h = auxTrace(s, st, o, ai, uy, a.kappaScreen);
-        nc++;
+        const AuxHit h = auxTrace(s, st, o, ai, uy, a.kappaScreen, lg, i, kAuxRecord, nc);
         const f3 g = (ai - w * dot(w, ai)) * h.c;
         Sw += h.w;
         Sd = Sd + h.d * h.w;
@@ -379,8 +407,7 @@ FHP_DEV void screenWarpAdjoint(const SceneDev& s, const GeomArgs& a, uint32_t vi
     for (uint32_t i = 0; i < a.aux; i++) {
         float uy;
         const f3 ai = gen.next(i, uy);
-        const AuxHit h = auxTrace(s, st, o, ai, uy, a.kappaScreen);
-        nc++;
+        const AuxHit h = auxTrace(s, st, o, ai, uy, a.kappaScreen, lg, i, kAuxReplay, nc);
         if (!h.hit) continue;
         const f3 g = (ai - w * dot(w, ai)) * h.c;
         const float hx = clampDW(dot(cx, g)), hy = clampDW(dot(cy, g));
@@ -402,7 +429,7 @@ FHP_DEVI void gaussianJitter(float ux, float uy, float sigma, float& gx, float& 
 
 // One path: primal replay of tracePaths/tracePath for maxBounces = 0, then the reverse sweep.
 FHP_DEV void geomPath(const SceneDev& s, const GeomArgs& a, uint32_t view, uint32_t px, uint32_t py, uint32_t seed, f3 dL,
-                         uint32_t row, TraceStack& st, unsigned long long& nc) {
+                      uint32_t row, TraceStack& st, const AuxLog& lg, unsigned long long& nc) {
     Rng rng = rngSeed(px, py, seed / 2);
     for (int i = 0; i < 6; i++) rng.u1();
     const float pcx = float(px) + 0.5f, pcy = float(py) + 0.5f;
@@ -489,7 +516,7 @@ FHP_DEV void geomPath(const SceneDev& s, const GeomArgs& a, uint32_t view, uint3
                     gWi = gWi + sd.N * gci + wo * gcio;
                     const f3 gF = sd.N * gco + sd.V * gcio;
                     f3 gO, gOm;
-                    warpAdjoint(s, a, st, rayOriginOffset(x1, faceO), wo, rng, gF, QN, false, row, gO, gOm, nc);
+                    warpAdjoint(s, a, st, lg, rayOriginOffset(x1, faceO), wo, rng, gF, QN, false, row, gO, gOm, nc);
                     gX = gX + gO;
                 } else if (phi != 0.f && cosL > 0.f) {
                     // computeNEE (PTUtils.slang:211-249): Q_N = phi(x1, N, d) (a2s / a2s^) detJ with
@@ -501,7 +528,7 @@ FHP_DEV void geomPath(const SceneDev& s, const GeomArgs& a, uint32_t view, uint3
                     gWi = gWi + sd.N * gci + wo * gcio;
                     const f3 gD = sd.N * gco + sd.V * gcio + ls.normalW * (phi / cosL);  // dS/dd
                     f3 gO, gOm;
-                    warpAdjoint(s, a, st, rayOriginOffset(x1, faceO), wo, rng, gD, QN, true, row, gO, gOm, nc);
+                    warpAdjoint(s, a, st, lg, rayOriginOffset(x1, faceO), wo, rng, gD, QN, true, row, gO, gOm, nc);
                     const f3 gWo = gD + gOm;
                     gX = gX + gO - (gWo - wo * dot(wo, gWo)) / dist - wo * (2.f * phi / dist);
                 }
@@ -534,7 +561,7 @@ FHP_DEV void geomPath(const SceneDev& s, const GeomArgs& a, uint32_t view, uint3
                     gWi = gWi + sd.N * gci + woS * gcio;
                     const f3 gDir = sd.N * gco + sd.V * gcio;
                     f3 gO, gOm;
-                    warpAdjoint(s, a, st, scO, woS, rng, gDir, QS, false, row, gO, gOm, nc);
+                    warpAdjoint(s, a, st, lg, scO, woS, rng, gDir, QS, false, row, gO, gOm, nc);
                     gX = gX + gO;
                 }
             }
@@ -596,11 +623,12 @@ FHP_DEV void geomPath(const SceneDev& s, const GeomArgs& a, uint32_t view, uint3
         gVx += dot(cam1.U, q) * (2.f / float(a.W));
         gVy += dot(cam1.V, q) * (-2.f / float(a.H));
     }
-    if (Qt != 0.f || gVx != 0.f || gVy != 0.f) screenWarpAdjoint(s, a, view, st, psx, psy, rngScreen, gVx, gVy, Qt, row, nc);
+    if (Qt != 0.f || gVx != 0.f || gVy != 0.f) screenWarpAdjoint(s, a, view, st, lg, psx, psy, rngScreen, gVx, gVy, Qt, row, nc);
 }
 
 // One item of the kernel's grid loop: path id = (view * W * H + pixel) * spp + sample.
-FHP_DEVI void geomItem(const SceneDev& s, const GeomArgs& a, uint32_t id, TraceStack& st, unsigned long long& nc) {
+FHP_DEVI void geomItem(const SceneDev& s, const GeomArgs& a, uint32_t id, TraceStack& st, unsigned long long& nc,
+                       const AuxLog& lg = noAuxLog()) {
     uint32_t view = 0, local = id, frame = a.frame;
     const float* dLdI = a.dLdI;
     if (a.views) {
@@ -617,7 +645,7 @@ FHP_DEVI void geomItem(const SceneDev& s, const GeomArgs& a, uint32_t id, TraceS
         dL = mk3(dl[0], dl[1], dl[2]);
     }
     if (dL.x == 0.f && dL.y == 0.f && dL.z == 0.f) return;
-    geomPath(s, a, view, pix % a.W, pix / a.W, frame + k, dL, jenkins(pix) % a.hashSize, st, nc);
+    geomPath(s, a, view, pix % a.W, pix / a.W, frame + k, dL, jenkins(pix) % a.hashSize, st, lg, nc);
 }
 
 }  // namespace

@@ -18,7 +18,9 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(kGeomWavesPer
     const uint32_t n = a.W * a.H * a.spp * (a.views ? a.nViews : 1u);
     const uint32_t G = gridDim.x * B;
     unsigned long long nc = 0;
-    for (uint32_t id = blockIdx.x * B + threadIdx.x; id < n; id += G) geomItem(s, a, id, st, nc);
+    // Aux-hit log of this thread: 3 * aux words [word][thread] in a.auxLog (3 * aux * G words).
+    const AuxLog lg = a.auxLog ? AuxLog{(GlobalU32*)(a.auxLog + blockIdx.x * B + threadIdx.x), G} : noAuxLog();
+    for (uint32_t id = blockIdx.x * B + threadIdx.x; id < n; id += G) geomItem(s, a, id, st, nc, lg);
     if (a.rays) atomicAdd(&a.rays[0], nc);
 }
 

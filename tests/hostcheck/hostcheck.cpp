@@ -276,10 +276,12 @@ int main(int argc, char** argv) {
                 if (!in) throw std::runtime_error("bad step: " + line);
                 std::string gradOut;
                 uint32_t useNEE = 1, neeHeld = 0;
+                bool useLog = false;
                 for (std::string opt; in >> opt;) {
                     if (opt.rfind("out=", 0) == 0) gradOut = opt.substr(4);
                     else if (opt == "nee=0") useNEE = 0;
                     else if (opt == "neeheld=1") neeHeld = 1;
+                    else if (opt == "log=1") useLog = true;
                     else throw std::runtime_error("bad step option: " + opt);
                 }
                 if (ndl != 3 * size_t(W) * H) throw std::runtime_error("dLdI size does not match the frame");
@@ -305,9 +307,18 @@ int main(int argc, char** argv) {
                 unsigned long long nc = 0;
                 uint32_t peak = 0;
                 const uint64_t n = uint64_t(W) * H * spp;
+                // The aux-hit log of the one thread here: exact size (3 words per aux ray).
+                std::unique_ptr<uint32_t[]> logBuf(useLog ? new uint32_t[3 * size_t(aux)]() : nullptr);
+                const AuxLog lg = useLog ? AuxLog{logBuf.get(), 1u} : noAuxLog();
                 for (uint32_t id = 0; id < n; id++) {
-                    geomItem(s, a, id, ls.st, nc);
+                    geomItem(s, a, id, ls.st, nc, lg);
                     peak = std::max(peak, ls.st.peak);
+                }
+                {
+                    uint64_t fp = 0;
+                    for (size_t i = 0; i < size_t(hash) * 3 * nvtx; i++)
+                        fp += uint64_t(f2u(gp[i])) * 2654435761u + uint64_t(f2u(gn[i])) * 2246822519u;
+                    std::printf("geom grad_fingerprint=%016llx log=%d\n", (unsigned long long)fp, int(useLog));
                 }
                 double sum = 0.0;
                 bool finite = true;

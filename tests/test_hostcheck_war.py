@@ -133,6 +133,27 @@ def test_hit_log_replay_equals_retrace(which):
         assert rays[i] == 2 * rays[i + 1], out
 
 
+def test_aux_log_replay_equals_retrace(box_sphere):
+    """The vertex-gradient kernel's second loop over each warp's aux rays reads the first loop's
+    hits from the aux-hit log (3 words per aux ray, exact-size array): gradients equal, bit for
+    bit, those of a second loop that traces again, for both NEE forms, and fewer rays are traced."""
+    data, mesh = box_sphere
+    W = H = 24
+    steps = []
+    for opt in ("", " neeheld=1"):
+        for log in ("", " log=1"):
+            steps.append(f"geom {W} {H} 2 3 16 3 3 {mesh} 8 24{opt}{log}")
+    out = _run(data, steps, _ramp(W, H))
+    fps = re.findall(r"geom grad_fingerprint=([0-9a-f]+) log=(\d)", out)
+    rays = [int(v) for v in re.findall(r"geom paths=\d+ rays=(\d+)", out)]
+    assert len(fps) == 4 and len(rays) == 4
+    for i in (0, 2):
+        assert fps[i][1] == "0" and fps[i + 1][1] == "1"
+        assert fps[i][0] == fps[i + 1][0], out
+        assert rays[i + 1] < 0.6 * rays[i], out
+    assert "finite=0" not in out
+
+
 def test_undersized_stack_reports_overflow_in_bounds(box_sphere):
     data, _ = box_sphere
     W = H = 32
