@@ -1254,7 +1254,7 @@ int fhp_pass_execute(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t spp
             }
             HIP_OR_FAIL(launchMptPrologue(a, st));
             HIP_OR_FAIL(hipGraphLaunch(p->graph, st));
-            HIP_OR_FAIL(launchMptEpilogue(a, reinterpret_cast<float4*>(d_color), h_stats ? dstats : nullptr, st));
+            HIP_OR_FAIL(launchMptEpilogue(s->dev, a, reinterpret_cast<float4*>(d_color), h_stats ? dstats : nullptr, st));
         }
         if ((rc = collectTimer(p, st))) return rc;
     } else {

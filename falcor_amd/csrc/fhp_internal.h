@@ -18,6 +18,9 @@ constexpr int kCounterStride = 32;
 // Counter words: [0, 32) closest-hit queue lengths by depth, [32, 64) visibility queue lengths by
 // slot, [64] frame number, [kHitCount, +32) compacted hit counts by depth.
 constexpr int kHitCount = 3 * kCounterStride;
+// Paths with a primary hit (the primary shading's compacted list); the last hit-count slot,
+// beyond any bounce depth, so the frame's counter reset clears it.
+constexpr int kPrimaryHitCount = kHitCount + kCounterStride - 1;
 constexpr int kCounterWords = 4 * kCounterStride;  // counters: [0, 32) ray queues by depth, [32, 64) shadow queues
 
 // One view of a batched frame: its scene (index into the batch's scene table), camera, frame
@@ -142,7 +145,8 @@ hipError_t launchMptFrame(const SceneDev& s, const MptArgs& a, const TraceScratc
 hipError_t launchMptBounces(const SceneDev& s, const MptArgs& a, const TraceScratch& ts, bool count,
                             unsigned long long* d_counts, hipStream_t stream, KernelTimer* timer);
 hipError_t launchMptPrologue(const MptArgs& a, hipStream_t stream);
-hipError_t launchMptEpilogue(const MptArgs& a, float4* out, unsigned long long* d_counts, hipStream_t stream);
+hipError_t launchMptEpilogue(const SceneDev& s, const MptArgs& a, float4* out, unsigned long long* d_counts,
+                             hipStream_t stream);
 
 // WARDiffPathTracer primal and adjoint (megakernel, one path per thread).
 // One view of a batched WARDiffPathTracer launch: its camera, frame seed base, dL/dI image

@@ -4,7 +4,8 @@
 //   primary rays (per pixel) -> closest hit -> shadePrimary (per path: emission, NEE, scatter)
 //   -> for depth 0..maxBounces: [visibility rays of the previous shading + closest hit, hits
 //   compacted] -> shade -> resolve.
-// Queues are compacted with a wave ballot + one atomic per wave; per-path state lives in
+// Queues are compacted with a wave ballot + one atomic per wave (per block in the shading
+// kernels); per-path state lives in
 // path-indexed slots so queue order never changes results.
 #include "fhp_internal.h"
 #include "fhp_trace.h"
@@ -21,6 +22,58 @@ __device__ __forceinline__ uint32_t waveAppend(bool pred, uint32_t* counter) {
         base = __shfl(base, leader);
     }
     return base + uint32_t(__popcll(mask & __lanemask_lt()));
+}
+
+// Block-wide appends: one atomic per block instead of one per wave (same-address atomics from
+// every CU serialise in the L2). Every thread of the block calls them the same number of times;
+// lds holds T / 64 + 1 words. blockAppendN reserves cnt consecutive slots per thread, in thread
+// order, and returns the first; blockAppend reserves one slot where pred holds.
+template <int T>
+__device__ __forceinline__ uint32_t blockAppendN(uint32_t cnt, uint32_t* counter, uint32_t* lds) {
+    static_assert(T % 64 == 0 && T <= 1024, "block of whole waves");
+    const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
+    uint32_t x = cnt;  // inclusive scan over the wave
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d);
+        if (lane >= uint32_t(d)) x += y;
+    }
+    if (lane == 63) lds[wave] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot = 0;
+        for (int w = 0; w < T / 64; w++) {
+            const uint32_t v = lds[w];
+            lds[w] = tot;
+            tot += v;
+        }
+        lds[T / 64] = tot ? atomicAdd(counter, tot) : 0u;
+    }
+    __syncthreads();
+    const uint32_t r = lds[T / 64] + lds[wave] + x - cnt;
+    __syncthreads();  // lds is reused by the next call
+    return r;
+}
+template <int T>
+__device__ __forceinline__ uint32_t blockAppend(bool pred, uint32_t* counter, uint32_t* lds) {
+    static_assert(T % 64 == 0 && T <= 1024, "block of whole waves");
+    const uint64_t mask = __ballot(pred);
+    const uint32_t wave = threadIdx.x >> 6;
+    if (__lane_id() == 0) lds[wave] = uint32_t(__popcll(mask));
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot = 0;
+        for (int w = 0; w < T / 64; w++) {
+            const uint32_t v = lds[w];
+            lds[w] = tot;
+            tot += v;
+        }
+        lds[T / 64] = tot ? atomicAdd(counter, tot) : 0u;
+    }
+    __syncthreads();
+    const uint32_t r = lds[T / 64] + lds[wave] + uint32_t(__popcll(mask & __lanemask_lt()));
+    __syncthreads();
+    return r;
 }
 
 
@@ -391,83 +444,94 @@ __device__ __forceinline__ void pushRay(const MptArgs& a, int q, uint32_t slot, 
     a.pathQ[q][slot] = pathId;
 }
 
-// Primary-hit shading, one thread per path (MinimalPathTracer.rt.slang:392-449).
+// Paths whose pixel has a primary hit, appended to pathQ[1] (free until the first bounce's
+// shading) with their count at counters[kPrimaryHitCount]: one thread per pixel appends its spp
+// path ids, so the primary shading runs on paths with a hit only and the resolve writes the
+// background of the other pixels. Launched with kCompactBlock threads per block (large blocks:
+// one atomic per block).
+constexpr int kShadeBlock = 256, kCompactBlock = 1024;
+template <bool MULTI>
+__global__ __launch_bounds__(kCompactBlock) void kCompactPrimary(SceneDev s0, MptArgs a) {
+    __shared__ uint32_t lds[kCompactBlock / 64 + 1];
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t cnt = 0, first = 0;
+    if (q < a.nPix && a.pixHit[q].x != kInvalid) {
+        const ViewCtx c = viewCtx<MULTI>(s0, a, q, false);
+        cnt = c.spp;
+        first = c.pathBase + (q - c.pixBase) * c.spp;
+    }
+    const uint32_t slot = blockAppendN<kCompactBlock>(cnt, &a.counters[kPrimaryHitCount], lds);
+    for (uint32_t k = 0; k < cnt; k++) a.pathQ[1][slot + k] = first + k;
+}
+
+// Primary-hit shading (MinimalPathTracer.rt.slang:392-449), one thread per path with a primary
+// hit (the compacted list of kCompactPrimary). The path state (throughput, last normal, sample
+// generator) is written for paths that continue only: nothing reads it for the others.
 template <bool MULTI, int LM>
-__global__ void kShadePrimary(SceneDev s0, MptArgs a) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= a.nPaths) return;
-    const ViewCtx c = viewCtx<MULTI>(s0, a, p, true);
-    const SceneDev& s = *c.s;
-    const uint32_t lp = p - c.pathBase;
-    const uint32_t pix = lp / c.spp, k = lp % c.spp;
-    const uint32_t px = pix % c.W, py = pix / c.W;
-    const uint4 h = a.pixHit[c.pixBase + pix];
-    bool cont = false, shadow = false;
-    f3 newO, newD, shO, shD, shC;
-    float shT = 0.f;
-    if (h.x != kInvalid) {
-        f3 o, d;
-        float tmin, tmax;
-        if (!MULTI && a.viewW) {
-            // getPrimaryRayDir with a bound viewW (MinimalPathTracer.rt.slang: -viewW).
-            const float4 v = a.viewW[pix];
-            d = mk3(-v.x, -v.y, -v.z);
-        } else {
-            primaryRay(*c.cam, px, py, c.W, c.H, o, d, tmin, tmax);
-        }
-        Shading sd = shadingFromHit(s, h.x, u2f(h.z), u2f(h.w), d);
-        Material m = loadMaterial(s, s.mats, s.texels, sd, sd, nullptr);
-        Rng rng = rngSeed(px, py, c.frame * c.spp + k);
-        f3 org = rayOriginFromShading(sd);
-        f3 Lp = mk3(0.f);
-        if (a.computeDirect) {
-            Lp = Lp + m.emission;
-            shadow = directSample<LM>(s, a, sd, m, rng, shO, shD, shT, shC);
-        }
-        f3 thp = mk3(1.f);
-        Scatter sc;
-        float pdf = 0.f;
-        uint32_t flags = 0;
-        if (matSample(sd, m, rng, sc, a.importance != 0)) {
-            thp = thp * sc.weight;
-            cont = anyPos(thp);
-            newO = sc.transmission ? rayOriginFromShading(sd, false) : org;  // computeRayOrigin(false) for transmission
-            newD = sc.wo;
-            pdf = sc.pdf;
-            flags = sc.delta ? 1u : 0u;
-        }
-        a.thp[p] = make_float4(thp.x, thp.y, thp.z, pdf);
-        a.prevN[p] = make_float4(sd.N.x, sd.N.y, sd.N.z, u2f(flags));
-        a.rng[p] = make_uint4(rng.s0, rng.s1, rng.s2, rng.s3);
-        a.Lp[p] = make_float4(Lp.x, Lp.y, Lp.z, 0.f);
-    } else {
-        // Background pixel (MinimalPathTracer.rt.slang:462-465).
-        f3 bg = mk3(0.f);
-        if (s.envTex && (s.envFlags & kEnvBackground)) {
+__global__ __launch_bounds__(kShadeBlock) void kShadePrimary(SceneDev s0, MptArgs a) {
+    __shared__ uint32_t lds[kShadeBlock / 64 + 1];
+    const uint32_t n = a.counters[kPrimaryHitCount];
+    const uint32_t G = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += G) {
+        const uint32_t i = base + threadIdx.x;
+        bool cont = false, shadow = false;
+        f3 newO, newD, shO, shD, shC;
+        float shT = 0.f;
+        uint32_t p = 0;
+        if (i < n) {
+            p = a.pathQ[1][i];
+            const ViewCtx c = viewCtx<MULTI>(s0, a, p, true);
+            const SceneDev& s = *c.s;
+            const uint32_t lp = p - c.pathBase;
+            const uint32_t pix = lp / c.spp, k = lp % c.spp;
+            const uint32_t px = pix % c.W, py = pix / c.W;
+            const uint4 h = a.pixHit[c.pixBase + pix];
             f3 o, d;
             float tmin, tmax;
             if (!MULTI && a.viewW) {
+                // getPrimaryRayDir with a bound viewW (MinimalPathTracer.rt.slang: -viewW).
                 const float4 v = a.viewW[pix];
                 d = mk3(-v.x, -v.y, -v.z);
             } else {
                 primaryRay(*c.cam, px, py, c.W, c.H, o, d, tmin, tmax);
             }
-            bg = envEval(s, d);
+            Shading sd = shadingFromHit(s, h.x, u2f(h.z), u2f(h.w), d);
+            Material m = loadMaterial(s, s.mats, s.texels, sd, sd, nullptr);
+            Rng rng = rngSeed(px, py, c.frame * c.spp + k);
+            f3 org = rayOriginFromShading(sd);
+            f3 Lp = mk3(0.f);
+            if (a.computeDirect) {
+                Lp = Lp + m.emission;
+                shadow = directSample<LM>(s, a, sd, m, rng, shO, shD, shT, shC);
+            }
+            Scatter sc;
+            if (matSample(sd, m, rng, sc, a.importance != 0)) {
+                const f3 thp = mk3(1.f) * sc.weight;
+                cont = anyPos(thp);
+                newO = sc.transmission ? rayOriginFromShading(sd, false) : org;  // computeRayOrigin(false) for transmission
+                newD = sc.wo;
+                if (cont) {
+                    a.thp[p] = make_float4(thp.x, thp.y, thp.z, sc.pdf);
+                    a.prevN[p] = make_float4(sd.N.x, sd.N.y, sd.N.z, u2f(sc.delta ? 1u : 0u));
+                    a.rng[p] = make_uint4(rng.s0, rng.s1, rng.s2, rng.s3);
+                }
+            }
+            a.Lp[p] = make_float4(Lp.x, Lp.y, Lp.z, 0.f);
+            a.Ls[p] = make_float4(0.f, 0.f, 0.f, 0.f);
         }
-        a.Lp[p] = make_float4(bg.x, bg.y, bg.z, 0.f);
+        const uint32_t slot = blockAppend<kShadeBlock>(cont, &a.counters[0], lds);
+        if (cont) pushRay(a, 0, slot, newO, newD, p);
+        const uint32_t sslot = blockAppend<kShadeBlock>(shadow, &a.counters[kCounterStride + 0], lds);
+        if (shadow) pushShadow(a, sslot, shO, shD, shT, shC, p, 1u);
     }
-    a.Ls[p] = make_float4(0.f, 0.f, 0.f, 0.f);
-    uint32_t slot = waveAppend(cont, &a.counters[0]);
-    if (cont) pushRay(a, 0, slot, newO, newD, p);
-    uint32_t sslot = waveAppend(shadow, &a.counters[kCounterStride + 0]);
-    if (shadow) pushShadow(a, sslot, shO, shD, shT, shC, p, 1u);
 }
 
 // handleHit for scatter depth `depth` (MinimalPathTracer.rt.slang:324-376), one thread per queued ray.
 // Runs on the compacted hits of the bounce (entry k: hit hitQ[k] of ray hitIdx[k]) unless
 // visibility rays have launches of their own, then on every queued ray (hitQ[i] of ray i).
 template <bool MULTI, int LM>
-__global__ void kShade(SceneDev s0, MptArgs a, uint32_t depth) {
+__global__ __launch_bounds__(kShadeBlock) void kShade(SceneDev s0, MptArgs a, uint32_t depth) {
+    __shared__ uint32_t lds[kShadeBlock / 64 + 1];
     const int qi = depth & 1, qo = qi ^ 1;
     const bool compact = !a.separateShadow;
     const uint32_t n = compact ? a.counters[kHitCount + depth] : a.counters[depth];
@@ -532,9 +596,9 @@ __global__ void kShade(SceneDev s0, MptArgs a, uint32_t depth) {
                 a.Ls[pathId] = make_float4(ls4.x + t4.x * Le.x, ls4.y + t4.y * Le.y, ls4.z + t4.z * Le.z, 0.f);
             }
         }
-        uint32_t slot = waveAppend(cont, &a.counters[depth + 1]);
+        const uint32_t slot = blockAppend<kShadeBlock>(cont, &a.counters[depth + 1], lds);
         if (cont) pushRay(a, qo, slot, newO, newD, pathId);
-        uint32_t sslot = waveAppend(shadow, &a.counters[kCounterStride + depth + 1]);
+        const uint32_t sslot = blockAppend<kShadeBlock>(shadow, &a.counters[kCounterStride + depth + 1], lds);
         if (shadow) pushShadow(a, sslot, shO, shD, shT, shC, pathId, 0u);
     }
 }
@@ -643,24 +707,48 @@ __global__ __launch_bounds__(B) void kTraceBounce(SceneDev s, MptArgs a, const f
     }
 }
 
-// outColor + radiance per sample, averaged over samples like AccumulatePass Single mode.
+// outColor + radiance per sample, averaged over samples like AccumulatePass Single mode. A pixel
+// without a primary hit shows the background (MinimalPathTracer.rt.slang:462-465: the env map
+// seen along the primary ray, else black) in each of its samples.
 template <bool MULTI>
-__global__ void kResolve(MptArgs a, float4* out) {
+__global__ void kResolve(SceneDev s0, MptArgs a, float4* out) {
     const uint32_t bpix = blockIdx.x * blockDim.x + threadIdx.x;
     if (bpix >= a.nPix) return;
-    uint32_t pix = bpix, pathBase = 0, spp = a.spp;
+    uint32_t pix = bpix, pathBase = 0, spp = a.spp, W = a.W, H = a.H;
+    const SceneDev* s = &s0;
+    const CameraDev* cam = &s0.cam;
     if (MULTI) {
         const ViewDev& v = a.views[findView(a.views, a.nViews, bpix, false)];
         pix = bpix - v.pixBase;
         pathBase = v.pathBase;
         spp = v.spp;
         out = v.out;
+        W = v.W;
+        H = v.H;
+        s = a.scenes + v.scene;
+        cam = &v.cam;
     }
     f3 sum = mk3(0.f);
-    for (uint32_t k = 0; k < spp; k++) {
-        const uint32_t p = pathBase + pix * spp + k;
-        const float4 lp = a.Lp[p], ls = a.Ls[p];
-        sum = sum + (mk3(lp.x, lp.y, lp.z) + mk3(ls.x, ls.y, ls.z));
+    if (a.pixHit[bpix].x != kInvalid) {
+        for (uint32_t k = 0; k < spp; k++) {
+            const uint32_t p = pathBase + pix * spp + k;
+            const float4 lp = a.Lp[p], ls = a.Ls[p];
+            sum = sum + (mk3(lp.x, lp.y, lp.z) + mk3(ls.x, ls.y, ls.z));
+        }
+    } else {
+        f3 bg = mk3(0.f);
+        if (s->envTex && (s->envFlags & kEnvBackground)) {
+            f3 o, d;
+            float tmin, tmax;
+            if (!MULTI && a.viewW) {
+                const float4 v = a.viewW[pix];
+                d = mk3(-v.x, -v.y, -v.z);
+            } else {
+                primaryRay(*cam, pix % W, pix / W, W, H, o, d, tmin, tmax);
+            }
+            bg = envEval(*s, d);
+        }
+        for (uint32_t k = 0; k < spp; k++) sum = sum + (bg + mk3(0.f));
     }
     const float w = 1.0f / float(spp);
     out[pix] = make_float4(sum.x * w, sum.y * w, sum.z * w, 1.f);
@@ -686,9 +774,9 @@ hipError_t launchMptPrologue(const MptArgs& a, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t launchMptEpilogue(const MptArgs& a, float4* out, unsigned long long* d_counts, hipStream_t st) {
+hipError_t launchMptEpilogue(const SceneDev& s, const MptArgs& a, float4* out, unsigned long long* d_counts, hipStream_t st) {
     const uint32_t npix = a.nPix;
-    hipLaunchKernelGGL(kResolve<false>, dim3((npix + 255) / 256), dim3(256), 0, st, a, out);
+    hipLaunchKernelGGL(kResolve<false>, dim3((npix + 255) / 256), dim3(256), 0, st, s, a, out);
     if (d_counts) hipLaunchKernelGGL(kRayCounts, dim3(1), dim3(64), 0, st, a, a.maxBounces + 1, d_counts);
     return hipGetLastError();
 }
@@ -699,7 +787,7 @@ hipError_t launchMptFrame(const SceneDev& s, const MptArgs& a, const TraceScratc
     if (e != hipSuccess) return e;
     e = launchMptBounces(s, a, ts, count, d_counts, st, tm);
     if (e != hipSuccess) return e;
-    return launchMptEpilogue(a, out, d_counts, st);
+    return launchMptEpilogue(s, a, out, d_counts, st);
 }
 
 template <bool MULTI>
@@ -708,7 +796,7 @@ hipError_t mptBounces(const SceneDev& s, const MptArgs& a, const TraceScratch& t
     KernelTimer dummy;
     if (!tm) tm = &dummy;
     const uint32_t npix = a.nPix;
-    const int T = 256;
+    const int T = kShadeBlock;
     hipLaunchKernelGGL(kZeroU32, dim3(1), dim3(2 * kCounterStride), 0, st, a.counters, 2u * kCounterStride);
     hipLaunchKernelGGL(kZeroU32, dim3(1), dim3(kCounterStride), 0, st, a.counters + kHitCount, uint32_t(kCounterStride));
     const bool vbuf = !MULTI && a.vbuf;
@@ -766,14 +854,16 @@ hipError_t mptBounces(const SceneDev& s, const MptArgs& a, const TraceScratch& t
                                       (npix + tsc.block - 1) / tsc.block);
     if (e != hipSuccess) return e;
     tm->begin(KK_SHADE, st);
-    if (a.lightMode == kLightsNone)
-        hipLaunchKernelGGL((kShadePrimary<MULTI, kLightsNone>), dim3((a.nPaths + T - 1) / T), dim3(T), 0, st, s, a);
-    else if (a.lightMode == kLightsEmissive)
-        hipLaunchKernelGGL((kShadePrimary<MULTI, kLightsEmissive>), dim3((a.nPaths + T - 1) / T), dim3(T), 0, st, s, a);
-    else
-        hipLaunchKernelGGL((kShadePrimary<MULTI, kLightsAny>), dim3((a.nPaths + T - 1) / T), dim3(T), 0, st, s, a);
-    tm->end(st);
     const uint32_t shadeBlocks = std::min<uint32_t>((a.nPaths + T - 1) / T, uint32_t(numCUs()) * 32);
+    hipLaunchKernelGGL(kCompactPrimary<MULTI>, dim3((npix + kCompactBlock - 1) / kCompactBlock), dim3(kCompactBlock), 0, st,
+                       s, a);
+    if (a.lightMode == kLightsNone)
+        hipLaunchKernelGGL((kShadePrimary<MULTI, kLightsNone>), dim3(shadeBlocks), dim3(T), 0, st, s, a);
+    else if (a.lightMode == kLightsEmissive)
+        hipLaunchKernelGGL((kShadePrimary<MULTI, kLightsEmissive>), dim3(shadeBlocks), dim3(T), 0, st, s, a);
+    else
+        hipLaunchKernelGGL((kShadePrimary<MULTI, kLightsAny>), dim3(shadeBlocks), dim3(T), 0, st, s, a);
+    tm->end(st);
     for (uint32_t b = 0; b <= a.maxBounces; b++) {
         // Closest-hit rays of bounce b with the visibility rays produced alongside them.
         if ((e = (a.separateShadow ? traceQ(a.rayQ[b & 1], a.counters + b, 0u, a.hitQ, 0xffffffffu)
@@ -803,7 +893,7 @@ hipError_t launchMptBatch(const MptArgs& a, const TraceScratch& ts, bool count, 
     const SceneDev none{};
     hipError_t e = mptBounces<true>(none, a, ts, count, d_counts, st, tm);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(kResolve<true>, dim3((a.nPix + 255) / 256), dim3(256), 0, st, a, nullptr);
+    hipLaunchKernelGGL(kResolve<true>, dim3((a.nPix + 255) / 256), dim3(256), 0, st, none, a, nullptr);
     if (d_counts) hipLaunchKernelGGL(kRayCounts, dim3(1), dim3(64), 0, st, a, a.maxBounces + 1, d_counts);
     return hipGetLastError();
 }
