@@ -54,6 +54,7 @@ METRIC = "Mrays/s fwd; ms per fwd+adjoint iter at 512² 16spp; grad rel-err vs F
 HBM_PEAK_GBS = 8000.0
 NODE_BYTES, TRI_BYTES, RAY_BYTES, HIT_BYTES, VIS_RECORD_BYTES = 128, 48, 32, 16, 48
 TUNE_TAG = "sphere_field_1920x1080_4spp_b6_lightbvh_nee"
+BATCH_TUNE_TAG = "batch_64views_16scenes_512x512_b6_lightbvh_nee"
 
 
 TRAFFIC_PROFILE = ROOT / "profiles" / "kTraceBounce_sphere_field_1920x1080_4spp.md"
@@ -183,11 +184,13 @@ def forward_bench(steps: int, warmup: int, tune: bool, tune_out: str | None):
     return res
 
 
-def batch_bench(reps: int = 2):
+def batch_bench(reps: int = 2, tune: bool = False, tune_out: str | None = None):
     """Batched forward: 64 views (512x512, spp in {1,2,4,8,16}, 6 bounces, LightBVH NEE) over 16
-    sphere fields of 10k-2M triangles, rendered in one launch sequence, against the same views
+    sphere fields of 10k-2M triangles, rendered in one launch sequence (captured bounce loop)
+    with the traversal launch the autotuner chose for this workload, against the same views
     rendered one after the other (one launch sequence per view)."""
     import torch
+    from falcor_amd import autotune
     from falcor_amd import scene as S
     from falcor_amd.render_graph import MinimalPathTracer, Scene
 
@@ -200,6 +203,19 @@ def batch_bench(reps: int = 2):
     for v in vs:
         v["out"] = torch.empty((v["height"], v["width"], 4), device="cuda")
     p = MinimalPathTracer({"maxBounces": 6, "useEmissiveNEE": True, "emissiveSampler": "LightBVH"})
+    launch = autotune.load("MinimalPathTracer", BATCH_TUNE_TAG)
+    if tune:
+        # Untimed: every traversal launch configuration on the whole batch.
+        tr = autotune.tune(p, lambda i: p.render_batch(vs), reps=2, warmup=1)
+        launch = dict(tr["best"])
+        table = sorted(tr["table"], key=lambda r: r[1])
+        log("batch autotune (fastest first): " + "; ".join(
+            f"{c['traceBlockSize']}/{c['traceStackDepth']}/{c['traceBlocksPerCU']}: {ms:.2f} ms" for c, ms in table))
+        if tune_out:
+            autotune.save("MinimalPathTracer", BATCH_TUNE_TAG, launch, path=tune_out)
+    if launch is not None:
+        autotune.apply(p, launch)
+    launch = {k: int(p.get_property(k)) for k in autotune.LAUNCH_KEYS}
     _, st = p.render_batch(vs, stats=True)  # warm-up; also allocates the batch buffers
     rays = int(st[0]) + int(st[1])
     torch.cuda.synchronize()
@@ -209,6 +225,7 @@ def batch_bench(reps: int = 2):
     torch.cuda.synchronize()
     batched = (time.perf_counter() - t) / reps * 1e3
     q = MinimalPathTracer({"maxBounces": 6, "useEmissiveNEE": True, "emissiveSampler": "LightBVH"})
+    autotune.apply(q, launch)
     for v in vs:
         q.render_batch([v])
     torch.cuda.synchronize()
@@ -223,7 +240,7 @@ def batch_bench(reps: int = 2):
         f"{build:.1f}s), {paths / 1e6:.1f}M paths, {rays / 1e6:.1f}M rays: batched {batched:.1f} ms "
         f"({rays / batched / 1e3:.1f} Mrays/s), per-view {seq:.1f} ms ({rays / seq / 1e3:.1f} Mrays/s)")
     return dict(views=len(views), scenes=len(datas), paths=paths, rays=rays, ms_batched=batched, ms_per_view_loop=seq,
-                mrays_batched=rays / batched / 1e3, speedup=seq / batched)
+                mrays_batched=rays / batched / 1e3, speedup=seq / batched, traversal_launch=launch)
 
 
 def cpu_baseline(data, W, H, spp, B, budget_s: float = 12.0, max_frames: int = 12):
@@ -472,7 +489,7 @@ def main():
     _lib.load()
     if args.only:
         r = {"forward": lambda: forward_bench(args.steps, args.warmup, args.tune, args.tune_out),
-             "batch": batch_bench, "diff": diff_bench,
+             "batch": lambda: batch_bench(tune=args.tune, tune_out=args.tune_out), "diff": diff_bench,
              "war-iter": lambda: diff_bench(iters=1, fd_texels=0, bounces=(1,), plain_loop=False),
              "geom": lambda: geom_bench(args.geom_iters)}[args.only]()
         log(f"{args.only}: " + json.dumps({k: v for k, v in r.items() if isinstance(v, (int, float, str))}))
@@ -511,7 +528,8 @@ def main():
                      "launches_per_step": fwd["trace_launches_per_step"]},
     }
     if not args.skip_batch:
-        line["batch"] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in batch_bench().items()}
+        line["batch"] = {k: (round(v, 3) if isinstance(v, float) else v)
+                         for k, v in batch_bench(tune=args.tune, tune_out=args.tune_out).items()}
     if not args.skip_diff:
         d = diff_bench()
         line["diff"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in d.items() if k != "loop"}

@@ -194,16 +194,32 @@ struct FhpPass_ {
     hipGraphExec_t graph = nullptr;
     hipStream_t capStream = nullptr;
     std::vector<uint8_t> graphKey;
+    // Captured bounce loop of the batch mode, keyed by its arguments (the view and scene tables
+    // are read from device memory, so new cameras, frames and outputs replay the same graph).
+    hipGraphExec_t batchGraph = nullptr;
+    std::vector<uint8_t> batchKey;
+    // Pinned staging of the batch tables; stageDone marks the end of the last copy out of it.
+    void* stage = nullptr;
+    size_t stageBytes = 0;
+    hipEvent_t stageDone = nullptr;
     void dropGraph() {
         if (graph) (void)hipGraphExecDestroy(graph);
         graph = nullptr;
         graphKey.clear();
+        if (batchGraph) (void)hipGraphExecDestroy(batchGraph);
+        batchGraph = nullptr;
+        batchKey.clear();
     }
     ~FhpPass_() {
         if (timer.created)
             for (auto& e : timer.ev) (void)hipEventDestroy(e);
         dropGraph();
         if (capStream) (void)hipStreamDestroy(capStream);
+        if (stageDone) {
+            (void)hipEventSynchronize(stageDone);
+            (void)hipEventDestroy(stageDone);
+        }
+        if (stage) (void)hipHostFree(stage);
     }
     double get(const char* k) const {
         auto it = props.find(k);
@@ -389,7 +405,7 @@ int ensurePassBuffersN(FhpPass p, uint64_t n, uint64_t npix) {
 // MinimalPathTracer arguments of a frame from the pass properties and the scene (everything but
 // the wavefront buffers). Returns FHP_OK or an argument error.
 int mptFrameArgs(const FhpPass_* p, const FhpScene_* s, uint32_t W, uint32_t H, uint32_t spp, uint32_t frame, MptArgs& a) {
-    a = MptArgs{};
+    std::memset(&a, 0, sizeof(a));  // padding too: the bytes key the captured graph
     a.W = W; a.H = H; a.spp = spp; a.frame = frame;
     a.maxBounces = uint32_t(p->get("maxBounces"));
     a.computeDirect = p->get("computeDirect") != 0;
@@ -1439,12 +1455,28 @@ int fhp_pass_execute_batch(FhpPass p, uint32_t nv, const FhpView* views, uint64_
     if (rc) return rc;
     HIP_OR_FAIL(p->viewTab.alloc(vt.size() * sizeof(ViewDev)));
     HIP_OR_FAIL(p->sceneTab.alloc(stab.size() * sizeof(SceneDev)));
-    HIP_OR_FAIL(hipMemcpyAsync(p->viewTab.p, vt.data(), vt.size() * sizeof(ViewDev), hipMemcpyHostToDevice, st));
-    HIP_OR_FAIL(hipMemcpyAsync(p->sceneTab.p, stab.data(), stab.size() * sizeof(SceneDev), hipMemcpyHostToDevice, st));
+    // Tables go through pinned staging memory, so the copies are asynchronous and the call does
+    // not wait for the batch; the previous call's copies must have left the staging buffer first.
+    const size_t vb = vt.size() * sizeof(ViewDev), sb = stab.size() * sizeof(SceneDev);
+    if (p->stageDone) HIP_OR_FAIL(hipEventSynchronize(p->stageDone));
+    else HIP_OR_FAIL(hipEventCreateWithFlags(&p->stageDone, hipEventDisableTiming));
+    if (p->stageBytes < vb + sb) {
+        if (p->stage) HIP_OR_FAIL(hipHostFree(p->stage));
+        p->stage = nullptr;
+        p->stageBytes = 0;
+        HIP_OR_FAIL(hipHostMalloc(&p->stage, vb + sb, hipHostMallocDefault));
+        p->stageBytes = vb + sb;
+    }
+    std::memcpy(p->stage, vt.data(), vb);
+    std::memcpy(static_cast<uint8_t*>(p->stage) + vb, stab.data(), sb);
+    HIP_OR_FAIL(hipMemcpyAsync(p->viewTab.p, p->stage, vb, hipMemcpyHostToDevice, st));
+    HIP_OR_FAIL(hipMemcpyAsync(p->sceneTab.p, static_cast<uint8_t*>(p->stage) + vb, sb, hipMemcpyHostToDevice, st));
+    HIP_OR_FAIL(hipEventRecord(p->stageDone, st));
     const bool count = p->get("countTraversal") != 0;
     unsigned long long* dstats = p->stats.as<unsigned long long>();
     if (h_stats) HIP_OR_FAIL(launchFill(reinterpret_cast<float*>(dstats), 12, 0.f, st));
-    MptArgs a{};
+    MptArgs a;
+    std::memset(&a, 0, sizeof(a));  // padding too: the bytes key the captured graph
     a.maxBounces = uint32_t(p->get("maxBounces"));
     a.computeDirect = p->get("computeDirect") != 0;
     a.importance = p->get("useImportanceSampling") != 0;
@@ -1474,16 +1506,40 @@ int fhp_pass_execute_batch(FhpPass p, uint32_t nv, const FhpView* views, uint64_
     if ((rc = prepareTimer(p))) return rc;
     TraceScratch ts;
     if ((rc = passTrace(deepest, p, ts))) return rc;
-    HIP_OR_FAIL(launchMptBatch(a, ts, count, h_stats ? dstats : nullptr, st, &p->timer));
+    const bool graph = p->get("useGraph") != 0 && !count && !p->timer.on;
+    if (!graph) {
+        HIP_OR_FAIL(launchMptBatch(a, ts, count, h_stats ? dstats : nullptr, st, &p->timer));
+    } else {
+        // Captured bounce loop, keyed by the arguments and the traversal configuration (the
+        // tables' contents are read from device memory at run time).
+        std::vector<uint8_t> key(sizeof(MptArgs) + sizeof(TraceScratch));
+        std::memcpy(key.data(), &a, sizeof(MptArgs));
+        std::memcpy(key.data() + sizeof(MptArgs), &ts, sizeof(TraceScratch));
+        if (key != p->batchKey) {
+            if (p->batchGraph) (void)hipGraphExecDestroy(p->batchGraph);
+            p->batchGraph = nullptr;
+            p->batchKey.clear();
+            if (!p->capStream) HIP_OR_FAIL(hipStreamCreateWithFlags(&p->capStream, hipStreamNonBlocking));
+            hipGraph_t g = nullptr;
+            HIP_OR_FAIL(hipStreamBeginCapture(p->capStream, hipStreamCaptureModeThreadLocal));
+            hipError_t ce = launchMptBatchBounces(a, ts, false, nullptr, p->capStream, nullptr);
+            hipError_t ee = hipStreamEndCapture(p->capStream, &g);
+            HIP_OR_FAIL(ce);
+            HIP_OR_FAIL(ee);
+            hipError_t ie = hipGraphInstantiate(&p->batchGraph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            HIP_OR_FAIL(ie);
+            p->batchKey = key;
+        }
+        HIP_OR_FAIL(hipGraphLaunch(p->batchGraph, st));
+        HIP_OR_FAIL(launchMptBatchEpilogue(a, h_stats ? dstats : nullptr, st));
+    }
     if ((rc = collectTimer(p, st))) return rc;
     if (h_stats) {
         unsigned long long hs[6];
         HIP_OR_FAIL(hipMemcpyAsync(hs, dstats, 48, hipMemcpyDeviceToHost, st));
         HIP_OR_FAIL(hipStreamSynchronize(st));
         for (int i = 0; i < 6; i++) h_stats[i] = hs[i];
-    } else {
-        // The tables above are copied from this call's host vectors.
-        HIP_OR_FAIL(hipStreamSynchronize(st));
     }
     return FHP_OK;
 }

@@ -138,6 +138,10 @@ hipError_t launchTraceRays(const SceneDev& s, const float4* rays, uint32_t n, ui
 // Full frame: frame-number store, bounce loop, resolve, optional ray counts.
 hipError_t launchMptBatch(const MptArgs& a, const TraceScratch& ts, bool count, unsigned long long* d_counts,
                           hipStream_t stream, KernelTimer* timer);
+// The batch's bounce loop (capturable) and its resolve / ray-count epilogue.
+hipError_t launchMptBatchBounces(const MptArgs& a, const TraceScratch& ts, bool count, unsigned long long* d_counts,
+                                 hipStream_t stream, KernelTimer* timer);
+hipError_t launchMptBatchEpilogue(const MptArgs& a, unsigned long long* d_counts, hipStream_t stream);
 hipError_t launchMptFrame(const SceneDev& s, const MptArgs& a, const TraceScratch& ts, float4* out, bool count,
                           unsigned long long* d_counts, hipStream_t stream, KernelTimer* timer);
 // Pieces used with graph capture: the bounce loop (capturable, no host-varying arguments) and

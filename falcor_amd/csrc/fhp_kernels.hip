@@ -887,15 +887,24 @@ hipError_t launchMptBounces(const SceneDev& s, const MptArgs& a, const TraceScra
     return mptBounces<false>(s, a, ts, count, d_counts, st, tm);
 }
 
-// A batch of views in one launch sequence (no graph: the view table changes per call).
-hipError_t launchMptBatch(const MptArgs& a, const TraceScratch& ts, bool count, unsigned long long* d_counts, hipStream_t st,
-                          KernelTimer* tm) {
+// A batch of views in one launch sequence. The kernels read the view and scene tables from device
+// memory, so the bounce loop can be captured once per batch layout and replayed with new tables.
+hipError_t launchMptBatchBounces(const MptArgs& a, const TraceScratch& ts, bool count, unsigned long long* d_counts,
+                                 hipStream_t st, KernelTimer* tm) {
     const SceneDev none{};
-    hipError_t e = mptBounces<true>(none, a, ts, count, d_counts, st, tm);
-    if (e != hipSuccess) return e;
+    return mptBounces<true>(none, a, ts, count, d_counts, st, tm);
+}
+hipError_t launchMptBatchEpilogue(const MptArgs& a, unsigned long long* d_counts, hipStream_t st) {
+    const SceneDev none{};
     hipLaunchKernelGGL(kResolve<true>, dim3((a.nPix + 255) / 256), dim3(256), 0, st, none, a, nullptr);
     if (d_counts) hipLaunchKernelGGL(kRayCounts, dim3(1), dim3(64), 0, st, a, a.maxBounces + 1, d_counts);
     return hipGetLastError();
+}
+hipError_t launchMptBatch(const MptArgs& a, const TraceScratch& ts, bool count, unsigned long long* d_counts, hipStream_t st,
+                          KernelTimer* tm) {
+    hipError_t e = launchMptBatchBounces(a, ts, count, d_counts, st, tm);
+    if (e != hipSuccess) return e;
+    return launchMptBatchEpilogue(a, d_counts, st);
 }
 
 // ------------------------------------------------------------------ small utilities

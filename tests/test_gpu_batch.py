@@ -49,6 +49,36 @@ def test_batch_views_equal_single_view_renders(gpu, nee):
     assert int(st[0]) == rays
 
 
+def test_batch_graph_replays_equal_eager_batches(gpu):
+    """The batch's bounce loop is captured once per batch layout (useGraph, the default) and
+    replayed with the new view table of every call: cameras, frames and outputs change between
+    replays, a differently shaped batch recaptures, and every replay equals the eager launch
+    sequence (useGraph = 0) bit for bit, with eager work and synchronisations in between."""
+    import torch
+    from falcor_amd.render_graph import MinimalPathTracer, Scene
+    box = Scene(S.closed_box(3))
+    field = Scene(S.sphere_field(5, num_spheres=120, num_emissive=300, box=25.0))
+    props = {"maxBounces": 4, "useEmissiveNEE": True}
+    pg = MinimalPathTracer(dict(props, useGraph=1))
+    pe = MinimalPathTracer(dict(props, useGraph=0))
+
+    def views(step, n):
+        cam = S.Camera((0.05 * step, 0.3, 1.1), (0.0, 0.25, 0.0), (0, 1, 0), math.radians(35.0 + step), 48 / 40)
+        vs = [dict(scene=box, width=48, height=40, spp=2, frame=step, camera=cam),
+              dict(scene=field, width=40, height=32, spp=3, frame=7 * step + 1),
+              dict(scene=box, width=32, height=24, spp=1, frame=step + 3)]
+        return vs[:n]
+
+    for step, n in enumerate((3, 3, 2, 3, 3)):
+        got = pg.render_batch(views(step, n))
+        torch.cuda.synchronize()
+        ref = pe.render_batch(views(step, n))
+        torch.cuda.synchronize()
+        for i in range(n):
+            assert torch.equal(got[i], ref[i]), f"call {step} view {i}"
+            assert float(got[i].abs().sum()) > 0
+
+
 def test_batch_rejects_bad_views(gpu):
     from falcor_amd import _lib
     from falcor_amd.render_graph import MinimalPathTracer, Scene, WARDiffPathTracer
