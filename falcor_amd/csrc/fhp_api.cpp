@@ -168,6 +168,7 @@ struct FhpPass_ {
     // Wavefront buffers (forward pass).
     DevBuf thp, prevN, rng, Lp, Ls, rayQ0, rayQ1, pathQ0, pathQ1, hitQ, hitIdx, shQ, pixHit, counters, stats;
     DevBuf viewTab, sceneTab;  // batch mode tables
+    DevBuf pathView;           // batch mode: view index per path
     DevBuf warViewTab;         // batched WARDiffPathTracer views
     DevBuf warLog;             // WARDiffPathTracer adjoint hit log
     DevBuf auxLog;             // WARDiffPathTracer vertex-gradient aux-hit log
@@ -1413,6 +1414,7 @@ int fhp_pass_execute_batch(FhpPass p, uint32_t nv, const FhpView* views, uint64_
     if (p->kind == "WARDiffPathTracer") return warExecuteBatch(p, nv, views, h_stats, stream);
     if (p->kind != "MinimalPathTracer")
         return fail(FHP_EINVAL, "batched execution is available for MinimalPathTracer and WARDiffPathTracer");
+    if (nv > 65535) return fail(FHP_EINVAL, "at most 65535 views per batch");
     hipStream_t st = hipStream_t(stream);
     std::vector<ViewDev> vt(nv);
     std::vector<FhpScene> scenes;
@@ -1496,6 +1498,8 @@ int fhp_pass_execute_batch(FhpPass p, uint32_t nv, const FhpView* views, uint64_
     a.views = p->viewTab.as<ViewDev>();
     a.scenes = p->sceneTab.as<SceneDev>();
     a.nViews = nv;
+    HIP_OR_FAIL(p->pathView.alloc(size_t(nPaths) * 2));
+    a.pathView = p->pathView.as<uint16_t>();
     a.thp = p->thp.as<float4>(); a.prevN = p->prevN.as<float4>(); a.rng = p->rng.as<uint4>();
     a.Lp = p->Lp.as<float4>(); a.Ls = p->Ls.as<float4>();
     a.rayQ[0] = p->rayQ0.as<float4>(); a.rayQ[1] = p->rayQ1.as<float4>();

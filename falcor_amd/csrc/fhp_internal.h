@@ -61,6 +61,7 @@ struct MptArgs {
     // and scenes are device tables; W, H, spp and frame above are unused.
     const ViewDev* views;
     const SceneDev* scenes;
+    uint16_t* pathView;   // view index of each path with a primary hit (written by the primary compaction)
     uint32_t nViews;
     uint32_t nPix;        // pixels over all views (W * H for one view)
     // Bound inputs of a single-view frame (MinimalPathTracer.cpp:48-59): the primary hits as a

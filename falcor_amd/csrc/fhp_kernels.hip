@@ -139,10 +139,12 @@ __device__ __forceinline__ uint32_t findView(const ViewDev* v, uint32_t n, uint3
     }
     return lo;
 }
+// The view of path x (byPath: a path with a primary hit, whose view index kCompactPrimary stored
+// in a.pathView) or of pixel x (binary search).
 template <bool MULTI>
 __device__ __forceinline__ ViewCtx viewCtx(const SceneDev& s0, const MptArgs& a, uint32_t x, bool byPath) {
     if (!MULTI) return ViewCtx{&s0, &s0.cam, a.W, a.H, a.spp, a.counters[2 * kCounterStride], 0u, 0u, nullptr};
-    const ViewDev& v = a.views[findView(a.views, a.nViews, x, byPath)];
+    const ViewDev& v = a.views[byPath ? uint32_t(a.pathView[x]) : findView(a.views, a.nViews, x, false)];
     return ViewCtx{a.scenes + v.scene, &v.cam, v.W, v.H, v.spp, v.frame, v.pixBase, v.pathBase, v.out};
 }
 
@@ -454,14 +456,18 @@ template <bool MULTI>
 __global__ __launch_bounds__(kCompactBlock) void kCompactPrimary(SceneDev s0, MptArgs a) {
     __shared__ uint32_t lds[kCompactBlock / 64 + 1];
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t cnt = 0, first = 0;
+    uint32_t cnt = 0, first = 0, view = 0;
     if (q < a.nPix && a.pixHit[q].x != kInvalid) {
+        if (MULTI) view = findView(a.views, a.nViews, q, false);
         const ViewCtx c = viewCtx<MULTI>(s0, a, q, false);
         cnt = c.spp;
         first = c.pathBase + (q - c.pixBase) * c.spp;
     }
     const uint32_t slot = blockAppendN<kCompactBlock>(cnt, &a.counters[kPrimaryHitCount], lds);
-    for (uint32_t k = 0; k < cnt; k++) a.pathQ[1][slot + k] = first + k;
+    for (uint32_t k = 0; k < cnt; k++) {
+        a.pathQ[1][slot + k] = first + k;
+        if (MULTI) a.pathView[first + k] = uint16_t(view);  // read by every later per-path view lookup
+    }
 }
 
 // Primary-hit shading (MinimalPathTracer.rt.slang:392-449), one thread per path with a primary
@@ -664,9 +670,12 @@ __global__ __launch_bounds__(B) void kTraceBounce(SceneDev s, MptArgs a, const f
         const float4 r0 = q[0], r1 = q[1];
         GeomRef g{s.bvh, s.triPos};
         if (MULTI) {
+            // Visibility and bounce rays belong to paths with a primary hit (view index stored per
+            // path); primary rays (no path ids) are found by pixel.
             const bool byPath = vis || pathIds != nullptr;
             const uint32_t key = vis ? f2u(r0.w) : (pathIds ? pathIds[j] : j);
-            const SceneDev& sv = a.scenes[a.views[findView(a.views, a.nViews, key, byPath)].scene];
+            const uint32_t view = byPath ? uint32_t(a.pathView[key]) : findView(a.views, a.nViews, key, false);
+            const SceneDev& sv = a.scenes[a.views[view].scene];
             g = GeomRef{sv.bvh, sv.triPos};
         }
         TravState r;

@@ -264,8 +264,11 @@ int fhp_pass_execute_noalloc(FhpPass pass, FhpScene scene, uint32_t width, uint3
  * h_camera is 14 floats as in FhpSceneDesc, or NULL for the scene's camera; each view is
  * resolved into its d_color (height * width * 4 floats) exactly as fhp_pass_execute would render
  * it. No counterpart in the reference (one graph execution per view there,
- * RenderGraph::execute, Source/Falcor/RenderGraph/RenderGraph.cpp). Synchronises the stream
- * before returning (the view tables are staged from host memory). */
+ * RenderGraph::execute, Source/Falcor/RenderGraph/RenderGraph.cpp). MinimalPathTracer: at most
+ * 65535 views; with useGraph the bounce loop is captured once per batch layout (view count,
+ * path and pixel totals, traversal launch) and replayed with each call's view table, which is
+ * staged through pinned memory, so the call does not wait for the batch (only for the previous
+ * call's table copy). WARDiffPathTracer synchronises the stream before returning. */
 typedef struct FhpView {
     FhpScene scene;
     const float* h_camera;
