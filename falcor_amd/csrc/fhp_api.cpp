@@ -172,6 +172,7 @@ struct FhpPass_ {
     DevBuf warViewTab;         // batched WARDiffPathTracer views
     DevBuf warLog;             // WARDiffPathTracer adjoint hit log
     DevBuf auxLog;             // WARDiffPathTracer vertex-gradient aux-hit log
+    DevBuf vtxLog;             // WARDiffPathTracer vertex-gradient path vertex records
     uint32_t cap = 0, capPix = 0;
     KernelTimer timer;
     // AccumulatePass state: running sums of the current precision mode, frames accumulated,
@@ -253,6 +254,8 @@ const std::map<std::string, double>& mptDefaults() {
 }
 // WARDiffPathTracer's DiffMode (WARDiffPathTracer.h:56-62) and DiffVariableType (SharedTypes.slang:78-84).
 constexpr int kDiffPrimal = 0, kDiffBackward = 1, kDiffForwardDebug = 2, kDiffBackwardDebug = 3;
+// Deepest path of the vertex-gradient kernel (its per-thread vertex records grow with it).
+constexpr int kMaxGeomBounces = 8;
 constexpr int kDiffVarNone = 0, kDiffVarMaterial = 1, kDiffVarGeometryTranslation = 2;
 const std::map<std::string, double>& warDefaults() {
     // WARDiffPathTracer.h:120-170 (StaticParams) and Params.slang.
@@ -1739,8 +1742,10 @@ int warBackward(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t spp, uin
         return fail(FHP_EINVAL, "frame too large (width * height * spp paths)");
     const bool war = p->get("useWAR") != 0;
     if (war) {
-        if (p->get("maxBounces") != 0)
-            return fail(FHP_EINVAL, "useWAR=1 (vertex gradients) supports maxBounces = 0 (direct illumination) only");
+        if (p->get("maxBounces") > kMaxGeomBounces)
+            return fail(FHP_EINVAL, "useWAR=1 (vertex gradients) supports maxBounces up to " + std::to_string(kMaxGeomBounces));
+        if (s->dev.hasTransmission)
+            return fail(FHP_EINVAL, "useWAR=1 (vertex gradients) supports reflective materials only (no transmission lobes)");
         if (!g->withMesh) return fail(FHP_ESTATE, "useWAR=1 needs mesh gradients (fhp_grads_set_mesh)");
         const double aux = p->get("auxSampleCount");
         if (aux < 1 || aux > 64 || double(uint32_t(aux)) != aux) return fail(FHP_EINVAL, "auxSampleCount must be in [1, 64]");
@@ -1798,10 +1803,16 @@ int warBackward(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t spp, uin
         ga.rays = h_stats ? dstats + 2 : nullptr;
         ga.views = a.views;
         ga.nViews = a.nViews;
+        // The kernel's grid has at most ts.gridBlocks blocks of ts.block threads.
         if (p->get("useHitLog") != 0) {
-            // The kernel's grid has at most ts.gridBlocks blocks of ts.block threads.
             HIP_OR_FAIL(p->auxLog.alloc(size_t(3) * ga.aux * ts.gridBlocks * ts.block * 4));
             ga.auxLog = p->auxLog.as<uint32_t>();
+        }
+        ga.maxBounces = a.maxBounces;
+        if (a.maxBounces > 0) {
+            // maxBounces = 0 runs the one-pass direct-illumination instances, which keep no records.
+            HIP_OR_FAIL(p->vtxLog.alloc(size_t(kGeomVertexWords) * (a.maxBounces + 2) * ts.gridBlocks * ts.block * 4));
+            ga.vtxLog = p->vtxLog.as<uint32_t>();
         }
         HIP_OR_FAIL(launchWarGeom(s->dev, ga, ts, st, &p->timer));
     }

@@ -218,7 +218,12 @@ struct GeomArgs {
     // Aux-hit log: 3 * aux words per thread of the persistent grid ([word][thread]), so each
     // warp's second loop over its aux rays reads the first loop's hits; null: both loops trace.
     uint32_t* auxLog;
+    uint32_t maxBounces;
+    // Vertex records of the forward pass: kGeomVertexWords * (maxBounces + 2) words per thread of
+    // the persistent grid ([word][thread]), read back by the reverse sweep.
+    uint32_t* vtxLog;
 };
+constexpr uint32_t kGeomVertexWords = 17;
 hipError_t launchWarGeom(const SceneDev& s, const GeomArgs& a, const TraceScratch& ts, hipStream_t stream, KernelTimer* timer);
 hipError_t launchAggregate(const float* tmp, uint32_t hashSize, uint32_t dim, float* out, hipStream_t stream);
 hipError_t launchFill(float* p, uint64_t n, float v, hipStream_t stream);

@@ -1,21 +1,22 @@
-// WARDiffPathTracer geometry gradients (per-path code, shared by the kernel and the host check build): warped-area reparameterization (useWAR = 1), direct
-// illumination paths (maxBounces = 0, the configuration of the reference's shape optimization,
-// scripts/inv-rendering/shape_optimization/shape_example.py:23).
+// WARDiffPathTracer geometry gradients (per-path code, shared by the kernel and the host check
+// build): warped-area reparameterization (useWAR = 1) at any maxBounces (the reference's shape
+// optimization runs at maxBounces = 0, scripts/inv-rendering/shape_optimization/shape_example.py:23).
 //
 // The path (WARDiffPathTracer.rt.slang:75-143, PTUtils.slang:253-341) is: a screen sample ps with
-// Gaussian jitter; the camera ray hits x1 (AttachToRay, DiffSceneQuery.slang:131-219); x1 adds its
-// emission; NEE toward a light sample (computeNEE, PTUtils.slang:211-249); a BSDF-sampled ray hits
-// x2, which adds its MIS-weighted emission. With useWAR the screen sample and the NEE and scatter
-// directions are warped (WarpedAreaReparam.slang:199-332): each gets a warp V (zero in value)
-// and a Jacobian determinant detJ (one in value) whose parameter derivatives carry the boundary
-// terms. Sampled directions, light samples, pdfs and MIS weights are detached as in the reference.
+// Gaussian jitter; the camera ray hits x1 (AttachToRay, DiffSceneQuery.slang:131-219); each vertex
+// x_k adds its (MIS-weighted) emission, then, below the bounce limit, NEE toward a light sample
+// (computeNEE, PTUtils.slang:211-249) and a BSDF-sampled ray that hits x_k+1, attached to that ray.
+// With useWAR the screen sample and every NEE and scatter direction are warped
+// (WarpedAreaReparam.slang:199-332): each gets a warp V (zero in value) and a Jacobian
+// determinant detJ (one in value) whose parameter derivatives carry the boundary terms. Sampled
+// directions, light samples, pdfs and MIS weights are detached as in the reference.
 //
-// Writing S = dLdI . radiance and Q_E, Q_N, Q_S for the three contributions of the path,
-//   S = thp0 [Q_E + w_N (a2s/a2s^) (f_N/f_N^) Q_N + detJ_S (f_S/f_S^) Q_S],
-//   thp0 = detJ_screen G(ps + V_screen - pc) / G^,
-// so the adjoint is a short reverse sweep: BSDF partials at x1 (isotropic BSDF: a function of
-// N.wi, N.wo and wi.wo), the NEE geometry factor, the two secondary warps, the interpolated shading
-// normal, the barycentrics and the ray/plane intersection of x1, and finally the screen warp.
+// With S = dLdI . radiance, thp0 = detJ_screen G(ps + V_screen - pc) / G^ and thp_k the product
+// of the scatter factors detJ_j f_j / pdf_j before x_k,
+//   S = thp0 sum_k thp_k [E_k + w_N (a2s/a2s^) (f_N/f_N^) Q_N,k],
+// so the adjoint is a reverse sweep over the vertices: BSDF partials (isotropic BSDF: a function
+// of N.wi, N.wo and wi.wo), the NEE geometry factor, the warps, the interpolated shading normal,
+// the barycentrics and the ray/plane intersection of each vertex, then the screen warp.
 //
 // Warp adjoint. With aux directions a_i (vMF around the base direction, antithetic pairs), aux
 // hit directions d_i (AttachToGeometry: fixed barycentrics, so d_i moves with the aux triangle's
@@ -349,7 +350,8 @@ FHP_DEVI CameraDev geomCamera(const SceneDev& s, const GeomArgs& a, uint32_t vie
 
 // Adjoint of reparameterizeScreenSample (WarpedAreaReparam.slang:269-332) at screen sample ps:
 // gVs = dS/dV (screen units), gDet = dS/d detJ. Adds aux-triangle vertex gradients.
-FHP_DEV void screenWarpAdjoint(const SceneDev& s, const GeomArgs& a, uint32_t view, TraceStack& st, const AuxLog& lg,
+// Inlined into its callers: as a call it costs the kernels a register save and restore around it.
+FHP_DEVI void screenWarpAdjoint(const SceneDev& s, const GeomArgs& a, uint32_t view, TraceStack& st, const AuxLog& lg,
                                float psx, float psy, Rng rng, float gVx, float gVy, float gDet, uint32_t row,
                                unsigned long long& nc) {
     const CameraDev cam = geomCamera(s, a, view);
@@ -427,8 +429,12 @@ FHP_DEVI void gaussianJitter(float ux, float uy, float sigma, float& gx, float& 
     gy = (r * sinf(kTwoPi * uy)) * sigma;
 }
 
-// One path: primal replay of tracePaths/tracePath for maxBounces = 0, then the reverse sweep.
-FHP_DEV void geomPath(const SceneDev& s, const GeomArgs& a, uint32_t view, uint32_t px, uint32_t py, uint32_t seed, f3 dL,
+// One path at maxBounces = 0 (direct illumination, the reference's shape optimization setting):
+// the primal replay of tracePaths / tracePath (x1 with its emission, NEE and a BSDF-sampled ray to
+// x2's emission) and its reverse sweep in one pass, keeping x1's state in registers. The same
+// estimator as geomPath without the vertex records (x2 adds emission only, which the parameters
+// do not move).
+FHP_DEV void geomPathDirect(const SceneDev& s, const GeomArgs& a, uint32_t view, uint32_t px, uint32_t py, uint32_t seed, f3 dL,
                       uint32_t row, TraceStack& st, const AuxLog& lg, unsigned long long& nc) {
     Rng rng = rngSeed(px, py, seed / 2);
     for (int i = 0; i < 6; i++) rng.u1();
@@ -626,9 +632,354 @@ FHP_DEV void geomPath(const SceneDev& s, const GeomArgs& a, uint32_t view, uint3
     if (Qt != 0.f || gVx != 0.f || gVy != 0.f) screenWarpAdjoint(s, a, view, st, lg, psx, psy, rngScreen, gVx, gVy, Qt, row, nc);
 }
 
+// Per-path vertex records of the forward pass, read back in reverse order by the adjoint sweep.
+// Record k (words k * kVtxWords + w at p[(k * kVtxWords + w) * stride]) holds: w0 the triangle
+// hit by the incoming ray (bit 31: the vertex's light sample is visible), w1-3 the incoming ray's
+// origin and w4-6 its direction (held values), w7-10 the sample generator's state at the vertex,
+// w11-13 the path throughput at the vertex and w14-16 the (MIS-weighted) emission the vertex adds
+// to the path radiance, so the reverse sweep need not shade a vertex that only adds emission.
+constexpr uint32_t kVtxWords = kGeomVertexWords;
+constexpr uint32_t kVtxLit = 0x80000000u;
+FHP_DEVI bool anyNonZero(f3 v) { return v.x != 0.f || v.y != 0.f || v.z != 0.f; }
+struct VertexLog {
+    GlobalU32* p;     // this thread's word 0
+    uint32_t stride;  // threads of the persistent grid
+    uint32_t depth;   // records per thread (maxBounces + 2)
+    FHP_DEVI void put(uint32_t k, uint32_t w, uint32_t v) const { p[(k * kVtxWords + w) * stride] = v; }
+    FHP_DEVI uint32_t get(uint32_t k, uint32_t w) const { return p[(k * kVtxWords + w) * stride]; }
+    FHP_DEVI void put3(uint32_t k, uint32_t w, f3 v) const {
+        put(k, w, f2u(v.x));
+        put(k, w + 1, f2u(v.y));
+        put(k, w + 2, f2u(v.z));
+    }
+    FHP_DEVI f3 get3(uint32_t k, uint32_t w) const { return mk3(u2f(get(k, w)), u2f(get(k, w + 1)), u2f(get(k, w + 2))); }
+};
+
+// AttachToRay (DiffSceneQuery.slang:131-219): the ray/plane distance t of triangle tri, the area
+// barycentrics of the point on the ray, the attached point x and the interpolated (unnormalized)
+// vertex normal.
+struct Attached {
+    f3 v[3], n[3], e1, e2, x, nrm;
+    float t, b[3];
+};
+FHP_DEVI void attachRay(const SceneDev& s, uint32_t tri, f3 o, f3 d, Attached& A) {
+    float tu, tv;
+    for (int k = 0; k < 3; k++) loadVertex(s, s.idx[3 * tri + k], A.v[k], A.n[k], tu, tv);
+    A.e1 = A.v[1] - A.v[0];
+    A.e2 = A.v[2] - A.v[0];
+    const f3 pvec = cross(d, A.e2);
+    const float det = dot(A.e1, pvec);
+    const f3 qvec = cross(o - A.v[0], A.e1);
+    A.t = dot(A.e2, qvec) / det;
+    const f3 p = o + d * A.t;
+    const f3 ep0 = A.v[0] - p, ep1 = A.v[1] - p, ep2 = A.v[2] - p;
+    const float A0 = length(cross(ep1, ep2)), A1 = length(cross(ep2, ep0)), A2 = length(cross(ep0, ep1));
+    const float At = A0 + A1 + A2;
+    A.b[0] = A0 / At;
+    A.b[1] = A1 / At;
+    A.b[2] = A2 / At;
+    A.x = A.v[0] * A.b[0] + A.v[1] * A.b[1] + A.v[2] * A.b[2];
+    A.nrm = A.n[0] * A.b[0] + A.n[1] * A.b[1] + A.n[2] * A.b[2];
+}
+
+// NEE at a vertex (generateEmissiveSample + computeNEE, PTUtils.slang:177-249): the MIS weight of
+// a visible light sample toward wo (the held light point seen from x).
+FHP_DEVI float neeMis(const SceneDev& s, const GeomArgs& a, const Shading& sd, const Material& m, const LightSample& ls,
+                      f3 ldir) {
+    if (!a.useMIS) return 1.f;
+    float sp;
+    if (a.useBSDFSampling || m.type == MAT_PBRT_DIFFUSE) sp = matPdf(sd, m, ldir);
+    else {
+        f3 wiL = sd.toLocal(sd.V), woL = sd.toLocal(ldir);
+        sp = (validRT(sd, wiL, woL, ldir) && fminf_(wiL.z, woL.z) >= kMinCosTheta) ? woL.z * kInvPi : 0.f;
+    }
+    return misPower(ls.pdf, sp);
+}
+
+// One path. The forward pass replays the primal (tracePaths / tracePath with handleHit,
+// PTUtils.slang:255-341) and records its vertices. The reverse sweep visits them from the last
+// to the first. At vertex k it takes R, the radiance the later vertices add, and the adjoints of
+// vertex k's outgoing ray origin and direction, handed back by vertex k + 1. It then:
+//   * differentiates the NEE term and the scatter factor f_k (dS/df_k = dLdI R / f_k, the path
+//     replay identity), with their warps (dS/d detJ = the radiance the warped ray carries);
+//   * routes the shading normal through the vertex normals and barycentrics;
+//   * differentiates AttachToRay into the triangle's vertices and the incoming ray's origin and
+//     direction, which it hands to vertex k - 1. The origin follows the previous vertex with an
+//     identity derivative (computeRayOrigin, GeometryHelpers.slang:70-75). The direction is the
+//     previous scatter sample plus that vertex's warp.
+// The camera ray's direction adjoint ends in the screen warp.
+FHP_DEV void geomPath(const SceneDev& s, const GeomArgs& a, uint32_t view, uint32_t px, uint32_t py, uint32_t seed, f3 dL,
+                      uint32_t row, TraceStack& st, const AuxLog& lg, const VertexLog& vl, unsigned long long& nc) {
+    Rng rng = rngSeed(px, py, seed / 2);
+    for (int i = 0; i < 6; i++) rng.u1();
+    const float pcx = float(px) + 0.5f, pcy = float(py) + 0.5f;
+    float ux = rng.u1(), uy = rng.u1();
+    float jx, jy;
+    gaussianJitter(ux, uy, 0.5f, jx, jy);
+    if (a.useAntithetic && seed % 2 == 0) { jx = -jx; jy = -jy; }
+    const float psx = pcx + jx, psy = pcy + jy;
+    const Rng rngScreen = rng;  // reparameterizeScreenSample takes the generator by value
+    const CameraDev cam0 = geomCamera(s, a, view);
+    const f3 r0 = camRaw(cam0, psx, psy, a.W, a.H);
+    const float rl = length(r0);
+    const f3 w0 = rl > 0.f ? r0 / rl : mk3(0.f);
+
+    // ---------------------------------------------------------------- forward: record the vertices
+    f3 rayO = cam0.pos, rayD = w0;
+    HitRec h;
+    nc++;
+    if (!traceBVH4<false, false>(s, rayO, rayD, 0.f, kFltMax, st, h)) return;
+    f3 thp = mk3(1.f), prevN = mk3(0.f);
+    float prevPdf = 0.f;
+    uint32_t nv = 0;
+    for (uint32_t len = 0; len < vl.depth; len++) {
+        // The light sample and its visibility ray first: only the attached point and normal are
+        // live across that traversal (the shading data and material are loaded after it).
+        f3 x, nrm;
+        {
+            Attached A;
+            attachRay(s, h.tri, rayO, rayD, A);
+            x = A.x;
+            nrm = A.nrm;
+        }
+        vl.put3(len, 1, rayO);
+        vl.put3(len, 4, rayD);
+        vl.put(len, 7, rng.s0);
+        vl.put(len, 8, rng.s1);
+        vl.put(len, 9, rng.s2);
+        vl.put(len, 10, rng.s3);
+        vl.put3(len, 11, thp);
+        nv = len + 1;
+        const bool stop = len >= a.maxBounces + 1;
+        bool lit = false;
+        if (!stop && a.useNEE) {
+            LightSample ls;
+            if (sampleLight(s, LS_UNIFORM, x, nrm, rng, ls)) {
+                const f3 Li = ls.pdf > 0.f ? ls.Le / ls.pdf : mk3(0.f);
+                const f3 origin = rayOriginOffset(x, nrm);
+                const f3 toL = rayOriginOffset(ls.posW, ls.normalW) - origin;
+                HitRec sh;
+                lit = anyPos(Li) && !traceBVH4<true, false>(s, origin, normalize(toL), 0.f, length(toL), st, sh);
+            }
+        }
+        vl.put(len, 0, h.tri | (lit ? kVtxLit : 0u));
+        Shading sd;
+        {
+            Attached A;
+            attachRay(s, h.tri, rayO, rayD, A);
+            sd = shadingFromHit(s, h.tri, A.b[1], A.b[2], rayD);
+        }
+        sd.V = -rayD;
+        setFrame(sd, nrm);
+        const Shading sd0 = sd;
+        const Material m = loadMaterial(s, s.mats, s.texels, sd0, sd, nullptr);
+        f3 emitted = mk3(0.f);
+        const bool isPrimary = len == 0;
+        if ((isPrimary || !a.useNEE || a.useMIS) && anyPos(m.emission)) {
+            float misE = 1.f;
+            if (!isPrimary && a.useNEE && a.useMIS)
+                misE = misPower(prevPdf, lightPdf(s, LS_UNIFORM, rayO, prevN, s.triToLight[h.tri], sd.posW, sd.orientedFaceN()));
+            emitted = thp * m.emission * misE;
+        }
+        vl.put3(len, 14, emitted);
+        if (stop) break;
+        Scatter sc;
+        if (!(matSample(sd, m, rng, sc, a.useBSDFSampling != 0) && sc.pdf > 0.f)) break;
+        const f3 fS = matEval(sd, m, sc.wo);
+        thp = thp * (fS / sc.pdf);
+        prevN = sd.orientedFaceN();
+        prevPdf = sc.pdf;
+        if (!anyPos(thp)) break;
+        rayO = rayOriginOffset(x, sd.frontFacing ? sd.faceN : -sd.faceN);
+        rayD = sc.wo;
+        nc++;
+        if (!traceBVH4<false, false>(s, rayO, rayD, 0.f, kFltMax, st, h)) break;
+    }
+
+    // ---------------------------------------------------------------- reverse sweep
+    f3 R = mk3(0.f);                     // radiance added after the current vertex
+    f3 gOo = mk3(0.f), gOd = mk3(0.f);  // adjoints of the current vertex's outgoing ray origin / direction
+    for (int k = int(nv) - 1; k >= 0; k--) {
+        const uint32_t w0k = vl.get(uint32_t(k), 0);
+        const uint32_t tri = w0k & ~kVtxLit;
+        const bool lit = (w0k & kVtxLit) != 0u;
+        const f3 o = vl.get3(uint32_t(k), 1), d = vl.get3(uint32_t(k), 4);
+        Rng r{vl.get(uint32_t(k), 7), vl.get(uint32_t(k), 8), vl.get(uint32_t(k), 9), vl.get(uint32_t(k), 10)};
+        f3 contrib = vl.get3(uint32_t(k), 14);
+        if (uint32_t(k) >= a.maxBounces + 1) {
+            // A terminating vertex only adds its emission, which the parameters do not move.
+            R = R + contrib;
+            gOo = gOd = mk3(0.f);
+            continue;
+        }
+        const f3 thpk = vl.get3(uint32_t(k), 11);
+        // Local terms first (BSDF partials of the NEE and scatter factors, the shading normal's
+        // routing into vertex normals and barycentrics), then the warps, then the attachment of
+        // x to its incoming ray: the shading data and material are dead before the warps' aux
+        // traversals, which keeps the kernel's spill traffic down.
+        f3 gX = gOo, gWi = mk3(0.f), gTri[3] = {mk3(0.f), mk3(0.f), mk3(0.f)};
+        f3 x, faceO;
+        // NEE warp (origin x's view-side offset, base direction wo; the reference form also takes
+        // the direction derivative of its harmonic weights).
+        bool neeWarp = false, neeRef = false;
+        f3 neeWo = mk3(0.f), neeG = mk3(0.f);
+        float neePhi = 0.f, neeDist = 1.f;
+        Rng rN = r;
+        // Scatter warp (same origin, base direction the BSDF sample).
+        bool scWarp = false;
+        f3 scWo = mk3(0.f), scG = mk3(0.f);
+        float scDet = 0.f;
+        Rng rS = r;
+        {
+            Attached A;
+            attachRay(s, tri, o, d, A);
+            x = A.x;
+            Shading sd = shadingFromHit(s, tri, A.b[1], A.b[2], d);
+            sd.V = -d;
+            setFrame(sd, A.nrm);
+            const Shading sd0 = sd;
+            const Material m = loadMaterial(s, s.mats, s.texels, sd0, sd, nullptr);
+            faceO = sd.frontFacing ? sd.faceN : -sd.faceN;
+            f3 gNs = mk3(0.f);
+            if (a.useNEE) {
+                LightSample ls;
+                if (sampleLight(s, LS_UNIFORM, A.x, A.nrm, r, ls) && lit) {
+                    const f3 Li = ls.pdf > 0.f ? ls.Le / ls.pdf : mk3(0.f);
+                    const f3 origin = rayOriginOffset(A.x, A.nrm);
+                    const f3 ldir = normalize(rayOriginOffset(ls.posW, ls.normalW) - origin);
+                    f3 wo = ls.posW - A.x;
+                    const float dist = length(wo);
+                    wo = wo / dist;
+                    const float misN = neeMis(s, a, sd, m, ls, ldir);
+                    contrib = contrib + thpk * (Li * matEval(sd, m, wo) * misN);
+                    float gci, gco, gcio;
+                    const float phi = bsdfGeomGrad(sd, m, wo, dL * thpk * Li * misN, gci, gco, gcio);
+                    const float cosL = dot(ls.normalW, -wo);
+                    if (phi != 0.f && (a.neeHeld || cosL > 0.f)) {
+                        // useSolidAngleNEEWarp: the sampled direction wo is held (a solid-angle
+                        // sample) and only the warp moves it, d = wo + V; no area factor. The
+                        // warp then follows the light point seen from the moving vertex and detJ
+                        // carries its solid-angle change, the derivative of the area-measure
+                        // integrand. Otherwise computeNEE (PTUtils.slang:211-249): Q_N =
+                        // phi(x, N, d) (a2s / a2s^) detJ with wo = normalize(y - x) toward the
+                        // held light point y, d = wo + V and a2s = |y - x|^2 / dot(n_y, -d).
+                        gNs = gNs + sd.V * gci + wo * gco;
+                        gWi = gWi + sd.N * gci + wo * gcio;
+                        neeWarp = true;
+                        neeRef = !a.neeHeld;
+                        neeWo = wo;
+                        neePhi = phi;
+                        neeDist = dist;
+                        neeG = sd.N * gco + sd.V * gcio + (neeRef ? ls.normalW * (phi / cosL) : mk3(0.f));
+                        rN = r;
+                    }
+                }
+            }
+            Scatter sc;
+            if (matSample(sd, m, r, sc, a.useBSDFSampling != 0) && sc.pdf > 0.f) {
+                // The scatter factor f_k and the warp of the outgoing ray: dS/df_k = dLdI R / f_k,
+                // dS/d detJ_k = dLdI . R, and the later vertices' adjoint of the outgoing direction.
+                const f3 woS = sc.wo;
+                const f3 fS = matEval(sd, m, woS);
+                const f3 adj = mk3(fS.x != 0.f ? dL.x * R.x / fS.x : 0.f, fS.y != 0.f ? dL.y * R.y / fS.y : 0.f,
+                                   fS.z != 0.f ? dL.z * R.z / fS.z : 0.f);
+                float gci, gco, gcio;
+                const float phi = bsdfGeomGrad(sd, m, woS, adj, gci, gco, gcio);
+                gNs = gNs + sd.V * gci + woS * gco;
+                gWi = gWi + sd.N * gci + woS * gcio;
+                scG = sd.N * gco + sd.V * gcio + gOd;
+                scDet = dot(dL, R);
+                scWo = woS;
+                scWarp = phi != 0.f || scDet != 0.f || anyNonZero(scG);
+                rS = r;
+            }
+            // Shading normal N = sgn normalize(sum b_k n_k) -> vertex normals and barycentrics.
+            const float nl = length(A.nrm);
+            const f3 nh = A.nrm / nl;
+            const float sgn = dot(sd.N, nh) < 0.f ? -1.f : 1.f;  // double-sided back-face flip
+            const f3 gNw = (gNs - nh * dot(nh, gNs)) * (sgn / nl);
+            const float gB[3] = {dot(gNw, A.n[0]), dot(gNw, A.n[1]), dot(gNw, A.n[2])};
+            {
+                const f3 gn[3] = {gNw * A.b[0], gNw * A.b[1], gNw * A.b[2]};
+                addVertexGrads3(s, a.gradNrm, a, row, tri, gn);
+            }
+            // Barycentrics of x (b1, b2 solve the 2x2 Gram system of the edges; b0 = 1 - b1 - b2).
+            const float g1 = gB[1] - gB[0], g2 = gB[2] - gB[0];
+            const f3 q = A.x - A.v[0];
+            const float a11 = dot(A.e1, A.e1), a12 = dot(A.e1, A.e2), a22 = dot(A.e2, A.e2);
+            const float dg = a11 * a22 - a12 * a12;
+            if (dg != 0.f) {
+                const float gr1 = (a22 * g1 - a12 * g2) / dg, gr2 = (-a12 * g1 + a11 * g2) / dg;
+                const float b1 = A.b[1], b2 = A.b[2];
+                const float gA11 = -gr1 * b1, gA12 = -(gr1 * b2 + gr2 * b1), gA22 = -gr2 * b2;
+                const f3 gE1 = q * gr1 + A.e1 * (2.f * gA11) + A.e2 * gA12;
+                const f3 gE2 = q * gr2 + A.e1 * gA12 + A.e2 * (2.f * gA22);
+                const f3 gQ = A.e1 * gr1 + A.e2 * gr2;
+                gX = gX + gQ;
+                gTri[0] = gTri[0] - gQ - gE1 - gE2;
+                gTri[1] = gTri[1] + gE1;
+                gTri[2] = gTri[2] + gE2;
+            }
+        }
+        R = R + contrib;
+        const f3 wOrigin = rayOriginOffset(x, faceO);
+        if (neeWarp) {
+            f3 gO, gOm;
+            warpAdjoint(s, a, st, lg, wOrigin, neeWo, rN, neeG, neePhi, neeRef, row, gO, gOm, nc);
+            gX = gX + gO;
+            if (neeRef) {
+                const f3 gWo = neeG + gOm;
+                gX = gX - (gWo - neeWo * dot(neeWo, gWo)) / neeDist - neeWo * (2.f * neePhi / neeDist);
+            }
+        }
+        if (scWarp) {
+            f3 gO, gOm;
+            warpAdjoint(s, a, st, lg, wOrigin, scWo, rS, scG, scDet, false, row, gO, gOm, nc);
+            gX = gX + gO;
+        }
+        // x = o + t d with t = nf . (v0 - o) / (nf . d): into the triangle's vertices, the incoming
+        // direction (also the view direction wi = -d of the shading) and the incoming origin.
+        f3 gD = gWi * -1.f, gO = mk3(0.f);
+        {
+            Attached A;
+            attachRay(s, tri, o, d, A);
+            const f3 nf = cross(A.e1, A.e2);
+            const float den = dot(nf, d);
+            if (den != 0.f) {
+                const float gT = dot(gX, d);
+                gD = gD + gX * A.t - nf * (gT * A.t / den);
+                gO = gX - nf * (gT / den);
+                gTri[0] = gTri[0] + nf * (gT / den);
+                const f3 gNf = (A.v[0] - A.x) * (gT / den);
+                const f3 gE1 = cross(A.e2, gNf), gE2 = cross(gNf, A.e1);
+                gTri[1] = gTri[1] + gE1;
+                gTri[2] = gTri[2] + gE2;
+                gTri[0] = gTri[0] - gE1 - gE2;
+            }
+        }
+        addVertexGrads3(s, a.gradPos, a, row, tri, gTri);
+        gOo = gO;
+        gOd = gD;
+    }
+
+    // Screen warp: thp0 = detJ * G(ps + V - pc) / G^ and w0 = camDir(ps + V); gOd is now the
+    // camera direction's adjoint and R the path's radiance.
+    const float Qt = dot(dL, R);
+    const float inv2s2 = 1.f / (0.5f * 0.5f);
+    float gVx = -Qt * (psx - pcx) * inv2s2, gVy = -Qt * (psy - pcy) * inv2s2;
+    {
+        const f3 q = (gOd - w0 * dot(w0, gOd)) / rl;
+        gVx += dot(cam0.U, q) * (2.f / float(a.W));
+        gVy += dot(cam0.V, q) * (-2.f / float(a.H));
+    }
+    if (Qt != 0.f || gVx != 0.f || gVy != 0.f) screenWarpAdjoint(s, a, view, st, lg, psx, psy, rngScreen, gVx, gVy, Qt, row, nc);
+}
+
 // One item of the kernel's grid loop: path id = (view * W * H + pixel) * spp + sample.
+// DIRECT: the maxBounces = 0 specialisation (geomPathDirect), otherwise the general reverse sweep.
+template <bool DIRECT>
 FHP_DEVI void geomItem(const SceneDev& s, const GeomArgs& a, uint32_t id, TraceStack& st, unsigned long long& nc,
-                       const AuxLog& lg = noAuxLog()) {
+                       const AuxLog& lg, const VertexLog& vl) {
     uint32_t view = 0, local = id, frame = a.frame;
     const float* dLdI = a.dLdI;
     if (a.views) {
@@ -645,7 +996,8 @@ FHP_DEVI void geomItem(const SceneDev& s, const GeomArgs& a, uint32_t id, TraceS
         dL = mk3(dl[0], dl[1], dl[2]);
     }
     if (dL.x == 0.f && dL.y == 0.f && dL.z == 0.f) return;
-    geomPath(s, a, view, pix % a.W, pix / a.W, frame + k, dL, jenkins(pix) % a.hashSize, st, lg, nc);
+    if (DIRECT) geomPathDirect(s, a, view, pix % a.W, pix / a.W, frame + k, dL, jenkins(pix) % a.hashSize, st, lg, nc);
+    else geomPath(s, a, view, pix % a.W, pix / a.W, frame + k, dL, jenkins(pix) % a.hashSize, st, lg, vl, nc);
 }
 
 }  // namespace

@@ -1,5 +1,6 @@
-// WARDiffPathTracer geometry-gradient kernel (warped-area reparameterization, maxBounces = 0);
-// the per-path code is in fhp_war_geom.h.
+// WARDiffPathTracer geometry-gradient kernel (warped-area reparameterization); the per-path code
+// is in fhp_war_geom.h. DIRECT instances run maxBounces = 0 paths in one pass, the others record
+// each path's vertices and sweep them in reverse.
 #include "fhp_war_geom.h"
 
 namespace fhp {
@@ -10,7 +11,7 @@ namespace {
 // more but runs the 256x256, 8 spp, 16-aux vertex-gradient pass about 1.2x faster.
 constexpr int kGeomWavesPerEU = 4;
 
-template <int B, int D>
+template <int B, int D, bool DIRECT>
 __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(kGeomWavesPerEU))) void kWarGeom(SceneDev s, GeomArgs a,
                                                                                                      TraceScratch ts) {
     __shared__ uint32_t lds[D * B];
@@ -20,7 +21,9 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(kGeomWavesPer
     unsigned long long nc = 0;
     // Aux-hit log of this thread: 3 * aux words [word][thread] in a.auxLog (3 * aux * G words).
     const AuxLog lg = a.auxLog ? AuxLog{(GlobalU32*)(a.auxLog + blockIdx.x * B + threadIdx.x), G} : noAuxLog();
-    for (uint32_t id = blockIdx.x * B + threadIdx.x; id < n; id += G) geomItem(s, a, id, st, nc, lg);
+    // Vertex records of this thread: kGeomVertexWords * (maxBounces + 2) words [word][thread].
+    const VertexLog vl{(GlobalU32*)(a.vtxLog + blockIdx.x * B + threadIdx.x), G, a.maxBounces + 2};
+    for (uint32_t id = blockIdx.x * B + threadIdx.x; id < n; id += G) geomItem<DIRECT>(s, a, id, st, nc, lg, vl);
     if (a.rays) atomicAdd(&a.rays[0], nc);
 }
 
@@ -34,7 +37,8 @@ hipError_t launchWarGeom(const SceneDev& s, const GeomArgs& a, const TraceScratc
         // Persistent grid: no more blocks than are resident at kGeomWavesPerEU waves per SIMD.
         const uint32_t resident = uint32_t(numCUs()) * std::max<uint32_t>(1u, kGeomWavesPerEU * 4 * 64 / B);
         const uint32_t blocks = std::min<uint32_t>(std::min<uint32_t>(ts.gridBlocks, resident), (n + B - 1) / B);
-        hipLaunchKernelGGL((kWarGeom<B, D>), dim3(blocks), dim3(B), 0, st, s, a, ts);
+        if (a.maxBounces == 0) hipLaunchKernelGGL((kWarGeom<B, D, true>), dim3(blocks), dim3(B), 0, st, s, a, ts);
+        else hipLaunchKernelGGL((kWarGeom<B, D, false>), dim3(blocks), dim3(B), 0, st, s, a, ts);
         return hipGetLastError();
     });
     if (tm) tm->end(st);

@@ -1,5 +1,5 @@
-// WARDiffPathTracer vertex gradients at maxBounces = 0: the oracle's restatement of the
-// estimator (test infrastructure only; the product never links this file).
+// WARDiffPathTracer vertex gradients: the oracle's restatement of the estimator (test
+// infrastructure only; the product never links this file).
 //
 // The reference differentiates its warped-area estimator (WARDiffPathTracer.rt.slang:66-143,
 // PTUtils.slang:136-341, WarpedAreaReparam.slang:50-332, DiffSceneQuery.slang:131-261,
@@ -211,26 +211,32 @@ void camScreen(const CamD& c, V3d dir, V3d v, uint32_t W, uint32_t H, double& sx
 
 double det3(const V3d c[3]) { return dot(c[0], cross(c[1], c[2])); }
 
+// One path vertex of F(theta), taken at theta = 0: the triangle hit by the incoming ray, the
+// material set-up, the emission the vertex adds (with its held MIS weight), its NEE sample and
+// warp, and its scatter sample (with the outgoing ray's FP32 origin) and warp.
+struct VertexRec {
+    uint32_t tri = kInvalid;
+    uint32_t mat = 0;
+    bool front = true, flip = false;
+    V3d faceN;
+    bool emit = false;
+    V3d E;
+    double misE = 1.0;
+    bool lit = false;
+    V3d lightPos, lightN, Li;
+    double misN = 1.0;
+    Warp nee;
+    bool scat = false;
+    V3d woS, oS0;
+    double pdfS = 0.0;
+    Warp sc;
+};
+
 // Everything a path's F(theta) needs, taken at theta = 0.
 struct PathRec {
     double psx, psy, pcx, pcy;
     Warp screen;
-    uint32_t tri1 = kInvalid;
-    V3d E1;
-    uint32_t mat1;
-    bool front1, flip1;
-    V3d faceN1;
-    bool lit = false;
-    V3d lightPos, lightN, Li;
-    double misN;
-    Warp nee;
-    bool scat = false;
-    V3d woS;
-    double pdfS;
-    Warp sc;
-    bool emit2 = false;
-    V3d E2;
-    double mis2;
+    std::vector<VertexRec> v;
     V3d dL;
 };
 
@@ -239,10 +245,14 @@ struct Params {
     float kappa, kappaS;
     bool useWAR, bsdfSampling, nee, mis, anti;
     bool neeHeld;  // extension: the NEE direction held (a solid-angle sample), no area factor
+    uint32_t maxBounces;
 };
 
-// The oracle's WAR primal path at theta = 0 (fo_impl.cpp warTracePaths, maxBounces = 0),
-// recording the decisions. Returns false for a path with no primary hit.
+// The oracle's WAR primal path at theta = 0 (fo_impl.cpp warTracePaths, the reference's tracePath
+// with handleHit, PTUtils.slang:255-341), recording the decisions of every vertex and tracing the
+// auxiliary rays of each warp from a copy of the sample generator at the warp (the screen warp
+// before the primary ray, the NEE warp after the light sample, the scatter warp after the BSDF
+// sample). Returns false for a path with no primary hit.
 bool recordPath(const Scene& s, const Params& P, uint32_t px, uint32_t py, uint32_t frameSeed, Tracer& tr, PathRec& r) {
     SampleGen sg = makeSampleGen(px, py, frameSeed / 2);
     for (int i = 0; i < 6; i++) sg.next1D();
@@ -260,111 +270,121 @@ bool recordPath(const Scene& s, const Params& P, uint32_t px, uint32_t py, uint3
     dir = dl > 0.f ? dir / dl : V3f(0.f);
     // reparameterizeScreenSample takes the generator by value (WarpedAreaReparam.slang:305).
     if (P.useWAR) traceAux(s, tr, sg, s.cam.pos, dir, P.kappaS, P.aux, P.anti, r.screen);
-    const V3f o = s.cam.pos;
+    V3f o = s.cam.pos;
     Hit h = tr.closest(o, dir, 0.f, std::numeric_limits<float>::max());
     if (!h.valid()) return false;
-    r.tri1 = h.tri;
-    V3f posW, bary, faceN, normalW;
-    computeIntersectionAD(s, h, o, dir, posW, bary, faceN, normalW);
-    Hit th = h; th.u = bary.y; th.v = bary.z;
-    ShadingData sd = loadShadingData(s, th, dir);
-    sd.V = -dir;
-    sd.frame = frameFromNormal(normalW);
-    MatInst mi = makeMatInst(s, sd);
-    r.mat1 = sd.mat;
-    r.front1 = sd.frontFacing;
-    r.flip1 = !sd.frontFacing && matDoubleSided(s.mat(sd.mat));
-    r.faceN1 = widen(sd.faceN);
-    r.E1 = anyPositive(mi.emission) ? widen(mi.emission) : V3d(0.0);
-    const V3f faceO = sd.frontFacing ? sd.faceN : -sd.faceN;
-    if (P.nee) {
-        LightSample ls;
-        if (sampleLight(s, LS_UNIFORM, posW, normalW, sg, ls)) {
-            V3f Li = ls.pdf > 0.f ? ls.Le / ls.pdf : V3f(0.f);
-            V3f lightPos = computeRayOrigin(ls.posW, ls.normalW);
-            V3f origin = computeRayOrigin(posW, normalW);
-            V3f toLight = lightPos - origin;
-            float distance = length(toLight);
-            V3f ldir = normalize(toLight);
-            if (anyPositive(Li) && tr.visible(origin, ldir, distance)) {
-                V3f wo = ls.posW - posW;
-                wo = wo / length(wo);
-                float misW = 1.f;
-                if (P.mis) {
-                    float scatterPdf = miEvalPdf(sd, mi, ldir);
-                    if (!P.bsdfSampling) {
-                        V3f wiL = mi.sf.toLocal(sd.V), woL = mi.sf.toLocal(ldir);
-                        scatterPdf = (mi.d.type == MAT_PBRT_DIFFUSE) ? scatterPdf
-                                     : (validHemisphere(sd, wiL, woL, ldir) && std::min(wiL.z, woL.z) >= kMinCosTheta ? woL.z * Consts<float>::inv_pi : 0.f);
+    V3f thp(1.f), prevNormal(0.f);
+    float prevPdf = 0.f;
+    for (uint32_t len = 0;; len++) {
+        VertexRec vr;
+        vr.tri = h.tri;
+        V3f posW, bary, faceN, normalW;
+        computeIntersectionAD(s, h, o, dir, posW, bary, faceN, normalW);
+        Hit th = h; th.u = bary.y; th.v = bary.z;
+        ShadingData sd = loadShadingData(s, th, dir);
+        sd.V = -dir;
+        sd.frame = frameFromNormal(normalW);
+        MatInst mi = makeMatInst(s, sd);
+        vr.mat = sd.mat;
+        vr.front = sd.frontFacing;
+        vr.flip = !sd.frontFacing && matDoubleSided(s.mat(sd.mat));
+        vr.faceN = widen(sd.faceN);
+        const bool isPrimary = len == 0;
+        if ((isPrimary || !P.nee || P.mis) && anyPositive(mi.emission)) {
+            float misW = 1.f;
+            if (!isPrimary && P.nee && P.mis) {
+                const float lp = lightPdf(s, LS_UNIFORM, o, prevNormal, s.triToLight[h.tri], sd.posW, orientedFaceN(sd));
+                misW = evalMIS(1.f, prevPdf, 1.f, lp);
+            }
+            vr.emit = true;
+            vr.E = widen(mi.emission);
+            vr.misE = misW;
+        }
+        if (len >= P.maxBounces + 1) {
+            r.v.push_back(vr);
+            break;
+        }
+        const V3f faceO = sd.frontFacing ? sd.faceN : -sd.faceN;
+        if (P.nee) {
+            LightSample ls;
+            if (sampleLight(s, LS_UNIFORM, posW, normalW, sg, ls)) {
+                V3f Li = ls.pdf > 0.f ? ls.Le / ls.pdf : V3f(0.f);
+                V3f lightPos = computeRayOrigin(ls.posW, ls.normalW);
+                V3f origin = computeRayOrigin(posW, normalW);
+                V3f toLight = lightPos - origin;
+                float distance = length(toLight);
+                V3f ldir = normalize(toLight);
+                if (anyPositive(Li) && tr.visible(origin, ldir, distance)) {
+                    V3f wo = ls.posW - posW;
+                    wo = wo / length(wo);
+                    float misW = 1.f;
+                    if (P.mis) {
+                        float scatterPdf = miEvalPdf(sd, mi, ldir);
+                        if (!P.bsdfSampling) {
+                            V3f wiL = mi.sf.toLocal(sd.V), woL = mi.sf.toLocal(ldir);
+                            scatterPdf = (mi.d.type == MAT_PBRT_DIFFUSE) ? scatterPdf
+                                         : (validHemisphere(sd, wiL, woL, ldir) && std::min(wiL.z, woL.z) >= kMinCosTheta ? woL.z * Consts<float>::inv_pi : 0.f);
+                        }
+                        misW = evalMIS(1.f, ls.pdf, 1.f, scatterPdf);
                     }
-                    misW = evalMIS(1.f, ls.pdf, 1.f, scatterPdf);
+                    vr.lit = true;
+                    vr.lightPos = widen(ls.posW);
+                    vr.lightN = widen(ls.normalW);
+                    vr.Li = widen(Li);
+                    vr.misN = misW;
+                    // computeNEE (PTUtils.slang:211-249): the reparameterized ray starts at the
+                    // view-side offset of the hit point, along wo.
+                    if (P.useWAR) traceAux(s, tr, sg, computeRayOrigin(posW, faceO), wo, P.kappa, P.aux, P.anti, vr.nee);
                 }
-                r.lit = true;
-                r.lightPos = widen(ls.posW);
-                r.lightN = widen(ls.normalW);
-                r.Li = widen(Li);
-                r.misN = misW;
-                // computeNEE (PTUtils.slang:211-249): the reparameterized ray starts at the
-                // view-side offset of the hit point, along wo.
-                if (P.useWAR) traceAux(s, tr, sg, computeRayOrigin(posW, faceO), wo, P.kappa, P.aux, P.anti, r.nee);
             }
         }
-    }
-    BsdfSample bs;
-    if (miSample(sd, mi, sg, bs, P.bsdfSampling) && bs.pdf > 0.f) {
+        BsdfSample bs;
+        if (!(miSample(sd, mi, sg, bs, P.bsdfSampling) && bs.pdf > 0.f)) {
+            r.v.push_back(vr);
+            break;
+        }
         const V3f rayO = computeRayOrigin(posW, faceO);
         const V3f f = miEval(sd, mi, bs.wo);
-        if (anyPositive(f / bs.pdf)) {
-            r.scat = true;
-            r.woS = widen(bs.wo);
-            r.pdfS = bs.pdf;
-            if (P.useWAR) traceAux(s, tr, sg, rayO, bs.wo, P.kappa, P.aux, P.anti, r.sc);
-            Hit h2 = tr.closest(rayO, bs.wo, 0.f, std::numeric_limits<float>::max());
-            if (h2.valid() && (!P.nee || P.mis)) {
-                V3f p2, b2, fn2, n2;
-                computeIntersectionAD(s, h2, rayO, bs.wo, p2, b2, fn2, n2);
-                Hit t2 = h2; t2.u = b2.y; t2.v = b2.z;
-                ShadingData sd2 = loadShadingData(s, t2, bs.wo);
-                sd2.V = -bs.wo;
-                sd2.frame = frameFromNormal(n2);
-                MatInst mi2 = makeMatInst(s, sd2);
-                if (anyPositive(mi2.emission)) {
-                    float misW = 1.f;
-                    if (P.nee && P.mis) {
-                        float lp = lightPdf(s, LS_UNIFORM, rayO, orientedFaceN(sd), s.triToLight[h2.tri], sd2.posW, orientedFaceN(sd2));
-                        misW = evalMIS(1.f, bs.pdf, 1.f, lp);
-                    }
-                    r.emit2 = true;
-                    r.E2 = widen(mi2.emission);
-                    r.mis2 = misW;
-                }
-            }
-        }
+        vr.scat = true;
+        vr.woS = widen(bs.wo);
+        vr.oS0 = widen(rayO);
+        vr.pdfS = bs.pdf;
+        // generateScatterRay (PTUtils.slang:136-172): the warp follows the BSDF sample.
+        if (P.useWAR) traceAux(s, tr, sg, rayO, bs.wo, P.kappa, P.aux, P.anti, vr.sc);
+        r.v.push_back(vr);
+        thp = thp * (f / bs.pdf);
+        prevNormal = orientedFaceN(sd);
+        prevPdf = bs.pdf;
+        if (!anyPositive(thp)) break;
+        h = tr.closest(rayO, bs.wo, 0.f, std::numeric_limits<float>::max());
+        if (!h.valid()) break;
+        o = rayO;
+        dir = bs.wo;
     }
     return true;
 }
 
-// BSDF value of the hit's material (StandardMaterialInstance.slang:63-74, PBRT diffuse) in
-// double precision for a perturbed shading frame (normal N, view V) and direction wo.
-V3d evalBsdf(const Scene& s, const PathRec& r, V3d N, V3d V, V3d wo) {
-    const float* m = s.mat(r.mat1);
+// BSDF value of a vertex's material (StandardMaterialInstance.slang:63-74, PBRT diffuse) in double
+// precision for a perturbed shading frame (normal N, view V) and direction wo.
+V3d evalBsdf(const Scene& s, const VertexRec& r, V3d N, V3d V, V3d wo) {
+    const float* m = s.mat(r.mat);
     Frame<double> f = frameFromNormal(N);
-    if (r.flip1) f.N = -f.N;
+    if (r.flip) f.N = -f.N;
     const V3d wiL = f.toLocal(V), woL = f.toLocal(wo);
-    const V3d faceN = r.faceN1;
+    const V3d faceN = r.faceN;
     const bool woTop = dot(wo, faceN) >= 0.0;
     BsdfData<double> d;
     d.type = int(m[M_TYPE]);
     V3d base(m[M_BASE], m[M_BASE + 1], m[M_BASE + 2]);
     if (d.type == MAT_PBRT_DIFFUSE) {
-        if (std::min(wiL.z, woL.z) < double(kMinCosTheta) || r.front1 != woTop) return V3d(0.0);
+        if (std::min(wiL.z, woL.z) < double(kMinCosTheta) || r.front != woTop) return V3d(0.0);
         return base * std::max(woL.z, 0.0) * Consts<double>::inv_pi;
     }
     const bool isT = wiL.z * woL.z < 0.0;
     if (std::min(wiL.z, std::fabs(woL.z)) < double(kMinCosTheta)) return V3d(0.0);
-    if (isT ? (r.front1 == woTop) : (r.front1 != woTop)) return V3d(0.0);
+    if (isT ? (r.front == woTop) : (r.front != woTop)) return V3d(0.0);
     const double ior = m[M_IOR], fr = (ior - 1.0) / (ior + 1.0), F0 = fr * fr, metal = m[M_METAL];
-    d.eta = r.front1 ? 1.0 / ior : ior;
+    d.eta = r.front ? 1.0 / ior : ior;
     d.diffTrans = m[M_DIFF_TRANS];
     d.specTrans = m[M_SPEC_TRANS];
     if (d.diffTrans > 0.0 || d.specTrans > 0.0) {
@@ -396,7 +416,10 @@ void rayWarp(const Perturbed& P, const Perturbed& P0, const Warp& w, V3d origin,
 }
 
 // F(theta): dLdI . radiance of the recorded path (WARDiffPathTracer.rt.slang:75-111,
-// PTUtils.slang:136-341 with the decisions held).
+// PTUtils.slang:136-341 with the decisions held). Each vertex is attached to its incoming ray
+// (AttachToRay: the ray's origin follows the previous vertex, computeRayOrigin's identity
+// derivative, and its direction is the held sample plus the previous vertex's warp), so moving a
+// vertex moves every later vertex of the path.
 double pathValue(const Scene& s, const Params& Pp, const PathRec& r, const Perturbed& P, const Perturbed& P0) {
     const CamD cam{widen(s.cam.pos), widen(s.cam.U), widen(s.cam.V), widen(s.cam.W)};
     // Screen warp (reparameterizeScreenSample): posS + V_s and det(I + J_s(theta) - J_s(0)).
@@ -429,59 +452,66 @@ double pathValue(const Scene& s, const Params& Pp, const PathRec& r, const Pertu
     const double alpha = -1.0 / (2.0 * 0.25);
     const double g = std::exp(alpha * ((sx - r.pcx) * (sx - r.pcx) + (sy - r.pcy) * (sy - r.pcy)));
     const double g0 = std::exp(alpha * ((r.psx - r.pcx) * (r.psx - r.pcx) + (r.psy - r.pcy) * (r.psy - r.pcy)));
-    const double thp0 = detS * g / g0;
-    const V3d dir1 = normalizeSafeD(camRaw(cam, sx, sy, Pp.W, Pp.H));
-    // AttachToRay at the primary hit (loadVertexPosW with a differentiable ray).
-    auto attach = [&](const Perturbed& Q, V3d o, V3d d, V3d& x, V3d& n) {
+    // AttachToRay (loadVertexPosW with a differentiable ray): hit point and interpolated normal.
+    auto attach = [&](const Perturbed& Q, uint32_t tri, V3d o, V3d d, V3d& x, V3d& n) {
         V3d v[3];
-        Q.tri(r.tri1, v);
+        Q.tri(tri, v);
         const double t = computeRayDistanceToTriangle(v, o, d);
         const V3d p = o + d * t;
         const V3d b = computeBarycentrics(v, p);
         x = v[0] * b.x + v[1] * b.y + v[2] * b.z;
-        const uint32_t* id = &s.idx[3 * r.tri1];
+        const uint32_t* id = &s.idx[3 * tri];
         n = Q.nrm(id[0]) * b.x + Q.nrm(id[1]) * b.y + Q.nrm(id[2]) * b.z;
     };
-    V3d x1, n1, x10, n10;
-    attach(P, cam.pos, dir1, x1, n1);
-    const V3d dir10 = normalizeSafeD(camRaw(cam, r.psx, r.psy, Pp.W, Pp.H));
-    attach(P0, cam.pos, dir10, x10, n10);
-    const V3d V1 = -dir1;
-    V3d L = r.E1;
-    // The ray origins follow the hit point (fwd/bwd_computeRayOrigin: identity derivative).
-    if (r.lit) {
-        // computeNEE: wo = normalize(y - x1) toward the held light point y, and the area factor
-        // a2s / detach(a2s). With neeHeld the direction stays at its sampled value and the factor
-        // is 1.
-        V3d wo0 = r.lightPos - x10;
-        const double dist0 = length(wo0);
-        wo0 = wo0 / dist0;
-        V3d wo = Pp.neeHeld ? wo0 : r.lightPos - x1;
-        const double dist = Pp.neeHeld ? dist0 : length(wo);
-        if (!Pp.neeHeld) wo = wo / dist;
-        const V3d oN = r.nee.origin0 + (x1 - x10);
+    V3d thp(detS * g / g0);
+    V3d L(0.0);
+    V3d o = cam.pos, o0 = cam.pos;
+    V3d d = normalizeSafeD(camRaw(cam, sx, sy, Pp.W, Pp.H)), d0 = normalizeSafeD(camRaw(cam, r.psx, r.psy, Pp.W, Pp.H));
+    for (const VertexRec& v : r.v) {
+        V3d x, n, x0, n0;
+        attach(P, v.tri, o, d, x, n);
+        attach(P0, v.tri, o0, d0, x0, n0);
+        const V3d view = -d;
+        if (v.emit) L = L + thp * v.E * v.misE;
+        // The rays leaving the vertex start at its offset point, which follows the vertex
+        // (fwd/bwd_computeRayOrigin: identity derivative).
+        if (v.lit) {
+            // computeNEE: wo = normalize(y - x) toward the held light point y, and the area factor
+            // a2s / detach(a2s). With neeHeld the direction stays at its sampled value and the
+            // factor is 1.
+            V3d wo0 = v.lightPos - x0;
+            const double dist0 = length(wo0);
+            wo0 = wo0 / dist0;
+            V3d wo = Pp.neeHeld ? wo0 : v.lightPos - x;
+            const double dist = Pp.neeHeld ? dist0 : length(wo);
+            if (!Pp.neeHeld) wo = wo / dist;
+            const V3d oN = v.nee.origin0 + (x - x0);
+            V3d V;
+            double detJ;
+            rayWarp(P, P0, v.nee, oN, v.nee.origin0, wo, wo0, V, detJ);
+            const V3d rd = wo + V;
+            const V3d f = evalBsdf(s, v, n, view, rd);
+            // A light sample in the receiver's tangent plane (a hit point flush with the light) has a
+            // zero BSDF value and an unbounded area factor; its contribution is zero (in FP32 the
+            // factor cancels to 1 at the base point).
+            const double a2s = dist * dist / dot(v.lightN, -rd), a2s0 = dist0 * dist0 / dot(v.lightN, -wo0);
+            if (anyPositive(f) && a2s0 > 0.0 && std::isfinite(a2s0))
+                L = L + thp * v.Li * f * (Pp.neeHeld ? 1.0 : a2s / a2s0) * (v.misN * detJ);
+        }
+        if (!v.scat) break;
+        const V3d oS = v.oS0 + (x - x0);
         V3d V;
         double detJ;
-        rayWarp(P, P0, r.nee, oN, r.nee.origin0, wo, wo0, V, detJ);
-        const V3d rd = wo + V;
-        const V3d f = evalBsdf(s, r, n1, V1, rd);
-        // A light sample in the receiver's tangent plane (a hit point flush with the light) has a
-        // zero BSDF value and an unbounded area factor; its contribution is zero (in FP32 the
-        // factor cancels to 1 at the base point).
-        const double a2s = dist * dist / dot(r.lightN, -rd), a2s0 = dist0 * dist0 / dot(r.lightN, -wo0);
-        if (anyPositive(f) && a2s0 > 0.0 && std::isfinite(a2s0))
-            L = L + r.Li * f * (Pp.neeHeld ? 1.0 : a2s / a2s0) * (r.misN * detJ);
+        rayWarp(P, P0, v.sc, oS, v.oS0, v.woS, v.woS, V, detJ);
+        const V3d rd = v.woS + V;
+        const V3d f = evalBsdf(s, v, n, view, rd);
+        thp = thp * f * (detJ / v.pdfS);
+        o = oS;
+        o0 = v.oS0;
+        d = rd;
+        d0 = v.woS;
     }
-    if (r.scat && r.emit2) {
-        const V3d oS = r.sc.origin0 + (x1 - x10);
-        V3d V;
-        double detJ;
-        rayWarp(P, P0, r.sc, oS, r.sc.origin0, r.woS, r.woS, V, detJ);
-        const V3d rd = r.woS + V;
-        const V3d f = evalBsdf(s, r, n1, V1, rd);
-        L = L + f * (detJ / r.pdfS) * r.E2 * r.mis2;
-    }
-    return dot(r.dL, L) * thp0;
+    return dot(r.dL, L);
 }
 
 }  // namespace
@@ -490,7 +520,7 @@ double pathValue(const Scene& s, const Params& Pp, const PathRec& r, const Pertu
 using namespace fo;
 
 extern "C" {
-// Directional derivatives of the WARDiffPathTracer loss sum_pixels dLdI . I at maxBounces = 0
+// Directional derivatives of the WARDiffPathTracer loss sum_pixels dLdI . I at maxBounces
 // (useWAR = 1) along nprobe perturbation fields, and the loss itself. probes: [nprobe][nV][6]
 // (position then normal perturbation of each scene vertex). The image I is the per-pixel sum
 // over the spp samples with seeds frame0 + k (the adjoint's per-sample accumulation, not the
@@ -499,13 +529,13 @@ extern "C" {
 // out[k] = sum_paths dLdI . dF/dtheta along probe k by central differences with step h (checked
 // against step h/2 per path, see below); value
 // receives sum_paths dLdI . F(0) (the primal, for the record check).
-void fo_war_geom_probe(void* hs, uint32_t W, uint32_t H, uint32_t spp, uint32_t frame0, uint32_t aux, float log10Kappa,
-                       float log10KappaScreen, uint32_t flags, const float* dLdI, uint32_t nprobe, const float* probes,
-                       double h, int threads, double* out, double* value) {
+void fo_war_geom_probe_b(void* hs, uint32_t W, uint32_t H, uint32_t spp, uint32_t frame0, uint32_t maxBounces, uint32_t aux,
+                         float log10Kappa, float log10KappaScreen, uint32_t flags, const float* dLdI, uint32_t nprobe,
+                         const float* probes, double h, int threads, double* out, double* value) {
     const Scene& s = *static_cast<Scene*>(hs);
     Params P{W, H, aux, float(std::pow(10.0, double(log10Kappa))), float(std::pow(10.0, double(log10KappaScreen))),
              (flags & 32) != 0, (flags & 1) != 0, (flags & 2) != 0, (flags & 4) != 0, (flags & 8) != 0,
-             (flags & 64) != 0};
+             (flags & 64) != 0, maxBounces};
     const size_t nV = s.pos.size();
     // Position and normal perturbation of each probe, [nV][3] each.
     std::vector<std::vector<float>> dps(nprobe, std::vector<float>(3 * nV)), dns(nprobe, std::vector<float>(3 * nV));
@@ -569,5 +599,12 @@ void fo_war_geom_probe(void* hs, uint32_t W, uint32_t H, uint32_t spp, uint32_t 
         if (q < nprobe) out[q] = sum;
         else if (value) *value = sum;
     }
+}
+// The same at maxBounces = 0 (direct illumination).
+void fo_war_geom_probe(void* hs, uint32_t W, uint32_t H, uint32_t spp, uint32_t frame0, uint32_t aux, float log10Kappa,
+                       float log10KappaScreen, uint32_t flags, const float* dLdI, uint32_t nprobe, const float* probes,
+                       double h, int threads, double* out, double* value) {
+    fo_war_geom_probe_b(hs, W, H, spp, frame0, 0, aux, log10Kappa, log10KappaScreen, flags, dLdI, nprobe, probes, h, threads,
+                        out, value);
 }
 }  // extern "C"

@@ -73,6 +73,8 @@ def oracle_lib():
             "fo_render_war": (None, [vp, u32, u32, u32, u32, u32, u32, i32, vp, vp]),
             "fo_war_geom_probe": (None, [vp, u32, u32, u32, u32, u32, f32, f32, u32, vp, u32, vp, C.c_double, i32, vp,
                                          vp]),
+            "fo_war_geom_probe_b": (None, [vp, u32, u32, u32, u32, u32, u32, f32, f32, u32, vp, u32, vp, C.c_double, i32,
+                                           vp, vp]),
             "fo_render_war_rect": (None, [vp, u32, u32, u32, u32, u32, u32, u32, u32, u32, u32, i32, vp, vp]),
         }
         for name, (res, args) in sig.items():
@@ -254,9 +256,9 @@ class OracleScene:
 
     def war_geom_probe(self, W, H, spp, frame, dLdI, probes, aux=16, log10_kappa=3.0, log10_kappa_screen=3.0,
                        bsdf_sampling=True, nee=True, mis=True, antithetic=True, use_war=True, nee_direction_held=False,
-                       h=1e-5, threads=0):
+                       h=1e-5, threads=0, max_bounces=0):
         """Directional derivatives of sum_pixels dLdI . I (I: per-pixel sum over the spp samples with
-        seeds frame + k) at maxBounces = 0 along perturbation fields probes [n, num_vertices, 6]
+        seeds frame + k) at max_bounces along perturbation fields probes [n, num_vertices, 6]
         (position, normal), from the oracle's restatement of WARDiffPathTracer's estimator with
         all sampled decisions held (fo_war_geom.cpp). Returns (derivatives [n], loss)."""
         d = np.ascontiguousarray(dLdI, np.float32).reshape(H, W, 3)
@@ -266,8 +268,9 @@ class OracleScene:
         val = np.zeros(1, np.float64)
         flags = int(bsdf_sampling) | (int(nee) << 1) | (int(mis) << 2) | (int(antithetic) << 3) | (int(use_war) << 5) | \
             (int(nee_direction_held) << 6)
-        oracle_lib().fo_war_geom_probe(self.h, W, H, spp, frame, aux, float(log10_kappa), float(log10_kappa_screen), flags,
-                                       _p(d), pr.shape[0], _p(pr), float(h), threads, _p(out), _p(val))
+        oracle_lib().fo_war_geom_probe_b(self.h, W, H, spp, frame, max_bounces, aux, float(log10_kappa),
+                                         float(log10_kappa_screen), flags, _p(d), pr.shape[0], _p(pr), float(h), threads,
+                                         _p(out), _p(val))
         return out, float(val[0])
 
     def render_war(self, W, H, spp=1, frame=0, max_bounces=0, bsdf_sampling=True, nee=True, mis=True, antithetic=True,

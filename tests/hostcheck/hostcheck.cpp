@@ -275,13 +275,15 @@ int main(int argc, char** argv) {
                 in >> W >> H >> spp >> frame >> aux >> lk >> lks >> mesh >> hash >> lds;
                 if (!in) throw std::runtime_error("bad step: " + line);
                 std::string gradOut;
-                uint32_t useNEE = 1, neeHeld = 0;
-                bool useLog = false;
+                uint32_t useNEE = 1, neeHeld = 0, bounces = 0;
+                bool useLog = false, general = false;  // general: the recorded reverse sweep also at 0 bounces
                 for (std::string opt; in >> opt;) {
                     if (opt.rfind("out=", 0) == 0) gradOut = opt.substr(4);
                     else if (opt == "nee=0") useNEE = 0;
                     else if (opt == "neeheld=1") neeHeld = 1;
                     else if (opt == "log=1") useLog = true;
+                    else if (opt.rfind("b=", 0) == 0) bounces = uint32_t(std::stoul(opt.substr(2)));
+                    else if (opt == "general=1") general = true;
                     else throw std::runtime_error("bad step option: " + opt);
                 }
                 if (ndl != 3 * size_t(W) * H) throw std::runtime_error("dLdI size does not match the frame");
@@ -310,8 +312,13 @@ int main(int argc, char** argv) {
                 // The aux-hit log of the one thread here: exact size (3 words per aux ray).
                 std::unique_ptr<uint32_t[]> logBuf(useLog ? new uint32_t[3 * size_t(aux)]() : nullptr);
                 const AuxLog lg = useLog ? AuxLog{logBuf.get(), 1u} : noAuxLog();
+                // The vertex records of the one thread here, exact size.
+                a.maxBounces = bounces;
+                std::unique_ptr<uint32_t[]> vtxBuf(new uint32_t[size_t(kGeomVertexWords) * (bounces + 2)]());
+                const VertexLog vl{vtxBuf.get(), 1u, bounces + 2};
                 for (uint32_t id = 0; id < n; id++) {
-                    geomItem(s, a, id, ls.st, nc, lg);
+                    if (bounces == 0 && !general) geomItem<true>(s, a, id, ls.st, nc, lg, vl);
+                    else geomItem<false>(s, a, id, ls.st, nc, lg, vl);
                     peak = std::max(peak, ls.st.peak);
                 }
                 {

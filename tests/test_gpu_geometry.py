@@ -1,4 +1,4 @@
-"""WARDiffPathTracer vertex gradients (warped-area reparameterization, useWAR = 1, maxBounces = 0).
+"""WARDiffPathTracer vertex gradients (warped-area reparameterization, useWAR = 1).
 
 * Parity: the kernel's MeshPosition / MeshNormal gradients equal the oracle's restatement of the
   reference's estimator on the same seeds (test_vertex_gradients_match_oracle_estimator; the
@@ -125,14 +125,16 @@ def test_scale_gradient_matches_fd(gpu):
     assert abs(ad - fd) <= TOL * abs(fd)
 
 
-@pytest.mark.parametrize("aux,log_kappa,held", [(16, 3, False), (64, 4, False), (16, 3, True)])
-def test_vertex_gradients_match_oracle_estimator(gpu, oracle_mod, aux, log_kappa, held):
+@pytest.mark.parametrize("aux,log_kappa,held,bounces", [(16, 3, False, 0), (64, 4, False, 0), (16, 3, True, 0),
+                                                        (16, 3, True, 1), (16, 3, False, 2), (64, 4, True, 3)])
+def test_vertex_gradients_match_oracle_estimator(gpu, oracle_mod, aux, log_kappa, held, bounces):
     """The kernel's MeshPosition / MeshNormal gradients against the oracle's restatement of the
     same estimator (oracle/fo_war_geom.cpp: the path radiance with every sampled decision held,
     differentiated by central differences in double precision), same seeds: projections on three
     translations, a uniform scale, a random position field and a random normal field, each within
     2e-3 of the oracle's value (FP32 atomics on the GPU against FP64). 32 x 32, 8 spp, with the
-    reference's NEE formulation and with the held NEE direction (useSolidAngleNEEWarp)."""
+    reference's NEE formulation and with the held NEE direction (useSolidAngleNEEWarp), at 0 to 3
+    bounces (every vertex attached to its incoming ray, moving every later vertex)."""
     import torch
     from falcor_amd.render_graph import Scene, SceneGradients, WARDiffPathTracer
     W2 = H2 = 32
@@ -142,7 +144,7 @@ def test_vertex_gradients_match_oracle_estimator(gpu, oracle_mod, aux, log_kappa
     off = data.mesh_vertex_offsets
     v0, v1 = int(off[mesh]), int(off[mesh + 1])
     sc = Scene(data)
-    p = WARDiffPathTracer({"maxBounces": 0, "useWAR": True, "diffMode": "BackwardDiff", "auxSampleCount": aux,
+    p = WARDiffPathTracer({"maxBounces": bounces, "useWAR": True, "diffMode": "BackwardDiff", "auxSampleCount": aux,
                            "Log10vMFConcentration": log_kappa, "Log10vMFConcentrationScreen": log_kappa,
                            "useSolidAngleNEEWarp": int(held)})
     p.set_scene(sc)
@@ -168,10 +170,10 @@ def test_vertex_gradients_match_oracle_estimator(gpu, oracle_mod, aux, log_kappa
     pr[4, v0:v1, 0:3] = rng.normal(size=(v1 - v0, 3))
     pr[5, v0:v1, 3:6] = rng.normal(size=(v1 - v0, 3))
     ref, _ = osc.war_geom_probe(W2, H2, spp, frame * spp, d, pr, aux=aux, log10_kappa=log_kappa,
-                                log10_kappa_screen=log_kappa, nee_direction_held=held)
+                                log10_kappa_screen=log_kappa, nee_direction_held=held, max_bounces=bounces)
     got = np.array([(gp * pr[k, v0:v1, 0:3]).sum() + (gn * pr[k, v0:v1, 3:6]).sum() for k in range(6)])
     rel = np.abs(got - ref) / np.abs(ref)
-    print(f"aux {aux} kappa 1e{log_kappa} held {held}: kernel {np.round(got, 4)} oracle {np.round(ref, 4)} "
+    print(f"aux {aux} kappa 1e{log_kappa} held {held} bounces {bounces}: kernel {np.round(got, 4)} oracle {np.round(ref, 4)} "
           f"rel {np.array2string(rel, precision=2)}", flush=True)
     assert np.all(np.abs(ref) > 0)
     assert np.all(rel <= 2e-3)
@@ -208,12 +210,12 @@ def test_war_backward_argument_errors(gpu):
     import torch
     from falcor_amd import _lib
     from falcor_amd.render_graph import SceneGradients
-    sc, p, mesh = _setup({"maxBounces": 1})
+    sc, p, mesh = _setup({"maxBounces": 9})
     g = SceneGradients(sc, hash_size=4, texture_grads=False, mesh_id=mesh)
     p.scene_gradients = g
     p.set_mesh_to_optimize(mesh)
     p.dL_dI = torch.ones((H, W, 3), device="cuda")
-    with pytest.raises(_lib.FhpError, match="maxBounces"):
+    with pytest.raises(_lib.FhpError, match="maxBounces up to 8"):
         p.backward(W, H, 1, frame=0)
     p.set_property("maxBounces", 0)
     p.set_property("auxSampleCount", 15)

@@ -118,3 +118,68 @@ def test_reference_nee_differs_from_held_only_through_nee(box):
                                  nee_direction_held=True)
     print(f"vertical translation: reference NEE {ref[1] / spp:.2f}, held NEE {held[1] / spp:.2f}", flush=True)
     assert ref[1] / held[1] > 1.4
+
+
+# ------------------------------------------------------------------ maxBounces > 0
+# Every vertex of a longer path is attached to its incoming ray, whose origin follows the previous
+# vertex and whose direction carries the previous vertex's warp, so a vertex's motion propagates
+# to every later vertex. The same three pins hold at one and two bounces.
+
+def _fd_b(osc, data, v0, v1, field, kind, eps, spp, bounces):
+    p0, n0 = data.positions.copy(), data.normals.copy()
+    vals = []
+    for sg in (1, -1):
+        p, n = p0.copy(), n0.copy()
+        if kind == "pos":
+            p[v0:v1] += sg * eps * field
+        else:
+            n[v0:v1] += sg * eps * field
+        osc.animate(p, n)
+        im, _ = osc.render_war(W, H, spp=spp, frame=0, max_bounces=bounces)
+        vals.append(float((im.astype(np.float64) * _ramp()).sum()))
+    osc.animate(p0, n0)
+    return (vals[0] - vals[1]) / (2 * eps)
+
+
+@pytest.mark.parametrize("bounces", [1, 2])
+def test_restatement_reproduces_the_primal_multibounce(box, bounces):
+    data, v0, v1, osc = box
+    spp = 8
+    pr = _position_probes(data, v0, v1, osc.num_vertices)[:1]
+    _, val = osc.war_geom_probe(W, H, spp, 40, _ramp(), pr, aux=16, log10_kappa=3, log10_kappa_screen=3,
+                                max_bounces=bounces)
+    img, _ = osc.render_war(W, H, spp=spp, frame=40, max_bounces=bounces)
+    ref = float((img.astype(np.float64) * _ramp()).sum() * spp)
+    assert val == pytest.approx(ref, rel=1e-6)
+    _, val0 = osc.war_geom_probe(W, H, spp, 40, _ramp(), pr, aux=16, log10_kappa=3, log10_kappa_screen=3)
+    assert val > val0 * 1.05  # the indirect light is there
+
+
+def test_normal_gradients_match_fd_two_bounces(box):
+    data, v0, v1, osc = box
+    rng = np.random.default_rng(4)
+    fields = [rng.normal(size=(v1 - v0, 3)).astype(np.float32), np.tile(np.float32([0, 1, 0]), (v1 - v0, 1))]
+    pr = np.zeros((2, osc.num_vertices, 6), np.float32)
+    for k, f in enumerate(fields):
+        pr[k, v0:v1, 3:6] = f
+    spp = 1024
+    d, _ = osc.war_geom_probe(W, H, spp, 0, _ramp(), pr, aux=16, log10_kappa=3, log10_kappa_screen=3, max_bounces=2)
+    for k, f in enumerate(fields):
+        fd = _fd_b(osc, data, v0, v1, f, "nrm", 1e-2, 1024, 2)
+        est = d[k] / spp
+        print(f"two bounces, normal field {k}: estimator {est:.5f}  fd {fd:.5f}", flush=True)
+        assert abs(est - fd) <= 0.1 * abs(fd)
+
+
+@pytest.mark.parametrize("bounces", [1, 2])
+def test_held_nee_position_gradients_match_fd_multibounce(box, bounces):
+    data, v0, v1, osc = box
+    pr = _position_probes(data, v0, v1, osc.num_vertices)
+    spp = 512
+    d, _ = osc.war_geom_probe(W, H, spp, 100 * spp, _ramp(), pr, aux=64, log10_kappa=4, log10_kappa_screen=4,
+                              nee_direction_held=True, max_bounces=bounces)
+    est = d / spp
+    fd = np.array([_fd_b(osc, data, v0, v1, pr[k, v0:v1, 0:3], "pos", 4e-3, 1024, bounces) for k in range(4)])
+    rel = float(np.linalg.norm(est - fd) / np.linalg.norm(fd))
+    print(f"{bounces} bounces, held NEE: estimator {np.round(est, 3)}  fd {np.round(fd, 3)}  rel L2 {rel:.3f}", flush=True)
+    assert rel <= 0.2
