@@ -98,7 +98,7 @@ class DiffRenderer:
     """Differentiable WARDiffPathTracer renderer of one scene.
 
     props are WARDiffPathTracer properties; mesh_id selects the mesh whose vertices receive
-    gradients (useWAR = 1, maxBounces = 0 for vertex gradients).
+    gradients (useWAR = 1, maxBounces up to 8 for vertex gradients).
     """
 
     def __init__(self, scene: Scene, width: int, height: int, spp_primal: int = 16, spp_grad: int = 16,

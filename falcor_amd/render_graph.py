@@ -617,7 +617,7 @@ class WARDiffPathTracer(_NativePass):
     def backward(self, width: int, height: int, spp: int = 1, frame: int | None = None, stats: bool = False):
         """Accumulates d(sum dL_dI . color)/d(params) into scene_gradients (raw sum over samples).
         With useWAR, vertex position/normal gradients of the mesh set by set_mesh_to_optimize
-        are accumulated too (maxBounces = 0)."""
+        are accumulated too (maxBounces up to 8, reflective materials)."""
         debug = self.get_property("diffMode") == 3  # BackwardDiffDebug: dL/dI = (1, 0, 0) everywhere
         if self.scene_gradients is None or (self.dL_dI is None and not debug):
             raise _lib.FhpError("backward needs scene_gradients and dL_dI")

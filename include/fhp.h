@@ -314,6 +314,10 @@ int fhp_grads_copy(FhpGrads grads, int which, float* d_dst, uint64_t num_floats,
  * height * width * 3 floats. Frame/sample numbering as in fhp_pass_execute. Needs diffMode
  * BackwardDiff (1), or BackwardDiffDebug (3), where dLdI = (1, 0, 0) at every pixel and d_dLdI
  * is ignored (WARDiffPathTracer.rt.slang:201-211); other modes are refused.
+ * With useWAR = 1 the vertex gradients of the mesh enabled on grads are accumulated too, for
+ * maxBounces up to 8 and scenes without transmissive materials (other cases are refused).
+ * Extension property useHitLog (default 1): the replay reads the first pass's traversal answers
+ * from a per-pass hit log instead of tracing again; 0 traces again (same gradients).
  * Replaces rayGen's bwd_diff(tracePaths) branch (WARDiffPathTracer.rt.slang:161-180) and
  * the host loop of render_grad (scripts/inv-rendering/common.py:90-114). */
 int fhp_pass_backward(FhpPass pass, FhpScene scene, uint32_t width, uint32_t height, uint32_t spp, uint32_t frame,
