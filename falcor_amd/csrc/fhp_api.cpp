@@ -488,7 +488,7 @@ int buildAndUpload(FhpScene_* s) {
     HIP_OR_FAIL(s->lights.upload(h.lights));
     HIP_OR_FAIL(s->active.upload(h.activeLights));
     HIP_OR_FAIL(s->triToLight.upload(h.triToLight));
-    HIP_OR_FAIL(s->lbvh.upload(h.lbvh));
+    HIP_OR_FAIL(s->lbvh.upload(h.lbvhPacked));
     HIP_OR_FAIL(s->lbvhTri.upload(h.lbvhTri));
     HIP_OR_FAIL(s->lbvhMask.upload(h.lbvhMask));
     HIP_OR_FAIL(s->meshTri.upload(h.meshTriOff));
@@ -522,7 +522,7 @@ int buildAndUpload(FhpScene_* s) {
     dv.lights = s->lights.as<float4>();
     dv.activeLights = s->active.as<uint32_t>();
     dv.triToLight = s->triToLight.as<uint32_t>();
-    dv.lbvh = s->lbvh.as<float4>();
+    dv.lbvh = s->lbvh.as<uint4>();
     dv.lbvhTri = s->lbvhTri.as<uint32_t>();
     dv.lbvhMask = s->lbvhMask.as<uint2>();
     dv.cam.pos = mk3(h.cam[0], h.cam[1], h.cam[2]);
@@ -751,7 +751,7 @@ int fhp_scene_update_node_transform(FhpScene s, uint32_t mesh, const float* h_ma
         l.nLights = h.nLights;
         l.vtx = s->vtx.as<float4>();
         l.idx = s->idx.as<uint32_t>();
-        l.lbvh = s->lbvh.as<float>();
+        l.lbvh = s->lbvh.as<uint32_t>();
         l.lbvhTri = s->lbvhTri.as<uint32_t>();
         l.nodes = s->lrefit.as<uint32_t>();
         l.off = h.lrefitOff.data();
@@ -759,9 +759,10 @@ int fhp_scene_update_node_transform(FhpScene s, uint32_t mesh, const float* h_ma
         HIP_OR_FAIL(launchLightRefit(l, st));
         // Host mirrors of the light records and light BVH (fhp_scene_lights / fhp_scene_light_bvh).
         HIP_OR_FAIL(hipMemcpyAsync(h.lights.data(), s->lights.p, h.lights.size() * 4, hipMemcpyDeviceToHost, st));
-        if (!h.lbvh.empty())
-            HIP_OR_FAIL(hipMemcpyAsync(h.lbvh.data(), s->lbvh.p, h.lbvh.size() * 4, hipMemcpyDeviceToHost, st));
+        if (!h.lbvhPacked.empty())
+            HIP_OR_FAIL(hipMemcpyAsync(h.lbvhPacked.data(), s->lbvh.p, h.lbvhPacked.size() * 4, hipMemcpyDeviceToHost, st));
         HIP_OR_FAIL(hipStreamSynchronize(st));
+        decodeLightNodes(h);
     }
     return FHP_OK;
 }

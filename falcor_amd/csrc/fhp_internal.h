@@ -265,7 +265,7 @@ struct LightRefitArgs {
     uint32_t nLights;
     const float4* vtx;
     const uint32_t* idx;
-    float* lbvh;                // 16 floats per node
+    uint32_t* lbvh;             // packed nodes, 8 words each
     const uint32_t* lbvhTri;
     const uint32_t* nodes;      // device: refit schedule
     const uint32_t* off;        // host: depth + 2 offsets (internal nodes by depth, then leaves)

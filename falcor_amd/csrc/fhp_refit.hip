@@ -137,20 +137,34 @@ __global__ void kUpdateLights(float* __restrict__ lights, uint32_t n, const floa
 }
 
 constexpr float kInvalidCone = -1.f;  // LightBVHTypes.slang:43 (kInvalidCosConeAngle)
-// Node record: (origin, flux) (extent, cosCone) (coneDir, 0) (right child | leaf info).
-__device__ __forceinline__ void setAABB(float* q, f3 lo, f3 hi) {
-    const f3 o = (hi + lo) * 0.5f, e = (hi - lo) * 0.5f;
-    q[0] = o.x; q[1] = o.y; q[2] = o.z;
-    q[4] = e.x; q[5] = e.y; q[6] = e.z;
+// Packed light-BVH nodes (packLightNode): the refit reads and writes the packed layout, as
+// LightBVHRefit.cs.slang does through PackedNode; internal nodes see their children's decoded
+// (quantised) attributes.
+__device__ __forceinline__ LNodeAttr readLightNode(const uint32_t* lbvh, uint32_t n) {
+    const uint4* q = reinterpret_cast<const uint4*>(lbvh) + 2 * size_t(n);
+    return unpackLightNode(q[0], q[1]);
+}
+__device__ __forceinline__ void writeLightNode(uint32_t* lbvh, uint32_t n, const LNodeAttr& a) {
+    uint32_t w[8];
+    packLightNode(w, lbvh[8 * size_t(n)], a);
+    uint4* q = reinterpret_cast<uint4*>(lbvh) + 2 * size_t(n);
+    q[0] = make_uint4(w[0], w[1], w[2], w[3]);
+    q[1] = make_uint4(w[4], w[5], w[6], w[7]);
+}
+__device__ __forceinline__ void setAABB(LNodeAttr& a, f3 lo, f3 hi) {
+    a.origin = (hi + lo) * 0.5f;
+    a.extent = (hi - lo) * 0.5f;
 }
 
 // LightBVHRefit.cs.slang:46-93 (updateLeafNodes).
-__global__ void kLightRefitLeaves(float* __restrict__ lbvh, const uint32_t* __restrict__ nodes, uint32_t count,
+__global__ void kLightRefitLeaves(uint32_t* __restrict__ lbvh, const uint32_t* __restrict__ nodes, uint32_t count,
                                   const float* __restrict__ lights, const uint32_t* __restrict__ lbvhTri) {
     const uint32_t k = blockIdx.x * kRefitBlock + threadIdx.x;
     if (k >= count) return;
-    float* q = lbvh + 16 * size_t(nodes[k]);
-    const uint32_t first = __float_as_uint(q[13]), cnt = __float_as_uint(q[14]);
+    const uint32_t n = nodes[k];
+    const uint32_t w0 = lbvh[8 * size_t(n)];
+    const uint32_t first = w0 & kLNodeOffsetMask, cnt = (w0 >> kLNodeCountShift) & kLNodeCountMask;
+    LNodeAttr a = readLightNode(lbvh, n);
     f3 lo = mk3(kFltMax), hi = mk3(-kFltMax), sum = mk3(0.f);
     for (uint32_t i = 0; i < cnt; i++) {
         const float* r = lights + 20 * size_t(lbvhTri[first + i]);
@@ -161,7 +175,7 @@ __global__ void kLightRefitLeaves(float* __restrict__ lbvh, const uint32_t* __re
         }
         sum = sum + mk3(r[12], r[13], r[14]);
     }
-    setAABB(q, lo, hi);
+    setAABB(a, lo, hi);
     const float len = length(sum);
     const f3 dir = sum / len;
     float cosCone = kInvalidCone;
@@ -173,26 +187,28 @@ __global__ void kLightRefitLeaves(float* __restrict__ lbvh, const uint32_t* __re
         }
         cosCone = fmaxf(cosCone, -1.f);
     }
-    q[7] = cosCone;
-    q[8] = dir.x; q[9] = dir.y; q[10] = dir.z;
+    a.cosCone = cosCone;
+    a.coneDir = dir;
+    writeLightNode(lbvh, n, a);
 }
 
 __device__ __forceinline__ float sinFromCos(float c) { return sqrtf(fmaxf(0.f, 1.f - c * c)); }
 
 // LightBVHRefit.cs.slang:101-170 (updateInternalNodes): box union, and the normal cone that holds
 // both children's cones (each child's spread rotated away from the merged direction).
-__global__ void kLightRefitInternal(float* __restrict__ lbvh, const uint32_t* __restrict__ nodes, uint32_t count) {
+__global__ void kLightRefitInternal(uint32_t* __restrict__ lbvh, const uint32_t* __restrict__ nodes, uint32_t count) {
     const uint32_t k = blockIdx.x * kRefitBlock + threadIdx.x;
     if (k >= count) return;
     const uint32_t ni = nodes[k];
-    float* q = lbvh + 16 * size_t(ni);
-    const float* L = lbvh + 16 * size_t(ni + 1);
-    const float* R = lbvh + 16 * size_t(__float_as_uint(q[12]));
-    const f3 lo = mk3(fminf(L[0] - L[4], R[0] - R[4]), fminf(L[1] - L[5], R[1] - R[5]), fminf(L[2] - L[6], R[2] - R[6]));
-    const f3 hi = mk3(fmaxf(L[0] + L[4], R[0] + R[4]), fmaxf(L[1] + L[5], R[1] + R[5]), fmaxf(L[2] + L[6], R[2] + R[6]));
-    setAABB(q, lo, hi);
-    const f3 ld = mk3(L[8], L[9], L[10]), rd = mk3(R[8], R[9], R[10]);
-    const float lc = L[7], rc = R[7];
+    LNodeAttr a = readLightNode(lbvh, ni);
+    const LNodeAttr L = readLightNode(lbvh, ni + 1), R = readLightNode(lbvh, lbvh[8 * size_t(ni)]);
+    const f3 lo = mk3(fminf(L.origin.x - L.extent.x, R.origin.x - R.extent.x), fminf(L.origin.y - L.extent.y, R.origin.y - R.extent.y),
+                      fminf(L.origin.z - L.extent.z, R.origin.z - R.extent.z));
+    const f3 hi = mk3(fmaxf(L.origin.x + L.extent.x, R.origin.x + R.extent.x), fmaxf(L.origin.y + L.extent.y, R.origin.y + R.extent.y),
+                      fmaxf(L.origin.z + L.extent.z, R.origin.z + R.extent.z));
+    setAABB(a, lo, hi);
+    const f3 ld = L.coneDir, rd = R.coneDir;
+    const float lc = L.cosCone, rc = R.cosCone;
     const f3 sum = ld + rd;
     const float len = length(sum);
     const f3 dir = sum / len;
@@ -209,8 +225,9 @@ __global__ void kLightRefitInternal(float* __restrict__ lbvh, const uint32_t* __
             cosCone = fmaxf(fminf(cosLt, cosRt), -1.f);
         }
     }
-    q[7] = cosCone;
-    q[8] = dir.x; q[9] = dir.y; q[10] = dir.z;
+    a.cosCone = cosCone;
+    a.coneDir = dir;
+    writeLightNode(lbvh, ni, a);
 }
 
 uint32_t blocksFor(uint64_t n) { return uint32_t((n + kRefitBlock - 1) / kRefitBlock); }

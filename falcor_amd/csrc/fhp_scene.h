@@ -7,7 +7,7 @@
 //   idx      3 x u32 per triangle (original order); vtx 32 B per vertex: (p.xyz, n.x) (n.yz, uv).
 //   triMat   u32 material id per triangle; mats 80 B per material (20 floats, see MatField).
 //   texels   RGB fp32 texels of all textures; texDesc int4 (offset, width, height, 0).
-//   lights   80 B per emissive triangle; lbvh 64 B per light-BVH node.
+//   lights   80 B per emissive triangle; lbvh 32 B per light-BVH node.
 #pragma once
 #include "fhp_math.h"
 
@@ -62,7 +62,7 @@ struct SceneDev {
     const float4* lights;        // 5 x float4 per light: (p0, area) (p1, flux) (p2, tri) (normal, 0) (Le, 0)
     const uint32_t* activeLights;
     const uint32_t* triToLight;
-    const float4* lbvh;          // 4 x float4 per node: (origin, flux) (extent, cosCone) (coneDir, 0) (info)
+    const uint4* lbvh;           // 2 x uint4 per node, the packed layout of packLightNode
     const uint32_t* lbvhTri;
     const uint2* lbvhMask;
     CameraDev cam;
@@ -785,15 +785,110 @@ FHP_HD float triPdf(const SceneDev& s, f3 posW, uint32_t li, f3 hitPos, f3 hitN)
     return d2 / denom;
 }
 
+// ------------------------------------------------------------------ packed light-BVH nodes
+// The reference's PackedNode (LightBVHTypes.slang:97-198), 32 bytes: word 0 the right child's
+// index (internal node, bit 31 clear) or bit 31 | triangle count << 27 | first triangle (leaf);
+// words 1-3 the box centre; word 4 the half extents x, y as IEEE half floats; word 5 the half
+// extent z (low half) and the cosine of the normal cone's spread quantised to 16 bits,
+// (uint)((cos + 1) * 32767) rounding towards a wider cone; word 6 the cone axis in the octahedral
+// map as two 16-bit snorms (encodeNormal2x16, PackedFormats.slang:53-66); word 7 the flux.
+// Sampling uses the decoded values, as the reference does.
+constexpr uint32_t kLNodeCountShift = 27, kLNodeCountMask = 0xfu, kLNodeOffsetMask = (1u << 27) - 1u;
+struct LNodeAttr {
+    f3 origin, extent, coneDir;
+    float cosCone, flux;
+};
+// IEEE 754 binary16 with round to nearest even (overflow to infinity, NaN kept quiet).
+FHP_HD uint32_t f32ToF16(float f) {
+    const uint32_t x = f2u(f), sign = (x >> 16) & 0x8000u;
+    const uint32_t ax = x & 0x7fffffffu;
+    if (ax >= 0x7f800000u) return sign | 0x7c00u | (ax > 0x7f800000u ? 0x200u : 0u);
+    if (ax >= 0x477ff000u) return sign | 0x7c00u;  // rounds to beyond the largest half (65504)
+    if (ax < 0x38800000u) {
+        // Subnormal half (or zero): the value in units of 2^-24, rounded to nearest even.
+        if (ax < 0x33000000u) return sign;  // below half of the smallest subnormal
+        const uint32_t m = (ax & 0x7fffffu) | 0x800000u;
+        const uint32_t shift = 126u - (ax >> 23);  // in units of 2^-24 the value is m * 2^(e - 126)
+        const uint32_t q = m >> shift, rem = m & ((1u << shift) - 1u), half = 1u << (shift - 1u);
+        return sign | (q + ((rem > half || (rem == half && (q & 1u))) ? 1u : 0u));
+    }
+    const uint32_t q = ((ax >> 13) - ((127u - 15u) << 10)), rem = ax & 0x1fffu;
+    return sign | (q + ((rem > 0x1000u || (rem == 0x1000u && (q & 1u))) ? 1u : 0u));
+}
+FHP_HD float f16ToF32(uint32_t h) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    // The hardware conversion (exact, like the bit-field form below).
+    return float(__builtin_bit_cast(_Float16, uint16_t(h)));
+#endif
+    const uint32_t sign = (h & 0x8000u) << 16, e = (h >> 10) & 0x1fu, m = h & 0x3ffu;
+    if (e == 0u) {
+        const float v = float(m) * (1.f / 16777216.f);  // m * 2^-24, exact
+        return sign ? -v : v;
+    }
+    if (e == 31u) return u2f(sign | 0x7f800000u | (m << 13));
+    return u2f(sign | ((e + 112u) << 23) | (m << 13));
+}
+// floatToSnorm16 / unpackSnorm16 (FormatConversion.slang:235-251).
+FHP_HD int32_t floatToSnorm16(float v) {
+    v = v != v ? 0.f : fminf(fmaxf(v, -1.f), 1.f);
+    return int32_t(truncf(v * 32767.f + (v >= 0.f ? 0.5f : -0.5f)));
+}
+FHP_HD float snorm16ToFloat(uint32_t bits16) {
+    const int32_t b = int32_t(bits16 << 16) >> 16;
+    return fmaxf(float(b) / 32767.f, -1.f);
+}
+// ndir_to_oct_snorm / oct_to_ndir_snorm (MathHelpers.slang:156-194) with 16-bit snorms.
+FHP_HD uint32_t encodeOct16(f3 n) {
+    const float inv = 1.f / (fabsf(n.x) + fabsf(n.y) + fabsf(n.z));
+    float px = n.x * inv, py = n.y * inv;
+    if (n.z < 0.f) {
+        const float wx = (1.f - fabsf(py)) * (px >= 0.f ? 1.f : -1.f), wy = (1.f - fabsf(px)) * (py >= 0.f ? 1.f : -1.f);
+        px = wx;
+        py = wy;
+    }
+    return (uint32_t(floatToSnorm16(px)) & 0xffffu) | (uint32_t(floatToSnorm16(py)) << 16);
+}
+FHP_HD f3 decodeOct16(uint32_t w) {
+    float px = snorm16ToFloat(w & 0xffffu), py = snorm16ToFloat(w >> 16);
+    const float z = 1.f - fabsf(px) - fabsf(py);
+    if (z < 0.f) {
+        const float wx = (1.f - fabsf(py)) * (px >= 0.f ? 1.f : -1.f), wy = (1.f - fabsf(px)) * (py >= 0.f ? 1.f : -1.f);
+        px = wx;
+        py = wy;
+    }
+    return normalize(mk3(px, py, z));
+}
+FHP_HD void packLightNode(uint32_t w[8], uint32_t info, const LNodeAttr& a) {
+    w[0] = info;
+    w[1] = f2u(a.origin.x);
+    w[2] = f2u(a.origin.y);
+    w[3] = f2u(a.origin.z);
+    w[4] = f32ToF16(a.extent.x) | (f32ToF16(a.extent.y) << 16);
+    w[5] = f32ToF16(a.extent.z) | (uint32_t((a.cosCone + 1.f) * 32767.f) << 16);
+    w[6] = encodeOct16(a.coneDir);
+    w[7] = f2u(a.flux);
+}
+FHP_HD LNodeAttr unpackLightNode(uint4 q0, uint4 q1) {
+    LNodeAttr a;
+    a.origin = mk3(u2f(q0.y), u2f(q0.z), u2f(q0.w));
+    a.extent = mk3(f16ToF32(q1.x & 0xffffu), f16ToF32(q1.x >> 16), f16ToF32(q1.y & 0xffffu));
+    a.cosCone = float(q1.y >> 16) * (1.f / 32767.f) - 1.f;
+    a.coneDir = decodeOct16(q1.z);
+    a.flux = u2f(q1.w);
+    return a;
+}
+
 // Light BVH node accessors: info = (right child | leaf bit, triOffset, triCount, 0).
 FHP_HD float cosSubClamped(float sA, float cA, float sB, float cB) { return cA > cB ? 1.f : cA * cB + sA * sB; }
 FHP_HD float sinSubClamped(float sA, float cA, float sB, float cB) { return cA > cB ? 0.f : sA * cB - cA * sB; }
 // LightBVHSampler.slang:219-273 (sphere solid-angle bound, bounding and lighting cones)
 FHP_HD float lbvhImportance(const SceneDev& s, f3 posW, f3 n, uint32_t node) {
-    const float4* N = s.lbvh + 4 * size_t(node);
-    float4 q0 = N[0], q1 = N[1], q2 = N[2];
-    f3 origin = mk3(q0.x, q0.y, q0.z), extent = mk3(q1.x, q1.y, q1.z), coneDir = mk3(q2.x, q2.y, q2.z);
-    float flux = q0.w, cosCone = q1.w;
+    // The packed node's fields, decoded as unpackLightNode does; the cone axis only where the
+    // orientation bound needs it.
+    const uint4 q0 = s.lbvh[2 * size_t(node)], q1 = s.lbvh[2 * size_t(node) + 1];
+    const f3 origin = mk3(u2f(q0.y), u2f(q0.z), u2f(q0.w));
+    const f3 extent = mk3(f16ToF32(q1.x & 0xffffu), f16ToF32(q1.x >> 16), f16ToF32(q1.y & 0xffffu));
+    const float flux = u2f(q1.w), cosCone = float(q1.y >> 16) * (1.f / 32767.f) - 1.f;
     float distance = length(origin - posW);
     // boundCosineTerm (LightBVHSampler.slang:181-210) with GeometryHelpers.slang:222-240.
     float sinBC = 0.f, cosBC;
@@ -809,6 +904,7 @@ FHP_HD float lbvhImportance(const SceneDev& s, f3 posW, f3 n, uint32_t node) {
     float orient = 1.f;
     f3 dirTo = (origin - posW) / distance;
     if (cosCone != -1.f && cosCone > 0.f) {
+        const f3 coneDir = decodeOct16(q1.z);
         float sinCone = sqrtf(fmaxf_(0.f, 1.f - cosCone * cosCone));
         float cosT = dot(coneDir, -dirTo);
         float sinT = sqrtf(fmaxf_(0.f, 1.f - cosT * cosT));
@@ -823,8 +919,9 @@ FHP_HD float lbvhImportance(const SceneDev& s, f3 posW, f3 n, uint32_t node) {
     return (flux * NdotL) * orient / (distance * distance);
 }
 FHP_HD uint4 lbvhInfo(const SceneDev& s, uint32_t node) {
-    float4 q = s.lbvh[4 * size_t(node) + 3];
-    return make_uint4(f2u(q.x), f2u(q.y), f2u(q.z), f2u(q.w));
+    const uint32_t w = s.lbvh[2 * size_t(node)].x;
+    if (!(w & kLeafBit)) return make_uint4(w, 0u, 0u, 0u);
+    return make_uint4(kLeafBit, w & kLNodeOffsetMask, (w >> kLNodeCountShift) & kLNodeCountMask, 0u);
 }
 // LightBVHSampler.slang:284-323, :375-419 (uniform pick in leaf), :430-446
 FHP_HD bool lbvhSample(const SceneDev& s, f3 posW, f3 n, float u, float& pdf, uint32_t& light) {

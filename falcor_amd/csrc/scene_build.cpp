@@ -670,6 +670,32 @@ void buildLightBVH(HostScene& s) {
         L.cones(0, c);
     }
     s.nLbvh = uint32_t(s.lbvh.size() / 16);
+    // Pack the nodes (LightBVHTypes.slang:97-198); the host mirror keeps the decoded attributes,
+    // which are what sampling sees.
+    s.lbvhPacked.assign(8 * size_t(s.nLbvh), 0u);
+    for (uint32_t n = 0; n < s.nLbvh; n++) {
+        float* q = &s.lbvh[16 * size_t(n)];
+        uint32_t info[3];
+        std::memcpy(info, &q[12], 12);
+        uint32_t w0 = info[0];
+        if (info[0] & kLeafBit) {
+            if (info[2] > kLNodeCountMask || info[1] > kLNodeOffsetMask)
+                throw std::runtime_error("light BVH leaf beyond the packed node's triangle count / offset range");
+            w0 = kLeafBit | (info[2] << kLNodeCountShift) | info[1];
+        }
+        LNodeAttr a;
+        a.origin = mk3(q[0], q[1], q[2]);
+        a.flux = q[3];
+        a.extent = mk3(q[4], q[5], q[6]);
+        a.cosCone = q[7];
+        a.coneDir = mk3(q[8], q[9], q[10]);
+        uint32_t* w = &s.lbvhPacked[8 * size_t(n)];
+        packLightNode(w, w0, a);
+        const LNodeAttr d = unpackLightNode(make_uint4(w[0], w[1], w[2], w[3]), make_uint4(w[4], w[5], w[6], w[7]));
+        q[4] = d.extent.x; q[5] = d.extent.y; q[6] = d.extent.z;
+        q[7] = d.cosCone;
+        q[8] = d.coneDir.x; q[9] = d.coneDir.y; q[10] = d.coneDir.z;
+    }
     // Refit schedule: depth of every node (nodes are depth-first, left child = parent + 1).
     s.lrefitNodes.clear();
     s.lrefitOff.assign(2, 0u);  // no internal level, an empty leaf group
@@ -691,6 +717,20 @@ void buildLightBVH(HostScene& s) {
     s.lrefitNodes.assign(s.nLbvh, 0u);
     std::vector<uint32_t> fill(s.lrefitOff.begin(), s.lrefitOff.end() - 1);
     for (uint32_t n = 0; n < s.nLbvh; n++) s.lrefitNodes[fill[leaf[n] ? s.lrefitDepth : depth[n]]++] = n;
+}
+
+void decodeLightNodes(HostScene& s) {
+    for (uint32_t n = 0; n < s.nLbvh; n++) {
+        const uint32_t* w = &s.lbvhPacked[8 * size_t(n)];
+        const LNodeAttr d = unpackLightNode(make_uint4(w[0], w[1], w[2], w[3]), make_uint4(w[4], w[5], w[6], w[7]));
+        float* q = &s.lbvh[16 * size_t(n)];
+        q[0] = d.origin.x; q[1] = d.origin.y; q[2] = d.origin.z; q[3] = d.flux;
+        q[4] = d.extent.x; q[5] = d.extent.y; q[6] = d.extent.z; q[7] = d.cosCone;
+        q[8] = d.coneDir.x; q[9] = d.coneDir.y; q[10] = d.coneDir.z; q[11] = 0.f;
+        uint32_t info[4] = {w[0], 0u, 0u, 0u};
+        if (w[0] & kLeafBit) info[0] = kLeafBit, info[1] = w[0] & kLNodeOffsetMask, info[2] = (w[0] >> kLNodeCountShift) & kLNodeCountMask;
+        std::memcpy(&q[12], info, 16);
+    }
 }
 
 namespace {

@@ -44,7 +44,11 @@ struct HostScene {
     // Lights: 20 floats per light, see SceneDev::lights.
     std::vector<float> lights;
     std::vector<uint32_t> activeLights, triToLight;
-    // Light BVH: 16 floats per node, leaf triangle lists and per-light traversal bitmasks.
+    // Light BVH: the packed nodes the device samples (8 words per node, packLightNode), their
+    // decoded attributes as 16 floats per node ((origin, flux) (extent, cosCone) (coneDir, 0)
+    // (right child | leaf bit, first triangle, triangle count, 0)), leaf triangle lists and
+    // per-light traversal bitmasks.
+    std::vector<uint32_t> lbvhPacked;
     std::vector<float> lbvh;
     std::vector<uint32_t> lbvhTri;
     std::vector<uint32_t> lbvhMask;  // 2 words per light
@@ -75,5 +79,7 @@ void buildBVH4(HostScene& s, uint32_t maxLeaf = 4);
 void buildLights(HostScene& s);
 void refreshLightEmission(HostScene& s);
 void buildLightBVH(HostScene& s);
+// Rewrites the decoded host mirror (lbvh) from the packed nodes (after a device refit).
+void decodeLightNodes(HostScene& s);
 
 }  // namespace fhp

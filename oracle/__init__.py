@@ -3,4 +3,4 @@
 Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg import this package,
 always as the checker. The fhp product never imports it.
 """
-from .pyoracle import Oracle, OracleScene, build_oracle, oracle_lib  # noqa: F401
+from .pyoracle import Oracle, OracleScene, build_oracle, oracle_lib, packed_precision  # noqa: F401

@@ -505,4 +505,16 @@ void fo_render_war(void* h, uint32_t W, uint32_t H, uint32_t spp, uint32_t frame
     fo_render_war_rect(h, W, H, 0, 0, W, H, spp, frame0, maxBounces, flags, threads, out, stats);
 }
 
+// Packed light-BVH node precision of n values (test hook): half-float values, quantised cone
+// cosines and octahedral 2 x 16-bit directions (3 floats each).
+void fo_packed_precision(uint32_t n, const float* v, float* half, float* cone, const float* dirs, float* oct) {
+    for (uint32_t i = 0; i < n; i++) {
+        half[i] = halfPrecision(v[i]);
+        cone[i] = conePrecision(v[i]);
+        const V3f d = octNormalPrecision(V3f(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]));
+        oct[3 * i] = d.x;
+        oct[3 * i + 1] = d.y;
+        oct[3 * i + 2] = d.z;
+    }
+}
 }  // extern "C"

@@ -402,6 +402,55 @@ struct Builder {
 };
 }  // namespace
 
+// ---------------------------------------------------------------- packed node precision
+// The device light BVH stores PackedNode records (LightBVHTypes.slang:97-198): half-float box
+// extents, the cone's cosine quantised to 16 bits and the cone axis as two 16-bit snorms in the
+// octahedral map, while the origin and flux stay fp32. Sampling sees the decoded values, so the
+// oracle keeps every node at the precision of its packed record.
+
+// v rounded to the nearest IEEE binary16 value (ties to even; beyond 65504 to infinity), computed
+// on the value in double precision rather than on bit fields.
+float halfPrecision(float v) {
+    if (!std::isfinite(v) || v == 0.f) return v;
+    const double a = std::fabs(double(v));
+    int e;
+    std::frexp(a, &e);                            // a in [2^(e-1), 2^e)
+    const double q = std::ldexp(1.0, std::max(e - 11, -24));  // spacing: 11 significant bits, or 2^-24
+    const double r = a / q, fl = std::floor(r), frac = r - fl;
+    const double n = frac > 0.5 ? fl + 1.0 : (frac < 0.5 ? fl : (std::fmod(fl, 2.0) == 0.0 ? fl : fl + 1.0));
+    double res = n * q;
+    if (res > 65504.0) res = std::numeric_limits<double>::infinity();
+    return float(v < 0.f ? -res : res);
+}
+// floatToSnorm16 then unpackSnorm16 (FormatConversion.slang:235-251).
+static float snorm16Precision(float v) {
+    v = std::isnan(v) ? 0.f : std::min(std::max(v, -1.f), 1.f);
+    const int bits = int(std::trunc(v * 32767.f + (v >= 0.f ? 0.5f : -0.5f)));
+    return std::max(float(bits) / 32767.f, -1.0f);
+}
+// encodeNormal2x16 then decodeNormal2x16 (PackedFormats.slang:53-66, MathHelpers.slang:156-194).
+V3f octNormalPrecision(V3f n) {
+    auto wrap = [](float x, float y, float& ox, float& oy) {
+        ox = (1.f - std::fabs(y)) * (x >= 0.f ? 1.f : -1.f);
+        oy = (1.f - std::fabs(x)) * (y >= 0.f ? 1.f : -1.f);
+    };
+    const float inv = 1.f / (std::fabs(n.x) + std::fabs(n.y) + std::fabs(n.z));
+    float px = n.x * inv, py = n.y * inv;
+    if (n.z < 0.f) wrap(px, py, px, py);
+    px = snorm16Precision(px);
+    py = snorm16Precision(py);
+    V3f d(px, py, 1.f - std::fabs(px) - std::fabs(py));
+    if (d.z < 0.f) wrap(d.x, d.y, d.x, d.y);
+    return normalize(d);
+}
+// (uint)((cos + 1) * 32767), decoded as packed / 32767 - 1.
+float conePrecision(float c) { return float(uint32_t((c + 1.f) * 32767.f)) * (1.f / 32767.f) - 1.f; }
+static void toPackedPrecision(Scene::LNode& n) {
+    n.extent = V3f(halfPrecision(n.extent.x), halfPrecision(n.extent.y), halfPrecision(n.extent.z));
+    n.cosCone = conePrecision(n.cosCone);
+    n.coneDir = octNormalPrecision(n.coneDir);
+}
+
 void buildLightBVH(Scene& s) {
     s.lnodes.clear();
     s.lTriIdx.clear();
@@ -424,6 +473,8 @@ void buildLightBVH(Scene& s) {
     B.build(0ull, 0, 0, uint32_t(B.tris.size()));
     float c;
     B.cones(0, c);
+    // The builder works in fp32; the records it writes hold the packed precision.
+    for (Scene::LNode& n : s.lnodes) toPackedPrecision(n);
 }
 
 // ---------------------------------------------------------------- animation update
@@ -488,6 +539,7 @@ void refitLightBVH(Scene& s) {
         }
         node.cosCone = cosCone;
         node.coneDir = dir;
+        toPackedPrecision(node);
     }
     for (int d = int(height) - 1; d >= 0; d--) {
         for (uint32_t i = 0; i < n; i++) {
@@ -518,6 +570,7 @@ void refitLightBVH(Scene& s) {
             }
             node.cosCone = cosCone;
             node.coneDir = dir;
+            toPackedPrecision(node);
         }
     }
 }

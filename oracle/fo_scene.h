@@ -427,6 +427,11 @@ void buildLightCollection(Scene& s);
 void updateLightPositions(Scene& s);
 void refitLightBVH(Scene& s);
 void buildLightBVH(Scene& s);
+// Precision of the packed light-BVH node fields (fo_impl.cpp): half-float extents, the quantised
+// cone cosine, the octahedral 2 x 16-bit cone axis.
+float halfPrecision(float v);
+float conePrecision(float c);
+V3f octNormalPrecision(V3f n);
 
 struct LightSample { V3f posW, normalW, Le, dir; float pdf, distance; uint32_t light; };
 // EmissiveLightSamplerHelpers.slang:54-101

@@ -75,6 +75,7 @@ def oracle_lib():
                                          vp]),
             "fo_war_geom_probe_b": (None, [vp, u32, u32, u32, u32, u32, u32, f32, f32, u32, vp, u32, vp, C.c_double, i32,
                                            vp, vp]),
+            "fo_packed_precision": (None, [u32, vp, vp, vp, vp, vp]),
             "fo_render_war_rect": (None, [vp, u32, u32, u32, u32, u32, u32, u32, u32, u32, u32, i32, vp, vp]),
         }
         for name, (res, args) in sig.items():
@@ -386,6 +387,17 @@ class Oracle:
         out = np.zeros((wi.shape[0], 8), np.float32)
         oracle_lib().fo_bsdf_sample(wi.shape[0], _p(m), _p(wi), _p(s), _p(out))
         return out
+
+
+def packed_precision(values, dirs):
+    """Light-BVH packed-node precision (fo_impl.cpp): (half-float values, quantised cone cosines of
+    the same values, octahedral 2 x 16-bit round trips of the unit directions dirs [n, 3])."""
+    v = np.ascontiguousarray(values, np.float32).ravel()
+    d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+    assert d.shape[0] == v.size
+    half, cone, oct = np.zeros_like(v), np.zeros_like(v), np.zeros_like(d)
+    oracle_lib().fo_packed_precision(v.size, _p(v), _p(half), _p(cone), _p(d), _p(oct))
+    return half, cone, oct
 
 
 def oct_equal_area(uv):

@@ -91,7 +91,7 @@ SceneDev bindScene(const HostScene& h, Arena& ar) {
     dv.lights = ar.dup<float4>(h.lights);
     dv.activeLights = ar.dup<uint32_t>(h.activeLights);
     dv.triToLight = ar.dup<uint32_t>(h.triToLight);
-    dv.lbvh = ar.dup<float4>(h.lbvh);
+    dv.lbvh = ar.dup<uint4>(h.lbvhPacked);
     dv.lbvhTri = ar.dup<uint32_t>(h.lbvhTri);
     dv.lbvhMask = ar.dup<uint2>(h.lbvhMask);
     dv.cam.pos = mk3(h.cam[0], h.cam[1], h.cam[2]);

@@ -270,26 +270,30 @@ def test_light_bvh_invariants(oracle_mod):
     assert counts.max() <= 10 and counts.sum() == L.shape[0]
     assert sorted(tri.tolist()) == list(range(L.shape[0]))
 
+    # Nodes carry the packed precision (half-float extents rounded to nearest, octahedral 16-bit
+    # cone axes): a box may miss its content by half a half-float step of its extent.
     def check(i, depth, mask):
         n = nodes[i]
         lo, hi = n[0:3] - n[3:6], n[0:3] + n[3:6]
+        tol = 1e-4 + 2.0 ** -11 * np.abs(n[3:6])
         if n[11] == 1:
             off, cnt = int(n[13]), int(n[14])
             ids = tri[off:off + cnt]
             pts = L[ids, :9].reshape(-1, 3)
-            assert np.all(pts >= lo - 1e-4) and np.all(pts <= hi + 1e-4)
+            assert np.all(pts >= lo - tol) and np.all(pts <= hi + tol)
             np.testing.assert_allclose(n[6], L[ids, 16].sum(), rtol=1e-4)
             for li in ids:
                 assert int(masks[li]) == mask  # bitmask encodes the path to the leaf
             if n[10] != -1:  # cone bounds all normals
-                assert np.all(L[ids, 9:12] @ n[7:10] >= n[10] - 1e-4)
+                assert np.all(L[ids, 9:12] @ n[7:10] >= n[10] - 2e-4)
             return n[6]
         left, right = i + 1, int(n[12])
         fl = check(left, depth + 1, mask)
         fr = check(right, depth + 1, mask | (1 << depth))
         for c in (left, right):
             cl, ch = nodes[c, 0:3] - nodes[c, 3:6], nodes[c, 0:3] + nodes[c, 3:6]
-            assert np.all(cl >= lo - 1e-4) and np.all(ch <= hi + 1e-4)
+            tc = tol + 2.0 ** -11 * np.abs(nodes[c, 3:6])  # both boxes rounded, possibly towards each other
+            assert np.all(cl >= lo - tc) and np.all(ch <= hi + tc)
         np.testing.assert_allclose(n[6], fl + fr, rtol=1e-4)
         return n[6]
 
