@@ -13,7 +13,7 @@ their exact ray counts and afterwards with HIP events on the launch stream for k
 Also reported on the same JSON line:
   * roofline of the dominant kernel (kTraceBounce, the BVH4 traversal of every bounce's
     closest-hit and visibility rays): algorithmic bytes per closest-hit ray (32 B ray + 16 B hit
-    + 128 B per visited node + 48 B per tested triangle) and per visibility ray (48 B record +
+    + 64 B per visited node + 48 B per tested triangle) and per visibility ray (48 B record +
     nodes + triangles), node/triangle counts measured on the same frame, over the kernel's average
     launch duration; beside it the rocprofv3-measured HBM traffic per launch and L2 hit rate
     (committed profile summary) and the measured HBM fraction of peak;
@@ -52,7 +52,7 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "Mrays/s fwd; ms per fwd+adjoint iter at 512² 16spp; grad rel-err vs FD"
 HBM_PEAK_GBS = 8000.0
-NODE_BYTES, TRI_BYTES, RAY_BYTES, HIT_BYTES, VIS_RECORD_BYTES = 128, 48, 32, 16, 48
+NODE_BYTES, TRI_BYTES, RAY_BYTES, HIT_BYTES, VIS_RECORD_BYTES = 64, 48, 32, 16, 48
 TUNE_TAG = "sphere_field_1920x1080_4spp_b6_lightbvh_nee"
 BATCH_TUNE_TAG = "batch_64views_16scenes_512x512_b6_lightbvh_nee"
 

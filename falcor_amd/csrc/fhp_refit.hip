@@ -47,18 +47,22 @@ __global__ void kUpdateTriPos(float4* __restrict__ triPos, uint32_t nT, const ui
 __device__ __forceinline__ float padLo(float v, float absPad) { return v - (fabsf(v) * 1e-6f + absPad); }
 __device__ __forceinline__ float padHi(float v, float absPad) { return v + (fabsf(v) * 1e-6f + absPad); }
 
-// One depth level: node layout per node (32 floats) = lo/hi per axis for children 0..3
-// ([8a + i], [8a + 4 + i]) then the child references ([24 + i]).
+// One depth level: each node's four child boxes (a leaf child's from its triangles, an internal
+// child's as the union of the boxes its own record decodes to), padded as the build pads them,
+// then repacked into the node's quantised record (packBVH4Node). Children are one level deeper,
+// so their records are already refitted.
 __global__ void kRefitLevel(float* __restrict__ bvh, const uint32_t* __restrict__ nodes, uint32_t count,
                             const float4* __restrict__ triPos, float absPad) {
     const uint32_t k = blockIdx.x * kRefitBlock + threadIdx.x;
     if (k >= count) return;
-    float* q = bvh + 32 * size_t(nodes[k]);
+    uint32_t* q = reinterpret_cast<uint32_t*>(bvh) + kBVHNodeWords * size_t(nodes[k]);
+    uint32_t ref[4];
+    float clo[4][3], chi[4][3];
     for (int i = 0; i < 4; i++) {
-        const uint32_t c = __float_as_uint(q[24 + i]);
-        if (c == kInvalid) continue;
+        const uint32_t c = q[12 + i];
+        ref[i] = c;
         float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-        if (childIsLeaf(c)) {
+        if (c != kInvalid && childIsLeaf(c)) {
             const uint32_t first = leafFirst(c), cnt = leafCount(c);
             for (uint32_t s = first; s < first + cnt; s++)
                 for (int j = 0; j < 3; j++) {
@@ -67,21 +71,25 @@ __global__ void kRefitLevel(float* __restrict__ bvh, const uint32_t* __restrict_
                     lo[1] = fminf(lo[1], p.y); hi[1] = fmaxf(hi[1], p.y);
                     lo[2] = fminf(lo[2], p.z); hi[2] = fmaxf(hi[2], p.z);
                 }
-        } else {
-            const float* cq = bvh + 32 * size_t(c);
+        } else if (c != kInvalid) {
+            const uint32_t* cq = reinterpret_cast<const uint32_t*>(bvh) + kBVHNodeWords * size_t(c);
             for (int ci = 0; ci < 4; ci++) {
-                if (__float_as_uint(cq[24 + ci]) == kInvalid) continue;
+                if (cq[12 + ci] == kInvalid) continue;
                 for (int a = 0; a < 3; a++) {
-                    lo[a] = fminf(lo[a], cq[8 * a + ci]);
-                    hi[a] = fmaxf(hi[a], cq[8 * a + 4 + ci]);
+                    const float o = __uint_as_float(cq[a]), st = __uint_as_float(cq[3 + a]);
+                    lo[a] = fminf(lo[a], bvhPlane(o, st, (cq[6 + 2 * a] >> (8 * ci)) & 0xffu));
+                    hi[a] = fmaxf(hi[a], bvhPlane(o, st, (cq[7 + 2 * a] >> (8 * ci)) & 0xffu));
                 }
             }
         }
         for (int a = 0; a < 3; a++) {
-            q[8 * a + i] = padLo(lo[a], absPad);
-            q[8 * a + 4 + i] = padHi(hi[a], absPad);
+            clo[i][a] = c != kInvalid ? padLo(lo[a], absPad) : 0.f;
+            chi[i][a] = c != kInvalid ? padHi(hi[a], absPad) : 0.f;
         }
     }
+    uint32_t w[kBVHNodeWords];
+    packBVH4Node(w, clo, chi, ref);
+    for (uint32_t j = 0; j < kBVHNodeWords; j++) q[j] = w[j];
 }
 
 // Node transform: rest-pose records (p.xyz, n.x) (n.yz, uv) -> (M p, N n); uvs kept. The normal

@@ -28,6 +28,51 @@ FHP_HD bool childIsLeaf(uint32_t c) { return (c & kLeafBit) != 0u; }
 FHP_HD uint32_t leafFirst(uint32_t c) { return c & 0x0fffffffu; }
 FHP_HD uint32_t leafCount(uint32_t c) { return ((c >> 28) & 7u) + 1u; }
 FHP_HD uint32_t makeLeaf(uint32_t first, uint32_t count) { return kLeafBit | ((count - 1u) << 28) | first; }
+// BVH4 node, 64 bytes (16 words): words 0-2 the origin o of the node's box, words 3-5 the
+// quantisation step s per axis, words 6-11 the children's boxes as 8-bit multiples of s from o
+// (word 6 + 2a the four children's lower planes on axis a, byte i for child i; word 7 + 2a the
+// upper planes), words 12-15 the child references. A child's decoded plane is fmaf(float(q), s, o)
+// in fp32 (the same on host and device); packBVH4Node chooses every code with that arithmetic, so a
+// decoded box always contains the child's box (lower planes rounded down, upper planes up).
+constexpr uint32_t kBVHNodeWords = 16;
+FHP_HD float bvhPlane(float o, float s, uint32_t q) { return fmaf(float(q), s, o); }
+FHP_HD void packBVH4Node(uint32_t w[16], const float lo[4][3], const float hi[4][3], const uint32_t ref[4]) {
+    float nlo[3] = {INFINITY, INFINITY, INFINITY}, nhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int i = 0; i < 4; i++) {
+        if (ref[i] == 0xffffffffu) continue;
+        for (int a = 0; a < 3; a++) {
+            nlo[a] = fminf(nlo[a], lo[i][a]);
+            nhi[a] = fmaxf(nhi[a], hi[i][a]);
+        }
+    }
+    for (int k = 0; k < 16; k++) w[k] = 0u;
+    for (int a = 0; a < 3; a++) {
+        if (!(nlo[a] <= nhi[a])) nlo[a] = nhi[a] = 0.f;  // no child
+        const float o = nlo[a];
+        float st = (nhi[a] - o) / 255.f;
+        // The step must reach the top of the box with the decode's own arithmetic.
+        while (bvhPlane(o, st, 255u) < nhi[a]) st = st > 0.f ? u2f(f2u(st) + 1u) : 1e-30f;
+        w[a] = f2u(o);
+        w[3 + a] = f2u(st);
+        for (int i = 0; i < 4; i++) {
+            if (ref[i] == 0xffffffffu) continue;
+            uint32_t ql = 0u, qh = 255u;
+            if (st > 0.f) {
+                const float fl = floorf((lo[i][a] - o) / st), fh = ceilf((hi[i][a] - o) / st);
+                ql = uint32_t(fminf(fmaxf(fl, 0.f), 255.f));
+                qh = uint32_t(fminf(fmaxf(fh, 0.f), 255.f));
+                while (ql > 0u && bvhPlane(o, st, ql) > lo[i][a]) ql--;
+                while (ql < 255u && bvhPlane(o, st, ql + 1u) <= lo[i][a]) ql++;
+                while (qh < 255u && bvhPlane(o, st, qh) < hi[i][a]) qh++;
+                while (qh > 0u && bvhPlane(o, st, qh - 1u) >= hi[i][a]) qh--;
+            }
+            w[6 + 2 * a] |= ql << (8 * i);
+            w[7 + 2 * a] |= qh << (8 * i);
+        }
+    }
+    for (int i = 0; i < 4; i++) w[12 + i] = ref[i];
+}
+
 // What a pop from beyond a traversal stack's capacity returns (the overflow is already reported):
 // the leaf of triangle 0 alone, which every scene has. Testing it reads in-bounds memory and can
 // only report a true intersection.
@@ -49,7 +94,7 @@ struct HitInfoLayout {
 };
 
 struct SceneDev {
-    const float4* bvh;
+    const float4* bvh;           // BVH4 nodes, 4 x float4 each (packBVH4Node)
     const float4* triPos;
     const uint32_t* idx;
     const float4* vtx;

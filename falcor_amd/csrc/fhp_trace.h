@@ -86,23 +86,26 @@ struct TraceCounters {
     uint64_t nodes, tris;
 };
 
-// Slab test of the 4 children of one node. Returns a 4-bit hit mask and entry distances.
-// Conservative: the exit distance is scaled by (1 + 4 ulp) and boxes are padded at build time.
+// Slab test of the 4 children of one node (packBVH4Node's layout: the children's planes decoded
+// from 8-bit codes). Returns a 4-bit hit mask and entry distances. Conservative: the decoded boxes
+// contain the padded child boxes and the exit distance is scaled by (1 + 4 ulp).
 FHP_DEVI uint32_t testNode4(const float4* __restrict__ node, f3 o, f3 inv, float tmin, float tmax,
                                               float tn[4], uint4& child) {
-    const float4 lx = node[0], hx = node[1], ly = node[2], hy = node[3], lz = node[4], hz = node[5];
-    const float4 cb = node[6];
+    const float4 n0 = node[0], n1 = node[1], n2 = node[2], cb = node[3];
     child = make_uint4(f2u(cb.x), f2u(cb.y), f2u(cb.z), f2u(cb.w));
-    const float lox[4] = {lx.x, lx.y, lx.z, lx.w}, hix[4] = {hx.x, hx.y, hx.z, hx.w};
-    const float loy[4] = {ly.x, ly.y, ly.z, ly.w}, hiy[4] = {hy.x, hy.y, hy.z, hy.w};
-    const float loz[4] = {lz.x, lz.y, lz.z, lz.w}, hiz[4] = {hz.x, hz.y, hz.z, hz.w};
+    // Planes go straight to ray distance: (o + q s - org) inv = q (s inv) + (o - org) inv. The
+    // rounding this adds is far below the build's box padding, so the test stays conservative.
+    const float bx0 = (n0.x - o.x) * inv.x, by0 = (n0.y - o.y) * inv.y, bz0 = (n0.z - o.z) * inv.z;
+    const float kx = n0.w * inv.x, ky = n1.x * inv.y, kz = n1.y * inv.z;
+    const uint32_t qlx = f2u(n1.z), qhx = f2u(n1.w), qly = f2u(n2.x), qhy = f2u(n2.y), qlz = f2u(n2.z), qhz = f2u(n2.w);
     const uint32_t ch[4] = {child.x, child.y, child.z, child.w};
     uint32_t mask = 0;
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-        float ax = (lox[i] - o.x) * inv.x, bx = (hix[i] - o.x) * inv.x;
-        float ay = (loy[i] - o.y) * inv.y, by = (hiy[i] - o.y) * inv.y;
-        float az = (loz[i] - o.z) * inv.z, bz = (hiz[i] - o.z) * inv.z;
+        const uint32_t sh = 8u * uint32_t(i);
+        float ax = fmaf(float((qlx >> sh) & 0xffu), kx, bx0), bx = fmaf(float((qhx >> sh) & 0xffu), kx, bx0);
+        float ay = fmaf(float((qly >> sh) & 0xffu), ky, by0), by = fmaf(float((qhy >> sh) & 0xffu), ky, by0);
+        float az = fmaf(float((qlz >> sh) & 0xffu), kz, bz0), bz = fmaf(float((qhz >> sh) & 0xffu), kz, bz0);
         float n = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), tmin));
         float f = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), tmax));
         f = f * 1.00000048f;
@@ -158,7 +161,7 @@ struct TravState {
     template <bool COUNT, typename SceneT>
     FHP_DEVI bool step(const SceneT& s, TraceStack& st, TraceCounters* cnt) {
         if (!childIsLeaf(cur)) {
-            const float4* node = s.bvh + 8 * size_t(cur);
+            const float4* node = s.bvh + 4 * size_t(cur);
             float tn[4];
             uint4 child;
             if (COUNT) cnt->nodes++;

@@ -253,8 +253,8 @@ void buildBVH4(HostScene& s, uint32_t maxLeaf) {
             ch.push_back(B.nodes[opened].l);
             ch.push_back(B.nodes[opened].r);
         }
-        uint32_t me = uint32_t(out.size() / 32);
-        out.resize(out.size() + 32, 0.f);
+        uint32_t me = uint32_t(out.size() / kBVHNodeWords);
+        out.resize(out.size() + kBVHNodeWords, 0.f);
         s.bvhSah += nd.box.area() / rootArea;
         uint32_t refs[4] = {kInvalid, kInvalid, kInvalid, kInvalid};
         Box boxes[4];
@@ -266,31 +266,36 @@ void buildBVH4(HostScene& s, uint32_t maxLeaf) {
             maxDepth = std::max(maxDepth, r.depth);
             maxNeed = std::max(maxNeed, r.need);
         }
-        float* q = &out[32 * size_t(me)];
+        float clo[4][3] = {}, chi[4][3] = {};
         for (int i = 0; i < 4; i++) {
-            bool ok = refs[i] != kInvalid;
+            if (refs[i] == kInvalid) continue;
             for (int a = 0; a < 3; a++) {
-                q[8 * a + i] = ok ? padLo(boxes[i].lo[a]) : 0.f;
-                q[8 * a + 4 + i] = ok ? padHi(boxes[i].hi[a]) : 0.f;
+                clo[i][a] = padLo(boxes[i].lo[a]);
+                chi[i][a] = padHi(boxes[i].hi[a]);
             }
-            std::memcpy(&q[24 + i], &refs[i], 4);
         }
+        uint32_t w[kBVHNodeWords];
+        packBVH4Node(w, clo, chi, refs);
+        std::memcpy(&out[kBVHNodeWords * size_t(me)], w, sizeof(w));
         return Ret{me, maxDepth, uint32_t(ch.size() - 1) + maxNeed};
     };
     Ret root;
     if (B.nodes[0].l < 0) {
         // A single leaf: wrap it in an internal root with one child.
-        out.resize(32, 0.f);
+        out.resize(kBVHNodeWords, 0.f);
         const N2& nd = B.nodes[0];
         uint32_t refs[4] = {makeLeaf(nd.first, nd.count), kInvalid, kInvalid, kInvalid};
-        for (int a = 0; a < 3; a++) { out[8 * a] = padLo(nd.box.lo[a]); out[8 * a + 4] = padHi(nd.box.hi[a]); }
-        std::memcpy(&out[24], refs, 16);
+        float clo[4][3] = {}, chi[4][3] = {};
+        for (int a = 0; a < 3; a++) { clo[0][a] = padLo(nd.box.lo[a]); chi[0][a] = padHi(nd.box.hi[a]); }
+        uint32_t w[kBVHNodeWords];
+        packBVH4Node(w, clo, chi, refs);
+        std::memcpy(out.data(), w, sizeof(w));
         s.bvhLeaves = 1;
         root = Ret{0, 1, 0};
     } else {
         root = collapse(0, 1);
     }
-    s.bvhNodes = uint32_t(out.size() / 32);
+    s.bvhNodes = uint32_t(out.size() / kBVHNodeWords);
     s.bvhDepth = root.depth;
     s.stackNeed = root.need;
 
@@ -302,7 +307,7 @@ void buildBVH4(HostScene& s, uint32_t maxLeaf) {
         maxD = std::max(maxD, depth[n]);
         for (int i = 0; i < 4; i++) {
             uint32_t c;
-            std::memcpy(&c, &out[32 * size_t(n) + 24 + i], 4);
+            std::memcpy(&c, &out[kBVHNodeWords * size_t(n) + 12 + i], 4);
             if (c != kInvalid && !childIsLeaf(c)) depth[c] = depth[n] + 1;
         }
     }

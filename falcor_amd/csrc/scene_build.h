@@ -32,7 +32,8 @@ struct HostScene {
     bool meshTriContiguous = false;
     uint32_t hitCompressed = 0, hitTypeBits = 0, hitInstBits = 0, hitPrimBits = 0;
 
-    // BVH4 (8 float4 per node) and triangles in leaf order (12 floats per triangle).
+    // BVH4 (quantised, 4 float4 per node: packBVH4Node) and triangles in leaf order (12 floats
+    // per triangle).
     std::vector<float> bvh;
     std::vector<float> triPos;
     uint32_t bvhNodes = 0, bvhDepth = 0, stackNeed = 0, bvhLeaves = 0;
