@@ -21,6 +21,10 @@ constexpr int kHitCount = 3 * kCounterStride;
 // Paths with a primary hit (the primary shading's compacted list); the last hit-count slot,
 // beyond any bounce depth, so the frame's counter reset clears it.
 constexpr int kPrimaryHitCount = kHitCount + kCounterStride - 1;
+// Closest-hit and visibility rays traced by the bounce loop's tail kernel (slots beyond any
+// bounce depth: maxBounces <= kMaxBounceSlots - 2).
+constexpr int kTailClosestCount = kHitCount + kCounterStride - 3;
+constexpr int kTailVisibilityCount = kHitCount + kCounterStride - 2;
 constexpr int kCounterWords = 4 * kCounterStride;  // counters: [0, 32) ray queues by depth, [32, 64) shadow queues
 
 // One view of a batched frame: its scene (index into the batch's scene table), camera, frame
@@ -70,6 +74,8 @@ struct MptArgs {
     const uint32_t* vbuf;
     const float4* viewW;
     uint32_t keepMisses;  // 1: closest-hit misses are queued for shading too (environment light)
+    uint32_t tailPaths;   // a depth with at most this many hits runs its paths to the end in one
+                          // launch (kTail); 0: every depth has its own launches
 };
 
 // Bits of a scene's trace error word (TraceScratch::err), read back by fhp_scene_trace_errors.

@@ -532,6 +532,10 @@ __global__ __launch_bounds__(kShadeBlock) void kShadePrimary(SceneDev s0, MptArg
     }
 }
 
+// True when the hits of a depth (n of them) are few enough for the tail kernel, which then runs
+// their paths to the end; the per-depth shading launch leaves them alone.
+__device__ __forceinline__ bool tailActive(const MptArgs& a, uint32_t n) { return a.tailPaths != 0u && n <= a.tailPaths; }
+
 // handleHit for scatter depth `depth` (MinimalPathTracer.rt.slang:324-376), one thread per queued ray.
 // Runs on the compacted hits of the bounce (entry k: hit hitQ[k] of ray hitIdx[k]) unless
 // visibility rays have launches of their own, then on every queued ray (hitQ[i] of ray i).
@@ -541,6 +545,7 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(SceneDev s0, MptArgs a, ui
     const int qi = depth & 1, qo = qi ^ 1;
     const bool compact = !a.separateShadow;
     const uint32_t n = compact ? a.counters[kHitCount + depth] : a.counters[depth];
+    if (compact && tailActive(a, n)) return;
     const uint32_t G = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += G) {
         const uint32_t k = base + threadIdx.x;
@@ -606,6 +611,104 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(SceneDev s0, MptArgs a, ui
         if (cont) pushRay(a, qo, slot, newO, newD, pathId);
         const uint32_t sslot = blockAppend<kShadeBlock>(shadow, &a.counters[kCounterStride + depth + 1], lds);
         if (shadow) pushShadow(a, sslot, shO, shD, shT, shC, pathId, 0u);
+    }
+}
+
+// Tail of the bounce loop. Late bounces hold few paths, and a launch per bounce then lasts as long
+// as its slowest ray's chain of dependent node loads. Once a depth has at most tailPaths hits,
+// this kernel takes them over (kShade of that depth returns at once) and runs each path to its end
+// in one thread: handleHit, the visibility ray, the next closest-hit ray, and so on. The steps are
+// kShade's and kTraceBounce's with the path state in registers, in the same order (the
+// visibility ray of depth d adds its contribution after the emission of depth d and before that
+// of depth d + 1; the library is built without implicit FMA contraction), so the radiance is
+// bit-identical to the per-depth launches. Later depths find
+// their queues empty. Rays traced here are counted in counters[kTailClosestCount] and
+// counters[kTailVisibilityCount].
+template <int B, int D, bool MULTI, int LM>
+__global__ __launch_bounds__(B) void kTail(SceneDev s0, MptArgs a, uint32_t depth, TraceScratch ts) {
+    __shared__ uint32_t lds[D * B];
+    const uint32_t n = a.counters[kHitCount + depth];
+    if (!tailActive(a, n)) return;
+    TraceStack st = makeStack<B, D>(lds, ts, blockIdx.x, threadIdx.x);
+    const int qi = depth & 1;
+    const uint32_t G = gridDim.x * B;
+    uint32_t nClosest = 0, nVis = 0;
+    for (uint32_t k = blockIdx.x * B + threadIdx.x; k < n; k += G) {
+        const uint32_t i = a.hitIdx[k];
+        const uint32_t pathId = a.pathQ[qi][i];
+        const SceneDev& s = *viewCtx<MULTI>(s0, a, pathId, true).s;
+        const GeomRef g{s.bvh, s.triPos};
+        uint4 h = a.hitQ[k];
+        const float4 ro = a.rayQ[qi][2 * size_t(i)], rd = a.rayQ[qi][2 * size_t(i) + 1];
+        f3 o = mk3(ro.x, ro.y, ro.z), d = mk3(rd.x, rd.y, rd.z);
+        float4 t4 = a.thp[pathId], n4 = a.prevN[pathId];
+        const float4 ls4 = a.Ls[pathId];
+        f3 Ls = mk3(ls4.x, ls4.y, ls4.z);
+        const uint4 r4 = a.rng[pathId];
+        Rng rng{r4.x, r4.y, r4.z, r4.w};
+        for (uint32_t dep = depth;; dep++) {
+            f3 thp = mk3(t4.x, t4.y, t4.z);
+            if (h.x == kInvalid) {
+                // scatterMiss (MinimalPathTracer.rt.slang:475-486) with MIS against env sampling.
+                if (s.envTex && (s.envFlags & kEnvLight) && (a.computeDirect || dep > 0)) {
+                    f3 Le = envEval(s, d);
+                    const float pEnv = LM == kLightsAny ? lightTypes(s, a).pEnv : 0.f;
+                    if (pEnv > 0.f && !(f2u(n4.w) & 1u)) Le = Le * misPower(t4.w, envPdf(s, d) * pEnv);
+                    Ls = mk3(Ls.x + t4.x * Le.x, Ls.y + t4.y * Le.y, Ls.z + t4.z * Le.z);
+                }
+                break;
+            }
+            Shading sd = shadingFromHit(s, h.x, u2f(h.z), u2f(h.w), d);
+            Material m = loadMaterial(s, s.mats, s.texels, sd, sd, nullptr);
+            if (a.computeDirect || dep > 0) {
+                f3 e = m.emission;
+                if (a.nee && anyPos(e) && !(f2u(n4.w) & 1u)) {
+                    float lp = lightPdf(s, int(a.lightKind), o, mk3(n4.x, n4.y, n4.z), s.triToLight[h.x], sd.posW,
+                                        sd.orientedFaceN());
+                    e = e * misPower(t4.w, LM == kLightsAny ? lp * lightTypes(s, a).pEmissive : lp);
+                }
+                Ls = Ls + thp * e;
+            }
+            if (dep >= a.maxBounces) break;
+            const f3 org = rayOriginFromShading(sd);
+            f3 shO, shD, shC;
+            float shT = 0.f;
+            const bool shadow = directSample<LM>(s, a, sd, m, rng, shO, shD, shT, shC);
+            if (shadow) shC = thp * shC;
+            bool cont = false;
+            f3 newO, newD;
+            Scatter sc;
+            if (matSample(sd, m, rng, sc, a.importance != 0)) {
+                thp = thp * sc.weight;
+                cont = anyPos(thp);
+                newO = sc.transmission ? rayOriginFromShading(sd, false) : org;
+                newD = sc.wo;
+                t4 = make_float4(thp.x, thp.y, thp.z, sc.pdf);
+                n4 = make_float4(sd.N.x, sd.N.y, sd.N.z, u2f(sc.delta ? 1u : 0u));
+            }
+            if (shadow) {
+                nVis++;
+                HitRec hr;
+                if (!traceBVH4<true, false>(g, shO, shD, 0.f, shT, st, hr)) Ls = Ls + shC;
+            }
+            if (!cont) break;
+            nClosest++;
+            HitRec hr;
+            traceBVH4<false, false>(g, newO, newD, 0.f, kFltMax, st, hr);
+            if (hr.tri == kInvalid && !a.keepMisses) break;
+            h = make_uint4(hr.tri, f2u(hr.t), f2u(hr.b1), f2u(hr.b2));
+            o = newO;
+            d = newD;
+        }
+        a.Ls[pathId] = make_float4(Ls.x, Ls.y, Ls.z, 0.f);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        nClosest += __shfl_xor(nClosest, off);
+        nVis += __shfl_xor(nVis, off);
+    }
+    if ((threadIdx.x & 63u) == 0u) {
+        if (nClosest) atomicAdd(&a.counters[kTailClosestCount], nClosest);
+        if (nVis) atomicAdd(&a.counters[kTailVisibilityCount], nVis);
     }
 }
 
@@ -769,6 +872,8 @@ __global__ void kRayCounts(const MptArgs a, uint32_t nb, unsigned long long* out
         unsigned long long c = a.vbuf ? 0ull : (unsigned long long)a.nPix, v = 0;
         for (uint32_t b = 0; b < nb; b++) c += a.counters[b];
         for (uint32_t b = 0; b <= nb; b++) v += a.counters[kCounterStride + b];
+        c += a.counters[kTailClosestCount];
+        v += a.counters[kTailVisibilityCount];
         out[0] += c;
         out[1] += v;
     }
@@ -800,8 +905,11 @@ hipError_t launchMptFrame(const SceneDev& s, const MptArgs& a, const TraceScratc
 }
 
 template <bool MULTI>
-hipError_t mptBounces(const SceneDev& s, const MptArgs& a, const TraceScratch& ts, bool count, unsigned long long* d_counts,
+hipError_t mptBounces(const SceneDev& s, const MptArgs& a0, const TraceScratch& ts, bool count, unsigned long long* d_counts,
                       hipStream_t st, KernelTimer* tm) {
+    // Counting frames trace every ray in the per-depth launches (their node and triangle counters).
+    MptArgs a = a0;
+    if (count || a.separateShadow) a.tailPaths = 0u;
     KernelTimer dummy;
     if (!tm) tm = &dummy;
     const uint32_t npix = a.nPix;
@@ -854,6 +962,22 @@ hipError_t mptBounces(const SceneDev& s, const MptArgs& a, const TraceScratch& t
         tm->end(st);
         return e;
     };
+    auto tail = [&](uint32_t depth) {
+        tm->begin(KK_OTHER, st);
+        hipError_t e = withTraceConfig(tsc, [&](auto Bc, auto Dc) {
+            constexpr int B = decltype(Bc)::value, D = decltype(Dc)::value;
+            const uint32_t blocks = std::max<uint32_t>(1u, std::min<uint32_t>(tsc.gridBlocks, (a.tailPaths + B - 1) / B));
+            if (a.lightMode == kLightsNone)
+                hipLaunchKernelGGL((kTail<B, D, MULTI, kLightsNone>), dim3(blocks), dim3(B), 0, st, s, a, depth, tsc);
+            else if (a.lightMode == kLightsEmissive)
+                hipLaunchKernelGGL((kTail<B, D, MULTI, kLightsEmissive>), dim3(blocks), dim3(B), 0, st, s, a, depth, tsc);
+            else
+                hipLaunchKernelGGL((kTail<B, D, MULTI, kLightsAny>), dim3(blocks), dim3(B), 0, st, s, a, depth, tsc);
+            return hipGetLastError();
+        });
+        tm->end(st);
+        return e;
+    };
     // Primary rays: one per pixel (the reference's V-buffer), shared by the pixel's samples;
     // with a bound V-buffer its hits are decoded instead.
     hipError_t e = hipSuccess;
@@ -887,6 +1011,7 @@ hipError_t mptBounces(const SceneDev& s, const MptArgs& a, const TraceScratch& t
             hipLaunchKernelGGL((kShade<MULTI, kLightsEmissive>), dim3(shadeBlocks), dim3(T), 0, st, s, a, b);
         else hipLaunchKernelGGL((kShade<MULTI, kLightsAny>), dim3(shadeBlocks), dim3(T), 0, st, s, a, b);
         tm->end(st);
+        if (a.tailPaths && (e = tail(b)) != hipSuccess) return e;
     }
     return hipGetLastError();
 }

@@ -42,6 +42,30 @@ FHP_HD float comp(f3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
 FHP_HD float fminf_(float a, float b) { return a < b ? a : b; }
 FHP_HD float fmaxf_(float a, float b) { return a > b ? a : b; }
 FHP_HD float saturate(float v) { return fminf_(fmaxf_(v, 0.f), 1.f); }
+// Hardware square root, reciprocal square root and reciprocal (about 1 ulp) on the device, for
+// heuristics whose exact rounding does not matter (both the sampling and the pdf evaluation use
+// the same function); the host build uses the correctly rounded operations.
+FHP_HD float hwSqrt(float x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return __builtin_amdgcn_sqrtf(x);
+#else
+    return sqrtf(x);
+#endif
+}
+FHP_HD float hwRsqrt(float x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return __builtin_amdgcn_rsqf(x);
+#else
+    return 1.f / sqrtf(x);
+#endif
+}
+FHP_HD float hwRcp(float x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return __builtin_amdgcn_rcpf(x);
+#else
+    return 1.f / x;
+#endif
+}
 FHP_HD float luminance(f3 c) { return c.x * 0.2126f + c.y * 0.7152f + c.z * 0.0722f; }  // ColorHelpers.slang:45-49
 
 constexpr float kPi = 3.14159265358979323846f;

@@ -929,39 +929,49 @@ FHP_HD float sinSubClamped(float sA, float cA, float sB, float cB) { return cA >
 // LightBVHSampler.slang:219-273 (sphere solid-angle bound, bounding and lighting cones)
 FHP_HD float lbvhImportance(const SceneDev& s, f3 posW, f3 n, uint32_t node) {
     // The packed node's fields, decoded as unpackLightNode does; the cone axis only where the
-    // orientation bound needs it.
+    // orientation bound needs it. The reference's terms with shared subexpressions and hardware
+    // square roots and reciprocals: normalize(origin - posW) and (origin - posW) / distance are one
+    // vector, the bounding sphere's sine sqrt(1 - cos^2) is its sqrt(r^2 / d^2).
     const uint4 q0 = s.lbvh[2 * size_t(node)], q1 = s.lbvh[2 * size_t(node) + 1];
     const f3 origin = mk3(u2f(q0.y), u2f(q0.z), u2f(q0.w));
     const f3 extent = mk3(f16ToF32(q1.x & 0xffffu), f16ToF32(q1.x >> 16), f16ToF32(q1.y & 0xffffu));
     const float flux = u2f(q1.w), cosCone = float(q1.y >> 16) * (1.f / 32767.f) - 1.f;
-    float distance = length(origin - posW);
+    const f3 cv = origin - posW;
+    const float cd2 = dot(cv, cv);
+    float distance = hwSqrt(cd2);
+    const f3 L = cv * hwRcp(distance);
     // boundCosineTerm (LightBVHSampler.slang:181-210) with GeometryHelpers.slang:222-240.
-    float sinBC = 0.f, cosBC;
-    float sqrR = dot(extent, extent);
-    f3 cv = origin - posW;
-    float cd2 = dot(cv, cv);
-    if (cd2 < sqrR) { sinBC = 0.f; cosBC = -1.f; }
-    else { float s2 = sqrR / cd2; cosBC = sqrtf(1.f - s2); sinBC = sqrtf(s2); }
-    f3 L = normalize(origin - posW);
-    float cosL = fminf_(fmaxf_(dot(n, L), -1.f), 1.f);
-    float sinL = sqrtf(1.f - cosL * cosL);
-    float NdotL = saturate(cosSubClamped(sinL, cosL, sinBC, cosBC));
-    float orient = 1.f;
-    f3 dirTo = (origin - posW) / distance;
-    if (cosCone != -1.f && cosCone > 0.f) {
-        const f3 coneDir = decodeOct16(q1.z);
-        float sinCone = sqrtf(fmaxf_(0.f, 1.f - cosCone * cosCone));
-        float cosT = dot(coneDir, -dirTo);
-        float sinT = sqrtf(fmaxf_(0.f, 1.f - cosT * cosT));
-        float sinBC2 = sqrtf(fmaxf_(0.f, 1 - cosBC * cosBC));
-        float c0 = cosSubClamped(sinT, cosT, sinCone, cosCone);
-        float s0 = sinSubClamped(sinT, cosT, sinCone, cosCone);
-        float cp = cosSubClamped(s0, c0, sinBC2, cosBC);
-        orient = fmaxf_(0.f, cp);
+    const float sqrR = dot(extent, extent);
+    float sinBC = 0.f, cosBC = -1.f;
+    if (!(cd2 < sqrR)) {
+        const float s2 = sqrR * hwRcp(cd2);
+        cosBC = hwSqrt(1.f - s2);
+        sinBC = hwSqrt(s2);
     }
-    float halfR = fmaxf_(extent.x, fmaxf_(extent.y, extent.z));
+    const float cosL = fminf_(fmaxf_(dot(n, L), -1.f), 1.f);
+    const float sinL = hwSqrt(1.f - cosL * cosL);
+    const float NdotL = saturate(cosSubClamped(sinL, cosL, sinBC, cosBC));
+    if (NdotL == 0.f) return 0.f;
+    float orient = 1.f;
+    if (cosCone > 0.f) {
+        float px = snorm16ToFloat(q1.z & 0xffffu), py = snorm16ToFloat(q1.z >> 16);
+        const float z = 1.f - fabsf(px) - fabsf(py);
+        if (z < 0.f) {
+            const float wx = (1.f - fabsf(py)) * (px >= 0.f ? 1.f : -1.f), wy = (1.f - fabsf(px)) * (py >= 0.f ? 1.f : -1.f);
+            px = wx;
+            py = wy;
+        }
+        const f3 coneDir = mk3(px, py, z) * hwRsqrt(px * px + py * py + z * z);
+        const float sinCone = hwSqrt(fmaxf_(0.f, 1.f - cosCone * cosCone));
+        const float cosT = -dot(coneDir, L);
+        const float sinT = hwSqrt(fmaxf_(0.f, 1.f - cosT * cosT));
+        const float c0 = cosSubClamped(sinT, cosT, sinCone, cosCone);
+        const float s0 = sinSubClamped(sinT, cosT, sinCone, cosCone);
+        orient = fmaxf_(0.f, cosSubClamped(s0, c0, sinBC, cosBC));
+    }
+    const float halfR = fmaxf_(extent.x, fmaxf_(extent.y, extent.z));
     distance = fmaxf_(halfR, distance);
-    return (flux * NdotL) * orient / (distance * distance);
+    return (flux * NdotL) * orient * hwRcp(distance * distance);
 }
 FHP_HD uint4 lbvhInfo(const SceneDev& s, uint32_t node) {
     const uint32_t w = s.lbvh[2 * size_t(node)].x;

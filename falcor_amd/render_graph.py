@@ -491,7 +491,9 @@ class MinimalPathTracer(_NativePass):
     """MinimalPathTracer (MinimalPathTracer.cpp). Properties: maxBounces, computeDirect,
     useImportanceSampling; extensions useEmissiveNEE, useEnvNEE (env map light sampling),
     emissiveSampler ('Uniform'|'LightBVH'). With several light types enabled (env map, emissive,
-    analytic) one is chosen per vertex with equal probability."""
+    analytic) one is chosen per vertex with equal probability. tailPaths (default 65536, 0 off):
+    a bounce depth with at most that many hits runs its paths to their end in one launch (same
+    image, bit for bit)."""
 
     kind = "MinimalPathTracer"
     outputs = ("color",)
