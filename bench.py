@@ -130,11 +130,13 @@ def forward_bench(steps: int, warmup: int, tune: bool, tune_out: str | None):
     for i in range(warmup):
         p.render(W, H, SPP, frame=10 + i, out=out)
     # Ray counts of the timed frames (untimed).
-    rays_c = rays_v = 0
+    rays_c = rays_v = tail_c = tail_v = 0
     for f in frames:
         _, st = p.render(W, H, SPP, frame=f, out=out, stats=True)
         rays_c += int(st[0])
         rays_v += int(st[1])
+        tail_c += int(st[6])  # of those, traced by the bounce loop's tail launch
+        tail_v += int(st[7])
     torch.cuda.synchronize()
     ts = time.perf_counter()
     for f in frames:
@@ -160,12 +162,13 @@ def forward_bench(steps: int, warmup: int, tune: bool, tune_out: str | None):
     s_node = cs[4] / max(cs[1], 1)
     s_tri = cs[5] / max(cs[1], 1)
     # Algorithmic bytes of the traversal kernel (kTraceBounce: primary, closest-hit and visibility
-    # rays of every bounce): each closest-hit ray reads its 32 B ray and writes a 16 B hit, each
-    # visibility ray reads its 48 B record, and every visited node / tested triangle is read once.
+    # rays of every bounce up to the tail launch): each closest-hit ray reads its 32 B ray and
+    # writes a 16 B hit, each visibility ray reads its 48 B record, and every visited node /
+    # tested triangle is read once (per-ray averages of the whole frame).
     bytes_per_ray = RAY_BYTES + HIT_BYTES + NODE_BYTES * n_node + TRI_BYTES * n_tri
     bytes_per_vis = VIS_RECORD_BYTES + NODE_BYTES * s_node + TRI_BYTES * s_tri
     trace_ms, trace_launches = kt["trace"]
-    trace_bytes = bytes_per_ray * rays_c + bytes_per_vis * rays_v
+    trace_bytes = bytes_per_ray * (rays_c - tail_c) + bytes_per_vis * (rays_v - tail_v)
     achieved = trace_bytes / (trace_ms * 1e-3) / 1e9 if trace_ms > 0 else 0.0
     img = out[..., :3].float()
     res = dict(
@@ -174,7 +177,9 @@ def forward_bench(steps: int, warmup: int, tune: bool, tune_out: str | None):
         bytes_per_vis=float(bytes_per_vis), bytes_per_launch=trace_bytes / max(trace_launches, 1),
         trace_launches_per_step=trace_launches / steps, trace_avg_launch_ms=trace_ms / max(trace_launches, 1),
         achieved_gbs=achieved, image_mean=float(img.mean()), build_s=t2 - t1, info=info, W=W, H=H, spp=SPP, B=B,
-        data=data, launch=launch, shadow_nodes_per_ray=float(s_node), shadow_tris_per_ray=float(s_tri))
+        data=data, launch=launch, shadow_nodes_per_ray=float(s_node), shadow_tris_per_ray=float(s_tri),
+        tail_rays_closest=tail_c / steps, tail_rays_visibility=tail_v / steps,
+        tail_ms_per_step=kt["other"][0] / steps)
     log(f"forward: {mrays:.1f} Mrays/s, {res['ms_per_step']:.2f} ms/frame, {rays_c / steps / 1e6:.2f}M closest + "
         f"{rays_v / steps / 1e6:.2f}M visibility rays/frame; nodes/ray {n_node:.1f} tris/ray {n_tri:.1f} "
         f"(visibility {s_node:.1f} / {s_tri:.1f}); launch {launch}; kernels {kt}")
@@ -509,6 +514,8 @@ def main():
                    "traversal_launch": fwd["launch"],
                    "visibility_nodes_per_ray": round(fwd["shadow_nodes_per_ray"], 2),
                    "visibility_tris_per_ray": round(fwd["shadow_tris_per_ray"], 2),
+                   "tail_rays_per_step": [int(fwd["tail_rays_closest"]), int(fwd["tail_rays_visibility"])],
+                   "tail_ms_per_step": round(fwd["tail_ms_per_step"], 4),
                    "bvh_build_s": round(fwd["build_s"], 2)},
         "roofline": {"bound": "hbm", "achieved": round(fwd["achieved_gbs"], 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(fwd["achieved_gbs"] / HBM_PEAK_GBS, 5), "traffic": traffic,

@@ -1235,7 +1235,7 @@ int fhp_pass_execute(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t spp
     if (rc) return rc;
     const bool count = p->get("countTraversal") != 0;
     unsigned long long* dstats = p->stats.as<unsigned long long>();
-    if (h_stats) HIP_OR_FAIL(launchFill(reinterpret_cast<float*>(dstats), 12, 0.f, st));
+    if (h_stats) HIP_OR_FAIL(launchFill(reinterpret_cast<float*>(dstats), 16, 0.f, st));
     TraceScratch ts;
     if ((rc = passTrace(s, p, ts))) return rc;
     if (p->kind == "MinimalPathTracer") {
@@ -1320,10 +1320,10 @@ int fhp_pass_execute(FhpPass p, FhpScene s, uint32_t W, uint32_t H, uint32_t spp
         if ((rc = collectTimer(p, st))) return rc;
     }
     if (h_stats) {
-        unsigned long long hs[6];
-        HIP_OR_FAIL(hipMemcpyAsync(hs, dstats, 48, hipMemcpyDeviceToHost, st));
+        unsigned long long hs[8];
+        HIP_OR_FAIL(hipMemcpyAsync(hs, dstats, 64, hipMemcpyDeviceToHost, st));
         HIP_OR_FAIL(hipStreamSynchronize(st));
-        for (int i = 0; i < 6; i++) h_stats[i] = hs[i];
+        for (int i = 0; i < 8; i++) h_stats[i] = hs[i];
     }
     return FHP_OK;
 }
@@ -1483,7 +1483,7 @@ int fhp_pass_execute_batch(FhpPass p, uint32_t nv, const FhpView* views, uint64_
     HIP_OR_FAIL(hipEventRecord(p->stageDone, st));
     const bool count = p->get("countTraversal") != 0;
     unsigned long long* dstats = p->stats.as<unsigned long long>();
-    if (h_stats) HIP_OR_FAIL(launchFill(reinterpret_cast<float*>(dstats), 12, 0.f, st));
+    if (h_stats) HIP_OR_FAIL(launchFill(reinterpret_cast<float*>(dstats), 16, 0.f, st));
     MptArgs a;
     std::memset(&a, 0, sizeof(a));  // padding too: the bytes key the captured graph
     a.maxBounces = uint32_t(p->get("maxBounces"));
@@ -1548,10 +1548,10 @@ int fhp_pass_execute_batch(FhpPass p, uint32_t nv, const FhpView* views, uint64_
     }
     if ((rc = collectTimer(p, st))) return rc;
     if (h_stats) {
-        unsigned long long hs[6];
-        HIP_OR_FAIL(hipMemcpyAsync(hs, dstats, 48, hipMemcpyDeviceToHost, st));
+        unsigned long long hs[8];
+        HIP_OR_FAIL(hipMemcpyAsync(hs, dstats, 64, hipMemcpyDeviceToHost, st));
         HIP_OR_FAIL(hipStreamSynchronize(st));
-        for (int i = 0; i < 6; i++) h_stats[i] = hs[i];
+        for (int i = 0; i < 8; i++) h_stats[i] = hs[i];
     }
     return FHP_OK;
 }
@@ -1688,7 +1688,7 @@ static int warExecuteBatch(FhpPass p, uint32_t nv, const FhpView* views, uint64_
     int rc = ensurePassBuffersN(p, n, npix);
     if (rc) return rc;
     unsigned long long* dstats = p->stats.as<unsigned long long>();
-    if (h_stats) HIP_OR_FAIL(launchFill(reinterpret_cast<float*>(dstats), 12, 0.f, st));
+    if (h_stats) HIP_OR_FAIL(launchFill(reinterpret_cast<float*>(dstats), 16, 0.f, st));
     const bool fixed = p->get("useFixedSeed") != 0;
     std::vector<WarView> vt(nv);
     for (uint32_t i = 0; i < nv; i++)
@@ -1714,10 +1714,10 @@ static int warExecuteBatch(FhpPass p, uint32_t nv, const FhpView* views, uint64_
     HIP_OR_FAIL(launchWarPrimal(s->dev, a, ts, st, &p->timer));
     if ((rc = collectTimer(p, st))) return rc;
     if (h_stats) {
-        unsigned long long hs[6];
-        HIP_OR_FAIL(hipMemcpyAsync(hs, dstats, 48, hipMemcpyDeviceToHost, st));
+        unsigned long long hs[8];
+        HIP_OR_FAIL(hipMemcpyAsync(hs, dstats, 64, hipMemcpyDeviceToHost, st));
         HIP_OR_FAIL(hipStreamSynchronize(st));
-        for (int i = 0; i < 6; i++) h_stats[i] = hs[i];
+        for (int i = 0; i < 8; i++) h_stats[i] = hs[i];
     }
     return FHP_OK;
 }

@@ -867,7 +867,8 @@ __global__ void kResolve(SceneDev s0, MptArgs a, float4* out) {
 }
 
 __global__ void kRayCounts(const MptArgs a, uint32_t nb, unsigned long long* out) {
-    // out[0] += closest-hit rays (primary + all scatter queues), out[1] += shadow rays.
+    // out[0] += closest-hit rays (primary + all scatter queues), out[1] += shadow rays; out[6],
+    // out[7] += the closest-hit and shadow rays of those that the tail launch traced.
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         unsigned long long c = a.vbuf ? 0ull : (unsigned long long)a.nPix, v = 0;
         for (uint32_t b = 0; b < nb; b++) c += a.counters[b];
@@ -876,6 +877,8 @@ __global__ void kRayCounts(const MptArgs a, uint32_t nb, unsigned long long* out
         v += a.counters[kTailVisibilityCount];
         out[0] += c;
         out[1] += v;
+        out[6] += a.counters[kTailClosestCount];
+        out[7] += a.counters[kTailVisibilityCount];
     }
 }
 

@@ -135,16 +135,19 @@ FHP_HD f3 safeInv(f3 d) {
     return mk3(1.f / x, 1.f / y, 1.f / z);
 }
 
-// Resumable traversal of one ray. Each `step` visits one node: an internal node's children are
-// tested and the nearest hit child becomes current (the others are pushed), or a leaf's triangles
-// are tested and the next node is popped. `step` returns false once the ray is finished (stack
-// empty, or the first hit of an any-hit ray), so a kernel can start a new ray on a lane as soon as
-// the lane's previous ray is finished instead of waiting for the slowest lane of the wave.
+// Resumable traversal of one ray. Each `step` does one unit of work: at an internal node its
+// children are tested and the nearest hit child becomes current (the others are pushed); at a
+// leaf one triangle is tested, and after the leaf's last one the next node is popped. One
+// triangle per step keeps a wave's lanes in leaves and at internal nodes advancing together
+// (a step costs one node test or one triangle test) instead of every lane waiting for the
+// largest leaf among them. `step` returns false once the ray is finished (stack empty, or the
+// first hit of an any-hit ray).
 struct TravState {
     f3 o, d, inv;
     float tmin;
     HitRec hit;
     uint32_t cur, sp;
+    uint32_t k;  // next triangle of the current leaf (0 unless cur is a leaf being tested)
     bool any;
     FHP_DEVI void start(f3 org, f3 dir, float t0, float t1, bool anyHit) {
         o = org;
@@ -156,6 +159,7 @@ struct TravState {
         hit.b1 = hit.b2 = 0.f;
         cur = 0;  // root node (always internal)
         sp = 0;
+        k = 0;
         any = anyHit;
     }
     template <bool COUNT, typename SceneT>
@@ -188,20 +192,19 @@ struct TravState {
                 return true;
             }
         } else {
-            const uint32_t first = leafFirst(cur), count = leafCount(cur);
-            for (uint32_t j = 0; j < count; j++) {
-                const float4* tp = s.triPos + 3 * size_t(first + j);
-                const float4 a = tp[0], b = tp[1], c = tp[2];
-                const uint32_t id = f2u(a.w);
-                float t, b1, b2;
-                if (COUNT) cnt->tris++;
-                if (watertight(o, d, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), mk3(c.x, c.y, c.z), t, b1, b2)) {
-                    if (t >= tmin && t <= hit.t && (t < hit.t || hit.tri == kInvalid || id < hit.tri)) {
-                        hit.tri = id; hit.t = t; hit.b1 = b1; hit.b2 = b2;
-                        if (any) return false;
-                    }
+            const float4* tp = s.triPos + 3 * size_t(leafFirst(cur) + k);
+            const float4 a = tp[0], b = tp[1], c = tp[2];
+            const uint32_t id = f2u(a.w);
+            float t, b1, b2;
+            if (COUNT) cnt->tris++;
+            if (watertight(o, d, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), mk3(c.x, c.y, c.z), t, b1, b2)) {
+                if (t >= tmin && t <= hit.t && (t < hit.t || hit.tri == kInvalid || id < hit.tri)) {
+                    hit.tri = id; hit.t = t; hit.b1 = b1; hit.b2 = b2;
+                    if (any) return false;
                 }
             }
+            if (++k < leafCount(cur)) return true;
+            k = 0;
         }
         if (sp == 0) return false;
         cur = st.get(--sp);

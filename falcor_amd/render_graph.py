@@ -446,7 +446,7 @@ class _NativePass(RenderPass):
         if out is None:
             out = torch.empty((height, width, 4), dtype=torch.float32, device=dev)
         f = self.frame_count if frame is None else frame
-        st = np.zeros(6, np.uint64) if stats else None
+        st = np.zeros(8, np.uint64) if stats else None
         _lib.check(_lib.load().fhp_pass_execute(self.handle, self.scene.handle if self.scene else None, width, height, spp, f,
                                                 out.data_ptr(), st.ctypes.data if stats else None, _lib.stream_ptr()))
         if frame is None:
@@ -481,7 +481,7 @@ class _NativePass(RenderPass):
             arr[i].width, arr[i].height = w, h
             arr[i].spp, arr[i].frame = int(v.get("spp", 1)), int(v.get("frame", 0))
             arr[i].d_color = out.data_ptr()
-        st = np.zeros(6, np.uint64) if stats else None
+        st = np.zeros(8, np.uint64) if stats else None
         _lib.check(_lib.load().fhp_pass_execute_batch(self.handle, len(views), arr, st.ctypes.data if stats else None,
                                                       _lib.stream_ptr()))
         return (outs, st) if stats else outs

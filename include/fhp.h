@@ -239,9 +239,11 @@ int fhp_pass_get_property(FhpPass pass, const char* key, double* value);
  * AccumulatePass Single mode does (Accumulate.cs.slang:76-114); sample k of frame f uses
  * sample number f * spp + k (MinimalPathTracer) or frame seed f * spp + k (WARDiffPathTracer).
  * Replaces RenderPass::execute (MinimalPathTracer.cpp:110-199, WARDiffPathTracer.cpp:221-...).
- * stats (optional, 6 words): closest-hit rays, visibility rays, then BVH nodes visited and
- * triangles tested by closest-hit and by visibility queries (the last four only when the pass
- * property "countTraversal" is 1; MinimalPathTracer only). */
+ * stats (optional, 8 words): closest-hit rays, visibility rays, then BVH nodes visited and
+ * triangles tested by closest-hit and by visibility queries (these four only when the pass
+ * property "countTraversal" is 1; MinimalPathTracer only), then the closest-hit and visibility
+ * rays of the first two words that the bounce loop's tail launch traced (MinimalPathTracer
+ * property "tailPaths"; zero otherwise). */
 int fhp_pass_execute(FhpPass pass, FhpScene scene, uint32_t width, uint32_t height, uint32_t spp, uint32_t frame,
                      float* d_color, uint64_t* h_stats, FhpStream stream);
 
@@ -268,7 +270,8 @@ int fhp_pass_execute_noalloc(FhpPass pass, FhpScene scene, uint32_t width, uint3
  * 65535 views; with useGraph the bounce loop is captured once per batch layout (view count,
  * path and pixel totals, traversal launch) and replayed with each call's view table, which is
  * staged through pinned memory, so the call does not wait for the batch (only for the previous
- * call's table copy). WARDiffPathTracer synchronises the stream before returning. */
+ * call's table copy). WARDiffPathTracer synchronises the stream before returning. stats
+ * (optional) as fhp_pass_execute's, summed over the views. */
 typedef struct FhpView {
     FhpScene scene;
     const float* h_camera;
@@ -279,7 +282,8 @@ int fhp_pass_execute_batch(FhpPass pass, uint32_t num_views, const FhpView* view
 
 /* Accumulated kernel times (ms) and launch counts by kernel kind, recorded with HIP events on
  * the launch stream when the pass property "timeKernels" is 1: [0] closest-hit traversal,
- * [1] shadow traversal, [2] shading, [3] WAR primal, [4] WAR adjoint, [5] other.
+ * [1] shadow traversal, [2] shading, [3] WAR primal, [4] WAR adjoint, [5] other (the bounce loop's
+ * tail launch).
  * reset != 0 clears the accumulators. Replaces the GPU timer events of FALCOR_PROFILE
  * (Source/Falcor/Utils/Timing/Profiler.h:305). */
 int fhp_pass_kernel_times(FhpPass pass, double* ms, uint64_t* launches, int reset);

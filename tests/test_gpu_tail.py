@@ -65,6 +65,10 @@ def test_tail_equals_per_depth_launches(gpu):
             img, st = p.render(W, H, spp, frame=5, stats=True)
             assert torch.equal(img, ref), f"{name}, tailPaths {t}"
             assert (int(st[0]), int(st[1])) == rays, f"{name}, tailPaths {t}"
+            assert int(st[6]) <= rays[0] and int(st[7]) <= rays[1]
+            if t == 1 << 24 and props["maxBounces"] > 0:
+                assert int(st[6]) > 0, name  # the tail started at depth 0 and traced every bounce ray
+        assert int(rs[6]) == 0 and int(rs[7]) == 0
 
 
 def test_tail_in_batches_and_graph_replays(gpu):
