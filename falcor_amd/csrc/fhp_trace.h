@@ -61,6 +61,16 @@ struct TraceStack {
 #endif
         }
     }
+    // Entry i, known to be an LDS entry (i < ldsDepth).
+    FHP_DEVI void putLds(uint32_t i, uint32_t v) { lds[i * ldsStride] = v; }
+    // Records a stack height reached by putLds writes (the host check build's peak).
+    FHP_DEVI void notePeak(uint32_t top) {
+#ifdef FHP_HOST_CHECK
+        if (top > peak) peak = top;
+#else
+        (void)top;
+#endif
+    }
     FHP_DEVI uint32_t get(uint32_t i) const {
         if (i < ldsDepth) return lds[i * ldsStride];
         return i - ldsDepth < ovfDepth ? ovf[(i - ldsDepth) * ovfStride] : kOverflowLeaf;
@@ -183,11 +193,24 @@ struct TravState {
                 sortPair(t1, c1, t3, c3);
                 sortPair(t1, c1, t2, c2);
                 // The scene builder sizes the stack to the tree's exact worst case
-                // (sum of (children - 1) along any root-to-leaf path).
+                // (sum of (children - 1) along any root-to-leaf path). The k - 1 pushed entries
+                // are p0, p1, c1 cut to their first k - 1; while three entries fit in LDS all
+                // three are written without branches (entries above the new top are free).
                 const uint32_t k = popc32(mask);
-                if (k > 3u) st.put(sp++, c3);
-                if (k > 2u) st.put(sp++, c2);
-                if (k > 1u) st.put(sp++, c1);
+                if (k > 1u) {
+                    const uint32_t p0 = k > 3u ? c3 : (k > 2u ? c2 : c1), p1 = k > 3u ? c2 : c1;
+                    if (sp + 3u <= st.ldsDepth) {
+                        st.putLds(sp, p0);
+                        st.putLds(sp + 1u, p1);
+                        st.putLds(sp + 2u, c1);
+                        st.notePeak(sp + k - 1u);
+                    } else {
+                        st.put(sp, p0);
+                        if (k > 2u) st.put(sp + 1u, p1);
+                        if (k > 3u) st.put(sp + 2u, c1);
+                    }
+                    sp += k - 1u;
+                }
                 cur = c0;
                 return true;
             }
