@@ -107,17 +107,23 @@ FHP_DEVI uint32_t testNode4(const float4* __restrict__ node, f3 o, f3 inv, float
     // rounding this adds is far below the build's box padding, so the test stays conservative.
     const float bx0 = (n0.x - o.x) * inv.x, by0 = (n0.y - o.y) * inv.y, bz0 = (n0.z - o.z) * inv.z;
     const float kx = n0.w * inv.x, ky = n1.x * inv.y, kz = n1.y * inv.z;
+    // The distance is monotone in the code (fmaf is), non-decreasing where k >= 0: the near plane
+    // of every child on an axis is its low plane there and its high plane elsewhere, so the codes
+    // are sorted into near and far once per node instead of a min and a max per child and axis.
     const uint32_t qlx = f2u(n1.z), qhx = f2u(n1.w), qly = f2u(n2.x), qhy = f2u(n2.y), qlz = f2u(n2.z), qhz = f2u(n2.w);
+    const bool px = kx >= 0.f, py = ky >= 0.f, pz = kz >= 0.f;
+    const uint32_t nqx = px ? qlx : qhx, fqx = px ? qhx : qlx, nqy = py ? qly : qhy, fqy = py ? qhy : qly;
+    const uint32_t nqz = pz ? qlz : qhz, fqz = pz ? qhz : qlz;
     const uint32_t ch[4] = {child.x, child.y, child.z, child.w};
     uint32_t mask = 0;
 #pragma unroll
     for (int i = 0; i < 4; i++) {
         const uint32_t sh = 8u * uint32_t(i);
-        float ax = fmaf(float((qlx >> sh) & 0xffu), kx, bx0), bx = fmaf(float((qhx >> sh) & 0xffu), kx, bx0);
-        float ay = fmaf(float((qly >> sh) & 0xffu), ky, by0), by = fmaf(float((qhy >> sh) & 0xffu), ky, by0);
-        float az = fmaf(float((qlz >> sh) & 0xffu), kz, bz0), bz = fmaf(float((qhz >> sh) & 0xffu), kz, bz0);
-        float n = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), tmin));
-        float f = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), tmax));
+        const float nx = fmaf(float((nqx >> sh) & 0xffu), kx, bx0), fx = fmaf(float((fqx >> sh) & 0xffu), kx, bx0);
+        const float ny = fmaf(float((nqy >> sh) & 0xffu), ky, by0), fy = fmaf(float((fqy >> sh) & 0xffu), ky, by0);
+        const float nz = fmaf(float((nqz >> sh) & 0xffu), kz, bz0), fz = fmaf(float((fqz >> sh) & 0xffu), kz, bz0);
+        float n = fmaxf(fmaxf(nx, ny), fmaxf(nz, tmin));
+        float f = fminf(fminf(fx, fy), fminf(fz, tmax));
         f = f * 1.00000048f;
         tn[i] = n;
         if (n <= f && ch[i] != kInvalid) mask |= 1u << i;
