@@ -249,7 +249,8 @@ const std::map<std::string, double>& mptDefaults() {
                                                    {"countTraversal", 0},
                                                    {"timeKernels", 0}, {"useGraph", 1}, {"traceBlockSize", kDefaultBlock},
                                                    {"traceStackDepth", kDefaultLds}, {"traceBlocksPerCU", kDefaultBlocksPerCU},
-                                                   {"splitVisibilityLaunch", 0}, {"tailPaths", 65536}};
+                                                   {"splitVisibilityLaunch", 0}, {"tailPaths", 65536},
+                                                   {"tailLanes", 8}};
     return d;
 }
 // WARDiffPathTracer's DiffMode (WARDiffPathTracer.h:56-62) and DiffVariableType (SharedTypes.slang:78-84).
@@ -418,6 +419,7 @@ int mptFrameArgs(const FhpPass_* p, const FhpScene_* s, uint32_t W, uint32_t H, 
     a.lightKind = uint32_t(p->get("emissiveSampler"));
     a.separateShadow = p->get("splitVisibilityLaunch") != 0;
     a.tailPaths = uint32_t(p->get("tailPaths"));
+    a.tailLanes = uint32_t(p->get("tailLanes"));
     a.envNee = p->get("useEnvNEE") != 0;
     if (a.nee && !a.computeDirect) return fail(FHP_EINVAL, "useEmissiveNEE requires computeDirect");
     if (a.envNee && !a.computeDirect) return fail(FHP_EINVAL, "useEnvNEE requires computeDirect");
@@ -1198,6 +1200,8 @@ int fhp_pass_set_property(FhpPass p, const char* key, double v) {
     }
     if (k == "tailPaths" && !(v >= 0 && v <= 16777216 && v == double(uint32_t(v))))
         return fail(FHP_EINVAL, "tailPaths: an integer in [0, 2^24]");
+    if (k == "tailLanes" && !(v >= 1 && v <= 64 && v == double(uint32_t(v))))
+        return fail(FHP_EINVAL, "tailLanes: an integer in [1, 64]");
     if (k == "traceBlockSize" || k == "traceStackDepth" || k == "traceBlocksPerCU") {
         const double b = k == "traceBlockSize" ? v : p->get("traceBlockSize");
         const double l = k == "traceStackDepth" ? v : p->get("traceStackDepth");
@@ -1500,6 +1504,7 @@ int fhp_pass_execute_batch(FhpPass p, uint32_t nv, const FhpView* views, uint64_
     a.lightMode = otherLights ? 2u : (a.nee ? 1u : 0u);
     a.separateShadow = 0;
     a.tailPaths = uint32_t(p->get("tailPaths"));
+    a.tailLanes = uint32_t(p->get("tailLanes"));
     a.nPaths = uint32_t(nPaths);
     a.nPix = uint32_t(nPix);
     for (FhpScene sc : scenes) a.keepMisses |= (sc->dev.envTex && (sc->dev.envFlags & kEnvLight)) ? 1u : 0u;

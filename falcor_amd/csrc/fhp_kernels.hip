@@ -631,9 +631,14 @@ __global__ __launch_bounds__(B) void kTail(SceneDev s0, MptArgs a, uint32_t dept
     if (!tailActive(a, n)) return;
     TraceStack st = makeStack<B, D>(lds, ts, blockIdx.x, threadIdx.x);
     const int qi = depth & 1;
-    const uint32_t G = gridDim.x * B;
+    // a.tailLanes paths per wave (the other lanes idle): a wave's bounce lasts as long as its
+    // slowest path's, and fewer paths per wave make that maximum smaller and let more waves hide
+    // each other's latency.
+    const uint32_t P = a.tailLanes - 1u < 64u ? a.tailLanes : 64u;
+    const uint32_t lane = threadIdx.x & 63u, waves = (gridDim.x * B) >> 6;
+    const uint32_t wave = (blockIdx.x * B + threadIdx.x) >> 6;
     uint32_t nClosest = 0, nVis = 0;
-    for (uint32_t k = blockIdx.x * B + threadIdx.x; k < n; k += G) {
+    for (uint32_t k = lane < P ? wave * P + lane : n; k < n; k += waves * P) {
         const uint32_t i = a.hitIdx[k];
         const uint32_t pathId = a.pathQ[qi][i];
         const SceneDev& s = *viewCtx<MULTI>(s0, a, pathId, true).s;
@@ -969,7 +974,10 @@ hipError_t mptBounces(const SceneDev& s, const MptArgs& a0, const TraceScratch& 
         tm->begin(KK_OTHER, st);
         hipError_t e = withTraceConfig(tsc, [&](auto Bc, auto Dc) {
             constexpr int B = decltype(Bc)::value, D = decltype(Dc)::value;
-            const uint32_t blocks = std::max<uint32_t>(1u, std::min<uint32_t>(tsc.gridBlocks, (a.tailPaths + B - 1) / B));
+            // One wave per tailLanes paths, within the persistent grid (the overflow stack's size).
+            const uint32_t lanes = a.tailLanes - 1u < 64u ? a.tailLanes : 64u;
+            const uint64_t threads = 64ull * ((uint64_t(a.tailPaths) + lanes - 1) / lanes);
+            const uint32_t blocks = uint32_t(std::max<uint64_t>(1u, std::min<uint64_t>(tsc.gridBlocks, (threads + B - 1) / B)));
             if (a.lightMode == kLightsNone)
                 hipLaunchKernelGGL((kTail<B, D, MULTI, kLightsNone>), dim3(blocks), dim3(B), 0, st, s, a, depth, tsc);
             else if (a.lightMode == kLightsEmissive)

@@ -493,7 +493,7 @@ class MinimalPathTracer(_NativePass):
     emissiveSampler ('Uniform'|'LightBVH'). With several light types enabled (env map, emissive,
     analytic) one is chosen per vertex with equal probability. tailPaths (default 65536, 0 off):
     a bounce depth with at most that many hits runs its paths to their end in one launch (same
-    image, bit for bit)."""
+    image, bit for bit), tailLanes (default 8) of them per wave."""
 
     kind = "MinimalPathTracer"
     outputs = ("color",)

@@ -16,7 +16,9 @@ from falcor_amd import scene as S
 
 pytestmark = pytest.mark.gpu
 
-THRESHOLDS = (1, 300, 4000, 1 << 24)
+# (tailPaths, tailLanes): thresholds from the last depths only to the first, with 64, 8 and 1
+# paths per wave of the tail launch.
+THRESHOLDS = ((1, 64), (300, 8), (4000, 1), (4000, 64), (1 << 24, 8))
 
 
 def _transmissive(data, seed):
@@ -60,11 +62,12 @@ def test_tail_equals_per_depth_launches(gpu):
         ref = ref.clone()
         rays = (int(rs[0]), int(rs[1]))
         print(f"{name}: mean {float(ref[..., :3].mean()):.5f}, rays {rays}", flush=True)
-        for t in THRESHOLDS:
+        for t, lanes in THRESHOLDS:
             p.set_property("tailPaths", t)
+            p.set_property("tailLanes", lanes)
             img, st = p.render(W, H, spp, frame=5, stats=True)
-            assert torch.equal(img, ref), f"{name}, tailPaths {t}"
-            assert (int(st[0]), int(st[1])) == rays, f"{name}, tailPaths {t}"
+            assert torch.equal(img, ref), f"{name}, tailPaths {t}, tailLanes {lanes}"
+            assert (int(st[0]), int(st[1])) == rays, f"{name}, tailPaths {t}, tailLanes {lanes}"
             assert int(st[6]) <= rays[0] and int(st[7]) <= rays[1]
             if t == 1 << 24 and props["maxBounces"] > 0:
                 assert int(st[6]) > 0, name  # the tail started at depth 0 and traced every bounce ray
@@ -102,6 +105,10 @@ def test_tail_property_checks(gpu):
     from falcor_amd.render_graph import MinimalPathTracer
     p = MinimalPathTracer({})
     assert p.get_property("tailPaths") == 65536
+    assert p.get_property("tailLanes") == 8
     for bad in (-1, 0.5, float(1 << 25)):
         with pytest.raises(Exception):
             p.set_property("tailPaths", bad)
+    for bad in (0, 65, 2.5):
+        with pytest.raises(Exception):
+            p.set_property("tailLanes", bad)
