@@ -526,6 +526,7 @@ int buildAndUpload(FhpScene_* s) {
     dv.activeLights = s->active.as<uint32_t>();
     dv.triToLight = s->triToLight.as<uint32_t>();
     dv.lbvh = s->lbvh.as<uint4>();
+    dv.lbvhS = reinterpret_cast<const float4*>(s->lbvh.as<uint32_t>() + 8 * size_t(h.nLbvh));
     dv.lbvhTri = s->lbvhTri.as<uint32_t>();
     dv.lbvhMask = s->lbvhMask.as<uint2>();
     dv.cam.pos = mk3(h.cam[0], h.cam[1], h.cam[2]);
@@ -755,6 +756,7 @@ int fhp_scene_update_node_transform(FhpScene s, uint32_t mesh, const float* h_ma
         l.vtx = s->vtx.as<float4>();
         l.idx = s->idx.as<uint32_t>();
         l.lbvh = s->lbvh.as<uint32_t>();
+        l.nNodes = h.nLbvh;
         l.lbvhTri = s->lbvhTri.as<uint32_t>();
         l.nodes = s->lrefit.as<uint32_t>();
         l.off = h.lrefitOff.data();

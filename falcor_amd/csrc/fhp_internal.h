@@ -272,7 +272,8 @@ struct LightRefitArgs {
     uint32_t nLights;
     const float4* vtx;
     const uint32_t* idx;
-    uint32_t* lbvh;             // packed nodes, 8 words each
+    uint32_t* lbvh;             // packed nodes, 8 words each, then their sampling records
+    uint32_t nNodes;
     const uint32_t* lbvhTri;
     const uint32_t* nodes;      // device: refit schedule
     const uint32_t* off;        // host: depth + 2 offsets (internal nodes by depth, then leaves)

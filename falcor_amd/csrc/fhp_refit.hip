@@ -152,12 +152,21 @@ __device__ __forceinline__ LNodeAttr readLightNode(const uint32_t* lbvh, uint32_
     const uint4* q = reinterpret_cast<const uint4*>(lbvh) + 2 * size_t(n);
     return unpackLightNode(q[0], q[1]);
 }
-__device__ __forceinline__ void writeLightNode(uint32_t* lbvh, uint32_t n, const LNodeAttr& a) {
+// Packs node n and rewrites its sampling record (after the nNodes packed nodes) from the packed
+// values, as the scene builder derives it.
+__device__ __forceinline__ void writeLightNode(uint32_t* lbvh, uint32_t nNodes, uint32_t n, const LNodeAttr& a) {
     uint32_t w[8];
     packLightNode(w, lbvh[8 * size_t(n)], a);
     uint4* q = reinterpret_cast<uint4*>(lbvh) + 2 * size_t(n);
-    q[0] = make_uint4(w[0], w[1], w[2], w[3]);
-    q[1] = make_uint4(w[4], w[5], w[6], w[7]);
+    const uint4 q0 = make_uint4(w[0], w[1], w[2], w[3]), q1 = make_uint4(w[4], w[5], w[6], w[7]);
+    q[0] = q0;
+    q[1] = q1;
+    float rec[kLNodeSampleWords];
+    lightNodeSampleRecord(rec, w[0], unpackLightNode(q0, q1));
+    float4* r = reinterpret_cast<float4*>(lbvh + 8 * size_t(nNodes)) + 3 * size_t(n);
+    r[0] = make_float4(rec[0], rec[1], rec[2], rec[3]);
+    r[1] = make_float4(rec[4], rec[5], rec[6], rec[7]);
+    r[2] = make_float4(rec[8], rec[9], rec[10], rec[11]);
 }
 __device__ __forceinline__ void setAABB(LNodeAttr& a, f3 lo, f3 hi) {
     a.origin = (hi + lo) * 0.5f;
@@ -165,7 +174,7 @@ __device__ __forceinline__ void setAABB(LNodeAttr& a, f3 lo, f3 hi) {
 }
 
 // LightBVHRefit.cs.slang:46-93 (updateLeafNodes).
-__global__ void kLightRefitLeaves(uint32_t* __restrict__ lbvh, const uint32_t* __restrict__ nodes, uint32_t count,
+__global__ void kLightRefitLeaves(uint32_t* __restrict__ lbvh, uint32_t nNodes, const uint32_t* __restrict__ nodes, uint32_t count,
                                   const float* __restrict__ lights, const uint32_t* __restrict__ lbvhTri) {
     const uint32_t k = blockIdx.x * kRefitBlock + threadIdx.x;
     if (k >= count) return;
@@ -197,14 +206,15 @@ __global__ void kLightRefitLeaves(uint32_t* __restrict__ lbvh, const uint32_t* _
     }
     a.cosCone = cosCone;
     a.coneDir = dir;
-    writeLightNode(lbvh, n, a);
+    writeLightNode(lbvh, nNodes, n, a);
 }
 
 __device__ __forceinline__ float sinFromCos(float c) { return sqrtf(fmaxf(0.f, 1.f - c * c)); }
 
 // LightBVHRefit.cs.slang:101-170 (updateInternalNodes): box union, and the normal cone that holds
 // both children's cones (each child's spread rotated away from the merged direction).
-__global__ void kLightRefitInternal(uint32_t* __restrict__ lbvh, const uint32_t* __restrict__ nodes, uint32_t count) {
+__global__ void kLightRefitInternal(uint32_t* __restrict__ lbvh, uint32_t nNodes, const uint32_t* __restrict__ nodes,
+                                    uint32_t count) {
     const uint32_t k = blockIdx.x * kRefitBlock + threadIdx.x;
     if (k >= count) return;
     const uint32_t ni = nodes[k];
@@ -235,7 +245,7 @@ __global__ void kLightRefitInternal(uint32_t* __restrict__ lbvh, const uint32_t*
     }
     a.cosCone = cosCone;
     a.coneDir = dir;
-    writeLightNode(lbvh, ni, a);
+    writeLightNode(lbvh, nNodes, ni, a);
 }
 
 uint32_t blocksFor(uint64_t n) { return uint32_t((n + kRefitBlock - 1) / kRefitBlock); }
@@ -277,12 +287,12 @@ hipError_t launchLightRefit(const LightRefitArgs& a, hipStream_t st) {
                            a.idx);
     const uint32_t lb = a.off[a.depth], le = a.off[a.depth + 1];
     if (le > lb)
-        hipLaunchKernelGGL(kLightRefitLeaves, dim3(blocksFor(le - lb)), dim3(kRefitBlock), 0, st, a.lbvh, a.nodes + lb,
+        hipLaunchKernelGGL(kLightRefitLeaves, dim3(blocksFor(le - lb)), dim3(kRefitBlock), 0, st, a.lbvh, a.nNodes, a.nodes + lb,
                            le - lb, a.lights, a.lbvhTri);
     for (int d = int(a.depth) - 1; d >= 0; d--) {
         const uint32_t b = a.off[d], e = a.off[d + 1];
         if (e > b)
-            hipLaunchKernelGGL(kLightRefitInternal, dim3(blocksFor(e - b)), dim3(kRefitBlock), 0, st, a.lbvh, a.nodes + b,
+            hipLaunchKernelGGL(kLightRefitInternal, dim3(blocksFor(e - b)), dim3(kRefitBlock), 0, st, a.lbvh, a.nNodes, a.nodes + b,
                                e - b);
     }
     return hipGetLastError();

@@ -108,6 +108,7 @@ struct SceneDev {
     const uint32_t* activeLights;
     const uint32_t* triToLight;
     const uint4* lbvh;           // 2 x uint4 per node, the packed layout of packLightNode
+    const float4* lbvhS;         // 3 x float4 per node, lightNodeSampleRecord of the packed node
     const uint32_t* lbvhTri;
     const uint2* lbvhMask;
     CameraDev cam;
@@ -923,25 +924,37 @@ FHP_HD LNodeAttr unpackLightNode(uint4 q0, uint4 q1) {
     return a;
 }
 
+// Sampling record of a light-BVH node: its packed node decoded (unpackLightNode) together with
+// the importance terms that do not depend on the shading point, so the sampler does not decode
+// half floats and the octahedral axis at every visit (3 x float4, kept beside the packed nodes and
+// rewritten whenever they are):
+//   (origin, flux), (cone axis, cos cone), (|extent|^2, max extent, sin cone, packed word 0)
+constexpr uint32_t kLNodeSampleWords = 12;
+FHP_HD void lightNodeSampleRecord(float w[12], uint32_t word0, const LNodeAttr& a) {
+    w[0] = a.origin.x; w[1] = a.origin.y; w[2] = a.origin.z; w[3] = a.flux;
+    w[4] = a.coneDir.x; w[5] = a.coneDir.y; w[6] = a.coneDir.z; w[7] = a.cosCone;
+    w[8] = dot(a.extent, a.extent);
+    w[9] = fmaxf_(a.extent.x, fmaxf_(a.extent.y, a.extent.z));
+    w[10] = sqrtf(fmaxf_(0.f, 1.f - a.cosCone * a.cosCone));
+    w[11] = u2f(word0);
+}
+
 // Light BVH node accessors: info = (right child | leaf bit, triOffset, triCount, 0).
 FHP_HD float cosSubClamped(float sA, float cA, float sB, float cB) { return cA > cB ? 1.f : cA * cB + sA * sB; }
 FHP_HD float sinSubClamped(float sA, float cA, float sB, float cB) { return cA > cB ? 0.f : sA * cB - cA * sB; }
 // LightBVHSampler.slang:219-273 (sphere solid-angle bound, bounding and lighting cones)
 FHP_HD float lbvhImportance(const SceneDev& s, f3 posW, f3 n, uint32_t node) {
-    // The packed node's fields, decoded as unpackLightNode does; the cone axis only where the
-    // orientation bound needs it. The reference's terms with shared subexpressions and hardware
-    // square roots and reciprocals: normalize(origin - posW) and (origin - posW) / distance are one
-    // vector, the bounding sphere's sine sqrt(1 - cos^2) is its sqrt(r^2 / d^2).
-    const uint4 q0 = s.lbvh[2 * size_t(node)], q1 = s.lbvh[2 * size_t(node) + 1];
-    const f3 origin = mk3(u2f(q0.y), u2f(q0.z), u2f(q0.w));
-    const f3 extent = mk3(f16ToF32(q1.x & 0xffffu), f16ToF32(q1.x >> 16), f16ToF32(q1.y & 0xffffu));
-    const float flux = u2f(q1.w), cosCone = float(q1.y >> 16) * (1.f / 32767.f) - 1.f;
-    const f3 cv = origin - posW;
+    // The node's sampling record (the packed node decoded). The reference's terms with shared
+    // subexpressions and hardware square roots and reciprocals: normalize(origin - posW) and
+    // (origin - posW) / distance are one vector, the bounding sphere's sine sqrt(1 - cos^2) is its
+    // sqrt(r^2 / d^2).
+    const float4 r0 = s.lbvhS[3 * size_t(node)], r1 = s.lbvhS[3 * size_t(node) + 1], r2 = s.lbvhS[3 * size_t(node) + 2];
+    const f3 cv = mk3(r0.x - posW.x, r0.y - posW.y, r0.z - posW.z);
+    const float flux = r0.w, cosCone = r1.w, sqrR = r2.x, halfR = r2.y;
     const float cd2 = dot(cv, cv);
     float distance = hwSqrt(cd2);
     const f3 L = cv * hwRcp(distance);
     // boundCosineTerm (LightBVHSampler.slang:181-210) with GeometryHelpers.slang:222-240.
-    const float sqrR = dot(extent, extent);
     float sinBC = 0.f, cosBC = -1.f;
     if (!(cd2 < sqrR)) {
         const float s2 = sqrR * hwRcp(cd2);
@@ -954,27 +967,18 @@ FHP_HD float lbvhImportance(const SceneDev& s, f3 posW, f3 n, uint32_t node) {
     if (NdotL == 0.f) return 0.f;
     float orient = 1.f;
     if (cosCone > 0.f) {
-        float px = snorm16ToFloat(q1.z & 0xffffu), py = snorm16ToFloat(q1.z >> 16);
-        const float z = 1.f - fabsf(px) - fabsf(py);
-        if (z < 0.f) {
-            const float wx = (1.f - fabsf(py)) * (px >= 0.f ? 1.f : -1.f), wy = (1.f - fabsf(px)) * (py >= 0.f ? 1.f : -1.f);
-            px = wx;
-            py = wy;
-        }
-        const f3 coneDir = mk3(px, py, z) * hwRsqrt(px * px + py * py + z * z);
-        const float sinCone = hwSqrt(fmaxf_(0.f, 1.f - cosCone * cosCone));
-        const float cosT = -dot(coneDir, L);
+        const float sinCone = r2.z;
+        const float cosT = -dot(mk3(r1.x, r1.y, r1.z), L);
         const float sinT = hwSqrt(fmaxf_(0.f, 1.f - cosT * cosT));
         const float c0 = cosSubClamped(sinT, cosT, sinCone, cosCone);
         const float s0 = sinSubClamped(sinT, cosT, sinCone, cosCone);
         orient = fmaxf_(0.f, cosSubClamped(s0, c0, sinBC, cosBC));
     }
-    const float halfR = fmaxf_(extent.x, fmaxf_(extent.y, extent.z));
     distance = fmaxf_(halfR, distance);
     return (flux * NdotL) * orient * hwRcp(distance * distance);
 }
 FHP_HD uint4 lbvhInfo(const SceneDev& s, uint32_t node) {
-    const uint32_t w = s.lbvh[2 * size_t(node)].x;
+    const uint32_t w = f2u(s.lbvhS[3 * size_t(node) + 2].w);
     if (!(w & kLeafBit)) return make_uint4(w, 0u, 0u, 0u);
     return make_uint4(kLeafBit, w & kLNodeOffsetMask, (w >> kLNodeCountShift) & kLNodeCountMask, 0u);
 }

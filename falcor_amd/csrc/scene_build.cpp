@@ -675,9 +675,9 @@ void buildLightBVH(HostScene& s) {
         L.cones(0, c);
     }
     s.nLbvh = uint32_t(s.lbvh.size() / 16);
-    // Pack the nodes (LightBVHTypes.slang:97-198); the host mirror keeps the decoded attributes,
-    // which are what sampling sees.
-    s.lbvhPacked.assign(8 * size_t(s.nLbvh), 0u);
+    // Pack the nodes (LightBVHTypes.slang:97-198), then their sampling records after all packed
+    // nodes; the host mirror keeps the decoded attributes, which are what sampling sees.
+    s.lbvhPacked.assign((8 + kLNodeSampleWords) * size_t(s.nLbvh), 0u);
     for (uint32_t n = 0; n < s.nLbvh; n++) {
         float* q = &s.lbvh[16 * size_t(n)];
         uint32_t info[3];
@@ -697,6 +697,9 @@ void buildLightBVH(HostScene& s) {
         uint32_t* w = &s.lbvhPacked[8 * size_t(n)];
         packLightNode(w, w0, a);
         const LNodeAttr d = unpackLightNode(make_uint4(w[0], w[1], w[2], w[3]), make_uint4(w[4], w[5], w[6], w[7]));
+        float rec[kLNodeSampleWords];
+        lightNodeSampleRecord(rec, w[0], d);
+        std::memcpy(&s.lbvhPacked[8 * size_t(s.nLbvh) + kLNodeSampleWords * size_t(n)], rec, sizeof(rec));
         q[4] = d.extent.x; q[5] = d.extent.y; q[6] = d.extent.z;
         q[7] = d.cosCone;
         q[8] = d.coneDir.x; q[9] = d.coneDir.y; q[10] = d.coneDir.z;

@@ -49,7 +49,7 @@ struct HostScene {
     // decoded attributes as 16 floats per node ((origin, flux) (extent, cosCone) (coneDir, 0)
     // (right child | leaf bit, first triangle, triangle count, 0)), leaf triangle lists and
     // per-light traversal bitmasks.
-    std::vector<uint32_t> lbvhPacked;
+    std::vector<uint32_t> lbvhPacked;  // packed nodes (8 words each), then their sampling records (12 each)
     std::vector<float> lbvh;
     std::vector<uint32_t> lbvhTri;
     std::vector<uint32_t> lbvhMask;  // 2 words per light

@@ -92,6 +92,7 @@ SceneDev bindScene(const HostScene& h, Arena& ar) {
     dv.activeLights = ar.dup<uint32_t>(h.activeLights);
     dv.triToLight = ar.dup<uint32_t>(h.triToLight);
     dv.lbvh = ar.dup<uint4>(h.lbvhPacked);
+    dv.lbvhS = reinterpret_cast<const float4*>(reinterpret_cast<const uint32_t*>(dv.lbvh) + 8 * size_t(h.nLbvh));
     dv.lbvhTri = ar.dup<uint32_t>(h.lbvhTri);
     dv.lbvhMask = ar.dup<uint2>(h.lbvhMask);
     dv.cam.pos = mk3(h.cam[0], h.cam[1], h.cam[2]);
