@@ -1202,8 +1202,8 @@ int fhp_pass_set_property(FhpPass p, const char* key, double v) {
     }
     if (k == "tailPaths" && !(v >= 0 && v <= 16777216 && v == double(uint32_t(v))))
         return fail(FHP_EINVAL, "tailPaths: an integer in [0, 2^24]");
-    if (k == "tailLanes" && !(v >= 1 && v <= 64 && v == double(uint32_t(v))))
-        return fail(FHP_EINVAL, "tailLanes: an integer in [1, 64]");
+    if (k == "tailLanes" && !(v >= 1 && v <= 32 && v == double(uint32_t(v))))
+        return fail(FHP_EINVAL, "tailLanes: an integer in [1, 32]");
     if (k == "traceBlockSize" || k == "traceStackDepth" || k == "traceBlocksPerCU") {
         const double b = k == "traceBlockSize" ? v : p->get("traceBlockSize");
         const double l = k == "traceStackDepth" ? v : p->get("traceStackDepth");

@@ -76,7 +76,7 @@ struct MptArgs {
     uint32_t keepMisses;  // 1: closest-hit misses are queued for shading too (environment light)
     uint32_t tailPaths;   // a depth with at most this many hits runs its paths to the end in one
                           // launch (kTail); 0: every depth has its own launches
-    uint32_t tailLanes;   // paths per wave of that launch (1..64; the other lanes idle)
+    uint32_t tailLanes;   // paths per wave of that launch (1..32, two lanes each; the other lanes idle)
 };
 
 // Bits of a scene's trace error word (TraceScratch::err), read back by fhp_scene_trace_errors.

@@ -631,14 +631,18 @@ __global__ __launch_bounds__(B) void kTail(SceneDev s0, MptArgs a, uint32_t dept
     if (!tailActive(a, n)) return;
     TraceStack st = makeStack<B, D>(lds, ts, blockIdx.x, threadIdx.x);
     const int qi = depth & 1;
-    // a.tailLanes paths per wave (the other lanes idle): a wave's bounce lasts as long as its
-    // slowest path's, and fewer paths per wave make that maximum smaller and let more waves hide
-    // each other's latency.
-    const uint32_t P = a.tailLanes - 1u < 64u ? a.tailLanes : 64u;
+    // a.tailLanes paths per wave (at most 32; the other lanes idle): a wave's bounce lasts as
+    // long as its slowest path's, and fewer paths per wave make that maximum smaller and let more
+    // waves hide each other's latency. Each path has two lanes that shade it alike; then the even
+    // lane traces the vertex's next closest-hit ray while the odd lane traces its visibility ray,
+    // and each reads the other's answer, so a bounce waits for the longer of the two traversals
+    // instead of both in turn.
+    const uint32_t P = a.tailLanes - 1u < 32u ? a.tailLanes : 32u;
     const uint32_t lane = threadIdx.x & 63u, waves = (gridDim.x * B) >> 6;
     const uint32_t wave = (blockIdx.x * B + threadIdx.x) >> 6;
+    const uint32_t role = lane & 1u, slot = lane >> 1;  // role 0: closest-hit ray, 1: visibility ray
     uint32_t nClosest = 0, nVis = 0;
-    for (uint32_t k = lane < P ? wave * P + lane : n; k < n; k += waves * P) {
+    for (uint32_t k = slot < P ? wave * P + slot : n; k < n; k += waves * P) {
         const uint32_t i = a.hitIdx[k];
         const uint32_t pathId = a.pathQ[qi][i];
         const SceneDev& s = *viewCtx<MULTI>(s0, a, pathId, true).s;
@@ -691,21 +695,28 @@ __global__ __launch_bounds__(B) void kTail(SceneDev s0, MptArgs a, uint32_t dept
                 t4 = make_float4(thp.x, thp.y, thp.z, sc.pdf);
                 n4 = make_float4(sd.N.x, sd.N.y, sd.N.z, u2f(sc.delta ? 1u : 0u));
             }
+            // Both lanes of the path hold the same shadow, cont and rays: they trace one each.
+            HitRec hr{kInvalid, 0.f, 0.f, 0.f};
+            bool found = false;
+            if (role ? shadow : cont)  // one call site: the pair's two traversals run side by side
+                found = traceBVH4<false, false>(g, role ? shO : newO, role ? shD : newD, 0.f, role ? shT : kFltMax, st,
+                                                hr, nullptr, role != 0u);
+            const bool occluded = __shfl(found ? 1u : 0u, int(lane | 1u)) != 0u;
+            const uint32_t htri = __shfl(hr.tri, int(lane & ~1u));
+            const uint32_t ht = __shfl(f2u(hr.t), int(lane & ~1u)), hb1 = __shfl(f2u(hr.b1), int(lane & ~1u));
+            const uint32_t hb2 = __shfl(f2u(hr.b2), int(lane & ~1u));
             if (shadow) {
-                nVis++;
-                HitRec hr;
-                if (!traceBVH4<true, false>(g, shO, shD, 0.f, shT, st, hr)) Ls = Ls + shC;
+                nVis += role ^ 1u;
+                if (!occluded) Ls = Ls + shC;
             }
             if (!cont) break;
-            nClosest++;
-            HitRec hr;
-            traceBVH4<false, false>(g, newO, newD, 0.f, kFltMax, st, hr);
-            if (hr.tri == kInvalid && !a.keepMisses) break;
-            h = make_uint4(hr.tri, f2u(hr.t), f2u(hr.b1), f2u(hr.b2));
+            nClosest += role ^ 1u;
+            if (htri == kInvalid && !a.keepMisses) break;
+            h = make_uint4(htri, ht, hb1, hb2);
             o = newO;
             d = newD;
         }
-        a.Ls[pathId] = make_float4(Ls.x, Ls.y, Ls.z, 0.f);
+        if (role == 0u) a.Ls[pathId] = make_float4(Ls.x, Ls.y, Ls.z, 0.f);
     }
     for (int off = 32; off > 0; off >>= 1) {
         nClosest += __shfl_xor(nClosest, off);
@@ -975,7 +986,7 @@ hipError_t mptBounces(const SceneDev& s, const MptArgs& a0, const TraceScratch& 
         hipError_t e = withTraceConfig(tsc, [&](auto Bc, auto Dc) {
             constexpr int B = decltype(Bc)::value, D = decltype(Dc)::value;
             // One wave per tailLanes paths, within the persistent grid (the overflow stack's size).
-            const uint32_t lanes = a.tailLanes - 1u < 64u ? a.tailLanes : 64u;
+            const uint32_t lanes = a.tailLanes - 1u < 32u ? a.tailLanes : 32u;
             const uint64_t threads = 64ull * ((uint64_t(a.tailPaths) + lanes - 1) / lanes);
             const uint32_t blocks = uint32_t(std::max<uint64_t>(1u, std::min<uint64_t>(tsc.gridBlocks, (threads + B - 1) / B)));
             if (a.lightMode == kLightsNone)

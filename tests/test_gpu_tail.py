@@ -16,9 +16,9 @@ from falcor_amd import scene as S
 
 pytestmark = pytest.mark.gpu
 
-# (tailPaths, tailLanes): thresholds from the last depths only to the first, with 64, 8 and 1
+# (tailPaths, tailLanes): thresholds from the last depths only to the first, with 32, 8 and 1
 # paths per wave of the tail launch.
-THRESHOLDS = ((1, 64), (300, 8), (4000, 1), (4000, 64), (1 << 24, 8))
+THRESHOLDS = ((1, 32), (300, 8), (4000, 1), (4000, 32), (1 << 24, 8))
 
 
 def _transmissive(data, seed):
@@ -109,6 +109,6 @@ def test_tail_property_checks(gpu):
     for bad in (-1, 0.5, float(1 << 25)):
         with pytest.raises(Exception):
             p.set_property("tailPaths", bad)
-    for bad in (0, 65, 2.5):
+    for bad in (0, 33, 2.5):
         with pytest.raises(Exception):
             p.set_property("tailLanes", bad)
