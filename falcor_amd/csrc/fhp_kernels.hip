@@ -781,7 +781,12 @@ __global__ __launch_bounds__(B) void kTraceBounce(SceneDev s, MptArgs a, const f
     const uint32_t n = nS + (nCPtr ? *nCPtr : nCConst);
     const uint32_t G = gridDim.x * B;
     TraceCounters cc{0, 0}, cs{0, 0};
-    for (uint32_t i = blockIdx.x * B + threadIdx.x; i < n; i += G) {
+    // R consecutive rays per wave: all 64 lanes while the rays fill the grid; fewer rays than
+    // lanes are spread over every wave (more waves per SIMD to hide latency, and a wave waits for
+    // the slowest of fewer rays) instead of filling the first waves.
+    const uint32_t waves = G >> 6, lane = threadIdx.x & 63u, wave = (blockIdx.x * B + threadIdx.x) >> 6;
+    const uint32_t R = n >= G ? 64u : (n + waves - 1u) / waves;
+    for (uint32_t i = lane < R ? wave * R + lane : n; i < n; i += waves * R) {
         // Ray i: visibility record i (i < nS) or closest-hit ray i - nS.
         const bool vis = i < nS;
         const uint32_t j = vis ? i : i - nS;
