@@ -993,9 +993,11 @@ FHP_HD bool lbvhSample(const SceneDev& s, f3 posW, f3 n, float u, float& pdf, ui
         float li = lbvhImportance(s, posW, n, l), ri = lbvhImportance(s, posW, n, r);
         float tot = li + ri;
         if (tot == 0.f) return false;
+        // pL exact (the pdf replays it); the random number is rescaled to the chosen child with a
+        // hardware reciprocal, which moves the sample by about an ulp, not the pdf.
         float pL = li / tot, pR = 1.0f - pL;
-        if (u < pL) { u = u / pL; pdf *= pL; node = l; }
-        else { u = (u - pL) / pR; pdf *= pR; node = r; }
+        if (u < pL) { u = u * hwRcp(pL); pdf *= pL; node = l; }
+        else { u = (u - pL) * hwRcp(pR); pdf *= pR; node = r; }
         info = lbvhInfo(s, node);
     }
     uint32_t cnt = info.z;
