@@ -470,11 +470,15 @@ __global__ __launch_bounds__(kCompactBlock) void kCompactPrimary(SceneDev s0, Mp
     }
 }
 
+// The shading kernels are held to 128 VGPRs (4 waves per SIMD instead of 3 at their natural
+// 134-142): the light-BVH descent is a chain of dependent loads, and the extra wave hides more of
+// it than the few spilled values cost (the emissive-only instances spill nothing).
+//
 // Primary-hit shading (MinimalPathTracer.rt.slang:392-449), one thread per path with a primary
 // hit (the compacted list of kCompactPrimary). The path state (throughput, last normal, sample
 // generator) is written for paths that continue only: nothing reads it for the others.
 template <bool MULTI, int LM>
-__global__ __launch_bounds__(kShadeBlock) void kShadePrimary(SceneDev s0, MptArgs a) {
+__global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(4))) void kShadePrimary(SceneDev s0, MptArgs a) {
     __shared__ uint32_t lds[kShadeBlock / 64 + 1];
     const uint32_t n = a.counters[kPrimaryHitCount];
     const uint32_t G = gridDim.x * blockDim.x;
@@ -540,7 +544,7 @@ __device__ __forceinline__ bool tailActive(const MptArgs& a, uint32_t n) { retur
 // Runs on the compacted hits of the bounce (entry k: hit hitQ[k] of ray hitIdx[k]) unless
 // visibility rays have launches of their own, then on every queued ray (hitQ[i] of ray i).
 template <bool MULTI, int LM>
-__global__ __launch_bounds__(kShadeBlock) void kShade(SceneDev s0, MptArgs a, uint32_t depth) {
+__global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(4))) void kShade(SceneDev s0, MptArgs a, uint32_t depth) {
     __shared__ uint32_t lds[kShadeBlock / 64 + 1];
     const int qi = depth & 1, qo = qi ^ 1;
     const bool compact = !a.separateShadow;
