@@ -240,7 +240,7 @@ hipError_t launchFillRGBA(float4* p, uint64_t n, float r, float g, float b, floa
 struct RefitArgs {
     float4* vtx;             // vertex records of the scene
     float4* triPos;          // leaf-order triangle records
-    float* bvh;              // BVH4 nodes (32 floats each)
+    float* bvh;              // BVH4 nodes (16 words each, packBVH4Node)
     const uint32_t* idx;     // triangle vertex indices
     const float* pos;        // new positions of vertices [v0, v0 + nVerts) (3 floats each, device);
                              // null when the vertex records already hold them

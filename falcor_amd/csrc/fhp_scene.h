@@ -1,13 +1,14 @@
 // Device-resident scene layout and shading routines of the fhp core.
 //
 // HBM layout (all arrays 16-byte aligned, built once per scene by fhp_scene_create):
-//   bvh      BVH4 nodes, 128 B each = 8 x float4: lo.x[4] hi.x[4] lo.y[4] hi.y[4] lo.z[4] hi.z[4]
-//            child[4] (uint bits) and one spare float4. One node is one 128-byte cache line.
+//   bvh      BVH4 nodes, 64 B each = 4 x float4, child boxes quantised to 8-bit plane codes
+//            (layout at packBVH4Node).
 //   triPos   48 B per triangle in BVH leaf order: (v0, origTri) (v1, 0) (v2, 0).
 //   idx      3 x u32 per triangle (original order); vtx 32 B per vertex: (p.xyz, n.x) (n.yz, uv).
 //   triMat   u32 material id per triangle; mats 80 B per material (20 floats, see MatField).
 //   texels   RGB fp32 texels of all textures; texDesc int4 (offset, width, height, 0).
-//   lights   80 B per emissive triangle; lbvh 32 B per light-BVH node.
+//   lights   80 B per emissive triangle; lbvh 32 B per light-BVH node (packed), then 48 B per
+//            node of sampling records (lightNodeSampleRecord).
 #pragma once
 #include "fhp_math.h"
 
