@@ -788,12 +788,24 @@ __global__ __launch_bounds__(B) void kTraceBounce(SceneDev s, MptArgs a, const f
     // R consecutive rays per wave: all 64 lanes while the rays fill the grid; fewer rays than
     // lanes are spread over every wave (more waves per SIMD to hide latency, and a wave waits for
     // the slowest of fewer rays) instead of filling the first waves.
+    // A single view's primary rays (one per pixel, their hits stored by pixel) are visited in 8x8
+    // pixel tiles: a wave's rays then leave a compact patch of the image instead of a 64-pixel
+    // strip of one row. Index i runs over whole tiles; positions outside the frame are skipped.
+    const bool tiled = !MULTI && !hitCount && !nSPtr && !pathIds && a.W > 0u;
+    const uint32_t tilesX = (a.W + 7u) / 8u;
+    const uint32_t nI = tiled ? tilesX * ((a.H + 7u) / 8u) * 64u : n;
     const uint32_t waves = G >> 6, lane = threadIdx.x & 63u, wave = (blockIdx.x * B + threadIdx.x) >> 6;
-    const uint32_t R = n >= G ? 64u : (n + waves - 1u) / waves;
-    for (uint32_t i = lane < R ? wave * R + lane : n; i < n; i += waves * R) {
-        // Ray i: visibility record i (i < nS) or closest-hit ray i - nS.
-        const bool vis = i < nS;
-        const uint32_t j = vis ? i : i - nS;
+    const uint32_t R = nI >= G ? 64u : (nI + waves - 1u) / waves;
+    for (uint32_t i = lane < R ? wave * R + lane : nI; i < nI; i += waves * R) {
+        uint32_t pix = i;
+        if (tiled) {
+            const uint32_t t = i >> 6, x = (t % tilesX) * 8u + (i & 7u), y = (t / tilesX) * 8u + ((i >> 3) & 7u);
+            if (x >= a.W || y >= a.H) continue;
+            pix = y * a.W + x;
+        }
+        // Ray i: visibility record i (i < nS) or closest-hit ray i - nS (primary rays: pixel pix).
+        const bool vis = pix < nS;
+        const uint32_t j = vis ? pix : pix - nS;
         const float4* q = vis ? a.shQ + 3 * size_t(j) : rays + 2 * size_t(j);
         const float4 r0 = q[0], r1 = q[1];
         GeomRef g{s.bvh, s.triPos};
