@@ -455,9 +455,19 @@ constexpr int kShadeBlock = 256, kCompactBlock = 1024;
 template <bool MULTI>
 __global__ __launch_bounds__(kCompactBlock) void kCompactPrimary(SceneDev s0, MptArgs a) {
     __shared__ uint32_t lds[kCompactBlock / 64 + 1];
-    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t q = idx;
+    bool inFrame = idx < a.nPix;
+    if (!MULTI && a.W > 0u) {
+        // A single view's pixels in 8x8 tiles (the primary traversal's order), so the primary
+        // shading and the rays it starts work on compact patches of the image.
+        const uint32_t tilesX = (a.W + 7u) / 8u, t = idx >> 6;
+        const uint32_t x = (t % tilesX) * 8u + (idx & 7u), y = (t / tilesX) * 8u + ((idx >> 3) & 7u);
+        inFrame = x < a.W && y < a.H;
+        q = y * a.W + x;
+    }
     uint32_t cnt = 0, first = 0, view = 0;
-    if (q < a.nPix && a.pixHit[q].x != kInvalid) {
+    if (inFrame && a.pixHit[q].x != kInvalid) {
         if (MULTI) view = findView(a.views, a.nViews, q, false);
         const ViewCtx c = viewCtx<MULTI>(s0, a, q, false);
         cnt = c.spp;
@@ -1031,7 +1041,8 @@ hipError_t mptBounces(const SceneDev& s, const MptArgs& a0, const TraceScratch& 
     if (e != hipSuccess) return e;
     tm->begin(KK_SHADE, st);
     const uint32_t shadeBlocks = std::min<uint32_t>((a.nPaths + T - 1) / T, uint32_t(numCUs()) * 32);
-    hipLaunchKernelGGL(kCompactPrimary<MULTI>, dim3((npix + kCompactBlock - 1) / kCompactBlock), dim3(kCompactBlock), 0, st,
+    const uint32_t nCompact = MULTI ? npix : ((a.W + 7u) / 8u) * ((a.H + 7u) / 8u) * 64u;  // whole 8x8 tiles
+    hipLaunchKernelGGL(kCompactPrimary<MULTI>, dim3((nCompact + kCompactBlock - 1) / kCompactBlock), dim3(kCompactBlock), 0, st,
                        s, a);
     if (a.lightMode == kLightsNone)
         hipLaunchKernelGGL((kShadePrimary<MULTI, kLightsNone>), dim3(shadeBlocks), dim3(T), 0, st, s, a);
