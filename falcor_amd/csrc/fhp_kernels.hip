@@ -631,13 +631,12 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(4))
 // Tail of the bounce loop. Late bounces hold few paths, and a launch per bounce then lasts as long
 // as its slowest ray's chain of dependent node loads. Once a depth has at most tailPaths hits,
 // this kernel takes them over (kShade of that depth returns at once) and runs each path to its end
-// in one thread: handleHit, the visibility ray, the next closest-hit ray, and so on. The steps are
-// kShade's and kTraceBounce's with the path state in registers, in the same order (the
+// in one pair of lanes: handleHit, the visibility ray, the next closest-hit ray, and so on. The
+// steps are kShade's and kTraceBounce's with the path state in registers, in the same order (the
 // visibility ray of depth d adds its contribution after the emission of depth d and before that
 // of depth d + 1; the library is built without implicit FMA contraction), so the radiance is
-// bit-identical to the per-depth launches. Later depths find
-// their queues empty. Rays traced here are counted in counters[kTailClosestCount] and
-// counters[kTailVisibilityCount].
+// bit-identical to the per-depth launches. Later depths find their queues empty. Rays traced here
+// are counted in counters[kTailClosestCount] and counters[kTailVisibilityCount].
 template <int B, int D, bool MULTI, int LM>
 __global__ __launch_bounds__(B) void kTail(SceneDev s0, MptArgs a, uint32_t depth, TraceScratch ts) {
     __shared__ uint32_t lds[D * B];
