@@ -782,12 +782,15 @@ __global__ __launch_bounds__(B) void kShadow(SceneDev s, MptArgs a, uint32_t slo
 //
 // In a batch (MULTI) each ray traverses the scene of its view: visibility and bounce rays find it
 // through their path id (pathIds[j] for bounce ray j), primary rays through their pixel.
+//
+// The register budget is held to 8 waves per SIMD (64 VGPRs): with a 16-entry LDS stack a CU
+// then holds 8 blocks of 256 threads, and the extra waves hide more of the node fetch latency
+// than the registers given up cost (a 24-entry stack is limited to 6 blocks by LDS either way).
 template <int B, int D, bool COUNT, bool MULTI>
-__global__ __launch_bounds__(B) void kTraceBounce(SceneDev s, MptArgs a, const float4* __restrict__ rays,
-                                                  const uint32_t* nCPtr, uint32_t nCConst, uint4* __restrict__ hits,
-                                                  uint32_t* __restrict__ hitIdx, uint32_t* hitCount, const uint32_t* nSPtr,
-                                                  const uint32_t* __restrict__ pathIds, TraceScratch ts,
-                                                  unsigned long long* stats) {
+__global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(8))) void kTraceBounce(
+    SceneDev s, MptArgs a, const float4* __restrict__ rays, const uint32_t* nCPtr, uint32_t nCConst,
+    uint4* __restrict__ hits, uint32_t* __restrict__ hitIdx, uint32_t* hitCount, const uint32_t* nSPtr,
+    const uint32_t* __restrict__ pathIds, TraceScratch ts, unsigned long long* stats) {
     __shared__ uint32_t lds[D * B];
     TraceStack st = makeStack<B, D>(lds, ts, blockIdx.x, threadIdx.x);
     const uint32_t nS = nSPtr ? *nSPtr : 0u;
