@@ -798,8 +798,9 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(8))) void kTr
     const uint32_t G = gridDim.x * B;
     TraceCounters cc{0, 0}, cs{0, 0};
     // R consecutive rays per wave: all 64 lanes while the rays fill the grid; fewer rays than
-    // lanes are spread over every wave (more waves per SIMD to hide latency, and a wave waits for
-    // the slowest of fewer rays) instead of filling the first waves.
+    // lanes are spread over more waves (more waves per SIMD to hide latency, and a wave waits for
+    // the slowest of fewer rays) instead of filling the first waves, but at least 32 per wave:
+    // thinner waves compete for the same VALU issue slots with most of their lanes idle.
     // A single view's primary rays (one per pixel, their hits stored by pixel) are visited in 8x8
     // pixel tiles: a wave's rays then leave a compact patch of the image instead of a 64-pixel
     // strip of one row. Index i runs over whole tiles; positions outside the frame are skipped.
@@ -807,7 +808,7 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(8))) void kTr
     const uint32_t tilesX = (a.W + 7u) / 8u;
     const uint32_t nI = tiled ? tilesX * ((a.H + 7u) / 8u) * 64u : n;
     const uint32_t waves = G >> 6, lane = threadIdx.x & 63u, wave = (blockIdx.x * B + threadIdx.x) >> 6;
-    const uint32_t R = nI >= G ? 64u : (nI + waves - 1u) / waves;
+    const uint32_t R = nI >= G ? 64u : max((nI + waves - 1u) / waves, 32u);
     for (uint32_t i = lane < R ? wave * R + lane : nI; i < nI; i += waves * R) {
         uint32_t pix = i;
         if (tiled) {
